@@ -1,0 +1,168 @@
+"""Seeded synthetic relativistic CCSD inputs in relccsd's packed layouts.
+
+Recipe (one per system, reproducible from ``seed``):
+
+* orbital energies: occupied ~ U[-2.0, -0.3] Eh, virtual ~ U[0.2, 5.0] Eh,
+  each set sorted ascending;
+* a complex rank-``rank`` factor B[P, p, r] with Re and Im ~ N(0, scale^2),
+  made Hermitian in (p, r) (upper triangle drawn, diagonal real);
+* Mulliken integrals (pr|qs) = sum_P B[P,p,r] B[P,q,s], physicist's
+  <pq|rs> = (pr|qs), antisymmetrised <pq||rs> = <pq|rs> - <pq|sr>;
+* Fock matrix f = diag(eps) (+ an optional Hermitian perturbation, in which
+  case eps is re-taken as Re diag(f), as relccsd's RDINTS does when it
+  rebuilds the Fock diagonal, ccints.F:1252-1287).
+
+The six integral classes are laid out exactly as relccsd's sorted files
+hold them (SYMTAB offsets ccdriv.F:1150-1187; readers ccgetv.F:89-402), for
+the C1 double group (one fermion irrep):
+
+* packed occupied pair IJ (I>J), enumerated J outer / I inner
+  (DENOM cceqns.F:2408-2448); same for virtual pairs AB (A>B);
+* VO pair (A,K) at K*nv + A (A fastest, IIVO ccdriv.F:1050-1069);
+* every class is column-major: rows = bra pair, columns = ket pair:
+  OOOO <KL||IJ>, VOOO <AK||IJ>, VVOO <AB||IJ>, VOVO <AK||CI>,
+  VOVV <AK||CD>, VVVV <AB||CD>;
+* Fock blocks FOO(K,I) = f_KI, FVO(A,I) = f_AI, FVV(A,C) = f_AC,
+  column-major (FMTOFILE ccgetv.F:34-87).
+
+All arrays are returned as flat complex128 tensors (interleaved re/im in
+memory, i.e. the REAL*8 pairs relccsd uses with RCW=2).  The generator runs
+on whatever torch device is given; large systems are built on the GPU in
+column blocks so the host never holds more than one class at a time.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+__all__ = ["pair_index", "draw_factors", "make_system", "class_shapes"]
+
+
+def pair_index(n: int) -> tuple[np.ndarray, np.ndarray]:
+    """(p, q) with p > q for the packed pair list of an n-set, in relccsd's
+    order: q outer, p inner (cceqns.F:2416-2438)."""
+    p, q = [], []
+    for qq in range(n):
+        for pp in range(qq + 1, n):
+            p.append(pp)
+            q.append(qq)
+    return np.asarray(p, dtype=np.int64), np.asarray(q, dtype=np.int64)
+
+
+def class_shapes(no: int, nv: int) -> dict:
+    noot = no * (no - 1) // 2
+    nvvt = nv * (nv - 1) // 2
+    nvo = nv * no
+    return {
+        "FOO": (no, no), "FVO": (nv, no), "FVV": (nv, nv),
+        "OOOO": (noot, noot), "VOOO": (nvo, noot), "VVOO": (nvvt, noot),
+        "VOVO": (nvo, nvo), "VOVV": (nvo, nvvt), "VVVV": (nvvt, nvvt),
+        "T1": (nv, no), "T2": (nvvt, noot),
+    }
+
+
+def draw_factors(no: int, nv: int, seed: int, rank: int = 120,
+                 scale: float = 0.05):
+    """Orbital energies and the Hermitian factor B[P,p,r] (numpy, host)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    eps_o = np.sort(rng.uniform(-2.0, -0.3, size=no))
+    eps_v = np.sort(rng.uniform(0.2, 5.0, size=nv))
+    n = no + nv
+    re = rng.normal(0.0, scale, size=(rank, n, n))
+    im = rng.normal(0.0, scale, size=(rank, n, n))
+    z = re + 1j * im
+    up = np.triu(z, 1)
+    b = up + np.conj(np.transpose(up, (0, 2, 1)))
+    idx = np.arange(n)
+    b[:, idx, idx] = re[:, idx, idx]
+    return np.concatenate([eps_o, eps_v]), b
+
+
+def _ket_block(bra_p, bra_q, ket_r, ket_s):
+    """<pq||rs> for all (p in bra_p-set, q in bra_q-set) and a list of ket
+    pairs.  bra_p: B[:, Sp, r-values] etc. are gathered by the caller:
+      Bpr[P, p, k] = B[P, p, r_k], Bqs[P, q, k] = B[P, q, s_k],
+      Bps[P, p, k] = B[P, p, s_k], Bqr[P, q, k] = B[P, q, r_k].
+    Returns X[k, p, q]."""
+    Bpr, Bqs, Bps, Bqr = bra_p[0], bra_q[0], bra_p[1], bra_q[1]
+    # (pr|qs): sum_P Bpr[P,p,k] Bqs[P,q,k]  -> per ket pair k: Bpr_k^T Bqs_k
+    a = Bpr.permute(2, 1, 0).contiguous()          # k, p, P
+    b = Bqs.permute(2, 0, 1).contiguous()          # k, P, q
+    c = Bps.permute(2, 1, 0).contiguous()
+    d = Bqr.permute(2, 0, 1).contiguous()
+    return torch.bmm(a, b) - torch.bmm(c, d)
+
+
+def _build_class(B, prow, qrow, rcol, scol, rows_packed, device, kblock):
+    """Column-major class array: rows enumerate (prow[i], qrow[i]) bra pairs,
+    columns enumerate (rcol[k], scol[k]) ket pairs.  Returns flat complex128.
+    prow/qrow index the full spinor set; all are 1-D long tensors."""
+    nrow = prow.numel()
+    ncol = rcol.numel()
+    out = torch.empty((ncol, nrow), dtype=torch.complex128, device=device)
+    up = torch.unique(prow)
+    uq = torch.unique(qrow)
+    # local positions of prow/qrow inside the unique sets
+    mp = torch.searchsorted(up, prow)
+    mq = torch.searchsorted(uq, qrow)
+    for k0 in range(0, ncol, kblock):
+        k1 = min(ncol, k0 + kblock)
+        r = rcol[k0:k1]
+        s = scol[k0:k1]
+        Bpr = B[:, up][:, :, r]
+        Bps = B[:, up][:, :, s]
+        Bqs = B[:, uq][:, :, s]
+        Bqr = B[:, uq][:, :, r]
+        X = _ket_block((Bpr, Bps), (Bqs, Bqr), r, s)   # k, |up|, |uq|
+        out[k0:k1] = X[:, mp, mq]
+    return out.reshape(-1)
+
+
+def make_system(no: int, nv: int, seed: int, rank: int = 120,
+                scale: float = 0.05, device="cpu", fock_perturb: float = 0.0,
+                classes=("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"),
+                kblock: int | None = None) -> dict:
+    """Build one seeded system.  Returns a dict with 'eps' (float64, occupied
+    then virtual), the Fock blocks and the requested integral classes as flat
+    complex128 tensors on ``device`` in relccsd layout."""
+    eps, b = draw_factors(no, nv, seed, rank, scale)
+    n = no + nv
+    dev = torch.device(device)
+    B = torch.from_numpy(b).to(dev)
+    f = np.diag(eps).astype(np.complex128)
+    if fock_perturb:
+        rng = np.random.Generator(np.random.PCG64(seed + 7919))
+        v = rng.normal(0.0, fock_perturb, (n, n)) + 1j * rng.normal(0.0, fock_perturb, (n, n))
+        v = 0.5 * (v + v.conj().T)
+        f = f + v
+        eps = np.real(np.diag(f)).copy()
+    out = {"no": no, "nv": nv, "eps": torch.from_numpy(eps).to(dev)}
+    o = np.arange(no)
+    vv = np.arange(no, n)
+    out["FOO"] = torch.from_numpy(f[np.ix_(o, o)].T.reshape(-1).copy()).to(dev)
+    out["FVO"] = torch.from_numpy(f[np.ix_(vv, o)].T.reshape(-1).copy()).to(dev)
+    out["FVV"] = torch.from_numpy(f[np.ix_(vv, vv)].T.reshape(-1).copy()).to(dev)
+
+    def t(x):
+        return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+    ip, jp = pair_index(no)            # I > J
+    ap, bp = pair_index(nv)            # A > B
+    ak_a = np.tile(np.arange(nv), no) + no      # VO pair (A,K): A fastest
+    ak_k = np.repeat(np.arange(no), nv)
+    kb = kblock or max(1, min(4096, (1 << 27) // max(1, rank * max(nv, no) * 4)))
+    spec = {
+        "OOOO": (ip, jp, ip, jp),
+        "VOOO": (ak_a, ak_k, ip, jp),
+        "VVOO": (ap + no, bp + no, ip, jp),
+        "VOVO": (ak_a, ak_k, ak_a, ak_k),
+        "VOVV": (ak_a, ak_k, ap + no, bp + no),
+        "VVVV": (ap + no, bp + no, ap + no, bp + no),
+    }
+    for name in classes:
+        pr, qr, rc, sc = spec[name]
+        if len(pr) == 0 or len(rc) == 0:
+            out[name] = torch.zeros(len(pr) * len(rc), dtype=torch.complex128, device=dev)
+            continue
+        out[name] = _build_class(B, t(pr), t(qr), t(rc), t(sc), True, dev, kb)
+    return out

@@ -1,0 +1,208 @@
+C  Injection harness for the relccsd CPU reference (test infrastructure).
+C
+C  relccsd only ingests Kramers-format MDCINT files (ccints.F:765-1029),
+C  which the seeded synthetic generator cannot produce.  This driver
+C  replaces PROGRAM RELCCSD / CCMAIN (ccmain.F:34, ccdriv.F:375) with a
+C  short sequence of calls into the unmodified reference routines:
+C
+C    RDSYMI (ccdriv.F:2934)  reads the MRCONEE file this program writes
+C    CCINPT (ccdriv.F:126)   reads the MOLFDIR.INP file this program writes
+C    CCDINI (ccdriv.F:2620), SYMTAB (ccdriv.F:746)
+C    RFILE  (ccgenp.F:227)   opens ft40..ft46 as RDINTS would
+C    FMTOFILE (ccgetv.F:34) + RWRIT (ccgenp.F:441) store the Fock blocks
+C                            and the six integral classes in the layout
+C                            GETOOOO..GETVVVV read back (ccgetv.F:89-402)
+C
+C  Mode 1 then runs CCENER (ccdriv.F:1574): MP2 + CCSD, and writes the
+C  final amplitudes.  Mode 2 reads given T1/T2 and calls the per-term
+C  routines in the order CCEQN uses them (cceqns.F:313-546), writing
+C  every intermediate, so each GPU term can be checked on its own.
+C
+C  Input  : ccin.bin  (stream; header of 8 int32, then float64 data)
+C  Output : ccout.bin (stream; float64 data), plus the reference log
+C           on unit 6.
+C
+      PROGRAM CCHRNS
+      IMPLICIT INTEGER (A-Z)
+      INCLUDE "files.inc"
+      INCLUDE "iobuf.inc"
+      INCLUDE "diis.inc"
+      INCLUDE "inpt.inc"
+      INCLUDE "symm.inc"
+      INCLUDE "eqns.inc"
+      INCLUDE "complex.inc"
+      INCLUDE "ccpar.inc"
+      INCLUDE "results.inc"
+      PARAMETER (MAXSP=1000)
+      INTEGER IREPSP(MAXSP*4),IREPSPI(MAXSP*16*2)
+      REAL*8 EPS(MAXSP),EPS1(MAXSP)
+      INTEGER HDR(8)
+      REAL*8, ALLOCATABLE :: CC(:)
+      REAL*8, ALLOCATABLE :: FOO(:),FVO(:),FVV(:),V1(:),V2(:),V3(:),
+     &                       V4(:),V5(:),V6(:)
+      REAL*8, ALLOCATABLE :: T1(:),T2(:),S1(:),S2(:),AZ(:),H(:),
+     &                       HO(:),HV(:),HOV(:),GOO(:),GVV(:),
+     &                       BUF1(:),BUF2(:),BUF3(:)
+      REAL*8 DUM(4),ECC
+      CHARACTER*4 RNAM(2)
+      INTEGER MULT(2,2)
+      LOGICAL BRT
+C
+      MASTER = 0
+      MYPROC = 0
+      NMPROC = 1
+      IN = 5
+      IW = 6
+      INTERFACE = 'MOLFDIR   '
+C
+C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, unused, unused
+C
+      OPEN (30,FILE='ccin.bin',ACCESS='STREAM',FORM='UNFORMATTED',
+     &      STATUS='OLD')
+      READ (30) HDR
+      NOCC  = HDR(1)
+      NVIR  = HDR(2)
+      MODE  = HDR(3)
+      NSP   = NOCC + NVIR
+      READ (30) (EPS1(I),I=1,NSP)
+C
+C     MRCONEE records 1-5 exactly as RDSYMI reads them (ccdriv.F:2978-
+C     2988): C1 double group, one fermion irrep and one boson irrep.
+C
+      OPEN (MRCONEE,FILE='MRCONEE',FORM='UNFORMATTED',
+     &      STATUS='REPLACE')
+      BRT = .FALSE.
+      WRITE (MRCONEE) NSP,BRT,0.0D0
+      WRITE (MRCONEE) 1,'A             ',NOCC
+      RNAM(1) = '   A'
+      RNAM(2) = '   a'
+      WRITE (MRCONEE) 1,RNAM(1),RNAM(2)
+      MULT(1,1) = 2
+      MULT(2,1) = 1
+      MULT(1,2) = 1
+      MULT(2,2) = 2
+      WRITE (MRCONEE) ((MULT(I,J),I=1,2),J=1,2)
+      WRITE (MRCONEE) (1,1,EPS1(I),I=1,NSP)
+      CLOSE (MRCONEE)
+C
+C     Namelist input read by CCINPT (ccdriv.F:126-373)
+C
+      OPEN (31,FILE='MOLFDIR.INP',FORM='FORMATTED',STATUS='REPLACE')
+      WRITE (31,'(A,I4,A)') ' &RELCCSD NELEC=',NOCC,
+     &   ', CARITH=T, DOSORT=F, DOENER=T, IPRNT=1, TIMING=T /'
+      WRITE (31,'(A,I4,A,I4,A,I4,A)') ' &CCENER DOMP2=T, DOCCSD=T,'//
+     &   ' DOCCSDT=F, MAXIT=',HDR(4),', MAXDIM=',HDR(5),
+     &   ', NTOL=',HDR(6),' /'
+      CLOSE (31)
+C
+C     Set-up sequence of CCMAIN (ccdriv.F:426-529)
+C
+      CALL XTIME(1,0,' ')
+      CALL RDSYMI (NSPR,EPS1,IREPSP)
+      CALL CCINPT (IN,IW,MIOBUF)
+      IRW = INTOWP(1)
+      RCW = 2
+      IF (.NOT.CARITH) CALL QUIT('HARNESS NEEDS CARITH=T')
+      MINBUF = SEC2I(1)
+      MINSEC = IRW * RCW
+      CALL CCDINI
+      CALL SYMTAB (NELEC,NFROZ,NINACT,NACT,NSP,EPS1,EPS,IREPSP,
+     &             IREPSPI,.FALSE.,.TRUE.)
+C
+C     Read the Fock blocks and the six classes (reference layout)
+C
+      ALLOCATE (FOO(2*NFOO),FVO(2*NFVO),FVV(2*NFVV))
+      ALLOCATE (V1(2*NV1),V2(2*NV2),V3(2*NV3),V4(2*NV4),V5(2*NV5),
+     &          V6(2*NV6))
+      READ (30) FOO,FVO,FVV,V1,V2,V3,V4,V5,V6
+C
+      DO K = 0, 6
+         CALL RFILE(ITAPT+K,20)
+      ENDDO
+      CALL FMTOFILE (.TRUE.,FVO,FOO,FVV)
+      CALL RWRIT(ITAPT+1,V1,NV1*RCW*IRW,1)
+      CALL RWRIT(ITAPT+2,V2,NV2*RCW*IRW,1)
+      CALL RWRIT(ITAPT+3,V3,NV3*RCW*IRW,1)
+      CALL RWRIT(ITAPT+4,V4,NV4*RCW*IRW,1)
+      CALL RWRIT(ITAPT+5,V5,NV5*RCW*IRW,1)
+      CALL RWRIT(ITAPT+6,V6,NV6*RCW*IRW,1)
+      DEALLOCATE (V1,V2,V3,V4,V5,V6)
+C
+      OPEN (32,FILE='ccout.bin',ACCESS='STREAM',FORM='UNFORMATTED',
+     &      STATUS='REPLACE')
+C
+      IF (MODE.EQ.1) THEN
+C
+C        MP2 + CCSD through the reference energy driver
+C
+         KFREE = 1
+         LFREE = 2000000000
+         MXCORR = 0
+         CALL CCENER (1,DUM,KFREE,LFREE,MXCORR,EPS)
+         ALLOCATE (CC(MXCORR+1024))
+         KFREE = 1
+         LFREE = MXCORR
+         CALL CCENER (2,CC,KFREE,LFREE,MXCORR,EPS)
+         ALLOCATE (T1(2*NDIMT1),T2(2*NDIMT2))
+         CALL GETAMPT (T1,T2)
+         WRITE (32) EMP2,ECCSD,DBLE(NIT)
+         WRITE (32) T1,T2
+C
+      ELSE
+C
+C        Per-term evaluation on given amplitudes, CCEQN order
+C
+         EQNS = 'AMPLTD'
+         NBUF1 = MAX0(JOOOO(NREP+1),JVOOO(NREP+1),
+     &                JVVOO(NREP+1),JVOVO(NREP+1))
+         NBUF2 = NBUF1
+         NV5IRP = 1
+         NV6IRP = 1
+         DO IRP = 1, NREP
+            NV5IRP = MAX0(NV5IRP,NVO (IRP)*NVVT(IRP))
+            NV6IRP = MAX0(NV6IRP,NVVT(IRP)*NVVT(IRP))
+         ENDDO
+         NBUF3 = MAX0(NBUF1,NV5IRP,NV6IRP)
+         ALLOCATE (T1(2*NDIMT1),T2(2*NDIMT2),S1(2*NDIMT1),
+     &             S2(2*NDIMT2),AZ(2*NV1),H(2*NV4),HO(2*NFOO),
+     &             HV(2*NFVV),HOV(2*NFVO),GOO(2*NFOO),GVV(2*NFVV),
+     &             BUF1(2*NBUF1),BUF2(2*NBUF2),BUF3(2*NBUF3))
+         READ (30) T1,T2
+         S1 = 0.0D0
+         S2 = 0.0D0
+         AZ = 0.0D0
+         H = 0.0D0
+         BUF1 = 0.0D0
+         BUF2 = 0.0D0
+         BUF3 = 0.0D0
+         CALL HOINTM (FOO,T1,T2,BUF1,BUF2,BUF3,HO)
+         WRITE (32) HO
+         CALL HVINTM (FVV,T1,T2,BUF1,BUF2,BUF3,HV)
+         WRITE (32) HV
+         CALL HOVINTM (FVO,T1,BUF1,BUF2,HOV)
+         WRITE (32) HOV
+         CALL T1EQNS (FVO,T1,T2,HO,HV,HOV,H,DUM,GOO,GVV,
+     &                BUF1,BUF2,NBUF2,BUF3,NBUF3,S1)
+         WRITE (32) S1
+         CALL GOINTM (FVO,T1,BUF1,BUF2,HO)
+         WRITE (32) HO
+         CALL GVINTM (FVO,T1,BUF3,NBUF3,BUF2,NBUF2,HV)
+         WRITE (32) HV
+         CALL AINTM (T1,T2,BUF1,BUF2,AZ)
+         WRITE (32) AZ
+         CALL HINTM (T1,T2,BUF1,BUF2,BUF3,NBUF3,H)
+         WRITE (32) H
+         CALL T2EQNS (T1,T2,HO,HV,HOV,H,AZ,DUM,GOO,GVV,
+     &                BUF1,BUF2,BUF3,NBUF3,S2)
+         WRITE (32) S2
+         CALL BINTM (T1,T2,BUF1,BUF2,NBUF2,BUF3,NBUF3,S2,IW)
+         WRITE (32) S2
+         CALL DENOM (EPS,S1,S2,S1,S2)
+         CALL ZCORE (S1,S2)
+         WRITE (32) S1,S2
+         CALL CCENG (S1,S2,FVO,BUF1,BUF2,ECC)
+         WRITE (32) ECC
+      ENDIF
+      CLOSE (32)
+      CLOSE (30)
+      END

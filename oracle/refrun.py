@@ -1,0 +1,108 @@
+"""Runs the compiled relccsd reference (oracle/_ref/cchrns.x) on a system.
+
+Test infrastructure only: used by tests/ and tests/golden/ scripts to obtain
+reference outputs.  The executable is built by ``make -f Makefile.ref`` from
+the unmodified relccsd sources plus harness/cchrns.F.
+
+Mode 1 ("ccsd") runs relccsd's MP2 + CCSD driver CCENER (ccdriv.F:1574)
+with the given MAXIT / MAXDIM / NTOL and returns the final amplitudes and
+the per-iteration table CCEQN prints (cceqns.F:270-272, 550-551).
+Mode 2 ("terms") evaluates every CCEQN term once on given T1/T2 and returns
+each intermediate (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before and after BINTM,
+S1/S2 after DENOM, and CCENG's energy).
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(HERE, "_ref", "cchrns.x")
+
+CLASS_ORDER = ("FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+
+
+def available() -> bool:
+    return os.path.exists(EXE)
+
+
+def _c(x):
+    x = np.asarray(x.detach().cpu().numpy() if hasattr(x, "detach") else x)
+    return np.ascontiguousarray(x.astype(np.complex128).reshape(-1))
+
+
+def _write_input(path, sysd, mode, maxit, maxdim, ntol, t1=None, t2=None):
+    no, nv = sysd["no"], sysd["nv"]
+    hdr = np.array([no, nv, mode, maxit, maxdim, ntol, 0, 0], dtype=np.int32)
+    eps = np.asarray(sysd["eps"].cpu().numpy() if hasattr(sysd["eps"], "cpu") else sysd["eps"],
+                     dtype=np.float64)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(eps.tobytes())
+        for name in CLASS_ORDER:
+            f.write(_c(sysd[name]).tobytes())
+        if mode == 2:
+            f.write(_c(t1).tobytes())
+            f.write(_c(t2).tobytes())
+
+
+_ITER_RE = re.compile(r"^\s+(\d+)\s+(-?[\d.]+(?:[EeDd][-+]?\d+)?)\s+(-?[\d.]+(?:[EeDd][-+]?\d+)?)\s+(-?[\d.]+)\s*$")
+
+
+def parse_iterations(log: str):
+    """The CCEQN table rows 'NIT ENERGY RMS T1-DIAGN' (format 6001)."""
+    rows = []
+    in_table = False
+    for line in log.splitlines():
+        if "NIT" in line and "ENERGY" in line and "RMS" in line:
+            in_table = True
+            rows = []
+            continue
+        if in_table:
+            m = _ITER_RE.match(line)
+            if m:
+                rows.append(tuple(float(g.replace("D", "E")) for g in m.groups()))
+            elif "CCSD results" in line or "Requested convergence" in line:
+                in_table = False
+    return rows
+
+
+def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
+        workdir=None, timeout=3600):
+    no, nv = sysd["no"], sysd["nv"]
+    noot, nvvt = no * (no - 1) // 2, nv * (nv - 1) // 2
+    nt1, nt2 = nv * no, nvvt * noot
+    m = 1 if mode == "ccsd" else 2
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        _write_input(os.path.join(d, "ccin.bin"), sysd, m, maxit, maxdim, ntol, t1, t2)
+        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
+            raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
+        raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+    out = {"log": log}
+    if m == 1:
+        out["emp2"], out["eccsd"], out["nit"] = raw[0], raw[1], int(raw[2])
+        z = raw[3:].view(np.complex128)
+        out["T1"], out["T2"] = z[:nt1].copy(), z[nt1:nt1 + nt2].copy()
+        out["iterations"] = parse_iterations(log)
+        mt = re.search(r"Total time in CCEQN :\s+([\d.]+)", log)
+        out["cceqn_cpu_s"] = float(mt.group(1)) if mt else None
+        return out
+    nfoo, nfvo, nfvv = no * no, nv * no, nv * nv
+    nv1, nv4 = noot * noot, nt1 * nt1
+    sizes = [("HO", nfoo), ("HV", nfvv), ("HOV", nfvo), ("S1", nt1), ("GO", nfoo),
+             ("GV", nfvv), ("AZ", nv1), ("H", nv4), ("S2_T2EQNS", nt2),
+             ("S2_BINTM", nt2), ("S1_DENOM", nt1), ("S2_DENOM", nt2)]
+    z = raw[:-1].view(np.complex128)
+    pos = 0
+    for name, n in sizes:
+        out[name] = z[pos:pos + n].copy()
+        pos += n
+    assert pos == z.size, (pos, z.size)
+    out["ECC"] = raw[-1]
+    return out
