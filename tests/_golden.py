@@ -1,0 +1,44 @@
+"""Loading of the committed golden fixtures (outputs of the compiled relccsd
+reference, see tests/golden/make_golden.py)."""
+import glob
+import os
+
+import numpy as np
+
+from molfdir_amd import synth
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def names(kind):
+    return sorted(os.path.basename(p)[len(kind) + 1:-4]
+                  for p in glob.glob(os.path.join(GOLDEN, f"{kind}_*.npz")))
+
+
+def load(kind, name):
+    z = np.load(os.path.join(GOLDEN, f"{kind}_{name}.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def system_for(g, device="cpu"):
+    p = g["params"]
+    no, nv, seed = int(p[0]), int(p[1]), int(p[2])
+    return synth.make_system(no, nv, seed=seed, scale=float(g["scale"]),
+                             fock_perturb=float(g["fock_perturb"]), device=device)
+
+
+def fingerprint(sysd):
+    keys = ("eps", "FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    out = []
+    for k in keys:
+        v = np.asarray(sysd[k].cpu().numpy())
+        out.append([np.abs(v).sum(), np.real(v).sum(), np.imag(v).sum() if np.iscomplexobj(v) else 0.0])
+    return np.asarray(out)
+
+
+def random_amplitudes(no, nv, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    noot, nvvt = no * (no - 1) // 2, nv * (nv - 1) // 2
+    t1 = 0.05 * (rng.normal(size=no * nv) + 1j * rng.normal(size=no * nv))
+    t2 = 0.05 * (rng.normal(size=noot * nvvt) + 1j * rng.normal(size=noot * nvvt))
+    return t1, t2

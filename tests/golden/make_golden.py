@@ -1,0 +1,90 @@
+"""Writes the golden fixtures in this directory from the relccsd reference.
+
+Run in the build container (where /root/reference and oracle/_ref exist):
+    make -C oracle -f Makefile.ref && python tests/golden/make_golden.py
+
+Fixtures hold only outputs of the compiled relccsd reference (built from its
+unmodified sources plus the injection harness oracle/harness/cchrns.F) and
+the generator parameters; the inputs are regenerated from their seeds by
+molfdir_amd.synth, and a fingerprint of every input class is stored so that a
+change in the generator is caught.
+
+* ccsd_<name>.npz: MP2 energy, the per-iteration (E, RMS) table that CCEQN
+  prints, and the final T1/T2 (packed relccsd layout) of a run with the given
+  MAXIT / MAXDIM / NTOL.
+* terms_<name>.npz: every CCEQN intermediate evaluated once on seeded random
+  T1/T2 (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before/after BINTM, DENOM, CCENG).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
+
+from molfdir_amd import synth  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+CCSD_CASES = [
+    # name, no, nv, seed, scale, fock_perturb, maxit, maxdim, ntol
+    ("cfg1_6o40v", 6, 40, 1, 0.05, 0.0, 20, 6, 12),
+    ("small_4o10v", 4, 10, 1, 0.05, 0.0, 20, 6, 12),
+    ("conv_5o16v", 5, 16, 2, 0.02, 0.0, 40, 8, 11),
+    ("fock_4o12v", 4, 12, 5, 0.03, 0.03, 30, 6, 10),
+    ("diis2_3o8v", 3, 8, 9, 0.05, 0.0, 12, 2, 12),
+]
+TERM_CASES = [
+    ("t_4o9v", 4, 9, 3, 0.05, 0.05, 11),
+    ("t_5o14v", 5, 14, 4, 0.05, 0.05, 12),
+    ("t_2o5v", 2, 5, 6, 0.05, 0.05, 13),
+]
+
+
+def fingerprint(sysd):
+    keys = ("eps", "FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    out = []
+    for k in keys:
+        v = np.asarray(sysd[k].cpu().numpy())
+        out.append([np.abs(v).sum(), np.real(v).sum(), np.imag(v).sum() if np.iscomplexobj(v) else 0.0])
+    return np.asarray(out)
+
+
+def random_amplitudes(no, nv, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    noot, nvvt = no * (no - 1) // 2, nv * (nv - 1) // 2
+    t1 = 0.05 * (rng.normal(size=no * nv) + 1j * rng.normal(size=no * nv))
+    t2 = 0.05 * (rng.normal(size=noot * nvvt) + 1j * rng.normal(size=noot * nvvt))
+    return t1, t2
+
+
+def main():
+    if not refrun.available():
+        raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
+    for name, no, nv, seed, scale, fp, maxit, maxdim, ntol in CCSD_CASES:
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+        r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol)
+        it = np.asarray(r["iterations"], dtype=np.float64)
+        np.savez_compressed(
+            os.path.join(HERE, f"ccsd_{name}.npz"),
+            params=np.array([no, nv, seed, maxit, maxdim, ntol], dtype=np.int64),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
+            T1=r["T1"], T2=r["T2"])
+        print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
+    for name, no, nv, seed, scale, fp, tseed in TERM_CASES:
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+        t1, t2 = random_amplitudes(no, nv, tseed)
+        r = refrun.run(s, "terms", t1=t1, t2=t2)
+        keys = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
+                "S2_BINTM", "S1_DENOM", "S2_DENOM"]
+        np.savez_compressed(
+            os.path.join(HERE, f"terms_{name}.npz"),
+            params=np.array([no, nv, seed, tseed], dtype=np.int64),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            ECC=r["ECC"], **{k: r[k] for k in keys})
+        print(name, r["ECC"])
+
+
+if __name__ == "__main__":
+    main()
