@@ -1,0 +1,169 @@
+"""Benchmark: relativistic CCSD iteration on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
+
+A step is one CCEQN iteration (HOINTM .. BINTM, DENOM, DIIS, CCENG, S->T)
+over the whole synthetic system with every integral class resident in HBM.
+Prints one JSON line: algorithmic TFLOP/s of the iteration (the flop count of
+relccsd's own contraction structure), s/iteration, the roofline of the
+dominant kernel (the VVVV ladder GEMM, timed with HIP events on the solver's
+stream) and the relccsd CPU reference timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CCSD s/iter + TFLOP/s at 30o/300v cplx; split-prec vs FP64 at |ΔEc|<1µEh"
+CONFIGS = {
+    # name: (no, nv, seed, description)
+    "cfg1": (6, 40, 1, "C1 6o/40v complex128"),
+    "cfg2": (20, 200, 2, "C1 20o/200v complex128"),
+    "cfg3": (30, 300, 3, "C1 30o/300v complex128"),
+}
+PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
+CPU_SAMPLE = (10, 100)    # relccsd reference sample size (o, v)
+
+
+def ccsd_flops(no: int, nv: int) -> float:
+    """Real flops of one CCSD iteration counted with relccsd's contraction
+    structure (complex FMA = 8 flops): ladder (ccintm.F:712), the two
+    (ov)^3 rings (ccintm.F:445, cceqns.F:1256), OVVV.tau (ccintm.F:783),
+    AINTM / tau.A, GV.T2 / HVINTM, HOINTM / T2.GO, the OVVV.T1 terms,
+    T1EQN1 and HINTM's VOOO.T1."""
+    o, v = no, nv
+    noot, nvvt, ov = o * (o - 1) // 2, v * (v - 1) // 2, o * v
+    cf = (nvvt ** 2 * noot + 2 * ov ** 3 + ov * nvvt * noot + 2 * noot ** 2 * nvvt
+          + 2 * v ** 3 * noot + 2 * o ** 3 * nvvt + 4 * noot * v * nvvt + 4 * nvvt * o * ov
+          + v * v * o * noot + o ** 3 * v * v)
+    return 8.0 * cf
+
+
+def cpu_baseline(workload_flops: float):
+    """relccsd's own CPU CCEQN (built from the reference sources by
+    oracle/Makefile.ref) on a bounded sample, extrapolated by flop count."""
+    from molfdir_amd import synth
+    from oracle import refrun
+    no, nv = CPU_SAMPLE
+    if not refrun.available():
+        return None
+    s = synth.make_system(no, nv, seed=3)
+    t0 = time.perf_counter()
+    r = refrun.run(s, "ccsd", maxit=2, maxdim=6, ntol=12)
+    wall = time.perf_counter() - t0
+    cpu_iter = (r["cceqn_cpu_s"] or wall) / max(1, r["nit"])
+    rate = ccsd_flops(no, nv) / cpu_iter
+    return {
+        "value": rate / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+        "s_per_iter_extrapolated": workload_flops / rate,
+        "sample": (f"relccsd CCEQN (unmodified reference sources, single-threaded netlib "
+                   f"BLAS) at C1 {no}o/{nv}v, 2 iterations, {cpu_iter:.2f} s/iter CPU; "
+                   f"rate extrapolated to the workload by algorithmic flops"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
+    ap.add_argument("--scale", type=float, default=0.05,
+                    help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+
+    no, nv, seed, desc = CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.perf_counter()
+    sysd = synth.make_system(no, nv, seed=seed, scale=args.scale, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] generated {desc} in {t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    r = RelCCSD(no, nv, maxdim=6, maxit=args.steps + args.warmup, ntol=12)
+    r.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
+    for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        r.set_ints(name, sysd[name])
+        sysd[name] = None
+    del sysd
+    torch.cuda.empty_cache()
+    emp2 = r.mp2()
+    print(f"[bench] MP2 {emp2:.10f}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        nit, e, rms = r.iterate()
+        print(f"[bench] warmup it {nit} E {e:.10e} rms {rms:.3e} {r.last_iteration_ms():.1f} ms",
+              file=sys.stderr, flush=True)
+    r.synchronize()
+    torch.cuda.synchronize()
+    iter_ms, kt = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nit, e, rms = r.iterate()
+        iter_ms.append(r.last_iteration_ms())
+        kt.append(r.kernel_times())
+        print(f"[bench] it {nit} E {e:.10e} rms {rms:.3e} {iter_ms[-1]:.1f} ms", file=sys.stderr,
+              flush=True)
+    r.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s_iter = wall / args.steps
+    F = ccsd_flops(no, nv)
+    noot, nvvt, nvo = no * (no - 1) // 2, nv * (nv - 1) // 2, no * nv
+    ladder_ms = sum(k["ladder"] for k in kt) / len(kt)
+    ladder_flops = 8.0 * nvvt * nvvt * noot
+    achieved = ladder_flops / (ladder_ms * 1e-3) / 1e12
+    ring_ms = sum(k["ring_t2"] for k in kt) / len(kt)
+    out = {
+        "metric": METRIC,
+        "value": F / s_iter / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s_iter * 1e3,
+        "s_per_iter": s_iter,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": (f"synthetic (seeded generator: eps_o U[-2,-0.3], eps_v U[0.2,5], rank-120 "
+                 f"Hermitian factor, Re/Im N(0,{args.scale}^2), seed {seed})"),
+        "config": {"workload": f"CCSD iteration, {desc}, FP64", "no": no, "nv": nv,
+                   "precision": "fp64", "diis_maxdim": 6,
+                   "algorithmic_flops_per_iter": F},
+        "roofline": {"bound": "mfma", "kernel": "VVVV ladder ZGEMM (BINTM)",
+                     "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "flops_per_launch": ladder_flops, "avg_launch_ms": ladder_ms},
+        "kernels_ms": {k: sum(x[k] for x in kt) / len(kt) for k in kt[0]},
+        "iteration_event_ms": sum(iter_ms) / len(iter_ms),
+        "energy": {"emp2": emp2, "last_ecc": e, "last_rms": rms},
+    }
+    out["kernels_tflops"] = {
+        "ring_t2": 8.0 * nvo ** 3 / (ring_ms * 1e-3) / 1e12 if ring_ms > 0 else None}
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(F)
+        except Exception as exc:  # reported, never silently dropped
+            out["cpu_baseline"] = {"error": str(exc)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/* relccsd_hip.h -- C ABI of the MI355X relativistic CCSD amplitude solver.
+ *
+ * Drop-in for the CCSD amplitude path of MOLFDIR's relccsd: the Fortran
+ * driver (CCENER ccdriv.F:1574 -> CCEQN cceqns.F:141) keeps its call
+ * sequence and hands its symmetry-sorted complex integral classes and Fock
+ * blocks to this library, which holds them in HBM and runs the per-term
+ * contractions, denominators, DIIS and energy on the GPU.
+ *
+ * Conventions (all as relccsd stores them):
+ *  - complex numbers are interleaved (re, im) doubles (REAL*8 pairs, RCW=2);
+ *  - every array is column-major; class rows are the bra pair, columns the
+ *    ket pair; packed pairs are P>Q, enumerated Q outer / P inner;
+ *  - VO pairs (A,K) are indexed K*nv + A (A fastest);
+ *  - classes: OOOO <KL||IJ>, VOOO <AK||IJ>, VVOO <AB||IJ>, VOVO <AK||CI>,
+ *    VOVV <AK||CD>, VVVV <AB||CD>; Fock FOO(K,I), FVO(A,I), FVV(A,C);
+ *  - amplitudes T1(A,I) (NDIMT1 = nv*no) and packed T2(AB,IJ) (NDIMT2).
+ * Every call returns 0 on success; non-zero means failure and
+ * rcc_last_error() describes it (the Fortran shim turns it into QUIT, the
+ * reference abort convention quit.F:33).  One handle = one device, one HIP
+ * stream; a handle is not shared between host threads.
+ */
+#ifndef RELCCSD_HIP_H
+#define RELCCSD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rcc_handle rcc_t;
+
+/* Integral classes, in the order of the sorted files ft41..ft46
+ * (GETOOOO..GETVVVV, ccgetv.F:89-402). */
+enum {
+    RCC_OOOO = 0, RCC_VOOO = 1, RCC_VVOO = 2, RCC_VOVO = 3, RCC_VOVV = 4, RCC_VVVV = 5
+};
+
+/* Intermediates readable with rcc_get_array (for per-term checks). */
+enum {
+    RCC_ARR_HO = 0, RCC_ARR_HV = 1, RCC_ARR_HOV = 2, RCC_ARR_GO = 3, RCC_ARR_GV = 4,
+    RCC_ARR_AZ = 5, RCC_ARR_H = 6, RCC_ARR_S1 = 7, RCC_ARR_S2 = 8, RCC_ARR_T1 = 9,
+    RCC_ARR_T2 = 10, RCC_ARR_TAU = 11
+};
+
+/* Symmetry description, as RDSYMI / SYMTAB hold it (ccdriv.F:2934, :746).
+ * nrep: number of fermion irreps of the abelian double group (this build:
+ * nrep must be 1, the C1 case); no/nv: occupied / virtual spinors per irrep;
+ * frozen: per occupied spinor (irrep-major) 1 = frozen core (IFROZ,
+ * freeze.inc), may be NULL. */
+typedef struct {
+    int nrep;
+    const int* no;
+    const int* nv;
+    const int* frozen;
+} rcc_sym_t;
+
+/* Replaces SYMTAB's dimension/offset set-up (ccdriv.F:746-1425) and the
+ * CCENER work-array allocation (ccdriv.F:1622-1690) on the device. */
+int rcc_create(const rcc_sym_t* sym, rcc_t** handle);
+void rcc_destroy(rcc_t* h);
+const char* rcc_last_error(const rcc_t* h);
+
+/* MAXDIM (DIIS subspace), MAXIT and NTOL of the &CCENER namelist
+ * (ccdriv.F:208-225).  precision: 0 = FP64 (the only mode of this build). */
+int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision);
+
+/* Fock blocks and orbital energies as FMTOFILE reads them (ccgetv.F:34-87);
+ * eps: occupied then virtual (SYMTAB reordering ccdriv.F:1393-1420). */
+int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,
+                 const double* eps);
+
+/* One column batch [col0, col0+ncols) of integral class cls of irrep
+ * (0-based) from host memory, as GETVOVV / GETVVVV stream them
+ * (ccgetv.F:225-402); whole classes are col0 = 0, ncols = all columns. */
+int rcc_set_ints(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols, const double* buf);
+/* Same, from device memory already resident on this GPU. */
+int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols,
+                        const void* dbuf);
+
+/* Amplitudes in / out (PUTAMPT / GETAMPT ccgetv.F:577-709). */
+int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2);
+int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2);
+/* Copy of a device-resident array (RCC_ARR_*) to host memory. */
+int rcc_get_array(rcc_t* h, int which, double* out, int64_t nelem_complex);
+
+/* MP2EQN (cceqns.F:33-139): T1 = FVO/D, T2 = VVOO/D, ZCORE, MP2ENG energy. */
+int rcc_mp2(rcc_t* h, double* emp2);
+
+/* Per-term entry points of one CCEQN iteration (cceqns.F:313-483), acting
+ * on the handle's current T1/T2 and its device-resident intermediates. */
+int rcc_hointm(rcc_t* h);   /* HOINTM  ccintm.F:455  */
+int rcc_hvintm(rcc_t* h);   /* HVINTM  ccintm.F:567  */
+int rcc_hovintm(rcc_t* h);  /* HOVINTM ccintm.F:514  */
+int rcc_t1eqns(rcc_t* h);   /* T1EQNS  cceqns.F:645 (+T1EQN1 :2521, T1EQN2 :2601) */
+int rcc_gointm(rcc_t* h);   /* GOINTM  ccintm.F:95   */
+int rcc_gvintm(rcc_t* h);   /* GVINTM  ccintm.F:148  */
+int rcc_aintm(rcc_t* h);    /* AINTM   ccintm.F:34   */
+int rcc_hintm(rcc_t* h);    /* HINTM   ccintm.F:221  */
+int rcc_t2eqns(rcc_t* h);   /* T2EQNS  cceqns.F:937  */
+int rcc_bintm(rcc_t* h);    /* BINTM   ccintm.F:625  */
+int rcc_denom(rcc_t* h);    /* DENOM   cceqns.F:2355 */
+int rcc_zcore(rcc_t* h);    /* ZCORE   cceqns.F:1864 */
+int rcc_diis(rcc_t* h, double* rms);       /* DIISX  cceqns.F:1985 */
+int rcc_energy(rcc_t* h, double* ecc);     /* CCENG  cceqns.F:2180 (on S) */
+int rcc_t1diag(rcc_t* h, double* diag);    /* T1DIAG cceqns.F:2453 (on S) */
+/* PUT S1,S2 INTO T1,T2 (cceqns.F:560-561). */
+int rcc_accept(rcc_t* h);
+
+/* One whole iteration (HOINTM .. T1DIAG, then S -> T) = one pass of the
+ * CCEQN loop body (cceqns.F:293-569).  nit counts calls since rcc_mp2. */
+int rcc_iterate(rcc_t* h, int* nit, double* ecc, double* rms);
+
+/* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
+int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);
+
+/* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms. */
+int rcc_last_iteration_ms(rcc_t* h, double* ms);
+/* HIP-event durations (ms, on the handle's stream) of the last iteration's
+ * dominant launches: [0] VVVV ladder (BINTM), [1] HINTM (ov)^3 ring,
+ * [2] T2EQNS (ov)^3 ring, [3] BINTM VOVV.tau product. */
+int rcc_kernel_times(rcc_t* h, double* ms4);
+
+/* Diagnostics: the library's complex GEMM engine on device pointers
+ * (column-major, op 0=N 1=T 2=C), and the MFMA lane-layout probe. */
+int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
+                     const void* A, int64_t lda, const void* B, int64_t ldb,
+                     const double* beta, void* C, int64_t ldc);
+int rcc_probe_mfma(rcc_t* h, double* out256);
+int rcc_synchronize(rcc_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RELCCSD_HIP_H */

@@ -1,0 +1,83 @@
+// Internal declarations shared by the relccsd HIP library sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rcc {
+
+enum { OP_N = 0, OP_T = 1, OP_C = 2 };
+
+// Strided-batched complex GEMM, column-major: C_b = alpha op(A_b) op(B_b) + beta C_b.
+struct GemmArgs {
+    int opA = OP_N, opB = OP_N;
+    int m = 0, n = 0, k = 0;
+    const double2* A = nullptr; int64_t lda = 0, sA = 0;
+    const double2* B = nullptr; int64_t ldb = 0, sB = 0;
+    double2* C = nullptr; int64_t ldc = 0, sC = 0;
+    double2 alpha = {1.0, 0.0}, beta = {0.0, 0.0};
+    int batch = 1;
+    // distinct kernel symbol for profiling: 0 generic, 1 VVVV ladder, 2 (ov)^3 rings
+    int tag = 0;
+};
+
+// ws may be null (no K-split); ws_elems = capacity in complex elements.
+int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
+int mfma_probe(double* dout, hipStream_t st);
+
+// ---- element-wise / permutation kernels (rcc_kernels.hip) ----
+// Packed pair tables: pair p of an n-set -> (x, y), x > y, y outer.
+struct PairTab {
+    const int2* tab;   // device, npair entries
+    int n;
+    int64_t npair;
+};
+
+void k_gettau(const double2* t1, const double2* t2, double2* tau, double s2,
+              PairTab vv, PairTab oo, int nv, hipStream_t st);
+// out[r + R*(y + no*x)] = sgn * P[r, pair(x,y)]  (0 for x == y)
+void k_expand_e2(const double2* P, double2* out, int64_t R, PairTab oo, int no, hipStream_t st);
+// out[a + nv*(d + nv*col)] = sgn * P[pair(a,d), col]
+void k_expand_f1(const double2* P, double2* out, int64_t ncol, PairTab vv, int nv,
+                 hipStream_t st);
+// out[(a + nv*i) + nvo*(c + nv*k)] = s * t2[a,c,i,k] - (t1 ? t1[a,k] t1[c,i] : 0)
+void k_expand_ring(const double2* T2, const double2* t1, double s, double2* out, PairTab vv,
+                   PairTab oo, int nv, int no, hipStream_t st);
+// out[row + R*(c + nv*d)] = sgn * P[row, pair(c,d)]   (column pair expansion)
+void k_expand_cols(const double2* P, double2* out, int64_t R, PairTab vv, int nv,
+                   hipStream_t st);
+// S[R, pair(i,j)] += alpha * (Y[R + NR*(i + no*j)] - Y[R + NR*(j + no*i)])
+void k_pack_pij(const double2* Y, double2* S, int64_t NR, double alpha, PairTab oo, int no,
+                hipStream_t st);
+// S[pair(a,b), col] += alpha * (Y[a + nv*(b + nv*col)] - Y[b + nv*(a + nv*col)])
+void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTab vv, int nv,
+                hipStream_t st);
+// S[ab, ij] += Q[ai,bj] - Q[bi,aj] - Q[aj,bi] + Q[bj,ai]   (ring layout Q)
+void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
+                  hipStream_t st);
+// H[(a,i),(c,k)] (+)= X[(a,k),(c,i)]
+void k_perm_akci(const double2* X, double2* H, int nv, int no, bool accumulate,
+                 hipStream_t st);
+// S1[a,i] += sum_k sgn X[(a,k), pair(i,k)]
+void k_trace_t1eqn2(const double2* X, double2* S1, PairTab oo, int nv, int no, hipStream_t st);
+// GV[a,c] += alpha * sum_k H1[(a,k),(c,k)]   (H1 in (AK,CI) layout)
+void k_trace_gv(const double2* H1, double2* GV, double alpha, int nv, int no, hipStream_t st);
+// eps == nullptr: only the frozen-spinor zeroing (ZCORE); frozen == nullptr: none
+void k_denom(double2* S1, double2* S2, const double* eps, const int* frozen, PairTab vv,
+             PairTab oo, int nv, int no, hipStream_t st);
+// y = a*x + b*y (complex a, b)
+void k_axpby(int64_t n, double2 a, const double2* x, double2 b, double2* y, hipStream_t st);
+void k_conj(int64_t n, const double2* x, double2* y, hipStream_t st);
+// M (n x n) := M with the diagonal zeroed (square, column-major)
+void k_zero_diag(double2* M, int n, hipStream_t st);
+// generic gather: out[i] = sgn[i] * (conj ? conj(in[idx[i]]) : in[idx[i]]), idx<0 -> 0
+void k_gather(const double2* in, const int64_t* idx, const signed char* sgn, bool conj,
+              double2* out, int64_t n, hipStream_t st);
+// part[b] = sum over chunk b of conj(x) * y   (deterministic, fixed order)
+void k_zdotc_multi(const double2* const* xs, int nx, const double2* y, int64_t n,
+                   double2* dst, double2* part, int nblk, hipStream_t st);
+void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, int nblk,
+                    hipStream_t st);
+void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
+               hipStream_t st);
+
+}  // namespace rcc

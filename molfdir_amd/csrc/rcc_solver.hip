@@ -1,0 +1,1031 @@
+// Host side of the MI355X relccsd CCSD amplitude solver and its C ABI.
+//
+// One handle holds, in HBM, the six integral classes in relccsd's packed
+// layouts, the constant re-orderings of them that the contractions read
+// (built once, since the integrals do not change during the iteration), the
+// amplitudes, the intermediates and the DIIS store.  Each per-term function
+// mirrors one relccsd routine: the same intermediate, the same packed
+// layout, and the same sums (triangular C>D / K>L sums are kept wherever the
+// reference keeps them, so the flop count of the ladder and of tau.A is the
+// reference's), expressed as FP64-MFMA complex GEMMs plus HBM-bound
+// expansion / antisymmetrising kernels.
+//
+// Device layouts (C1; o occupied, v virtual, nvo = o v, NOOT / NVVT packed
+// pair counts):
+//   T = [T1 (v x o) | T2 (NVVT x NOOT)], S likewise, TAU (NVVT x NOOT)
+//   VVOO_E2 [(CD,L), K] = <cd||kl>              (HOINTM operand)
+//   VVOO_F1 [C, (D,KL)] = <cd||kl>              (HVINTM operand)
+//   VVOO_R  [(C,K),(D,L)] = <cd||kl>            (HOVINTM, HINTM operand)
+//   VOVO_R  [(A,I),(C,K)] = <ak||ci>            (T1EQNS operand)
+//   VOVV_Q  [(A,K), C + v D] = <ak||cd>         (HINTM / GVINTM operand)
+//   M1 [(C,KL), I] = conj VOOO(CI,KL)           (T1EQN1 operand)
+//   MGO [(C,L),(K,I)] = conj <ci||kl>           (GOINTM operand)
+//   MH  [L,(K,(C,I))] = conj <ci||lk>           (HINTM operand)
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "rcc_internal.h"
+#include "../../include/relccsd_hip.h"
+
+using namespace rcc;
+
+namespace {
+
+constexpr int RED_BLOCKS = 1024;
+constexpr int MAXDIM_CAP = 16;
+
+struct Buf {
+    double2* p = nullptr;
+    int64_t n = 0;
+};
+
+}  // namespace
+
+struct rcc_handle {
+    int no = 0, nv = 0;
+    int64_t noot = 0, nvvt = 0, nvo = 0, nt1 = 0, nt2 = 0;
+    int maxdim = 8, maxit = 30, ntol = 12, precision = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::vector<void*> allocs;
+
+    int2* oo_tab = nullptr;
+    int2* vv_tab = nullptr;
+    PairTab oo{}, vv{};
+    double* eps = nullptr;
+    int* frozen = nullptr;
+    bool any_frozen = false;
+
+    double2 *FOO = nullptr, *FVO = nullptr, *FVV = nullptr;
+    double2* cls[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t cls_rows[6] = {0}, cls_cols[6] = {0};
+    bool derived_ok = false;
+
+    double2 *VVOO_E2 = nullptr, *VVOO_F1 = nullptr, *VVOO_R = nullptr, *VOVO_R = nullptr,
+            *VOVV_Q = nullptr, *M1 = nullptr, *MGO = nullptr, *MH = nullptr;
+
+    double2 *T = nullptr, *S = nullptr, *TAU = nullptr;
+    double2 *HO = nullptr, *HV = nullptr, *HOV = nullptr, *GO = nullptr, *GV = nullptr;
+    double2 *AZ = nullptr, *H = nullptr, *X = nullptr;
+    double2 *E2 = nullptr, *F1 = nullptr, *RING = nullptr, *RING2 = nullptr, *HAKCI = nullptr;
+    double2 *YP = nullptr, *ZX = nullptr, *WC = nullptr, *Y1 = nullptr, *YAZ = nullptr;
+    double2 *SMALL = nullptr;    // o*o + v*o scratch
+    double2* ws = nullptr;
+    int64_t ws_elems = 0;
+    double2* red_part = nullptr;
+    double2* red_dst = nullptr;
+
+    std::vector<double2*> diis_err, diis_amp;
+    int it = 0, itc = 0;
+    std::vector<double> bb2;  // (MAXDIM+1) x (MAXDIM+2), column-major like BB2
+    int nit = 0;
+    bool tau_ok = false, x_ok = false, ring_ok = false;
+    double last_iter_ms = 0.0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // kernel timers: 0 ladder, 1 HINTM ring, 2 T2EQNS ring, 3 BINTM X
+    hipEvent_t kev[4][2] = {};
+    double kms[4] = {0, 0, 0, 0};
+    bool ktimed[4] = {false, false, false, false};
+
+    double2* T1() { return T; }
+    double2* T2() { return T + nt1; }
+    double2* S1() { return S; }
+    double2* S2() { return S + nt1; }
+};
+
+namespace {
+
+#define HIPCHK(h, call)                                                               \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+int dalloc(rcc_t* h, double2** p, int64_t n) {
+    *p = nullptr;
+    if (n <= 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, (size_t)n * sizeof(double2));
+    if (e != hipSuccess) {
+        h->err = "hipMalloc of " + std::to_string((double)n * 16.0 / 1e9) + " GB failed: " +
+                 hipGetErrorString(e);
+        *p = nullptr;
+        return -1;
+    }
+    h->allocs.push_back(*p);
+    return 0;
+}
+
+template <class T>
+int dalloc_raw(rcc_t* h, T** p, int64_t bytes) {
+    hipError_t e = hipMalloc((void**)p, (size_t)std::max<int64_t>(bytes, 16));
+    if (e != hipSuccess) {
+        h->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+        return -1;
+    }
+    h->allocs.push_back(*p);
+    return 0;
+}
+
+inline double2 cplx(double r, double i) { return make_double2(r, i); }
+
+int gemm(rcc_t* h, int opa, int opb, int m, int n, int k, double2 alpha, const double2* A,
+         int64_t lda, const double2* B, int64_t ldb, double2 beta, double2* C, int64_t ldc,
+         int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0, int tag = 0) {
+    GemmArgs g;
+    g.tag = tag;
+    g.opA = opa; g.opB = opb; g.m = m; g.n = n; g.k = k;
+    g.A = A; g.lda = lda; g.sA = sA;
+    g.B = B; g.ldb = ldb; g.sB = sB;
+    g.C = C; g.ldc = ldc; g.sC = sC;
+    g.alpha = alpha; g.beta = beta; g.batch = batch;
+    int rc = zgemm(g, h->ws, h->ws_elems, h->st);
+    if (rc != 0) {
+        h->err = "zgemm launch failed (" + std::to_string(rc) + ")";
+        return -1;
+    }
+    return 0;
+}
+
+void ktick(rcc_t* h, int which, int phase) {
+    (void)hipEventRecord(h->kev[which][phase], h->st);
+    if (phase == 1) h->ktimed[which] = true;
+}
+
+#define CK(x)                 \
+    do {                      \
+        if ((x) != 0) return -1; \
+    } while (0)
+
+inline int64_t pair_index(int x, int y, int n) { return (int64_t)y * (2 * n - y - 1) / 2 + (x - y - 1); }
+
+// Builds the derived constant layouts once all classes are present.
+int build_derived(rcc_t* h) {
+    if (h->derived_ok) return 0;
+    for (int c = 0; c < 6; ++c)
+        if (!h->cls[c]) {
+            h->err = "integral class " + std::to_string(c) + " not set";
+            return -1;
+        }
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
+    hipStream_t st = h->st;
+    k_expand_e2(h->cls[RCC_VVOO], h->VVOO_E2, nvvt, h->oo, o, st);
+    k_expand_f1(h->cls[RCC_VVOO], h->VVOO_F1, noot, h->vv, v, st);
+    k_expand_ring(h->cls[RCC_VVOO], nullptr, 1.0, h->VVOO_R, h->vv, h->oo, v, o, st);
+    k_perm_akci(h->cls[RCC_VOVO], h->VOVO_R, v, o, false, st);
+    k_expand_cols(h->cls[RCC_VOVV], h->VOVV_Q, nvo, h->vv, v, st);
+    // small VOOO re-orderings through host-built gather maps
+    auto gather = [&](double2* out, const std::vector<int64_t>& idx,
+                      const std::vector<signed char>& sg) -> int {
+        const int64_t n = (int64_t)idx.size();
+        if (n == 0) return 0;
+        int64_t* didx = nullptr;
+        signed char* dsg = nullptr;
+        HIPCHK(h, hipMalloc((void**)&didx, n * sizeof(int64_t)));
+        HIPCHK(h, hipMalloc((void**)&dsg, n));
+        HIPCHK(h, hipMemcpyAsync(didx, idx.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HIPCHK(h, hipMemcpyAsync(dsg, sg.data(), n, hipMemcpyHostToDevice, st));
+        k_gather(h->cls[RCC_VOOO], didx, dsg, true, out, n, st);
+        HIPCHK(h, hipStreamSynchronize(st));
+        HIPCHK(h, hipFree(didx));
+        HIPCHK(h, hipFree(dsg));
+        return 0;
+    };
+    {   // M1[(c + v*kl) + v*noot*i] = conj VOOO[(c + v*i) + nvo*kl]
+        std::vector<int64_t> idx((size_t)v * noot * o);
+        std::vector<signed char> sg(idx.size(), 1);
+        for (int i = 0; i < o; ++i)
+            for (int64_t kl = 0; kl < noot; ++kl)
+                for (int c = 0; c < v; ++c)
+                    idx[c + v * kl + (int64_t)v * noot * i] = (c + (int64_t)v * i) + nvo * kl;
+        CK(gather(h->M1, idx, sg));
+    }
+    {   // MGO[(c + v*l) + nvo*(k + o*i)] = conj VOOO_E[(c + v*i), (k,l)]
+        std::vector<int64_t> idx((size_t)nvo * o * o);
+        std::vector<signed char> sg(idx.size(), 1);
+        for (int i = 0; i < o; ++i)
+            for (int k = 0; k < o; ++k)
+                for (int l = 0; l < o; ++l)
+                    for (int c = 0; c < v; ++c) {
+                        const int64_t e = (c + (int64_t)v * l) + nvo * (k + (int64_t)o * i);
+                        if (k == l) { idx[e] = -1; continue; }
+                        const int64_t p = k > l ? pair_index(k, l, o) : pair_index(l, k, o);
+                        idx[e] = (c + (int64_t)v * i) + nvo * p;
+                        sg[e] = k > l ? 1 : -1;
+                    }
+        CK(gather(h->MGO, idx, sg));
+    }
+    {   // MH[l + o*(k + o*ci)] = conj VOOO_E[ci, (l,k)]
+        std::vector<int64_t> idx((size_t)o * o * nvo);
+        std::vector<signed char> sg(idx.size(), 1);
+        for (int64_t ci = 0; ci < nvo; ++ci)
+            for (int k = 0; k < o; ++k)
+                for (int l = 0; l < o; ++l) {
+                    const int64_t e = l + (int64_t)o * (k + (int64_t)o * ci);
+                    if (k == l) { idx[e] = -1; continue; }
+                    const int64_t p = l > k ? pair_index(l, k, o) : pair_index(k, l, o);
+                    idx[e] = ci + nvo * p;
+                    sg[e] = l > k ? 1 : -1;
+                }
+        CK(gather(h->MH, idx, sg));
+    }
+    HIPCHK(h, hipStreamSynchronize(st));
+    h->derived_ok = true;
+    return 0;
+}
+
+int ensure_tau(rcc_t* h) {
+    if (!h->tau_ok) {
+        k_gettau(h->T1(), h->T2(), h->TAU, 1.0, h->vv, h->oo, h->nv, h->st);
+        h->tau_ok = true;
+    }
+    return 0;
+}
+
+int ensure_ring(rcc_t* h) {
+    if (!h->ring_ok) {
+        k_expand_ring(h->T2(), nullptr, 1.0, h->RING, h->vv, h->oo, h->nv, h->no, h->st);
+        h->ring_ok = true;
+    }
+    return 0;
+}
+
+// BINTM's X(AK,IJ) = sum_{C>D} VOVV(AK,CD) TAU(CD,IJ)  (ccintm.F:772-788); also the
+// sum that T1EQN2 (cceqns.F:2601-2714) takes the (I,K) diagonal of.
+int ensure_x(rcc_t* h) {
+    if (!h->x_ok) {
+        CK(ensure_tau(h));
+        ktick(h, 3, 0);
+        CK(gemm(h, OP_N, OP_N, (int)h->nvo, (int)h->noot, (int)h->nvvt, cplx(1, 0),
+                h->cls[RCC_VOVV], h->nvo, h->TAU, h->nvvt, cplx(0, 0), h->X, h->nvo));
+        ktick(h, 3, 1);
+        h->x_ok = true;
+    }
+    return 0;
+}
+
+void invalidate_t(rcc_t* h) {
+    h->tau_ok = false;
+    h->x_ok = false;
+    h->ring_ok = false;
+}
+
+int prep(rcc_t* h) {
+    if (!h->derived_ok) CK(build_derived(h));
+    return 0;
+}
+
+// ---------------------------------------------------------------- terms
+
+int do_hointm(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    const int o = h->no;
+    HIPCHK(h, hipMemcpyAsync(h->HO, h->FOO, (size_t)o * o * 16, hipMemcpyDeviceToDevice, h->st));
+    k_zero_diag(h->HO, o, h->st);
+    k_expand_e2(h->TAU, h->E2, h->nvvt, h->oo, o, h->st);
+    const int64_t K = h->nvvt * o;
+    return gemm(h, OP_C, OP_N, o, o, (int)K, cplx(1, 0), h->VVOO_E2, K, h->E2, K, cplx(1, 0),
+                h->HO, o);
+}
+
+int do_hvintm(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    const int v = h->nv;
+    HIPCHK(h, hipMemcpyAsync(h->HV, h->FVV, (size_t)v * v * 16, hipMemcpyDeviceToDevice, h->st));
+    k_zero_diag(h->HV, v, h->st);
+    k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, h->st);
+    const int64_t K = (int64_t)v * h->noot;
+    return gemm(h, OP_N, OP_C, v, v, (int)K, cplx(-1, 0), h->F1, v, h->VVOO_F1, v, cplx(1, 0),
+                h->HV, v);
+}
+
+int do_hovintm(rcc_t* h) {
+    CK(prep(h));
+    k_conj(h->nvo, h->FVO, h->HOV, h->st);
+    return gemm(h, OP_C, OP_N, (int)h->nvo, 1, (int)h->nvo, cplx(1, 0), h->VVOO_R, h->nvo,
+                h->T1(), h->nvo, cplx(1, 0), h->HOV, h->nvo);
+}
+
+int do_t1eqns(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    CK(ensure_x(h));
+    CK(ensure_ring(h));
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo;
+    hipStream_t st = h->st;
+    double2* S1 = h->S1();
+    double2* W = h->SMALL;             // v x o
+    double2* B2 = h->SMALL + nvo;      // o x o
+    HIPCHK(h, hipMemcpyAsync(S1, h->FVO, (size_t)nvo * 16, hipMemcpyDeviceToDevice, st));
+    // T(A,K) [conj HOV - 2 F]^H T(C,I)   (cceqns.F:736-752)
+    k_conj(nvo, h->FVO, W, st);
+    k_axpby(nvo, cplx(1, 0), h->HOV, cplx(-2, 0), W, st);
+    CK(gemm(h, OP_T, OP_N, o, o, v, cplx(1, 0), W, v, h->T1(), v, cplx(0, 0), B2, o));
+    CK(gemm(h, OP_N, OP_N, v, o, o, cplx(1, 0), h->T1(), v, B2, o, cplx(1, 0), S1, v));
+    // HOV(CK) T2(A,C,I,K)   (cceqns.F:764-771)
+    CK(gemm(h, OP_N, OP_N, (int)nvo, 1, (int)nvo, cplx(1, 0), h->RING, nvo, h->HOV, nvo,
+            cplx(1, 0), S1, nvo));
+    // HV.T - T.HO   (cceqns.F:792,796)
+    CK(gemm(h, OP_N, OP_N, v, o, v, cplx(1, 0), h->HV, v, h->T1(), v, cplx(1, 0), S1, v));
+    CK(gemm(h, OP_N, OP_N, v, o, o, cplx(-1, 0), h->T1(), v, h->HO, o, cplx(1, 0), S1, v));
+    // T1EQN1: V(K,L,C,I) TAU(A,C,K,L)   (cceqns.F:2521-2599)
+    k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, st);
+    const int64_t K = (int64_t)v * h->noot;
+    CK(gemm(h, OP_N, OP_N, v, o, (int)K, cplx(1, 0), h->F1, v, h->M1, K, cplx(1, 0), S1, v));
+    // T1EQN2: V(A,K,C,D) TAU(C,D,I,K)   (cceqns.F:2601-2714)
+    k_trace_t1eqn2(h->X, S1, h->oo, v, o, st);
+    // - W(A,K,C,I) T(C,K)   (cceqns.F:843-855)
+    return gemm(h, OP_N, OP_N, (int)nvo, 1, (int)nvo, cplx(-1, 0), h->VOVO_R, nvo, h->T1(), nvo,
+                cplx(1, 0), S1, nvo);
+}
+
+int do_gointm(rcc_t* h) {
+    CK(prep(h));
+    const int o = h->no, v = h->nv;
+    HIPCHK(h, hipMemcpyAsync(h->GO, h->HO, (size_t)o * o * 16, hipMemcpyDeviceToDevice, h->st));
+    CK(gemm(h, OP_C, OP_N, o, o, v, cplx(1, 0), h->FVO, v, h->T1(), v, cplx(1, 0), h->GO, o));
+    return gemm(h, OP_T, OP_N, o * o, 1, (int)h->nvo, cplx(-1, 0), h->MGO, h->nvo, h->T1(),
+                h->nvo, cplx(1, 0), h->GO, (int64_t)o * o);
+}
+
+int do_gvintm(rcc_t* h) {
+    CK(prep(h));
+    const int o = h->no, v = h->nv;
+    HIPCHK(h, hipMemcpyAsync(h->GV, h->HV, (size_t)v * v * 16, hipMemcpyDeviceToDevice, h->st));
+    CK(gemm(h, OP_N, OP_C, v, v, o, cplx(-1, 0), h->T1(), v, h->FVO, v, cplx(1, 0), h->GV, v));
+    // HAKCI[(a,k),(c,i)] = - sum_d <ak||cd> t(d,i); its (k = i) trace is the
+    // GV term W(AK,CD) T(D,K) (ccintm.F:208-218); HINTM reuses HAKCI.
+    CK(gemm(h, OP_N, OP_N, (int)(h->nvo * v), o, v, cplx(-1, 0), h->VOVV_Q, h->nvo * v, h->T1(), v,
+            cplx(0, 0), h->HAKCI, h->nvo * v));
+    k_trace_gv(h->HAKCI, h->GV, -1.0, v, o, h->st);
+    return 0;
+}
+
+int do_aintm(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    const int o = h->no, v = h->nv;
+    const int64_t noot = h->noot;
+    HIPCHK(h, hipMemcpyAsync(h->AZ, h->cls[RCC_OOOO], (size_t)noot * noot * 16,
+                             hipMemcpyDeviceToDevice, h->st));
+    if (o > 0 && noot > 0) {
+        // YAZ[kl + noot*(j + o*i)] = sum_c conj VOOO[(c,i),kl] t(c,j)
+        CK(gemm(h, OP_C, OP_N, (int)noot, o, v, cplx(1, 0), h->cls[RCC_VOOO], h->nvo, h->T1(), v,
+                cplx(0, 0), h->YAZ, noot, o, v, 0, noot * o));
+        k_pack_pij(h->YAZ, h->AZ, noot, 1.0, h->oo, o, h->st);
+    }
+    return gemm(h, OP_C, OP_N, (int)noot, (int)noot, (int)h->nvvt, cplx(1, 0), h->cls[RCC_VVOO],
+                h->nvvt, h->TAU, h->nvvt, cplx(1, 0), h->AZ, noot);
+}
+
+int do_hintm(rcc_t* h, bool hakci_has_vovv) {
+    CK(prep(h));
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo;
+    hipStream_t st = h->st;
+    if (!hakci_has_vovv)
+        CK(gemm(h, OP_N, OP_N, (int)(nvo * v), o, v, cplx(-1, 0), h->VOVV_Q, nvo * v, h->T1(), v,
+                cplx(0, 0), h->HAKCI, nvo * v));
+    // - W(AK,CI)  (ccintm.F:263)
+    k_axpby(nvo * nvo, cplx(-1, 0), h->cls[RCC_VOVO], cplx(1, 0), h->HAKCI, st);
+    // + T(A,L) W(LK,CI)  (ccintm.F:268-273)
+    CK(gemm(h, OP_N, OP_N, v, (int)(o * nvo), o, cplx(1, 0), h->T1(), v, h->MH, o, cplx(1, 0),
+            h->HAKCI, v));
+    // (AK,CI) -> (AI,CK)  (SRT16, ccintm.F:385)
+    k_perm_akci(h->HAKCI, h->H, v, o, false, st);
+    // + [1/2 T(AD,IL) - T(A,L) T(D,I)] W(KL,CD)  (ccintm.F:393-450)
+    k_expand_ring(h->T2(), h->T1(), 0.5, h->RING2, h->vv, h->oo, v, o, st);
+    ktick(h, 1, 0);
+    CK(gemm(h, OP_N, OP_C, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->RING2, nvo, h->VVOO_R,
+            nvo, cplx(1, 0), h->H, nvo, 1, 0, 0, 0, 2));
+    ktick(h, 1, 1);
+    return 0;
+}
+
+int do_t2eqns(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    CK(ensure_ring(h));
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
+    hipStream_t st = h->st;
+    double2* S2 = h->S2();
+    HIPCHK(h, hipMemcpyAsync(S2, h->cls[RCC_VVOO], (size_t)h->nt2 * 16, hipMemcpyDeviceToDevice, st));
+    // TAU(AB,KL) A(KL,IJ)  (cceqns.F:1013-1026)
+    CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)noot, cplx(1, 0), h->TAU, nvvt, h->AZ, noot,
+            cplx(1, 0), S2, nvvt));
+    if (o > 1) {
+        // P(ij) W(AB,CJ) T1(C,I)  (cceqns.F:1086-1139):  YP[AB,i,j] = sum_c conj VOVV[(c,j),AB] t(c,i)
+        CK(gemm(h, OP_C, OP_N, (int)nvvt, o, v, cplx(1, 0), h->cls[RCC_VOVV], nvo, h->T1(), v,
+                cplx(0, 0), h->YP, nvvt, o, v, 0, nvvt * o));
+        // - P(ij) T2(AB,IK) GO(K,J)  (cceqns.F:1051-1056): E2(T2)[(AB,i),k] = t2(AB,k,i)
+        k_expand_e2(h->T2(), h->E2, nvvt, h->oo, o, st);
+        CK(gemm(h, OP_N, OP_N, (int)(nvvt * o), o, o, cplx(1, 0), h->E2, nvvt * o, h->GO, o,
+                cplx(1, 0), h->YP, nvvt * o));
+        k_pack_pij(h->YP, S2, nvvt, 1.0, h->oo, o, st);
+    }
+    // P(ab)P(ij) T1(A,K) W(BK,IC) T1(C,J) and - P(ab) T1(A,K) W(KB,IJ)  (cceqns.F:1158-1221)
+    k_axpby(nvo * noot, cplx(-1, 0), h->cls[RCC_VOOO], cplx(0, 0), h->WC, st);
+    if (o > 1) {
+        CK(gemm(h, OP_N, OP_N, (int)nvo, o, v, cplx(1, 0), h->cls[RCC_VOVO], nvo, h->T1(), v,
+                cplx(0, 0), h->Y1, nvo, o, v * nvo, 0, nvo * o));
+        k_pack_pij(h->Y1, h->WC, nvo, -1.0, h->oo, o, st);
+    }
+    // P(ab) GV(A,C) T2(CB,IJ)  (cceqns.F:1183-1184)
+    k_expand_f1(h->T2(), h->F1, noot, h->vv, v, st);
+    CK(gemm(h, OP_N, OP_N, v, (int)(v * noot), v, cplx(1, 0), h->GV, v, h->F1, v, cplx(0, 0),
+            h->ZX, v));
+    CK(gemm(h, OP_N, OP_T, v, v, o, cplx(-1, 0), h->T1(), v, h->WC, v, cplx(1, 0), h->ZX, v,
+            (int)noot, 0, nvo, (int64_t)v * v));
+    k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, st);
+    // P(ab)P(ij) H(AI,CK) T2(CK,BJ)  (cceqns.F:1243-1263)
+    ktick(h, 2, 0);
+    CK(gemm(h, OP_N, OP_N, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->H, nvo, h->RING, nvo,
+            cplx(0, 0), h->RING2, nvo, 1, 0, 0, 0, 2));
+    ktick(h, 2, 1);
+    k_pack_pabij(h->RING2, S2, h->vv, h->oo, v, o, st);
+    return 0;
+}
+
+int do_bintm(rcc_t* h) {
+    CK(prep(h));
+    CK(ensure_tau(h));
+    CK(ensure_x(h));
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
+    double2* S2 = h->S2();
+    // ladder: S(AB,IJ) += V(AB,CD) TAU(CD,IJ), C>D  (ccintm.F:700-722)
+    ktick(h, 0, 0);
+    CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VVVV], nvvt,
+            h->TAU, nvvt, cplx(1, 0), S2, nvvt, 1, 0, 0, 0, 1));
+    ktick(h, 0, 1);
+    // - P(ab) T(B,K) X(AK,IJ)  (ccintm.F:760-829)
+    CK(gemm(h, OP_N, OP_T, v, v, o, cplx(-1, 0), h->X, v, h->T1(), v, cplx(0, 0), h->ZX, v,
+            (int)noot, nvo, 0, (int64_t)v * v));
+    k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, h->st);
+    return 0;
+}
+
+// deterministic complex reductions
+int zdot_multi(rcc_t* h, const std::vector<const double2*>& xs, const double2* y, int64_t n,
+               std::vector<double2>& out) {
+    const int nx = (int)xs.size();
+    out.assign(nx, cplx(0, 0));
+    for (int off = 0; off < nx; off += MAXDIM_CAP) {
+        const int cnt = std::min(MAXDIM_CAP, nx - off);
+        k_zdotc_multi(xs.data() + off, cnt, y, n, h->red_dst, h->red_part, RED_BLOCKS, h->st);
+        HIPCHK(h, hipMemcpyAsync(out.data() + off, h->red_dst, cnt * sizeof(double2),
+                                 hipMemcpyDeviceToHost, h->st));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// FLIN (ccgenp.F:666-740) restated; a is IDIM x (n+nr) column-major
+double dott(const double* a, int na, const double* b, int nb, int n) {
+    double d = 0.0;
+    for (int i = 0; i < n; ++i) d = d + a[(int64_t)i * na] * b[(int64_t)i * nb];
+    return d;
+}
+
+double flin(double* A, int idim, int n, int nr) {
+    auto a = [&](int i, int j) -> double& { return A[i + (int64_t)j * idim]; };
+    const int jmax = n + nr;
+    double sign = 1.0, d = 0.0;
+    for (int m = 0; m < n; ++m) {
+        double temp = 0.0;
+        int imax = m;
+        for (int i = m; i < n; ++i) {
+            if (m > 0) a(i, m) = a(i, m) - dott(&a(i, 0), idim, &a(0, m), 1, m);
+            const double aval = a(i, m);
+            if (std::fabs(aval) <= temp) continue;
+            temp = std::fabs(aval);
+            imax = i;
+        }
+        if (temp <= 0.0) return 0.0;
+        if (imax != m) {
+            sign = -sign;
+            for (int j = 0; j < jmax; ++j) std::swap(a(m, j), a(imax, j));
+        }
+        const int jj = m + 1;
+        if (jj >= jmax) continue;
+        if (m == 0) {
+            for (int j = jj; j < jmax; ++j) a(0, j) = a(0, j) / a(0, 0);
+            d = a(0, 0);
+            continue;
+        }
+        for (int j = jj; j < jmax; ++j) a(m, j) = (a(m, j) - dott(&a(m, 0), idim, &a(0, j), 1, m)) / a(m, m);
+        d = d * a(m, m);
+    }
+    for (int i = 0; i < nr; ++i) {
+        const int npi = n + i;
+        for (int k = 2; k <= n; ++k) {
+            const int j = n - k;
+            a(j, npi) = a(j, npi) - dott(&a(j, j + 1), idim, &a(j + 1, npi), 1, k - 1);
+        }
+    }
+    return d * sign;
+}
+
+// DIISX (cceqns.F:1985-2178) on device vectors [S1|S2] vs [T1|T2].
+int do_diis(rcc_t* h, double* rms_out) {
+    const int64_t n = h->nt1 + h->nt2;
+    hipStream_t st = h->st;
+    const int md = h->maxdim;
+    h->it = std::min(h->it + 1, md);
+    h->itc = h->itc + 1;
+    if (h->itc > md) h->itc = 1;
+    const int c = h->itc - 1;
+    double2* d = h->diis_err[c];
+    // d = S - T ; store S
+    HIPCHK(h, hipMemcpyAsync(d, h->S, n * 16, hipMemcpyDeviceToDevice, st));
+    k_axpby(n, cplx(-1, 0), h->T, cplx(1, 0), d, st);
+    HIPCHK(h, hipMemcpyAsync(h->diis_amp[c], h->S, n * 16, hipMemcpyDeviceToDevice, st));
+    // RMS = sqrt(| sum |d|^2 - |sum d|^2 |)
+    k_sum_and_norm(d, n, h->red_dst, h->red_part, RED_BLOCKS, st);
+    double2 sn[2];
+    HIPCHK(h, hipMemcpyAsync(sn, h->red_dst, sizeof(sn), hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    const double rms = std::sqrt(std::fabs(sn[1].x - (sn[0].x * sn[0].x + sn[0].y * sn[0].y)));
+    // B row: Re + Im of <d_n | d_ITC>
+    std::vector<const double2*> xs;
+    for (int k = 0; k < h->it; ++k) xs.push_back(h->diis_err[k]);
+    std::vector<double2> dots;
+    CK(zdot_multi(h, xs, d, n, dots));
+    const int ld = md + 1;
+    auto BB2 = [&](int i, int j) -> double& { return h->bb2[i + (int64_t)j * ld]; };
+    for (int k = 0; k < h->it; ++k) {
+        const double vv = dots[k].x + dots[k].y;
+        BB2(k, c) = vv;
+        BB2(c, k) = vv;
+    }
+    const int it = h->it;
+    std::vector<double> bb((size_t)ld * (md + 2), 0.0);
+    auto BB = [&](int i, int j) -> double& { return bb[i + (int64_t)j * ld]; };
+    for (int i = 0; i < it; ++i)
+        for (int j = 0; j < it; ++j) BB(i, j) = BB2(i, j);
+    double xm = std::fabs(BB(0, 0));
+    for (int i = 0; i < it; ++i)
+        for (int j = 0; j <= i; ++j) xm = std::max(xm, std::fabs(BB(i, j)));
+    xm = 1.0 / xm;
+    for (int i = 0; i < it; ++i)
+        for (int j = 0; j < it; ++j) BB(i, j) *= xm;
+    for (int i = 0; i < it; ++i) {
+        BB(i, it) = -1.0;
+        BB(it, i) = -1.0;
+    }
+    BB(it, it) = 0.0;
+    for (int i = 0; i < it; ++i) BB(i, it + 1) = 0.0;
+    BB(it, it + 1) = -1.0;
+    flin(bb.data(), ld, it + 1, 1);
+    std::vector<double> coef(it);
+    for (int i = 0; i < it; ++i) coef[i] = BB(i, it + 1);
+    std::vector<const double2*> amps;
+    for (int k = 0; k < it; ++k) amps.push_back(h->diis_amp[k]);
+    k_lincomb(amps.data(), coef.data(), it, h->S, n, st);
+    if (rms_out) *rms_out = rms;
+    return 0;
+}
+
+// CCENG (cceqns.F:2180-2270) on S: Re[zdotc(FVO,S1) + zdotc(VVOO,tau(S))]
+int do_energy(rcc_t* h, double* ecc, double* eimag) {
+    k_gettau(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, h->st);
+    h->tau_ok = false;  // TAU now holds tau(S), not tau(T)
+    std::vector<double2> a, b;
+    CK(zdot_multi(h, {h->FVO}, h->S1(), h->nt1, a));
+    CK(zdot_multi(h, {h->cls[RCC_VVOO]}, h->TAU, h->nt2, b));
+    if (ecc) *ecc = a[0].x + b[0].x;
+    if (eimag) *eimag = a[0].y + b[0].y;
+    return 0;
+}
+
+int do_t1diag(rcc_t* h, const double2* t1, double* diag) {
+    std::vector<double2> a;
+    CK(zdot_multi(h, {t1}, t1, h->nt1, a));
+    *diag = std::sqrt(a[0].x / std::max(1, h->no));
+    return 0;
+}
+
+int do_iteration(rcc_t* h, double* ecc, double* rms) {
+    HIPCHK(h, hipEventRecord(h->ev0, h->st));
+    CK(do_hointm(h));
+    CK(do_hvintm(h));
+    CK(do_hovintm(h));
+    CK(do_t1eqns(h));
+    CK(do_gointm(h));
+    CK(do_gvintm(h));
+    CK(do_aintm(h));
+    CK(do_hintm(h, true));
+    CK(do_t2eqns(h));
+    CK(do_bintm(h));
+    k_denom(h->S1(), h->S2(), h->eps, h->any_frozen ? h->frozen : nullptr, h->vv, h->oo, h->nv,
+            h->no, h->st);
+    CK(do_diis(h, rms));
+    CK(do_energy(h, ecc, nullptr));
+    HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
+    invalidate_t(h);
+    HIPCHK(h, hipEventRecord(h->ev1, h->st));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_iter_ms = ms;
+    for (int w = 0; w < 4; ++w) {
+        float km = 0.f;
+        if (h->ktimed[w] && hipEventElapsedTime(&km, h->kev[w][0], h->kev[w][1]) == hipSuccess)
+            h->kms[w] = km;
+        h->ktimed[w] = false;
+    }
+    h->nit += 1;
+    return 0;
+}
+
+int alloc_all(rcc_t* h) {
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
+    CK(dalloc(h, &h->FOO, (int64_t)o * o));
+    CK(dalloc(h, &h->FVO, nvo));
+    CK(dalloc(h, &h->FVV, (int64_t)v * v));
+    CK(dalloc_raw(h, &h->eps, (int64_t)(o + v) * 8));
+    CK(dalloc_raw(h, &h->frozen, (int64_t)std::max(o, 1) * 4));
+    h->cls_rows[RCC_OOOO] = noot; h->cls_cols[RCC_OOOO] = noot;
+    h->cls_rows[RCC_VOOO] = nvo;  h->cls_cols[RCC_VOOO] = noot;
+    h->cls_rows[RCC_VVOO] = nvvt; h->cls_cols[RCC_VVOO] = noot;
+    h->cls_rows[RCC_VOVO] = nvo;  h->cls_cols[RCC_VOVO] = nvo;
+    h->cls_rows[RCC_VOVV] = nvo;  h->cls_cols[RCC_VOVV] = nvvt;
+    h->cls_rows[RCC_VVVV] = nvvt; h->cls_cols[RCC_VVVV] = nvvt;
+    for (int c = 0; c < 6; ++c) CK(dalloc(h, &h->cls[c], h->cls_rows[c] * h->cls_cols[c]));
+    CK(dalloc(h, &h->VVOO_E2, nvvt * o * o));
+    CK(dalloc(h, &h->VVOO_F1, (int64_t)v * v * noot));
+    CK(dalloc(h, &h->VVOO_R, nvo * nvo));
+    CK(dalloc(h, &h->VOVO_R, nvo * nvo));
+    CK(dalloc(h, &h->VOVV_Q, nvo * v * v));
+    CK(dalloc(h, &h->M1, (int64_t)v * noot * o));
+    CK(dalloc(h, &h->MGO, nvo * o * o));
+    CK(dalloc(h, &h->MH, (int64_t)o * o * nvo));
+    const int64_t namp = h->nt1 + h->nt2;
+    CK(dalloc(h, &h->T, namp));
+    CK(dalloc(h, &h->S, namp));
+    CK(dalloc(h, &h->TAU, h->nt2));
+    CK(dalloc(h, &h->HO, (int64_t)o * o));
+    CK(dalloc(h, &h->HV, (int64_t)v * v));
+    CK(dalloc(h, &h->HOV, nvo));
+    CK(dalloc(h, &h->GO, (int64_t)o * o));
+    CK(dalloc(h, &h->GV, (int64_t)v * v));
+    CK(dalloc(h, &h->AZ, noot * noot));
+    CK(dalloc(h, &h->H, nvo * nvo));
+    CK(dalloc(h, &h->X, nvo * noot));
+    CK(dalloc(h, &h->E2, nvvt * o * o));
+    CK(dalloc(h, &h->F1, (int64_t)v * v * noot));
+    CK(dalloc(h, &h->RING, nvo * nvo));
+    CK(dalloc(h, &h->RING2, nvo * nvo));
+    CK(dalloc(h, &h->HAKCI, nvo * nvo));
+    CK(dalloc(h, &h->YP, nvvt * o * o));
+    CK(dalloc(h, &h->ZX, (int64_t)v * v * noot));
+    CK(dalloc(h, &h->WC, nvo * noot));
+    CK(dalloc(h, &h->Y1, nvo * o * o));
+    CK(dalloc(h, &h->YAZ, noot * o * o));
+    CK(dalloc(h, &h->SMALL, (int64_t)o * o + nvo));
+    h->ws_elems = std::max<int64_t>(1 << 22, std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
+                                                               64 * (int64_t)v * v + (int64_t)256 * o * o));
+    CK(dalloc(h, &h->ws, h->ws_elems));
+    CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
+    CK(dalloc(h, &h->red_dst, MAXDIM_CAP));
+    return 0;
+}
+
+int alloc_diis(rcc_t* h) {
+    const int64_t namp = h->nt1 + h->nt2;
+    while ((int)h->diis_err.size() < h->maxdim) {
+        double2 *e = nullptr, *a = nullptr;
+        CK(dalloc(h, &e, namp));
+        CK(dalloc(h, &a, namp));
+        h->diis_err.push_back(e);
+        h->diis_amp.push_back(a);
+    }
+    h->bb2.assign((size_t)(h->maxdim + 1) * (h->maxdim + 2), 0.0);
+    h->it = 0;
+    h->itc = 0;
+    return 0;
+}
+
+int upload(rcc_t* h, double2* dst, const void* src, int64_t n, hipMemcpyKind kind) {
+    HIPCHK(h, hipMemcpyAsync(dst, src, (size_t)n * 16, kind, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
+    if (!out) return -1;
+    *out = nullptr;
+    rcc_t* h = new rcc_t();
+    *out = h;
+    if (!sym || sym->nrep != 1 || !sym->no || !sym->nv) {
+        h->err = "rcc_create: this build supports the C1 double group (nrep = 1) only";
+        return -1;
+    }
+    h->no = sym->no[0];
+    h->nv = sym->nv[0];
+    if (h->no < 0 || h->nv < 0 || h->no > 4096 || h->nv > 8192) {
+        h->err = "rcc_create: unsupported orbital-space size";
+        return -1;
+    }
+    h->noot = (int64_t)h->no * (h->no - 1) / 2;
+    h->nvvt = (int64_t)h->nv * (h->nv - 1) / 2;
+    h->nvo = (int64_t)h->no * h->nv;
+    h->nt1 = h->nvo;
+    h->nt2 = h->nvvt * h->noot;
+    HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    for (int w = 0; w < 4; ++w)
+        for (int p = 0; p < 2; ++p) HIPCHK(h, hipEventCreate(&h->kev[w][p]));
+    CK(alloc_all(h));
+    // pair tables
+    std::vector<int2> oo, vv;
+    for (int y = 0; y < h->no; ++y)
+        for (int x = y + 1; x < h->no; ++x) oo.push_back(make_int2(x, y));
+    for (int y = 0; y < h->nv; ++y)
+        for (int x = y + 1; x < h->nv; ++x) vv.push_back(make_int2(x, y));
+    CK(dalloc_raw(h, &h->oo_tab, (int64_t)std::max<size_t>(oo.size(), 1) * sizeof(int2)));
+    CK(dalloc_raw(h, &h->vv_tab, (int64_t)std::max<size_t>(vv.size(), 1) * sizeof(int2)));
+    if (!oo.empty()) HIPCHK(h, hipMemcpy(h->oo_tab, oo.data(), oo.size() * sizeof(int2), hipMemcpyHostToDevice));
+    if (!vv.empty()) HIPCHK(h, hipMemcpy(h->vv_tab, vv.data(), vv.size() * sizeof(int2), hipMemcpyHostToDevice));
+    h->oo = PairTab{h->oo_tab, h->no, h->noot};
+    h->vv = PairTab{h->vv_tab, h->nv, h->nvvt};
+    std::vector<int> fr(std::max(h->no, 1), 0);
+    if (sym->frozen)
+        for (int i = 0; i < h->no; ++i) {
+            fr[i] = sym->frozen[i] ? 1 : 0;
+            h->any_frozen = h->any_frozen || fr[i];
+        }
+    HIPCHK(h, hipMemcpy(h->frozen, fr.data(), fr.size() * sizeof(int), hipMemcpyHostToDevice));
+    CK(alloc_diis(h));
+    return 0;
+}
+
+void rcc_destroy(rcc_t* h) {
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (void* p : h->allocs) (void)hipFree(p);
+    for (int w = 0; w < 4; ++w)
+        for (int p = 0; p < 2; ++p)
+            if (h->kev[w][p]) (void)hipEventDestroy(h->kev[w][p]);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+}
+
+const char* rcc_last_error(const rcc_t* h) { return h ? h->err.c_str() : "null handle"; }
+
+int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision) {
+    if (!h) return -1;
+    if (maxdim < 1 || maxdim > MAXDIM_CAP) {
+        h->err = "rcc_set_options: maxdim must be 1.." + std::to_string(MAXDIM_CAP);
+        return -1;
+    }
+    if (precision != 0) {
+        h->err = "rcc_set_options: only precision 0 (FP64) is available";
+        return -1;
+    }
+    h->maxdim = maxdim;
+    h->maxit = maxit;
+    h->ntol = ntol;
+    h->precision = precision;
+    return alloc_diis(h);
+}
+
+int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,
+                 const double* eps) {
+    if (!h) return -1;
+    CK(upload(h, h->FOO, foo, (int64_t)h->no * h->no, hipMemcpyHostToDevice));
+    CK(upload(h, h->FVO, fvo, h->nvo, hipMemcpyHostToDevice));
+    CK(upload(h, h->FVV, fvv, (int64_t)h->nv * h->nv, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->eps, eps, (size_t)(h->no + h->nv) * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols,
+                           const void* buf, hipMemcpyKind kind) {
+    if (!h) return -1;
+    if (cls < 0 || cls > 5) {
+        h->err = "rcc_set_ints: bad class";
+        return -1;
+    }
+    if (irrep != 0) {
+        h->err = "rcc_set_ints: irrep out of range (C1 build)";
+        return -1;
+    }
+    if (col0 < 0 || ncols < 0 || col0 + ncols > h->cls_cols[cls]) {
+        h->err = "rcc_set_ints: column batch outside the class";
+        return -1;
+    }
+    const int64_t rows = h->cls_rows[cls];
+    CK(upload(h, h->cls[cls] + col0 * rows, buf, rows * ncols, kind));
+    h->derived_ok = false;
+    return 0;
+}
+
+int rcc_set_ints(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols, const double* buf) {
+    return set_ints_common(h, cls, irrep, col0, ncols, buf, hipMemcpyHostToDevice);
+}
+
+int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols,
+                        const void* dbuf) {
+    return set_ints_common(h, cls, irrep, col0, ncols, dbuf, hipMemcpyDeviceToDevice);
+}
+
+int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
+    if (!h) return -1;
+    if (t1) CK(upload(h, h->T1(), t1, h->nt1, hipMemcpyHostToDevice));
+    if (t2) CK(upload(h, h->T2(), t2, h->nt2, hipMemcpyHostToDevice));
+    invalidate_t(h);
+    return 0;
+}
+
+int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2) {
+    if (!h) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (t1) HIPCHK(h, hipMemcpy(t1, h->T1(), h->nt1 * 16, hipMemcpyDeviceToHost));
+    if (t2) HIPCHK(h, hipMemcpy(t2, h->T2(), h->nt2 * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
+    if (!h) return -1;
+    const double2* src = nullptr;
+    int64_t len = 0;
+    const int o = h->no, v = h->nv;
+    switch (which) {
+        case RCC_ARR_HO: src = h->HO; len = (int64_t)o * o; break;
+        case RCC_ARR_HV: src = h->HV; len = (int64_t)v * v; break;
+        case RCC_ARR_HOV: src = h->HOV; len = h->nvo; break;
+        case RCC_ARR_GO: src = h->GO; len = (int64_t)o * o; break;
+        case RCC_ARR_GV: src = h->GV; len = (int64_t)v * v; break;
+        case RCC_ARR_AZ: src = h->AZ; len = h->noot * h->noot; break;
+        case RCC_ARR_H: src = h->H; len = h->nvo * h->nvo; break;
+        case RCC_ARR_S1: src = h->S1(); len = h->nt1; break;
+        case RCC_ARR_S2: src = h->S2(); len = h->nt2; break;
+        case RCC_ARR_T1: src = h->T1(); len = h->nt1; break;
+        case RCC_ARR_T2: src = h->T2(); len = h->nt2; break;
+        case RCC_ARR_TAU: src = h->TAU; len = h->nt2; break;
+        default: h->err = "rcc_get_array: unknown array"; return -1;
+    }
+    if (n != len) {
+        h->err = "rcc_get_array: size mismatch (expected " + std::to_string(len) + ")";
+        return -1;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(out, src, len * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rcc_mp2(rcc_t* h, double* emp2) {
+    if (!h) return -1;
+    CK(prep(h));
+    HIPCHK(h, hipMemcpyAsync(h->T1(), h->FVO, h->nt1 * 16, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->T2(), h->cls[RCC_VVOO], h->nt2 * 16, hipMemcpyDeviceToDevice, h->st));
+    k_denom(h->T1(), h->T2(), h->eps, h->any_frozen ? h->frozen : nullptr, h->vv, h->oo, h->nv,
+            h->no, h->st);
+    invalidate_t(h);
+    std::vector<double2> a, b;
+    CK(zdot_multi(h, {h->FVO}, h->T1(), h->nt1, a));
+    CK(zdot_multi(h, {h->cls[RCC_VVOO]}, h->T2(), h->nt2, b));
+    if (emp2) *emp2 = a[0].x + b[0].x;
+    h->nit = 0;
+    h->it = 0;
+    h->itc = 0;
+    std::fill(h->bb2.begin(), h->bb2.end(), 0.0);
+    return 0;
+}
+
+#define TERM(name, call) \
+    int name(rcc_t* h) { if (!h) return -1; return call; }
+TERM(rcc_hointm, do_hointm(h))
+TERM(rcc_hvintm, do_hvintm(h))
+TERM(rcc_hovintm, do_hovintm(h))
+TERM(rcc_t1eqns, do_t1eqns(h))
+TERM(rcc_gointm, do_gointm(h))
+TERM(rcc_gvintm, do_gvintm(h))
+TERM(rcc_aintm, do_aintm(h))
+TERM(rcc_hintm, do_hintm(h, false))
+TERM(rcc_t2eqns, do_t2eqns(h))
+TERM(rcc_bintm, do_bintm(h))
+#undef TERM
+
+int rcc_denom(rcc_t* h) {
+    if (!h) return -1;
+    k_denom(h->S1(), h->S2(), h->eps, nullptr, h->vv, h->oo, h->nv, h->no, h->st);
+    return 0;
+}
+
+int rcc_zcore(rcc_t* h) {
+    if (!h) return -1;
+    if (h->any_frozen)
+        k_denom(h->S1(), h->S2(), nullptr, h->frozen, h->vv, h->oo, h->nv, h->no, h->st);
+    return 0;
+}
+
+int rcc_diis(rcc_t* h, double* rms) { return h ? do_diis(h, rms) : -1; }
+
+int rcc_energy(rcc_t* h, double* ecc) { return h ? do_energy(h, ecc, nullptr) : -1; }
+
+int rcc_t1diag(rcc_t* h, double* diag) { return h ? do_t1diag(h, h->S1(), diag) : -1; }
+
+int rcc_accept(rcc_t* h) {
+    if (!h) return -1;
+    HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
+    invalidate_t(h);
+    return 0;
+}
+
+int rcc_iterate(rcc_t* h, int* nit, double* ecc, double* rms) {
+    if (!h) return -1;
+    double e = 0, r = 0;
+    CK(do_iteration(h, &e, &r));
+    if (nit) *nit = h->nit;
+    if (ecc) *ecc = e;
+    if (rms) *rms = r;
+    return 0;
+}
+
+int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms) {
+    if (!h) return -1;
+    const double conv = std::pow(10.0, -h->ntol);
+    double e = 0, r = 1.0;
+    int n = 0;
+    while (n < h->maxit) {
+        CK(do_iteration(h, &e, &r));
+        ++n;
+        if (r < conv) break;
+    }
+    if (nit) *nit = n;
+    if (ecc) *ecc = e;
+    if (rms) *rms = r;
+    return 0;
+}
+
+int rcc_last_iteration_ms(rcc_t* h, double* ms) {
+    if (!h || !ms) return -1;
+    *ms = h->last_iter_ms;
+    return 0;
+}
+
+int rcc_kernel_times(rcc_t* h, double* ms4) {
+    if (!h || !ms4) return -1;
+    for (int w = 0; w < 4; ++w) ms4[w] = h->kms[w];
+    return 0;
+}
+
+int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
+                     const void* A, int64_t lda, const void* B, int64_t ldb, const double* beta,
+                     void* C, int64_t ldc) {
+    if (!h) return -1;
+    if (opa < 0 || opa > 2 || opb < 0 || opb > 2) {
+        h->err = "rcc_zgemm_device: op must be 0, 1 or 2";
+        return -1;
+    }
+    CK(gemm(h, opa, opb, m, n, k, cplx(alpha[0], alpha[1]), (const double2*)A, lda,
+            (const double2*)B, ldb, cplx(beta[0], beta[1]), (double2*)C, ldc));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int rcc_probe_mfma(rcc_t* h, double* out256) {
+    if (!h) return -1;
+    double* d = nullptr;
+    HIPCHK(h, hipMalloc((void**)&d, 256 * 8));
+    if (mfma_probe(d, h->st) != 0) {
+        h->err = "probe launch failed";
+        (void)hipFree(d);
+        return -1;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(out256, d, 256 * 8, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipFree(d));
+    return 0;
+}
+
+int rcc_synchronize(rcc_t* h) {
+    if (!h) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+}  // extern "C"

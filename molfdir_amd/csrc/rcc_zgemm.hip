@@ -1,0 +1,375 @@
+// Complex double-precision GEMM on CDNA4 FP64 matrix cores (gfx950).
+//
+//   C(m x n) = alpha * op(A)(m x k) * op(B)(k x n) + beta * C      (column-major)
+//
+// op is N (as stored), T (transposed) or C (conjugate-transposed).  Operands
+// are interleaved complex (re, im) doubles, the layout relccsd's REAL*8 pairs
+// use (RCW = 2).  Tiles are staged through LDS de-interleaved into planar
+// real / imaginary arrays; each complex product is four real
+// v_mfma_f64_16x16x4_f64 products (re*re - im*im, re*im + im*re) accumulating
+// in FP64.  The imaginary part of B is kept negated in a register for the
+// re*re - im*im term, so no accumulator is ever subtracted.
+//
+// Work decomposition: a 256-thread workgroup (4 wave64) owns a 64 x 64
+// complex C tile, each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks of 16 x 16).
+// K is walked in steps of 16 with the next global tile prefetched into
+// registers while the current LDS tile feeds the MFMAs.  grid.z enumerates
+// (batch, K-split).  K-splits write raw partial sums to a workspace that a
+// second kernel reduces in a fixed order, so results are deterministic.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <algorithm>
+#include "rcc_internal.h"
+
+namespace rcc {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64;
+constexpr int BN = 64;
+constexpr int BK = 16;
+constexpr int PAD = 16;          // LDS row padding (doubles), breaks bank aliasing
+constexpr int LDA_S = BM + PAD;
+constexpr int LDB_S = BN + PAD;
+
+template <int OP>
+__device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t idx, bool ok) {
+    double2 v = ok ? p[idx] : make_double2(0.0, 0.0);
+    if (OP == OP_C) v.y = -v.y;
+    return v;
+}
+
+// Loads the BM x BK tile of op(A) starting at (m0, k0) into registers.
+template <int OPA>
+__device__ __forceinline__ void load_a(const double2* __restrict__ A, int64_t lda, int m, int k,
+                                       int m0, int k0, int tid, double2 (&ra)[4]) {
+    if (OPA == OP_N) {
+        const int row = tid & 63;
+        const int kc = tid >> 6;                 // 0..3
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gm = m0 + row, gk = k0 + kc + 4 * j;
+            ra[j] = ld_op<OPA>(A, (int64_t)gm + (int64_t)gk * lda, gm < m && gk < k);
+        }
+    } else {
+        const int kk = tid & 15;
+        const int row = tid >> 4;                // 0..15
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gm = m0 + row + 16 * j, gk = k0 + kk;
+            ra[j] = ld_op<OPA>(A, (int64_t)gk + (int64_t)gm * lda, gm < m && gk < k);
+        }
+    }
+}
+
+template <int OPA>
+__device__ __forceinline__ void store_a(double* __restrict__ sre, double* __restrict__ sim,
+                                        int tid, const double2 (&ra)[4]) {
+    if (OPA == OP_N) {
+        const int row = tid & 63;
+        const int kc = tid >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sre[(kc + 4 * j) * LDA_S + row] = ra[j].x;
+            sim[(kc + 4 * j) * LDA_S + row] = ra[j].y;
+        }
+    } else {
+        const int kk = tid & 15;
+        const int row = tid >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sre[kk * LDA_S + row + 16 * j] = ra[j].x;
+            sim[kk * LDA_S + row + 16 * j] = ra[j].y;
+        }
+    }
+}
+
+// Loads the BK x BN tile of op(B) starting at (k0, n0).
+template <int OPB>
+__device__ __forceinline__ void load_b(const double2* __restrict__ B, int64_t ldb, int n, int k,
+                                       int n0, int k0, int tid, double2 (&rb)[4]) {
+    if (OPB == OP_N) {
+        const int kk = tid & 15;
+        const int col = tid >> 4;                // 0..15
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + col + 16 * j, gk = k0 + kk;
+            rb[j] = ld_op<OPB>(B, (int64_t)gk + (int64_t)gn * ldb, gn < n && gk < k);
+        }
+    } else {
+        const int col = tid & 63;
+        const int kc = tid >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + col, gk = k0 + kc + 4 * j;
+            rb[j] = ld_op<OPB>(B, (int64_t)gn + (int64_t)gk * ldb, gn < n && gk < k);
+        }
+    }
+}
+
+template <int OPB>
+__device__ __forceinline__ void store_b(double* __restrict__ sre, double* __restrict__ sim,
+                                        int tid, const double2 (&rb)[4]) {
+    if (OPB == OP_N) {
+        const int kk = tid & 15;
+        const int col = tid >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sre[kk * LDB_S + col + 16 * j] = rb[j].x;
+            sim[kk * LDB_S + col + 16 * j] = rb[j].y;
+        }
+    } else {
+        const int col = tid & 63;
+        const int kc = tid >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            sre[(kc + 4 * j) * LDB_S + col] = rb[j].x;
+            sim[(kc + 4 * j) * LDB_S + col] = rb[j].y;
+        }
+    }
+}
+
+// One workgroup: 64 x 64 tile of C for batch index / K-split given by blockIdx.z.
+// split == 1: C = alpha * acc + beta * C.  split > 1: raw acc goes to
+// W[(z) * m * n + i + j * m] and zgemm_reduce finishes the epilogue.
+template <int OPA, int OPB, int TAG>
+__global__ void __launch_bounds__(256)
+zgemm_mfma_kernel(int m, int n, int k,
+                  const double2* __restrict__ A, int64_t lda, int64_t sA,
+                  const double2* __restrict__ B, int64_t ldb, int64_t sB,
+                  double2* __restrict__ C, int64_t ldc, int64_t sC,
+                  double2 alpha, double2 beta, int nsplit, int kchunk,
+                  double2* __restrict__ W) {
+    __shared__ double As_re[BK * LDA_S];
+    __shared__ double As_im[BK * LDA_S];
+    __shared__ double Bs_re[BK * LDB_S];
+    __shared__ double Bs_im[BK * LDB_S];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.x * BM;
+    const int n0 = blockIdx.y * BN;
+    const int z = blockIdx.z;
+    const int batch = z / nsplit;
+    const int split = z - batch * nsplit;
+    A += (int64_t)batch * sA;
+    B += (int64_t)batch * sB;
+    const int kbeg = split * kchunk;
+    const int kend = min(k, kbeg + kchunk);
+
+    v4d acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc_re[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc_im[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
+
+    double2 ra[4], rb[4];
+    if (kbeg < kend) {
+        load_a<OPA>(A, lda, m, kend, m0, kbeg, tid, ra);
+        load_b<OPB>(B, ldb, n, kend, n0, kbeg, tid, rb);
+    }
+    const int lr = lane & 15;        // row (A) / column (B) inside a 16-block
+    const int lk = lane >> 4;        // k inside a 4-step
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        __syncthreads();
+        store_a<OPA>(As_re, As_im, tid, ra);
+        store_b<OPB>(Bs_re, Bs_im, tid, rb);
+        __syncthreads();
+        if (k0 + BK < kend) {
+            load_a<OPA>(A, lda, m, kend, m0, k0 + BK, tid, ra);
+            load_b<OPB>(B, ldb, n, kend, n0, k0 + BK, tid, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double are[2], aim[2], bre[2], bim[2], bnim[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = wm * 32 + i * 16 + lr;
+                are[i] = As_re[(kk + lk) * LDA_S + r];
+                aim[i] = As_im[(kk + lk) * LDA_S + r];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int c = wn * 32 + j * 16 + lr;
+                bre[j] = Bs_re[(kk + lk) * LDB_S + c];
+                bim[j] = Bs_im[(kk + lk) * LDB_S + c];
+                bnim[j] = -bim[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc_re[i][j], 0, 0, 0);
+                    acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bnim[j], acc_re[i][j], 0, 0, 0);
+                    acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc_im[i][j], 0, 0, 0);
+                    acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc_im[i][j], 0, 0, 0);
+                }
+        }
+    }
+
+    // D layout of v_mfma_f64_16x16x4_f64: lane l, element r -> row 4*(l/16)+r, col l%16
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * 32 + i * 16 + 4 * lk + r;
+                const int gn = n0 + wn * 32 + j * 16 + lr;
+                if (gm < m && gn < n) {
+                    const double xr = acc_re[i][j][r], xi = acc_im[i][j][r];
+                    if (nsplit > 1) {
+                        W[(int64_t)z * m * n + gm + (int64_t)gn * m] = make_double2(xr, xi);
+                    } else {
+                        double2* cp = C + (int64_t)batch * sC + gm + (int64_t)gn * ldc;
+                        double2 o;
+                        o.x = alpha.x * xr - alpha.y * xi;
+                        o.y = alpha.x * xi + alpha.y * xr;
+                        if (beta.x != 0.0 || beta.y != 0.0) {
+                            const double2 c = *cp;
+                            o.x += beta.x * c.x - beta.y * c.y;
+                            o.y += beta.x * c.y + beta.y * c.x;
+                        }
+                        *cp = o;
+                    }
+                }
+            }
+}
+
+// Sums the K-split partials in split order and applies alpha / beta.
+__global__ void zgemm_reduce_kernel(int m, int n, int nsplit, int batch,
+                                    const double2* __restrict__ W,
+                                    double2* __restrict__ C, int64_t ldc, int64_t sC,
+                                    double2 alpha, double2 beta) {
+    const int64_t mn = (int64_t)m * n;
+    const int64_t total = mn * batch;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / mn;
+        const int64_t e = t - b * mn;
+        const int64_t i = e % m, j = e / m;
+        double sr = 0.0, si = 0.0;
+        const double2* w = W + b * nsplit * mn + e;
+        for (int s = 0; s < nsplit; ++s) {
+            const double2 v = w[(int64_t)s * mn];
+            sr += v.x;
+            si += v.y;
+        }
+        double2* cp = C + b * sC + i + j * ldc;
+        double2 o;
+        o.x = alpha.x * sr - alpha.y * si;
+        o.y = alpha.x * si + alpha.y * sr;
+        if (beta.x != 0.0 || beta.y != 0.0) {
+            const double2 c = *cp;
+            o.x += beta.x * c.x - beta.y * c.y;
+            o.y += beta.x * c.y + beta.y * c.x;
+        }
+        *cp = o;
+    }
+}
+
+__global__ void zscale_kernel(int m, int n, int batch, double2* __restrict__ C, int64_t ldc,
+                              int64_t sC, double2 beta) {
+    const int64_t mn = (int64_t)m * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < mn * batch;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / mn, e = t - b * mn;
+        double2* cp = C + b * sC + (e % m) + (e / m) * ldc;
+        const double2 c = *cp;
+        double2 o;
+        o.x = beta.x * c.x - beta.y * c.y;
+        o.y = beta.x * c.y + beta.y * c.x;
+        if (beta.x == 0.0 && beta.y == 0.0) o = make_double2(0.0, 0.0);
+        *cp = o;
+    }
+}
+
+template <int OPA, int OPB, int TAG>
+static void launch_t(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
+    dim3 grid((g.m + BM - 1) / BM, (g.n + BN - 1) / BN, g.batch * nsplit);
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG>), grid, dim3(256), 0, st,
+                       g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
+                       g.alpha, g.beta, nsplit, kchunk, W);
+}
+
+template <int OPA, int OPB>
+static void launch(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
+    if (OPA == OP_N && OPB == OP_N && g.tag == 1) launch_t<OPA, OPB, 1>(g, nsplit, kchunk, W, st);
+    else if (OPA == OP_N && (OPB == OP_N || OPB == OP_C) && g.tag == 2)
+        launch_t<OPA, OPB, 2>(g, nsplit, kchunk, W, st);
+    else launch_t<OPA, OPB, 0>(g, nsplit, kchunk, W, st);
+}
+
+int choose_split(const GemmArgs& g, int64_t ws_elems) {
+    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN) * g.batch;
+    if (tiles >= 512 || g.k < 512) return 1;
+    int64_t s = std::min<int64_t>((1024 + tiles - 1) / tiles, (g.k + 255) / 256);
+    s = std::min<int64_t>(s, 256);
+    const int64_t per = (int64_t)g.m * g.n * g.batch;
+    if (per > 0) s = std::min<int64_t>(s, ws_elems / per);
+    return (int)std::max<int64_t>(1, s);
+}
+
+int zgemm(const GemmArgs& g0, double2* ws, int64_t ws_elems, hipStream_t st) {
+    GemmArgs g = g0;
+    if (g.m <= 0 || g.n <= 0 || g.batch <= 0) return 0;
+    if (g.k <= 0) {
+        const int64_t tot = (int64_t)g.m * g.n * g.batch;
+        const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65535);
+        hipLaunchKernelGGL(zscale_kernel, dim3(blocks), dim3(256), 0, st, g.m, g.n, g.batch,
+                           g.C, g.ldc, g.sC, g.beta);
+        return 0;
+    }
+    int nsplit = ws ? choose_split(g, ws_elems) : 1;
+    int kchunk = g.k;
+    if (nsplit > 1) {
+        kchunk = (g.k + nsplit - 1) / nsplit;
+        kchunk = (kchunk + BK - 1) / BK * BK;
+        nsplit = (g.k + kchunk - 1) / kchunk;
+    }
+    if (nsplit <= 1) {
+        nsplit = 1;
+        kchunk = g.k;
+    }
+    const double2* W = nullptr;
+    double2* Wm = nsplit > 1 ? ws : nullptr;
+#define RCC_L(a, b) if (g.opA == a && g.opB == b) { launch<a, b>(g, nsplit, kchunk, Wm, st); }
+    RCC_L(OP_N, OP_N) else RCC_L(OP_N, OP_T) else RCC_L(OP_N, OP_C)
+    else RCC_L(OP_T, OP_N) else RCC_L(OP_T, OP_T) else RCC_L(OP_T, OP_C)
+    else RCC_L(OP_C, OP_N) else RCC_L(OP_C, OP_T) else RCC_L(OP_C, OP_C)
+    else return -1;
+#undef RCC_L
+    (void)W;
+    if (nsplit > 1) {
+        const int64_t tot = (int64_t)g.m * g.n * g.batch;
+        const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65535);
+        hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(blocks), dim3(256), 0, st, g.m, g.n, nsplit,
+                           g.batch, Wm, g.C, g.ldc, g.sC, g.alpha, g.beta);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// Layout probe: one v_mfma_f64_16x16x4_f64 with A[i][k] = i + 16 k and
+// B[k][j] = (k == j % 4) ? 1 : 0... the host checks D against the assumed
+// lane mapping.  out[64 * 4]: lane-major D registers.
+__global__ void mfma_probe_kernel(double* out) {
+    const int l = threadIdx.x;
+    const int row = l & 15, kk = l >> 4;
+    const double a = (double)(row + 16 * kk);
+    const double b = (double)((kk + 1) * 1000 + (l & 15));
+    v4d c = (v4d){0.0, 0.0, 0.0, 0.0};
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[l * 4 + r] = c[r];
+}
+
+int mfma_probe(double* dout, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, st, dout);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace rcc

@@ -1,0 +1,266 @@
+"""Host-side mirror of relccsd's CCSD amplitude path over the HIP C ABI.
+
+``RelCCSD`` plays the part of relccsd's energy driver CCENER
+(ccdriv.F:1574-1861): it takes the Fock blocks and the six sorted integral
+classes in relccsd's layouts (see molfdir_amd.synth), forms the MP2 guess
+(MP2EQN cceqns.F:33) and runs the CCEQN iteration (cceqns.F:141-643) -- all
+on the GPU through librelccsd_hip.so.  The per-term methods carry the names
+of the relccsd routines they replace.
+
+There is no CPU fallback: constructing a solver without the built library
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librelccsd_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "relccsd_hip.h")
+
+CLASSES = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5}
+ARRAYS = {"HO": 0, "HV": 1, "HOV": 2, "GO": 3, "GV": 4, "AZ": 5, "H": 6, "S1": 7,
+          "S2": 8, "T1": 9, "T2": 10, "TAU": 11}
+TERMS = ("hointm", "hvintm", "hovintm", "t1eqns", "gointm", "gvintm", "aintm", "hintm",
+         "t2eqns", "bintm", "denom", "zcore", "accept")
+
+_lib = None
+
+
+class RccError(RuntimeError):
+    pass
+
+
+class _Sym(ctypes.Structure):
+    _fields_ = [("nrep", ctypes.c_int), ("no", ctypes.POINTER(ctypes.c_int)),
+                ("nv", ctypes.POINTER(ctypes.c_int)), ("frozen", ctypes.POINTER(ctypes.c_int))]
+
+
+def declared_symbols() -> list[str]:
+    """Names of every function include/relccsd_hip.h declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(rcc_\w+)\s*\(", text, re.M)))
+
+
+def load():
+    """Loads librelccsd_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RccError(f"{LIB_PATH} is missing: build it with `python -m molfdir_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i64, ci, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    lib.rcc_create.argtypes = [ctypes.POINTER(_Sym), ctypes.POINTER(vp)]
+    lib.rcc_destroy.argtypes = [vp]
+    lib.rcc_destroy.restype = None
+    lib.rcc_last_error.argtypes = [vp]
+    lib.rcc_last_error.restype = ctypes.c_char_p
+    lib.rcc_set_options.argtypes = [vp, ci, ci, ci, ci]
+    lib.rcc_set_fock.argtypes = [vp, vp, vp, vp, vp]
+    lib.rcc_set_ints.argtypes = [vp, ci, ci, i64, i64, vp]
+    lib.rcc_set_ints_device.argtypes = [vp, ci, ci, i64, i64, vp]
+    lib.rcc_set_amplitudes.argtypes = [vp, vp, vp]
+    lib.rcc_get_amplitudes.argtypes = [vp, vp, vp]
+    lib.rcc_get_array.argtypes = [vp, ci, vp, i64]
+    lib.rcc_mp2.argtypes = [vp, dp]
+    for t in TERMS:
+        getattr(lib, "rcc_" + t).argtypes = [vp]
+    lib.rcc_diis.argtypes = [vp, dp]
+    lib.rcc_energy.argtypes = [vp, dp]
+    lib.rcc_t1diag.argtypes = [vp, dp]
+    lib.rcc_iterate.argtypes = [vp, ctypes.POINTER(ci), dp, dp]
+    lib.rcc_cceqn.argtypes = [vp, ctypes.POINTER(ci), dp, dp]
+    lib.rcc_last_iteration_ms.argtypes = [vp, dp]
+    lib.rcc_kernel_times.argtypes = [vp, dp]
+    lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
+    lib.rcc_probe_mfma.argtypes = [vp, dp]
+    lib.rcc_synchronize.argtypes = [vp]
+    _lib = lib
+    return lib
+
+
+def _host(x) -> np.ndarray:
+    if hasattr(x, "detach"):
+        x = x.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(x))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class RelCCSD:
+    """One C1 relativistic CCSD problem resident on the current GPU."""
+
+    def __init__(self, no: int, nv: int, frozen=None, maxdim: int = 8, maxit: int = 30,
+                 ntol: int = 12):
+        self.lib = load()
+        self.no, self.nv = int(no), int(nv)
+        self.noot, self.nvvt = self.no * (self.no - 1) // 2, self.nv * (self.nv - 1) // 2
+        self.nt1, self.nt2 = self.no * self.nv, self.noot * self.nvvt
+        self._no = (ctypes.c_int * 1)(self.no)
+        self._nv = (ctypes.c_int * 1)(self.nv)
+        fr = None
+        if frozen is not None:
+            self._fr = (ctypes.c_int * max(1, self.no))(*[int(bool(x)) for x in frozen])
+            fr = ctypes.cast(self._fr, ctypes.POINTER(ctypes.c_int))
+        sym = _Sym(1, ctypes.cast(self._no, ctypes.POINTER(ctypes.c_int)),
+                   ctypes.cast(self._nv, ctypes.POINTER(ctypes.c_int)), fr)
+        h = ctypes.c_void_p()
+        rc = self.lib.rcc_create(ctypes.byref(sym), ctypes.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.rcc_last_error(h).decode() if h.value else "rcc_create failed"
+            self.close()
+            raise RccError(msg)
+        self.set_options(maxdim, maxit, ntol)
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc):
+        if rc != 0:
+            raise RccError(self.lib.rcc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.rcc_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, maxdim=8, maxit=30, ntol=12, precision=0):
+        self.maxdim, self.maxit, self.ntol = maxdim, maxit, ntol
+        self._check(self.lib.rcc_set_options(self.h, maxdim, maxit, ntol, precision))
+
+    # ------------------------------------------------------------ inputs
+    def set_fock(self, foo, fvo, fvv, eps):
+        """FMTOFILE's Fock blocks (ccgetv.F:34) and the orbital energies."""
+        a = [np.ascontiguousarray(_host(x), dtype=np.complex128) for x in (foo, fvo, fvv)]
+        e = np.ascontiguousarray(_host(eps), dtype=np.float64)
+        self._check(self.lib.rcc_set_fock(self.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(e)))
+
+    def set_ints(self, name, data, col0=0, ncols=None):
+        cls = CLASSES[name]
+        rows = {"OOOO": self.noot, "VOOO": self.nt1, "VVOO": self.nvvt, "VOVO": self.nt1,
+                "VOVV": self.nt1, "VVVV": self.nvvt}[name]
+        if hasattr(data, "is_cuda") and data.is_cuda:
+            import torch
+            torch.cuda.synchronize()     # the producer ran on torch's stream
+            n = data.numel()
+            ncols = n // max(rows, 1) if ncols is None else ncols
+            if n != rows * ncols or data.dtype.__str__() != "torch.complex128" or not data.is_contiguous():
+                raise RccError(f"{name}: need a contiguous complex128 tensor of {rows}x{ncols}")
+            self._check(self.lib.rcc_set_ints_device(self.h, cls, 0, col0, ncols,
+                                                     ctypes.c_void_p(data.data_ptr())))
+            return
+        a = np.ascontiguousarray(_host(data), dtype=np.complex128).reshape(-1)
+        ncols = a.size // max(rows, 1) if ncols is None else ncols
+        if a.size != rows * ncols:
+            raise RccError(f"{name}: {a.size} elements is not {rows} x {ncols}")
+        self._check(self.lib.rcc_set_ints(self.h, cls, 0, col0, ncols, _ptr(a)))
+
+    def set_system(self, sysd):
+        """Uploads a system dict in relccsd layout (molfdir_amd.synth)."""
+        self.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
+        for name in CLASSES:
+            self.set_ints(name, sysd[name])
+
+    def set_amplitudes(self, t1, t2):
+        a = np.ascontiguousarray(_host(t1), dtype=np.complex128)
+        b = np.ascontiguousarray(_host(t2), dtype=np.complex128)
+        assert a.size == self.nt1 and b.size == self.nt2
+        self._check(self.lib.rcc_set_amplitudes(self.h, _ptr(a), _ptr(b)))
+
+    def get_amplitudes(self):
+        a = np.empty(self.nt1, np.complex128)
+        b = np.empty(self.nt2, np.complex128)
+        self._check(self.lib.rcc_get_amplitudes(self.h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def get(self, name):
+        n = {"HO": self.no ** 2, "HV": self.nv ** 2, "HOV": self.nt1, "GO": self.no ** 2,
+             "GV": self.nv ** 2, "AZ": self.noot ** 2, "H": self.nt1 ** 2, "S1": self.nt1,
+             "S2": self.nt2, "T1": self.nt1, "T2": self.nt2, "TAU": self.nt2}[name]
+        out = np.empty(n, np.complex128)
+        self._check(self.lib.rcc_get_array(self.h, ARRAYS[name], _ptr(out), n))
+        return out
+
+    # ------------------------------------------------------------ relccsd routines
+    def mp2(self):
+        """MP2EQN: T = (FVO, VVOO)/D; returns the MP2 correlation energy."""
+        e = ctypes.c_double()
+        self._check(self.lib.rcc_mp2(self.h, ctypes.byref(e)))
+        return e.value
+
+    def term(self, name):
+        self._check(getattr(self.lib, "rcc_" + name)(self.h))
+
+    def diis(self):
+        r = ctypes.c_double()
+        self._check(self.lib.rcc_diis(self.h, ctypes.byref(r)))
+        return r.value
+
+    def energy(self):
+        e = ctypes.c_double()
+        self._check(self.lib.rcc_energy(self.h, ctypes.byref(e)))
+        return e.value
+
+    def t1diag(self):
+        d = ctypes.c_double()
+        self._check(self.lib.rcc_t1diag(self.h, ctypes.byref(d)))
+        return d.value
+
+    def iterate(self):
+        """One CCEQN iteration; returns (nit, Ec, rms)."""
+        n, e, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.rcc_iterate(self.h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(r)))
+        return n.value, e.value, r.value
+
+    def cceqn(self):
+        """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
+        Returns dict(emp2, rows=[(nit, Ec, rms)], eccsd, nit)."""
+        emp2 = self.mp2()
+        rows = []
+        conv = 10.0 ** (-self.ntol)
+        for _ in range(self.maxit):
+            nit, e, r = self.iterate()
+            rows.append((nit, e, r))
+            if r < conv:
+                break
+        return {"emp2": emp2, "rows": rows, "eccsd": rows[-1][1] if rows else 0.0,
+                "nit": len(rows)}
+
+    def last_iteration_ms(self):
+        ms = ctypes.c_double()
+        self._check(self.lib.rcc_last_iteration_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_times(self):
+        """HIP-event ms of the last iteration's ladder, two rings and VOVV.tau."""
+        out = (ctypes.c_double * 4)()
+        self._check(self.lib.rcc_kernel_times(self.h, out))
+        return {"ladder": out[0], "ring_hintm": out[1], "ring_t2": out[2], "vovv_tau": out[3]}
+
+    def synchronize(self):
+        self._check(self.lib.rcc_synchronize(self.h))
+
+    # ------------------------------------------------------------ diagnostics
+    def zgemm_device(self, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        al = (ctypes.c_double * 2)(alpha.real, alpha.imag)
+        be = (ctypes.c_double * 2)(beta.real, beta.imag)
+        self._check(self.lib.rcc_zgemm_device(
+            self.h, opa, opb, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
+            ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc))
+
+    def probe_mfma(self):
+        out = np.empty(256, np.float64)
+        self._check(self.lib.rcc_probe_mfma(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out.reshape(64, 4)
