@@ -24,6 +24,22 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = os.environ.get("FC_RCC", "flang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The ISO_C_BINDING module and the CCENER/CCEQN-sequence driver."""
+    exe = os.path.join(FORTRAN, "ccdrv_hip.x")
+    cmd = [FC, "-O2", "-J", FORTRAN, os.path.join(FORTRAN, "relccsd_hip.f90"),
+           os.path.join(FORTRAN, "ccdrv_hip.f90"), "-L" + HERE, "-lrelccsd_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     objs = []
     for s in SOURCES:
@@ -47,3 +63,4 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -1,0 +1,40 @@
+"""The Fortran driver (ISO_C_BINDING over the C ABI, molfdir_amd/fortran)
+reproduces the relccsd reference trajectory on the GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import refrun
+
+from _golden import load, system_for
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "molfdir_amd", "fortran", "ccdrv_hip.x")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+@pytest.mark.parametrize("name", ["cfg1_6o40v", "conv_5o16v"])
+def test_fortran_driver_matches_reference(name, tmp_path):
+    g = load("ccsd", name)
+    no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    s = system_for(g)
+    refrun._write_input(str(tmp_path / "ccin.bin"), s, 1, maxit, maxdim, ntol)
+    p = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    rows = refrun.parse_iterations(p.stdout)
+    ref = g["iterations"]
+    assert len(rows) == len(ref)
+    for a, b in zip(rows[1:], ref[1:]):
+        assert int(a[0]) == int(b[0])
+        assert abs(a[1] - b[1]) < 1e-10
+        assert abs(a[2] - b[2]) < 1e-9 * max(1.0, b[2])
+    raw = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
+    z = raw[3:].view(np.complex128)
+    nt1 = no * nv
+    assert abs(raw[0] - float(g["emp2"])) < 1e-10
+    assert np.abs(z[:nt1] - g["T1"]).max() < 1e-9
+    assert np.abs(z[nt1:] - g["T2"]).max() < 1e-9
