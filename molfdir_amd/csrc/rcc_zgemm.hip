@@ -212,14 +212,15 @@ zgemm_mfma_kernel(int m, int n, int k,
         }
     }
 
-    // D layout of v_mfma_f64_16x16x4_f64: lane l, element r -> row 4*(l/16)+r, col l%16
+    // D layout of v_mfma_f64_16x16x4_f64 on gfx950 (pinned by the lane-layout
+    // probe test): lane l, element r -> row (l/16) + 4 r, column l%16
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * 32 + i * 16 + 4 * lk + r;
+                const int gm = m0 + wm * 32 + i * 16 + lk + 4 * r;
                 const int gn = n0 + wn * 32 + j * 16 + lr;
                 if (gm < m && gn < n) {
                     const double xr = acc_re[i][j][r], xi = acc_im[i][j][r];

