@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--scale", type=float, default=0.05,
                     help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times) and exit: the "
+                         "reference point that scales rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
@@ -86,6 +89,17 @@ def main():
     import torch
     from molfdir_amd import synth
     from molfdir_amd.relccsd import RelCCSD
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "bytes_written": n * 8,
+                          "launches": 3}), flush=True)
+        return
 
     no, nv, seed, desc = CONFIGS[args.config]
     dev = torch.device("cuda:0")
