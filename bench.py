@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--scale", type=float, default=0.05,
                     help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--complex-product", type=int, default=3, choices=(3, 4),
+                    help="real MFMA products per complex product in the FP64 GEMMs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the "
                          "reference point that scales rocprof FETCH_SIZE / WRITE_SIZE")
@@ -111,6 +113,7 @@ def main():
     print(f"[bench] generated {desc} in {t_gen:.1f} s", file=sys.stderr, flush=True)
 
     r = RelCCSD(no, nv, maxdim=6, maxit=args.steps + args.warmup, ntol=12)
+    r.set_complex_product(args.complex_product == 3)
     r.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
     for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
         r.set_ints(name, sysd[name])
@@ -160,6 +163,7 @@ def main():
                  f"Hermitian factor, Re/Im N(0,{args.scale}^2), seed {seed})"),
         "config": {"workload": f"CCSD iteration, {desc}, FP64", "no": no, "nv": nv,
                    "precision": "fp64", "diis_maxdim": 6,
+                   "complex_product": f"{args.complex_product} real MFMA products",
                    "algorithmic_flops_per_iter": F},
         "roofline": {"bound": "mfma", "kernel": "VVVV ladder ZGEMM (BINTM)",
                      "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",

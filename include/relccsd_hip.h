@@ -65,6 +65,12 @@ const char* rcc_last_error(const rcc_t* h);
  * (ccdriv.F:208-225).  precision: 0 = FP64 (the only mode of this build). */
 int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision);
 
+/* How the FP64 complex GEMMs form a complex product: three_mult = 1 (the
+ * default) uses Gauss's three real products per complex product (3/4 of the
+ * matrix-core work), 0 the classical four.  No counterpart in the reference
+ * (its ZGEMM, general/blas.F:13489, is the classical form). */
+int rcc_set_complex_product(rcc_t* h, int three_mult);
+
 /* Fock blocks and orbital energies as FMTOFILE reads them (ccgetv.F:34-87);
  * eps: occupied then virtual (SYMTAB reordering ccdriv.F:1393-1420). */
 int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,

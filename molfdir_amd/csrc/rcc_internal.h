@@ -18,6 +18,8 @@ struct GemmArgs {
     int batch = 1;
     // distinct kernel symbol for profiling: 0 generic, 1 VVVV ladder, 2 (ov)^3 rings
     int tag = 0;
+    // 1: Gauss three-multiplication complex product (3/4 of the MFMA work)
+    int gauss = 0;
 };
 
 // ws may be null (no K-split); ws_elems = capacity in complex elements.

@@ -50,6 +50,7 @@ struct rcc_handle {
     int no = 0, nv = 0;
     int64_t noot = 0, nvvt = 0, nvo = 0, nt1 = 0, nt2 = 0;
     int maxdim = 8, maxit = 30, ntol = 12, precision = 0;
+    int gemm_gauss = 1;    // complex products: 1 = three real MFMA products, 0 = four
     hipStream_t st = nullptr;
     std::string err;
     std::vector<void*> allocs;
@@ -141,6 +142,7 @@ int gemm(rcc_t* h, int opa, int opb, int m, int n, int k, double2 alpha, const d
          int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0, int tag = 0) {
     GemmArgs g;
     g.tag = tag;
+    g.gauss = h->gemm_gauss;
     g.opA = opa; g.opB = opb; g.m = m; g.n = n; g.k = k;
     g.A = A; g.lda = lda; g.sA = sA;
     g.B = B; g.ldb = ldb; g.sB = sB;
@@ -809,6 +811,12 @@ int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision) {
     h->ntol = ntol;
     h->precision = precision;
     return alloc_diis(h);
+}
+
+int rcc_set_complex_product(rcc_t* h, int three_mult) {
+    if (!h) return -1;
+    h->gemm_gauss = three_mult ? 1 : 0;
+    return 0;
 }
 
 int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,

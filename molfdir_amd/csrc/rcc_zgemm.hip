@@ -5,17 +5,28 @@
 // op is N (as stored), T (transposed) or C (conjugate-transposed).  Operands
 // are interleaved complex (re, im) doubles, the layout relccsd's REAL*8 pairs
 // use (RCW = 2).  Tiles are staged through LDS de-interleaved into planar
-// real / imaginary arrays; each complex product is four real
-// v_mfma_f64_16x16x4_f64 products (re*re - im*im, re*im + im*re) accumulating
-// in FP64.  The imaginary part of B is kept negated in a register for the
-// re*re - im*im term, so no accumulator is ever subtracted.
+// real / imaginary arrays.  Each complex product is formed on
+// v_mfma_f64_16x16x4_f64 in FP64 either
+//   - with four real products: re = Ar Br - Ai Bi, im = Ar Bi + Ai Br (the
+//     imaginary part of B is negated in a register, no accumulator is ever
+//     subtracted), or
+//   - with Gauss's three-multiplication form: P1 = Ar Br, P2 = Ai Bi,
+//     P3 = (Ar + Ai)(Br + Bi), re = P1 - P2, im = P3 - P1 - P2, which does 3/4
+//     of the matrix-core work (the "3M" product of ZGEMM3M libraries; its
+//     rounding error bound is normwise, see Higham, "Stability of a method for
+//     multiplying complex matrices with three real matrix multiplications",
+//     SIAM J. Matrix Anal. Appl. 13 (1992) 681).
 //
 // Work decomposition: a 256-thread workgroup (4 wave64) owns a 64 x 64
 // complex C tile, each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks of 16 x 16).
 // K is walked in steps of 16 with the next global tile prefetched into
-// registers while the current LDS tile feeds the MFMAs.  grid.z enumerates
-// (batch, K-split).  K-splits write raw partial sums to a workspace that a
-// second kernel reduces in a fixed order, so results are deterministic.
+// registers while the current LDS tile feeds the MFMAs.  Tiles are numbered
+// so that each of the 8 XCDs (workgroups are dealt to XCDs round-robin) owns a
+// contiguous range of tiles ordered N-fastest: the workgroups sharing one
+// M-panel of A run together on one XCD and read it through that XCD's L2
+// once.  grid.z enumerates (batch, K-split).  K-splits write raw partial sums
+// to a workspace that a second kernel reduces in a fixed order, so results
+// are deterministic.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <algorithm>
@@ -132,7 +143,9 @@ __device__ __forceinline__ void store_b(double* __restrict__ sre, double* __rest
 // One workgroup: 64 x 64 tile of C for batch index / K-split given by blockIdx.z.
 // split == 1: C = alpha * acc + beta * C.  split > 1: raw acc goes to
 // W[(z) * m * n + i + j * m] and zgemm_reduce finishes the epilogue.
-template <int OPA, int OPB, int TAG>
+constexpr int NXCD = 8;
+
+template <int OPA, int OPB, int TAG, int GAUSS>
 __global__ void __launch_bounds__(256)
 zgemm_mfma_kernel(int m, int n, int k,
                   const double2* __restrict__ A, int64_t lda, int64_t sA,
@@ -149,8 +162,16 @@ zgemm_mfma_kernel(int m, int n, int k,
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const int m0 = blockIdx.x * BM;
-    const int n0 = blockIdx.y * BN;
+    // XCD-aware tile order (see header): block b runs on XCD b % 8
+    const int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
+    const int T = mt * nt;
+    const int b = blockIdx.x;
+    const int xcd = b % NXCD, idx = b / NXCD;
+    const int q = T / NXCD, rr = T % NXCD;
+    const int start = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
+    const int tile = start + idx;
+    const int m0 = (tile / nt) * BM;
+    const int n0 = (tile % nt) * BN;
     const int z = blockIdx.z;
     const int batch = z / nsplit;
     const int split = z - batch * nsplit;
@@ -159,13 +180,15 @@ zgemm_mfma_kernel(int m, int n, int k,
     const int kbeg = split * kchunk;
     const int kend = min(k, kbeg + kchunk);
 
-    v4d acc_re[2][2], acc_im[2][2];
+    // 4M: acc0 = re, acc1 = im.  3M: acc0 = P1, acc1 = P3, acc2 = P2.
+    v4d acc0[2][2], acc1[2][2], acc2[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            acc_re[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc_im[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc0[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc1[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc2[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
         }
 
     double2 ra[4], rb[4];
@@ -200,15 +223,31 @@ zgemm_mfma_kernel(int m, int n, int k,
                 bim[j] = Bs_im[(kk + lk) * LDB_S + c];
                 bnim[j] = -bim[j];
             }
+            if (GAUSS) {
+                double asum[2], bsum[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i) asum[i] = are[i] + aim[i];
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc_re[i][j], 0, 0, 0);
-                    acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bnim[j], acc_re[i][j], 0, 0, 0);
-                    acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc_im[i][j], 0, 0, 0);
-                    acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc_im[i][j], 0, 0, 0);
-                }
+                for (int j = 0; j < 2; ++j) bsum[j] = bre[j] + bim[j];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
+                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(asum[i], bsum[j], acc1[i][j], 0, 0, 0);
+                        acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], acc2[i][j], 0, 0, 0);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
+                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc1[i][j], 0, 0, 0);
+                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bnim[j], acc0[i][j], 0, 0, 0);
+                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc1[i][j], 0, 0, 0);
+                    }
+            }
         }
     }
 
@@ -223,7 +262,14 @@ zgemm_mfma_kernel(int m, int n, int k,
                 const int gm = m0 + wm * 32 + i * 16 + lk + 4 * r;
                 const int gn = n0 + wn * 32 + j * 16 + lr;
                 if (gm < m && gn < n) {
-                    const double xr = acc_re[i][j][r], xi = acc_im[i][j][r];
+                    double xr, xi;
+                    if (GAUSS) {
+                        xr = acc0[i][j][r] - acc2[i][j][r];
+                        xi = acc1[i][j][r] - acc0[i][j][r] - acc2[i][j][r];
+                    } else {
+                        xr = acc0[i][j][r];
+                        xi = acc1[i][j][r];
+                    }
                     if (nsplit > 1) {
                         W[(int64_t)z * m * n + gm + (int64_t)gn * m] = make_double2(xr, xi);
                     } else {
@@ -292,10 +338,16 @@ __global__ void zscale_kernel(int m, int n, int batch, double2* __restrict__ C, 
 
 template <int OPA, int OPB, int TAG>
 static void launch_t(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
-    dim3 grid((g.m + BM - 1) / BM, (g.n + BN - 1) / BN, g.batch * nsplit);
-    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG>), grid, dim3(256), 0, st,
-                       g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
-                       g.alpha, g.beta, nsplit, kchunk, W);
+    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
+    dim3 grid((unsigned)tiles, 1, g.batch * nsplit);
+    if (g.gauss)
+        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 1>), grid, dim3(256), 0, st,
+                           g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
+                           g.alpha, g.beta, nsplit, kchunk, W);
+    else
+        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 0>), grid, dim3(256), 0, st,
+                           g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
+                           g.alpha, g.beta, nsplit, kchunk, W);
 }
 
 template <int OPA, int OPB>

@@ -62,6 +62,7 @@ def load():
     lib.rcc_last_error.restype = ctypes.c_char_p
     lib.rcc_set_options.argtypes = [vp, ci, ci, ci, ci]
     lib.rcc_set_fock.argtypes = [vp, vp, vp, vp, vp]
+    lib.rcc_set_complex_product.argtypes = [vp, ci]
     lib.rcc_set_ints.argtypes = [vp, ci, ci, i64, i64, vp]
     lib.rcc_set_ints_device.argtypes = [vp, ci, ci, i64, i64, vp]
     lib.rcc_set_amplitudes.argtypes = [vp, vp, vp]
@@ -139,6 +140,10 @@ class RelCCSD:
     def set_options(self, maxdim=8, maxit=30, ntol=12, precision=0):
         self.maxdim, self.maxit, self.ntol = maxdim, maxit, ntol
         self._check(self.lib.rcc_set_options(self.h, maxdim, maxit, ntol, precision))
+
+    def set_complex_product(self, three_mult: bool):
+        """Complex GEMM products: Gauss 3 real MFMA products (True, default) or 4."""
+        self._check(self.lib.rcc_set_complex_product(self.h, 1 if three_mult else 0))
 
     # ------------------------------------------------------------ inputs
     def set_fock(self, foo, fvo, fvv, eps):

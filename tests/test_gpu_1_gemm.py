@@ -23,10 +23,12 @@ def test_mfma_f64_lane_layout():
     assert np.array_equal(got, ref), d
 
 
+@pytest.mark.parametrize("three_mult", [True, False])
 @pytest.mark.parametrize("opa", [0, 1, 2])
 @pytest.mark.parametrize("opb", [0, 1, 2])
-@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (37, 91, 53), (5, 3, 3000), (130, 70, 1)])
-def test_zgemm_against_numpy(opa, opb, m, n, k):
+@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (37, 91, 53), (5, 3, 3000), (130, 70, 1),
+                                   (257, 129, 77)])
+def test_zgemm_against_numpy(opa, opb, m, n, k, three_mult):
     rng = np.random.default_rng(m * 7 + n * 3 + k + 10 * opa + opb)
 
     def mat(r, c):
@@ -46,6 +48,7 @@ def test_zgemm_against_numpy(opa, opb, m, n, k):
 
     tA, tB, tC = dev(A), dev(B), dev(C)
     r = RelCCSD(2, 4)
+    r.set_complex_product(three_mult)
     r.zgemm_device(opa, opb, m, n, k, alpha, tA, A.shape[0], tB, B.shape[0], beta, tC, m)
     got = tC.cpu().numpy().reshape(n, m).T
     assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5

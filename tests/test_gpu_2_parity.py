@@ -58,12 +58,14 @@ def test_terms_match_reference(name):
     assert abs(r.energy() - float(g["ECC"])) < 1e-12
 
 
+@pytest.mark.parametrize("three_mult", [True, False])
 @pytest.mark.parametrize("name", names("ccsd"))
-def test_trajectory_matches_reference(name):
+def test_trajectory_matches_reference(name, three_mult):
     g = load("ccsd", name)
     maxit, maxdim, ntol = (int(x) for x in g["params"][3:6])
     s = system_for(g)
     r = solver_for(s, maxdim=maxdim, maxit=maxit, ntol=ntol)
+    r.set_complex_product(three_mult)
     out = r.cceqn()
     ref_rows = g["iterations"][1:]
     assert abs(out["emp2"] - float(g["emp2"])) < 1e-10
