@@ -19,8 +19,9 @@
 //
 // Work decomposition: a 256-thread workgroup (4 wave64) owns a 64 x 64
 // complex C tile, each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks of 16 x 16).
-// K is walked in steps of 16 with the next global tile prefetched into
-// registers while the current LDS tile feeds the MFMAs.  Tiles are numbered
+// K is walked in steps of 16 through two LDS stages: the next global tile is
+// loaded into registers and written to the idle stage while the current stage
+// feeds the MFMAs, so one barrier per K-step suffices.  Tiles are numbered
 // so that each of the 8 XCDs (workgroups are dealt to XCDs round-robin) owns a
 // contiguous range of tiles ordered N-fastest: the workgroups sharing one
 // M-panel of A run together on one XCD and read it through that XCD's L2
@@ -153,10 +154,11 @@ zgemm_mfma_kernel(int m, int n, int k,
                   double2* __restrict__ C, int64_t ldc, int64_t sC,
                   double2 alpha, double2 beta, int nsplit, int kchunk,
                   double2* __restrict__ W) {
-    __shared__ double As_re[BK * LDA_S];
-    __shared__ double As_im[BK * LDA_S];
-    __shared__ double Bs_re[BK * LDB_S];
-    __shared__ double Bs_im[BK * LDB_S];
+    // two LDS stages: the next K-tile is written while the current one feeds the MFMAs
+    __shared__ double As_re[2][BK * LDA_S];
+    __shared__ double As_im[2][BK * LDA_S];
+    __shared__ double Bs_re[2][BK * LDB_S];
+    __shared__ double Bs_im[2][BK * LDB_S];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -192,18 +194,19 @@ zgemm_mfma_kernel(int m, int n, int k,
         }
 
     double2 ra[4], rb[4];
+    const int lr = lane & 15;        // row (A) / column (B) inside a 16-block
+    const int lk = lane >> 4;        // k inside a 4-step
     if (kbeg < kend) {
         load_a<OPA>(A, lda, m, kend, m0, kbeg, tid, ra);
         load_b<OPB>(B, ldb, n, kend, n0, kbeg, tid, rb);
+        store_a<OPA>(As_re[0], As_im[0], tid, ra);
+        store_b<OPB>(Bs_re[0], Bs_im[0], tid, rb);
     }
-    const int lr = lane & 15;        // row (A) / column (B) inside a 16-block
-    const int lk = lane >> 4;        // k inside a 4-step
+    __syncthreads();
+    int cur = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        __syncthreads();
-        store_a<OPA>(As_re, As_im, tid, ra);
-        store_b<OPB>(Bs_re, Bs_im, tid, rb);
-        __syncthreads();
-        if (k0 + BK < kend) {
+        const bool more = k0 + BK < kend;
+        if (more) {
             load_a<OPA>(A, lda, m, kend, m0, k0 + BK, tid, ra);
             load_b<OPB>(B, ldb, n, kend, n0, k0 + BK, tid, rb);
         }
@@ -213,14 +216,14 @@ zgemm_mfma_kernel(int m, int n, int k,
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int r = wm * 32 + i * 16 + lr;
-                are[i] = As_re[(kk + lk) * LDA_S + r];
-                aim[i] = As_im[(kk + lk) * LDA_S + r];
+                are[i] = As_re[cur][(kk + lk) * LDA_S + r];
+                aim[i] = As_im[cur][(kk + lk) * LDA_S + r];
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int c = wn * 32 + j * 16 + lr;
-                bre[j] = Bs_re[(kk + lk) * LDB_S + c];
-                bim[j] = Bs_im[(kk + lk) * LDB_S + c];
+                bre[j] = Bs_re[cur][(kk + lk) * LDB_S + c];
+                bim[j] = Bs_im[cur][(kk + lk) * LDB_S + c];
                 bnim[j] = -bim[j];
             }
             if (GAUSS) {
@@ -249,6 +252,12 @@ zgemm_mfma_kernel(int m, int n, int k,
                     }
             }
         }
+        if (more) {
+            store_a<OPA>(As_re[cur ^ 1], As_im[cur ^ 1], tid, ra);
+            store_b<OPB>(Bs_re[cur ^ 1], Bs_im[cur ^ 1], tid, rb);
+        }
+        __syncthreads();
+        cur ^= 1;
     }
 
     // D layout of v_mfma_f64_16x16x4_f64 on gfx950 (pinned by the lane-layout
