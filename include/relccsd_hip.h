@@ -133,6 +133,9 @@ int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const doub
                      const void* A, int64_t lda, const void* B, int64_t ldb,
                      const double* beta, void* C, int64_t ldc);
 int rcc_probe_mfma(rcc_t* h, double* out256);
+/* Achievable FP64 matrix-core rate: nblocks x 4 waves, each running iters
+ * rounds of `chains` independent v_mfma_f64_16x16x4_f64 (1, 2, 4, 8, 12). */
+int rcc_probe_mfma_rate(rcc_t* h, int chains, int iters, int nblocks, double* tflops);
 int rcc_synchronize(rcc_t* h);
 
 #ifdef __cplusplus

@@ -25,6 +25,7 @@ struct GemmArgs {
 // ws may be null (no K-split); ws_elems = capacity in complex elements.
 int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
 int mfma_probe(double* dout, hipStream_t st);
+int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 
 // ---- element-wise / permutation kernels (rcc_kernels.hip) ----
 // Packed pair tables: pair p of an n-set -> (x, y), x > y, y outer.

@@ -1030,6 +1030,26 @@ int rcc_probe_mfma(rcc_t* h, double* out256) {
     return 0;
 }
 
+int rcc_probe_mfma_rate(rcc_t* h, int chains, int iters, int nblocks, double* tflops) {
+    if (!h || !tflops) return -1;
+    double* d = nullptr;
+    HIPCHK(h, hipMalloc((void**)&d, 256 * 8));
+    HIPCHK(h, hipEventRecord(h->ev0, h->st));
+    if (mfma_rate(chains, iters, nblocks, d, h->st) != 0) {
+        h->err = "rcc_probe_mfma_rate: chains must be 1, 2, 4, 8 or 12";
+        (void)hipFree(d);
+        return -1;
+    }
+    HIPCHK(h, hipEventRecord(h->ev1, h->st));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    HIPCHK(h, hipFree(d));
+    // 4 waves per block, each MFMA 16*16*4 FMAs = 2048 flops
+    *tflops = 2048.0 * 4.0 * chains * (double)iters * nblocks / (ms * 1e-3) / 1e12;
+    return 0;
+}
+
 int rcc_synchronize(rcc_t* h) {
     if (!h) return -1;
     HIPCHK(h, hipStreamSynchronize(h->st));

@@ -44,34 +44,96 @@ constexpr int PAD = 16;          // LDS row padding (doubles), breaks bank alias
 constexpr int LDA_S = BM + PAD;
 constexpr int LDB_S = BN + PAD;
 
+// Per-thread global -> register loader for one operand tile.  Each thread
+// owns 4 elements of the BM x BK (A) or BK x BN (B) tile; their addresses are
+// formed once and advanced by a constant stride per K-step.  Rows / columns
+// past the matrix edge are clamped to the last valid one: they only feed
+// output elements that are never stored.  Only the last K-step of a split
+// checks k (elements past kend are zero and are not read).
+struct Loader {
+    const double2* p[4];
+    int kofs[4];          // k offset of element j inside a K-step
+    int64_t step;         // pointer advance per K-step (elements)
+};
+
 template <int OP>
-__device__ __forceinline__ double2 ld_op(const double2* __restrict__ p, int64_t idx, bool ok) {
-    double2 v = ok ? p[idx] : make_double2(0.0, 0.0);
+__device__ __forceinline__ double2 cj(double2 v) {
     if (OP == OP_C) v.y = -v.y;
     return v;
 }
 
-// Loads the BM x BK tile of op(A) starting at (m0, k0) into registers.
+// A tile (BM x BK of op(A)); A stored m x k (OP_N) or k x m (OP_T / OP_C).
 template <int OPA>
-__device__ __forceinline__ void load_a(const double2* __restrict__ A, int64_t lda, int m, int k,
-                                       int m0, int k0, int tid, double2 (&ra)[4]) {
+__device__ __forceinline__ Loader make_loader_a(const double2* __restrict__ A, int64_t lda, int m,
+                                                int m0, int kbeg, int tid) {
+    Loader L;
     if (OPA == OP_N) {
-        const int row = tid & 63;
-        const int kc = tid >> 6;                 // 0..3
+        const int row = tid & 63, kc = tid >> 6;
+        const int64_t gm = min(m0 + row, m - 1);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int gm = m0 + row, gk = k0 + kc + 4 * j;
-            ra[j] = ld_op<OPA>(A, (int64_t)gm + (int64_t)gk * lda, gm < m && gk < k);
+            L.kofs[j] = kc + 4 * j;
+            L.p[j] = A + gm + (int64_t)(kbeg + kc + 4 * j) * lda;
         }
+        L.step = (int64_t)BK * lda;
     } else {
-        const int kk = tid & 15;
-        const int row = tid >> 4;                // 0..15
+        const int kk = tid & 15, row = tid >> 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int gm = m0 + row + 16 * j, gk = k0 + kk;
-            ra[j] = ld_op<OPA>(A, (int64_t)gk + (int64_t)gm * lda, gm < m && gk < k);
+            const int64_t gm = min(m0 + row + 16 * j, m - 1);
+            L.kofs[j] = kk;
+            L.p[j] = A + (kbeg + kk) + gm * lda;
         }
+        L.step = BK;
     }
+    return L;
+}
+
+// B tile (BK x BN of op(B)); B stored k x n (OP_N) or n x k (OP_T / OP_C).
+template <int OPB>
+__device__ __forceinline__ Loader make_loader_b(const double2* __restrict__ B, int64_t ldb, int n,
+                                                int n0, int kbeg, int tid) {
+    Loader L;
+    if (OPB == OP_N) {
+        const int kk = tid & 15, col = tid >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t gn = min(n0 + col + 16 * j, n - 1);
+            L.kofs[j] = kk;
+            L.p[j] = B + (kbeg + kk) + gn * ldb;
+        }
+        L.step = BK;
+    } else {
+        const int col = tid & 63, kc = tid >> 6;
+        const int64_t gn = min(n0 + col, n - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            L.kofs[j] = kc + 4 * j;
+            L.p[j] = B + gn + (int64_t)(kbeg + kc + 4 * j) * ldb;
+        }
+        L.step = (int64_t)BK * ldb;
+    }
+    return L;
+}
+
+// Loads the K-step starting at k0 (relative offset kr = k0 - kbeg is implied
+// by how far the pointers were advanced); full = every k of the step < kend.
+template <int OP>
+__device__ __forceinline__ void load_step(const Loader& L, int k0, int kend, bool full,
+                                          double2 (&r)[4]) {
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[j] = cj<OP>(L.p[j][0]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            r[j] = (k0 + L.kofs[j] < kend) ? cj<OP>(L.p[j][0]) : make_double2(0.0, 0.0);
+    }
+}
+
+__device__ __forceinline__ void advance(Loader& L) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) L.p[j] += L.step;
 }
 
 template <int OPA>
@@ -92,29 +154,6 @@ __device__ __forceinline__ void store_a(double* __restrict__ sre, double* __rest
         for (int j = 0; j < 4; ++j) {
             sre[kk * LDA_S + row + 16 * j] = ra[j].x;
             sim[kk * LDA_S + row + 16 * j] = ra[j].y;
-        }
-    }
-}
-
-// Loads the BK x BN tile of op(B) starting at (k0, n0).
-template <int OPB>
-__device__ __forceinline__ void load_b(const double2* __restrict__ B, int64_t ldb, int n, int k,
-                                       int n0, int k0, int tid, double2 (&rb)[4]) {
-    if (OPB == OP_N) {
-        const int kk = tid & 15;
-        const int col = tid >> 4;                // 0..15
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gn = n0 + col + 16 * j, gk = k0 + kk;
-            rb[j] = ld_op<OPB>(B, (int64_t)gk + (int64_t)gn * ldb, gn < n && gk < k);
-        }
-    } else {
-        const int col = tid & 63;
-        const int kc = tid >> 6;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gn = n0 + col, gk = k0 + kc + 4 * j;
-            rb[j] = ld_op<OPB>(B, (int64_t)gn + (int64_t)gk * ldb, gn < n && gk < k);
         }
     }
 }
@@ -196,9 +235,12 @@ zgemm_mfma_kernel(int m, int n, int k,
     double2 ra[4], rb[4];
     const int lr = lane & 15;        // row (A) / column (B) inside a 16-block
     const int lk = lane >> 4;        // k inside a 4-step
+    Loader LA = make_loader_a<OPA>(A, lda, m, m0, kbeg, tid);
+    Loader LB = make_loader_b<OPB>(B, ldb, n, n0, kbeg, tid);
     if (kbeg < kend) {
-        load_a<OPA>(A, lda, m, kend, m0, kbeg, tid, ra);
-        load_b<OPB>(B, ldb, n, kend, n0, kbeg, tid, rb);
+        const bool full = kbeg + BK <= kend;
+        load_step<OPA>(LA, kbeg, kend, full, ra);
+        load_step<OPB>(LB, kbeg, kend, full, rb);
         store_a<OPA>(As_re[0], As_im[0], tid, ra);
         store_b<OPB>(Bs_re[0], Bs_im[0], tid, rb);
     }
@@ -207,8 +249,11 @@ zgemm_mfma_kernel(int m, int n, int k,
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         const bool more = k0 + BK < kend;
         if (more) {
-            load_a<OPA>(A, lda, m, kend, m0, k0 + BK, tid, ra);
-            load_b<OPB>(B, ldb, n, kend, n0, k0 + BK, tid, rb);
+            advance(LA);
+            advance(LB);
+            const bool full = k0 + 2 * BK <= kend;
+            load_step<OPA>(LA, k0 + BK, kend, full, ra);
+            load_step<OPB>(LB, k0 + BK, kend, full, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
@@ -427,6 +472,37 @@ __global__ void mfma_probe_kernel(double* out) {
     v4d c = (v4d){0.0, 0.0, 0.0, 0.0};
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     for (int r = 0; r < 4; ++r) out[l * 4 + r] = c[r];
+}
+
+// Throughput probe: every wave runs `iters` rounds of CH independent
+// v_mfma_f64_16x16x4_f64 chains on register data (no memory traffic in the
+// loop); the host times it to obtain the achievable FP64 matrix-core rate.
+template <int CH>
+__global__ void __launch_bounds__(256) mfma_rate_kernel(int iters, double seed, double* out) {
+    v4d acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double a = seed * (threadIdx.x + 1), b = seed * 0.5;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) t += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+    if (t == 12345.678) out[threadIdx.x] = t;   // keeps the chains live, never true for the probe
+}
+
+int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st) {
+    switch (chains) {
+        case 1: hipLaunchKernelGGL(mfma_rate_kernel<1>, dim3(nblocks), dim3(256), 0, st, iters, 1e-30, out); break;
+        case 2: hipLaunchKernelGGL(mfma_rate_kernel<2>, dim3(nblocks), dim3(256), 0, st, iters, 1e-30, out); break;
+        case 4: hipLaunchKernelGGL(mfma_rate_kernel<4>, dim3(nblocks), dim3(256), 0, st, iters, 1e-30, out); break;
+        case 8: hipLaunchKernelGGL(mfma_rate_kernel<8>, dim3(nblocks), dim3(256), 0, st, iters, 1e-30, out); break;
+        case 12: hipLaunchKernelGGL(mfma_rate_kernel<12>, dim3(nblocks), dim3(256), 0, st, iters, 1e-30, out); break;
+        default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int mfma_probe(double* dout, hipStream_t st) {

@@ -80,6 +80,7 @@ def load():
     lib.rcc_kernel_times.argtypes = [vp, dp]
     lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_probe_mfma.argtypes = [vp, dp]
+    lib.rcc_probe_mfma_rate.argtypes = [vp, ci, ci, ci, dp]
     lib.rcc_synchronize.argtypes = [vp]
     _lib = lib
     return lib
@@ -264,6 +265,12 @@ class RelCCSD:
         self._check(self.lib.rcc_zgemm_device(
             self.h, opa, opb, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
             ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc))
+
+    def mfma_rate(self, chains=8, iters=20000, nblocks=2048):
+        """Measured FP64 MFMA throughput (TFLOP/s) of a register-only loop."""
+        t = ctypes.c_double()
+        self._check(self.lib.rcc_probe_mfma_rate(self.h, chains, iters, nblocks, ctypes.byref(t)))
+        return t.value
 
     def probe_mfma(self):
         out = np.empty(256, np.float64)
