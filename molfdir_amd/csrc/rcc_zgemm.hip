@@ -44,6 +44,17 @@ constexpr int PAD = 16;          // LDS row padding (doubles), breaks bank alias
 constexpr int LDA_S = BM + PAD;
 constexpr int LDB_S = BN + PAD;
 
+// LDS column swizzle.  An operand whose global layout is k-fastest (A with op
+// T/C, B with op N) is written to its [k][row] LDS tile by lanes that differ
+// in k, which would put 16 lanes on the same banks; XOR-ing the in-row index
+// with the k-row (bits 0-3 only) spreads them.  Readers apply the same map;
+// the 16-element MFMA blocks stay contiguous because bit 4 and above are
+// untouched.
+template <bool SW>
+__device__ __forceinline__ int swz(int krow, int col) {
+    return SW ? (col ^ (krow & 15)) : col;
+}
+
 // Per-thread global -> register loader for one operand tile.  Each thread
 // owns 4 elements of the BM x BK (A) or BK x BN (B) tile; their addresses are
 // formed once and advanced by a constant stride per K-step.  Rows / columns
@@ -152,8 +163,8 @@ __device__ __forceinline__ void store_a(double* __restrict__ sre, double* __rest
         const int row = tid >> 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            sre[kk * LDA_S + row + 16 * j] = ra[j].x;
-            sim[kk * LDA_S + row + 16 * j] = ra[j].y;
+            sre[kk * LDA_S + swz<true>(kk, row + 16 * j)] = ra[j].x;
+            sim[kk * LDA_S + swz<true>(kk, row + 16 * j)] = ra[j].y;
         }
     }
 }
@@ -166,8 +177,8 @@ __device__ __forceinline__ void store_b(double* __restrict__ sre, double* __rest
         const int col = tid >> 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            sre[kk * LDB_S + col + 16 * j] = rb[j].x;
-            sim[kk * LDB_S + col + 16 * j] = rb[j].y;
+            sre[kk * LDB_S + swz<true>(kk, col + 16 * j)] = rb[j].x;
+            sim[kk * LDB_S + swz<true>(kk, col + 16 * j)] = rb[j].y;
         }
     } else {
         const int col = tid & 63;
@@ -258,17 +269,18 @@ zgemm_mfma_kernel(int m, int n, int k,
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double are[2], aim[2], bre[2], bim[2], bnim[2];
+            const int kr = kk + lk;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const int r = wm * 32 + i * 16 + lr;
-                are[i] = As_re[cur][(kk + lk) * LDA_S + r];
-                aim[i] = As_im[cur][(kk + lk) * LDA_S + r];
+                const int r = swz<OPA != OP_N>(kr, wm * 32 + i * 16 + lr);
+                are[i] = As_re[cur][kr * LDA_S + r];
+                aim[i] = As_im[cur][kr * LDA_S + r];
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const int c = wn * 32 + j * 16 + lr;
-                bre[j] = Bs_re[cur][(kk + lk) * LDB_S + c];
-                bim[j] = Bs_im[cur][(kk + lk) * LDB_S + c];
+                const int c = swz<OPB == OP_N>(kr, wn * 32 + j * 16 + lr);
+                bre[j] = Bs_re[cur][kr * LDB_S + c];
+                bim[j] = Bs_im[cur][kr * LDB_S + c];
                 bnim[j] = -bim[j];
             }
             if (GAUSS) {
