@@ -28,6 +28,8 @@ CONFIGS = {
     "cfg3": (30, 300, 3, "C1 30o/300v complex128"),
 }
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
+PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
+PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
 CPU_SAMPLE = (10, 100)    # relccsd reference sample size (o, v)
 
 
@@ -79,6 +81,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--complex-product", type=int, default=3, choices=(3, 4),
                     help="real MFMA products per complex product in the FP64 GEMMs")
+    ap.add_argument("--split", default="all", choices=("none", "bf16", "i8x4", "i8x5", "all"),
+                    help="after the FP64 run, rerun the same iterations from the same MP2 "
+                         "guess with the ladder, VOVV.tau and rings as split products "
+                         "(bf16: 3 BF16 slices; i8xS: S INT8 digits) and report s/iter and "
+                         "the per-iteration |dEc| against FP64")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the "
                          "reference point that scales rocprof FETCH_SIZE / WRITE_SIZE")
@@ -120,25 +127,36 @@ def main():
         sysd[name] = None
     del sysd
     torch.cuda.empty_cache()
-    emp2 = r.mp2()
-    print(f"[bench] MP2 {emp2:.10f}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        nit, e, rms = r.iterate()
-        print(f"[bench] warmup it {nit} E {e:.10e} rms {rms:.3e} {r.last_iteration_ms():.1f} ms",
-              file=sys.stderr, flush=True)
-    r.synchronize()
-    torch.cuda.synchronize()
-    iter_ms, kt = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        nit, e, rms = r.iterate()
-        iter_ms.append(r.last_iteration_ms())
-        kt.append(r.kernel_times())
-        print(f"[bench] it {nit} E {e:.10e} rms {rms:.3e} {iter_ms[-1]:.1f} ms", file=sys.stderr,
-              flush=True)
-    r.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+
+    def run(label):
+        emp2 = r.mp2()
+        print(f"[bench] {label} MP2 {emp2:.10f}", file=sys.stderr, flush=True)
+        energies = []
+        for _ in range(args.warmup):
+            nit, e, rms = r.iterate()
+            energies.append(e)
+            print(f"[bench] {label} warmup it {nit} E {e:.10e} rms {rms:.3e} "
+                  f"{r.last_iteration_ms():.1f} ms", file=sys.stderr, flush=True)
+        r.synchronize()
+        torch.cuda.synchronize()
+        iter_ms, kt = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nit, e, rms = r.iterate()
+            energies.append(e)
+            iter_ms.append(r.last_iteration_ms())
+            kt.append(r.kernel_times())
+            print(f"[bench] {label} it {nit} E {e:.10e} rms {rms:.3e} {iter_ms[-1]:.1f} ms",
+                  file=sys.stderr, flush=True)
+        r.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return {"emp2": emp2, "wall": wall, "iter_ms": iter_ms, "kt": kt, "energies": energies,
+                "rms": rms}
+
+    base = run("fp64")
+    emp2, wall, iter_ms, kt = base["emp2"], base["wall"], base["iter_ms"], base["kt"]
+    e, rms = base["energies"][-1], base["rms"]
     s_iter = wall / args.steps
     F = ccsd_flops(no, nv)
     noot, nvvt, nvo = no * (no - 1) // 2, nv * (nv - 1) // 2, no * nv
@@ -175,6 +193,41 @@ def main():
     }
     out["kernels_tflops"] = {
         "ring_t2": 8.0 * nvo ** 3 / (ring_ms * 1e-3) / 1e12 if ring_ms > 0 else None}
+    modes = ["bf16", "i8x5", "i8x4"] if args.split == "all" else (
+        [] if args.split == "none" else [args.split])
+    if modes:
+        out["split"] = {}
+    for mode in modes:
+        if mode == "bf16":
+            r.set_options(6, args.steps + args.warmup, 12, 1)
+            # 4 real products x 6 slice products per complex product
+            ops, peak = 24 * 2.0 * nvvt * nvvt * noot, PEAK_BF16_MFMA
+            scheme = ("3 BF16 slices per FP64 value, 6 slice products, FP32 accumulation over "
+                      "128-term chunks, FP64 across chunks")
+            counts = "executed BF16 MFMA flops (24 real products per complex product)"
+        else:
+            d = int(mode[-1])
+            r.set_options(6, args.steps + args.warmup, 12, 2)
+            r.set_split_digits(d)
+            pairs = d * (d + 1) // 2
+            ops, peak = 3 * pairs * 2.0 * nvvt * nvvt * noot, PEAK_INT8_MFMA
+            scheme = (f"Re, Im, Re+Im as {d} signed 7-bit digits (fixed point per row and 8192-k "
+                      f"chunk), {pairs} digit pairs, exact int32 accumulation, FP64 across chunks")
+            counts = f"executed INT8 MFMA ops (3 real products x {pairs} digit pairs)"
+        sp = run("split-" + mode)
+        s_split = sp["wall"] / args.steps
+        lad = sum(k["ladder"] for k in sp["kt"]) / len(sp["kt"])
+        de = [abs(a - b) for a, b in zip(sp["energies"], base["energies"])]
+        out["split"][mode] = {
+            "scheme": scheme,
+            "s_per_iter": s_split, "speedup_vs_fp64": s_iter / s_split,
+            "tflops_algorithmic": F / s_split / 1e12,
+            "dEc_per_iteration": de, "max_dEc": max(de),
+            "kernels_ms": {k: sum(x[k] for x in sp["kt"]) / len(sp["kt"]) for k in sp["kt"][0]},
+            "ladder_roofline": {"bound": "mfma", "achieved": ops / (lad * 1e-3) / 1e12,
+                                "peak": peak, "unit": "TFLOP/s" if mode == "bf16" else "TOP/s",
+                                "frac": ops / (lad * 1e-3) / 1e12 / peak, "counts": counts},
+        }
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(F)

@@ -62,7 +62,14 @@ void rcc_destroy(rcc_t* h);
 const char* rcc_last_error(const rcc_t* h);
 
 /* MAXDIM (DIIS subspace), MAXIT and NTOL of the &CCENER namelist
- * (ccdriv.F:208-225).  precision: 0 = FP64 (the only mode of this build). */
+ * (ccdriv.F:208-225).  precision: 0 = FP64 everywhere; 1 and 2 run the VVVV
+ * ladder, the VOVV.tau product and both (ov)^3 ring contractions as split
+ * matrix-core products while denominators, residuals, DIIS and the energy stay
+ * FP64.  1 = split-BF16: three BF16 slices per value, FP32 accumulation over
+ * 128-term chunks, FP64 across chunks.  2 = split-INT8: Re, Im and Re+Im as
+ * signed 7-bit digits (rcc_set_split_digits; fixed point per row and 8192-term
+ * chunk), exact int32 accumulation, FP64 across chunks.  No counterpart in the
+ * reference (FP64 only). */
 int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision);
 
 /* How the FP64 complex GEMMs form a complex product: three_mult = 1 (the
@@ -70,6 +77,9 @@ int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision);
  * matrix-core work), 0 the classical four.  No counterpart in the reference
  * (its ZGEMM, general/blas.F:13489, is the classical form). */
 int rcc_set_complex_product(rcc_t* h, int three_mult);
+/* Digits per value of the split-INT8 mode: 4 (28 bits) or 5 (35 bits, the
+ * default).  No counterpart in the reference. */
+int rcc_set_split_digits(rcc_t* h, int digits);
 
 /* Fock blocks and orbital energies as FMTOFILE reads them (ccgetv.F:34-87);
  * eps: occupied then virtual (SYMTAB reordering ccdriv.F:1393-1420). */
@@ -132,6 +142,15 @@ int rcc_kernel_times(rcc_t* h, double* ms4);
 int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb,
                      const double* beta, void* C, int64_t ldc);
+/* The split-BF16 GEMM on device pointers: C = alpha A B + beta C with A (m x k)
+ * and B (k x n) column-major complex (diagnostic; allocates its own planes). */
+int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, const void* A,
+                           int64_t lda, const void* B, int64_t ldb, const double* beta, void* C,
+                           int64_t ldc);
+/* The split-INT8 GEMM on device pointers, same conventions (diagnostic). */
+int rcc_zgemm_i8_device(rcc_t* h, int digits, int m, int n, int k, const double* alpha,
+                        const void* A, int64_t lda, const void* B, int64_t ldb,
+                        const double* beta, void* C, int64_t ldc);
 int rcc_probe_mfma(rcc_t* h, double* out256);
 /* Achievable FP64 matrix-core rate: nblocks x 4 waves, each running iters
  * rounds of `chains` independent v_mfma_f64_16x16x4_f64 (1, 2, 4, 8, 12). */

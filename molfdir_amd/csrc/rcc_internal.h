@@ -83,4 +83,38 @@ void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, in
 void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
                hipStream_t st);
 
+// ---- split-BF16 operands and GEMM (rcc_split.hip) ----
+// Six BF16 planes (Re slices 1-3, Im slices 1-3), K-major, rows padded to 64
+// and k to 128, plus one power-of-two exponent per (64-row, 128-k) block.
+struct SplitOperand {
+    int rows = 0, k = 0, rows_pad = 0, nkc = 0;
+    int64_t k_pad = 0, plane = 0;
+    void* planes = nullptr;          // 6 * plane bf16 values
+    signed char* ex = nullptr;       // (rows_pad / 64) * nkc exponents
+};
+void split_layout(SplitOperand& S, int rows, int k);
+// X: logical rows x k complex operand; k_contiguous: element (r, kk) at
+// X[kk + r*ld], else X[r + kk*ld]; conj: use conj(X).
+int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_contiguous,
+                  bool conj, hipStream_t st);
+// C(m x n) = alpha * A . B^T + beta * C with A (m rows) and B (n rows) split
+// operands over the same k.
+int zgemm_split(const SplitOperand& A, const SplitOperand& B, double2* C, int64_t ldc,
+                double2 alpha, double2 beta, hipStream_t st);
+
+// ---- split-INT8 operands and GEMM (rcc_split_i8.hip) ----
+// Re, Im and Re+Im as `digits` signed 7-bit digits per value (fixed point per
+// row and 2048-k chunk), fragment-major; exact int32 accumulation.
+struct I8Operand {
+    int rows = 0, k = 0, digits = 5, rows_pad = 0, nks = 0, nch = 0;
+    int64_t plane_bytes = 0, mx_elems = 0;
+    void* planes = nullptr;           // plane_bytes
+    unsigned long long* mx = nullptr; // mx_elems chunk maxima (bit patterns of doubles)
+};
+void i8_layout(I8Operand& S, int rows, int k, int digits);
+int i8_split(const I8Operand& S, const double2* X, int64_t ld, bool k_contiguous, bool conj,
+             hipStream_t st);
+int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, double2 alpha,
+             double2 beta, hipStream_t st);
+
 }  // namespace rcc

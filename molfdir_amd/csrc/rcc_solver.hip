@@ -46,6 +46,12 @@ struct Buf {
 
 }  // namespace
 
+// One logical split operand, in the format of the current precision mode.
+struct SOp {
+    SplitOperand bf;
+    I8Operand i8;
+};
+
 struct rcc_handle {
     int no = 0, nv = 0;
     int64_t noot = 0, nvvt = 0, nvo = 0, nt1 = 0, nt2 = 0;
@@ -86,6 +92,11 @@ struct rcc_handle {
     std::vector<double> bb2;  // (MAXDIM+1) x (MAXDIM+2), column-major like BB2
     int nit = 0;
     bool tau_ok = false, x_ok = false, ring_ok = false;
+    // split operands (precision 1: BF16 slices, 2: INT8 digits): the integral
+    // operands are built once per set of integrals, the rest per iteration
+    SOp sVVVV, sVOVV, sVVOOR, sTAU, sP, sH, sRING;
+    bool split_const_ok = false, split_tau_ok = false, split_ring_ok = false;
+    int digits = 5;
     double last_iter_ms = 0.0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // kernel timers: 0 ladder, 1 HINTM ring, 2 T2EQNS ring, 3 BINTM X
@@ -262,22 +273,112 @@ int ensure_ring(rcc_t* h) {
 
 // BINTM's X(AK,IJ) = sum_{C>D} VOVV(AK,CD) TAU(CD,IJ)  (ccintm.F:772-788); also the
 // sum that T1EQN2 (cceqns.F:2601-2714) takes the (I,K) diagonal of.
-int ensure_x(rcc_t* h) {
-    if (!h->x_ok) {
-        CK(ensure_tau(h));
-        ktick(h, 3, 0);
-        CK(gemm(h, OP_N, OP_N, (int)h->nvo, (int)h->noot, (int)h->nvvt, cplx(1, 0),
-                h->cls[RCC_VOVV], h->nvo, h->TAU, h->nvvt, cplx(0, 0), h->X, h->nvo));
-        ktick(h, 3, 1);
-        h->x_ok = true;
-    }
-    return 0;
-}
-
 void invalidate_t(rcc_t* h) {
     h->tau_ok = false;
     h->x_ok = false;
     h->ring_ok = false;
+    h->split_tau_ok = false;
+    h->split_ring_ok = false;
+}
+
+void dfree(rcc_t* h, void* p) {
+    if (!p) return;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+void free_split_op(rcc_t* h, SOp& S) {
+    dfree(h, S.bf.planes);
+    dfree(h, S.bf.ex);
+    S.bf = SplitOperand();
+    dfree(h, S.i8.planes);
+    dfree(h, S.i8.mx);
+    S.i8 = I8Operand();
+}
+
+void free_split(rcc_t* h) {
+    for (SOp* S : {&h->sVVVV, &h->sVOVV, &h->sVVOOR, &h->sTAU, &h->sP, &h->sH, &h->sRING})
+        free_split_op(h, *S);
+    h->split_const_ok = h->split_tau_ok = h->split_ring_ok = false;
+}
+
+// Split X (rows x k; k_contiguous: element (r, kk) at X[kk + r ld], else
+// X[r + kk ld]; conj: of conj(X)) into S in the current precision's format.
+int do_split(rcc_t* h, SOp& S, int rows, int64_t k, const double2* X, int64_t ld, bool kcont,
+             bool conj) {
+    if (h->precision == 1) {
+        if (!S.bf.planes) {
+            split_layout(S.bf, rows, (int)k);
+            CK(dalloc_raw(h, (char**)&S.bf.planes, 6 * S.bf.plane * 2));
+            CK(dalloc_raw(h, &S.bf.ex, (int64_t)(S.bf.rows_pad / 64) * S.bf.nkc));
+        }
+        if (split_operand(S.bf, X, ld, kcont, conj, h->st) != 0) {
+            h->err = "split_operand launch failed";
+            return -1;
+        }
+        return 0;
+    }
+    if (!S.i8.planes || S.i8.digits != h->digits) {
+        free_split_op(h, S);
+        i8_layout(S.i8, rows, (int)k, h->digits);
+        CK(dalloc_raw(h, (char**)&S.i8.planes, S.i8.plane_bytes));
+        CK(dalloc_raw(h, &S.i8.mx, S.i8.mx_elems * 8));
+    }
+    if (i8_split(S.i8, X, ld, kcont, conj, h->st) != 0) {
+        h->err = "i8_split launch failed";
+        return -1;
+    }
+    return 0;
+}
+
+int sgemm(rcc_t* h, const SOp& A, const SOp& B, double2* C, int64_t ldc, double2 alpha,
+          double2 beta) {
+    const int rc = h->precision == 1 ? zgemm_split(A.bf, B.bf, C, ldc, alpha, beta, h->st)
+                                     : zgemm_i8(A.i8, B.i8, C, ldc, alpha, beta, h->st);
+    if (rc != 0) {
+        h->err = "split GEMM launch failed";
+        return -1;
+    }
+    return 0;
+}
+
+// Constant split operands (integrals), built once per set of integrals.
+int ensure_split_const(rcc_t* h) {
+    if (h->split_const_ok) return 0;
+    const int64_t nvo = h->nvo, nvvt = h->nvvt;
+    CK(do_split(h, h->sVVVV, (int)nvvt, nvvt, h->cls[RCC_VVVV], nvvt, false, false));
+    CK(do_split(h, h->sVOVV, (int)nvo, nvvt, h->cls[RCC_VOVV], nvo, false, false));
+    // HINTM ring B operand: B(k=(d,l), n=(c,k)) = conj VVOO_R[(c,k),(d,l)]
+    CK(do_split(h, h->sVVOOR, (int)nvo, nvo, h->VVOO_R, nvo, false, true));
+    h->split_const_ok = true;
+    return 0;
+}
+
+int ensure_split_tau(rcc_t* h) {
+    if (h->split_tau_ok) return 0;
+    CK(do_split(h, h->sTAU, (int)h->noot, h->nvvt, h->TAU, h->nvvt, true, false));
+    h->split_tau_ok = true;
+    return 0;
+}
+
+int ensure_x(rcc_t* h) {
+    if (!h->x_ok) {
+        CK(ensure_tau(h));
+        if (h->precision != 0) {
+            CK(ensure_split_const(h));
+            CK(ensure_split_tau(h));
+        }
+        ktick(h, 3, 0);
+        if (h->precision != 0)
+            CK(sgemm(h, h->sVOVV, h->sTAU, h->X, h->nvo, cplx(1, 0), cplx(0, 0)));
+        else
+            CK(gemm(h, OP_N, OP_N, (int)h->nvo, (int)h->noot, (int)h->nvvt, cplx(1, 0),
+                    h->cls[RCC_VOVV], h->nvo, h->TAU, h->nvvt, cplx(0, 0), h->X, h->nvo));
+        ktick(h, 3, 1);
+        h->x_ok = true;
+    }
+    return 0;
 }
 
 int prep(rcc_t* h) {
@@ -408,6 +509,14 @@ int do_hintm(rcc_t* h, bool hakci_has_vovv) {
     k_perm_akci(h->HAKCI, h->H, v, o, false, st);
     // + [1/2 T(AD,IL) - T(A,L) T(D,I)] W(KL,CD)  (ccintm.F:393-450)
     k_expand_ring(h->T2(), h->T1(), 0.5, h->RING2, h->vv, h->oo, v, o, st);
+    if (h->precision != 0) {
+        CK(ensure_split_const(h));
+        CK(do_split(h, h->sP, (int)nvo, nvo, h->RING2, nvo, false, false));
+        ktick(h, 1, 0);
+        CK(sgemm(h, h->sP, h->sVVOOR, h->H, nvo, cplx(1, 0), cplx(1, 0)));
+        ktick(h, 1, 1);
+        return 0;
+    }
     ktick(h, 1, 0);
     CK(gemm(h, OP_N, OP_C, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->RING2, nvo, h->VVOO_R,
             nvo, cplx(1, 0), h->H, nvo, 1, 0, 0, 0, 2));
@@ -452,10 +561,21 @@ int do_t2eqns(rcc_t* h) {
             (int)noot, 0, nvo, (int64_t)v * v));
     k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, st);
     // P(ab)P(ij) H(AI,CK) T2(CK,BJ)  (cceqns.F:1243-1263)
-    ktick(h, 2, 0);
-    CK(gemm(h, OP_N, OP_N, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->H, nvo, h->RING, nvo,
-            cplx(0, 0), h->RING2, nvo, 1, 0, 0, 0, 2));
-    ktick(h, 2, 1);
+    if (h->precision != 0) {
+        CK(do_split(h, h->sH, (int)nvo, nvo, h->H, nvo, false, false));
+        if (!h->split_ring_ok) {
+            CK(do_split(h, h->sRING, (int)nvo, nvo, h->RING, nvo, true, false));
+            h->split_ring_ok = true;
+        }
+        ktick(h, 2, 0);
+        CK(sgemm(h, h->sH, h->sRING, h->RING2, nvo, cplx(1, 0), cplx(0, 0)));
+        ktick(h, 2, 1);
+    } else {
+        ktick(h, 2, 0);
+        CK(gemm(h, OP_N, OP_N, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->H, nvo, h->RING, nvo,
+                cplx(0, 0), h->RING2, nvo, 1, 0, 0, 0, 2));
+        ktick(h, 2, 1);
+    }
     k_pack_pabij(h->RING2, S2, h->vv, h->oo, v, o, st);
     return 0;
 }
@@ -468,10 +588,18 @@ int do_bintm(rcc_t* h) {
     const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
     double2* S2 = h->S2();
     // ladder: S(AB,IJ) += V(AB,CD) TAU(CD,IJ), C>D  (ccintm.F:700-722)
-    ktick(h, 0, 0);
-    CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VVVV], nvvt,
-            h->TAU, nvvt, cplx(1, 0), S2, nvvt, 1, 0, 0, 0, 1));
-    ktick(h, 0, 1);
+    if (h->precision != 0) {
+        CK(ensure_split_const(h));
+        CK(ensure_split_tau(h));
+        ktick(h, 0, 0);
+        CK(sgemm(h, h->sVVVV, h->sTAU, S2, nvvt, cplx(1, 0), cplx(1, 0)));
+        ktick(h, 0, 1);
+    } else {
+        ktick(h, 0, 0);
+        CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VVVV], nvvt,
+                h->TAU, nvvt, cplx(1, 0), S2, nvvt, 1, 0, 0, 0, 1));
+        ktick(h, 0, 1);
+    }
     // - P(ab) T(B,K) X(AK,IJ)  (ccintm.F:760-829)
     CK(gemm(h, OP_N, OP_T, v, v, o, cplx(-1, 0), h->X, v, h->T1(), v, cplx(0, 0), h->ZX, v,
             (int)noot, nvo, 0, (int64_t)v * v));
@@ -802,15 +930,33 @@ int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision) {
         h->err = "rcc_set_options: maxdim must be 1.." + std::to_string(MAXDIM_CAP);
         return -1;
     }
-    if (precision != 0) {
-        h->err = "rcc_set_options: only precision 0 (FP64) is available";
+    if (precision < 0 || precision > 2) {
+        h->err = "rcc_set_options: precision must be 0 (FP64), 1 (split-BF16) or 2 (split-INT8)";
         return -1;
+    }
+    if (precision != h->precision) {
+        free_split(h);
+        invalidate_t(h);
     }
     h->maxdim = maxdim;
     h->maxit = maxit;
     h->ntol = ntol;
     h->precision = precision;
     return alloc_diis(h);
+}
+
+int rcc_set_split_digits(rcc_t* h, int digits) {
+    if (!h) return -1;
+    if (digits < 4 || digits > 5) {
+        h->err = "rcc_set_split_digits: digits must be 4 or 5";
+        return -1;
+    }
+    if (digits != h->digits) {
+        h->digits = digits;
+        h->split_const_ok = h->split_tau_ok = h->split_ring_ok = false;
+        h->x_ok = false;
+    }
+    return 0;
 }
 
 int rcc_set_complex_product(rcc_t* h, int three_mult) {
@@ -847,6 +993,7 @@ static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t n
     const int64_t rows = h->cls_rows[cls];
     CK(upload(h, h->cls[cls] + col0 * rows, buf, rows * ncols, kind));
     h->derived_ok = false;
+    h->split_const_ok = false;
     return 0;
 }
 
@@ -1013,6 +1160,66 @@ int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const doub
             (const double2*)B, ldb, cplx(beta[0], beta[1]), (double2*)C, ldc));
     HIPCHK(h, hipStreamSynchronize(h->st));
     return 0;
+}
+
+int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, const void* A,
+                           int64_t lda, const void* B, int64_t ldb, const double* beta, void* C,
+                           int64_t ldc) {
+    if (!h) return -1;
+    SplitOperand a, b;
+    split_layout(a, m, k);
+    split_layout(b, n, k);
+    void* mem[4] = {nullptr, nullptr, nullptr, nullptr};
+    HIPCHK(h, hipMalloc(&mem[0], 6 * a.plane * 2));
+    HIPCHK(h, hipMalloc(&mem[1], std::max<int64_t>(1, (int64_t)(a.rows_pad / 64) * a.nkc)));
+    HIPCHK(h, hipMalloc(&mem[2], 6 * b.plane * 2));
+    HIPCHK(h, hipMalloc(&mem[3], std::max<int64_t>(1, (int64_t)(b.rows_pad / 64) * b.nkc)));
+    a.planes = mem[0];
+    a.ex = (signed char*)mem[1];
+    b.planes = mem[2];
+    b.ex = (signed char*)mem[3];
+    int rc = split_operand(a, (const double2*)A, lda, false, false, h->st);
+    // B is k x n column-major: as a split operand its rows are n, k contiguous
+    if (rc == 0) rc = split_operand(b, (const double2*)B, ldb, true, false, h->st);
+    if (rc == 0)
+        rc = zgemm_split(a, b, (double2*)C, ldc, cplx(alpha[0], alpha[1]), cplx(beta[0], beta[1]),
+                         h->st);
+    if (rc != 0) h->err = "rcc_zgemm_split_device: launch failed";
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    for (void* p : mem) HIPCHK(h, hipFree(p));
+    return rc;
+}
+
+int rcc_zgemm_i8_device(rcc_t* h, int digits, int m, int n, int k, const double* alpha,
+                        const void* A, int64_t lda, const void* B, int64_t ldb,
+                        const double* beta, void* C, int64_t ldc) {
+    if (!h) return -1;
+    if (digits < 4 || digits > 5 || m < 0 || n < 0 || k <= 0) {
+        h->err = "rcc_zgemm_i8_device: digits must be 4 or 5, m, n >= 0, k > 0";
+        return -1;
+    }
+    I8Operand a, b;
+    i8_layout(a, m, k, digits);
+    i8_layout(b, n, k, digits);
+    void* mem[4] = {nullptr, nullptr, nullptr, nullptr};
+    HIPCHK(h, hipMalloc(&mem[0], std::max<int64_t>(16, a.plane_bytes)));
+    HIPCHK(h, hipMalloc(&mem[1], std::max<int64_t>(16, a.mx_elems * 8)));
+    HIPCHK(h, hipMalloc(&mem[2], std::max<int64_t>(16, b.plane_bytes)));
+    HIPCHK(h, hipMalloc(&mem[3], std::max<int64_t>(16, b.mx_elems * 8)));
+    a.planes = mem[0];
+    a.mx = (unsigned long long*)mem[1];
+    b.planes = mem[2];
+    b.mx = (unsigned long long*)mem[3];
+    int rc = i8_split(a, (const double2*)A, lda, false, false, h->st);
+    // B is k x n column-major: as a split operand its rows are n, k contiguous
+    if (rc == 0) rc = i8_split(b, (const double2*)B, ldb, true, false, h->st);
+    if (rc == 0)
+        rc = zgemm_i8(a, b, (double2*)C, ldc, cplx(alpha[0], alpha[1]), cplx(beta[0], beta[1]),
+                      h->st);
+    if (rc != 0) h->err = "rcc_zgemm_i8_device: launch failed";
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    for (void* p : mem) HIPCHK(h, hipFree(p));
+    return rc;
 }
 
 int rcc_probe_mfma(rcc_t* h, double* out256) {

@@ -80,6 +80,9 @@ def load():
     lib.rcc_kernel_times.argtypes = [vp, dp]
     lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_probe_mfma.argtypes = [vp, dp]
+    lib.rcc_zgemm_split_device.argtypes = [vp, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
+    lib.rcc_zgemm_i8_device.argtypes = [vp, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
+    lib.rcc_set_split_digits.argtypes = [vp, ci]
     lib.rcc_probe_mfma_rate.argtypes = [vp, ci, ci, ci, dp]
     lib.rcc_synchronize.argtypes = [vp]
     _lib = lib
@@ -100,7 +103,7 @@ class RelCCSD:
     """One C1 relativistic CCSD problem resident on the current GPU."""
 
     def __init__(self, no: int, nv: int, frozen=None, maxdim: int = 8, maxit: int = 30,
-                 ntol: int = 12):
+                 ntol: int = 12, precision: int = 0):
         self.lib = load()
         self.no, self.nv = int(no), int(nv)
         self.noot, self.nvvt = self.no * (self.no - 1) // 2, self.nv * (self.nv - 1) // 2
@@ -120,7 +123,7 @@ class RelCCSD:
             msg = self.lib.rcc_last_error(h).decode() if h.value else "rcc_create failed"
             self.close()
             raise RccError(msg)
-        self.set_options(maxdim, maxit, ntol)
+        self.set_options(maxdim, maxit, ntol, precision)
 
     # ------------------------------------------------------------ plumbing
     def _check(self, rc):
@@ -139,7 +142,9 @@ class RelCCSD:
             pass
 
     def set_options(self, maxdim=8, maxit=30, ntol=12, precision=0):
-        self.maxdim, self.maxit, self.ntol = maxdim, maxit, ntol
+        """precision 0: FP64; 1: split-BF16, 2: split-INT8 products for the ladder,
+        VOVV.tau and the rings."""
+        self.maxdim, self.maxit, self.ntol, self.precision = maxdim, maxit, ntol, precision
         self._check(self.lib.rcc_set_options(self.h, maxdim, maxit, ntol, precision))
 
     def set_complex_product(self, three_mult: bool):
@@ -271,6 +276,24 @@ class RelCCSD:
         t = ctypes.c_double()
         self._check(self.lib.rcc_probe_mfma_rate(self.h, chains, iters, nblocks, ctypes.byref(t)))
         return t.value
+
+    def zgemm_split_device(self, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        al = (ctypes.c_double * 2)(alpha.real, alpha.imag)
+        be = (ctypes.c_double * 2)(beta.real, beta.imag)
+        self._check(self.lib.rcc_zgemm_split_device(
+            self.h, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
+            ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc))
+
+    def zgemm_i8_device(self, digits, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+        al = (ctypes.c_double * 2)(alpha.real, alpha.imag)
+        be = (ctypes.c_double * 2)(beta.real, beta.imag)
+        self._check(self.lib.rcc_zgemm_i8_device(
+            self.h, digits, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
+            ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc))
+
+    def set_split_digits(self, digits: int):
+        """Digits (4 or 5) per value of the split-INT8 mode (precision 2)."""
+        self._check(self.lib.rcc_set_split_digits(self.h, digits))
 
     def probe_mfma(self):
         out = np.empty(256, np.float64)
