@@ -70,6 +70,45 @@ def cpu_baseline(workload_flops: float):
     }
 
 
+def set_mode(r, mode, maxit, ntol):
+    if mode == "fp64":
+        r.set_options(6, maxit, ntol, 0)
+    elif mode == "bf16":
+        r.set_options(6, maxit, ntol, 1)
+    else:
+        r.set_options(6, maxit, ntol, 2)
+        r.set_split_digits(int(mode[-1]))
+
+
+def converged_accuracy(no, nv, seed, scale, modes, maxit=40, ntol=9):
+    """Converge FP64 and each split mode from the same MP2 guess on a system
+    drawn with a smaller factor std (one whose CCSD converges) and return the
+    converged energies and |dEc| against FP64."""
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+    sysd = synth.make_system(no, nv, seed=seed, scale=scale, device="cuda")
+    r = RelCCSD(no, nv, maxdim=6, maxit=maxit, ntol=ntol)
+    r.set_system(sysd)
+    del sysd
+    torch.cuda.empty_cache()
+    res = {}
+    for mode in ["fp64"] + list(modes):
+        set_mode(r, mode, maxit, ntol)
+        t0 = time.perf_counter()
+        o = r.cceqn()
+        res[mode] = {"scale": scale, "emp2": o["emp2"], "ecc": o["eccsd"], "nit": o["nit"],
+                     "rms": o["rows"][-1][2], "converged": o["rows"][-1][2] < 10.0 ** -ntol,
+                     "wall_s": time.perf_counter() - t0}
+        print(f"[bench] converge {mode} scale {scale}: Ec {o['eccsd']:.12f} nit {o['nit']} "
+              f"rms {o['rows'][-1][2]:.2e}", file=sys.stderr, flush=True)
+    for mode in modes:
+        res[mode]["dEc"] = abs(res[mode]["ecc"] - res["fp64"]["ecc"])
+    r.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +125,9 @@ def main():
                          "guess with the ladder, VOVV.tau and rings as split products "
                          "(bf16: 3 BF16 slices; i8xS: S INT8 digits) and report s/iter and "
                          "the per-iteration |dEc| against FP64")
+    ap.add_argument("--accuracy-scale", type=float, default=0.005,
+                    help="also converge FP64 and each split mode on a same-size system whose "
+                         "factor has this std (0: skip) and report the converged |dEc|")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) and exit: the "
                          "reference point that scales rocprof FETCH_SIZE / WRITE_SIZE")
@@ -198,8 +240,8 @@ def main():
     if modes:
         out["split"] = {}
     for mode in modes:
+        set_mode(r, mode, args.steps + args.warmup, 12)
         if mode == "bf16":
-            r.set_options(6, args.steps + args.warmup, 12, 1)
             # 4 real products x 6 slice products per complex product
             ops, peak = 24 * 2.0 * nvvt * nvvt * noot, PEAK_BF16_MFMA
             scheme = ("3 BF16 slices per FP64 value, 6 slice products, FP32 accumulation over "
@@ -207,8 +249,6 @@ def main():
             counts = "executed BF16 MFMA flops (24 real products per complex product)"
         else:
             d = int(mode[-1])
-            r.set_options(6, args.steps + args.warmup, 12, 2)
-            r.set_split_digits(d)
             pairs = d * (d + 1) // 2
             ops, peak = 3 * pairs * 2.0 * nvvt * nvvt * noot, PEAK_INT8_MFMA
             scheme = (f"Re, Im, Re+Im as {d} signed 7-bit digits (fixed point per row and 8192-k "
@@ -228,6 +268,20 @@ def main():
                                 "peak": peak, "unit": "TFLOP/s" if mode == "bf16" else "TOP/s",
                                 "frac": ops / (lad * 1e-3) / 1e12 / peak, "counts": counts},
         }
+    if modes and args.accuracy_scale > 0:
+        r.close()
+        del r
+        torch.cuda.empty_cache()
+        acc = converged_accuracy(no, nv, seed, args.accuracy_scale, modes)
+        for mode in modes:
+            out["split"][mode]["converged"] = acc[mode]
+        out["split_converged_fp64"] = acc["fp64"]
+        ok = [m for m in modes if acc[m]["dEc"] < 1e-6]
+        if ok:
+            best = max(ok, key=lambda m: out["split"][m]["speedup_vs_fp64"])
+            out["split_best_at_1uEh"] = {"mode": best,
+                                         "speedup_vs_fp64": out["split"][best]["speedup_vs_fp64"],
+                                         "converged_dEc": acc[best]["dEc"]}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(F)

@@ -137,18 +137,6 @@ __global__ void __launch_bounds__(256) i8_digits_kernel(const double2* __restric
 // Stages run chunk by chunk, and within a chunk variant by variant, so only
 // one variant's S int32 accumulators are live; after each (chunk, variant)
 // they are scaled and added into C (FP64).
-struct I8Stage {
-    int c, v, ks;
-};
-
-__device__ __forceinline__ bool i8_next(I8Stage& s, int nks) {
-    const int ks0 = s.c * I8_KS_PER_CH, ks1 = min(nks, ks0 + I8_KS_PER_CH);
-    if (s.ks + 1 < ks1) { ++s.ks; return true; }
-    if (s.v < 2) { ++s.v; s.ks = ks0; return true; }
-    if (ks1 < nks) { ++s.c; s.v = 0; s.ks = ks1; return true; }
-    return false;
-}
-
 struct I8Tile {
     int m, n, nks, nch, tm, tn, wm, wn, lane, wave;
     int row0, col0;   // first row / column of this wave's 32 x 32 block
@@ -214,70 +202,70 @@ __device__ __forceinline__ void i8_flush(const I8Tile& t, int c, int v, i32x4 (&
             for (int j = 0; j < 2; ++j) acc[o][i][j] = (i32x4){0, 0, 0, 0};
 }
 
-// One stage: issue the global loads of the next stage into registers,
-// multiply out of LDS buffer `buf`, write the next stage into the other
-// buffer, flush when a (chunk, variant) ends, barrier.  Returns false after
-// the last stage.  The tile has NA = 2 WM A row blocks and 4 B row blocks;
-// wave w copies A row block w and, if w < 4, B row block w.
-template <int S, int WM>
-__device__ __forceinline__ bool i8_stage(const I8Tile& t, i32x4 (*lds)[(2 * WM + 4) * S][64],
-                                         const i32x4* ga, const i32x4* gb, I8Stage& cur,
-                                         int& buf, i32x4 (&acc)[S][2][2]) {
-    constexpr int NA = 2 * WM;
-    I8Stage n1 = cur;
-    const bool has1 = i8_next(n1, t.nks);
-    const bool copy_b = t.wave < 4;
-    i32x4 sa[S], sb[S];
-    if (has1) {
-        const int64_t off = ((int64_t)n1.ks * 3 + n1.v) * S * 64;
-#pragma unroll
-        for (int d = 0; d < S; ++d) {
-            sa[d] = ga[off + d * 64];
-            if (copy_b) sb[d] = gb[off + d * 64];
-        }
+// s_waitcnt vmcnt(N) only (expcnt and lgkmcnt at their no-wait maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr;
+
+// Stage (ks, v) of a (chunk, variant)-ordered sequence: chunk c, variant v,
+// k-step ks; `last` when the next stage starts another (chunk, variant).
+struct I8Stage {
+    int c, v, ks;
+    bool last;
+};
+
+__device__ __forceinline__ I8Stage i8_stage_of(int st, int nks) {
+    const int full = nks / I8_KS_PER_CH, rem = nks % I8_KS_PER_CH;
+    I8Stage s;
+    int r, len;
+    if (st < full * 3 * I8_KS_PER_CH) {
+        s.c = st / (3 * I8_KS_PER_CH);
+        r = st % (3 * I8_KS_PER_CH);
+        len = I8_KS_PER_CH;
+    } else {
+        s.c = full;
+        r = st - full * 3 * I8_KS_PER_CH;
+        len = rem;
     }
-    {
-        i32x4 a[2][S], bl[2][S];
+    s.v = r / len;
+    s.ks = s.c * I8_KS_PER_CH + r % len;
+    s.last = r % len == len - 1;
+    return s;
+}
+
+// Asynchronous copy of stage st into LDS buffer `dst` (global -> LDS DMA, no
+// registers): wave w copies the S digit fragments of A row block w and, if
+// w < 4, of B row block w; lane l writes bytes 16 l .. 16 l + 15 of each
+// 1 KiB fragment, exactly the fragment-major layout of the planes.
+template <int S, int NA>
+__device__ __forceinline__ void i8_copy(i32x4 (*dst)[64], const i32x4* ga, const i32x4* gb,
+                                        int wave, int st, int nks) {
+    const I8Stage g = i8_stage_of(st, nks);
+    const int64_t off = ((int64_t)g.ks * 3 + g.v) * S * 64;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+    for (int d = 0; d < S; ++d)
+        __builtin_amdgcn_global_load_lds((const void*)(ga + off + d * 64),
+                                         (lds_ptr)&dst[wave * S + d][0], 16, 0, 0);
+    if (wave < 4) {
 #pragma unroll
-            for (int d = 0; d < S; ++d) {
-                a[i][d] = lds[buf][(2 * t.wm + i) * S + d][t.lane];
-                bl[i][d] = lds[buf][(NA + 2 * t.wn + i) * S + d][t.lane];
-            }
-#pragma unroll
-        for (int da = 0; da < S; ++da)
-#pragma unroll
-            for (int db = 0; db + da < S; ++db)
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j)
-                        acc[da + db][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                            a[i][da], bl[j][db], acc[da + db][i][j], 0, 0, 0);
+        for (int d = 0; d < S; ++d)
+            __builtin_amdgcn_global_load_lds((const void*)(gb + off + d * 64),
+                                             (lds_ptr)&dst[(NA + wave) * S + d][0], 16, 0, 0);
     }
-    if (has1) {
-#pragma unroll
-        for (int d = 0; d < S; ++d) {
-            lds[buf ^ 1][t.wave * S + d][t.lane] = sa[d];
-            if (copy_b) lds[buf ^ 1][(NA + t.wave) * S + d][t.lane] = sb[d];
-        }
-    }
-    if (!has1 || n1.v != cur.v) i8_flush<S>(t, cur.c, cur.v, acc);
-    __syncthreads();
-    cur = n1;
-    buf ^= 1;
-    return has1;
 }
 
 // C(m x n) = alpha A B^T + beta C.  One workgroup of 2 WM waves per
 // (32 WM) x 64 tile, waves WM x 2 of 32 x 32 (2 x 2 MFMA blocks).  A stage is
 // one (k-step, variant): the S digit fragments of the tile's 2 WM A and 4 B
-// row blocks, double-buffered in LDS with one barrier per stage, the next
-// stage fetched while this one multiplies.  Stages run chunk by chunk, and
-// within a chunk variant by variant, so only one variant's S int32
-// accumulators are live; after each (chunk, variant) they are scaled and
-// added into C (FP64).
+// row blocks.  Three LDS buffers: stage st + 2 is copied by DMA while st is
+// multiplied, and a wave waits only for its own copies of stage st before
+// the stage barrier.  Stages run chunk by chunk, and within a chunk variant
+// by variant, so only one variant's S int32 accumulators are live; after each
+// (chunk, variant) they are scaled and added into C (FP64).
 template <int S, int WM>
 __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
     int m, int n, int nks, int nch, const i32x4* __restrict__ PA,
@@ -285,7 +273,7 @@ __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
     const unsigned long long* __restrict__ MB, double2* __restrict__ C, int64_t ldc,
     double2 alpha, double2 beta) {
     constexpr int NA = 2 * WM;
-    __shared__ i32x4 lds[2][(NA + 4) * S][64];
+    __shared__ i32x4 lds[3][(NA + 4) * S][64];
     const int mt = (m + 32 * WM - 1) / (32 * WM), nt = (n + 63) / 64;
     const int T = mt * nt;
     const int b = blockIdx.x;
@@ -315,13 +303,7 @@ __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
     const int64_t rbs = (int64_t)nks * 3 * S * 64;   // i32x4 per row block
     const i32x4* ga = PA + (NA * t.tm + t.wave) * rbs + t.lane;
     const i32x4* gb = PB + (4 * t.tn + (t.wave & 3)) * rbs + t.lane;
-
-#pragma unroll
-    for (int d = 0; d < S; ++d) {
-        lds[0][t.wave * S + d][t.lane] = ga[d * 64];
-        if (t.wave < 4) lds[0][(NA + t.wave) * S + d][t.lane] = gb[d * 64];
-    }
-    __syncthreads();
+    const bool copy_b = t.wave < 4;
 
     i32x4 acc[S][2][2];
 #pragma unroll
@@ -331,9 +313,41 @@ __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[o][i][j] = (i32x4){0, 0, 0, 0};
 
-    I8Stage cur = {0, 0, 0};
-    int buf = 0;
-    while (i8_stage<S, WM>(t, lds, ga, gb, cur, buf, acc)) {
+    const int nst = 3 * nks;
+    i8_copy<S, NA>(lds[0], ga, gb, t.wave, 0, nks);
+    if (nst > 1) i8_copy<S, NA>(lds[1], ga, gb, t.wave, 1, nks);
+    for (int st = 0; st < nst; ++st) {
+        // this wave's copies of stage st have landed (those of st + 1 may not)
+        if (st + 1 < nst) {
+            if (copy_b) wait_vmcnt<2 * S>();
+            else wait_vmcnt<S>();
+        } else {
+            wait_vmcnt<0>();
+        }
+        __builtin_amdgcn_s_barrier();
+        // every wave is past stage st - 1, so its buffer can be refilled
+        if (st + 2 < nst) i8_copy<S, NA>(lds[(st + 2) % 3], ga, gb, t.wave, st + 2, nks);
+        const int cb = st % 3;
+        i32x4 a[2][S], bl[2][S];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int d = 0; d < S; ++d) {
+                a[i][d] = lds[cb][(2 * t.wm + i) * S + d][t.lane];
+                bl[i][d] = lds[cb][(NA + 2 * t.wn + i) * S + d][t.lane];
+            }
+#pragma unroll
+        for (int da = 0; da < S; ++da)
+#pragma unroll
+            for (int db = 0; db + da < S; ++db)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[da + db][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                            a[i][da], bl[j][db], acc[da + db][i][j], 0, 0, 0);
+        const I8Stage g = i8_stage_of(st, nks);
+        if (g.last) i8_flush<S>(t, g.c, g.v, acc);
     }
 }
 
@@ -342,7 +356,7 @@ void i8_layout(I8Operand& S, int rows, int k, int digits) {
     S.rows = rows;
     S.k = k;
     S.digits = digits;
-    S.rows_pad = (rows + 127) / 128 * 128;   // whole 128-row GEMM tiles
+    S.rows_pad = (rows + 383) / 384 * 384;   // whole 64-, 96- and 128-row GEMM tiles
     S.nks = (k + I8_KSTEP - 1) / I8_KSTEP;
     S.nch = (S.nks + I8_KS_PER_CH - 1) / I8_KS_PER_CH;
     S.plane_bytes = (int64_t)(S.rows_pad / 16) * S.nks * 3 * digits * 1024;
@@ -399,12 +413,13 @@ int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, do
         const char* e = getenv("RCC_I8_WM");
         return e ? atoi(e) : 4;
     }();
+    // three LDS buffers of (2 WM + 4) S KiB: 144 KiB for (4, 4), 150 KiB for (5, 3)
     if (A.digits == 4) {
         if (wm == 2) launch_i8<4, 2>(A, B, C, ldc, alpha, beta, st);
         else launch_i8<4, 4>(A, B, C, ldc, alpha, beta, st);
     } else if (A.digits == 5) {
         if (wm == 2) launch_i8<5, 2>(A, B, C, ldc, alpha, beta, st);
-        else launch_i8<5, 4>(A, B, C, ldc, alpha, beta, st);
+        else launch_i8<5, 3>(A, B, C, ldc, alpha, beta, st);
     } else {
         return -1;
     }
