@@ -44,16 +44,34 @@ enum {
 };
 
 /* Symmetry description, as RDSYMI / SYMTAB hold it (ccdriv.F:2934, :746).
- * nrep: number of fermion irreps of the abelian double group (this build:
- * nrep must be 1, the C1 case); no/nv: occupied / virtual spinors per irrep;
- * frozen: per occupied spinor (irrep-major) 1 = frozen core (IFROZ,
- * freeze.inc), may be NULL. */
+ * nrep: number of fermion irreps of the abelian double group (1..32);
+ * no/nv: occupied / virtual spinors per irrep (NO, NV of symm.inc);
+ * frozen: per occupied spinor in irrep-major order, 1 = frozen core (IFROZ,
+ * freeze.inc), may be NULL; multb: for nrep > 1, the nrep x nrep pair-irrep
+ * table MULTB(1:NREP,1:NREP,1) as RDSYMI leaves it (boson offset removed,
+ * values 1..nrep, column-major), may be NULL for nrep = 1.
+ * Every array of the interface is then in SYMTAB's symmetry-blocked layout:
+ * one block per compound irrep (classes, T2) or per irrep (Fock, T1), blocks
+ * consecutive (IOOOOTT .. IVVVVTT, IOO/IVO/IVV, ccdriv.F:1150-1187);
+ * rcc_layout_block reports each block's shape and offset.  The orbital
+ * energies are in SYMTAB's order: occupied irrep-major, then virtual
+ * irrep-major (ccdriv.F:1392-1402). */
 typedef struct {
     int nrep;
     const int* no;
     const int* nv;
     const int* frozen;
+    const int* multb;
 } rcc_sym_t;
+
+/* Layout ids for rcc_layout_block: the six class ids (RCC_OOOO..RCC_VVVV)
+ * and the following. */
+enum {
+    RCC_LAY_T1 = 6,    /* T1, S1, FVO, HOV: NV x NO per irrep (IVO) */
+    RCC_LAY_T2 = 7,    /* T2, S2, VVOO: NVVT x NOOT per compound irrep (IVVOOTT) */
+    RCC_LAY_FOO = 8,   /* FOO, HO, GO: NO x NO per irrep (IOO) */
+    RCC_LAY_FVV = 9    /* FVV, HV, GV: NV x NV per irrep (IVV) */
+};
 
 /* Replaces SYMTAB's dimension/offset set-up (ccdriv.F:746-1425) and the
  * CCENER work-array allocation (ccdriv.F:1622-1690) on the device. */
@@ -86,9 +104,10 @@ int rcc_set_split_digits(rcc_t* h, int digits);
 int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,
                  const double* eps);
 
-/* One column batch [col0, col0+ncols) of integral class cls of irrep
- * (0-based) from host memory, as GETVOVV / GETVVVV stream them
- * (ccgetv.F:225-402); whole classes are col0 = 0, ncols = all columns. */
+/* One column batch [col0, col0+ncols) of the block of compound irrep
+ * `irrep` (0-based, Fortran IRP - 1) of integral class cls from host memory,
+ * as GETVOVV / GETVVVV stream them (ccgetv.F:225-402); a whole block is
+ * col0 = 0, ncols = all its columns. */
 int rcc_set_ints(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols, const double* buf);
 /* Same, from device memory already resident on this GPU. */
 int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols,
@@ -99,6 +118,11 @@ int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2);
 int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2);
 /* Copy of a device-resident array (RCC_ARR_*) to host memory. */
 int rcc_get_array(rcc_t* h, int which, double* out, int64_t nelem_complex);
+/* Shape (rows x cols) and offset (complex elements) of block `block`
+ * (0..nrep-1) of a layout; block = nrep gives the layout's total size as
+ * offset.  Replaces reading SYMTAB's offset arrays (ccdriv.F:1150-1187). */
+int rcc_layout_block(rcc_t* h, int lay, int block, int64_t* rows, int64_t* cols,
+                     int64_t* offset);
 
 /* MP2EQN (cceqns.F:33-139): T1 = FVO/D, T2 = VVOO/D, ZCORE, MP2ENG energy. */
 int rcc_mp2(rcc_t* h, double* emp2);

@@ -83,6 +83,14 @@ void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, in
 void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
                hipStream_t st);
 
+// symmetry blocks (packed, column-major, rows x ncols) <-> C1 arrays:
+// C1 element of block element (r, c) = rowmap[r] + ld * colmap[c]
+void k_scatter_block(const double2* src, int64_t rows, int64_t ncols, const int* rowmap,
+                     const int* colmap, double2* dst, int64_t ld, hipStream_t st);
+void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int* colmap,
+                    int64_t rows, int64_t ncols, double2* dst, hipStream_t st);
+void k_fill_zero(double2* p, int64_t n, hipStream_t st);
+
 // ---- split-BF16 operands and GEMM (rcc_split.hip) ----
 // Six BF16 planes (Re slices 1-3, Im slices 1-3), K-major, rows padded to 64
 // and k to 128, plus one power-of-two exponent per (64-row, 128-k) block.

@@ -533,4 +533,52 @@ void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y,
     hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, cp, nx, y, n);
 }
 
+// ---- symmetry blocks <-> C1 layout ----
+// One thread per block element (r, c), r fastest: coalesced on the packed
+// block side; the C1 side is a scatter / gather through the pair maps.
+__global__ void scatter_block_kernel(const double2* __restrict__ src, int64_t rows, int64_t n,
+                                     const int* __restrict__ rowmap,
+                                     const int* __restrict__ colmap, double2* __restrict__ dst,
+                                     int64_t ld) {
+    GRID_LOOP(e, n) {
+        const int64_t r = e % rows, c = e / rows;
+        dst[(int64_t)rowmap[r] + ld * colmap[c]] = src[e];
+    }
+}
+
+__global__ void gather_block_kernel(const double2* __restrict__ src, int64_t ld,
+                                    const int* __restrict__ rowmap,
+                                    const int* __restrict__ colmap, int64_t rows, int64_t n,
+                                    double2* __restrict__ dst) {
+    GRID_LOOP(e, n) {
+        const int64_t r = e % rows, c = e / rows;
+        dst[e] = src[(int64_t)rowmap[r] + ld * colmap[c]];
+    }
+}
+
+__global__ void fill_zero_kernel(double2* __restrict__ p, int64_t n) {
+    GRID_LOOP(e, n) { p[e] = make_double2(0.0, 0.0); }
+}
+
+void k_scatter_block(const double2* src, int64_t rows, int64_t ncols, const int* rowmap,
+                     const int* colmap, double2* dst, int64_t ld, hipStream_t st) {
+    const int64_t n = rows * ncols;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(scatter_block_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, src, rows, n,
+                       rowmap, colmap, dst, ld);
+}
+
+void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int* colmap,
+                    int64_t rows, int64_t ncols, double2* dst, hipStream_t st) {
+    const int64_t n = rows * ncols;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(gather_block_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, src, ld, rowmap,
+                       colmap, rows, n, dst);
+}
+
+void k_fill_zero(double2* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, p, n);
+}
+
 }  // namespace rcc

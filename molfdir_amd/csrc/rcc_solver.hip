@@ -46,6 +46,24 @@ struct Buf {
 
 }  // namespace
 
+// Symmetry-blocked layouts of SYMTAB (ccdriv.F:746-1425).  The solver runs
+// in the C1 layout of the whole spinor set (occupied irrep-major, then
+// virtual irrep-major -- the order SYMTAB gives the orbital energies,
+// ccdriv.F:1392-1402); blocks are scattered into / gathered out of it
+// through per-block maps of C1 pair indices.  Off-symmetry elements stay
+// exactly zero through the iterations (every product they would receive has
+// a zero factor).
+enum { SK_OO = 0, SK_VV = 1, SK_VO = 2, SK_OCC = 3, SK_VIR = 4 };
+
+struct SymInfo {
+    int nrep = 1;
+    std::vector<int> no, nv, io, iv;
+    std::vector<int> ptab;                  // pair irrep of (i, j): ptab[i + nrep j], 0-based
+    std::vector<std::vector<int>> map[5];   // per block: C1 index of each packed row/column
+    std::vector<int64_t> doff[5];           // offset of map[k][b] in dmap
+    int* dmap = nullptr;
+};
+
 // One logical split operand, in the format of the current precision mode.
 struct SOp {
     SplitOperand bf;
@@ -103,6 +121,10 @@ struct rcc_handle {
     hipEvent_t kev[4][2] = {};
     double kms[4] = {0, 0, 0, 0};
     bool ktimed[4] = {false, false, false, false};
+
+    SymInfo sym;
+    double2* stage = nullptr;   // device staging for host blocks
+    int64_t stage_n = 0;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -855,6 +877,169 @@ int upload(rcc_t* h, double2* dst, const void* src, int64_t n, hipMemcpyKind kin
     return 0;
 }
 
+// ---- symmetry layouts ----
+// Build the pair orderings of SYMTAB: packed pairs of compound irrep c in
+// sub-blocks (i >= j, ptab(i, j) = c) ordered by j (IIOOT / IIVVT,
+// ccdriv.F:1071-1082), J outer and I inner (I > J on the diagonal; DENOM
+// cceqns.F:2408-2448); VO pairs in sub-blocks (a, k) ordered by k (IIVO,
+// ccdriv.F:1050-1057), K outer, A inner.
+int build_sym(rcc_t* h, const rcc_sym_t* sy) {
+    SymInfo& S = h->sym;
+    const int n = sy->nrep;
+    S.nrep = n;
+    S.no.assign(sy->no, sy->no + n);
+    S.nv.assign(sy->nv, sy->nv + n);
+    S.io.assign(n + 1, 0);
+    S.iv.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        S.io[i + 1] = S.io[i] + S.no[i];
+        S.iv[i + 1] = S.iv[i] + S.nv[i];
+    }
+    S.ptab.assign((size_t)n * n, 0);
+    if (n > 1)
+        for (int i = 0; i < n * n; ++i) S.ptab[i] = sy->multb[i] - 1;
+    // partner[j + n c] = the i with ptab(i, j) = c; the table must be a
+    // symmetric Latin square (an abelian group's)
+    std::vector<int> partner((size_t)n * n, -1);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const int c = S.ptab[i + n * j];
+            if (c < 0 || c >= n || partner[j + n * c] >= 0 || S.ptab[j + n * i] != c) {
+                h->err = "rcc_create: multb is not the pair-irrep table of an abelian group";
+                return -1;
+            }
+            partner[j + n * c] = i;
+        }
+    const int64_t NO = S.io[n], NV = S.iv[n];
+    auto c1pair = [](int64_t x, int64_t y, int64_t m) {
+        return y * (2 * m - y - 1) / 2 + (x - y - 1);
+    };
+    for (int k = 0; k < 5; ++k) S.map[k].assign(n, {});
+    for (int c = 0; c < n; ++c) {
+        for (int pass = 0; pass < 2; ++pass) {
+            const std::vector<int>& cnt = pass == 0 ? S.no : S.nv;
+            const std::vector<int>& off = pass == 0 ? S.io : S.iv;
+            const int64_t tot = pass == 0 ? NO : NV;
+            std::vector<int>& m = S.map[pass == 0 ? SK_OO : SK_VV][c];
+            for (int jr = 0; jr < n; ++jr) {
+                const int ir = partner[jr + n * c];
+                if (ir < jr) continue;
+                for (int j = 0; j < cnt[jr]; ++j)
+                    for (int i = ir == jr ? j + 1 : 0; i < cnt[ir]; ++i)
+                        m.push_back((int)c1pair(off[ir] + i, off[jr] + j, tot));
+            }
+        }
+        std::vector<int>& vo = S.map[SK_VO][c];
+        for (int kr = 0; kr < n; ++kr) {
+            const int ar = partner[kr + n * c];
+            for (int k = 0; k < S.no[kr]; ++k)
+                for (int a = 0; a < S.nv[ar]; ++a)
+                    vo.push_back((int)((S.io[kr] + k) * NV + S.iv[ar] + a));
+        }
+        for (int i = 0; i < S.no[c]; ++i) S.map[SK_OCC][c].push_back(S.io[c] + i);
+        for (int a = 0; a < S.nv[c]; ++a) S.map[SK_VIR][c].push_back(S.iv[c] + a);
+    }
+    int64_t tot = 0;
+    for (int k = 0; k < 5; ++k) {
+        S.doff[k].assign(n, 0);
+        for (int c = 0; c < n; ++c) {
+            S.doff[k][c] = tot;
+            tot += (int64_t)S.map[k][c].size();
+        }
+    }
+    CK(dalloc_raw(h, &S.dmap, std::max<int64_t>(tot, 1) * 4));
+    std::vector<int> all;
+    all.reserve(tot);
+    for (int k = 0; k < 5; ++k)
+        for (int c = 0; c < n; ++c) all.insert(all.end(), S.map[k][c].begin(), S.map[k][c].end());
+    if (tot) HIPCHK(h, hipMemcpy(S.dmap, all.data(), tot * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Layout ids: the six classes (RCC_OOOO..RCC_VVVV) and RCC_LAY_T1 / T2 /
+// FOO / FVV: row and column kinds of each and the C1 leading dimension.
+bool lay_spec(const rcc_t* h, int lay, int* rk, int* ck, int64_t* ld) {
+    switch (lay) {
+        case RCC_OOOO: *rk = SK_OO; *ck = SK_OO; *ld = h->noot; return true;
+        case RCC_VOOO: *rk = SK_VO; *ck = SK_OO; *ld = h->nvo; return true;
+        case RCC_VVOO: *rk = SK_VV; *ck = SK_OO; *ld = h->nvvt; return true;
+        case RCC_VOVO: *rk = SK_VO; *ck = SK_VO; *ld = h->nvo; return true;
+        case RCC_VOVV: *rk = SK_VO; *ck = SK_VV; *ld = h->nvo; return true;
+        case RCC_VVVV: *rk = SK_VV; *ck = SK_VV; *ld = h->nvvt; return true;
+        case RCC_LAY_T1: *rk = SK_VIR; *ck = SK_OCC; *ld = h->nv; return true;
+        case RCC_LAY_T2: *rk = SK_VV; *ck = SK_OO; *ld = h->nvvt; return true;
+        case RCC_LAY_FOO: *rk = SK_OCC; *ck = SK_OCC; *ld = h->no; return true;
+        case RCC_LAY_FVV: *rk = SK_VIR; *ck = SK_VIR; *ld = h->nv; return true;
+        default: return false;
+    }
+}
+
+int64_t lay_block_rows(const rcc_t* h, int kind, int b) {
+    return (int64_t)h->sym.map[kind][b].size();
+}
+
+// offset of block b in the concatenated layout (b = nrep: total size)
+int64_t lay_offset(const rcc_t* h, int rk, int ck, int b) {
+    int64_t off = 0;
+    for (int c = 0; c < b; ++c)
+        off += (int64_t)h->sym.map[rk][c].size() * (int64_t)h->sym.map[ck][c].size();
+    return off;
+}
+
+int ensure_stage(rcc_t* h, int64_t n) {
+    if (n <= h->stage_n) return 0;
+    if (h->stage) {
+        auto it = std::find(h->allocs.begin(), h->allocs.end(), (void*)h->stage);
+        if (it != h->allocs.end()) h->allocs.erase(it);
+        HIPCHK(h, hipFree(h->stage));
+        h->stage = nullptr;
+        h->stage_n = 0;
+    }
+    CK(dalloc(h, &h->stage, n));
+    h->stage_n = n;
+    return 0;
+}
+
+// Whole symmetry-layout array (host or device) -> C1 array, zero elsewhere.
+int sym_to_c1(rcc_t* h, int lay, const void* src, hipMemcpyKind kind, double2* dst,
+              int64_t c1_elems) {
+    int rk, ck;
+    int64_t ld;
+    lay_spec(h, lay, &rk, &ck, &ld);
+    const SymInfo& S = h->sym;
+    const int64_t total = lay_offset(h, rk, ck, S.nrep);
+    const double2* dsrc = (const double2*)src;
+    if (kind == hipMemcpyHostToDevice) {
+        CK(ensure_stage(h, total));
+        HIPCHK(h, hipMemcpyAsync(h->stage, src, (size_t)total * 16, kind, h->st));
+        dsrc = h->stage;
+    }
+    k_fill_zero(dst, c1_elems, h->st);
+    for (int b = 0; b < S.nrep; ++b)
+        k_scatter_block(dsrc + lay_offset(h, rk, ck, b), lay_block_rows(h, rk, b),
+                        lay_block_rows(h, ck, b), S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
+                        dst, ld, h->st);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// C1 array -> whole symmetry-layout array in host memory.
+int c1_to_sym_host(rcc_t* h, int lay, const double2* src, double* out) {
+    int rk, ck;
+    int64_t ld;
+    lay_spec(h, lay, &rk, &ck, &ld);
+    const SymInfo& S = h->sym;
+    const int64_t total = lay_offset(h, rk, ck, S.nrep);
+    CK(ensure_stage(h, total));
+    for (int b = 0; b < S.nrep; ++b)
+        k_gather_block(src, ld, S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
+                       lay_block_rows(h, rk, b), lay_block_rows(h, ck, b),
+                       h->stage + lay_offset(h, rk, ck, b), h->st);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(out, h->stage, (size_t)total * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
@@ -865,13 +1050,22 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
     *out = nullptr;
     rcc_t* h = new rcc_t();
     *out = h;
-    if (!sym || sym->nrep != 1 || !sym->no || !sym->nv) {
-        h->err = "rcc_create: this build supports the C1 double group (nrep = 1) only";
+    if (!sym || sym->nrep < 1 || sym->nrep > 32 || !sym->no || !sym->nv ||
+        (sym->nrep > 1 && !sym->multb)) {
+        h->err = "rcc_create: need 1 <= nrep <= 32, no[], nv[] and (nrep > 1) multb";
         return -1;
     }
-    h->no = sym->no[0];
-    h->nv = sym->nv[0];
-    if (h->no < 0 || h->nv < 0 || h->no > 4096 || h->nv > 8192) {
+    h->no = 0;
+    h->nv = 0;
+    for (int i = 0; i < sym->nrep; ++i) {
+        if (sym->no[i] < 0 || sym->nv[i] < 0) {
+            h->err = "rcc_create: negative orbital count";
+            return -1;
+        }
+        h->no += sym->no[i];
+        h->nv += sym->nv[i];
+    }
+    if (h->no > 4096 || h->nv > 8192) {
         h->err = "rcc_create: unsupported orbital-space size";
         return -1;
     }
@@ -886,6 +1080,15 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
     for (int w = 0; w < 4; ++w)
         for (int p = 0; p < 2; ++p) HIPCHK(h, hipEventCreate(&h->kev[w][p]));
     CK(alloc_all(h));
+    CK(build_sym(h, sym));
+    if (sym->nrep > 1) {
+        // off-symmetry blocks of the C1 arrays are never written
+        k_fill_zero(h->FOO, (int64_t)h->no * h->no, h->st);
+        k_fill_zero(h->FVO, h->nvo, h->st);
+        k_fill_zero(h->FVV, (int64_t)h->nv * h->nv, h->st);
+        for (int c = 0; c < 6; ++c) k_fill_zero(h->cls[c], h->cls_rows[c] * h->cls_cols[c], h->st);
+        HIPCHK(h, hipStreamSynchronize(h->st));
+    }
     // pair tables
     std::vector<int2> oo, vv;
     for (int y = 0; y < h->no; ++y)
@@ -968,6 +1171,14 @@ int rcc_set_complex_product(rcc_t* h, int three_mult) {
 int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* fvv,
                  const double* eps) {
     if (!h) return -1;
+    if (h->sym.nrep > 1) {
+        CK(sym_to_c1(h, RCC_LAY_FOO, foo, hipMemcpyHostToDevice, h->FOO, (int64_t)h->no * h->no));
+        CK(sym_to_c1(h, RCC_LAY_T1, fvo, hipMemcpyHostToDevice, h->FVO, h->nvo));
+        CK(sym_to_c1(h, RCC_LAY_FVV, fvv, hipMemcpyHostToDevice, h->FVV, (int64_t)h->nv * h->nv));
+        HIPCHK(h, hipMemcpy(h->eps, eps, (size_t)(h->no + h->nv) * 8, hipMemcpyHostToDevice));
+        h->derived_ok = false;
+        return 0;
+    }
     CK(upload(h, h->FOO, foo, (int64_t)h->no * h->no, hipMemcpyHostToDevice));
     CK(upload(h, h->FVO, fvo, h->nvo, hipMemcpyHostToDevice));
     CK(upload(h, h->FVV, fvv, (int64_t)h->nv * h->nv, hipMemcpyHostToDevice));
@@ -982,16 +1193,37 @@ static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t n
         h->err = "rcc_set_ints: bad class";
         return -1;
     }
-    if (irrep != 0) {
-        h->err = "rcc_set_ints: irrep out of range (C1 build)";
+    if (irrep < 0 || irrep >= h->sym.nrep) {
+        h->err = "rcc_set_ints: irrep out of range";
         return -1;
     }
-    if (col0 < 0 || ncols < 0 || col0 + ncols > h->cls_cols[cls]) {
-        h->err = "rcc_set_ints: column batch outside the class";
-        return -1;
+    if (h->sym.nrep == 1) {
+        if (col0 < 0 || ncols < 0 || col0 + ncols > h->cls_cols[cls]) {
+            h->err = "rcc_set_ints: column batch outside the class";
+            return -1;
+        }
+        const int64_t rows = h->cls_rows[cls];
+        CK(upload(h, h->cls[cls] + col0 * rows, buf, rows * ncols, kind));
+    } else {
+        int rk, ck;
+        int64_t ld;
+        lay_spec(h, cls, &rk, &ck, &ld);
+        const int64_t rows = lay_block_rows(h, rk, irrep), cols = lay_block_rows(h, ck, irrep);
+        if (col0 < 0 || ncols < 0 || col0 + ncols > cols) {
+            h->err = "rcc_set_ints: column batch outside the irrep block";
+            return -1;
+        }
+        const double2* dsrc = (const double2*)buf;
+        if (kind == hipMemcpyHostToDevice) {
+            CK(ensure_stage(h, rows * ncols));
+            HIPCHK(h, hipMemcpyAsync(h->stage, buf, (size_t)(rows * ncols) * 16, kind, h->st));
+            dsrc = h->stage;
+        }
+        const SymInfo& S = h->sym;
+        k_scatter_block(dsrc, rows, ncols, S.dmap + S.doff[rk][irrep],
+                        S.dmap + S.doff[ck][irrep] + col0, h->cls[cls], ld, h->st);
+        HIPCHK(h, hipStreamSynchronize(h->st));
     }
-    const int64_t rows = h->cls_rows[cls];
-    CK(upload(h, h->cls[cls] + col0 * rows, buf, rows * ncols, kind));
     h->derived_ok = false;
     h->split_const_ok = false;
     return 0;
@@ -1008,6 +1240,12 @@ int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncol
 
 int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
     if (!h) return -1;
+    if (h->sym.nrep > 1) {
+        if (t1) CK(sym_to_c1(h, RCC_LAY_T1, t1, hipMemcpyHostToDevice, h->T1(), h->nt1));
+        if (t2) CK(sym_to_c1(h, RCC_LAY_T2, t2, hipMemcpyHostToDevice, h->T2(), h->nt2));
+        invalidate_t(h);
+        return 0;
+    }
     if (t1) CK(upload(h, h->T1(), t1, h->nt1, hipMemcpyHostToDevice));
     if (t2) CK(upload(h, h->T2(), t2, h->nt2, hipMemcpyHostToDevice));
     invalidate_t(h);
@@ -1017,6 +1255,11 @@ int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
 int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2) {
     if (!h) return -1;
     HIPCHK(h, hipStreamSynchronize(h->st));
+    if (h->sym.nrep > 1) {
+        if (t1) CK(c1_to_sym_host(h, RCC_LAY_T1, h->T1(), t1));
+        if (t2) CK(c1_to_sym_host(h, RCC_LAY_T2, h->T2(), t2));
+        return 0;
+    }
     if (t1) HIPCHK(h, hipMemcpy(t1, h->T1(), h->nt1 * 16, hipMemcpyDeviceToHost));
     if (t2) HIPCHK(h, hipMemcpy(t2, h->T2(), h->nt2 * 16, hipMemcpyDeviceToHost));
     return 0;
@@ -1042,12 +1285,48 @@ int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
         case RCC_ARR_TAU: src = h->TAU; len = h->nt2; break;
         default: h->err = "rcc_get_array: unknown array"; return -1;
     }
+    if (h->sym.nrep > 1) {
+        int lay = RCC_LAY_T2;
+        switch (which) {
+            case RCC_ARR_HO: case RCC_ARR_GO: lay = RCC_LAY_FOO; break;
+            case RCC_ARR_HV: case RCC_ARR_GV: lay = RCC_LAY_FVV; break;
+            case RCC_ARR_HOV: case RCC_ARR_S1: case RCC_ARR_T1: lay = RCC_LAY_T1; break;
+            case RCC_ARR_AZ: lay = RCC_OOOO; break;
+            case RCC_ARR_H: lay = RCC_VOVO; break;
+            default: break;
+        }
+        int rk, ck;
+        int64_t ld;
+        lay_spec(h, lay, &rk, &ck, &ld);
+        len = lay_offset(h, rk, ck, h->sym.nrep);
+        if (n != len) {
+            h->err = "rcc_get_array: size mismatch (expected " + std::to_string(len) + ")";
+            return -1;
+        }
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        return c1_to_sym_host(h, lay, src, out);
+    }
     if (n != len) {
         h->err = "rcc_get_array: size mismatch (expected " + std::to_string(len) + ")";
         return -1;
     }
     HIPCHK(h, hipStreamSynchronize(h->st));
     HIPCHK(h, hipMemcpy(out, src, len * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rcc_layout_block(rcc_t* h, int lay, int block, int64_t* rows, int64_t* cols,
+                     int64_t* offset) {
+    if (!h) return -1;
+    int rk, ck;
+    int64_t ld;
+    if (!lay_spec(h, lay, &rk, &ck, &ld) || block < 0 || block > h->sym.nrep) {
+        h->err = "rcc_layout_block: unknown layout or block";
+        return -1;
+    }
+    if (rows) *rows = block < h->sym.nrep ? lay_block_rows(h, rk, block) : 0;
+    if (cols) *cols = block < h->sym.nrep ? lay_block_rows(h, ck, block) : 0;
+    if (offset) *offset = lay_offset(h, rk, ck, block);
     return 0;
 }
 

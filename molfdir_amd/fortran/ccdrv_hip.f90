@@ -36,6 +36,7 @@ program ccdrv_hip
   sym%no = c_loc(nos)
   sym%nv = c_loc(nvs)
   sym%frozen = c_null_ptr
+  sym%multb = c_null_ptr
   rc = rcc_create(sym, h)
   call rcc_check(h, rc, 'rcc_create')
   call rcc_check(h, rcc_set_options(h, maxdim, maxit, ntol, 0_c_int), 'rcc_set_options')

@@ -17,6 +17,7 @@ module relccsd_hip
     type(c_ptr) :: no
     type(c_ptr) :: nv
     type(c_ptr) :: frozen
+    type(c_ptr) :: multb
   end type rcc_sym_t
 
   interface
@@ -56,6 +57,13 @@ module relccsd_hip
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: t1(*), t2(*)
+    end function
+    integer(c_int) function rcc_layout_block(h, lay, block, rows, cols, offset) &
+        bind(c, name="rcc_layout_block")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: lay, block
+      integer(c_int64_t), intent(out) :: rows, cols, offset
     end function
     integer(c_int) function rcc_mp2(h, emp2) bind(c, name="rcc_mp2")
       import :: c_int, c_ptr, c_double

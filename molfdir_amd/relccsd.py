@@ -37,7 +37,22 @@ class RccError(RuntimeError):
 
 class _Sym(ctypes.Structure):
     _fields_ = [("nrep", ctypes.c_int), ("no", ctypes.POINTER(ctypes.c_int)),
-                ("nv", ctypes.POINTER(ctypes.c_int)), ("frozen", ctypes.POINTER(ctypes.c_int))]
+                ("nv", ctypes.POINTER(ctypes.c_int)), ("frozen", ctypes.POINTER(ctypes.c_int)),
+                ("multb", ctypes.POINTER(ctypes.c_int))]
+
+# layout ids of rcc_layout_block
+LAYOUTS = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5,
+           "T1": 6, "T2": 7, "FOO": 8, "FVV": 9}
+# layout of each readable array
+ARRAY_LAYOUT = {"HO": "FOO", "GO": "FOO", "HV": "FVV", "GV": "FVV", "HOV": "T1", "S1": "T1",
+                "T1": "T1", "AZ": "OOOO", "H": "VOVO", "S2": "T2", "T2": "T2", "TAU": "T2"}
+
+
+def xor_multb(nrep: int) -> np.ndarray:
+    """Pair-irrep table (1-based, column-major) of the abelian double group
+    Z2^k x {fermion, boson} with nrep = 2^k fermion irreps."""
+    g = np.arange(nrep)
+    return (np.bitwise_xor.outer(g, g) + 1).astype(np.int32)
 
 
 def declared_symbols() -> list[str]:
@@ -80,6 +95,8 @@ def load():
     lib.rcc_kernel_times.argtypes = [vp, dp]
     lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_probe_mfma.argtypes = [vp, dp]
+    lib.rcc_layout_block.argtypes = [vp, ci, ci, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                     ctypes.POINTER(i64)]
     lib.rcc_zgemm_split_device.argtypes = [vp, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_zgemm_i8_device.argtypes = [vp, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_set_split_digits.argtypes = [vp, ci]
@@ -100,22 +117,36 @@ def _ptr(a: np.ndarray):
 
 
 class RelCCSD:
-    """One C1 relativistic CCSD problem resident on the current GPU."""
+    """One relativistic CCSD problem resident on the current GPU.
+
+    sym = (occupied per irrep, virtual per irrep) selects an abelian double
+    group with that many fermion irreps (pair-irrep table `multb`, default the
+    Z2^k table); every array of the interface is then in relccsd's
+    symmetry-blocked layout, and `no` / `nv` are the totals."""
 
     def __init__(self, no: int, nv: int, frozen=None, maxdim: int = 8, maxit: int = 30,
-                 ntol: int = 12, precision: int = 0):
+                 ntol: int = 12, precision: int = 0, sym=None, multb=None):
         self.lib = load()
         self.no, self.nv = int(no), int(nv)
         self.noot, self.nvvt = self.no * (self.no - 1) // 2, self.nv * (self.nv - 1) // 2
         self.nt1, self.nt2 = self.no * self.nv, self.noot * self.nvvt
-        self._no = (ctypes.c_int * 1)(self.no)
-        self._nv = (ctypes.c_int * 1)(self.nv)
+        no_l, nv_l = ([self.no], [self.nv]) if sym is None else (list(sym[0]), list(sym[1]))
+        if sum(no_l) != self.no or sum(nv_l) != self.nv or len(no_l) != len(nv_l):
+            raise RccError("sym counts must sum to no, nv")
+        self.nrep = len(no_l)
+        self._no = (ctypes.c_int * self.nrep)(*no_l)
+        self._nv = (ctypes.c_int * self.nrep)(*nv_l)
         fr = None
         if frozen is not None:
             self._fr = (ctypes.c_int * max(1, self.no))(*[int(bool(x)) for x in frozen])
             fr = ctypes.cast(self._fr, ctypes.POINTER(ctypes.c_int))
-        sym = _Sym(1, ctypes.cast(self._no, ctypes.POINTER(ctypes.c_int)),
-                   ctypes.cast(self._nv, ctypes.POINTER(ctypes.c_int)), fr)
+        mt = None
+        if self.nrep > 1:
+            tab = xor_multb(self.nrep) if multb is None else np.asarray(multb, dtype=np.int32)
+            self._mt = (ctypes.c_int * (self.nrep ** 2))(*tab.T.reshape(-1).tolist())
+            mt = ctypes.cast(self._mt, ctypes.POINTER(ctypes.c_int))
+        sym = _Sym(self.nrep, ctypes.cast(self._no, ctypes.POINTER(ctypes.c_int)),
+                   ctypes.cast(self._nv, ctypes.POINTER(ctypes.c_int)), fr, mt)
         h = ctypes.c_void_p()
         rc = self.lib.rcc_create(ctypes.byref(sym), ctypes.byref(h))
         self.h = h
@@ -124,6 +155,19 @@ class RelCCSD:
             self.close()
             raise RccError(msg)
         self.set_options(maxdim, maxit, ntol, precision)
+        if self.nrep > 1:
+            self.nt1, self.nt2 = self.layout_size("T1"), self.layout_size("T2")
+
+    # ------------------------------------------------------------ layouts
+    def layout_block(self, name, block):
+        """(rows, cols, offset) of block `block` of a layout (SYMTAB offsets)."""
+        r, c, o = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.rcc_layout_block(self.h, LAYOUTS[name], block, ctypes.byref(r),
+                                              ctypes.byref(c), ctypes.byref(o)))
+        return r.value, c.value, o.value
+
+    def layout_size(self, name):
+        return self.layout_block(name, self.nrep)[2]
 
     # ------------------------------------------------------------ plumbing
     def _check(self, rc):
@@ -158,10 +202,27 @@ class RelCCSD:
         e = np.ascontiguousarray(_host(eps), dtype=np.float64)
         self._check(self.lib.rcc_set_fock(self.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(e)))
 
-    def set_ints(self, name, data, col0=0, ncols=None):
+    def set_ints(self, name, data, col0=0, ncols=None, irrep=None):
+        """A column batch of one class block (irrep: compound irrep, 0-based);
+        with symmetry and irrep=None, the whole class in symmetry layout."""
         cls = CLASSES[name]
-        rows = {"OOOO": self.noot, "VOOO": self.nt1, "VVOO": self.nvvt, "VOVO": self.nt1,
-                "VOVV": self.nt1, "VVVV": self.nvvt}[name]
+        if self.nrep > 1:
+            if irrep is None:
+                flat = data.reshape(-1) if hasattr(data, "reshape") else data
+                for b in range(self.nrep):
+                    r, c, o = self.layout_block(name, b)
+                    if r * c:
+                        self.set_ints(name, flat[o:o + r * c], 0, c, irrep=b)
+                return
+            r, c, _ = self.layout_block(name, irrep)
+            rows = r
+        else:
+            if irrep not in (None, 0):
+                raise RccError("C1 problem: irrep must be 0")
+            irrep = 0
+        if self.nrep == 1:
+            rows = {"OOOO": self.noot, "VOOO": self.nt1, "VVOO": self.nvvt, "VOVO": self.nt1,
+                    "VOVV": self.nt1, "VVVV": self.nvvt}[name]
         if hasattr(data, "is_cuda") and data.is_cuda:
             import torch
             torch.cuda.synchronize()     # the producer ran on torch's stream
@@ -169,17 +230,18 @@ class RelCCSD:
             ncols = n // max(rows, 1) if ncols is None else ncols
             if n != rows * ncols or data.dtype.__str__() != "torch.complex128" or not data.is_contiguous():
                 raise RccError(f"{name}: need a contiguous complex128 tensor of {rows}x{ncols}")
-            self._check(self.lib.rcc_set_ints_device(self.h, cls, 0, col0, ncols,
+            self._check(self.lib.rcc_set_ints_device(self.h, cls, irrep, col0, ncols,
                                                      ctypes.c_void_p(data.data_ptr())))
             return
         a = np.ascontiguousarray(_host(data), dtype=np.complex128).reshape(-1)
         ncols = a.size // max(rows, 1) if ncols is None else ncols
         if a.size != rows * ncols:
             raise RccError(f"{name}: {a.size} elements is not {rows} x {ncols}")
-        self._check(self.lib.rcc_set_ints(self.h, cls, 0, col0, ncols, _ptr(a)))
+        self._check(self.lib.rcc_set_ints(self.h, cls, irrep, col0, ncols, _ptr(a)))
 
     def set_system(self, sysd):
-        """Uploads a system dict in relccsd layout (molfdir_amd.synth)."""
+        """Uploads a system dict in the handle's relccsd layout (C1 arrays from
+        molfdir_amd.synth, or symmetry-blocked arrays for a symmetric handle)."""
         self.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
         for name in CLASSES:
             self.set_ints(name, sysd[name])
@@ -197,9 +259,12 @@ class RelCCSD:
         return a, b
 
     def get(self, name):
-        n = {"HO": self.no ** 2, "HV": self.nv ** 2, "HOV": self.nt1, "GO": self.no ** 2,
-             "GV": self.nv ** 2, "AZ": self.noot ** 2, "H": self.nt1 ** 2, "S1": self.nt1,
-             "S2": self.nt2, "T1": self.nt1, "T2": self.nt2, "TAU": self.nt2}[name]
+        if self.nrep > 1:
+            n = self.layout_size(ARRAY_LAYOUT[name])
+        else:
+            n = {"HO": self.no ** 2, "HV": self.nv ** 2, "HOV": self.nt1, "GO": self.no ** 2,
+                 "GV": self.nv ** 2, "AZ": self.noot ** 2, "H": self.nt1 ** 2, "S1": self.nt1,
+                 "S2": self.nt2, "T1": self.nt1, "T2": self.nt2, "TAU": self.nt2}[name]
         out = np.empty(n, np.complex128)
         self._check(self.lib.rcc_get_array(self.h, ARRAYS[name], _ptr(out), n))
         return out

@@ -35,7 +35,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-__all__ = ["pair_index", "draw_factors", "make_system", "class_shapes"]
+__all__ = ["pair_index", "draw_factors", "make_system", "class_shapes", "spinor_irreps"]
 
 
 def pair_index(n: int) -> tuple[np.ndarray, np.ndarray]:
@@ -118,15 +118,42 @@ def _build_class(B, prow, qrow, rcol, scol, rows_packed, device, kblock):
     return out.reshape(-1)
 
 
+def spinor_irreps(sym) -> np.ndarray:
+    """Fermion irrep of every spinor in global order (occupied irrep-major,
+    then virtual irrep-major) for sym = (occupied per irrep, virtual per
+    irrep)."""
+    no_l, nv_l = sym
+    return np.concatenate([np.repeat(np.arange(len(no_l)), no_l),
+                           np.repeat(np.arange(len(nv_l)), nv_l)])
+
+
 def make_system(no: int, nv: int, seed: int, rank: int = 120,
                 scale: float = 0.05, device="cpu", fock_perturb: float = 0.0,
                 classes=("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"),
-                kblock: int | None = None) -> dict:
+                kblock: int | None = None, sym=None) -> dict:
     """Build one seeded system.  Returns a dict with 'eps' (float64, occupied
     then virtual), the Fock blocks and the requested integral classes as flat
-    complex128 tensors on ``device`` in relccsd layout."""
+    complex128 tensors on ``device`` in relccsd's C1 layout.
+
+    sym = (occupied per irrep, virtual per irrep) gives the system the
+    symmetry of the abelian double group Z2^k x {fermion, boson} (2^k
+    fermion irreps, products adding the Z2^k labels): factor component P
+    carries irrep P mod 2^k and B[P,p,r] is kept only where the pair irrep of
+    (p, r) equals it, so <pq||rs> vanishes unless (pq) and (rs) share a pair
+    irrep; a Fock perturbation is kept only inside irreps.  Spinors are then
+    in global order (irrep-major within occupied and within virtual)."""
     eps, b = draw_factors(no, nv, seed, rank, scale)
     n = no + nv
+    lab = None
+    if sym is not None:
+        no_l, nv_l = (list(map(int, x)) for x in sym)
+        nrep = len(no_l)
+        if len(nv_l) != nrep or nrep & (nrep - 1) or sum(no_l) != no or sum(nv_l) != nv:
+            raise ValueError("sym must give 2^k irreps whose counts sum to no, nv")
+        lab = spinor_irreps((no_l, nv_l))
+        pair = np.bitwise_xor.outer(lab, lab)                    # pair irrep of (p, r)
+        keep = pair[None, :, :] == (np.arange(rank) % nrep)[:, None, None]
+        b = b * keep
     dev = torch.device(device)
     B = torch.from_numpy(b).to(dev)
     f = np.diag(eps).astype(np.complex128)
@@ -134,9 +161,13 @@ def make_system(no: int, nv: int, seed: int, rank: int = 120,
         rng = np.random.Generator(np.random.PCG64(seed + 7919))
         v = rng.normal(0.0, fock_perturb, (n, n)) + 1j * rng.normal(0.0, fock_perturb, (n, n))
         v = 0.5 * (v + v.conj().T)
+        if lab is not None:
+            v = v * (lab[:, None] == lab[None, :])
         f = f + v
         eps = np.real(np.diag(f)).copy()
     out = {"no": no, "nv": nv, "eps": torch.from_numpy(eps).to(dev)}
+    if sym is not None:
+        out["sym"] = (no_l, nv_l)
     o = np.arange(no)
     vv = np.arange(no, n)
     out["FOO"] = torch.from_numpy(f[np.ix_(o, o)].T.reshape(-1).copy()).to(dev)

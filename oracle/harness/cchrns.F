@@ -44,8 +44,10 @@ C
      &                       HO(:),HV(:),HOV(:),GOO(:),GVV(:),
      &                       BUF1(:),BUF2(:),BUF3(:)
       REAL*8 DUM(4),ECC
-      CHARACTER*4 RNAM(2)
-      INTEGER MULT(2,2)
+      CHARACTER*4 RNAM(32)
+      CHARACTER*14 SNAM(16)
+      INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16)
+      CHARACTER*200 LINE
       LOGICAL BRT
 C
       MASTER = 0
@@ -55,7 +57,10 @@ C
       IW = 6
       INTERFACE = 'MOLFDIR   '
 C
-C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, unused, unused
+C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, NREP (0 = C1), unused.
+C     For NREP > 1 the orbital energies are followed by the fermion irrep
+C     (1..NREP) of every spinor, the occupied count per irrep and the
+C     2NREP x 2NREP table MULTB(.,.,1), all int32.
 C
       OPEN (30,FILE='ccin.bin',ACCESS='STREAM',FORM='UNFORMATTED',
      &      STATUS='OLD')
@@ -63,33 +68,48 @@ C
       NOCC  = HDR(1)
       NVIR  = HDR(2)
       MODE  = HDR(3)
+      NREPI = MAX0(1,HDR(7))
       NSP   = NOCC + NVIR
       READ (30) (EPS1(I),I=1,NSP)
+      IF (NREPI.GT.1) THEN
+         READ (30) (IRRSP(I),I=1,NSP)
+         READ (30) (NELRP(I),I=1,NREPI)
+         READ (30) ((MULT(I,J),I=1,2*NREPI),J=1,2*NREPI)
+      ELSE
+         DO I = 1, NSP
+            IRRSP(I) = 1
+         ENDDO
+         NELRP(1) = NOCC
+         MULT(1,1) = 2
+         MULT(2,1) = 1
+         MULT(1,2) = 1
+         MULT(2,2) = 2
+      ENDIF
 C
 C     MRCONEE records 1-5 exactly as RDSYMI reads them (ccdriv.F:2978-
-C     2988): C1 double group, one fermion irrep and one boson irrep.
+C     2988); the higher point group is taken equal to the abelian one.
 C
       OPEN (MRCONEE,FILE='MRCONEE',FORM='UNFORMATTED',
      &      STATUS='REPLACE')
       BRT = .FALSE.
+      DO I = 1, NREPI
+         WRITE (SNAM(I),'(A1,I2.2,11X)') 'A',I
+         WRITE (RNAM(I),'(1X,A1,I2.2)') 'A',I
+         WRITE (RNAM(NREPI+I),'(1X,A1,I2.2)') 'a',I
+      ENDDO
       WRITE (MRCONEE) NSP,BRT,0.0D0
-      WRITE (MRCONEE) 1,'A             ',NOCC
-      RNAM(1) = '   A'
-      RNAM(2) = '   a'
-      WRITE (MRCONEE) 1,RNAM(1),RNAM(2)
-      MULT(1,1) = 2
-      MULT(2,1) = 1
-      MULT(1,2) = 1
-      MULT(2,2) = 2
-      WRITE (MRCONEE) ((MULT(I,J),I=1,2),J=1,2)
-      WRITE (MRCONEE) (1,1,EPS1(I),I=1,NSP)
+      WRITE (MRCONEE) NREPI,(SNAM(I),I=1,NREPI),(NELRP(I),I=1,NREPI)
+      WRITE (MRCONEE) NREPI,(RNAM(I),I=1,2*NREPI)
+      WRITE (MRCONEE) ((MULT(I,J),I=1,2*NREPI),J=1,2*NREPI)
+      WRITE (MRCONEE) (IRRSP(I),IRRSP(I),EPS1(I),I=1,NSP)
       CLOSE (MRCONEE)
 C
 C     Namelist input read by CCINPT (ccdriv.F:126-373)
 C
       OPEN (31,FILE='MOLFDIR.INP',FORM='FORMATTED',STATUS='REPLACE')
-      WRITE (31,'(A,I4,A)') ' &RELCCSD NELEC=',NOCC,
-     &   ', CARITH=T, DOSORT=F, DOENER=T, IPRNT=1, TIMING=T /'
+      WRITE (LINE,'(16(I4,A1))') (NELRP(I),',',I=1,NREPI)
+      WRITE (31,'(A,A,A)') ' &RELCCSD NELEC=',TRIM(LINE),
+     &   ' CARITH=T, DOSORT=F, DOENER=T, IPRNT=1, TIMING=T /'
       WRITE (31,'(A,I4,A,I4,A,I4,A)') ' &CCENER DOMP2=T, DOCCSD=T,'//
      &   ' DOCCSDT=F, MAXIT=',HDR(4),', MAXDIM=',HDR(5),
      &   ', NTOL=',HDR(6),' /'

@@ -35,19 +35,40 @@ def _c(x):
     return np.ascontiguousarray(x.astype(np.complex128).reshape(-1))
 
 
+def layout_of(sysd):
+    """The symmetry layout of a system: sysd['sym'] = (no per irrep, nv per
+    irrep) for an abelian double group of xor_table type, else C1."""
+    from oracle.symtab import SymLayout
+    if "sym" in sysd and sysd["sym"] is not None:
+        no_l, nv_l = sysd["sym"]
+        return SymLayout(no_l, nv_l)
+    return SymLayout([sysd["no"]], [sysd["nv"]])
+
+
 def _write_input(path, sysd, mode, maxit, maxdim, ntol, t1=None, t2=None):
+    """ccin.bin for harness/cchrns.F.  Classes (and t1/t2) are given in C1
+    layout and written in the reference's symmetry layout."""
+    from oracle.symtab import multb_full
+    lay = layout_of(sysd)
     no, nv = sysd["no"], sysd["nv"]
-    hdr = np.array([no, nv, mode, maxit, maxdim, ntol, 0, 0], dtype=np.int32)
+    nrep = lay.nrep
+    hdr = np.array([no, nv, mode, maxit, maxdim, ntol, nrep if nrep > 1 else 0, 0],
+                   dtype=np.int32)
     eps = np.asarray(sysd["eps"].cpu().numpy() if hasattr(sysd["eps"], "cpu") else sysd["eps"],
                      dtype=np.float64)
     with open(path, "wb") as f:
         f.write(hdr.tobytes())
         f.write(eps.tobytes())
+        if nrep > 1:
+            irr = np.concatenate([lay.occ_irreps(), lay.vir_irreps()]) + 1
+            f.write(irr.astype(np.int32).tobytes())
+            f.write(lay.no.astype(np.int32).tobytes())
+            f.write(multb_full(nrep).T.astype(np.int32).tobytes())   # column-major
         for name in CLASS_ORDER:
-            f.write(_c(sysd[name]).tobytes())
+            f.write(_c(lay.to_sym(name, _c(sysd[name]))).tobytes())
         if mode == 2:
-            f.write(_c(t1).tobytes())
-            f.write(_c(t2).tobytes())
+            f.write(_c(lay.to_sym("T1", _c(t1))).tobytes())
+            f.write(_c(lay.to_sym("T2", _c(t2))).tobytes())
 
 
 _ITER_RE = re.compile(r"^\s+(\d+)\s+(-?[\d.]+(?:[EeDd][-+]?\d+)?)\s+(-?[\d.]+(?:[EeDd][-+]?\d+)?)\s+(-?[\d.]+)\s*$")
@@ -73,9 +94,10 @@ def parse_iterations(log: str):
 
 def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
         workdir=None, timeout=3600):
-    no, nv = sysd["no"], sysd["nv"]
-    noot, nvvt = no * (no - 1) // 2, nv * (nv - 1) // 2
-    nt1, nt2 = nv * no, nvvt * noot
+    """Outputs are in the reference's (symmetry) layout; for a C1 system
+    that is the C1 layout."""
+    lay = layout_of(sysd)
+    nt1, nt2 = lay.size("T1"), lay.size("T2")
     m = 1 if mode == "ccsd" else 2
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         _write_input(os.path.join(d, "ccin.bin"), sysd, m, maxit, maxdim, ntol, t1, t2)
@@ -93,8 +115,8 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
         mt = re.search(r"Total time in CCEQN :\s+([\d.]+)", log)
         out["cceqn_cpu_s"] = float(mt.group(1)) if mt else None
         return out
-    nfoo, nfvo, nfvv = no * no, nv * no, nv * nv
-    nv1, nv4 = noot * noot, nt1 * nt1
+    nfoo, nfvo, nfvv = lay.size("FOO"), lay.size("FVO"), lay.size("FVV")
+    nv1, nv4 = lay.size("OOOO"), lay.size("VOVO")
     sizes = [("HO", nfoo), ("HV", nfvv), ("HOV", nfvo), ("S1", nt1), ("GO", nfoo),
              ("GV", nfvv), ("AZ", nv1), ("H", nv4), ("S2_T2EQNS", nt2),
              ("S2_BINTM", nt2), ("S1_DENOM", nt1), ("S2_DENOM", nt2)]

@@ -23,8 +23,33 @@ def load(kind, name):
 def system_for(g, device="cpu"):
     p = g["params"]
     no, nv, seed = int(p[0]), int(p[1]), int(p[2])
+    sym = None
+    if "sym_no" in g:
+        sym = (g["sym_no"].tolist(), g["sym_nv"].tolist())
     return synth.make_system(no, nv, seed=seed, scale=float(g["scale"]),
-                             fock_perturb=float(g["fock_perturb"]), device=device)
+                             fock_perturb=float(g["fock_perturb"]), device=device, sym=sym)
+
+
+def layout_for(g):
+    """oracle.symtab layout of a symmetric fixture."""
+    from oracle.symtab import SymLayout
+    return SymLayout(g["sym_no"].tolist(), g["sym_nv"].tolist())
+
+
+# layout of each per-term output
+TERM_LAYOUT = {"HO": "FOO", "GO": "FOO", "HV": "FVV", "GV": "FVV", "HOV": "T1", "S1": "T1",
+               "S1_DENOM": "T1", "AZ": "OOOO", "H": "VOVO", "S2_T2EQNS": "T2",
+               "S2_BINTM": "T2", "S2_DENOM": "T2"}
+
+
+def symmetric_amplitudes(lay, seed):
+    """Random T1/T2 with the layout's symmetry, in C1 layout (as the
+    symmetric fixtures were made)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n1, n2 = lay.size("T1"), lay.size("T2")
+    t1 = 0.05 * (rng.normal(size=n1) + 1j * rng.normal(size=n1))
+    t2 = 0.05 * (rng.normal(size=n2) + 1j * rng.normal(size=n2))
+    return lay.to_c1("T1", t1), lay.to_c1("T2", t2)
 
 
 def fingerprint(sysd):

@@ -14,6 +14,10 @@ change in the generator is caught.
   MAXIT / MAXDIM / NTOL.
 * terms_<name>.npz: every CCEQN intermediate evaluated once on seeded random
   T1/T2 (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before/after BINTM, DENOM, CCENG).
+* symccsd_<name>.npz / symterms_<name>.npz: the same for systems with the
+  symmetry of an abelian double group (2, 4 or 8 fermion irreps, Z2^k
+  products); every array is in relccsd's symmetry-blocked layout (SYMTAB,
+  ccdriv.F:746-1425), the per-irrep occupied / virtual counts are stored.
 """
 import os
 import sys
@@ -25,6 +29,7 @@ sys.path.insert(0, os.path.abspath(os.path.join(HERE, "..", "..")))
 
 from molfdir_amd import synth  # noqa: E402
 from oracle import refrun  # noqa: E402
+from oracle.symtab import SymLayout  # noqa: E402
 
 CCSD_CASES = [
     # name, no, nv, seed, scale, fock_perturb, maxit, maxdim, ntol
@@ -39,6 +44,27 @@ TERM_CASES = [
     ("t_5o14v", 5, 14, 4, 0.05, 0.05, 12),
     ("t_2o5v", 2, 5, 6, 0.05, 0.05, 13),
 ]
+
+
+SYM_CCSD_CASES = [
+    # name, (no per irrep), (nv per irrep), seed, scale, fock_perturb, maxit, maxdim, ntol
+    ("s2_4o12v", (2, 2), (7, 5), 21, 0.08, 0.0, 30, 6, 11),
+    ("s4_5o12v", (2, 1, 1, 1), (4, 3, 3, 2), 22, 0.1, 0.02, 30, 6, 11),
+    ("s8_8o24v", (2, 0, 1, 1, 2, 0, 1, 1), (4, 2, 3, 3, 4, 2, 3, 3), 23, 0.1, 0.0, 30, 6, 11),
+]
+SYM_TERM_CASES = [
+    ("s4_t_5o12v", (2, 1, 1, 1), (4, 3, 3, 2), 24, 0.1, 0.03, 14),
+    ("s8_t_8o24v", (2, 0, 1, 1, 2, 0, 1, 1), (4, 2, 3, 3, 4, 2, 3, 3), 25, 0.1, 0.03, 15),
+]
+
+
+def symmetric_amplitudes(lay, seed):
+    """Random T1/T2 with the symmetry of the layout, returned in C1 layout."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n1, n2 = lay.size("T1"), lay.size("T2")
+    t1 = 0.05 * (rng.normal(size=n1) + 1j * rng.normal(size=n1))
+    t2 = 0.05 * (rng.normal(size=n2) + 1j * rng.normal(size=n2))
+    return lay.to_c1("T1", t1), lay.to_c1("T2", t2)
 
 
 def fingerprint(sysd):
@@ -61,7 +87,8 @@ def random_amplitudes(no, nv, seed):
 def main():
     if not refrun.available():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
-    for name, no, nv, seed, scale, fp, maxit, maxdim, ntol in CCSD_CASES:
+    only_sym = "--sym-only" in sys.argv
+    for name, no, nv, seed, scale, fp, maxit, maxdim, ntol in ([] if only_sym else CCSD_CASES):
         s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
         r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol)
         it = np.asarray(r["iterations"], dtype=np.float64)
@@ -72,7 +99,7 @@ def main():
             emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
             T1=r["T1"], T2=r["T2"])
         print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
-    for name, no, nv, seed, scale, fp, tseed in TERM_CASES:
+    for name, no, nv, seed, scale, fp, tseed in ([] if only_sym else TERM_CASES):
         s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
         t1, t2 = random_amplitudes(no, nv, tseed)
         r = refrun.run(s, "terms", t1=t1, t2=t2)
@@ -81,6 +108,33 @@ def main():
         np.savez_compressed(
             os.path.join(HERE, f"terms_{name}.npz"),
             params=np.array([no, nv, seed, tseed], dtype=np.int64),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            ECC=r["ECC"], **{k: r[k] for k in keys})
+        print(name, r["ECC"])
+    for name, sno, snv, seed, scale, fp, maxit, maxdim, ntol in SYM_CCSD_CASES:
+        no, nv = sum(sno), sum(snv)
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=(sno, snv))
+        r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol)
+        it = np.asarray(r["iterations"], dtype=np.float64)
+        np.savez_compressed(
+            os.path.join(HERE, f"symccsd_{name}.npz"),
+            params=np.array([no, nv, seed, maxit, maxdim, ntol], dtype=np.int64),
+            sym_no=np.array(sno), sym_nv=np.array(snv),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
+            T1=r["T1"], T2=r["T2"])
+        print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
+    for name, sno, snv, seed, scale, fp, tseed in SYM_TERM_CASES:
+        no, nv = sum(sno), sum(snv)
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=(sno, snv))
+        t1, t2 = symmetric_amplitudes(SymLayout(sno, snv), tseed)
+        r = refrun.run(s, "terms", t1=t1, t2=t2)
+        keys = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
+                "S2_BINTM", "S1_DENOM", "S2_DENOM"]
+        np.savez_compressed(
+            os.path.join(HERE, f"symterms_{name}.npz"),
+            params=np.array([no, nv, seed, tseed], dtype=np.int64),
+            sym_no=np.array(sno), sym_nv=np.array(snv),
             scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
             ECC=r["ECC"], **{k: r[k] for k in keys})
         print(name, r["ECC"])

@@ -1,0 +1,112 @@
+"""GPU parity of the symmetry-blocked interface (nrep > 1) against the
+relccsd reference run with 2, 4 and 8 irreps (symmetric fixtures).
+
+Every array goes through the C ABI in relccsd's symmetry-blocked layout
+(SYMTAB, ccdriv.F:746-1425); the library scatters it into its C1 working
+layout and gathers results back.  Bars as for C1: per-term intermediates to
+1e-12 relative, trajectories to 1e-10 Eh and amplitudes to 1e-9.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from molfdir_amd.relccsd import RelCCSD  # noqa: E402
+
+from _golden import (load, names, system_for, layout_for, symmetric_amplitudes,  # noqa: E402
+                     TERM_LAYOUT)
+
+CLASSES = ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+TERM_ORDER = [("hointm", "HO"), ("hvintm", "HV"), ("hovintm", "HOV"), ("t1eqns", "S1"),
+              ("gointm", "GO"), ("gvintm", "GV"), ("aintm", "AZ"), ("hintm", "H"),
+              ("t2eqns", "S2_T2EQNS"), ("bintm", "S2_BINTM")]
+
+
+def sym_solver(g, device_ints=False, **kw):
+    s = system_for(g)
+    lay = layout_for(g)
+    r = RelCCSD(int(g["params"][0]), int(g["params"][1]),
+                sym=(g["sym_no"].tolist(), g["sym_nv"].tolist()), **kw)
+    r.set_fock(lay.to_sym("FOO", s["FOO"].numpy()), lay.to_sym("FVO", s["FVO"].numpy()),
+               lay.to_sym("FVV", s["FVV"].numpy()), s["eps"])
+    for name in CLASSES:
+        x = lay.to_sym(name, s[name].numpy())
+        if device_ints:
+            x = torch.from_numpy(x).to("cuda")
+        r.set_ints(name, x)
+    return r, lay, s
+
+
+@pytest.mark.parametrize("name", names("symccsd") + names("symterms"))
+def test_layout_blocks_match_symtab(name):
+    kind = "symccsd" if name in names("symccsd") else "symterms"
+    g = load(kind, name)
+    r, lay, _ = sym_solver(g)
+    for lname in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV", "T1", "T2", "FOO", "FVV"):
+        off = 0
+        for b, (rows, cols) in enumerate(lay.blocks(lname)):
+            assert r.layout_block(lname, b) == (len(rows), len(cols), off), (lname, b)
+            off += len(rows) * len(cols)
+        assert r.layout_size(lname) == off == lay.size(lname)
+
+
+@pytest.mark.parametrize("name", names("symterms"))
+def test_symmetric_terms_match_reference(name):
+    g = load("symterms", name)
+    r, lay, _ = sym_solver(g)
+    t1, t2 = symmetric_amplitudes(lay, int(g["params"][3]))
+    r.set_amplitudes(lay.to_sym("T1", t1), lay.to_sym("T2", t2))
+    for term, key in TERM_ORDER:
+        r.term(term)
+        arr = {"S2_T2EQNS": "S2", "S2_BINTM": "S2"}.get(key, key)
+        got, ref = r.get(arr), g[key]
+        assert got.shape == ref.shape, key
+        err = np.abs(got - ref).max()
+        assert err <= 1e-12 * max(1.0, np.abs(ref).max()), (term, key, err)
+    r.term("denom")
+    r.term("zcore")
+    np.testing.assert_allclose(r.get("S1"), g["S1_DENOM"], atol=1e-12, rtol=0)
+    np.testing.assert_allclose(r.get("S2"), g["S2_DENOM"], atol=1e-12, rtol=0)
+    assert abs(r.energy() - float(g["ECC"])) < 1e-12
+
+
+@pytest.mark.parametrize("device_ints", [False, True])
+@pytest.mark.parametrize("name", names("symccsd"))
+def test_symmetric_trajectory_matches_reference(name, device_ints):
+    g = load("symccsd", name)
+    maxit, maxdim, ntol = (int(x) for x in g["params"][3:6])
+    r, lay, _ = sym_solver(g, device_ints=device_ints, maxdim=maxdim, maxit=maxit, ntol=ntol)
+    out = r.cceqn()
+    ref_rows = g["iterations"][1:]
+    assert abs(out["emp2"] - float(g["emp2"])) < 1e-10
+    assert out["nit"] == int(g["nit"]) == len(ref_rows)
+    for (nit, e, rms), ref in zip(out["rows"], ref_rows):
+        assert abs(e - ref[1]) < 1e-10, (nit, e, ref[1])
+        assert abs(rms - ref[2]) < 1e-9 * max(1.0, ref[2]), (nit, rms, ref[2])
+    t1, t2 = r.get_amplitudes()
+    assert t1.shape == g["T1"].shape and t2.shape == g["T2"].shape
+    assert np.abs(t1 - g["T1"]).max() < 1e-9
+    assert np.abs(t2 - g["T2"]).max() < 1e-9
+
+
+def test_symmetric_column_batches_equal_whole_blocks():
+    # streaming a block in column batches (as GETVVVV does) gives the same
+    # handle state as uploading it whole
+    g = load("symccsd", names("symccsd")[-1])
+    r1, lay, s = sym_solver(g, maxit=3)
+    r2 = RelCCSD(int(g["params"][0]), int(g["params"][1]),
+                 sym=(g["sym_no"].tolist(), g["sym_nv"].tolist()), maxit=3)
+    r2.set_fock(lay.to_sym("FOO", s["FOO"].numpy()), lay.to_sym("FVO", s["FVO"].numpy()),
+                lay.to_sym("FVV", s["FVV"].numpy()), s["eps"])
+    for name in CLASSES:
+        x = lay.to_sym(name, s[name].numpy())
+        for b in range(r2.nrep):
+            rows, cols, off = r2.layout_block(name, b)
+            blk = x[off:off + rows * cols]
+            for c0 in range(0, cols, 3):
+                c1 = min(cols, c0 + 3)
+                r2.set_ints(name, blk[c0 * rows:c1 * rows], col0=c0, ncols=c1 - c0, irrep=b)
+    e1, e2 = r1.cceqn(), r2.cceqn()
+    assert [x[1] for x in e1["rows"]] == [x[1] for x in e2["rows"]]

@@ -15,14 +15,15 @@ from oracle import ccsd_oracle as O
 from oracle import diis as D
 from oracle import textbook as TB
 
-from _golden import load, names, system_for, fingerprint, random_amplitudes
+from _golden import (load, names, system_for, fingerprint, random_amplitudes, layout_for,
+                     symmetric_amplitudes, TERM_LAYOUT)
 
 TERM_KEYS = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
              "S2_BINTM", "S1_DENOM", "S2_DENOM"]
 
 
-@pytest.mark.parametrize("kind,name", [("ccsd", n) for n in names("ccsd")] +
-                         [("terms", n) for n in names("terms")])
+@pytest.mark.parametrize("kind,name", [(k, n) for k in ("ccsd", "terms", "symccsd", "symterms")
+                                       for n in names(k)])
 def test_generator_matches_fixture_inputs(kind, name):
     g = load(kind, name)
     s = system_for(g)
@@ -61,6 +62,68 @@ def test_trajectory_matches_reference(name):
         assert abs(rms - ref[2]) < 1e-10 * max(1.0, ref[2]), (nit, rms, ref[2])
     assert np.abs(r["T1"] - g["T1"]).max() < 1e-9
     assert np.abs(r["T2"] - g["T2"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", names("symterms"))
+def test_terms_match_symmetric_reference(name):
+    """Symmetric systems (2-8 irreps): the C1 oracle on the expanded system,
+    read out in relccsd's symmetry-blocked layout (oracle/symtab.py), against
+    the reference run with NREP > 1.  Pins the layout restatement."""
+    g = load("symterms", name)
+    tseed = int(g["params"][3])
+    s = system_for(g)
+    lay = layout_for(g)
+    t1, t2 = symmetric_amplitudes(lay, tseed)
+    o = O.terms(s, t1, t2)
+    for k in TERM_KEYS:
+        got, ref = lay.to_sym(TERM_LAYOUT[k], o[k]), g[k]
+        assert got.shape == ref.shape, k
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), k
+        # and nothing outside the symmetry blocks
+        assert np.array_equal(lay.to_c1(TERM_LAYOUT[k], got), o[k]), k
+    assert abs(o["ECC"] - float(g["ECC"])) < 1e-13
+
+
+@pytest.mark.parametrize("name", names("symccsd"))
+def test_trajectory_matches_symmetric_reference(name):
+    g = load("symccsd", name)
+    maxit, maxdim, ntol = (int(x) for x in g["params"][3:6])
+    s = system_for(g)
+    lay = layout_for(g)
+    r = O.cceqn(s, maxit=maxit, maxdim=maxdim, ntol=ntol)
+    ref_rows = g["iterations"][1:]
+    assert len(r["rows"]) == len(ref_rows) == int(g["nit"])
+    assert abs(r["emp2"] - float(g["emp2"])) < 1e-12
+    for (nit, ec, rms), ref in zip(r["rows"], ref_rows):
+        assert abs(ec - ref[1]) < 1e-10, (nit, ec, ref[1])
+        assert abs(rms - ref[2]) < 1e-10 * max(1.0, ref[2]), (nit, rms, ref[2])
+    assert np.abs(lay.to_sym("T1", r["T1"]) - g["T1"]).max() < 1e-9
+    assert np.abs(lay.to_sym("T2", r["T2"]) - g["T2"]).max() < 1e-9
+
+
+def test_symmetry_layout_sizes_and_round_trip():
+    """SYMTAB block sizes against their closed forms (ccdriv.F:1050-1125)
+    and exact round trips, for a group with an empty irrep."""
+    from oracle.symtab import SymLayout, xor_table
+    no, nv = [2, 0, 1, 3], [4, 2, 0, 5]
+    lay = SymLayout(no, nv)
+    P = xor_table(4)
+    for c in range(4):
+        noot = sum(no[j] * (no[j] - 1) // 2 for j in range(4) if P[j, j] == c) + \
+            sum(no[i] * no[j] for j in range(4) for i in range(j + 1, 4) if P[i, j] == c)
+        nvvt = sum(nv[j] * (nv[j] - 1) // 2 for j in range(4) if P[j, j] == c) + \
+            sum(nv[i] * nv[j] for j in range(4) for i in range(j + 1, 4) if P[i, j] == c)
+        nvo = sum(nv[a] * no[k] for a in range(4) for k in range(4) if P[a, k] == c)
+        assert (len(lay.oo[c]), len(lay.vv[c]), len(lay.vo[c])) == (noot, nvvt, nvo)
+    assert lay.size("T1") == sum(a * b for a, b in zip(no, nv))
+    # every C1 pair appears in exactly one compound irrep
+    for maps, n in ((lay.oo, sum(no)), (lay.vv, sum(nv))):
+        allp = np.sort(np.concatenate(maps))
+        assert np.array_equal(allp, np.arange(n * (n - 1) // 2))
+    rng = np.random.default_rng(0)
+    for name in ("FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        x = rng.normal(size=lay.size(name))
+        assert np.array_equal(lay.to_sym(name, lay.to_c1(name, x)), x), name
 
 
 @pytest.mark.parametrize("name", ["small_4o10v", "conv_5o16v", "diis2_3o8v"])
