@@ -3,20 +3,24 @@
 ! term handed to librelccsd_hip.so through ISO_C_BINDING.
 !
 ! Input  ccin.bin (stream): 8 int32 header (NO, NV, unused, MAXIT, MAXDIM,
-!        NTOL, 0, 0), orbital energies, then FOO, FVO, FVV and the classes
-!        OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd layout.  VOVV and VVVV
-!        are streamed to the device in column batches, as GETVOVV / GETVVVV
-!        read them (ccgetv.F:225-402).
+!        NTOL, NREP (0 = C1), 0), orbital energies, for NREP > 1 the fermion
+!        irrep of every spinor, the occupied count per irrep and MULTB(.,.,1)
+!        of the 2 NREP irreps (int32, as in MRCONEE), then FOO, FVO, FVV and
+!        the classes OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd's
+!        (symmetry-blocked) layout.  Every class block is streamed to the
+!        device in column batches, as GETVOVV / GETVVVV read them
+!        (ccgetv.F:225-402).
 ! Output the CCEQN iteration table on unit 6 (format of cceqns.F:6001) and
-!        ccout.bin: EMP2, ECCSD, NIT, T1, T2.
+!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout).
 program ccdrv_hip
   use relccsd_hip
   implicit none
   integer(c_int) :: hdr(8), rc, nit
-  integer :: no, nv, maxit, maxdim, ntol, cls
-  integer(c_int64_t) :: noot, nvvt, nvo, rows(0:5), cols(0:5), c0, nb, batch
+  integer :: no, nv, maxit, maxdim, ntol, cls, nrep, i, j, nsp
+  integer(c_int64_t) :: rows, cols, off, c0, nb, batch, nfoo, nfvo, nfvv, nt1, nt2
   real(c_double), allocatable :: eps(:), foo(:), fvo(:), fvv(:), buf(:), t1(:), t2(:)
-  integer(c_int), target :: nos(1), nvs(1)
+  integer(c_int), allocatable, target :: nos(:), nvs(:), multb(:, :)
+  integer(c_int), allocatable :: irrsp(:), mult(:, :)
   type(rcc_sym_t) :: sym
   type(c_ptr) :: h
   real(c_double) :: emp2, ecc, rms, diag, conv
@@ -24,37 +28,64 @@ program ccdrv_hip
   open (30, file='ccin.bin', access='stream', form='unformatted', status='old')
   read (30) hdr
   no = hdr(1); nv = hdr(2); maxit = hdr(4); maxdim = hdr(5); ntol = hdr(6)
-  noot = int(no, c_int64_t) * (no - 1) / 2
-  nvvt = int(nv, c_int64_t) * (nv - 1) / 2
-  nvo = int(no, c_int64_t) * nv
-  allocate (eps(no + nv), foo(2 * no * no), fvo(2 * nvo), fvv(2 * nv * nv))
-  read (30) eps, foo, fvo, fvv
+  nrep = max(1, hdr(7))
+  nsp = no + nv
+  allocate (eps(nsp), nos(nrep), nvs(nrep), multb(nrep, nrep))
+  read (30) eps
+  if (nrep > 1) then
+    ! RDSYMI's inputs (ccdriv.F:2978-2988): NO/NV per irrep and the
+    ! fermion x fermion -> boson products with the boson offset removed
+    allocate (irrsp(nsp), mult(2 * nrep, 2 * nrep))
+    read (30) irrsp
+    read (30) nos
+    read (30) mult
+    do i = 1, nrep
+      nvs(i) = count(irrsp == i) - nos(i)
+    end do
+    do j = 1, nrep
+      do i = 1, nrep
+        multb(i, j) = mult(i, j) - nrep
+      end do
+    end do
+  else
+    nos(1) = no
+    nvs(1) = nv
+  end if
 
-  nos(1) = no
-  nvs(1) = nv
-  sym%nrep = 1
+  sym%nrep = nrep
   sym%no = c_loc(nos)
   sym%nv = c_loc(nvs)
   sym%frozen = c_null_ptr
   sym%multb = c_null_ptr
+  if (nrep > 1) sym%multb = c_loc(multb)
   rc = rcc_create(sym, h)
   call rcc_check(h, rc, 'rcc_create')
   call rcc_check(h, rcc_set_options(h, maxdim, maxit, ntol, 0_c_int), 'rcc_set_options')
+
+  ! block sizes from the library's SYMTAB restatement (IOO/IVO/IVV totals)
+  call rcc_check(h, rcc_layout_block(h, RCC_LAY_FOO, nrep, rows, cols, nfoo), 'layout')
+  call rcc_check(h, rcc_layout_block(h, RCC_LAY_T1, nrep, rows, cols, nfvo), 'layout')
+  call rcc_check(h, rcc_layout_block(h, RCC_LAY_FVV, nrep, rows, cols, nfvv), 'layout')
+  allocate (foo(2 * nfoo), fvo(2 * nfvo), fvv(2 * nfvv))
+  read (30) foo, fvo, fvv
   call rcc_check(h, rcc_set_fock(h, foo, fvo, fvv, eps), 'rcc_set_fock')
 
-  rows = [noot, nvo, nvvt, nvo, nvo, nvvt]
-  cols = [noot, noot, noot, nvo, nvvt, nvvt]
   do cls = 0, 5
-    ! column batches of at most 64 MB, the way the sorted files are streamed
-    batch = max(1_c_int64_t, (8_c_int64_t * 1024 * 1024) / max(1_c_int64_t, rows(cls)))
-    c0 = 0
-    do while (c0 < cols(cls))
-      nb = min(batch, cols(cls) - c0)
-      if (allocated(buf)) deallocate (buf)
-      allocate (buf(2 * rows(cls) * nb))
-      read (30) buf
-      call rcc_check(h, rcc_set_ints(h, int(cls, c_int), 0_c_int, c0, nb, buf), 'rcc_set_ints')
-      c0 = c0 + nb
+    do i = 0, nrep - 1
+      call rcc_check(h, rcc_layout_block(h, int(cls, c_int), int(i, c_int), rows, cols, off), &
+                     'layout')
+      ! column batches of at most 64 MB, the way the sorted files are streamed
+      batch = max(1_c_int64_t, (8_c_int64_t * 1024 * 1024) / max(1_c_int64_t, rows))
+      c0 = 0
+      do while (c0 < cols)
+        nb = min(batch, cols - c0)
+        if (allocated(buf)) deallocate (buf)
+        allocate (buf(2 * rows * nb))
+        read (30) buf
+        call rcc_check(h, rcc_set_ints(h, int(cls, c_int), int(i, c_int), c0, nb, buf), &
+                       'rcc_set_ints')
+        c0 = c0 + nb
+      end do
     end do
   end do
   close (30)
@@ -94,7 +125,9 @@ program ccdrv_hip
   if (rms >= conv) write (*, '(//a,f20.15)') ' Requested convergence not reached in CCSD : RMS = ', rms
   write (*, '(a,t40,f25.15)') ' CCSD correlation energy :', ecc
 
-  allocate (t1(2 * nvo), t2(2 * nvvt * noot))
+  call rcc_check(h, rcc_layout_block(h, RCC_LAY_T1, nrep, rows, cols, nt1), 'layout')
+  call rcc_check(h, rcc_layout_block(h, RCC_LAY_T2, nrep, rows, cols, nt2), 'layout')
+  allocate (t1(2 * nt1), t2(2 * nt2))
   call rcc_check(h, rcc_get_amplitudes(h, t1, t2), 'GETAMPT')
   open (32, file='ccout.bin', access='stream', form='unformatted', status='replace')
   write (32) emp2, ecc, dble(nit)

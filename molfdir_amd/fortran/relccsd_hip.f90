@@ -12,6 +12,9 @@ module relccsd_hip
   integer(c_int), parameter :: RCC_OOOO = 0, RCC_VOOO = 1, RCC_VVOO = 2, &
                                RCC_VOVO = 3, RCC_VOVV = 4, RCC_VVVV = 5
 
+  ! layout ids of rcc_layout_block besides the class ids 0..5
+  integer(c_int), parameter :: RCC_LAY_T1 = 6, RCC_LAY_T2 = 7, RCC_LAY_FOO = 8, RCC_LAY_FVV = 9
+
   type, bind(c) :: rcc_sym_t
     integer(c_int) :: nrep
     type(c_ptr) :: no

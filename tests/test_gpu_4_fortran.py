@@ -17,9 +17,10 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
-@pytest.mark.parametrize("name", ["cfg1_6o40v", "conv_5o16v"])
-def test_fortran_driver_matches_reference(name, tmp_path):
-    g = load("ccsd", name)
+@pytest.mark.parametrize("kind,name", [("ccsd", "cfg1_6o40v"), ("ccsd", "conv_5o16v"),
+                                       ("symccsd", "s4_5o12v"), ("symccsd", "s8_8o24v")])
+def test_fortran_driver_matches_reference(kind, name, tmp_path):
+    g = load(kind, name)
     no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
     s = system_for(g)
     refrun._write_input(str(tmp_path / "ccin.bin"), s, 1, maxit, maxdim, ntol)
@@ -34,7 +35,8 @@ def test_fortran_driver_matches_reference(name, tmp_path):
         assert abs(a[2] - b[2]) < 1e-9 * max(1.0, b[2])
     raw = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
     z = raw[3:].view(np.complex128)
-    nt1 = no * nv
+    nt1 = g["T1"].size
+    assert z.size == nt1 + g["T2"].size
     assert abs(raw[0] - float(g["emp2"])) < 1e-10
     assert np.abs(z[:nt1] - g["T1"]).max() < 1e-9
     assert np.abs(z[nt1:] - g["T2"]).max() < 1e-9
