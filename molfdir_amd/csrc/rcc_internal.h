@@ -24,6 +24,25 @@ struct GemmArgs {
 
 // ws may be null (no K-split); ws_elems = capacity in complex elements.
 int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
+
+// Grouped GEMM: many independent C_g = alpha_g A_g B_g + beta_g C_g (op(A) = N,
+// op(B) = N or C) in one launch.  Group g owns blocks [tile_off, tile_off +
+// tiles(g)) of the grid, tile_off a multiple of 8 so the XCD-aware tile order
+// holds inside each group.
+struct ZGroup {
+    int m, n, k, pad;
+    int64_t lda, ldb, ldc;
+    const double2* A;
+    const double2* B;
+    double2* C;
+    double2 alpha, beta;
+    int64_t tile_off;
+};
+// Sets tile_off of every group; returns the grid size (blocks).
+int64_t zgroup_offsets(ZGroup* g, int ngroups);
+// dG: device copy of the descriptors (with offsets); tag as GemmArgs::tag.
+int zgemm_grouped(int opb, int tag, int gauss, const ZGroup* dG, int ngroups,
+                  int64_t blocks, hipStream_t st);
 int mfma_probe(double* dout, hipStream_t st);
 int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 

@@ -125,6 +125,18 @@ struct rcc_handle {
     SymInfo sym;
     double2* stage = nullptr;   // device staging for host blocks
     int64_t stage_n = 0;
+    // grouped contractions over irrep blocks (nrep > 1): VVVV and VOVV are
+    // also kept in their symmetry-blocked layout; the ladder, VOVV.tau and
+    // (for self-inverse irreps) the two rings run as one grouped GEMM each on
+    // block copies of their operands
+    bool ring_grouped = false;
+    double2 *VVVV_s = nullptr, *VOVV_s = nullptr, *TAU_s = nullptr, *S2_s = nullptr,
+            *X_s = nullptr, *H_s = nullptr, *RING_s = nullptr, *RING2_s = nullptr,
+            *VVOOR_s = nullptr;
+    ZGroup *g_ladder = nullptr, *g_x = nullptr, *g_ring1 = nullptr, *g_ring2 = nullptr;
+    int ng_ladder = 0, ng_x = 0, ng_ring1 = 0, ng_ring2 = 0;
+    int64_t nb_ladder = 0, nb_x = 0, nb_ring1 = 0, nb_ring2 = 0;
+    bool vvoor_s_ok = false, tau_s_ok = false, ring_s_ok = false;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -142,6 +154,12 @@ namespace {
             return -1;                                                                \
         }                                                                             \
     } while (0)
+
+// symmetry-layout helpers (defined with the layout code further down)
+void c1_to_sym_dev(rcc_t* h, int lay, const double2* src, double2* dst);
+void sym_to_c1_dev(rcc_t* h, int lay, const double2* src, double2* dst);
+int64_t lay_total(const rcc_t* h, int lay);
+int run_groups(rcc_t* h, int opb, int tag, const ZGroup* dG, int ng, int64_t nblocks);
 
 int dalloc(rcc_t* h, double2** p, int64_t n) {
     *p = nullptr;
@@ -215,6 +233,7 @@ int build_derived(rcc_t* h) {
     k_expand_e2(h->cls[RCC_VVOO], h->VVOO_E2, nvvt, h->oo, o, st);
     k_expand_f1(h->cls[RCC_VVOO], h->VVOO_F1, noot, h->vv, v, st);
     k_expand_ring(h->cls[RCC_VVOO], nullptr, 1.0, h->VVOO_R, h->vv, h->oo, v, o, st);
+    if (h->ring_grouped) c1_to_sym_dev(h, RCC_VOVO, h->VVOO_R, h->VVOOR_s);
     k_perm_akci(h->cls[RCC_VOVO], h->VOVO_R, v, o, false, st);
     k_expand_cols(h->cls[RCC_VOVV], h->VOVV_Q, nvo, h->vv, v, st);
     // small VOOO re-orderings through host-built gather maps
@@ -281,6 +300,11 @@ int ensure_tau(rcc_t* h) {
     if (!h->tau_ok) {
         k_gettau(h->T1(), h->T2(), h->TAU, 1.0, h->vv, h->oo, h->nv, h->st);
         h->tau_ok = true;
+        h->tau_s_ok = false;
+    }
+    if (h->sym.nrep > 1 && !h->tau_s_ok) {
+        c1_to_sym_dev(h, RCC_LAY_T2, h->TAU, h->TAU_s);
+        h->tau_s_ok = true;
     }
     return 0;
 }
@@ -289,6 +313,7 @@ int ensure_ring(rcc_t* h) {
     if (!h->ring_ok) {
         k_expand_ring(h->T2(), nullptr, 1.0, h->RING, h->vv, h->oo, h->nv, h->no, h->st);
         h->ring_ok = true;
+        h->ring_s_ok = false;
     }
     return 0;
 }
@@ -296,6 +321,8 @@ int ensure_ring(rcc_t* h) {
 // BINTM's X(AK,IJ) = sum_{C>D} VOVV(AK,CD) TAU(CD,IJ)  (ccintm.F:772-788); also the
 // sum that T1EQN2 (cceqns.F:2601-2714) takes the (I,K) diagonal of.
 void invalidate_t(rcc_t* h) {
+    h->tau_s_ok = false;
+    h->ring_s_ok = false;
     h->tau_ok = false;
     h->x_ok = false;
     h->ring_ok = false;
@@ -390,6 +417,17 @@ int ensure_x(rcc_t* h) {
         if (h->precision != 0) {
             CK(ensure_split_const(h));
             CK(ensure_split_tau(h));
+        }
+        if (h->sym.nrep > 1) {
+            // X blocks: VOVV(c) TAU(c), one grouped launch over the irreps
+            k_fill_zero(h->X_s, lay_total(h, RCC_VOOO), h->st);
+            ktick(h, 3, 0);
+            CK(run_groups(h, OP_N, 0, h->g_x, h->ng_x, h->nb_x));
+            ktick(h, 3, 1);
+            k_fill_zero(h->X, h->nvo * h->noot, h->st);
+            sym_to_c1_dev(h, RCC_VOOO, h->X_s, h->X);
+            h->x_ok = true;
+            return 0;
         }
         ktick(h, 3, 0);
         if (h->precision != 0)
@@ -531,6 +569,15 @@ int do_hintm(rcc_t* h, bool hakci_has_vovv) {
     k_perm_akci(h->HAKCI, h->H, v, o, false, st);
     // + [1/2 T(AD,IL) - T(A,L) T(D,I)] W(KL,CD)  (ccintm.F:393-450)
     k_expand_ring(h->T2(), h->T1(), 0.5, h->RING2, h->vv, h->oo, v, o, st);
+    if (h->ring_grouped) {
+        c1_to_sym_dev(h, RCC_VOVO, h->H, h->H_s);
+        c1_to_sym_dev(h, RCC_VOVO, h->RING2, h->RING2_s);
+        ktick(h, 1, 0);
+        CK(run_groups(h, OP_C, 2, h->g_ring1, h->ng_ring1, h->nb_ring1));
+        ktick(h, 1, 1);
+        sym_to_c1_dev(h, RCC_VOVO, h->H_s, h->H);
+        return 0;
+    }
     if (h->precision != 0) {
         CK(ensure_split_const(h));
         CK(do_split(h, h->sP, (int)nvo, nvo, h->RING2, nvo, false, false));
@@ -583,7 +630,19 @@ int do_t2eqns(rcc_t* h) {
             (int)noot, 0, nvo, (int64_t)v * v));
     k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, st);
     // P(ab)P(ij) H(AI,CK) T2(CK,BJ)  (cceqns.F:1243-1263)
-    if (h->precision != 0) {
+    if (h->ring_grouped) {
+        c1_to_sym_dev(h, RCC_VOVO, h->H, h->H_s);
+        if (!h->ring_s_ok) {
+            c1_to_sym_dev(h, RCC_VOVO, h->RING, h->RING_s);
+            h->ring_s_ok = true;
+        }
+        k_fill_zero(h->RING2_s, lay_total(h, RCC_VOVO), st);
+        ktick(h, 2, 0);
+        CK(run_groups(h, OP_N, 2, h->g_ring2, h->ng_ring2, h->nb_ring2));
+        ktick(h, 2, 1);
+        k_fill_zero(h->RING2, nvo * nvo, st);
+        sym_to_c1_dev(h, RCC_VOVO, h->RING2_s, h->RING2);
+    } else if (h->precision != 0) {
         CK(do_split(h, h->sH, (int)nvo, nvo, h->H, nvo, false, false));
         if (!h->split_ring_ok) {
             CK(do_split(h, h->sRING, (int)nvo, nvo, h->RING, nvo, true, false));
@@ -610,7 +669,14 @@ int do_bintm(rcc_t* h) {
     const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
     double2* S2 = h->S2();
     // ladder: S(AB,IJ) += V(AB,CD) TAU(CD,IJ), C>D  (ccintm.F:700-722)
-    if (h->precision != 0) {
+    if (h->sym.nrep > 1) {
+        // S2(c) += VVVV(c) TAU(c) over the compound irreps, one grouped launch
+        c1_to_sym_dev(h, RCC_LAY_T2, S2, h->S2_s);
+        ktick(h, 0, 0);
+        CK(run_groups(h, OP_N, 1, h->g_ladder, h->ng_ladder, h->nb_ladder));
+        ktick(h, 0, 1);
+        sym_to_c1_dev(h, RCC_LAY_T2, h->S2_s, S2);
+    } else if (h->precision != 0) {
         CK(ensure_split_const(h));
         CK(ensure_split_tau(h));
         ktick(h, 0, 0);
@@ -816,7 +882,11 @@ int alloc_all(rcc_t* h) {
     h->cls_rows[RCC_VOVO] = nvo;  h->cls_cols[RCC_VOVO] = nvo;
     h->cls_rows[RCC_VOVV] = nvo;  h->cls_cols[RCC_VOVV] = nvvt;
     h->cls_rows[RCC_VVVV] = nvvt; h->cls_cols[RCC_VVVV] = nvvt;
-    for (int c = 0; c < 6; ++c) CK(dalloc(h, &h->cls[c], h->cls_rows[c] * h->cls_cols[c]));
+    for (int c = 0; c < 6; ++c) {
+        // with symmetry the VVVV ladder runs on the blocks only (grouped GEMM)
+        const bool blocks_only = c == RCC_VVVV && h->sym.nrep > 1;
+        CK(dalloc(h, &h->cls[c], blocks_only ? 1 : h->cls_rows[c] * h->cls_cols[c]));
+    }
     CK(dalloc(h, &h->VVOO_E2, nvvt * o * o));
     CK(dalloc(h, &h->VVOO_F1, (int64_t)v * v * noot));
     CK(dalloc(h, &h->VVOO_R, nvo * nvo));
@@ -1023,6 +1093,116 @@ int sym_to_c1(rcc_t* h, int lay, const void* src, hipMemcpyKind kind, double2* d
     return 0;
 }
 
+// C1 array <-> whole symmetry-layout array, both on the device.
+void c1_to_sym_dev(rcc_t* h, int lay, const double2* src, double2* dst) {
+    int rk, ck;
+    int64_t ld;
+    lay_spec(h, lay, &rk, &ck, &ld);
+    const SymInfo& S = h->sym;
+    for (int b = 0; b < S.nrep; ++b)
+        k_gather_block(src, ld, S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
+                       lay_block_rows(h, rk, b), lay_block_rows(h, ck, b),
+                       dst + lay_offset(h, rk, ck, b), h->st);
+}
+
+void sym_to_c1_dev(rcc_t* h, int lay, const double2* src, double2* dst) {
+    int rk, ck;
+    int64_t ld;
+    lay_spec(h, lay, &rk, &ck, &ld);
+    const SymInfo& S = h->sym;
+    for (int b = 0; b < S.nrep; ++b)
+        k_scatter_block(src + lay_offset(h, rk, ck, b), lay_block_rows(h, rk, b),
+                        lay_block_rows(h, ck, b), S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
+                        dst, ld, h->st);
+}
+
+int64_t lay_total(const rcc_t* h, int lay) {
+    int rk, ck;
+    int64_t ld;
+    lay_spec(h, lay, &rk, &ck, &ld);
+    return lay_offset(h, rk, ck, h->sym.nrep);
+}
+
+// Descriptors of one grouped GEMM over the irrep blocks: C(b) = alpha A(b) op(B(b)) +
+// beta C(b), each operand a symmetry-layout array (layouts la, lb, lc), block b of
+// shape m(b) x k(b) etc. taken from the layouts; blocks with an empty side are left out.
+int make_groups(rcc_t* h, int la, const double2* A, int lb, const double2* B, int lc,
+                double2* C, bool bconj, double2 alpha, double2 beta, ZGroup** dG, int* ng,
+                int64_t* nblocks) {
+    std::vector<ZGroup> g;
+    for (int b = 0; b < h->sym.nrep; ++b) {
+        int64_t ar, ac, ao, br, bc, bo, cr, cc, co;
+        rcc_layout_block(h, la, b, &ar, &ac, &ao);
+        rcc_layout_block(h, lb, b, &br, &bc, &bo);
+        rcc_layout_block(h, lc, b, &cr, &cc, &co);
+        const int64_t k = ac;
+        if (cr == 0 || cc == 0 || k == 0) continue;
+        if (ar != cr || (bconj ? (bc != k || br != cc) : (br != k || bc != cc))) {
+            h->err = "grouped GEMM: block shapes do not chain";
+            return -1;
+        }
+        ZGroup z{};
+        z.m = (int)cr;
+        z.n = (int)cc;
+        z.k = (int)k;
+        z.A = A + ao;
+        z.lda = std::max<int64_t>(ar, 1);
+        z.B = B + bo;
+        z.ldb = std::max<int64_t>(br, 1);
+        z.C = C + co;
+        z.ldc = std::max<int64_t>(cr, 1);
+        z.alpha = alpha;
+        z.beta = beta;
+        g.push_back(z);
+    }
+    *ng = (int)g.size();
+    *nblocks = zgroup_offsets(g.data(), *ng);
+    CK(dalloc_raw(h, dG, std::max<int64_t>(1, (int64_t)g.size()) * (int64_t)sizeof(ZGroup)));
+    if (!g.empty())
+        HIPCHK(h, hipMemcpy(*dG, g.data(), g.size() * sizeof(ZGroup), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int run_groups(rcc_t* h, int opb, int tag, const ZGroup* dG, int ng, int64_t nblocks) {
+    if (zgemm_grouped(opb, tag, h->gemm_gauss, dG, ng, nblocks, h->st) != 0) {
+        h->err = "grouped GEMM launch failed";
+        return -1;
+    }
+    return 0;
+}
+
+// Block arrays and grouped-GEMM descriptors for nrep > 1 (after alloc_all).
+int setup_grouped(rcc_t* h) {
+    const SymInfo& S = h->sym;
+    bool selfinv = true;
+    for (int i = 0; i < S.nrep; ++i) selfinv = selfinv && S.ptab[i + S.nrep * i] == 0;
+    h->ring_grouped = selfinv;
+    CK(dalloc(h, &h->VVVV_s, lay_total(h, RCC_VVVV)));
+    CK(dalloc(h, &h->VOVV_s, lay_total(h, RCC_VOVV)));
+    CK(dalloc(h, &h->TAU_s, lay_total(h, RCC_LAY_T2)));
+    CK(dalloc(h, &h->S2_s, lay_total(h, RCC_LAY_T2)));
+    CK(dalloc(h, &h->X_s, lay_total(h, RCC_VOOO)));
+    k_fill_zero(h->VVVV_s, lay_total(h, RCC_VVVV), h->st);
+    k_fill_zero(h->VOVV_s, lay_total(h, RCC_VOVV), h->st);
+    CK(make_groups(h, RCC_VVVV, h->VVVV_s, RCC_LAY_T2, h->TAU_s, RCC_LAY_T2, h->S2_s, false,
+                   cplx(1, 0), cplx(1, 0), &h->g_ladder, &h->ng_ladder, &h->nb_ladder));
+    CK(make_groups(h, RCC_VOVV, h->VOVV_s, RCC_LAY_T2, h->TAU_s, RCC_VOOO, h->X_s, false,
+                   cplx(1, 0), cplx(0, 0), &h->g_x, &h->ng_x, &h->nb_x));
+    if (h->ring_grouped) {
+        const int64_t nr = lay_total(h, RCC_VOVO);
+        CK(dalloc(h, &h->H_s, nr));
+        CK(dalloc(h, &h->RING_s, nr));
+        CK(dalloc(h, &h->RING2_s, nr));
+        CK(dalloc(h, &h->VVOOR_s, nr));
+        CK(make_groups(h, RCC_VOVO, h->RING2_s, RCC_VOVO, h->VVOOR_s, RCC_VOVO, h->H_s, true,
+                       cplx(1, 0), cplx(1, 0), &h->g_ring1, &h->ng_ring1, &h->nb_ring1));
+        CK(make_groups(h, RCC_VOVO, h->H_s, RCC_VOVO, h->RING_s, RCC_VOVO, h->RING2_s, false,
+                       cplx(1, 0), cplx(0, 0), &h->g_ring2, &h->ng_ring2, &h->nb_ring2));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
 // C1 array -> whole symmetry-layout array in host memory.
 int c1_to_sym_host(rcc_t* h, int lay, const double2* src, double* out) {
     int rk, ck;
@@ -1079,14 +1259,16 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
     HIPCHK(h, hipEventCreate(&h->ev1));
     for (int w = 0; w < 4; ++w)
         for (int p = 0; p < 2; ++p) HIPCHK(h, hipEventCreate(&h->kev[w][p]));
-    CK(alloc_all(h));
     CK(build_sym(h, sym));
+    CK(alloc_all(h));
     if (sym->nrep > 1) {
+        CK(setup_grouped(h));
         // off-symmetry blocks of the C1 arrays are never written
         k_fill_zero(h->FOO, (int64_t)h->no * h->no, h->st);
         k_fill_zero(h->FVO, h->nvo, h->st);
         k_fill_zero(h->FVV, (int64_t)h->nv * h->nv, h->st);
-        for (int c = 0; c < 6; ++c) k_fill_zero(h->cls[c], h->cls_rows[c] * h->cls_cols[c], h->st);
+        for (int c = 0; c < 6; ++c)
+            if (c != RCC_VVVV) k_fill_zero(h->cls[c], h->cls_rows[c] * h->cls_cols[c], h->st);
         HIPCHK(h, hipStreamSynchronize(h->st));
     }
     // pair tables
@@ -1131,6 +1313,10 @@ int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision) {
     if (!h) return -1;
     if (maxdim < 1 || maxdim > MAXDIM_CAP) {
         h->err = "rcc_set_options: maxdim must be 1.." + std::to_string(MAXDIM_CAP);
+        return -1;
+    }
+    if (precision != 0 && h->sym.nrep > 1) {
+        h->err = "rcc_set_options: split precision is available for C1 handles (nrep = 1)";
         return -1;
     }
     if (precision < 0 || precision > 2) {
@@ -1220,8 +1406,14 @@ static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t n
             dsrc = h->stage;
         }
         const SymInfo& S = h->sym;
-        k_scatter_block(dsrc, rows, ncols, S.dmap + S.doff[rk][irrep],
-                        S.dmap + S.doff[ck][irrep] + col0, h->cls[cls], ld, h->st);
+        if (cls == RCC_VVVV || cls == RCC_VOVV) {
+            double2* blk = (cls == RCC_VVVV ? h->VVVV_s : h->VOVV_s) + lay_offset(h, rk, ck, irrep);
+            HIPCHK(h, hipMemcpyAsync(blk + col0 * rows, dsrc, (size_t)(rows * ncols) * 16,
+                                     hipMemcpyDeviceToDevice, h->st));
+        }
+        if (cls != RCC_VVVV)
+            k_scatter_block(dsrc, rows, ncols, S.dmap + S.doff[rk][irrep],
+                            S.dmap + S.doff[ck][irrep] + col0, h->cls[cls], ld, h->st);
         HIPCHK(h, hipStreamSynchronize(h->st));
     }
     h->derived_ok = false;

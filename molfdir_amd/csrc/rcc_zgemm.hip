@@ -196,14 +196,18 @@ __device__ __forceinline__ void store_b(double* __restrict__ sre, double* __rest
 // W[(z) * m * n + i + j * m] and zgemm_reduce finishes the epilogue.
 constexpr int NXCD = 8;
 
-template <int OPA, int OPB, int TAG, int GAUSS>
+// GROUPED: the problem (m, n, k, operands, alpha, beta) is read from the
+// group descriptor that owns this block (binary search on tile offsets); no
+// batch, no K-split.
+template <int OPA, int OPB, int TAG, int GAUSS, int GROUPED = 0>
 __global__ void __launch_bounds__(256)
 zgemm_mfma_kernel(int m, int n, int k,
                   const double2* __restrict__ A, int64_t lda, int64_t sA,
                   const double2* __restrict__ B, int64_t ldb, int64_t sB,
                   double2* __restrict__ C, int64_t ldc, int64_t sC,
                   double2 alpha, double2 beta, int nsplit, int kchunk,
-                  double2* __restrict__ W) {
+                  double2* __restrict__ W, const ZGroup* __restrict__ G = nullptr,
+                  int ngroups = 0) {
     // two LDS stages: the next K-tile is written while the current one feeds the MFMAs
     __shared__ double As_re[2][BK * LDA_S];
     __shared__ double As_im[2][BK * LDA_S];
@@ -215,9 +219,28 @@ zgemm_mfma_kernel(int m, int n, int k,
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     // XCD-aware tile order (see header): block b runs on XCD b % 8
+    int b = blockIdx.x;
+    if (GROUPED) {
+        int lo = 0, hi = ngroups - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (G[mid].tile_off <= (int64_t)b) lo = mid;
+            else hi = mid - 1;
+        }
+        const ZGroup& gg = G[lo];
+        m = gg.m; n = gg.n; k = gg.k;
+        A = gg.A; lda = gg.lda; sA = 0;
+        B = gg.B; ldb = gg.ldb; sB = 0;
+        C = gg.C; ldc = gg.ldc; sC = 0;
+        alpha = gg.alpha; beta = gg.beta;
+        nsplit = 1;
+        kchunk = k;
+        b -= (int)gg.tile_off;
+        // padding blocks of the group (whole workgroup, before any barrier)
+        if (b >= ((m + BM - 1) / BM) * ((n + BN - 1) / BN)) return;
+    }
     const int mt = (m + BM - 1) / BM, nt = (n + BN - 1) / BN;
     const int T = mt * nt;
-    const int b = blockIdx.x;
     const int xcd = b % NXCD, idx = b / NXCD;
     const int q = T / NXCD, rr = T % NXCD;
     const int start = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
@@ -422,6 +445,44 @@ static void launch(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStr
     else if (OPA == OP_N && (OPB == OP_N || OPB == OP_C) && g.tag == 2)
         launch_t<OPA, OPB, 2>(g, nsplit, kchunk, W, st);
     else launch_t<OPA, OPB, 0>(g, nsplit, kchunk, W, st);
+}
+
+int64_t zgroup_offsets(ZGroup* g, int ngroups) {
+    int64_t off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        g[i].tile_off = off;
+        const int64_t t = (int64_t)((g[i].m + BM - 1) / BM) * ((g[i].n + BN - 1) / BN);
+        off += (t + NXCD - 1) / NXCD * NXCD;
+    }
+    return off;
+}
+
+template <int OPB, int TAG, int GAUSS>
+static void launch_grouped(const ZGroup* dG, int ngroups, int64_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OP_N, OPB, TAG, GAUSS, 1>), dim3((unsigned)blocks),
+                       dim3(256), 0, st, 0, 0, 0, (const double2*)nullptr, (int64_t)0, (int64_t)0,
+                       (const double2*)nullptr, (int64_t)0, (int64_t)0, (double2*)nullptr,
+                       (int64_t)0, (int64_t)0, make_double2(0, 0), make_double2(0, 0), 1, 0,
+                       (double2*)nullptr, dG, ngroups);
+}
+
+int zgemm_grouped(int opb, int tag, int gauss, const ZGroup* dG, int ngroups, int64_t blocks,
+                  hipStream_t st) {
+    if (ngroups <= 0 || blocks <= 0) return 0;
+    if (opb == OP_N) {
+        if (tag == 1) gauss ? launch_grouped<OP_N, 1, 1>(dG, ngroups, blocks, st)
+                            : launch_grouped<OP_N, 1, 0>(dG, ngroups, blocks, st);
+        else if (tag == 2) gauss ? launch_grouped<OP_N, 2, 1>(dG, ngroups, blocks, st)
+                                 : launch_grouped<OP_N, 2, 0>(dG, ngroups, blocks, st);
+        else gauss ? launch_grouped<OP_N, 0, 1>(dG, ngroups, blocks, st)
+                   : launch_grouped<OP_N, 0, 0>(dG, ngroups, blocks, st);
+    } else if (opb == OP_C) {
+        gauss ? launch_grouped<OP_C, 2, 1>(dG, ngroups, blocks, st)
+              : launch_grouped<OP_C, 2, 0>(dG, ngroups, blocks, st);
+    } else {
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int choose_split(const GemmArgs& g, int64_t ws_elems) {
