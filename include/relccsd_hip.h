@@ -152,6 +152,18 @@ int rcc_accept(rcc_t* h);
 int rcc_iterate(rcc_t* h, int* nit, double* ecc, double* rms);
 
 /* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
+/* Replay the term sequence of rcc_iterate / rcc_cceqn / rcc_iterate_batch
+ * (HOINTM .. BINTM, DENOM: all device work) from a hipGraph: the first
+ * iteration runs eagerly, the second is captured, later ones are one graph
+ * launch.  Any change of integrals or precision options re-captures.  No
+ * counterpart in the reference. */
+int rcc_set_graph(rcc_t* h, int enable);
+/* One CCEQN iteration of each of n independent handles (each on its own
+ * stream): every handle's term sequence is enqueued first, then DIISX,
+ * CCENG and S -> T run handle by handle while the other handles' terms are
+ * still on the device.  nit/ecc/rms receive one value per handle.  No
+ * counterpart in the reference (which runs one system per process). */
+int rcc_iterate_batch(rcc_t* const* handles, int n, int* nit, double* ecc, double* rms);
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);
 
 /* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms. */

@@ -137,6 +137,10 @@ struct rcc_handle {
     int ng_ladder = 0, ng_x = 0, ng_ring1 = 0, ng_ring2 = 0;
     int64_t nb_ladder = 0, nb_x = 0, nb_ring1 = 0, nb_ring2 = 0;
     bool vvoor_s_ok = false, tau_s_ok = false, ring_s_ok = false;
+    // hipGraph of the term sequence HOINTM .. DENOM (replayed each iteration)
+    bool use_graph = false, graph_warm = false, graph_ok = false, capturing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -208,6 +212,7 @@ int gemm(rcc_t* h, int opa, int opb, int m, int n, int k, double2 alpha, const d
 }
 
 void ktick(rcc_t* h, int which, int phase) {
+    if (h->capturing) return;   // graph replays are timed as a whole
     (void)hipEventRecord(h->kev[which][phase], h->st);
     if (phase == 1) h->ktimed[which] = true;
 }
@@ -835,8 +840,8 @@ int do_t1diag(rcc_t* h, const double2* t1, double* diag) {
     return 0;
 }
 
-int do_iteration(rcc_t* h, double* ecc, double* rms) {
-    HIPCHK(h, hipEventRecord(h->ev0, h->st));
+// HOINTM .. BINTM, DENOM: device work only, no host synchronisation.
+int run_terms(rcc_t* h) {
     CK(do_hointm(h));
     CK(do_hvintm(h));
     CK(do_hovintm(h));
@@ -849,6 +854,54 @@ int do_iteration(rcc_t* h, double* ecc, double* rms) {
     CK(do_bintm(h));
     k_denom(h->S1(), h->S2(), h->eps, h->any_frozen ? h->frozen : nullptr, h->vv, h->oo, h->nv,
             h->no, h->st);
+    return 0;
+}
+
+void drop_graph(rcc_t* h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_ok = false;
+    h->graph_warm = false;
+}
+
+// The term phase of one iteration, enqueued on the handle's stream.  With the
+// graph enabled, the first call runs eagerly (so every buffer the sequence
+// needs exists), the second captures the sequence into a hipGraph, and every
+// call after that replays it with one launch.  The sequence is the same each
+// iteration: it starts from invalidated intermediates and fixed buffers.
+int terms_phase(rcc_t* h) {
+    CK(prep(h));
+    if (!h->use_graph) return run_terms(h);
+    if (h->graph_ok) {
+        HIPCHK(h, hipGraphLaunch(h->graph_exec, h->st));
+        return 0;
+    }
+    if (!h->graph_warm) {
+        h->graph_warm = true;
+        return run_terms(h);
+    }
+    HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    h->capturing = true;
+    const int rc = run_terms(h);
+    h->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->st, &g);
+    if (rc != 0 || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc == 0) h->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return -1;
+    }
+    h->graph = g;
+    HIPCHK(h, hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_ok = true;
+    HIPCHK(h, hipGraphLaunch(h->graph_exec, h->st));
+    return 0;
+}
+
+// DIISX, CCENG, S -> T and the timing of the iteration (host-synchronous).
+int finish_phase(rcc_t* h, double* ecc, double* rms) {
     CK(do_diis(h, rms));
     CK(do_energy(h, ecc, nullptr));
     HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
@@ -866,6 +919,12 @@ int do_iteration(rcc_t* h, double* ecc, double* rms) {
     }
     h->nit += 1;
     return 0;
+}
+
+int do_iteration(rcc_t* h, double* ecc, double* rms) {
+    HIPCHK(h, hipEventRecord(h->ev0, h->st));
+    CK(terms_phase(h));
+    return finish_phase(h, ecc, rms);
 }
 
 int alloc_all(rcc_t* h) {
@@ -1297,6 +1356,7 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
 void rcc_destroy(rcc_t* h) {
     if (!h) return;
     if (h->st) (void)hipStreamSynchronize(h->st);
+    drop_graph(h);
     for (void* p : h->allocs) (void)hipFree(p);
     for (int w = 0; w < 4; ++w)
         for (int p = 0; p < 2; ++p)
@@ -1324,6 +1384,7 @@ int rcc_set_options(rcc_t* h, int maxdim, int maxit, int ntol, int precision) {
         return -1;
     }
     if (precision != h->precision) {
+        drop_graph(h);
         free_split(h);
         invalidate_t(h);
     }
@@ -1341,6 +1402,7 @@ int rcc_set_split_digits(rcc_t* h, int digits) {
         return -1;
     }
     if (digits != h->digits) {
+        drop_graph(h);
         h->digits = digits;
         h->split_const_ok = h->split_tau_ok = h->split_ring_ok = false;
         h->x_ok = false;
@@ -1418,6 +1480,7 @@ static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t n
     }
     h->derived_ok = false;
     h->split_const_ok = false;
+    if (h->graph_ok || h->graph_warm) drop_graph(h);
     return 0;
 }
 
@@ -1588,6 +1651,30 @@ int rcc_iterate(rcc_t* h, int* nit, double* ecc, double* rms) {
     if (nit) *nit = h->nit;
     if (ecc) *ecc = e;
     if (rms) *rms = r;
+    return 0;
+}
+
+int rcc_set_graph(rcc_t* h, int enable) {
+    if (!h) return -1;
+    drop_graph(h);
+    h->use_graph = enable != 0;
+    return 0;
+}
+
+int rcc_iterate_batch(rcc_t* const* hs, int n, int* nit, double* ecc, double* rms) {
+    if (!hs || n < 0) return -1;
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i]) return -1;
+        HIPCHK(hs[i], hipEventRecord(hs[i]->ev0, hs[i]->st));
+        CK(terms_phase(hs[i]));      // all systems' term phases in flight together
+    }
+    for (int i = 0; i < n; ++i) {
+        double e = 0, r = 0;
+        CK(finish_phase(hs[i], &e, &r));
+        if (nit) nit[i] = hs[i]->nit;
+        if (ecc) ecc[i] = e;
+        if (rms) rms[i] = r;
+    }
     return 0;
 }
 

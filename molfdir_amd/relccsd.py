@@ -95,6 +95,8 @@ def load():
     lib.rcc_kernel_times.argtypes = [vp, dp]
     lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
     lib.rcc_probe_mfma.argtypes = [vp, dp]
+    lib.rcc_set_graph.argtypes = [vp, ci]
+    lib.rcc_iterate_batch.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
     lib.rcc_layout_block.argtypes = [vp, ci, ci, ctypes.POINTER(i64), ctypes.POINTER(i64),
                                      ctypes.POINTER(i64)]
     lib.rcc_zgemm_split_device.argtypes = [vp, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
@@ -299,6 +301,28 @@ class RelCCSD:
         n, e, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.rcc_iterate(self.h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(r)))
         return n.value, e.value, r.value
+
+    def set_graph(self, enable: bool = True):
+        """Replay the per-iteration term sequence from a captured hipGraph."""
+        self._check(self.lib.rcc_set_graph(self.h, 1 if enable else 0))
+
+    @staticmethod
+    def iterate_batch(solvers):
+        """One CCEQN iteration of every solver, their term phases in flight
+        together; returns a list of (nit, Ec, rms)."""
+        n = len(solvers)
+        if n == 0:
+            return []
+        lib = solvers[0].lib
+        hs = (ctypes.c_void_p * n)(*[s.h.value for s in solvers])
+        nit = (ctypes.c_int * n)()
+        e = (ctypes.c_double * n)()
+        r = (ctypes.c_double * n)()
+        rc = lib.rcc_iterate_batch(hs, n, nit, e, r)
+        if rc != 0:
+            msgs = [lib.rcc_last_error(s.h).decode() for s in solvers]
+            raise RccError("; ".join(m for m in msgs if m) or "rcc_iterate_batch failed")
+        return [(nit[i], e[i], r[i]) for i in range(n)]
 
     def cceqn(self):
         """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
