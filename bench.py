@@ -27,6 +27,7 @@ CONFIGS = {
     "cfg2": (20, 200, 2, "C1 20o/200v complex128"),
     "cfg3": (30, 300, 3, "C1 30o/300v complex128"),
 }
+BATCH = {"cfg4": (64, 8, 4)}   # systems, irreps, seed of the size / proportion draws
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
@@ -109,12 +110,87 @@ def converged_accuracy(no, nv, seed, scale, modes, maxit=40, ntol=9):
     return res
 
 
+def batch_systems(nsys, nrep, seed):
+    """Sizes of the batched workload: occupied ~ U{4..16}, virtual ~ U{40..160},
+    both split over the irreps with one Dirichlet(2) draw of proportions."""
+    import numpy as np
+    from molfdir_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(nsys):
+        o, v = int(rng.integers(4, 17)), int(rng.integers(40, 161))
+        p = rng.dirichlet([2.0] * nrep)
+        out.append((o, v, synth.dirichlet_split(o, p), synth.dirichlet_split(v, p), 1000 + k))
+    return out
+
+
+def bench_batch(args):
+    """Many small symmetric systems: a per-system loop of eager iterations
+    against rcc_iterate_batch (term phases of all systems in flight together)
+    with each system's term sequence replayed from a hipGraph."""
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+    nsys, nrep, seed = BATCH[args.config]
+    sizes = batch_systems(nsys, nrep, seed)
+    dev = torch.device("cuda:0")
+    solvers = []
+    t0 = time.perf_counter()
+    for o, v, no_l, nv_l, sd in sizes:
+        sym = (no_l, nv_l)
+        d = synth.make_system(o, v, seed=sd, scale=args.scale, device=dev, sym=sym)
+        maps = synth.block_maps(sym)
+        r = RelCCSD(o, v, sym=sym, maxdim=6, maxit=1000, ntol=12)
+        r.set_fock(*(synth.to_blocks(n, d[n], sym, maps).cpu() for n in ("FOO", "FVO", "FVV")),
+                   d["eps"])
+        for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+            r.set_ints(name, synth.to_blocks(name, d[name], sym, maps).contiguous())
+        del d
+        solvers.append(r)
+    torch.cuda.synchronize()
+    print(f"[bench] built {nsys} systems in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    K = args.steps
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    for r in solvers:
+        r.mp2()
+        r.iterate()                 # builds the derived layouts outside the timing
+    t_loop = timed(lambda: [r.iterate() for r in solvers for _ in range(K)])
+    t_batch = timed(lambda: [RelCCSD.iterate_batch(solvers) for _ in range(K)])
+    for r in solvers:
+        r.set_graph(True)
+    for _ in range(2):              # eager, then captured
+        RelCCSD.iterate_batch(solvers)
+    t_graph = timed(lambda: [RelCCSD.iterate_batch(solvers) for _ in range(K)])
+    rate = {k: nsys * K / t for k, t in (("loop", t_loop), ("batch", t_batch),
+                                        ("batch_graph", t_graph))}
+    out = {
+        "metric": "CCSD system-iterations/s, 64 symmetric systems (o 4-16, v 40-160, 8 irreps)",
+        "value": rate["batch_graph"], "unit": "system-iterations/s", "n_gpus": 1, "steps": K,
+        "warmup": 3, "ms_per_step": t_graph / K * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic (seeded sizes seed {seed}; systems seeds 1000+; factor std {args.scale})",
+        "config": {"workload": f"{nsys} independent systems, Dirichlet(2) irrep proportions",
+                   "systems": [[o, v, no_l, nv_l] for o, v, no_l, nv_l, _ in sizes]},
+        "rates": rate, "speedup_vs_loop": rate["batch_graph"] / rate["loop"],
+        "seconds": {"loop": t_loop, "batch": t_batch, "batch_graph": t_graph},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + sorted(BATCH))
     ap.add_argument("--scale", type=float, default=0.05,
                     help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -141,6 +217,9 @@ def main():
     from molfdir_amd import synth
     from molfdir_amd.relccsd import RelCCSD
 
+    if args.config in BATCH:
+        bench_batch(args)
+        return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

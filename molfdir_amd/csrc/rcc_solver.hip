@@ -38,6 +38,8 @@ namespace {
 
 constexpr int RED_BLOCKS = 1024;
 constexpr int MAXDIM_CAP = 16;
+constexpr int RED_SLOTS = 40;
+constexpr int RED_E = 32;          // first energy slot in red_dst
 
 struct Buf {
     double2* p = nullptr;
@@ -103,7 +105,8 @@ struct rcc_handle {
     double2* ws = nullptr;
     int64_t ws_elems = 0;
     double2* red_part = nullptr;
-    double2* red_dst = nullptr;
+    double2* red_dst = nullptr;   // RED_SLOTS results: [0,2) norm, [2,2+16) DIIS dots, 32.. energy
+    double2* hred = nullptr;      // pinned host copy of red_dst
 
     std::vector<double2*> diis_err, diis_amp;
     int it = 0, itc = 0;
@@ -762,7 +765,12 @@ double flin(double* A, int idim, int n, int nr) {
 }
 
 // DIISX (cceqns.F:1985-2178) on device vectors [S1|S2] vs [T1|T2].
-int do_diis(rcc_t* h, double* rms_out) {
+// DIISX (cceqns.F:1985-2178) in two halves around one host synchronisation:
+// diis_begin enqueues the error vector, its norms and the B-row dots with an
+// asynchronous copy of the results to pinned host memory; diis_end (after the
+// stream has drained) forms the B matrix, solves it (FLIN) and enqueues the
+// extrapolation.
+int diis_begin(rcc_t* h) {
     const int64_t n = h->nt1 + h->nt2;
     hipStream_t st = h->st;
     const int md = h->maxdim;
@@ -777,15 +785,23 @@ int do_diis(rcc_t* h, double* rms_out) {
     HIPCHK(h, hipMemcpyAsync(h->diis_amp[c], h->S, n * 16, hipMemcpyDeviceToDevice, st));
     // RMS = sqrt(| sum |d|^2 - |sum d|^2 |)
     k_sum_and_norm(d, n, h->red_dst, h->red_part, RED_BLOCKS, st);
-    double2 sn[2];
-    HIPCHK(h, hipMemcpyAsync(sn, h->red_dst, sizeof(sn), hipMemcpyDeviceToHost, st));
-    HIPCHK(h, hipStreamSynchronize(st));
-    const double rms = std::sqrt(std::fabs(sn[1].x - (sn[0].x * sn[0].x + sn[0].y * sn[0].y)));
-    // B row: Re + Im of <d_n | d_ITC>
+    // B row: Re + Im of <d_k | d_ITC>
     std::vector<const double2*> xs;
     for (int k = 0; k < h->it; ++k) xs.push_back(h->diis_err[k]);
-    std::vector<double2> dots;
-    CK(zdot_multi(h, xs, d, n, dots));
+    k_zdotc_multi(xs.data(), (int)xs.size(), d, n, h->red_dst + 2, h->red_part, RED_BLOCKS, st);
+    HIPCHK(h, hipMemcpyAsync(h->hred, h->red_dst, (2 + xs.size()) * sizeof(double2),
+                             hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+int diis_end(rcc_t* h, double* rms_out) {
+    const int64_t n = h->nt1 + h->nt2;
+    hipStream_t st = h->st;
+    const int md = h->maxdim;
+    const int c = h->itc - 1;
+    const double2* sn = h->hred;
+    const double rms = std::sqrt(std::fabs(sn[1].x - (sn[0].x * sn[0].x + sn[0].y * sn[0].y)));
+    std::vector<double2> dots(h->hred + 2, h->hred + 2 + h->it);
     const int ld = md + 1;
     auto BB2 = [&](int i, int j) -> double& { return h->bb2[i + (int64_t)j * ld]; };
     for (int k = 0; k < h->it; ++k) {
@@ -821,15 +837,37 @@ int do_diis(rcc_t* h, double* rms_out) {
     return 0;
 }
 
+int do_diis(rcc_t* h, double* rms_out) {
+    CK(diis_begin(h));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return diis_end(h, rms_out);
+}
+
 // CCENG (cceqns.F:2180-2270) on S: Re[zdotc(FVO,S1) + zdotc(VVOO,tau(S))]
-int do_energy(rcc_t* h, double* ecc, double* eimag) {
+// CCENG (cceqns.F:2180-2270) on S, split like DIISX around one synchronisation.
+int energy_begin(rcc_t* h) {
     k_gettau(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, h->st);
     h->tau_ok = false;  // TAU now holds tau(S), not tau(T)
-    std::vector<double2> a, b;
-    CK(zdot_multi(h, {h->FVO}, h->S1(), h->nt1, a));
-    CK(zdot_multi(h, {h->cls[RCC_VVOO]}, h->TAU, h->nt2, b));
-    if (ecc) *ecc = a[0].x + b[0].x;
-    if (eimag) *eimag = a[0].y + b[0].y;
+    const double2* fvo = h->FVO;
+    const double2* vvoo = h->cls[RCC_VVOO];
+    k_zdotc_multi(&fvo, 1, h->S1(), h->nt1, h->red_dst + RED_E, h->red_part, RED_BLOCKS, h->st);
+    k_zdotc_multi(&vvoo, 1, h->TAU, h->nt2, h->red_dst + RED_E + 1, h->red_part, RED_BLOCKS,
+                  h->st);
+    HIPCHK(h, hipMemcpyAsync(h->hred + RED_E, h->red_dst + RED_E, 2 * sizeof(double2),
+                             hipMemcpyDeviceToHost, h->st));
+    return 0;
+}
+
+void energy_end(rcc_t* h, double* ecc, double* eimag) {
+    const double2 a = h->hred[RED_E], b = h->hred[RED_E + 1];
+    if (ecc) *ecc = a.x + b.x;
+    if (eimag) *eimag = a.y + b.y;
+}
+
+int do_energy(rcc_t* h, double* ecc, double* eimag) {
+    CK(energy_begin(h));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    energy_end(h, ecc, eimag);
     return 0;
 }
 
@@ -901,13 +939,23 @@ int terms_phase(rcc_t* h) {
 }
 
 // DIISX, CCENG, S -> T and the timing of the iteration (host-synchronous).
-int finish_phase(rcc_t* h, double* ecc, double* rms) {
-    CK(do_diis(h, rms));
-    CK(do_energy(h, ecc, nullptr));
+// The rest of an iteration in three stages separated by host synchronisation
+// of the stream (done by the caller, so a batch synchronises once per stage):
+// finish_a enqueues DIISX's reductions; finish_b solves DIISX, enqueues the
+// extrapolation, CCENG's reductions and S -> T; finish_c reads the energy.
+int finish_a(rcc_t* h) { return diis_begin(h); }
+
+int finish_b(rcc_t* h, double* rms) {
+    CK(diis_end(h, rms));
+    CK(energy_begin(h));
     HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
-    invalidate_t(h);
     HIPCHK(h, hipEventRecord(h->ev1, h->st));
-    HIPCHK(h, hipEventSynchronize(h->ev1));
+    return 0;
+}
+
+int finish_c(rcc_t* h, double* ecc) {
+    energy_end(h, ecc, nullptr);
+    invalidate_t(h);
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_iter_ms = ms;
@@ -924,7 +972,11 @@ int finish_phase(rcc_t* h, double* ecc, double* rms) {
 int do_iteration(rcc_t* h, double* ecc, double* rms) {
     HIPCHK(h, hipEventRecord(h->ev0, h->st));
     CK(terms_phase(h));
-    return finish_phase(h, ecc, rms);
+    CK(finish_a(h));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    CK(finish_b(h, rms));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return finish_c(h, ecc);
 }
 
 int alloc_all(rcc_t* h) {
@@ -981,7 +1033,8 @@ int alloc_all(rcc_t* h) {
                                                                64 * (int64_t)v * v + (int64_t)256 * o * o));
     CK(dalloc(h, &h->ws, h->ws_elems));
     CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
-    CK(dalloc(h, &h->red_dst, MAXDIM_CAP));
+    CK(dalloc(h, &h->red_dst, RED_SLOTS));
+    HIPCHK(h, hipHostMalloc((void**)&h->hred, RED_SLOTS * sizeof(double2), hipHostMallocDefault));
     return 0;
 }
 
@@ -1357,6 +1410,7 @@ void rcc_destroy(rcc_t* h) {
     if (!h) return;
     if (h->st) (void)hipStreamSynchronize(h->st);
     drop_graph(h);
+    if (h->hred) (void)hipHostFree(h->hred);
     for (void* p : h->allocs) (void)hipFree(p);
     for (int w = 0; w < 4; ++w)
         for (int p = 0; p < 2; ++p)
@@ -1668,12 +1722,14 @@ int rcc_iterate_batch(rcc_t* const* hs, int n, int* nit, double* ecc, double* rm
         HIPCHK(hs[i], hipEventRecord(hs[i]->ev0, hs[i]->st));
         CK(terms_phase(hs[i]));      // all systems' term phases in flight together
     }
+    // the host waits once per stage for all systems
+    for (int i = 0; i < n; ++i) CK(finish_a(hs[i]));
+    for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
+    for (int i = 0; i < n; ++i) CK(finish_b(hs[i], rms ? rms + i : nullptr));
+    for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
     for (int i = 0; i < n; ++i) {
-        double e = 0, r = 0;
-        CK(finish_phase(hs[i], &e, &r));
+        CK(finish_c(hs[i], ecc ? ecc + i : nullptr));
         if (nit) nit[i] = hs[i]->nit;
-        if (ecc) ecc[i] = e;
-        if (rms) rms[i] = r;
     }
     return 0;
 }

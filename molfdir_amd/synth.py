@@ -35,7 +35,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-__all__ = ["pair_index", "draw_factors", "make_system", "class_shapes", "spinor_irreps"]
+__all__ = ["pair_index", "draw_factors", "make_system", "class_shapes", "spinor_irreps",
+           "block_maps", "to_blocks", "dirichlet_split"]
 
 
 def pair_index(n: int) -> tuple[np.ndarray, np.ndarray]:
@@ -197,3 +198,90 @@ def make_system(no: int, nv: int, seed: int, rank: int = 120,
             continue
         out[name] = _build_class(B, t(pr), t(qr), t(rc), t(sc), True, dev, kb)
     return out
+
+
+def _pair_c1(x, y, n):
+    return y * (2 * n - y - 1) // 2 + (x - y - 1)
+
+
+def block_maps(sym):
+    """SYMTAB's symmetry-blocked orderings (ccdriv.F:1050-1187) for the
+    Z2^k double group of make_system(sym=...): for each kind ('oo', 'vv',
+    'vo', 'occ', 'vir'), one array per block of C1 indices (packed pair,
+    VO pair K*nv+A, or spinor).  Pair sub-blocks (i >= j, pair irrep c) are
+    ordered by j, J outer and I inner; VO sub-blocks by the occupied irrep,
+    K outer and A inner."""
+    no_l, nv_l = (list(map(int, x)) for x in sym)
+    n = len(no_l)
+    io = np.concatenate([[0], np.cumsum(no_l)])
+    iv = np.concatenate([[0], np.cumsum(nv_l)])
+    NO, NV = int(io[-1]), int(iv[-1])
+    g = np.arange(n)
+    ptab = np.bitwise_xor.outer(g, g)
+    partner = np.zeros((n, n), dtype=np.int64)      # partner[j, c]: i with ptab[i, j] = c
+    for i in range(n):
+        for j in range(n):
+            partner[j, ptab[i, j]] = i
+    maps = {"oo": [], "vv": [], "vo": [], "occ": [], "vir": []}
+    for c in range(n):
+        for key, cnt, off, tot in (("oo", no_l, io, NO), ("vv", nv_l, iv, NV)):
+            out = []
+            for jr in range(n):
+                ir = partner[jr, c]
+                if ir < jr:
+                    continue
+                for j in range(cnt[jr]):
+                    lo = j + 1 if ir == jr else 0
+                    i = np.arange(lo, cnt[ir])
+                    out.append(_pair_c1(off[ir] + i, off[jr] + j, tot))
+            maps[key].append(np.concatenate(out).astype(np.int64) if out else
+                             np.zeros(0, np.int64))
+        vo = []
+        for kr in range(n):
+            ar = partner[kr, c]
+            k = np.arange(no_l[kr])[:, None]
+            a = np.arange(nv_l[ar])[None, :]
+            vo.append(((io[kr] + k) * NV + iv[ar] + a).reshape(-1))
+        maps["vo"].append(np.concatenate(vo).astype(np.int64))
+        maps["occ"].append(np.arange(io[c], io[c + 1]))
+        maps["vir"].append(np.arange(iv[c], iv[c + 1]))
+    return maps
+
+
+_BLOCK_KINDS = {"FOO": ("occ", "occ"), "FVO": ("vir", "occ"), "T1": ("vir", "occ"),
+                "FVV": ("vir", "vir"), "OOOO": ("oo", "oo"), "VOOO": ("vo", "oo"),
+                "VVOO": ("vv", "oo"), "T2": ("vv", "oo"), "VOVO": ("vo", "vo"),
+                "VOVV": ("vo", "vv"), "VVVV": ("vv", "vv")}
+
+
+def to_blocks(name, x, sym, maps=None):
+    """C1 array of class / block kind `name` (flat, column-major, numpy or
+    torch) -> relccsd's symmetry-blocked layout: the blocks of every compound
+    irrep (or irrep), each column-major, concatenated."""
+    maps = maps or block_maps(sym)
+    rk, ck = _BLOCK_KINDS[name]
+    NO, NV = sum(map(int, sym[0])), sum(map(int, sym[1]))
+    nrows = {"oo": NO * (NO - 1) // 2, "vv": NV * (NV - 1) // 2, "vo": NO * NV,
+             "occ": NO, "vir": NV}[rk]
+    if isinstance(x, torch.Tensor):
+        X = x.reshape(-1, nrows) if nrows else x.reshape(0, 0)
+        parts = []
+        for r, c in zip(maps[rk], maps[ck]):
+            if len(r) and len(c):
+                rt = torch.as_tensor(r, device=x.device)
+                ct = torch.as_tensor(c, device=x.device)
+                parts.append(X.index_select(0, ct).index_select(1, rt).reshape(-1))
+        return torch.cat(parts) if parts else x.new_zeros(0)
+    X = np.asarray(x).reshape(-1, nrows) if nrows else np.asarray(x).reshape(0, 0)
+    parts = [X[np.ix_(c, r)].reshape(-1) for r, c in zip(maps[rk], maps[ck]) if len(r) and len(c)]
+    return np.concatenate(parts) if parts else np.zeros(0, np.asarray(x).dtype)
+
+
+def dirichlet_split(total: int, props) -> list:
+    """Integer counts summing to `total` in the given proportions (largest
+    remainders)."""
+    raw = np.asarray(props, dtype=np.float64) * total
+    out = np.floor(raw).astype(int)
+    for i in np.argsort(-(raw - out))[: total - int(out.sum())]:
+        out[i] += 1
+    return out.tolist()

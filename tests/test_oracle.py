@@ -225,3 +225,18 @@ def test_planted_mistake_in_oracle_is_caught(monkeypatch):
     monkeypatch.setattr(O, "gettau", bad_tau)
     o = O.terms(s, t1, t2)
     assert np.abs(o["S2_BINTM"] - g["S2_BINTM"]).max() > 1e-6
+
+
+@pytest.mark.parametrize("sym", [([2, 0, 1, 1, 2, 0, 1, 1], [4, 2, 3, 3, 4, 2, 3, 3]),
+                                 ([1, 0], [3, 2]), ([3, 1, 0, 2], [2, 5, 4, 0])])
+def test_product_block_conversion_matches_symtab(sym):
+    """molfdir_amd.synth.to_blocks (the product-side C1 -> symmetry-blocked
+    conversion bench.py uses) equals oracle/symtab.py's, for torch and numpy."""
+    from oracle.symtab import SymLayout
+    no, nv = sum(sym[0]), sum(sym[1])
+    s = synth.make_system(no, nv, seed=23, scale=0.1, sym=sym)
+    lay = SymLayout(*sym)
+    for n in ("FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        want = lay.to_sym(n, s[n].numpy())
+        assert np.array_equal(synth.to_blocks(n, s[n], sym).numpy(), want), n
+        assert np.array_equal(synth.to_blocks(n, s[n].numpy(), sym), want), n
