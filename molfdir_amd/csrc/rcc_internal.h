@@ -109,6 +109,17 @@ void k_scatter_block(const double2* src, int64_t rows, int64_t ncols, const int*
 void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int* colmap,
                     int64_t rows, int64_t ncols, double2* dst, hipStream_t st);
 void k_fill_zero(double2* p, int64_t n, hipStream_t st);
+// every block of a symmetry layout in one launch (gather: C1 -> blocks)
+struct BlockSet {
+    int nb;
+    struct Blk {
+        const int* rmap;
+        const int* cmap;
+        int64_t rows, cols, off;
+    } b[32];
+};
+void k_blockset(const double2* src, const BlockSet& bs, double2* dst, int64_t ld, bool gather,
+                hipStream_t st);
 
 // ---- split-BF16 operands and GEMM (rcc_split.hip) ----
 // Six BF16 planes (Re slices 1-3, Im slices 1-3), K-major, rows padded to 64

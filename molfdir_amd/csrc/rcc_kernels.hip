@@ -448,26 +448,45 @@ __global__ void __launch_bounds__(TPB) zdotc_part_kernel(PtrPack xs, int nx,
     }
 }
 
-__global__ void finish_kernel(const double2* __restrict__ part, int nblk, int nv,
-                              double2* __restrict__ dst) {
-    const int v = threadIdx.x;
-    if (v >= nv) return;
+// One workgroup per reduced value: strided partial sums then a tree, both in a
+// fixed order, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(TPB) finish_kernel(const double2* __restrict__ part, int nblk,
+                                                     int nv, double2* __restrict__ dst) {
+    __shared__ double sh[2][TPB];
+    const int v = blockIdx.x;
     double sr = 0.0, si = 0.0;
-    for (int b = 0; b < nblk; ++b) {
+    for (int b = threadIdx.x; b < nblk; b += TPB) {
         sr += part[v * nblk + b].x;
         si += part[v * nblk + b].y;
     }
-    dst[v] = make_double2(sr, si);
+    sh[0][threadIdx.x] = sr;
+    sh[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[v] = make_double2(sh[0][0], sh[1][0]);
+}
+
+// partial-sum blocks for n elements: about 4096 elements each, at most nblk
+static inline int red_blocks(int64_t n, int nblk) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (n + 4095) / 4096));
 }
 
 void k_zdotc_multi(const double2* const* xs, int nx, const double2* y, int64_t n, double2* dst,
                    double2* part, int nblk, hipStream_t st) {
+    if (nx <= 0) return;
     PtrPack pk{};
     for (int v = 0; v < nx && v < MAXV; ++v) pk.p[v] = xs[v];
+    nblk = red_blocks(n, nblk);
     const int64_t chunk = (n + nblk - 1) / nblk;
     hipLaunchKernelGGL(zdotc_part_kernel, dim3(nblk), dim3(TPB), 0, st, pk, nx, y, n,
                        std::max<int64_t>(chunk, 1), part, nblk);
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, part, nblk, nx, dst);
+    hipLaunchKernelGGL(finish_kernel, dim3(nx), dim3(TPB), 0, st, part, nblk, nx, dst);
 }
 
 // dst[0] = sum x, dst[1] = (sum |x|^2, 0)
@@ -504,10 +523,11 @@ __global__ void __launch_bounds__(TPB) sumnorm_part_kernel(const double2* __rest
 
 void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, int nblk,
                     hipStream_t st) {
+    nblk = red_blocks(n, nblk);
     const int64_t chunk = (n + nblk - 1) / nblk;
     hipLaunchKernelGGL(sumnorm_part_kernel, dim3(nblk), dim3(TPB), 0, st, x, n,
                        std::max<int64_t>(chunk, 1), part, nblk);
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, part, nblk, 2, dst);
+    hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(TPB), 0, st, part, nblk, 2, dst);
 }
 
 __global__ void lincomb_kernel(CoefPack cp, int nx, double2* __restrict__ y, int64_t n) {
@@ -574,6 +594,29 @@ void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int
     if (n <= 0) return;
     hipLaunchKernelGGL(gather_block_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, src, ld, rowmap,
                        colmap, rows, n, dst);
+}
+
+// All blocks of a layout in one launch: blockIdx.y = block.
+__global__ void scatter_set_kernel(const double2* __restrict__ src, BlockSet bs,
+                                   double2* __restrict__ dst, int64_t ld, int gather) {
+    const BlockSet::Blk& b = bs.b[blockIdx.y];
+    const int64_t n = b.rows * b.cols;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % b.rows, c = e / b.rows;
+        const int64_t c1 = (int64_t)b.rmap[r] + ld * b.cmap[c];
+        if (gather) dst[b.off + e] = src[c1];
+        else dst[c1] = src[b.off + e];
+    }
+}
+
+void k_blockset(const double2* src, const BlockSet& bs, double2* dst, int64_t ld, bool gather,
+                hipStream_t st) {
+    int64_t mx = 0;
+    for (int i = 0; i < bs.nb; ++i) mx = std::max<int64_t>(mx, bs.b[i].rows * bs.b[i].cols);
+    if (bs.nb <= 0 || mx <= 0) return;
+    hipLaunchKernelGGL(scatter_set_kernel, dim3(nblocks(mx), bs.nb), dim3(TPB), 0, st, src, bs,
+                       dst, ld, gather ? 1 : 0);
 }
 
 void k_fill_zero(double2* p, int64_t n, hipStream_t st) {

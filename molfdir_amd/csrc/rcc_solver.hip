@@ -1206,26 +1206,32 @@ int sym_to_c1(rcc_t* h, int lay, const void* src, hipMemcpyKind kind, double2* d
 }
 
 // C1 array <-> whole symmetry-layout array, both on the device.
-void c1_to_sym_dev(rcc_t* h, int lay, const double2* src, double2* dst) {
+BlockSet block_set(const rcc_t* h, int lay, int64_t* ld) {
     int rk, ck;
-    int64_t ld;
-    lay_spec(h, lay, &rk, &ck, &ld);
+    lay_spec(h, lay, &rk, &ck, ld);
     const SymInfo& S = h->sym;
-    for (int b = 0; b < S.nrep; ++b)
-        k_gather_block(src, ld, S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
-                       lay_block_rows(h, rk, b), lay_block_rows(h, ck, b),
-                       dst + lay_offset(h, rk, ck, b), h->st);
+    BlockSet bs{};
+    bs.nb = S.nrep;
+    for (int b = 0; b < S.nrep; ++b) {
+        bs.b[b].rmap = S.dmap + S.doff[rk][b];
+        bs.b[b].cmap = S.dmap + S.doff[ck][b];
+        bs.b[b].rows = lay_block_rows(h, rk, b);
+        bs.b[b].cols = lay_block_rows(h, ck, b);
+        bs.b[b].off = lay_offset(h, rk, ck, b);
+    }
+    return bs;
+}
+
+void c1_to_sym_dev(rcc_t* h, int lay, const double2* src, double2* dst) {
+    int64_t ld;
+    const BlockSet bs = block_set(h, lay, &ld);
+    k_blockset(src, bs, dst, ld, true, h->st);
 }
 
 void sym_to_c1_dev(rcc_t* h, int lay, const double2* src, double2* dst) {
-    int rk, ck;
     int64_t ld;
-    lay_spec(h, lay, &rk, &ck, &ld);
-    const SymInfo& S = h->sym;
-    for (int b = 0; b < S.nrep; ++b)
-        k_scatter_block(src + lay_offset(h, rk, ck, b), lay_block_rows(h, rk, b),
-                        lay_block_rows(h, ck, b), S.dmap + S.doff[rk][b], S.dmap + S.doff[ck][b],
-                        dst, ld, h->st);
+    const BlockSet bs = block_set(h, lay, &ld);
+    k_blockset(src, bs, dst, ld, false, h->st);
 }
 
 int64_t lay_total(const rcc_t* h, int lay) {
