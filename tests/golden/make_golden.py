@@ -18,6 +18,10 @@ change in the generator is caught.
   symmetry of an abelian double group (2, 4 or 8 fermion irreps, Z2^k
   products); every array is in relccsd's symmetry-blocked layout (SYMTAB,
   ccdriv.F:746-1425), the per-irrep occupied / virtual counts are stored.
+* lambda_<name>.npz: converged CCSD energies of the reference at f and at
+  f +- h V for a seeded Hermitian one-electron perturbation V (stored), the
+  finite-field gradient d Ec / d lam the CPU path gives (relccsd's own
+  Lambda / density code is switched off, ccdriv.F:1918-1924).
 """
 import os
 import sys
@@ -58,6 +62,15 @@ SYM_TERM_CASES = [
 ]
 
 
+LAMBDA_CASES = [
+    # name, no, nv, seed, scale, fock_perturb, V seed, h, maxit, maxdim, ntol
+    ("lam_3o8v", 3, 8, 11, 0.05, 0.02, 5, 1e-4, 80, 8, 13),
+    ("lam_4o12v", 4, 12, 5, 0.03, 0.03, 6, 1e-4, 80, 8, 13),
+    ("lam_5o16v", 5, 16, 2, 0.02, 0.0, 7, 1e-4, 80, 8, 13),
+    ("lam_6o40v", 6, 40, 1, 0.02, 0.01, 8, 1e-4, 80, 8, 13),
+]
+
+
 def symmetric_amplitudes(lay, seed):
     """Random T1/T2 with the symmetry of the layout, returned in C1 layout."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -84,10 +97,42 @@ def random_amplitudes(no, nv, seed):
     return t1, t2
 
 
+def lambda_case(name, no, nv, seed, scale, fp, vseed, h, maxit, maxdim, ntol):
+    from oracle import lambda_oracle as lo
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+    V = lo.hermitian_direction(no + nv, vseed)
+    out = {}
+    for key, lam in (("e0", 0.0), ("e_plus", h), ("e_minus", -h), ("e_plus2", 2 * h),
+                     ("e_minus2", -2 * h)):
+        r = refrun.run(lo.perturbed(s, V, lam) if lam else s, "ccsd", maxit=maxit,
+                       maxdim=maxdim, ntol=ntol)
+        it = np.asarray(r["iterations"], dtype=np.float64)
+        if r["nit"] >= maxit or it[-1, 2] >= 10.0 ** -ntol:
+            raise SystemExit(f"{name}: reference not converged at lam={lam}")
+        out[key] = r["eccsd"]
+        out["nit_" + key] = r["nit"]
+        if key == "e0":
+            out["T1"], out["T2"] = r["T1"], r["T2"]
+    fd = (out["e_plus"] - out["e_minus"]) / (2 * h)
+    # fourth-order central difference (Richardson on h, 2h)
+    fd4 = (8 * (out["e_plus"] - out["e_minus"]) - (out["e_plus2"] - out["e_minus2"])) / (12 * h)
+    np.savez_compressed(
+        os.path.join(HERE, f"lambda_{name}.npz"),
+        params=np.array([no, nv, seed, vseed, maxit, maxdim, ntol], dtype=np.int64),
+        scale=scale, fock_perturb=fp, h=h, V=V, fingerprint=fingerprint(s), fd=fd, fd4=fd4,
+        **out)
+    print(name, out["e0"], fd, fd4, out["nit_e0"])
+
+
 def main():
     if not refrun.available():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
+    only_lam = "--lambda-only" in sys.argv
+    for case in LAMBDA_CASES:
+        lambda_case(*case)
+    if only_lam:
+        return
     for name, no, nv, seed, scale, fp, maxit, maxdim, ntol in ([] if only_sym else CCSD_CASES):
         s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
         r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol)
