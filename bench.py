@@ -1,6 +1,8 @@
 """Benchmark: relativistic CCSD iteration on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
+    python bench.py --config cfg4        # 64 small symmetric systems, batched
+    python bench.py --config cfg5        # Lambda equations + response density
 
 A step is one CCEQN iteration (HOINTM .. BINTM, DENOM, DIIS, CCENG, S->T)
 over the whole synthetic system with every integral class resident in HBM.
@@ -28,6 +30,7 @@ CONFIGS = {
     "cfg3": (30, 300, 3, "C1 30o/300v complex128"),
 }
 BATCH = {"cfg4": (64, 8, 4)}   # systems, irreps, seed of the size / proportion draws
+LAMBDA = {"cfg5": (20, 200, 5, "C1 20o/200v complex128")}
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
@@ -46,6 +49,113 @@ def ccsd_flops(no: int, nv: int) -> float:
           + 2 * v ** 3 * noot + 2 * o ** 3 * nvvt + 4 * noot * v * nvvt + 4 * nvvt * o * ov
           + v * v * o * noot + o ** 3 * v * v)
     return 8.0 * cf
+
+
+def lambda_flops(no: int, nv: int) -> float:
+    """Real flops of one Lambda iteration (complex FMA = 8 flops): the
+    products of the adjoint sweep -- ladder^T, tau.A (two), VOVV^T X, the
+    three (ov)^3 ring adjoints, GV.F1 (two) and HVINTM, the T2.GO pair and
+    HOINTM, the OVVV.T1 adjoints (VOVV_Q^T, conj VOVV), AINTM, HINTM's MH
+    and the small T1EQNS / GO / GV products."""
+    o, v = no, nv
+    noot, nvvt, ov = o * (o - 1) // 2, v * (v - 1) // 2, o * v
+    cf = (nvvt ** 2 * noot + 3 * nvvt * noot ** 2 + nvvt * noot * ov + 4 * noot * v * v * o
+          + o * o * v * ov + 3 * v ** 3 * noot + o * o * v * nvvt + 3 * nvvt * o ** 3
+          + 3 * ov ** 3 + v * v * o * noot + v * o * o * ov + v * v * o * ov
+          + o * o * v * noot + 8 * ov ** 2)
+    return 8.0 * cf
+
+
+def bench_lambda(args):
+    """CCSD, then the Lambda equations and the response density on one C1
+    system; the analytic gradient tr(D V) along a seeded Hermitian V against
+    central differences (h = 1e-4) of converged CCSD energies."""
+    import numpy as np
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.autograd import PerturbedCCSD
+    no, nv, seed, desc = LAMBDA[args.config]
+    dev = torch.device("cuda:0")
+    n = no + nv
+    prob = None
+    for sc in [args.scale] + [x for x in (0.02, 0.01, 0.005) if x < args.scale]:
+        sysd = synth.make_system(no, nv, seed=seed, scale=sc, device=dev)
+        prob = PerturbedCCSD(sysd, maxit=args.lambda_maxit, maxdim=8, ntol=12)
+        del sysd
+        try:
+            t0 = time.perf_counter()
+            e0 = prob.energy(np.zeros((n, n)))
+            t_cc = time.perf_counter() - t0
+            scale = sc
+            break
+        except RuntimeError as exc:
+            print(f"[bench] scale {sc}: {exc}", file=sys.stderr, flush=True)
+            prob.r.close()
+            prob = None
+            torch.cuda.empty_cache()
+    if prob is None:
+        raise SystemExit("bench.py: CCSD did not converge at any factor scale")
+    r = prob.r
+    cc_nit = prob.last["nit"]
+    print(f"[bench] CCSD converged: scale {scale} Ec {e0:.12f} nit {cc_nit} "
+          f"({t_cc:.2f} s)", file=sys.stderr, flush=True)
+    r.synchronize()
+    t0 = time.perf_counter()
+    lam_nit, lam_rms = r.lambda_solve()
+    r.synchronize()
+    t_lam = time.perf_counter() - t0
+    lam_ms = r.last_lambda_ms()
+    kt = r.kernel_times()
+    kms = [kt["ladder"], kt["ring_hintm"], kt["ring_t2"], kt["vovv_tau"]]
+    t0 = time.perf_counter()
+    D = r.density()
+    t_dens = time.perf_counter() - t0
+    print(f"[bench] Lambda nit {lam_nit} rms {lam_rms:.2e} last {lam_ms:.2f} ms, total "
+          f"{t_lam:.2f} s; density {t_dens:.2f} s", file=sys.stderr, flush=True)
+    V = synth.hermitian_perturbation(n, seed + 100, 0.01)
+    g = float(np.real(np.trace(D @ V)))
+    h = 1e-4
+    ep = prob.energy(h * V)
+    em = prob.energy(-h * V)
+    fd = (ep - em) / (2 * h)
+    rel = abs(g - fd) / abs(fd)
+    # fourth-order difference from +-2h as well: separates the O(h^2) error
+    # of the central difference from that of the analytic gradient
+    ep2 = prob.energy(2 * h * V)
+    em2 = prob.energy(-2 * h * V)
+    fd4 = (8 * (ep - em) - (ep2 - em2)) / (12 * h)
+    rel4 = abs(g - fd4) / abs(fd4)
+    print(f"[bench] gradient {g:.12e} fd {fd:.12e} rel {rel:.2e}; fd4 {fd4:.12e} rel {rel4:.2e}",
+          file=sys.stderr, flush=True)
+    noot, nvvt = no * (no - 1) // 2, nv * (nv - 1) // 2
+    F = lambda_flops(no, nv)
+    ladder = 8.0 * nvvt * nvvt * noot
+    ach = ladder / (kms[0] * 1e-3) / 1e12 if kms[0] > 0 else None
+    out = {
+        "metric": f"CCSD Lambda s/iter + relative gradient error at {no}o/{nv}v cplx",
+        "value": lam_ms * 1e-3, "unit": "s/iter", "n_gpus": 1, "steps": lam_nit, "warmup": 0,
+        "ms_per_step": lam_ms, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": (f"synthetic (seed {seed}; factor Re/Im N(0,{scale}^2), the largest of "
+                 f"{args.scale}, 0.02, 0.01, 0.005 at which CCSD converges; V seed {seed + 100}, "
+                 f"N(0,0.01^2) Hermitian)"),
+        "config": {"workload": f"Lambda equations + response density, {desc}", "factor_std": scale},
+        "lambda_tflops": F / (lam_ms * 1e-3) / 1e12, "lambda_flops_per_iter": F,
+        "lambda": {"nit": lam_nit, "rms": lam_rms, "total_s": t_lam, "density_s": t_dens},
+        "ccsd": {"ecc": e0, "nit": cc_nit, "total_s": t_cc},
+        "gradient": {"analytic": g, "central_fd": fd, "h": h, "rel_error": rel,
+                     "fd_fourth_order": fd4, "rel_error_fourth_order": rel4,
+                     "e_plus": ep, "e_minus": em},
+        "roofline": {"bound": "mfma", "kernel": "Lambda ladder VVVV^T S2 (zgemm T,N)",
+                     "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": (ach / PEAK_FP64_MFMA) if ach else None, "traffic": None},
+        "kernels_ms": {"ladder_adj": kms[0], "hintm_ring_adj": kms[1],
+                       "t2eqns_ring_adj_pair": kms[2], "vovv_tau_adj": kms[3]},
+        "cpu_baseline": None,
+        "cpu_baseline_note": "relccsd's Lambda code is switched off (ccdriv.F:1918-1924); its "
+                             "CPU gradient is finite-field CCSD (tests/golden/lambda_*.npz)",
+    }
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(workload_flops: float):
@@ -249,7 +359,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS) + sorted(BATCH))
+    ap.add_argument("--config", default="cfg3",
+                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA))
+    ap.add_argument("--lambda-maxit", type=int, default=80,
+                    help="cfg5: MAXIT of the CCSD and Lambda solves")
     ap.add_argument("--scale", type=float, default=0.05,
                     help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -282,6 +395,9 @@ def main():
 
     if args.config in BATCH:
         bench_batch(args)
+        return
+    if args.config in LAMBDA:
+        bench_lambda(args)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8

@@ -151,7 +151,6 @@ int rcc_accept(rcc_t* h);
  * CCEQN loop body (cceqns.F:293-569).  nit counts calls since rcc_mp2. */
 int rcc_iterate(rcc_t* h, int* nit, double* ecc, double* rms);
 
-/* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
 /* Replay the term sequence of rcc_iterate / rcc_cceqn / rcc_iterate_batch
  * (HOINTM .. BINTM, DENOM: all device work) from a hipGraph: the first
  * iteration runs eagerly, the second is captured, later ones are one graph
@@ -164,7 +163,38 @@ int rcc_set_graph(rcc_t* h, int enable);
  * still on the device.  nit/ecc/rms receive one value per handle.  No
  * counterpart in the reference (which runs one system per process). */
 int rcc_iterate_batch(rcc_t* const* handles, int n, int* nit, double* ecc, double* rms);
+/* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);
+
+/* ---- CCSD Lambda equations and the one-particle response density.
+ * relccsd holds Lambda branches (CCEQN cceqns.F:277-288, T1EQNS :860-932,
+ * the Lambda-only routines of ccintm.F) but switches them off
+ * (ccdriv.F:1918-1924); these entry points have no live counterpart.  They
+ * solve the stationarity conditions of the CCSD Lagrangian
+ *     L = E(T) + Lam . (S(T) - D o T),
+ * S the pre-DENOM right-hand side of one CCEQN pass, D the DENOM
+ * denominators, E CCENG's complex sum:  Lam = (dE/dT + Lam . dS/dT) / D,
+ * iterated with DENOM, ZCORE and DIISX; Lam . dS/dT is the adjoint sweep of
+ * the iteration's own contractions at the handle's T.  C1 handles with
+ * precision 0.  Lam is in the T1 / T2 layouts. */
+/* Solve for Lam at the current T (normally after rcc_cceqn) with the
+ * handle's MAXIT / MAXDIM / NTOL; nit and rms of the last step.  Clears the
+ * DIIS history (a later rcc_cceqn restarts it). */
+int rcc_lambda(rcc_t* h, int* nit, double* rms);
+int rcc_set_lambda(rcc_t* h, const double* l1, const double* l2);
+int rcc_get_lambda(rcc_t* h, double* l1, double* l2);
+/* One adjoint sweep at the current T for the given Lam (no DENOM):
+ * g = with_energy * dE/dT + Lam . dS/dT. */
+int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy, double* g1,
+                   double* g2);
+/* One-particle response density of the current (T, Lam): dens is n x n
+ * complex (n = no + nv, occupied then virtual, column-major), Hermitian, with
+ * d Re(Ec) / d lam = tr(dens V) for the one-electron perturbation
+ * f -> f + lam V (FOO, FVO, FVV and the orbital energies shifted by the
+ * blocks and the diagonal of the Hermitian V). */
+int rcc_density(rcc_t* h, double* dens);
+/* Device time of the last rcc_lambda iteration, ms. */
+int rcc_last_lambda_ms(rcc_t* h, double* ms);
 
 /* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms. */
 int rcc_last_iteration_ms(rcc_t* h, double* ms);
@@ -174,7 +204,8 @@ int rcc_last_iteration_ms(rcc_t* h, double* ms);
 int rcc_kernel_times(rcc_t* h, double* ms4);
 
 /* Diagnostics: the library's complex GEMM engine on device pointers
- * (column-major, op 0=N 1=T 2=C), and the MFMA lane-layout probe. */
+ * (column-major, op 0=N 1=T 2=C 3=conjugate without transpose, 3 only with 0
+ * on the other operand), and the MFMA lane-layout probe. */
 int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb,
                      const double* beta, void* C, int64_t ldc);

@@ -5,7 +5,9 @@
 
 namespace rcc {
 
-enum { OP_N = 0, OP_T = 1, OP_C = 2 };
+// op(X): N as stored, T transposed, C conjugate-transposed, R conjugated (not
+// transposed; the adjoint contractions of the Lambda equations use it)
+enum { OP_N = 0, OP_T = 1, OP_C = 2, OP_R = 3 };
 
 // Strided-batched complex GEMM, column-major: C_b = alpha op(A_b) op(B_b) + beta C_b.
 struct GemmArgs {
@@ -73,9 +75,9 @@ void k_pack_pij(const double2* Y, double2* S, int64_t NR, double alpha, PairTab 
 // S[pair(a,b), col] += alpha * (Y[a + nv*(b + nv*col)] - Y[b + nv*(a + nv*col)])
 void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTab vv, int nv,
                 hipStream_t st);
-// S[ab, ij] += Q[ai,bj] - Q[bi,aj] - Q[aj,bi] + Q[bj,ai]   (ring layout Q)
+// S[ab, ij] += alpha (Q[ai,bj] - Q[bi,aj] - Q[aj,bi] + Q[bj,ai])   (ring layout Q)
 void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
-                  hipStream_t st);
+                  hipStream_t st, double alpha = 1.0);
 // H[(a,i),(c,k)] (+)= X[(a,k),(c,i)]
 void k_perm_akci(const double2* X, double2* H, int nv, int no, bool accumulate,
                  hipStream_t st);
@@ -83,6 +85,15 @@ void k_perm_akci(const double2* X, double2* H, int nv, int no, bool accumulate,
 void k_trace_t1eqn2(const double2* X, double2* S1, PairTab oo, int nv, int no, hipStream_t st);
 // GV[a,c] += alpha * sum_k H1[(a,k),(c,k)]   (H1 in (AK,CI) layout)
 void k_trace_gv(const double2* H1, double2* GV, double alpha, int nv, int no, hipStream_t st);
+// transposes (adjoints) of k_trace_t1eqn2 and k_trace_gv:
+// Xb[(a,y), pair(x,y)] += S1b[a,x] and Xb[(a,x), pair(x,y)] -= S1b[a,y]  (x > y)
+void k_trace_t1eqn2_adj(const double2* S1b, double2* Xb, PairTab oo, int nv, int no,
+                        hipStream_t st);
+// H1b[(a,k),(c,k)] += alpha * GVb[a,c]
+void k_trace_gv_adj(const double2* GVb, double2* H1b, double alpha, int nv, int no,
+                    hipStream_t st);
+// z = x * y element-wise (complex)
+void k_hadamard(int64_t n, const double2* x, const double2* y, double2* z, hipStream_t st);
 // eps == nullptr: only the frozen-spinor zeroing (ZCORE); frozen == nullptr: none
 void k_denom(double2* S1, double2* S2, const double* eps, const int* frozen, PairTab vv,
              PairTab oo, int nv, int no, hipStream_t st);

@@ -214,7 +214,8 @@ void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTa
 
 __global__ void pack_pabij_kernel(const double2* __restrict__ Q, double2* __restrict__ S,
                                   const int2* __restrict__ vv, int64_t nvvt,
-                                  const int2* __restrict__ oo, int64_t noot, int nv, int no) {
+                                  const int2* __restrict__ oo, int64_t noot, int nv, int no,
+                                  double alpha) {
     const int64_t nvo = (int64_t)nv * no;
     GRID_LOOP(e, nvvt * noot) {
         const int2 p = vv[e % nvvt], q = oo[e / nvvt];
@@ -223,18 +224,18 @@ __global__ void pack_pabij_kernel(const double2* __restrict__ Q, double2* __rest
         const double2 v1 = Q[ai + nvo * bj], v2 = Q[bi + nvo * aj];
         const double2 v3 = Q[aj + nvo * bi], v4 = Q[bj + nvo * ai];
         double2 s = S[e];
-        s.x += v1.x - v2.x - v3.x + v4.x;
-        s.y += v1.y - v2.y - v3.y + v4.y;
+        s.x += alpha * (v1.x - v2.x - v3.x + v4.x);
+        s.y += alpha * (v1.y - v2.y - v3.y + v4.y);
         S[e] = s;
     }
 }
 
 void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
-                  hipStream_t st) {
+                  hipStream_t st, double alpha) {
     const int64_t n = vv.npair * oo.npair;
     if (n == 0) return;
     hipLaunchKernelGGL(pack_pabij_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, Q, S, vv.tab,
-                       vv.npair, oo.tab, oo.npair, nv, no);
+                       vv.npair, oo.tab, oo.npair, nv, no, alpha);
 }
 
 // ---------------------------------------------------------------- ring permutation
@@ -306,6 +307,66 @@ void k_trace_gv(const double2* H1, double2* GV, double alpha, int nv, int no, hi
     if (n == 0) return;
     hipLaunchKernelGGL(trace_gv_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, H1, GV, alpha, nv,
                        no);
+}
+
+// ---------------------------------------------------------------- adjoints of the traces
+// Transpose of trace_t1eqn2: Xb[(a,y), p] += S1b[a,x], Xb[(a,x), p] -= S1b[a,y]
+// for p = (x > y).  Each X element is written by one thread.
+__global__ void trace_t1eqn2_adj_kernel(const double2* __restrict__ S1b, double2* __restrict__ Xb,
+                                        const int2* __restrict__ oo, int64_t noot, int nv,
+                                        int no) {
+    const int64_t nvo = (int64_t)nv * no;
+    GRID_LOOP(e, (int64_t)nv * noot) {
+        const int a = (int)(e % nv);
+        const int64_t p = e / nv;
+        const int2 q = oo[p];
+        double2* xy = Xb + (a + (int64_t)nv * q.y) + nvo * p;
+        double2* xx = Xb + (a + (int64_t)nv * q.x) + nvo * p;
+        *xy = cadd(*xy, S1b[a + (int64_t)nv * q.x]);
+        *xx = csub(*xx, S1b[a + (int64_t)nv * q.y]);
+    }
+}
+
+void k_trace_t1eqn2_adj(const double2* S1b, double2* Xb, PairTab oo, int nv, int no,
+                        hipStream_t st) {
+    const int64_t n = (int64_t)nv * oo.npair;
+    if (n == 0) return;
+    hipLaunchKernelGGL(trace_t1eqn2_adj_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, S1b, Xb,
+                       oo.tab, oo.npair, nv, no);
+}
+
+// Transpose of trace_gv: H1b[(a,k),(c,k)] += alpha * GVb[a,c] for every k.
+__global__ void trace_gv_adj_kernel(const double2* __restrict__ GVb, double2* __restrict__ H1b,
+                                    double alpha, int nv, int no) {
+    const int64_t nvo = (int64_t)nv * no;
+    const int64_t nv2 = (int64_t)nv * nv;
+    GRID_LOOP(e, nv2 * no) {
+        const int a = (int)(e % nv);
+        const int c = (int)((e / nv) % nv);
+        const int k = (int)(e / nv2);
+        double2* h = H1b + (a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * k);
+        const double2 g = GVb[a + (int64_t)nv * c];
+        h->x += alpha * g.x;
+        h->y += alpha * g.y;
+    }
+}
+
+void k_trace_gv_adj(const double2* GVb, double2* H1b, double alpha, int nv, int no,
+                    hipStream_t st) {
+    const int64_t n = (int64_t)nv * nv * no;
+    if (n == 0) return;
+    hipLaunchKernelGGL(trace_gv_adj_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, GVb, H1b, alpha,
+                       nv, no);
+}
+
+__global__ void hadamard_kernel(int64_t n, const double2* __restrict__ x,
+                                const double2* __restrict__ y, double2* __restrict__ z) {
+    GRID_LOOP(i, n) z[i] = cmul(x[i], y[i]);
+}
+
+void k_hadamard(int64_t n, const double2* x, const double2* y, double2* z, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(hadamard_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, n, x, y, z);
 }
 
 // ---------------------------------------------------------------- denominators

@@ -145,6 +145,19 @@ struct rcc_handle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 
+    // Lambda equations (C1, FP64): LAM the current Lambda, LAMS the next one
+    // (and the output of the adjoint sweep), b* the adjoints of the
+    // iteration's intermediates, LW / LB2 the T1EQNS products W = HOV -
+    // 2 conj(FVO) and W^T T1 at the converged T
+    double2 *LAM = nullptr, *LAMS = nullptr;
+    double2 *bTAU = nullptr, *bX = nullptr, *bWC = nullptr, *bRING = nullptr, *bRING2 = nullptr,
+            *bH = nullptr, *bE2 = nullptr, *bF1 = nullptr, *bP = nullptr, *ones = nullptr;
+    double2 *bHO = nullptr, *bHV = nullptr, *bHOV = nullptr, *bGO = nullptr, *bGV = nullptr,
+            *bAZ = nullptr, *bB2 = nullptr, *bW = nullptr, *bFVOc = nullptr, *LW = nullptr,
+            *LB2 = nullptr, *bPT2 = nullptr;
+    bool lam_alloc = false, lam_set = false;
+    double last_lambda_ms = 0.0;
+
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
     double2* S1() { return S; }
@@ -770,7 +783,9 @@ double flin(double* A, int idim, int n, int nr) {
 // asynchronous copy of the results to pinned host memory; diis_end (after the
 // stream has drained) forms the B matrix, solves it (FLIN) and enqueues the
 // extrapolation.
-int diis_begin(rcc_t* h) {
+int diis_begin(rcc_t* h, const double2* cur = nullptr, const double2* nw = nullptr) {
+    if (!cur) cur = h->T;
+    if (!nw) nw = h->S;
     const int64_t n = h->nt1 + h->nt2;
     hipStream_t st = h->st;
     const int md = h->maxdim;
@@ -780,9 +795,9 @@ int diis_begin(rcc_t* h) {
     const int c = h->itc - 1;
     double2* d = h->diis_err[c];
     // d = S - T ; store S
-    HIPCHK(h, hipMemcpyAsync(d, h->S, n * 16, hipMemcpyDeviceToDevice, st));
-    k_axpby(n, cplx(-1, 0), h->T, cplx(1, 0), d, st);
-    HIPCHK(h, hipMemcpyAsync(h->diis_amp[c], h->S, n * 16, hipMemcpyDeviceToDevice, st));
+    HIPCHK(h, hipMemcpyAsync(d, nw, n * 16, hipMemcpyDeviceToDevice, st));
+    k_axpby(n, cplx(-1, 0), cur, cplx(1, 0), d, st);
+    HIPCHK(h, hipMemcpyAsync(h->diis_amp[c], nw, n * 16, hipMemcpyDeviceToDevice, st));
     // RMS = sqrt(| sum |d|^2 - |sum d|^2 |)
     k_sum_and_norm(d, n, h->red_dst, h->red_part, RED_BLOCKS, st);
     // B row: Re + Im of <d_k | d_ITC>
@@ -794,7 +809,8 @@ int diis_begin(rcc_t* h) {
     return 0;
 }
 
-int diis_end(rcc_t* h, double* rms_out) {
+int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr) {
+    if (!nw) nw = h->S;
     const int64_t n = h->nt1 + h->nt2;
     hipStream_t st = h->st;
     const int md = h->maxdim;
@@ -832,7 +848,7 @@ int diis_end(rcc_t* h, double* rms_out) {
     for (int i = 0; i < it; ++i) coef[i] = BB(i, it + 1);
     std::vector<const double2*> amps;
     for (int k = 0; k < it; ++k) amps.push_back(h->diis_amp[k]);
-    k_lincomb(amps.data(), coef.data(), it, h->S, n, st);
+    k_lincomb(amps.data(), coef.data(), it, nw, n, st);
     if (rms_out) *rms_out = rms;
     return 0;
 }
@@ -1338,6 +1354,399 @@ int c1_to_sym_host(rcc_t* h, int lay, const double2* src, double* out) {
     return 0;
 }
 
+// ------------------------------------------------------------------ Lambda
+//
+// relccsd's Lambda branches (CCEQN cceqns.F:277-288, T1EQNS :860-932 and the
+// Lambda-only routines of ccintm.F) are switched off (ccdriv.F:1918-1924).
+// The equations solved here are those of the CCSD Lagrangian
+//     L(T, Lam) = E(T) + Lam . R(T),   R(T) = S(T) - D o T,
+// with S the pre-DENOM right-hand side of one CCEQN pass (the term sequence
+// above), D the DENOM denominators and E CCENG's complex sum zdotc(FVO, T1)
+// + zdotc(VVOO, tau).  Integrals and Fock enter E and S conjugated or not,
+// T never, so stationarity in T is holomorphic:
+//     Lam = (dE/dT + Lam . dS/dT) / D,
+// iterated like CCEQN (DENOM, ZCORE, DIISX).  Lam . dS/dT is the reverse
+// (adjoint) sweep of the term sequence at the converged T: a contraction
+// C += op(A) op(B) whose B depends on T sends Bbar += op(A)^T Cbar (and the
+// same for A); each antisymmetrising pack turns into the matching expansion
+// and back:
+//     pack_pij^T = -expand_e2      expand_e2^T = pack_pij(-1)
+//     pack_pab^T = expand_f1       expand_f1^T = pack_pab
+//     pack_pabij^T = expand_ring   expand_ring^T = pack_pabij
+//     perm_akci^T = perm_akci      trace_*^T = trace_*_adj.
+// dE/dT1 = conj(FVO) + conj(VVOO_R) T1 = HOV (VVOO_R is symmetric under
+// (ai) <-> (ck)) and dE/dT2 = conj(VVOO).
+
+int lam_check(rcc_t* h) {
+    if (h->sym.nrep > 1 || h->precision != 0) {
+        h->err = "Lambda equations: C1 handles (nrep = 1) with precision 0 only";
+        return -1;
+    }
+    return 0;
+}
+
+int lam_alloc(rcc_t* h) {
+    if (h->lam_alloc) return 0;
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt, namp = h->nt1 + h->nt2;
+    CK(dalloc(h, &h->LAM, namp));
+    CK(dalloc(h, &h->LAMS, namp));
+    CK(dalloc(h, &h->bTAU, h->nt2));
+    CK(dalloc(h, &h->bX, nvo * noot));
+    CK(dalloc(h, &h->bWC, nvo * noot));
+    CK(dalloc(h, &h->bRING, nvo * nvo));
+    CK(dalloc(h, &h->bRING2, nvo * nvo));
+    CK(dalloc(h, &h->bH, nvo * nvo));
+    CK(dalloc(h, &h->bE2, nvvt * o * o));
+    CK(dalloc(h, &h->bF1, (int64_t)v * v * noot));
+    CK(dalloc(h, &h->bP, nvo * std::max<int64_t>({noot, (int64_t)o, 1})));
+    const int64_t nones = std::max<int64_t>({nvvt, noot, (int64_t)o, 1});
+    CK(dalloc(h, &h->ones, nones));
+    std::vector<double2> one((size_t)nones, cplx(1, 0));
+    HIPCHK(h, hipMemcpy(h->ones, one.data(), (size_t)nones * 16, hipMemcpyHostToDevice));
+    CK(dalloc(h, &h->bHO, (int64_t)o * o));
+    CK(dalloc(h, &h->bHV, (int64_t)v * v));
+    CK(dalloc(h, &h->bHOV, nvo));
+    CK(dalloc(h, &h->bGO, (int64_t)o * o));
+    CK(dalloc(h, &h->bGV, (int64_t)v * v));
+    CK(dalloc(h, &h->bAZ, noot * noot));
+    CK(dalloc(h, &h->bB2, (int64_t)o * o));
+    CK(dalloc(h, &h->bW, nvo));
+    CK(dalloc(h, &h->bFVOc, nvo));
+    CK(dalloc(h, &h->LW, nvo));
+    CK(dalloc(h, &h->LB2, (int64_t)o * o));
+    CK(dalloc(h, &h->bPT2, nvvt + noot));
+    h->lam_alloc = true;
+    return 0;
+}
+
+// The forward quantities the sweep reuses, at the handle's current T: one
+// pass of the term sequence (HO, HV, HOV, GO, GV, A, H, X, TAU, RING, WC;
+// S is scratch here) and the T1EQNS products W, W^T T1.
+int lam_setup(rcc_t* h) {
+    CK(lam_check(h));
+    CK(prep(h));
+    CK(lam_alloc(h));
+    invalidate_t(h);
+    CK(run_terms(h));
+    const int o = h->no, v = h->nv;
+    k_conj(h->nvo, h->FVO, h->LW, h->st);
+    k_axpby(h->nvo, cplx(1, 0), h->HOV, cplx(-2, 0), h->LW, h->st);
+    return gemm(h, OP_T, OP_N, o, o, v, cplx(1, 0), h->LW, v, h->T1(), v, cplx(0, 0), h->LB2, o);
+}
+
+// out = [with_energy] dE/dT + Lin . dS/dT (packed T1 | T2 layouts).  The
+// forward pass of lam_setup must be current.  Leaves the Fock-block adjoints
+// bHO, bHV, bFVOc (d/d conj FVO, without the energy's T1) for the density.
+int lam_sweep(rcc_t* h, const double2* Lin, double2* out, bool with_energy) {
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt, nt1 = h->nt1, nt2 = h->nt2;
+    const int64_t K1 = (int64_t)v * noot, K2 = nvvt * o;
+    hipStream_t st = h->st;
+    const double2 one = cplx(1, 0), zero = cplx(0, 0), mone = cplx(-1, 0);
+    const double2* s1b = Lin;
+    const double2* s2b = Lin + nt1;
+    double2* t1b = out;
+    double2* t2b = out + nt1;
+    const double2* T1 = h->T1();
+    const double2* T2 = h->T2();
+    if (with_energy) {
+        HIPCHK(h, hipMemcpyAsync(t1b, h->HOV, (size_t)nt1 * 16, hipMemcpyDeviceToDevice, st));
+        k_conj(nt2, h->cls[RCC_VVOO], t2b, st);
+    } else {
+        k_fill_zero(out, nt1 + nt2, st);
+    }
+    // ---- adjoint of the S2 terms
+    // BINTM ladder S2 += VVVV TAU (ccintm.F:700-722):  TAUb = VVVV^T S2b
+    ktick(h, 0, 0);
+    CK(gemm(h, OP_T, OP_N, (int)nvvt, (int)noot, (int)nvvt, one, h->cls[RCC_VVVV], nvvt, s2b,
+            nvvt, zero, h->bTAU, nvvt, 1, 0, 0, 0, 1));
+    ktick(h, 0, 1);
+    // T2EQNS TAU A (cceqns.F:1013-1026):  TAUb += S2b A^T,  Ab = TAU^T S2b
+    CK(gemm(h, OP_N, OP_T, (int)nvvt, (int)noot, (int)noot, one, s2b, nvvt, h->AZ, noot, one,
+            h->bTAU, nvvt));
+    CK(gemm(h, OP_T, OP_N, (int)noot, (int)noot, (int)nvvt, one, h->TAU, nvvt, s2b, nvvt, zero,
+            h->bAZ, noot));
+    // S2 += pack_pab(ZX) in BINTM and T2EQNS:  ZXb = expand_f1(S2b)
+    double2* zxb = h->ZX;
+    k_expand_f1(s2b, zxb, noot, h->vv, v, st);
+    // BINTM ZX_b = -X_b T1^T (ccintm.F:760-829):  Xb_b = -ZXb_b T1
+    CK(gemm(h, OP_N, OP_N, v, o, v, mone, zxb, v, T1, v, zero, h->bX, v, (int)noot,
+            (int64_t)v * v, 0, nvo));
+    // T1EQN2 (cceqns.F:2601-2714) S1 += trace(X):  Xb += trace^T(S1b)
+    k_trace_t1eqn2_adj(s1b, h->bX, h->oo, v, o, st);
+    // X = VOVV TAU (ccintm.F:772-788):  TAUb += VOVV^T Xb
+    ktick(h, 3, 0);
+    CK(gemm(h, OP_T, OP_N, (int)nvvt, (int)noot, (int)nvo, one, h->cls[RCC_VOVV], nvo, h->bX, nvo,
+            one, h->bTAU, nvvt));
+    ktick(h, 3, 1);
+    // T1b -= sum_b (ZXb_b^T X_b + ZXb_b WC_b)  (BINTM; T2EQNS ZX_b -= T1 WC_b^T)
+    CK(gemm(h, OP_T, OP_N, v, o, v, one, zxb, v, h->X, v, zero, h->bP, v, (int)noot,
+            (int64_t)v * v, nvo, nvo));
+    CK(gemm(h, OP_N, OP_N, v, o, v, one, zxb, v, h->WC, v, one, h->bP, v, (int)noot,
+            (int64_t)v * v, nvo, nvo));
+    CK(gemm(h, OP_N, OP_N, (int)nvo, 1, (int)noot, mone, h->bP, nvo, h->ones, noot, one, t1b,
+            nvo));
+    // WCb_b = -ZXb_b^T T1;  WC = -VOOO + pack_pij(Y1, -1):  Y1b = expand_e2(WCb);
+    // Y1_i = VOVO_i T1 (cceqns.F:1158-1221):  T1b += sum_i VOVO_i^T Y1b_i
+    if (o > 1) {
+        CK(gemm(h, OP_T, OP_N, v, o, v, mone, zxb, v, T1, v, zero, h->bWC, v, (int)noot,
+                (int64_t)v * v, 0, nvo));
+        double2* y1b = h->Y1;
+        k_expand_e2(h->bWC, y1b, nvo, h->oo, o, st);
+        CK(gemm(h, OP_T, OP_N, v, o, (int)nvo, one, h->cls[RCC_VOVO], nvo, y1b, nvo, zero, h->bP,
+                v, o, (int64_t)v * nvo, nvo * o, nvo));
+        CK(gemm(h, OP_N, OP_N, (int)nvo, 1, o, one, h->bP, nvo, h->ones, o, one, t1b, nvo));
+    }
+    // T2EQNS GV F1(T2) (cceqns.F:1183-1184):  GVb = ZXb F1^T,  F1b = GV^T ZXb,
+    // T2b += pack_pab(F1b)
+    k_expand_f1(T2, h->F1, noot, h->vv, v, st);
+    CK(gemm(h, OP_N, OP_T, v, v, (int)K1, one, zxb, v, h->F1, v, zero, h->bGV, v));
+    CK(gemm(h, OP_T, OP_N, v, (int)K1, v, one, h->GV, v, zxb, v, zero, h->bF1, v));
+    k_pack_pab(h->bF1, t2b, noot, 1.0, h->vv, v, st);
+    // T2EQNS S2 += pack_pij(YP), YP_i = VOVV_i^H T1 + E2(T2) GO (cceqns.F:1051-1139):
+    // YPb = -expand_e2(S2b) (the sign is carried by the products' alpha)
+    if (o > 1) {
+        double2* ypb = h->YP;
+        k_expand_e2(s2b, ypb, nvvt, h->oo, o, st);
+        CK(gemm(h, OP_R, OP_N, v, o, (int)nvvt, mone, h->cls[RCC_VOVV], nvo, ypb, nvvt, zero,
+                h->bP, v, o, v, nvvt * o, nvo));
+        CK(gemm(h, OP_N, OP_N, (int)nvo, 1, o, one, h->bP, nvo, h->ones, o, one, t1b, nvo));
+        k_expand_e2(T2, h->E2, nvvt, h->oo, o, st);
+        CK(gemm(h, OP_N, OP_T, (int)K2, o, o, mone, ypb, K2, h->GO, o, zero, h->bE2, K2));
+        CK(gemm(h, OP_T, OP_N, o, o, (int)K2, mone, h->E2, K2, ypb, K2, zero, h->bGO, o));
+        k_pack_pij(h->bE2, t2b, nvvt, -1.0, h->oo, o, st);
+    } else {
+        k_fill_zero(h->bGO, (int64_t)o * o, st);
+    }
+    // T2EQNS ring S2 += pack_pabij(H RING) (cceqns.F:1243-1263):
+    // RING2b = expand_ring(S2b),  Hb = RING2b RING^T,  RINGb = H^T RING2b
+    k_expand_ring(s2b, nullptr, 1.0, h->bRING2, h->vv, h->oo, v, o, st);
+    ktick(h, 2, 0);
+    CK(gemm(h, OP_N, OP_T, (int)nvo, (int)nvo, (int)nvo, one, h->bRING2, nvo, h->RING, nvo, zero,
+            h->bH, nvo, 1, 0, 0, 0, 2));
+    CK(gemm(h, OP_T, OP_N, (int)nvo, (int)nvo, (int)nvo, one, h->H, nvo, h->bRING2, nvo, zero,
+            h->bRING, nvo, 1, 0, 0, 0, 2));
+    ktick(h, 2, 1);
+    // ---- adjoint of T1EQNS (cceqns.F:645-935)
+    // RING HOV:  RINGb += S1b HOV^T,  HOVb = RING^T S1b;  T2b += pack_pabij(RINGb)
+    CK(gemm(h, OP_N, OP_T, (int)nvo, (int)nvo, 1, one, s1b, nvo, h->HOV, nvo, one, h->bRING,
+            nvo));
+    CK(gemm(h, OP_T, OP_N, (int)nvo, 1, (int)nvo, one, h->RING, nvo, s1b, nvo, zero, h->bHOV,
+            nvo));
+    k_pack_pabij(h->bRING, t2b, h->vv, h->oo, v, o, st, 1.0);
+    // T1 (W^T T1), W = HOV - 2 conj(FVO):  T1b += S1b B2^T + W B2b,  B2b = T1^T S1b,
+    // Wb = T1 B2b^T;  HOVb += Wb,  d/d conj(FVO) = -2 Wb
+    CK(gemm(h, OP_N, OP_T, v, o, o, one, s1b, v, h->LB2, o, one, t1b, v));
+    CK(gemm(h, OP_T, OP_N, o, o, v, one, T1, v, s1b, v, zero, h->bB2, o));
+    CK(gemm(h, OP_N, OP_T, v, o, o, one, T1, v, h->bB2, o, zero, h->bW, v));
+    CK(gemm(h, OP_N, OP_N, v, o, o, one, h->LW, v, h->bB2, o, one, t1b, v));
+    k_axpby(nvo, one, h->bW, one, h->bHOV, st);
+    k_axpby(nvo, cplx(-2, 0), h->bW, zero, h->bFVOc, st);
+    // HV T1 - T1 HO:  HVb = S1b T1^T,  HOb = -T1^T S1b,  T1b += HV^T S1b - S1b HO^T
+    CK(gemm(h, OP_N, OP_T, v, v, o, one, s1b, v, T1, v, zero, h->bHV, v));
+    CK(gemm(h, OP_T, OP_N, v, o, v, one, h->HV, v, s1b, v, one, t1b, v));
+    CK(gemm(h, OP_N, OP_T, v, o, o, mone, s1b, v, h->HO, o, one, t1b, v));
+    CK(gemm(h, OP_T, OP_N, o, o, v, mone, T1, v, s1b, v, zero, h->bHO, o));
+    // - VOVO_R T1:  T1b -= VOVO_R^T S1b
+    CK(gemm(h, OP_T, OP_N, (int)nvo, 1, (int)nvo, mone, h->VOVO_R, nvo, s1b, nvo, one, t1b,
+            nvo));
+    // T1EQN1 F1(TAU) M1 (cceqns.F:2521-2599):  F1b = S1b M1^T  (HVINTM adds to it below)
+    CK(gemm(h, OP_N, OP_T, v, (int)K1, o, one, s1b, v, h->M1, K1, zero, h->bF1, v));
+    // ---- adjoint of HINTM (ccintm.F:221-453): H = perm(HAKCI) + RING2' VVOO_R^H,
+    // RING2' = expand_ring(T2, T1, 1/2):  RING2'b = Hb conj(VVOO_R)
+    ktick(h, 1, 0);
+    CK(gemm(h, OP_N, OP_R, (int)nvo, (int)nvo, (int)nvo, one, h->bH, nvo, h->VVOO_R, nvo, zero,
+            h->bRING2, nvo, 1, 0, 0, 0, 2));
+    ktick(h, 1, 1);
+    // T2b += pack_pabij(RING2'b, 1/2);  the -T1(a,k) T1(c,i) part through
+    // Q[(a,k),(c,i)] = RING2'b[(a,i),(c,k)]:  T1b -= Q T1 + Q^T T1
+    k_pack_pabij(h->bRING2, t2b, h->vv, h->oo, v, o, st, 0.5);
+    k_perm_akci(h->bRING2, h->RING2, v, o, false, st);
+    CK(gemm(h, OP_N, OP_N, (int)nvo, 1, (int)nvo, mone, h->RING2, nvo, T1, nvo, one, t1b, nvo));
+    CK(gemm(h, OP_T, OP_N, (int)nvo, 1, (int)nvo, mone, h->RING2, nvo, T1, nvo, one, t1b, nvo));
+    // HAKCIb = perm(Hb);  HAKCI += T1 MH (ccintm.F:268-273):  T1b += HAKCIb MH^T
+    double2* hakb = h->HAKCI;
+    k_perm_akci(h->bH, hakb, v, o, false, st);
+    CK(gemm(h, OP_N, OP_T, v, o, (int)(o * nvo), one, hakb, v, h->MH, o, one, t1b, v));
+    // GVINTM's trace of HAKCI_gv (ccintm.F:208-218):  HAKCIb -= trace^T(GVb);
+    // HAKCI_gv = -VOVV_Q T1:  T1b -= VOVV_Q^T HAKCIb
+    k_trace_gv_adj(h->bGV, hakb, -1.0, v, o, st);
+    CK(gemm(h, OP_T, OP_N, v, o, (int)(nvo * v), mone, h->VOVV_Q, nvo * v, hakb, nvo * v, one, t1b,
+            v));
+    // GVINTM GV = HV - T1 FVO^H:  HVb += GVb,  T1b -= GVb conj(FVO),  FVOcb -= GVb^T T1
+    k_axpby((int64_t)v * v, one, h->bGV, one, h->bHV, st);
+    CK(gemm(h, OP_N, OP_R, v, o, v, mone, h->bGV, v, h->FVO, v, one, t1b, v));
+    CK(gemm(h, OP_T, OP_N, v, o, v, mone, h->bGV, v, T1, v, one, h->bFVOc, v));
+    // GOINTM GO = HO + FVO^H T1 - MGO^T T1 (ccintm.F:95-146)
+    k_axpby((int64_t)o * o, one, h->bGO, one, h->bHO, st);
+    CK(gemm(h, OP_R, OP_N, v, o, o, one, h->FVO, v, h->bGO, o, one, t1b, v));
+    CK(gemm(h, OP_N, OP_T, v, o, o, one, T1, v, h->bGO, o, one, h->bFVOc, v));
+    CK(gemm(h, OP_N, OP_N, (int)nvo, 1, o * o, mone, h->MGO, nvo, h->bGO, (int64_t)o * o, one,
+            t1b, nvo));
+    // AINTM A = OOOO + pack_pij(YAZ) + VVOO^H TAU (ccintm.F:34-93):
+    // TAUb += conj(VVOO) Ab;  YAZb = -expand_e2(Ab);  T1b += sum_i conj(VOOO_i) YAZb_i
+    CK(gemm(h, OP_R, OP_N, (int)nvvt, (int)noot, (int)noot, one, h->cls[RCC_VVOO], nvvt, h->bAZ,
+            noot, one, h->bTAU, nvvt));
+    if (o > 0 && noot > 0) {
+        double2* yazb = h->YAZ;
+        k_expand_e2(h->bAZ, yazb, noot, h->oo, o, st);
+        CK(gemm(h, OP_R, OP_N, v, o, (int)noot, mone, h->cls[RCC_VOOO], nvo, yazb, noot, zero,
+                h->bP, v, o, v, noot * o, nvo));
+        CK(gemm(h, OP_N, OP_N, (int)nvo, 1, o, one, h->bP, nvo, h->ones, o, one, t1b, nvo));
+    }
+    // HOVINTM HOV = conj(FVO) + VVOO_R^H T1 (ccintm.F:514-565):
+    // FVOcb += HOVb,  T1b += conj(VVOO_R) HOVb
+    k_axpby(nvo, one, h->bHOV, one, h->bFVOc, st);
+    CK(gemm(h, OP_R, OP_N, (int)nvo, 1, (int)nvo, one, h->VVOO_R, nvo, h->bHOV, nvo, one, t1b,
+            nvo));
+    // HVINTM HV = FVV' - F1(TAU) VVOO_F1^H (ccintm.F:567-623):
+    // F1b -= HVb conj(VVOO_F1);  TAUb += pack_pab(F1b)
+    CK(gemm(h, OP_N, OP_R, v, (int)K1, v, mone, h->bHV, v, h->VVOO_F1, v, one, h->bF1, v));
+    k_pack_pab(h->bF1, h->bTAU, noot, 1.0, h->vv, v, st);
+    // HOINTM HO = FOO' + VVOO_E2^H E2(TAU) (ccintm.F:455-512):
+    // E2b = conj(VVOO_E2) HOb;  TAUb += pack_pij(E2b, -1)
+    CK(gemm(h, OP_R, OP_N, (int)K2, o, o, one, h->VVOO_E2, K2, h->bHO, o, zero, h->bE2, K2));
+    k_pack_pij(h->bE2, h->bTAU, nvvt, -1.0, h->oo, o, st);
+    // GETTAU (ccintm.F:833-928):  T2b += TAUb,  T1b += expand_ring(TAUb) T1
+    k_axpby(nt2, one, h->bTAU, one, t2b, st);
+    k_expand_ring(h->bTAU, nullptr, 1.0, h->RING2, h->vv, h->oo, v, o, st);
+    return gemm(h, OP_N, OP_N, (int)nvo, 1, (int)nvo, one, h->RING2, nvo, T1, nvo, one, t1b, nvo);
+}
+
+void reset_diis(rcc_t* h) {
+    h->it = 0;
+    h->itc = 0;
+    std::fill(h->bb2.begin(), h->bb2.end(), 0.0);
+}
+
+// Lam at the current T, from Lam0 = (dE/dT) / D, MAXIT / NTOL / MAXDIM of the
+// handle.  The DIIS history is the Lambda one afterwards, so it is cleared
+// for a later CCEQN.
+int lam_solve(rcc_t* h, int* nit, double* rms_out) {
+    CK(lam_setup(h));
+    CK(alloc_diis(h));
+    const int64_t nt1 = h->nt1, nt2 = h->nt2;
+    hipStream_t st = h->st;
+    const int* frz = h->any_frozen ? h->frozen : nullptr;
+    HIPCHK(h, hipMemcpyAsync(h->LAM, h->HOV, (size_t)nt1 * 16, hipMemcpyDeviceToDevice, st));
+    k_conj(nt2, h->cls[RCC_VVOO], h->LAM + nt1, st);
+    k_denom(h->LAM, h->LAM + nt1, h->eps, frz, h->vv, h->oo, h->nv, h->no, st);
+    h->lam_set = true;
+    reset_diis(h);
+    const double conv = std::pow(10.0, -h->ntol);
+    double r = 1.0;
+    int n = 0;
+    while (n < h->maxit) {
+        HIPCHK(h, hipEventRecord(h->ev0, st));
+        CK(lam_sweep(h, h->LAM, h->LAMS, true));
+        k_denom(h->LAMS, h->LAMS + nt1, h->eps, frz, h->vv, h->oo, h->nv, h->no, st);
+        CK(diis_begin(h, h->LAM, h->LAMS));
+        HIPCHK(h, hipStreamSynchronize(st));
+        CK(diis_end(h, &r, h->LAMS));
+        HIPCHK(h, hipMemcpyAsync(h->LAM, h->LAMS, (size_t)(nt1 + nt2) * 16,
+                                 hipMemcpyDeviceToDevice, st));
+        HIPCHK(h, hipEventRecord(h->ev1, st));
+        HIPCHK(h, hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_lambda_ms = ms;
+        for (int w = 0; w < 4; ++w) {
+            float km = 0.f;
+            if (h->ktimed[w] && hipEventElapsedTime(&km, h->kev[w][0], h->kev[w][1]) == hipSuccess)
+                h->kms[w] = km;
+            h->ktimed[w] = false;
+        }
+        ++n;
+        if (r < conv) break;
+    }
+    reset_diis(h);
+    if (nit) *nit = n;
+    if (rms_out) *rms_out = r;
+    return 0;
+}
+
+// The response density at (T, LAM): D (n x n, occupied then virtual,
+// column-major, Hermitian) with d Re(Ec) / d lam = tr(D V) for f -> f + lam V.
+// From the Fock-block adjoints of one sweep: dL/dFOO = HOb (off-diagonal;
+// HOINTM zeroes the diagonal, ccintm.F:495), dL/dFVV = HVb (off-diagonal,
+// ccintm.F:607), dL/dFVO = Lam1 (T1EQNS starts from FVO, cceqns.F:716),
+// dL/d conj(FVO) = FVOcb + T1 (CCENG's zdotc), and dL/d eps from D o T:
+//   eps_i: -sum Lam1 T1 over (a, i) - sum Lam2 T2 over the pairs holding i,
+//   eps_a: +sum Lam1 T1 over (a, i) + sum Lam2 T2 over the pairs holding a.
+// For Hermitian V the real part of dL/dlam is tr(D V) with
+//   D_oo = (conj(HOb) + HOb^T) / 2, D_vv likewise, D_vo = (conj(Lam1) +
+//   FVOcb + T1) / 2, D_ov = D_vo^H and the diagonal Re(dL/d eps).
+int lam_density(rcc_t* h, double* dens) {
+    CK(lam_setup(h));
+    CK(lam_sweep(h, h->LAM, h->LAMS, true));
+    const int o = h->no, v = h->nv, n = o + v;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt, nt1 = h->nt1, nt2 = h->nt2;
+    hipStream_t st = h->st;
+    double2* P = h->LAMS;       // Lam o T (the sweep's output is not needed here)
+    k_hadamard(nt1 + nt2, h->LAM, h->T, P, st);
+    // pair sums of Lam2 o T2: r over the IJ columns of each AB row, c over the AB rows
+    CK(gemm(h, OP_N, OP_N, (int)nvvt, 1, (int)noot, cplx(1, 0), P + nt1, nvvt, h->ones, noot,
+            cplx(0, 0), h->bPT2, nvvt));
+    CK(gemm(h, OP_T, OP_N, (int)noot, 1, (int)nvvt, cplx(1, 0), P + nt1, nvvt, h->ones, nvvt,
+            cplx(0, 0), h->bPT2 + nvvt, noot));
+    std::vector<double2> p1(nvo), rs(nvvt), cs(noot), ho((size_t)o * o), hv((size_t)v * v),
+        fvoc(nvo), l1(nvo), t1(nvo);
+    auto d2h = [&](std::vector<double2>& dst, const double2* src) -> int {
+        if (!dst.empty())
+            HIPCHK(h, hipMemcpyAsync(dst.data(), src, dst.size() * 16, hipMemcpyDeviceToHost, st));
+        return 0;
+    };
+    CK(d2h(p1, P));
+    CK(d2h(rs, h->bPT2));
+    CK(d2h(cs, h->bPT2 + nvvt));
+    CK(d2h(ho, h->bHO));
+    CK(d2h(hv, h->bHV));
+    CK(d2h(fvoc, h->bFVOc));
+    CK(d2h(l1, h->LAM));
+    CK(d2h(t1, h->T1()));
+    HIPCHK(h, hipStreamSynchronize(st));
+    std::vector<double> epsb((size_t)n, 0.0);
+    for (int i = 0; i < o; ++i)
+        for (int a = 0; a < v; ++a) {
+            const double x = p1[a + (size_t)v * i].x;
+            epsb[i] -= x;
+            epsb[o + a] += x;
+        }
+    for (int i = 0; i < o; ++i)
+        for (int j = 0; j < o; ++j)
+            if (i != j) epsb[i] -= cs[i > j ? pair_index(i, j, o) : pair_index(j, i, o)].x;
+    for (int a = 0; a < v; ++a)
+        for (int b = 0; b < v; ++b)
+            if (a != b) epsb[o + a] += rs[a > b ? pair_index(a, b, v) : pair_index(b, a, v)].x;
+    auto D = [&](int p, int q) -> double* { return dens + 2 * ((size_t)p + (size_t)n * q); };
+    auto put = [&](int p, int q, double re, double im) {
+        D(p, q)[0] = re;
+        D(p, q)[1] = im;
+    };
+    for (int i = 0; i < o; ++i)
+        for (int k = 0; k < o; ++k) {
+            if (i == k) { put(k, k, epsb[k], 0.0); continue; }
+            const double2 a = ho[k + (size_t)o * i], b = ho[i + (size_t)o * k];
+            put(k, i, 0.5 * (a.x + b.x), 0.5 * (-a.y + b.y));
+        }
+    for (int c = 0; c < v; ++c)
+        for (int a = 0; a < v; ++a) {
+            if (a == c) { put(o + a, o + a, epsb[o + a], 0.0); continue; }
+            const double2 x = hv[a + (size_t)v * c], y = hv[c + (size_t)v * a];
+            put(o + a, o + c, 0.5 * (x.x + y.x), 0.5 * (-x.y + y.y));
+        }
+    for (int i = 0; i < o; ++i)
+        for (int a = 0; a < v; ++a) {
+            const size_t e = a + (size_t)v * i;
+            const double re = 0.5 * (l1[e].x + fvoc[e].x + t1[e].x);
+            const double im = 0.5 * (-l1[e].y + fvoc[e].y + t1[e].y);
+            put(o + a, i, re, im);
+            put(i, o + a, re, -im);
+        }
+    return 0;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
@@ -1756,6 +2165,62 @@ int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms) {
     return 0;
 }
 
+int rcc_lambda(rcc_t* h, int* nit, double* rms) {
+    if (!h) return -1;
+    return lam_solve(h, nit, rms);
+}
+
+int rcc_set_lambda(rcc_t* h, const double* l1, const double* l2) {
+    if (!h || !l1 || !l2) return -1;
+    CK(lam_check(h));
+    CK(lam_alloc(h));
+    CK(upload(h, h->LAM, l1, h->nt1, hipMemcpyHostToDevice));
+    CK(upload(h, h->LAM + h->nt1, l2, h->nt2, hipMemcpyHostToDevice));
+    h->lam_set = true;
+    return 0;
+}
+
+int rcc_get_lambda(rcc_t* h, double* l1, double* l2) {
+    if (!h || !l1 || !l2) return -1;
+    if (!h->lam_set) {
+        h->err = "rcc_get_lambda: no Lambda (call rcc_lambda or rcc_set_lambda)";
+        return -1;
+    }
+    HIPCHK(h, hipMemcpy(l1, h->LAM, (size_t)h->nt1 * 16, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(l2, h->LAM + h->nt1, (size_t)h->nt2 * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy, double* g1,
+                   double* g2) {
+    if (!h || !l1 || !l2 || !g1 || !g2) return -1;
+    CK(lam_setup(h));
+    double2* in = h->LAMS;
+    double2* out = h->S;        // scratch once the forward pass is done
+    CK(upload(h, in, l1, h->nt1, hipMemcpyHostToDevice));
+    CK(upload(h, in + h->nt1, l2, h->nt2, hipMemcpyHostToDevice));
+    CK(lam_sweep(h, in, out, with_energy != 0));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(g1, out, (size_t)h->nt1 * 16, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(g2, out + h->nt1, (size_t)h->nt2 * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rcc_density(rcc_t* h, double* dens) {
+    if (!h || !dens) return -1;
+    if (!h->lam_set) {
+        h->err = "rcc_density: no Lambda (call rcc_lambda or rcc_set_lambda)";
+        return -1;
+    }
+    return lam_density(h, dens);
+}
+
+int rcc_last_lambda_ms(rcc_t* h, double* ms) {
+    if (!h || !ms) return -1;
+    *ms = h->last_lambda_ms;
+    return 0;
+}
+
 int rcc_last_iteration_ms(rcc_t* h, double* ms) {
     if (!h || !ms) return -1;
     *ms = h->last_iter_ms;
@@ -1772,8 +2237,9 @@ int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const doub
                      const void* A, int64_t lda, const void* B, int64_t ldb, const double* beta,
                      void* C, int64_t ldc) {
     if (!h) return -1;
-    if (opa < 0 || opa > 2 || opb < 0 || opb > 2) {
-        h->err = "rcc_zgemm_device: op must be 0, 1 or 2";
+    const bool conj_pair = (opa == OP_R && opb == OP_N) || (opa == OP_N && opb == OP_R);
+    if (opa < 0 || opb < 0 || ((opa > 2 || opb > 2) && !conj_pair)) {
+        h->err = "rcc_zgemm_device: op must be 0, 1 or 2 (3 only with 0 on the other operand)";
         return -1;
     }
     CK(gemm(h, opa, opb, m, n, k, cplx(alpha[0], alpha[1]), (const double2*)A, lda,

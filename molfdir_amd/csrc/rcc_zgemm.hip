@@ -2,7 +2,8 @@
 //
 //   C(m x n) = alpha * op(A)(m x k) * op(B)(k x n) + beta * C      (column-major)
 //
-// op is N (as stored), T (transposed) or C (conjugate-transposed).  Operands
+// op is N (as stored), T (transposed), C (conjugate-transposed) or R
+// (conjugated, not transposed; with the other operand N).  Operands
 // are interleaved complex (re, im) doubles, the layout relccsd's REAL*8 pairs
 // use (RCW = 2).  Tiles are staged through LDS de-interleaved into planar
 // real / imaginary arrays.  Each complex product is formed on
@@ -69,16 +70,20 @@ struct Loader {
 
 template <int OP>
 __device__ __forceinline__ double2 cj(double2 v) {
-    if (OP == OP_C) v.y = -v.y;
+    if (OP == OP_C || OP == OP_R) v.y = -v.y;
     return v;
 }
 
-// A tile (BM x BK of op(A)); A stored m x k (OP_N) or k x m (OP_T / OP_C).
+// N and R read the operand as stored (R conjugates on load), T and C transposed
+template <int OP>
+constexpr bool as_stored() { return OP == OP_N || OP == OP_R; }
+
+// A tile (BM x BK of op(A)); A stored m x k (OP_N / OP_R) or k x m (OP_T / OP_C).
 template <int OPA>
 __device__ __forceinline__ Loader make_loader_a(const double2* __restrict__ A, int64_t lda, int m,
                                                 int m0, int kbeg, int tid) {
     Loader L;
-    if (OPA == OP_N) {
+    if (as_stored<OPA>()) {
         const int row = tid & 63, kc = tid >> 6;
         const int64_t gm = min(m0 + row, m - 1);
 #pragma unroll
@@ -100,12 +105,12 @@ __device__ __forceinline__ Loader make_loader_a(const double2* __restrict__ A, i
     return L;
 }
 
-// B tile (BK x BN of op(B)); B stored k x n (OP_N) or n x k (OP_T / OP_C).
+// B tile (BK x BN of op(B)); B stored k x n (OP_N / OP_R) or n x k (OP_T / OP_C).
 template <int OPB>
 __device__ __forceinline__ Loader make_loader_b(const double2* __restrict__ B, int64_t ldb, int n,
                                                 int n0, int kbeg, int tid) {
     Loader L;
-    if (OPB == OP_N) {
+    if (as_stored<OPB>()) {
         const int kk = tid & 15, col = tid >> 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -150,7 +155,7 @@ __device__ __forceinline__ void advance(Loader& L) {
 template <int OPA>
 __device__ __forceinline__ void store_a(double* __restrict__ sre, double* __restrict__ sim,
                                         int tid, const double2 (&ra)[4]) {
-    if (OPA == OP_N) {
+    if (as_stored<OPA>()) {
         const int row = tid & 63;
         const int kc = tid >> 6;
 #pragma unroll
@@ -172,7 +177,7 @@ __device__ __forceinline__ void store_a(double* __restrict__ sre, double* __rest
 template <int OPB>
 __device__ __forceinline__ void store_b(double* __restrict__ sre, double* __restrict__ sim,
                                         int tid, const double2 (&rb)[4]) {
-    if (OPB == OP_N) {
+    if (as_stored<OPB>()) {
         const int kk = tid & 15;
         const int col = tid >> 4;
 #pragma unroll
@@ -295,13 +300,13 @@ zgemm_mfma_kernel(int m, int n, int k,
             const int kr = kk + lk;
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                const int r = swz<OPA != OP_N>(kr, wm * 32 + i * 16 + lr);
+                const int r = swz<!as_stored<OPA>()>(kr, wm * 32 + i * 16 + lr);
                 are[i] = As_re[cur][kr * LDA_S + r];
                 aim[i] = As_im[cur][kr * LDA_S + r];
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const int c = swz<OPB == OP_N>(kr, wn * 32 + j * 16 + lr);
+                const int c = swz<as_stored<OPB>()>(kr, wn * 32 + j * 16 + lr);
                 bre[j] = Bs_re[cur][kr * LDB_S + c];
                 bim[j] = Bs_im[cur][kr * LDB_S + c];
                 bnim[j] = -bim[j];
@@ -441,9 +446,12 @@ static void launch_t(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipS
 
 template <int OPA, int OPB>
 static void launch(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
-    if (OPA == OP_N && OPB == OP_N && g.tag == 1) launch_t<OPA, OPB, 1>(g, nsplit, kchunk, W, st);
-    else if (OPA == OP_N && (OPB == OP_N || OPB == OP_C) && g.tag == 2)
-        launch_t<OPA, OPB, 2>(g, nsplit, kchunk, W, st);
+    // tagged symbols: the VVVV ladder and its adjoint (1), the (ov)^3 rings
+    // and their adjoints (2)
+    constexpr bool ladder = OPB == OP_N && (OPA == OP_N || OPA == OP_T);
+    constexpr bool ring = OPA == OP_N || (OPA == OP_T && OPB == OP_N);
+    if (ladder && g.tag == 1) launch_t<OPA, OPB, ladder ? 1 : 0>(g, nsplit, kchunk, W, st);
+    else if (ring && g.tag == 2) launch_t<OPA, OPB, ring ? 2 : 0>(g, nsplit, kchunk, W, st);
     else launch_t<OPA, OPB, 0>(g, nsplit, kchunk, W, st);
 }
 
@@ -522,6 +530,7 @@ int zgemm(const GemmArgs& g0, double2* ws, int64_t ws_elems, hipStream_t st) {
     RCC_L(OP_N, OP_N) else RCC_L(OP_N, OP_T) else RCC_L(OP_N, OP_C)
     else RCC_L(OP_T, OP_N) else RCC_L(OP_T, OP_T) else RCC_L(OP_T, OP_C)
     else RCC_L(OP_C, OP_N) else RCC_L(OP_C, OP_T) else RCC_L(OP_C, OP_C)
+    else RCC_L(OP_R, OP_N) else RCC_L(OP_N, OP_R)
     else return -1;
 #undef RCC_L
     (void)W;

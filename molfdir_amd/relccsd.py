@@ -104,6 +104,12 @@ def load():
     lib.rcc_set_split_digits.argtypes = [vp, ci]
     lib.rcc_probe_mfma_rate.argtypes = [vp, ci, ci, ci, dp]
     lib.rcc_synchronize.argtypes = [vp]
+    lib.rcc_lambda.argtypes = [vp, ctypes.POINTER(ci), dp]
+    lib.rcc_set_lambda.argtypes = [vp, vp, vp]
+    lib.rcc_get_lambda.argtypes = [vp, vp, vp]
+    lib.rcc_lambda_vjp.argtypes = [vp, vp, vp, ci, vp, vp]
+    lib.rcc_density.argtypes = [vp, vp]
+    lib.rcc_last_lambda_ms.argtypes = [vp, dp]
     _lib = lib
     return lib
 
@@ -341,6 +347,50 @@ class RelCCSD:
     def last_iteration_ms(self):
         ms = ctypes.c_double()
         self._check(self.lib.rcc_last_iteration_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    # ------------------------------------------------------------ Lambda / density
+    def lambda_solve(self):
+        """Solve the CCSD Lambda equations at the current T (after cceqn);
+        returns (nit, rms)."""
+        n, r = ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.rcc_lambda(self.h, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value
+
+    def set_lambda(self, l1, l2):
+        a = np.ascontiguousarray(_host(l1), dtype=np.complex128)
+        b = np.ascontiguousarray(_host(l2), dtype=np.complex128)
+        assert a.size == self.nt1 and b.size == self.nt2
+        self._check(self.lib.rcc_set_lambda(self.h, _ptr(a), _ptr(b)))
+
+    def get_lambda(self):
+        a = np.empty(self.nt1, np.complex128)
+        b = np.empty(self.nt2, np.complex128)
+        self._check(self.lib.rcc_get_lambda(self.h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def lambda_vjp(self, l1, l2, with_energy=False):
+        """with_energy * dE/dT + Lam . dS/dT at the current T (S before DENOM)."""
+        a = np.ascontiguousarray(_host(l1), dtype=np.complex128)
+        b = np.ascontiguousarray(_host(l2), dtype=np.complex128)
+        assert a.size == self.nt1 and b.size == self.nt2
+        g1 = np.empty(self.nt1, np.complex128)
+        g2 = np.empty(self.nt2, np.complex128)
+        self._check(self.lib.rcc_lambda_vjp(self.h, _ptr(a), _ptr(b), 1 if with_energy else 0,
+                                            _ptr(g1), _ptr(g2)))
+        return g1, g2
+
+    def density(self):
+        """Hermitian response density D (n x n, occupied then virtual):
+        d Re(Ec)/d lam = tr(D V) for f -> f + lam V."""
+        n = self.no + self.nv
+        d = np.empty((n, n), np.complex128, order="F")
+        self._check(self.lib.rcc_density(self.h, _ptr(d)))
+        return d
+
+    def last_lambda_ms(self):
+        ms = ctypes.c_double()
+        self._check(self.lib.rcc_last_lambda_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
     def kernel_times(self):

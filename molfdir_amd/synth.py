@@ -200,6 +200,17 @@ def make_system(no: int, nv: int, seed: int, rank: int = 120,
     return out
 
 
+def hermitian_perturbation(n: int, seed: int, sigma: float = 0.01) -> np.ndarray:
+    """Seeded Hermitian one-electron perturbation V (n x n, spinors occupied
+    then virtual): V_pq (p < q) with Re, Im ~ N(0, sigma^2), V_qp its
+    conjugate, V_pp ~ N(0, sigma^2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    re = rng.normal(0.0, sigma, (n, n))
+    im = rng.normal(0.0, sigma, (n, n))
+    up = np.triu(re + 1j * im, 1)
+    return up + up.conj().T + np.diag(rng.normal(0.0, sigma, n))
+
+
 def _pair_c1(x, y, n):
     return y * (2 * n - y - 1) // 2 + (x - y - 1)
 

@@ -195,13 +195,9 @@ def density(sysd, T1, T2, L1, L2):
 
 
 def hermitian_direction(n, seed, sigma=0.01):
-    """Seeded Hermitian V: V_pq (p<q) with Re, Im ~ N(0, sigma^2), V_qp its
-    conjugate, V_pp ~ N(0, sigma^2)."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    re = rng.normal(0.0, sigma, (n, n))
-    im = rng.normal(0.0, sigma, (n, n))
-    up = np.triu(re + 1j * im, 1)
-    return up + up.conj().T + np.diag(rng.normal(0.0, sigma, n))
+    """The seeded Hermitian perturbation of the synthetic-input generator."""
+    from molfdir_amd.synth import hermitian_perturbation
+    return hermitian_perturbation(n, seed, sigma)
 
 
 def fd_gradient(energy_of, sysd, V, h=1e-4):

@@ -54,3 +54,28 @@ def test_zgemm_against_numpy(opa, opb, m, n, k, three_mult):
     assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
 
 
+
+
+@pytest.mark.parametrize("three_mult", [True, False])
+@pytest.mark.parametrize("opa,opb", [(3, 0), (0, 3)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 16), (37, 91, 53), (5, 3, 3000), (257, 129, 77)])
+def test_zgemm_conjugate_operand_against_numpy(opa, opb, m, n, k, three_mult):
+    # op 3: the operand conjugated, not transposed (the adjoint contractions)
+    rng = np.random.default_rng(m * 5 + n + k + opa)
+
+    def mat(r, c):
+        return rng.normal(size=(r, c)) + 1j * rng.normal(size=(r, c))
+
+    A, B, C = mat(m, k), mat(k, n), mat(m, n)
+    alpha, beta = 0.4 + 0.9j, 0.25 - 0.5j
+    ref = alpha * (A.conj() if opa == 3 else A) @ (B.conj() if opb == 3 else B) + beta * C
+
+    def dev(x):
+        return torch.from_numpy(x.reshape(-1, order="F").copy()).to("cuda")
+
+    tA, tB, tC = dev(A), dev(B), dev(C)
+    r = RelCCSD(2, 4)
+    r.set_complex_product(three_mult)
+    r.zgemm_device(opa, opb, m, n, k, alpha, tA, m, tB, k, beta, tC, m)
+    got = tC.cpu().numpy().reshape(n, m).T
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
