@@ -78,7 +78,8 @@ def bench_lambda(args):
     dev = torch.device("cuda:0")
     n = no + nv
     prob = None
-    for sc in [args.scale] + [x for x in (0.02, 0.01, 0.005) if x < args.scale]:
+    tried = [args.scale] + [x for x in (0.02, 0.01, 0.005) if x < args.scale]
+    for sc in tried:
         sysd = synth.make_system(no, nv, seed=seed, scale=sc, device=dev)
         prob = PerturbedCCSD(sysd, maxit=args.lambda_maxit, maxdim=8, ntol=12)
         del sysd
@@ -137,8 +138,8 @@ def bench_lambda(args):
         "ms_per_step": lam_ms, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": (f"synthetic (seed {seed}; factor Re/Im N(0,{scale}^2), the largest of "
-                 f"{args.scale}, 0.02, 0.01, 0.005 at which CCSD converges; V seed {seed + 100}, "
-                 f"N(0,0.01^2) Hermitian)"),
+                 f"{', '.join(str(x) for x in tried)} at which CCSD converges; V seed "
+                 f"{seed + 100}, N(0,0.01^2) Hermitian)"),
         "config": {"workload": f"Lambda equations + response density, {desc}", "factor_std": scale},
         "lambda_tflops": F / (lam_ms * 1e-3) / 1e12, "lambda_flops_per_iter": F,
         "lambda": {"nit": lam_nit, "rms": lam_rms, "total_s": t_lam, "density_s": t_dens},

@@ -3,7 +3,7 @@
 ! term handed to librelccsd_hip.so through ISO_C_BINDING.
 !
 ! Input  ccin.bin (stream): 8 int32 header (NO, NV, unused, MAXIT, MAXDIM,
-!        NTOL, NREP (0 = C1), 0), orbital energies, for NREP > 1 the fermion
+!        NTOL, NREP (0 = C1), LAMBDA), orbital energies, for NREP > 1 the fermion
 !        irrep of every spinor, the occupied count per irrep and MULTB(.,.,1)
 !        of the 2 NREP irreps (int32, as in MRCONEE), then FOO, FVO, FVV and
 !        the classes OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd's
@@ -11,14 +11,18 @@
 !        device in column batches, as GETVOVV / GETVVVV read them
 !        (ccgetv.F:225-402).
 ! Output the CCEQN iteration table on unit 6 (format of cceqns.F:6001) and
-!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout).
+!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout); with LAMBDA = 1
+!        (C1) the Lambda equations are solved after CCSD and NIT, RMS of that
+!        solve and the response density (n x n complex, column-major) follow.
 program ccdrv_hip
   use relccsd_hip
   implicit none
   integer(c_int) :: hdr(8), rc, nit
   integer :: no, nv, maxit, maxdim, ntol, cls, nrep, i, j, nsp
   integer(c_int64_t) :: rows, cols, off, c0, nb, batch, nfoo, nfvo, nfvv, nt1, nt2
-  real(c_double), allocatable :: eps(:), foo(:), fvo(:), fvv(:), buf(:), t1(:), t2(:)
+  real(c_double), allocatable :: eps(:), foo(:), fvo(:), fvv(:), buf(:), t1(:), t2(:), dens(:)
+  integer(c_int) :: lnit
+  real(c_double) :: lrms
   integer(c_int), allocatable, target :: nos(:), nvs(:), multb(:, :)
   integer(c_int), allocatable :: irrsp(:), mult(:, :)
   type(rcc_sym_t) :: sym
@@ -132,6 +136,15 @@ program ccdrv_hip
   open (32, file='ccout.bin', access='stream', form='unformatted', status='replace')
   write (32) emp2, ecc, dble(nit)
   write (32) t1, t2
+  if (hdr(8) /= 0) then
+    ! Lambda equations and the one-particle response density
+    call rcc_check(h, rcc_lambda(h, lnit, lrms), 'LAMBDA')
+    write (*, '(a,i4,a,es10.3)') ' Lambda equations: iterations', lnit, '  RMS', lrms
+    allocate (dens(2 * nsp * nsp))
+    call rcc_check(h, rcc_density(h, dens), 'DENSITY')
+    write (32) dble(lnit), lrms
+    write (32) dens
+  end if
   close (32)
   call rcc_destroy(h)
 6001 format(2x, i3, 5x, 2f20.15, f10.5)

@@ -73,6 +73,24 @@ module relccsd_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: emp2
     end function
+    ! CCSD Lambda equations and response density (no live relccsd
+    ! counterpart: its Lambda branches are switched off, ccdriv.F:1918-1924)
+    integer(c_int) function rcc_lambda(h, nit, rms) bind(c, name="rcc_lambda")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: nit
+      real(c_double), intent(out) :: rms
+    end function
+    integer(c_int) function rcc_get_lambda(h, l1, l2) bind(c, name="rcc_get_lambda")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: l1(*), l2(*)
+    end function
+    integer(c_int) function rcc_density(h, dens) bind(c, name="rcc_density")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: dens(*)
+    end function
     integer(c_int) function rcc_term(h) bind(c)
       import :: c_int, c_ptr
       type(c_ptr), value :: h

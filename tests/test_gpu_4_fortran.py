@@ -40,3 +40,27 @@ def test_fortran_driver_matches_reference(kind, name, tmp_path):
     assert abs(raw[0] - float(g["emp2"])) < 1e-10
     assert np.abs(z[:nt1] - g["T1"]).max() < 1e-9
     assert np.abs(z[nt1:] - g["T2"]).max() < 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_lambda_density_gives_reference_gradient(tmp_path):
+    g = load("lambda", "lam_4o12v")
+    no, nv = int(g["params"][0]), int(g["params"][1])
+    s = system_for(g)
+    path = tmp_path / "ccin.bin"
+    refrun._write_input(str(path), s, 1, 80, 8, 12)
+    raw = bytearray(path.read_bytes())
+    raw[28:32] = np.array([1], dtype=np.int32).tobytes()     # header word 8: LAMBDA
+    path.write_bytes(bytes(raw))
+    p = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    out = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
+    nt1, nt2 = no * nv, (no * (no - 1) // 2) * (nv * (nv - 1) // 2)
+    pos = 3 + 2 * (nt1 + nt2)
+    lnit, lrms = out[pos], out[pos + 1]
+    n = no + nv
+    D = out[pos + 2:].view(np.complex128).reshape(n, n, order="F")
+    assert lrms < 1e-12 and lnit > 0
+    grad = float(np.real(np.trace(D @ g["V"])))
+    fd = float(g["fd4"])
+    assert abs(grad - fd) <= 2e-6 * abs(fd) + 2e-11, (grad, fd)
