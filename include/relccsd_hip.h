@@ -98,6 +98,17 @@ int rcc_set_complex_product(rcc_t* h, int three_mult);
 /* Digits per value of the split-INT8 mode: 4 (28 bits) or 5 (35 bits, the
  * default).  No counterpart in the reference. */
 int rcc_set_split_digits(rcc_t* h, int digits);
+/* The autotuner of the complex GEMMs (process-wide, on by default unless
+ * RCC_AUTOTUNE=0 is in the environment): the first product of each shape
+ * met outside graph capture times its candidate work tiles on the stream --
+ * the K-split chunk (64 .. 4096) of FP64 products whose output tiles cannot
+ * fill the chip, the tile height (32 WM rows, WM 2..4) of split-INT8
+ * products -- and keeps the fastest for the process.  Candidates run in
+ * place with alpha = 0, beta = 1, which leaves C unchanged.  Different
+ * K-splits round differently (FP64 sums in another order).  No counterpart
+ * in the reference. */
+int rcc_set_autotune(rcc_t* h, int enable);
+int rcc_autotune_entries(rcc_t* h, int64_t* fp64_shapes, int64_t* int8_shapes);
 
 /* Fock blocks and orbital energies as FMTOFILE reads them (ccgetv.F:34-87);
  * eps: occupied then virtual (SYMTAB reordering ccdriv.F:1393-1420). */
@@ -163,6 +174,13 @@ int rcc_set_graph(rcc_t* h, int enable);
  * still on the device.  nit/ecc/rms receive one value per handle.  No
  * counterpart in the reference (which runs one system per process). */
 int rcc_iterate_batch(rcc_t* const* handles, int n, int* nit, double* ecc, double* rms);
+/* As rcc_iterate_batch, with the four dominant products of every handle
+ * (VOVV.tau, the HINTM and T2EQNS rings, the VVVV ladder; per irrep block
+ * for a symmetric handle) run as one grouped launch per term over all the
+ * handles.  Handles must use precision 0 and the same complex-product form
+ * and, with symmetry, self-inverse irreps.  The first call per list of
+ * handles builds the launch descriptors.  Same results as rcc_iterate. */
+int rcc_iterate_batch_grouped(rcc_t* const* handles, int n, int* nit, double* ecc, double* rms);
 /* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);
 

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace rcc {
 
@@ -26,6 +28,11 @@ struct GemmArgs {
 
 // ws may be null (no K-split); ws_elems = capacity in complex elements.
 int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
+// K-split autotuner of zgemm (process-wide): on / off, and the number of
+// shapes tuned so far.
+void zgemm_autotune(int enable);
+bool zgemm_autotune_enabled();
+int64_t zgemm_tuned_shapes();
 
 // Grouped GEMM: many independent C_g = alpha_g A_g B_g + beta_g C_g (op(A) = N,
 // op(B) = N or C) in one launch.  Group g owns blocks [tile_off, tile_off +
@@ -43,10 +50,52 @@ struct ZGroup {
 // Sets tile_off of every group; returns the grid size (blocks).
 int64_t zgroup_offsets(ZGroup* g, int ngroups);
 // dG: device copy of the descriptors (with offsets); tag as GemmArgs::tag.
+// gmap (optional): the group of every block, instead of a binary search.
 int zgemm_grouped(int opb, int tag, int gauss, const ZGroup* dG, int ngroups,
-                  int64_t blocks, hipStream_t st);
+                  int64_t blocks, hipStream_t st, const int* gmap = nullptr);
+// Grouped GEMM with op(A) and op(B) given (N/T/C/R combinations the term
+// sequence uses: NN, NC, NT, TN, CN); tag as GemmArgs::tag for NN / NC.
+int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, int ngroups,
+                      int64_t blocks, hipStream_t st, const int* gmap = nullptr);
+// The 32 x 32-tile grouped GEMM (same op combinations), its tile offsets.
+int64_t zgroup_offsets_small(ZGroup* g, int ngroups);
+int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
+                        int64_t blocks, hipStream_t st, const int* gmap);
 int mfma_probe(double* dout, hipStream_t st);
 int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
+
+// ---- launch recording and batched replay (rcc_kernels.hip) ----
+// With g_record set (per host thread), the element-wise wrappers below,
+// zgemm and the solver's grouped launches append a Rec instead of launching.
+// The recordings of several handles' term sequences (identical in kind,
+// position by position) compile into a BatchProgram that issues each
+// position once for all handles.
+enum RecKind {
+    RK_GETTAU, RK_EXPAND_E2, RK_EXPAND_F1, RK_EXPAND_RING, RK_PACK_PIJ, RK_PACK_PAB,
+    RK_PACK_PABIJ, RK_PERM_AKCI, RK_TRACE_T1, RK_TRACE_GV, RK_DENOM1, RK_DENOM2, RK_AXPBY,
+    RK_CONJ, RK_COPY, RK_ZERO_DIAG, RK_BLOCKSET, RK_FILL, RK_SPLIT_REDUCE, RK_GEMM, RK_GROUPED
+};
+struct Rec {
+    int kind = 0;
+    int64_t n = 0;                    // element-wise: elements (grid x extent)
+    int gy = 1;                       // element-wise: grid y of the kernel
+    std::vector<unsigned char> blob;  // element-wise: kernel parameters
+    GemmArgs g;                       // RK_GEMM: the product, its K split and workspace
+    int nsplit = 1, kchunk = 0;
+    double2* ws = nullptr;
+    std::vector<ZGroup> groups;       // RK_GROUPED: descriptors, op(B), tag, complex form
+    int opb = 0, tag = 0, gauss = 0;
+};
+extern thread_local std::vector<Rec>* g_record;
+struct BatchProgram;
+int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out,
+                  std::string& err);
+int batch_run(const BatchProgram* p, hipStream_t st);
+// times the 64 x 64 and 32 x 32 tiles of every grouped product, keeps the faster
+int batch_tune(BatchProgram* p, hipStream_t st);
+// grouped products that use the 32 x 32 tile (and, in *grouped, how many there are)
+int batch_small_tiles(const BatchProgram* p, int* grouped);
+void batch_free(BatchProgram* p);
 
 // ---- element-wise / permutation kernels (rcc_kernels.hip) ----
 // Packed pair tables: pair p of an n-set -> (x, y), x > y, y outer.
@@ -100,6 +149,8 @@ void k_denom(double2* S1, double2* S2, const double* eps, const int* frozen, Pai
 // y = a*x + b*y (complex a, b)
 void k_axpby(int64_t n, double2 a, const double2* x, double2 b, double2* y, hipStream_t st);
 void k_conj(int64_t n, const double2* x, double2* y, hipStream_t st);
+// y = x (device to device, as a kernel so that it records and replays)
+void k_copy(int64_t n, const double2* x, double2* y, hipStream_t st);
 // M (n x n) := M with the diagonal zeroed (square, column-major)
 void k_zero_diag(double2* M, int n, hipStream_t st);
 // generic gather: out[i] = sgn[i] * (conj ? conj(in[idx[i]]) : in[idx[i]]), idx<0 -> 0
@@ -165,5 +216,7 @@ int i8_split(const I8Operand& S, const double2* X, int64_t ld, bool k_contiguous
              hipStream_t st);
 int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, double2 alpha,
              double2 beta, hipStream_t st);
+// shapes whose tile height the INT8 autotuner has picked so far
+int64_t i8_tuned_shapes();
 
 }  // namespace rcc

@@ -11,6 +11,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
 #include "rcc_internal.h"
 
 namespace rcc {
@@ -47,74 +50,104 @@ __device__ __forceinline__ double2 cscale(double s, double2 a) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
          i += (int64_t)gridDim.x * blockDim.x)
 
+// ---------------------------------------------------------------- launching
+// Every element-wise kernel of the term sequence is a parameter struct P and
+// a device body(const P&) with a grid-stride loop over blockIdx.x; ew_one
+// runs one instance, ew_many one instance per blockIdx.z (a batch of
+// handles, parameters in device memory).  With a recording active
+// (g_record) the wrappers append the launch instead of issuing it, so that a
+// handle's whole term sequence can be replayed for many handles at once.
+thread_local std::vector<Rec>* g_record = nullptr;
+
+template <class P>
+__global__ void __launch_bounds__(TPB) ew_one(P p) { body(p); }
+
+template <class P>
+__global__ void __launch_bounds__(TPB) ew_many(const P* __restrict__ ps) { body(ps[blockIdx.z]); }
+
+template <class P>
+static void ew_launch(int kind, const P& p, int64_t n, int gy, hipStream_t st) {
+    if (g_record) {
+        Rec r;
+        r.kind = kind;
+        r.n = n;
+        r.gy = gy;
+        r.blob.assign((const unsigned char*)&p, (const unsigned char*)&p + sizeof(P));
+        g_record->push_back(std::move(r));
+        return;
+    }
+    if (n <= 0 || gy <= 0) return;
+    hipLaunchKernelGGL(ew_one<P>, dim3(nblocks(n), gy), dim3(TPB), 0, st, p);
+}
+
 // ---------------------------------------------------------------- tau
-__global__ void gettau_kernel(const double2* __restrict__ t1, const double2* __restrict__ t2,
-                              double2* __restrict__ tau, double s2, const int2* __restrict__ vv,
-                              int64_t nvvt, const int2* __restrict__ oo, int64_t noot, int nv) {
-    GRID_LOOP(e, nvvt * noot) {
-        const int64_t ab = e % nvvt, ij = e / nvvt;
-        const int2 p = vv[ab], q = oo[ij];
-        const int a = p.x, b = p.y, i = q.x, j = q.y;
-        double2 v = cscale(s2, t2[e]);
-        v = cadd(v, cmul(t1[a + nv * i], t1[b + nv * j]));
-        v = csub(v, cmul(t1[a + nv * j], t1[b + nv * i]));
-        tau[e] = v;
+struct GettauP {
+    const double2* t1; const double2* t2; double2* tau; double s2;
+    const int2* vv; int64_t nvvt; const int2* oo; int64_t noot; int nv;
+};
+__device__ __forceinline__ void body(const GettauP& p) {
+    GRID_LOOP(e, p.nvvt * p.noot) {
+        const int64_t ab = e % p.nvvt, ij = e / p.nvvt;
+        const int2 q1 = p.vv[ab], q2 = p.oo[ij];
+        const int a = q1.x, b = q1.y, i = q2.x, j = q2.y, nv = p.nv;
+        double2 v = cscale(p.s2, p.t2[e]);
+        v = cadd(v, cmul(p.t1[a + nv * i], p.t1[b + nv * j]));
+        v = csub(v, cmul(p.t1[a + nv * j], p.t1[b + nv * i]));
+        p.tau[e] = v;
     }
 }
 
 void k_gettau(const double2* t1, const double2* t2, double2* tau, double s2, PairTab vv,
               PairTab oo, int nv, hipStream_t st) {
-    const int64_t n = vv.npair * oo.npair;
-    if (n == 0) return;
-    hipLaunchKernelGGL(gettau_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, t1, t2, tau, s2,
-                       vv.tab, vv.npair, oo.tab, oo.npair, nv);
+    const GettauP p{t1, t2, tau, s2, vv.tab, vv.npair, oo.tab, oo.npair, nv};
+    ew_launch(RK_GETTAU, p, vv.npair * oo.npair, 1, st);
 }
 
 // ---------------------------------------------------------------- expansions
-__global__ void expand_e2_kernel(const double2* __restrict__ P, double2* __restrict__ out,
-                                 int64_t R, int no) {
-    GRID_LOOP(o, R * no * no) {
-        const int64_t r = o % R;
-        const int64_t q = o / R;
+struct ExpandE2P { const double2* P; double2* out; int64_t R; int no; };
+__device__ __forceinline__ void body(const ExpandE2P& p) {
+    const int no = p.no;
+    GRID_LOOP(o, p.R * no * no) {
+        const int64_t r = o % p.R;
+        const int64_t q = o / p.R;
         const int y = (int)(q % no), x = (int)(q / no);
         double2 v = make_double2(0.0, 0.0);
-        if (x > y) v = P[r + R * pair_of(x, y, no)];
-        else if (x < y) { v = P[r + R * pair_of(y, x, no)]; v.x = -v.x; v.y = -v.y; }
-        out[o] = v;
+        if (x > y) v = p.P[r + p.R * pair_of(x, y, no)];
+        else if (x < y) { v = p.P[r + p.R * pair_of(y, x, no)]; v.x = -v.x; v.y = -v.y; }
+        p.out[o] = v;
     }
 }
 
 void k_expand_e2(const double2* P, double2* out, int64_t R, PairTab oo, int no, hipStream_t st) {
-    const int64_t n = R * no * no;
-    if (n == 0) return;
-    hipLaunchKernelGGL(expand_e2_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, P, out, R, no);
+    (void)oo;
+    ew_launch(RK_EXPAND_E2, ExpandE2P{P, out, R, no}, R * no * no, 1, st);
 }
 
-__global__ void expand_f1_kernel(const double2* __restrict__ P, double2* __restrict__ out,
-                                 int64_t ncol, int nv, int64_t nvvt) {
-    const int64_t nv2 = (int64_t)nv * nv;
-    GRID_LOOP(o, nv2 * ncol) {
-        const int a = (int)(o % nv);
-        const int d = (int)((o / nv) % nv);
+struct ExpandF1P { const double2* P; double2* out; int64_t ncol; int nv; int64_t nvvt; };
+__device__ __forceinline__ void body(const ExpandF1P& p) {
+    const int64_t nv2 = (int64_t)p.nv * p.nv;
+    GRID_LOOP(o, nv2 * p.ncol) {
+        const int a = (int)(o % p.nv);
+        const int d = (int)((o / p.nv) % p.nv);
         const int64_t col = o / nv2;
         double2 v = make_double2(0.0, 0.0);
-        if (a > d) v = P[pair_of(a, d, nv) + nvvt * col];
-        else if (a < d) { v = P[pair_of(d, a, nv) + nvvt * col]; v.x = -v.x; v.y = -v.y; }
-        out[o] = v;
+        if (a > d) v = p.P[pair_of(a, d, p.nv) + p.nvvt * col];
+        else if (a < d) { v = p.P[pair_of(d, a, p.nv) + p.nvvt * col]; v.x = -v.x; v.y = -v.y; }
+        p.out[o] = v;
     }
 }
 
 void k_expand_f1(const double2* P, double2* out, int64_t ncol, PairTab vv, int nv,
                  hipStream_t st) {
-    const int64_t n = (int64_t)nv * nv * ncol;
-    if (n == 0) return;
-    hipLaunchKernelGGL(expand_f1_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, P, out, ncol, nv,
-                       vv.npair);
+    ew_launch(RK_EXPAND_F1, ExpandF1P{P, out, ncol, nv, vv.npair}, (int64_t)nv * nv * ncol, 1,
+              st);
 }
 
-__global__ void expand_ring_kernel(const double2* __restrict__ T2, const double2* __restrict__ t1,
-                                   double s, double2* __restrict__ out, int nv, int no,
-                                   int64_t nvvt) {
+struct ExpandRingP {
+    const double2* T2; const double2* t1; double s; double2* out; int nv; int no; int64_t nvvt;
+};
+__device__ __forceinline__ void body(const ExpandRingP& p) {
+    const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
     GRID_LOOP(o, nvo * nvo) {
         const int a = (int)(o % nv);
@@ -126,21 +159,20 @@ __global__ void expand_ring_kernel(const double2* __restrict__ T2, const double2
         if (a != c && i != k) {
             const int64_t ab = a > c ? pair_of(a, c, nv) : pair_of(c, a, nv);
             const int64_t ij = i > k ? pair_of(i, k, no) : pair_of(k, i, no);
-            v = T2[ab + nvvt * ij];
-            const double sg = ((a > c) == (i > k)) ? s : -s;
+            v = p.T2[ab + p.nvvt * ij];
+            const double sg = ((a > c) == (i > k)) ? p.s : -p.s;
             v = cscale(sg, v);
         }
-        if (t1) v = csub(v, cmul(t1[a + nv * k], t1[c + nv * i]));
-        out[o] = v;
+        if (p.t1) v = csub(v, cmul(p.t1[a + nv * k], p.t1[c + nv * i]));
+        p.out[o] = v;
     }
 }
 
 void k_expand_ring(const double2* T2, const double2* t1, double s, double2* out, PairTab vv,
                    PairTab oo, int nv, int no, hipStream_t st) {
+    (void)oo;
     const int64_t nvo = (int64_t)nv * no;
-    if (nvo == 0) return;
-    hipLaunchKernelGGL(expand_ring_kernel, dim3(nblocks(nvo * nvo)), dim3(TPB), 0, st, T2, t1,
-                       s, out, nv, no, vv.npair);
+    ew_launch(RK_EXPAND_RING, ExpandRingP{T2, t1, s, out, nv, no, vv.npair}, nvo * nvo, 1, st);
 }
 
 __global__ void expand_cols_kernel(const double2* __restrict__ P, double2* __restrict__ out,
@@ -159,154 +191,144 @@ __global__ void expand_cols_kernel(const double2* __restrict__ P, double2* __res
 
 void k_expand_cols(const double2* P, double2* out, int64_t R, PairTab vv, int nv,
                    hipStream_t st) {
+    (void)vv;
     const int64_t n = R * nv * nv;
     if (n == 0) return;
     hipLaunchKernelGGL(expand_cols_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, P, out, R, nv);
 }
 
 // ---------------------------------------------------------------- antisymmetrising packs
-__global__ void pack_pij_kernel(const double2* __restrict__ Y, double2* __restrict__ S,
-                                int64_t NR, double alpha, const int2* __restrict__ oo,
-                                int64_t noot, int no) {
-    GRID_LOOP(e, NR * noot) {
-        const int64_t r = e % NR, ij = e / NR;
-        const int2 q = oo[ij];
-        const double2 y1 = Y[r + NR * (q.x + (int64_t)no * q.y)];
-        const double2 y2 = Y[r + NR * (q.y + (int64_t)no * q.x)];
-        double2 s = S[e];
-        s.x += alpha * (y1.x - y2.x);
-        s.y += alpha * (y1.y - y2.y);
-        S[e] = s;
+struct PackPijP { const double2* Y; double2* S; int64_t NR; double alpha; const int2* oo;
+                  int64_t noot; int no; };
+__device__ __forceinline__ void body(const PackPijP& p) {
+    GRID_LOOP(e, p.NR * p.noot) {
+        const int64_t r = e % p.NR, ij = e / p.NR;
+        const int2 q = p.oo[ij];
+        const double2 y1 = p.Y[r + p.NR * (q.x + (int64_t)p.no * q.y)];
+        const double2 y2 = p.Y[r + p.NR * (q.y + (int64_t)p.no * q.x)];
+        double2 s = p.S[e];
+        s.x += p.alpha * (y1.x - y2.x);
+        s.y += p.alpha * (y1.y - y2.y);
+        p.S[e] = s;
     }
 }
 
 void k_pack_pij(const double2* Y, double2* S, int64_t NR, double alpha, PairTab oo, int no,
                 hipStream_t st) {
-    const int64_t n = NR * oo.npair;
-    if (n == 0) return;
-    hipLaunchKernelGGL(pack_pij_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, Y, S, NR, alpha,
-                       oo.tab, oo.npair, no);
+    ew_launch(RK_PACK_PIJ, PackPijP{Y, S, NR, alpha, oo.tab, oo.npair, no}, NR * oo.npair, 1, st);
 }
 
-__global__ void pack_pab_kernel(const double2* __restrict__ Y, double2* __restrict__ S,
-                                int64_t ncol, double alpha, const int2* __restrict__ vv,
-                                int64_t nvvt, int nv) {
-    const int64_t nv2 = (int64_t)nv * nv;
-    GRID_LOOP(e, nvvt * ncol) {
-        const int64_t ab = e % nvvt, col = e / nvvt;
-        const int2 p = vv[ab];
-        const double2 y1 = Y[p.x + (int64_t)nv * p.y + nv2 * col];
-        const double2 y2 = Y[p.y + (int64_t)nv * p.x + nv2 * col];
-        double2 s = S[e];
-        s.x += alpha * (y1.x - y2.x);
-        s.y += alpha * (y1.y - y2.y);
-        S[e] = s;
+struct PackPabP { const double2* Y; double2* S; int64_t ncol; double alpha; const int2* vv;
+                  int64_t nvvt; int nv; };
+__device__ __forceinline__ void body(const PackPabP& p) {
+    const int64_t nv2 = (int64_t)p.nv * p.nv;
+    GRID_LOOP(e, p.nvvt * p.ncol) {
+        const int64_t ab = e % p.nvvt, col = e / p.nvvt;
+        const int2 q = p.vv[ab];
+        const double2 y1 = p.Y[q.x + (int64_t)p.nv * q.y + nv2 * col];
+        const double2 y2 = p.Y[q.y + (int64_t)p.nv * q.x + nv2 * col];
+        double2 s = p.S[e];
+        s.x += p.alpha * (y1.x - y2.x);
+        s.y += p.alpha * (y1.y - y2.y);
+        p.S[e] = s;
     }
 }
 
 void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTab vv, int nv,
                 hipStream_t st) {
-    const int64_t n = vv.npair * ncol;
-    if (n == 0) return;
-    hipLaunchKernelGGL(pack_pab_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, Y, S, ncol, alpha,
-                       vv.tab, vv.npair, nv);
+    ew_launch(RK_PACK_PAB, PackPabP{Y, S, ncol, alpha, vv.tab, vv.npair, nv}, vv.npair * ncol, 1,
+              st);
 }
 
-__global__ void pack_pabij_kernel(const double2* __restrict__ Q, double2* __restrict__ S,
-                                  const int2* __restrict__ vv, int64_t nvvt,
-                                  const int2* __restrict__ oo, int64_t noot, int nv, int no,
-                                  double alpha) {
-    const int64_t nvo = (int64_t)nv * no;
-    GRID_LOOP(e, nvvt * noot) {
-        const int2 p = vv[e % nvvt], q = oo[e / nvvt];
-        const int64_t ai = p.x + (int64_t)nv * q.x, bj = p.y + (int64_t)nv * q.y;
-        const int64_t bi = p.y + (int64_t)nv * q.x, aj = p.x + (int64_t)nv * q.y;
-        const double2 v1 = Q[ai + nvo * bj], v2 = Q[bi + nvo * aj];
-        const double2 v3 = Q[aj + nvo * bi], v4 = Q[bj + nvo * ai];
-        double2 s = S[e];
-        s.x += alpha * (v1.x - v2.x - v3.x + v4.x);
-        s.y += alpha * (v1.y - v2.y - v3.y + v4.y);
-        S[e] = s;
+struct PackPabijP { const double2* Q; double2* S; const int2* vv; int64_t nvvt; const int2* oo;
+                    int64_t noot; int nv; int no; double alpha; };
+__device__ __forceinline__ void body(const PackPabijP& p) {
+    const int64_t nvo = (int64_t)p.nv * p.no;
+    GRID_LOOP(e, p.nvvt * p.noot) {
+        const int2 q1 = p.vv[e % p.nvvt], q2 = p.oo[e / p.nvvt];
+        const int64_t ai = q1.x + (int64_t)p.nv * q2.x, bj = q1.y + (int64_t)p.nv * q2.y;
+        const int64_t bi = q1.y + (int64_t)p.nv * q2.x, aj = q1.x + (int64_t)p.nv * q2.y;
+        const double2 v1 = p.Q[ai + nvo * bj], v2 = p.Q[bi + nvo * aj];
+        const double2 v3 = p.Q[aj + nvo * bi], v4 = p.Q[bj + nvo * ai];
+        double2 s = p.S[e];
+        s.x += p.alpha * (v1.x - v2.x - v3.x + v4.x);
+        s.y += p.alpha * (v1.y - v2.y - v3.y + v4.y);
+        p.S[e] = s;
     }
 }
 
 void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
                   hipStream_t st, double alpha) {
-    const int64_t n = vv.npair * oo.npair;
-    if (n == 0) return;
-    hipLaunchKernelGGL(pack_pabij_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, Q, S, vv.tab,
-                       vv.npair, oo.tab, oo.npair, nv, no, alpha);
+    ew_launch(RK_PACK_PABIJ, PackPabijP{Q, S, vv.tab, vv.npair, oo.tab, oo.npair, nv, no, alpha},
+              vv.npair * oo.npair, 1, st);
 }
 
 // ---------------------------------------------------------------- ring permutation
-__global__ void perm_akci_kernel(const double2* __restrict__ X, double2* __restrict__ H, int nv,
-                                 int no, int accumulate) {
+struct PermAkciP { const double2* X; double2* H; int nv; int no; int accumulate; };
+__device__ __forceinline__ void body(const PermAkciP& p) {
+    const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
     GRID_LOOP(o, nvo * nvo) {
         const int a = (int)(o % nv);
         const int i = (int)((o / nv) % no);
         const int64_t cc = o / nvo;
         const int c = (int)(cc % nv), k = (int)(cc / nv);
-        const double2 x = X[(a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * i)];
-        H[o] = accumulate ? cadd(H[o], x) : x;
+        const double2 x = p.X[(a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * i)];
+        p.H[o] = p.accumulate ? cadd(p.H[o], x) : x;
     }
 }
 
 void k_perm_akci(const double2* X, double2* H, int nv, int no, bool accumulate,
                  hipStream_t st) {
     const int64_t nvo = (int64_t)nv * no;
-    if (nvo == 0) return;
-    hipLaunchKernelGGL(perm_akci_kernel, dim3(nblocks(nvo * nvo)), dim3(TPB), 0, st, X, H, nv,
-                       no, accumulate ? 1 : 0);
+    ew_launch(RK_PERM_AKCI, PermAkciP{X, H, nv, no, accumulate ? 1 : 0}, nvo * nvo, 1, st);
 }
 
 // ---------------------------------------------------------------- traces
-__global__ void trace_t1eqn2_kernel(const double2* __restrict__ X, double2* __restrict__ S1,
-                                    int nv, int no) {
+struct TraceT1P { const double2* X; double2* S1; int nv; int no; };
+__device__ __forceinline__ void body(const TraceT1P& p) {
+    const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
     GRID_LOOP(e, nvo) {
         const int a = (int)(e % nv), i = (int)(e / nv);
-        double2 s = S1[e];
+        double2 s = p.S1[e];
         for (int k = 0; k < no; ++k) {
             if (k == i) continue;
-            const double2 x = X[(a + (int64_t)nv * k) + nvo * (i > k ? pair_of(i, k, no) : pair_of(k, i, no))];
+            const double2 x =
+                p.X[(a + (int64_t)nv * k) + nvo * (i > k ? pair_of(i, k, no) : pair_of(k, i, no))];
             if (i > k) { s.x += x.x; s.y += x.y; }
             else { s.x -= x.x; s.y -= x.y; }
         }
-        S1[e] = s;
+        p.S1[e] = s;
     }
 }
 
 void k_trace_t1eqn2(const double2* X, double2* S1, PairTab oo, int nv, int no, hipStream_t st) {
     (void)oo;
-    const int64_t n = (int64_t)nv * no;
-    if (n == 0) return;
-    hipLaunchKernelGGL(trace_t1eqn2_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, X, S1, nv, no);
+    ew_launch(RK_TRACE_T1, TraceT1P{X, S1, nv, no}, (int64_t)nv * no, 1, st);
 }
 
-__global__ void trace_gv_kernel(const double2* __restrict__ H1, double2* __restrict__ GV,
-                                double alpha, int nv, int no) {
+struct TraceGvP { const double2* H1; double2* GV; double alpha; int nv; int no; };
+__device__ __forceinline__ void body(const TraceGvP& p) {
+    const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
     GRID_LOOP(e, (int64_t)nv * nv) {
         const int a = (int)(e % nv), c = (int)(e / nv);
         double sr = 0.0, si = 0.0;
         for (int k = 0; k < no; ++k) {
-            const double2 x = H1[(a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * k)];
+            const double2 x = p.H1[(a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * k)];
             sr += x.x;
             si += x.y;
         }
-        double2 g = GV[e];
-        g.x += alpha * sr;
-        g.y += alpha * si;
-        GV[e] = g;
+        double2 g = p.GV[e];
+        g.x += p.alpha * sr;
+        g.y += p.alpha * si;
+        p.GV[e] = g;
     }
 }
 
 void k_trace_gv(const double2* H1, double2* GV, double alpha, int nv, int no, hipStream_t st) {
-    const int64_t n = (int64_t)nv * nv;
-    if (n == 0) return;
-    hipLaunchKernelGGL(trace_gv_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, H1, GV, alpha, nv,
-                       no);
+    ew_launch(RK_TRACE_GV, TraceGvP{H1, GV, alpha, nv, no}, (int64_t)nv * nv, 1, st);
 }
 
 // ---------------------------------------------------------------- adjoints of the traces
@@ -370,80 +392,85 @@ void k_hadamard(int64_t n, const double2* x, const double2* y, double2* z, hipSt
 }
 
 // ---------------------------------------------------------------- denominators
-__global__ void denom_t1_kernel(double2* __restrict__ S1, const double* __restrict__ eps,
-                                const int* __restrict__ frozen, int nv, int no) {
-    GRID_LOOP(e, (int64_t)nv * no) {
-        const int a = (int)(e % nv), i = (int)(e / nv);
-        double2 s = S1[e];
-        if (eps) {
-            const double d = eps[i] - eps[no + a];
+struct Denom1P { double2* S1; const double* eps; const int* frozen; int nv; int no; };
+__device__ __forceinline__ void body(const Denom1P& p) {
+    GRID_LOOP(e, (int64_t)p.nv * p.no) {
+        const int a = (int)(e % p.nv), i = (int)(e / p.nv);
+        double2 s = p.S1[e];
+        if (p.eps) {
+            const double d = p.eps[i] - p.eps[p.no + a];
             s.x /= d;
             s.y /= d;
         }
-        if (frozen && frozen[i]) s = make_double2(0.0, 0.0);
-        S1[e] = s;
+        if (p.frozen && p.frozen[i]) s = make_double2(0.0, 0.0);
+        p.S1[e] = s;
     }
 }
 
-__global__ void denom_t2_kernel(double2* __restrict__ S2, const double* __restrict__ eps,
-                                const int* __restrict__ frozen, const int2* __restrict__ vv,
-                                int64_t nvvt, const int2* __restrict__ oo, int64_t noot, int no) {
-    GRID_LOOP(e, nvvt * noot) {
-        const int2 p = vv[e % nvvt], q = oo[e / nvvt];
-        double2 s = S2[e];
-        if (eps) {
-            const double d = eps[q.x] + eps[q.y] - eps[no + p.x] - eps[no + p.y];
+struct Denom2P { double2* S2; const double* eps; const int* frozen; const int2* vv; int64_t nvvt;
+                 const int2* oo; int64_t noot; int no; };
+__device__ __forceinline__ void body(const Denom2P& p) {
+    GRID_LOOP(e, p.nvvt * p.noot) {
+        const int2 q1 = p.vv[e % p.nvvt], q2 = p.oo[e / p.nvvt];
+        double2 s = p.S2[e];
+        if (p.eps) {
+            const double d = p.eps[q2.x] + p.eps[q2.y] - p.eps[p.no + q1.x] - p.eps[p.no + q1.y];
             s.x /= d;
             s.y /= d;
         }
-        if (frozen && (frozen[q.x] || frozen[q.y])) s = make_double2(0.0, 0.0);
-        S2[e] = s;
+        if (p.frozen && (p.frozen[q2.x] || p.frozen[q2.y])) s = make_double2(0.0, 0.0);
+        p.S2[e] = s;
     }
 }
 
 void k_denom(double2* S1, double2* S2, const double* eps, const int* frozen, PairTab vv,
              PairTab oo, int nv, int no, hipStream_t st) {
-    if (S1 && (int64_t)nv * no > 0)
-        hipLaunchKernelGGL(denom_t1_kernel, dim3(nblocks((int64_t)nv * no)), dim3(TPB), 0, st,
-                           S1, eps, frozen, nv, no);
-    const int64_t n2 = vv.npair * oo.npair;
-    if (S2 && n2 > 0)
-        hipLaunchKernelGGL(denom_t2_kernel, dim3(nblocks(n2)), dim3(TPB), 0, st, S2, eps, frozen,
-                           vv.tab, vv.npair, oo.tab, oo.npair, no);
+    if (S1) ew_launch(RK_DENOM1, Denom1P{S1, eps, frozen, nv, no}, (int64_t)nv * no, 1, st);
+    if (S2)
+        ew_launch(RK_DENOM2, Denom2P{S2, eps, frozen, vv.tab, vv.npair, oo.tab, oo.npair, no},
+                  vv.npair * oo.npair, 1, st);
 }
 
 // ---------------------------------------------------------------- small BLAS-1
-__global__ void axpby_kernel(int64_t n, double2 a, const double2* __restrict__ x, double2 b,
-                             double2* __restrict__ y) {
-    GRID_LOOP(e, n) {
+struct AxpbyP { int64_t n; double2 a; const double2* x; double2 b; double2* y; };
+__device__ __forceinline__ void body(const AxpbyP& p) {
+    GRID_LOOP(e, p.n) {
         double2 r = make_double2(0.0, 0.0);
-        if (b.x != 0.0 || b.y != 0.0) r = cmul(b, y[e]);
-        if (x) r = cadd(r, cmul(a, x[e]));
-        y[e] = r;
+        if (p.b.x != 0.0 || p.b.y != 0.0) r = cmul(p.b, p.y[e]);
+        if (p.x) r = cadd(r, cmul(p.a, p.x[e]));
+        p.y[e] = r;
     }
 }
 
 void k_axpby(int64_t n, double2 a, const double2* x, double2 b, double2* y, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(axpby_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, n, a, x, b, y);
+    ew_launch(RK_AXPBY, AxpbyP{n, a, x, b, y}, n, 1, st);
 }
 
-__global__ void conj_kernel(int64_t n, const double2* __restrict__ x, double2* __restrict__ y) {
-    GRID_LOOP(e, n) { y[e] = make_double2(x[e].x, -x[e].y); }
+struct ConjP { int64_t n; const double2* x; double2* y; };
+__device__ __forceinline__ void body(const ConjP& p) {
+    GRID_LOOP(e, p.n) { p.y[e] = make_double2(p.x[e].x, -p.x[e].y); }
 }
 
 void k_conj(int64_t n, const double2* x, double2* y, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(conj_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, n, x, y);
+    ew_launch(RK_CONJ, ConjP{n, x, y}, n, 1, st);
 }
 
-__global__ void zero_diag_kernel(double2* M, int n) {
-    GRID_LOOP(e, n) { M[e + (int64_t)n * e] = make_double2(0.0, 0.0); }
+struct CopyP { int64_t n; const double2* x; double2* y; };
+__device__ __forceinline__ void body(const CopyP& p) {
+    GRID_LOOP(e, p.n) { p.y[e] = p.x[e]; }
+}
+
+void k_copy(int64_t n, const double2* x, double2* y, hipStream_t st) {
+    ew_launch(RK_COPY, CopyP{n, x, y}, n, 1, st);
+}
+
+struct ZeroDiagP { double2* M; int n; };
+__device__ __forceinline__ void body(const ZeroDiagP& p) {
+    GRID_LOOP(e, p.n) { p.M[e + (int64_t)p.n * e] = make_double2(0.0, 0.0); }
 }
 
 void k_zero_diag(double2* M, int n, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(zero_diag_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, M, n);
+    ew_launch(RK_ZERO_DIAG, ZeroDiagP{M, n}, n, 1, st);
 }
 
 __global__ void gather_kernel(const double2* __restrict__ in, const int64_t* __restrict__ idx,
@@ -637,10 +664,6 @@ __global__ void gather_block_kernel(const double2* __restrict__ src, int64_t ld,
     }
 }
 
-__global__ void fill_zero_kernel(double2* __restrict__ p, int64_t n) {
-    GRID_LOOP(e, n) { p[e] = make_double2(0.0, 0.0); }
-}
-
 void k_scatter_block(const double2* src, int64_t rows, int64_t ncols, const int* rowmap,
                      const int* colmap, double2* dst, int64_t ld, hipStream_t st) {
     const int64_t n = rows * ncols;
@@ -658,16 +681,17 @@ void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int
 }
 
 // All blocks of a layout in one launch: blockIdx.y = block.
-__global__ void scatter_set_kernel(const double2* __restrict__ src, BlockSet bs,
-                                   double2* __restrict__ dst, int64_t ld, int gather) {
-    const BlockSet::Blk& b = bs.b[blockIdx.y];
+struct BlockSetP { const double2* src; BlockSet bs; double2* dst; int64_t ld; int gather; };
+__device__ __forceinline__ void body(const BlockSetP& p) {
+    if ((int)blockIdx.y >= p.bs.nb) return;
+    const BlockSet::Blk& b = p.bs.b[blockIdx.y];
     const int64_t n = b.rows * b.cols;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e % b.rows, c = e / b.rows;
-        const int64_t c1 = (int64_t)b.rmap[r] + ld * b.cmap[c];
-        if (gather) dst[b.off + e] = src[c1];
-        else dst[c1] = src[b.off + e];
+        const int64_t c1 = (int64_t)b.rmap[r] + p.ld * b.cmap[c];
+        if (p.gather) p.dst[b.off + e] = p.src[c1];
+        else p.dst[c1] = p.src[b.off + e];
     }
 }
 
@@ -675,14 +699,397 @@ void k_blockset(const double2* src, const BlockSet& bs, double2* dst, int64_t ld
                 hipStream_t st) {
     int64_t mx = 0;
     for (int i = 0; i < bs.nb; ++i) mx = std::max<int64_t>(mx, bs.b[i].rows * bs.b[i].cols);
-    if (bs.nb <= 0 || mx <= 0) return;
-    hipLaunchKernelGGL(scatter_set_kernel, dim3(nblocks(mx), bs.nb), dim3(TPB), 0, st, src, bs,
-                       dst, ld, gather ? 1 : 0);
+    ew_launch(RK_BLOCKSET, BlockSetP{src, bs, dst, ld, gather ? 1 : 0}, mx, bs.nb, st);
+}
+
+struct FillP { double2* p; int64_t n; };
+__device__ __forceinline__ void body(const FillP& p) {
+    GRID_LOOP(e, p.n) { p.p[e] = make_double2(0.0, 0.0); }
 }
 
 void k_fill_zero(double2* p, int64_t n, hipStream_t st) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, p, n);
+    ew_launch(RK_FILL, FillP{p, n}, n, 1, st);
+}
+
+// ---------------------------------------------------------------- batched replay
+// The split-K reduction of zgemm as an element-wise body (sums the partials
+// in split order, then alpha / beta), for replayed split products.
+struct SplitReduceP { int m; int n; int nsplit; int batch; const double2* W; double2* C;
+                      int64_t ldc; int64_t sC; double2 alpha; double2 beta; };
+__device__ __forceinline__ void body(const SplitReduceP& p) {
+    const int64_t mn = (int64_t)p.m * p.n;
+    GRID_LOOP(t, mn * p.batch) {
+        const int64_t b = t / mn, e = t - b * mn;
+        const int64_t i = e % p.m, j = e / p.m;
+        double sr = 0.0, si = 0.0;
+        const double2* w = p.W + b * p.nsplit * mn + e;
+        for (int s = 0; s < p.nsplit; ++s) {
+            const double2 v = w[(int64_t)s * mn];
+            sr += v.x;
+            si += v.y;
+        }
+        double2* cp = p.C + b * p.sC + i + j * p.ldc;
+        double2 o;
+        o.x = p.alpha.x * sr - p.alpha.y * si;
+        o.y = p.alpha.x * si + p.alpha.y * sr;
+        if (p.beta.x != 0.0 || p.beta.y != 0.0) {
+            const double2 c = *cp;
+            o.x += p.beta.x * c.x - p.beta.y * c.y;
+            o.y += p.beta.x * c.y + p.beta.y * c.x;
+        }
+        *cp = o;
+    }
+}
+
+static size_t param_size(int kind) {
+    switch (kind) {
+        case RK_GETTAU: return sizeof(GettauP);
+        case RK_EXPAND_E2: return sizeof(ExpandE2P);
+        case RK_EXPAND_F1: return sizeof(ExpandF1P);
+        case RK_EXPAND_RING: return sizeof(ExpandRingP);
+        case RK_PACK_PIJ: return sizeof(PackPijP);
+        case RK_PACK_PAB: return sizeof(PackPabP);
+        case RK_PACK_PABIJ: return sizeof(PackPabijP);
+        case RK_PERM_AKCI: return sizeof(PermAkciP);
+        case RK_TRACE_T1: return sizeof(TraceT1P);
+        case RK_TRACE_GV: return sizeof(TraceGvP);
+        case RK_DENOM1: return sizeof(Denom1P);
+        case RK_DENOM2: return sizeof(Denom2P);
+        case RK_AXPBY: return sizeof(AxpbyP);
+        case RK_CONJ: return sizeof(ConjP);
+        case RK_COPY: return sizeof(CopyP);
+        case RK_ZERO_DIAG: return sizeof(ZeroDiagP);
+        case RK_BLOCKSET: return sizeof(BlockSetP);
+        case RK_FILL: return sizeof(FillP);
+        case RK_SPLIT_REDUCE: return sizeof(SplitReduceP);
+        default: return 0;
+    }
+}
+
+// One element-wise kind for `count` handles, parameters at dp (device).  The
+// handles' sizes differ, so the grid is not sized for the largest one times
+// the count (mostly empty workgroups): at most 64K workgroups in all, each
+// striding over its handle's elements.
+static int ew_many_launch(int kind, const void* dp, int count, int64_t maxn, int maxgy,
+                          hipStream_t st) {
+    if (count <= 0 || maxn <= 0 || maxgy <= 0) return 0;
+    const int64_t cap = std::max<int64_t>(1, 65536 / ((int64_t)count * maxgy));
+    const dim3 grid((unsigned)std::min<int64_t>(nblocks(maxn), cap), maxgy, count);
+#define RCC_EW(K, P)                                                                       \
+    case K:                                                                                \
+        hipLaunchKernelGGL(ew_many<P>, grid, dim3(TPB), 0, st, (const P*)dp);              \
+        break;
+    switch (kind) {
+        RCC_EW(RK_GETTAU, GettauP)
+        RCC_EW(RK_EXPAND_E2, ExpandE2P)
+        RCC_EW(RK_EXPAND_F1, ExpandF1P)
+        RCC_EW(RK_EXPAND_RING, ExpandRingP)
+        RCC_EW(RK_PACK_PIJ, PackPijP)
+        RCC_EW(RK_PACK_PAB, PackPabP)
+        RCC_EW(RK_PACK_PABIJ, PackPabijP)
+        RCC_EW(RK_PERM_AKCI, PermAkciP)
+        RCC_EW(RK_TRACE_T1, TraceT1P)
+        RCC_EW(RK_TRACE_GV, TraceGvP)
+        RCC_EW(RK_DENOM1, Denom1P)
+        RCC_EW(RK_DENOM2, Denom2P)
+        RCC_EW(RK_AXPBY, AxpbyP)
+        RCC_EW(RK_CONJ, ConjP)
+        RCC_EW(RK_COPY, CopyP)
+        RCC_EW(RK_ZERO_DIAG, ZeroDiagP)
+        RCC_EW(RK_BLOCKSET, BlockSetP)
+        RCC_EW(RK_FILL, FillP)
+        RCC_EW(RK_SPLIT_REDUCE, SplitReduceP)
+        default: return -1;
+    }
+#undef RCC_EW
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// A compiled batch: every recorded position becomes one launch over all
+// handles (element-wise: one ew_many; product: one grouped GEMM over every
+// handle's (batch index, k split) pieces, plus one split reduction).
+struct BatchOp {
+    int kind = 0;
+    const void* dp = nullptr;     // element-wise parameters / grouped descriptors
+    int count = 0;
+    int64_t maxn = 0;
+    int maxgy = 1;
+    int opa = 0, opb = 0, tag = 0, gauss = 0;
+    int64_t blocks = 0;
+    const void* dmap = nullptr;   // grouped: block -> group map
+    // grouped: the same groups numbered in 32 x 32 tiles, and which tile the
+    // autotuner picked (batch_tune)
+    const void* dp32 = nullptr;
+    const void* dmap32 = nullptr;
+    int64_t blocks32 = 0;
+    int small = 0;
+    const void* dred = nullptr;   // split reductions (redcount entries) or null
+    int redcount = 0;
+    int64_t redn = 0;
+};
+
+struct BatchProgram {
+    std::vector<BatchOp> ops;
+    void* arena = nullptr;
+};
+
+static bool as_stored_op(int op) { return op == OP_N || op == OP_R; }
+
+int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out,
+                  std::string& err) {
+    *out = nullptr;
+    const int nh = (int)recs.size();
+    if (nh == 0) return 0;
+    const size_t npos = recs[0]->size();
+    for (int h = 1; h < nh; ++h)
+        if (recs[h]->size() != npos) {
+            err = "batch: handles record term sequences of different lengths";
+            return -1;
+        }
+    auto* prog = new BatchProgram();
+    std::vector<unsigned char> host;
+    auto put = [&](const void* p, size_t n) -> size_t {
+        size_t off = (host.size() + 255) / 256 * 256;
+        host.resize(off + n);
+        memcpy(host.data() + off, p, n);
+        return off;
+    };
+    struct Pending { size_t off, red, map, off32, map32; bool has_red; };
+    std::vector<Pending> pend;
+    for (size_t j = 0; j < npos; ++j) {
+        const Rec& r0 = (*recs[0])[j];
+        BatchOp op;
+        op.kind = r0.kind;
+        op.count = nh;
+        Pending pd{0, 0, 0, 0, 0, false};
+        for (int h = 1; h < nh; ++h) {
+            const Rec& r = (*recs[h])[j];
+            bool same = r.kind == r0.kind;
+            if (same && r0.kind == RK_GEMM)
+                same = r.g.opA == r0.g.opA && r.g.opB == r0.g.opB && r.g.tag == r0.g.tag &&
+                       r.g.gauss == r0.g.gauss;
+            if (same && r0.kind == RK_GROUPED)
+                same = r.opb == r0.opb && r.tag == r0.tag && r.gauss == r0.gauss;
+            if (!same) {
+                err = "batch: handles record different term sequences (position " +
+                      std::to_string(j) + ")";
+                delete prog;
+                return -1;
+            }
+        }
+        if (r0.kind == RK_GEMM || r0.kind == RK_GROUPED) {
+            std::vector<ZGroup> all;
+            std::vector<SplitReduceP> red(nh);
+            bool any_red = false;
+            // K splits were chosen to fill the chip with one handle; over many
+            // handles fewer (longer) splits do: scale them down until the
+            // launch has about 16K workgroups
+            std::vector<int> ns(nh, 1), kc(nh, 0);
+            int64_t total = 0;
+            for (int h = 0; h < nh; ++h) {
+                const Rec& r = (*recs[h])[j];
+                ns[h] = r.nsplit;
+                kc[h] = r.kchunk;
+                if (r.kind != RK_GEMM) continue;
+                const int64_t tl = (int64_t)((r.g.m + 63) / 64) * ((r.g.n + 63) / 64);
+                total += tl * std::max(r.g.batch, 0) * r.nsplit;
+            }
+            const int64_t f = (total + 16383) / 16384;
+            for (int h = 0; h < nh && f > 1; ++h) {
+                const Rec& r = (*recs[h])[j];
+                if (r.kind != RK_GEMM || r.nsplit <= 1 || r.g.k <= 0) continue;
+                const int want = (int)std::max<int64_t>(1, (r.nsplit + f - 1) / f);
+                int c = (r.g.k + want - 1) / want;
+                c = (c + 15) / 16 * 16;
+                ns[h] = (r.g.k + c - 1) / c;
+                kc[h] = ns[h] > 1 ? c : r.g.k;
+            }
+            for (int h = 0; h < nh; ++h) {
+                Rec r = (*recs[h])[j];
+                r.nsplit = ns[h];
+                r.kchunk = kc[h];
+                red[h] = SplitReduceP{0, 0, 1, 0, nullptr, nullptr, 1, 0, {0, 0}, {0, 0}};
+                if (r.kind == RK_GROUPED) {
+                    all.insert(all.end(), r.groups.begin(), r.groups.end());
+                    continue;
+                }
+                const GemmArgs& g = r.g;
+                if (g.m <= 0 || g.n <= 0 || g.batch <= 0) continue;
+                if (g.k <= 0) {
+                    err = "batch: a product with k = 0 has no grouped form";
+                    delete prog;
+                    return -1;
+                }
+                for (int b = 0; b < g.batch; ++b)
+                    for (int s = 0; s < r.nsplit; ++s) {
+                        const int kb = s * r.kchunk;
+                        const int kl = std::min(r.kchunk, g.k - kb);
+                        if (kl <= 0) continue;
+                        ZGroup z{};
+                        z.m = g.m; z.n = g.n; z.k = kl;
+                        z.A = g.A + b * g.sA + (as_stored_op(g.opA) ? (int64_t)kb * g.lda : kb);
+                        z.lda = g.lda;
+                        z.B = g.B + b * g.sB + (as_stored_op(g.opB) ? kb : (int64_t)kb * g.ldb);
+                        z.ldb = g.ldb;
+                        if (r.nsplit > 1) {
+                            z.C = r.ws + ((int64_t)b * r.nsplit + s) * g.m * g.n;
+                            z.ldc = g.m;
+                            z.alpha = make_double2(1.0, 0.0);
+                            z.beta = make_double2(0.0, 0.0);
+                        } else {
+                            z.C = g.C + b * g.sC;
+                            z.ldc = g.ldc;
+                            z.alpha = g.alpha;
+                            z.beta = g.beta;
+                        }
+                        all.push_back(z);
+                    }
+                if (r.nsplit > 1) {
+                    red[h] = SplitReduceP{g.m, g.n, r.nsplit, g.batch, r.ws, g.C, g.ldc, g.sC,
+                                          g.alpha, g.beta};
+                    any_red = true;
+                    op.redn = std::max<int64_t>(op.redn, (int64_t)g.m * g.n * g.batch);
+                }
+            }
+            op.kind = RK_GROUPED;
+            op.opa = r0.kind == RK_GEMM ? r0.g.opA : OP_N;
+            op.opb = r0.kind == RK_GEMM ? r0.g.opB : r0.opb;
+            op.tag = r0.kind == RK_GEMM ? r0.g.tag : r0.tag;
+            op.gauss = r0.kind == RK_GEMM ? r0.g.gauss : r0.gauss;
+            op.count = (int)all.size();
+            op.blocks = zgroup_offsets(all.data(), op.count);
+            if (!all.empty()) {
+                pd.off = put(all.data(), all.size() * sizeof(ZGroup));
+                std::vector<int> gm((size_t)op.blocks, 0);
+                for (int q = 0; q < op.count; ++q) {
+                    const int64_t end = q + 1 < op.count ? all[q + 1].tile_off : op.blocks;
+                    for (int64_t b = all[q].tile_off; b < end; ++b) gm[(size_t)b] = q;
+                }
+                pd.map = put(gm.data(), gm.size() * sizeof(int));
+                std::vector<ZGroup> s32 = all;
+                op.blocks32 = zgroup_offsets_small(s32.data(), op.count);
+                pd.off32 = put(s32.data(), s32.size() * sizeof(ZGroup));
+                std::vector<int> gm32((size_t)op.blocks32, 0);
+                for (int q = 0; q < op.count; ++q) {
+                    const int64_t end = q + 1 < op.count ? s32[q + 1].tile_off : op.blocks32;
+                    for (int64_t b = s32[q].tile_off; b < end; ++b) gm32[(size_t)b] = q;
+                }
+                pd.map32 = put(gm32.data(), gm32.size() * sizeof(int));
+            }
+            if (any_red) {
+                pd.red = put(red.data(), red.size() * sizeof(SplitReduceP));
+                pd.has_red = true;
+                op.redcount = nh;
+            }
+        } else {
+            const size_t ps = param_size(r0.kind);
+            if (ps == 0) {
+                err = "batch: a recorded launch has no batched form";
+                delete prog;
+                return -1;
+            }
+            std::vector<unsigned char> blob((size_t)nh * ps);
+            for (int h = 0; h < nh; ++h) {
+                const Rec& r = (*recs[h])[j];
+                memcpy(blob.data() + (size_t)h * ps, r.blob.data(), ps);
+                op.maxn = std::max<int64_t>(op.maxn, r.n);
+                op.maxgy = std::max(op.maxgy, r.gy);
+            }
+            pd.off = put(blob.data(), blob.size());
+        }
+        prog->ops.push_back(op);
+        pend.push_back(pd);
+    }
+    if (!host.empty()) {
+        if (hipMalloc(&prog->arena, host.size()) != hipSuccess ||
+            hipMemcpy(prog->arena, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            err = "batch: parameter upload failed";
+            if (prog->arena) (void)hipFree(prog->arena);
+            delete prog;
+            return -1;
+        }
+    }
+    for (size_t j = 0; j < prog->ops.size(); ++j) {
+        prog->ops[j].dp = (const unsigned char*)prog->arena + pend[j].off;
+        if (prog->ops[j].kind == RK_GROUPED && prog->ops[j].count > 0) {
+            prog->ops[j].dmap = (const unsigned char*)prog->arena + pend[j].map;
+            prog->ops[j].dp32 = (const unsigned char*)prog->arena + pend[j].off32;
+            prog->ops[j].dmap32 = (const unsigned char*)prog->arena + pend[j].map32;
+        }
+        if (pend[j].has_red) prog->ops[j].dred = (const unsigned char*)prog->arena + pend[j].red;
+    }
+    *out = prog;
+    return 0;
+}
+
+static int grouped_launch(const BatchOp& op, int small, hipStream_t st) {
+    if (op.count <= 0) return 0;
+    if (small)
+        return zgemm_grouped_small(op.opa, op.opb, op.gauss, (const ZGroup*)op.dp32, op.count,
+                                   op.blocks32, st, (const int*)op.dmap32);
+    return zgemm_grouped_ops(op.opa, op.opb, op.tag, op.gauss, (const ZGroup*)op.dp, op.count,
+                             op.blocks, st, (const int*)op.dmap);
+}
+
+// Tile autotuner of a compiled batch: every grouped product is timed with
+// the 64 x 64 and the 32 x 32 tile (HIP events, best of two runs after a
+// warm-up) and keeps the faster.  Run before the first replay: the products
+// then write only intermediates that the term sequence recomputes from T.
+int batch_tune(BatchProgram* p, hipStream_t st) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+    for (BatchOp& op : p->ops) {
+        if (op.kind != RK_GROUPED || op.count <= 0) continue;
+        float best[2] = {1e30f, 1e30f};
+        for (int v = 0; v < 2; ++v)
+            for (int rep = 0; rep < 3; ++rep) {
+                (void)hipEventRecord(e0, st);
+                if (grouped_launch(op, v, st) != 0) {
+                    best[v] = 1e30f;
+                    break;
+                }
+                (void)hipEventRecord(e1, st);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rep > 0) best[v] = std::min(best[v], ms);
+            }
+        op.small = best[1] < best[0] ? 1 : 0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+int batch_small_tiles(const BatchProgram* p, int* grouped) {
+    int s = 0, g = 0;
+    for (const BatchOp& op : p->ops)
+        if (op.kind == RK_GROUPED && op.count > 0) {
+            ++g;
+            s += op.small;
+        }
+    if (grouped) *grouped = g;
+    return s;
+}
+
+int batch_run(const BatchProgram* p, hipStream_t st) {
+    for (const BatchOp& op : p->ops) {
+        if (op.kind == RK_GROUPED) {
+            if (grouped_launch(op, op.small, st) != 0) return -1;
+            if (op.dred &&
+                ew_many_launch(RK_SPLIT_REDUCE, op.dred, op.redcount, op.redn, 1, st) != 0)
+                return -1;
+        } else {
+            if (ew_many_launch(op.kind, op.dp, op.count, op.maxn, op.maxgy, st) != 0) return -1;
+        }
+    }
+    return 0;
+}
+
+void batch_free(BatchProgram* p) {
+    if (!p) return;
+    if (p->arena) (void)hipFree(p->arena);
+    delete p;
 }
 
 }  // namespace rcc

@@ -137,6 +137,9 @@ struct rcc_handle {
             *X_s = nullptr, *H_s = nullptr, *RING_s = nullptr, *RING2_s = nullptr,
             *VVOOR_s = nullptr;
     ZGroup *g_ladder = nullptr, *g_x = nullptr, *g_ring1 = nullptr, *g_ring2 = nullptr;
+    // host copies of the descriptors (0 ladder, 1 X, 2 HINTM ring, 3 T2EQNS
+    // ring), concatenated across handles by rcc_iterate_batch_grouped
+    std::vector<ZGroup> hg[4];
     int ng_ladder = 0, ng_x = 0, ng_ring1 = 0, ng_ring2 = 0;
     int64_t nb_ladder = 0, nb_x = 0, nb_ring1 = 0, nb_ring2 = 0;
     bool vvoor_s_ok = false, tau_s_ok = false, ring_s_ok = false;
@@ -432,32 +435,48 @@ int ensure_split_tau(rcc_t* h) {
     return 0;
 }
 
+// The four dominant products (X = VOVV.tau, the HINTM and T2EQNS rings and
+// the ladder) are each split into the work before the product, the product
+// and the work after it, so that a batch of handles can run every handle's
+// "before" parts, one grouped launch of the product over all handles, and
+// the "after" parts (batch_iterate_grouped).
+int x_pre(rcc_t* h) {
+    CK(ensure_tau(h));
+    if (h->precision != 0) {
+        CK(ensure_split_const(h));
+        CK(ensure_split_tau(h));
+    }
+    if (h->sym.nrep > 1) k_fill_zero(h->X_s, lay_total(h, RCC_VOOO), h->st);
+    return 0;
+}
+
+int x_gemm(rcc_t* h) {
+    ktick(h, 3, 0);
+    if (h->sym.nrep > 1)    // X blocks: VOVV(c) TAU(c), one grouped launch over the irreps
+        CK(run_groups(h, OP_N, 0, h->g_x, h->ng_x, h->nb_x));
+    else if (h->precision != 0)
+        CK(sgemm(h, h->sVOVV, h->sTAU, h->X, h->nvo, cplx(1, 0), cplx(0, 0)));
+    else
+        CK(gemm(h, OP_N, OP_N, (int)h->nvo, (int)h->noot, (int)h->nvvt, cplx(1, 0),
+                h->cls[RCC_VOVV], h->nvo, h->TAU, h->nvvt, cplx(0, 0), h->X, h->nvo));
+    ktick(h, 3, 1);
+    return 0;
+}
+
+int x_post(rcc_t* h) {
+    if (h->sym.nrep > 1) {
+        k_fill_zero(h->X, h->nvo * h->noot, h->st);
+        sym_to_c1_dev(h, RCC_VOOO, h->X_s, h->X);
+    }
+    h->x_ok = true;
+    return 0;
+}
+
 int ensure_x(rcc_t* h) {
     if (!h->x_ok) {
-        CK(ensure_tau(h));
-        if (h->precision != 0) {
-            CK(ensure_split_const(h));
-            CK(ensure_split_tau(h));
-        }
-        if (h->sym.nrep > 1) {
-            // X blocks: VOVV(c) TAU(c), one grouped launch over the irreps
-            k_fill_zero(h->X_s, lay_total(h, RCC_VOOO), h->st);
-            ktick(h, 3, 0);
-            CK(run_groups(h, OP_N, 0, h->g_x, h->ng_x, h->nb_x));
-            ktick(h, 3, 1);
-            k_fill_zero(h->X, h->nvo * h->noot, h->st);
-            sym_to_c1_dev(h, RCC_VOOO, h->X_s, h->X);
-            h->x_ok = true;
-            return 0;
-        }
-        ktick(h, 3, 0);
-        if (h->precision != 0)
-            CK(sgemm(h, h->sVOVV, h->sTAU, h->X, h->nvo, cplx(1, 0), cplx(0, 0)));
-        else
-            CK(gemm(h, OP_N, OP_N, (int)h->nvo, (int)h->noot, (int)h->nvvt, cplx(1, 0),
-                    h->cls[RCC_VOVV], h->nvo, h->TAU, h->nvvt, cplx(0, 0), h->X, h->nvo));
-        ktick(h, 3, 1);
-        h->x_ok = true;
+        CK(x_pre(h));
+        CK(x_gemm(h));
+        CK(x_post(h));
     }
     return 0;
 }
@@ -473,7 +492,7 @@ int do_hointm(rcc_t* h) {
     CK(prep(h));
     CK(ensure_tau(h));
     const int o = h->no;
-    HIPCHK(h, hipMemcpyAsync(h->HO, h->FOO, (size_t)o * o * 16, hipMemcpyDeviceToDevice, h->st));
+    k_copy((int64_t)o * o, h->FOO, h->HO, h->st);
     k_zero_diag(h->HO, o, h->st);
     k_expand_e2(h->TAU, h->E2, h->nvvt, h->oo, o, h->st);
     const int64_t K = h->nvvt * o;
@@ -485,7 +504,7 @@ int do_hvintm(rcc_t* h) {
     CK(prep(h));
     CK(ensure_tau(h));
     const int v = h->nv;
-    HIPCHK(h, hipMemcpyAsync(h->HV, h->FVV, (size_t)v * v * 16, hipMemcpyDeviceToDevice, h->st));
+    k_copy((int64_t)v * v, h->FVV, h->HV, h->st);
     k_zero_diag(h->HV, v, h->st);
     k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, h->st);
     const int64_t K = (int64_t)v * h->noot;
@@ -511,7 +530,7 @@ int do_t1eqns(rcc_t* h) {
     double2* S1 = h->S1();
     double2* W = h->SMALL;             // v x o
     double2* B2 = h->SMALL + nvo;      // o x o
-    HIPCHK(h, hipMemcpyAsync(S1, h->FVO, (size_t)nvo * 16, hipMemcpyDeviceToDevice, st));
+    k_copy(nvo, h->FVO, S1, st);
     // T(A,K) [conj HOV - 2 F]^H T(C,I)   (cceqns.F:736-752)
     k_conj(nvo, h->FVO, W, st);
     k_axpby(nvo, cplx(1, 0), h->HOV, cplx(-2, 0), W, st);
@@ -537,7 +556,7 @@ int do_t1eqns(rcc_t* h) {
 int do_gointm(rcc_t* h) {
     CK(prep(h));
     const int o = h->no, v = h->nv;
-    HIPCHK(h, hipMemcpyAsync(h->GO, h->HO, (size_t)o * o * 16, hipMemcpyDeviceToDevice, h->st));
+    k_copy((int64_t)o * o, h->HO, h->GO, h->st);
     CK(gemm(h, OP_C, OP_N, o, o, v, cplx(1, 0), h->FVO, v, h->T1(), v, cplx(1, 0), h->GO, o));
     return gemm(h, OP_T, OP_N, o * o, 1, (int)h->nvo, cplx(-1, 0), h->MGO, h->nvo, h->T1(),
                 h->nvo, cplx(1, 0), h->GO, (int64_t)o * o);
@@ -546,7 +565,7 @@ int do_gointm(rcc_t* h) {
 int do_gvintm(rcc_t* h) {
     CK(prep(h));
     const int o = h->no, v = h->nv;
-    HIPCHK(h, hipMemcpyAsync(h->GV, h->HV, (size_t)v * v * 16, hipMemcpyDeviceToDevice, h->st));
+    k_copy((int64_t)v * v, h->HV, h->GV, h->st);
     CK(gemm(h, OP_N, OP_C, v, v, o, cplx(-1, 0), h->T1(), v, h->FVO, v, cplx(1, 0), h->GV, v));
     // HAKCI[(a,k),(c,i)] = - sum_d <ak||cd> t(d,i); its (k = i) trace is the
     // GV term W(AK,CD) T(D,K) (ccintm.F:208-218); HINTM reuses HAKCI.
@@ -561,8 +580,7 @@ int do_aintm(rcc_t* h) {
     CK(ensure_tau(h));
     const int o = h->no, v = h->nv;
     const int64_t noot = h->noot;
-    HIPCHK(h, hipMemcpyAsync(h->AZ, h->cls[RCC_OOOO], (size_t)noot * noot * 16,
-                             hipMemcpyDeviceToDevice, h->st));
+    k_copy(noot * noot, h->cls[RCC_OOOO], h->AZ, h->st);
     if (o > 0 && noot > 0) {
         // YAZ[kl + noot*(j + o*i)] = sum_c conj VOOO[(c,i),kl] t(c,j)
         CK(gemm(h, OP_C, OP_N, (int)noot, o, v, cplx(1, 0), h->cls[RCC_VOOO], h->nvo, h->T1(), v,
@@ -573,7 +591,7 @@ int do_aintm(rcc_t* h) {
                 h->nvvt, h->TAU, h->nvvt, cplx(1, 0), h->AZ, noot);
 }
 
-int do_hintm(rcc_t* h, bool hakci_has_vovv) {
+int hintm_pre(rcc_t* h, bool hakci_has_vovv) {
     CK(prep(h));
     const int o = h->no, v = h->nv;
     const int64_t nvo = h->nvo;
@@ -593,28 +611,40 @@ int do_hintm(rcc_t* h, bool hakci_has_vovv) {
     if (h->ring_grouped) {
         c1_to_sym_dev(h, RCC_VOVO, h->H, h->H_s);
         c1_to_sym_dev(h, RCC_VOVO, h->RING2, h->RING2_s);
-        ktick(h, 1, 0);
-        CK(run_groups(h, OP_C, 2, h->g_ring1, h->ng_ring1, h->nb_ring1));
-        ktick(h, 1, 1);
-        sym_to_c1_dev(h, RCC_VOVO, h->H_s, h->H);
-        return 0;
-    }
-    if (h->precision != 0) {
+    } else if (h->precision != 0) {
         CK(ensure_split_const(h));
         CK(do_split(h, h->sP, (int)nvo, nvo, h->RING2, nvo, false, false));
-        ktick(h, 1, 0);
-        CK(sgemm(h, h->sP, h->sVVOOR, h->H, nvo, cplx(1, 0), cplx(1, 0)));
-        ktick(h, 1, 1);
-        return 0;
     }
+    return 0;
+}
+
+// + [1/2 T(AD,IL) - T(A,L) T(D,I)] W(KL,CD)  (ccintm.F:393-450)
+int hintm_gemm(rcc_t* h) {
+    const int64_t nvo = h->nvo;
     ktick(h, 1, 0);
-    CK(gemm(h, OP_N, OP_C, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->RING2, nvo, h->VVOO_R,
-            nvo, cplx(1, 0), h->H, nvo, 1, 0, 0, 0, 2));
+    if (h->ring_grouped)
+        CK(run_groups(h, OP_C, 2, h->g_ring1, h->ng_ring1, h->nb_ring1));
+    else if (h->precision != 0)
+        CK(sgemm(h, h->sP, h->sVVOOR, h->H, nvo, cplx(1, 0), cplx(1, 0)));
+    else
+        CK(gemm(h, OP_N, OP_C, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->RING2, nvo,
+                h->VVOO_R, nvo, cplx(1, 0), h->H, nvo, 1, 0, 0, 0, 2));
     ktick(h, 1, 1);
     return 0;
 }
 
-int do_t2eqns(rcc_t* h) {
+int hintm_post(rcc_t* h) {
+    if (h->ring_grouped) sym_to_c1_dev(h, RCC_VOVO, h->H_s, h->H);
+    return 0;
+}
+
+int do_hintm(rcc_t* h, bool hakci_has_vovv) {
+    CK(hintm_pre(h, hakci_has_vovv));
+    CK(hintm_gemm(h));
+    return hintm_post(h);
+}
+
+int t2eqns_pre(rcc_t* h) {
     CK(prep(h));
     CK(ensure_tau(h));
     CK(ensure_ring(h));
@@ -622,7 +652,7 @@ int do_t2eqns(rcc_t* h) {
     const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
     hipStream_t st = h->st;
     double2* S2 = h->S2();
-    HIPCHK(h, hipMemcpyAsync(S2, h->cls[RCC_VVOO], (size_t)h->nt2 * 16, hipMemcpyDeviceToDevice, st));
+    k_copy(h->nt2, h->cls[RCC_VVOO], S2, st);
     // TAU(AB,KL) A(KL,IJ)  (cceqns.F:1013-1026)
     CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)noot, cplx(1, 0), h->TAU, nvvt, h->AZ, noot,
             cplx(1, 0), S2, nvvt));
@@ -658,62 +688,89 @@ int do_t2eqns(rcc_t* h) {
             h->ring_s_ok = true;
         }
         k_fill_zero(h->RING2_s, lay_total(h, RCC_VOVO), st);
-        ktick(h, 2, 0);
-        CK(run_groups(h, OP_N, 2, h->g_ring2, h->ng_ring2, h->nb_ring2));
-        ktick(h, 2, 1);
-        k_fill_zero(h->RING2, nvo * nvo, st);
-        sym_to_c1_dev(h, RCC_VOVO, h->RING2_s, h->RING2);
     } else if (h->precision != 0) {
         CK(do_split(h, h->sH, (int)nvo, nvo, h->H, nvo, false, false));
         if (!h->split_ring_ok) {
             CK(do_split(h, h->sRING, (int)nvo, nvo, h->RING, nvo, true, false));
             h->split_ring_ok = true;
         }
-        ktick(h, 2, 0);
-        CK(sgemm(h, h->sH, h->sRING, h->RING2, nvo, cplx(1, 0), cplx(0, 0)));
-        ktick(h, 2, 1);
-    } else {
-        ktick(h, 2, 0);
-        CK(gemm(h, OP_N, OP_N, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->H, nvo, h->RING, nvo,
-                cplx(0, 0), h->RING2, nvo, 1, 0, 0, 0, 2));
-        ktick(h, 2, 1);
     }
-    k_pack_pabij(h->RING2, S2, h->vv, h->oo, v, o, st);
     return 0;
 }
 
-int do_bintm(rcc_t* h) {
+int t2eqns_gemm(rcc_t* h) {
+    const int64_t nvo = h->nvo;
+    ktick(h, 2, 0);
+    if (h->ring_grouped)
+        CK(run_groups(h, OP_N, 2, h->g_ring2, h->ng_ring2, h->nb_ring2));
+    else if (h->precision != 0)
+        CK(sgemm(h, h->sH, h->sRING, h->RING2, nvo, cplx(1, 0), cplx(0, 0)));
+    else
+        CK(gemm(h, OP_N, OP_N, (int)nvo, (int)nvo, (int)nvo, cplx(1, 0), h->H, nvo, h->RING, nvo,
+                cplx(0, 0), h->RING2, nvo, 1, 0, 0, 0, 2));
+    ktick(h, 2, 1);
+    return 0;
+}
+
+int t2eqns_post(rcc_t* h) {
+    if (h->ring_grouped) {
+        k_fill_zero(h->RING2, h->nvo * h->nvo, h->st);
+        sym_to_c1_dev(h, RCC_VOVO, h->RING2_s, h->RING2);
+    }
+    k_pack_pabij(h->RING2, h->S2(), h->vv, h->oo, h->nv, h->no, h->st);
+    return 0;
+}
+
+int do_t2eqns(rcc_t* h) {
+    CK(t2eqns_pre(h));
+    CK(t2eqns_gemm(h));
+    return t2eqns_post(h);
+}
+
+int bintm_pre(rcc_t* h) {
     CK(prep(h));
     CK(ensure_tau(h));
     CK(ensure_x(h));
-    const int o = h->no, v = h->nv;
-    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
-    double2* S2 = h->S2();
-    // ladder: S(AB,IJ) += V(AB,CD) TAU(CD,IJ), C>D  (ccintm.F:700-722)
-    if (h->sym.nrep > 1) {
-        // S2(c) += VVVV(c) TAU(c) over the compound irreps, one grouped launch
-        c1_to_sym_dev(h, RCC_LAY_T2, S2, h->S2_s);
-        ktick(h, 0, 0);
-        CK(run_groups(h, OP_N, 1, h->g_ladder, h->ng_ladder, h->nb_ladder));
-        ktick(h, 0, 1);
-        sym_to_c1_dev(h, RCC_LAY_T2, h->S2_s, S2);
-    } else if (h->precision != 0) {
+    if (h->sym.nrep > 1) c1_to_sym_dev(h, RCC_LAY_T2, h->S2(), h->S2_s);
+    else if (h->precision != 0) {
         CK(ensure_split_const(h));
         CK(ensure_split_tau(h));
-        ktick(h, 0, 0);
-        CK(sgemm(h, h->sVVVV, h->sTAU, S2, nvvt, cplx(1, 0), cplx(1, 0)));
-        ktick(h, 0, 1);
-    } else {
-        ktick(h, 0, 0);
-        CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VVVV], nvvt,
-                h->TAU, nvvt, cplx(1, 0), S2, nvvt, 1, 0, 0, 0, 1));
-        ktick(h, 0, 1);
     }
+    return 0;
+}
+
+// ladder: S(AB,IJ) += V(AB,CD) TAU(CD,IJ), C>D  (ccintm.F:700-722); with
+// symmetry S2(c) += VVVV(c) TAU(c) over the compound irreps, one grouped launch
+int bintm_gemm(rcc_t* h) {
+    const int64_t noot = h->noot, nvvt = h->nvvt;
+    ktick(h, 0, 0);
+    if (h->sym.nrep > 1)
+        CK(run_groups(h, OP_N, 1, h->g_ladder, h->ng_ladder, h->nb_ladder));
+    else if (h->precision != 0)
+        CK(sgemm(h, h->sVVVV, h->sTAU, h->S2(), nvvt, cplx(1, 0), cplx(1, 0)));
+    else
+        CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VVVV], nvvt,
+                h->TAU, nvvt, cplx(1, 0), h->S2(), nvvt, 1, 0, 0, 0, 1));
+    ktick(h, 0, 1);
+    return 0;
+}
+
+int bintm_post(rcc_t* h) {
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot;
+    double2* S2 = h->S2();
+    if (h->sym.nrep > 1) sym_to_c1_dev(h, RCC_LAY_T2, h->S2_s, S2);
     // - P(ab) T(B,K) X(AK,IJ)  (ccintm.F:760-829)
     CK(gemm(h, OP_N, OP_T, v, v, o, cplx(-1, 0), h->X, v, h->T1(), v, cplx(0, 0), h->ZX, v,
             (int)noot, nvo, 0, (int64_t)v * v));
     k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, h->st);
     return 0;
+}
+
+int do_bintm(rcc_t* h) {
+    CK(bintm_pre(h));
+    CK(bintm_gemm(h));
+    return bintm_post(h);
 }
 
 // deterministic complex reductions
@@ -1262,7 +1319,7 @@ int64_t lay_total(const rcc_t* h, int lay) {
 // shape m(b) x k(b) etc. taken from the layouts; blocks with an empty side are left out.
 int make_groups(rcc_t* h, int la, const double2* A, int lb, const double2* B, int lc,
                 double2* C, bool bconj, double2 alpha, double2 beta, ZGroup** dG, int* ng,
-                int64_t* nblocks) {
+                int64_t* nblocks, std::vector<ZGroup>* host) {
     std::vector<ZGroup> g;
     for (int b = 0; b < h->sym.nrep; ++b) {
         int64_t ar, ac, ao, br, bc, bo, cr, cc, co;
@@ -1291,6 +1348,7 @@ int make_groups(rcc_t* h, int la, const double2* A, int lb, const double2* B, in
     }
     *ng = (int)g.size();
     *nblocks = zgroup_offsets(g.data(), *ng);
+    *host = g;
     CK(dalloc_raw(h, dG, std::max<int64_t>(1, (int64_t)g.size()) * (int64_t)sizeof(ZGroup)));
     if (!g.empty())
         HIPCHK(h, hipMemcpy(*dG, g.data(), g.size() * sizeof(ZGroup), hipMemcpyHostToDevice));
@@ -1298,6 +1356,18 @@ int make_groups(rcc_t* h, int la, const double2* A, int lb, const double2* B, in
 }
 
 int run_groups(rcc_t* h, int opb, int tag, const ZGroup* dG, int ng, int64_t nblocks) {
+    if (g_record) {
+        // recorded for a batched replay: the host copy of the same descriptors
+        const int k = dG == h->g_ladder ? 0 : dG == h->g_x ? 1 : dG == h->g_ring1 ? 2 : 3;
+        Rec r;
+        r.kind = RK_GROUPED;
+        r.groups = h->hg[k];
+        r.opb = opb;
+        r.tag = tag;
+        r.gauss = h->gemm_gauss;
+        g_record->push_back(std::move(r));
+        return 0;
+    }
     if (zgemm_grouped(opb, tag, h->gemm_gauss, dG, ng, nblocks, h->st) != 0) {
         h->err = "grouped GEMM launch failed";
         return -1;
@@ -1319,9 +1389,9 @@ int setup_grouped(rcc_t* h) {
     k_fill_zero(h->VVVV_s, lay_total(h, RCC_VVVV), h->st);
     k_fill_zero(h->VOVV_s, lay_total(h, RCC_VOVV), h->st);
     CK(make_groups(h, RCC_VVVV, h->VVVV_s, RCC_LAY_T2, h->TAU_s, RCC_LAY_T2, h->S2_s, false,
-                   cplx(1, 0), cplx(1, 0), &h->g_ladder, &h->ng_ladder, &h->nb_ladder));
+                   cplx(1, 0), cplx(1, 0), &h->g_ladder, &h->ng_ladder, &h->nb_ladder, &h->hg[0]));
     CK(make_groups(h, RCC_VOVV, h->VOVV_s, RCC_LAY_T2, h->TAU_s, RCC_VOOO, h->X_s, false,
-                   cplx(1, 0), cplx(0, 0), &h->g_x, &h->ng_x, &h->nb_x));
+                   cplx(1, 0), cplx(0, 0), &h->g_x, &h->ng_x, &h->nb_x, &h->hg[1]));
     if (h->ring_grouped) {
         const int64_t nr = lay_total(h, RCC_VOVO);
         CK(dalloc(h, &h->H_s, nr));
@@ -1329,9 +1399,11 @@ int setup_grouped(rcc_t* h) {
         CK(dalloc(h, &h->RING2_s, nr));
         CK(dalloc(h, &h->VVOOR_s, nr));
         CK(make_groups(h, RCC_VOVO, h->RING2_s, RCC_VOVO, h->VVOOR_s, RCC_VOVO, h->H_s, true,
-                       cplx(1, 0), cplx(1, 0), &h->g_ring1, &h->ng_ring1, &h->nb_ring1));
+                       cplx(1, 0), cplx(1, 0), &h->g_ring1, &h->ng_ring1, &h->nb_ring1,
+                       &h->hg[2]));
         CK(make_groups(h, RCC_VOVO, h->H_s, RCC_VOVO, h->RING_s, RCC_VOVO, h->RING2_s, false,
-                       cplx(1, 0), cplx(0, 0), &h->g_ring2, &h->ng_ring2, &h->nb_ring2));
+                       cplx(1, 0), cplx(0, 0), &h->g_ring2, &h->ng_ring2, &h->nb_ring2,
+                       &h->hg[3]));
     }
     HIPCHK(h, hipStreamSynchronize(h->st));
     return 0;
@@ -1747,6 +1819,260 @@ int lam_density(rcc_t* h, double* dens) {
     return 0;
 }
 
+
+// ---------------------------------------------------- grouped batch iteration
+// Many independent handles: the four dominant products of every handle (X =
+// VOVV.tau, the HINTM and T2EQNS rings, the ladder; per irrep block for a
+// symmetric handle, one block for a C1 one) run as ONE grouped launch per
+// term over all handles, on a batch stream that joins the handles' streams
+// with events; the rest of each handle's term sequence stays on its own
+// stream.  A plan (descriptors on the device, streams, events) is built once
+// per list of handles.
+struct BatchPlan {
+    std::vector<rcc_t*> hs;
+    ZGroup* dG[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ng[4] = {0, 0, 0, 0};
+    int64_t nb[4] = {0, 0, 0, 0};
+    int gauss = 1;
+    hipStream_t st = nullptr;
+    hipEvent_t evb = nullptr;
+    std::vector<hipEvent_t> evh;
+    // the whole term sequence of every handle, recorded once and compiled to
+    // one launch per position over all handles (null: handles differ in
+    // structure, the segmented form above runs instead); optionally replayed
+    // from a hipGraph
+    BatchProgram* prog = nullptr;
+    std::string prog_err;
+    int runs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+std::vector<BatchPlan*> g_plans;
+
+void free_plan(BatchPlan* p) {
+    if (p->st) (void)hipStreamSynchronize(p->st);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    batch_free(p->prog);
+    for (int k = 0; k < 4; ++k)
+        if (p->dG[k]) (void)hipFree(p->dG[k]);
+    for (hipEvent_t e : p->evh) (void)hipEventDestroy(e);
+    if (p->evb) (void)hipEventDestroy(p->evb);
+    if (p->st) (void)hipStreamDestroy(p->st);
+    delete p;
+}
+
+void drop_plans_of(rcc_t* h) {
+    for (size_t i = 0; i < g_plans.size();) {
+        BatchPlan* p = g_plans[i];
+        if (std::find(p->hs.begin(), p->hs.end(), h) != p->hs.end()) {
+            free_plan(p);
+            g_plans.erase(g_plans.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+// The descriptors of a C1 FP64 handle's four products (one group each).
+std::vector<ZGroup> c1_group(rcc_t* h, int kind) {
+    ZGroup z{};
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
+    z.alpha = cplx(1, 0);
+    if (kind == 0) {          // S2 += VVVV TAU
+        z.m = (int)nvvt; z.n = (int)noot; z.k = (int)nvvt;
+        z.A = h->cls[RCC_VVVV]; z.lda = nvvt; z.B = h->TAU; z.ldb = nvvt;
+        z.C = h->S2(); z.ldc = nvvt; z.beta = cplx(1, 0);
+    } else if (kind == 1) {   // X = VOVV TAU
+        z.m = (int)nvo; z.n = (int)noot; z.k = (int)nvvt;
+        z.A = h->cls[RCC_VOVV]; z.lda = nvo; z.B = h->TAU; z.ldb = nvvt;
+        z.C = h->X; z.ldc = nvo; z.beta = cplx(0, 0);
+    } else if (kind == 2) {   // H += RING2' VVOO_R^H
+        z.m = (int)nvo; z.n = (int)nvo; z.k = (int)nvo;
+        z.A = h->RING2; z.lda = nvo; z.B = h->VVOO_R; z.ldb = nvo;
+        z.C = h->H; z.ldc = nvo; z.beta = cplx(1, 0);
+    } else {                  // RING2 = H RING
+        z.m = (int)nvo; z.n = (int)nvo; z.k = (int)nvo;
+        z.A = h->H; z.lda = nvo; z.B = h->RING; z.ldb = nvo;
+        z.C = h->RING2; z.ldc = nvo; z.beta = cplx(0, 0);
+    }
+    z.lda = std::max<int64_t>(z.lda, 1);
+    z.ldb = std::max<int64_t>(z.ldb, 1);
+    z.ldc = std::max<int64_t>(z.ldc, 1);
+    if (z.m <= 0 || z.n <= 0 || z.k <= 0) return {};
+    return {z};
+}
+
+bool batch_groupable(rcc_t* const* hs, int n) {
+    for (int i = 0; i < n; ++i) {
+        rcc_t* h = hs[i];
+        if (h->precision != 0 || h->gemm_gauss != hs[0]->gemm_gauss) return false;
+        if (h->sym.nrep > 1 && !h->ring_grouped) return false;
+    }
+    return n > 0;
+}
+
+int get_plan(rcc_t* const* hs, int n, BatchPlan** out) {
+    std::vector<rcc_t*> key(hs, hs + n);
+    for (BatchPlan* p : g_plans)
+        if (p->hs == key) {
+            *out = p;
+            return 0;
+        }
+    auto* p = new BatchPlan();
+    p->hs = key;
+    p->gauss = hs[0]->gemm_gauss;
+    rcc_t* h0 = hs[0];
+    auto fail = [&](const std::string& m) {
+        h0->err = m;
+        free_plan(p);
+        return -1;
+    };
+    if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->evb, hipEventDisableTiming) != hipSuccess)
+        return fail("batch plan: stream / event creation failed");
+    for (int i = 0; i < n; ++i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return fail("batch plan: event creation failed");
+        p->evh.push_back(e);
+    }
+    for (int k = 0; k < 4; ++k) {
+        std::vector<ZGroup> all;
+        for (int i = 0; i < n; ++i) {
+            const std::vector<ZGroup> g = hs[i]->sym.nrep > 1 ? hs[i]->hg[k] : c1_group(hs[i], k);
+            all.insert(all.end(), g.begin(), g.end());
+        }
+        p->ng[k] = (int)all.size();
+        p->nb[k] = zgroup_offsets(all.data(), p->ng[k]);
+        if (all.empty()) continue;
+        if (hipMalloc((void**)&p->dG[k], all.size() * sizeof(ZGroup)) != hipSuccess ||
+            hipMemcpy(p->dG[k], all.data(), all.size() * sizeof(ZGroup), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return fail("batch plan: descriptor upload failed");
+    }
+    // record every handle's term sequence (nothing is launched) and compile it
+    std::vector<std::vector<Rec>> recs((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        rcc_t* h = hs[i];
+        if (prep(h) != 0) return fail(h->err);
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        invalidate_t(h);
+        h->capturing = true;
+        g_record = &recs[i];
+        const int rc = run_terms(h);
+        g_record = nullptr;
+        h->capturing = false;
+        invalidate_t(h);
+        if (rc != 0) return fail("batch plan: recording failed: " + h->err);
+    }
+    std::vector<std::vector<Rec>*> ptrs;
+    for (auto& r : recs) ptrs.push_back(&r);
+    if (batch_compile(ptrs, &p->prog, p->prog_err) != 0) p->prog = nullptr;
+    if (p->prog && zgemm_autotune_enabled()) {
+        if (batch_tune(p->prog, p->st) != 0) return fail("batch plan: tile tuning failed");
+        HIPCHK(h0, hipStreamSynchronize(p->st));
+    }
+    g_plans.push_back(p);
+    *out = p;
+    return 0;
+}
+
+// The compiled term sequence of all handles on the batch stream, joined to
+// the handles' streams; with use_graph on every handle the sequence is
+// captured once (second call) and replayed from the graph afterwards.
+int batch_program_phase(BatchPlan* p, bool use_graph) {
+    rcc_t* h0 = p->hs[0];
+    for (size_t i = 0; i < p->hs.size(); ++i) {
+        rcc_t* h = p->hs[i];
+        HIPCHK(h, hipEventRecord(h->ev0, h->st));
+        HIPCHK(h0, hipEventRecord(p->evh[i], h->st));
+        HIPCHK(h0, hipStreamWaitEvent(p->st, p->evh[i], 0));
+    }
+    p->runs += 1;
+    if (use_graph && p->gexec) {
+        HIPCHK(h0, hipGraphLaunch(p->gexec, p->st));
+    } else if (use_graph && p->runs >= 2) {
+        HIPCHK(h0, hipStreamBeginCapture(p->st, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_run(p->prog, p->st);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(p->st, &g);
+        if (rc != 0 || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            h0->err = "batch: graph capture of the term sequence failed";
+            return -1;
+        }
+        p->graph = g;
+        HIPCHK(h0, hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+        HIPCHK(h0, hipGraphLaunch(p->gexec, p->st));
+    } else if (batch_run(p->prog, p->st) != 0) {
+        h0->err = "batch: replay launch failed";
+        return -1;
+    }
+    HIPCHK(h0, hipEventRecord(p->evb, p->st));
+    for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
+    return 0;
+}
+
+// One grouped product over all handles: the batch stream waits for every
+// handle's stream, runs the launch, and every handle's stream waits for it.
+int batch_product(BatchPlan* p, int kind) {
+    static const int opb[4] = {OP_N, OP_N, OP_C, OP_N};
+    static const int tag[4] = {1, 0, 2, 2};
+    rcc_t* h0 = p->hs[0];
+    for (size_t i = 0; i < p->hs.size(); ++i) {
+        HIPCHK(h0, hipEventRecord(p->evh[i], p->hs[i]->st));
+        HIPCHK(h0, hipStreamWaitEvent(p->st, p->evh[i], 0));
+    }
+    if (zgemm_grouped(opb[kind], tag[kind], p->gauss, p->dG[kind], p->ng[kind], p->nb[kind],
+                      p->st) != 0) {
+        h0->err = "batch grouped GEMM launch failed";
+        return -1;
+    }
+    HIPCHK(h0, hipEventRecord(p->evb, p->st));
+    for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
+    return 0;
+}
+
+// The term phase (HOINTM .. BINTM, DENOM) of every handle with the four
+// grouped products; the same arithmetic as run_terms, X formed first.
+int batch_terms_grouped(BatchPlan* p) {
+    for (rcc_t* h : p->hs) {
+        CK(prep(h));
+        HIPCHK(h, hipEventRecord(h->ev0, h->st));
+        CK(x_pre(h));
+    }
+    CK(batch_product(p, 1));
+    for (rcc_t* h : p->hs) {
+        CK(x_post(h));
+        CK(do_hointm(h));
+        CK(do_hvintm(h));
+        CK(do_hovintm(h));
+        CK(do_t1eqns(h));
+        CK(do_gointm(h));
+        CK(do_gvintm(h));
+        CK(do_aintm(h));
+        CK(hintm_pre(h, true));
+    }
+    CK(batch_product(p, 2));
+    for (rcc_t* h : p->hs) {
+        CK(hintm_post(h));
+        CK(t2eqns_pre(h));
+    }
+    CK(batch_product(p, 3));
+    for (rcc_t* h : p->hs) {
+        CK(t2eqns_post(h));
+        CK(bintm_pre(h));
+    }
+    CK(batch_product(p, 0));
+    for (rcc_t* h : p->hs) {
+        CK(bintm_post(h));
+        k_denom(h->S1(), h->S2(), h->eps, h->any_frozen ? h->frozen : nullptr, h->vv, h->oo,
+                h->nv, h->no, h->st);
+    }
+    return 0;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
@@ -1823,6 +2149,7 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
 
 void rcc_destroy(rcc_t* h) {
     if (!h) return;
+    drop_plans_of(h);
     if (h->st) (void)hipStreamSynchronize(h->st);
     drop_graph(h);
     if (h->hred) (void)hipHostFree(h->hred);
@@ -2149,6 +2476,33 @@ int rcc_iterate_batch(rcc_t* const* hs, int n, int* nit, double* ecc, double* rm
     return 0;
 }
 
+int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, double* rms) {
+    if (!hs || n < 0) return -1;
+    for (int i = 0; i < n; ++i)
+        if (!hs[i]) return -1;
+    if (n == 0) return 0;
+    if (!batch_groupable(hs, n)) {
+        hs[0]->err = "rcc_iterate_batch_grouped: every handle must have precision 0, the same "
+                     "complex-product form and, with symmetry, self-inverse irreps";
+        return -1;
+    }
+    BatchPlan* p = nullptr;
+    CK(get_plan(hs, n, &p));
+    bool graph = true;
+    for (int i = 0; i < n; ++i) graph = graph && hs[i]->use_graph;
+    if (p->prog) CK(batch_program_phase(p, graph));
+    else CK(batch_terms_grouped(p));
+    for (int i = 0; i < n; ++i) CK(finish_a(hs[i]));
+    for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
+    for (int i = 0; i < n; ++i) CK(finish_b(hs[i], rms ? rms + i : nullptr));
+    for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
+    for (int i = 0; i < n; ++i) {
+        CK(finish_c(hs[i], ecc ? ecc + i : nullptr));
+        if (nit) nit[i] = hs[i]->nit;
+    }
+    return 0;
+}
+
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms) {
     if (!h) return -1;
     const double conv = std::pow(10.0, -h->ntol);
@@ -2162,6 +2516,19 @@ int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms) {
     if (nit) *nit = n;
     if (ecc) *ecc = e;
     if (rms) *rms = r;
+    return 0;
+}
+
+int rcc_set_autotune(rcc_t* h, int enable) {
+    if (!h) return -1;
+    zgemm_autotune(enable);
+    return 0;
+}
+
+int rcc_autotune_entries(rcc_t* h, int64_t* fp64_shapes, int64_t* int8_shapes) {
+    if (!h) return -1;
+    if (fp64_shapes) *fp64_shapes = zgemm_tuned_shapes();
+    if (int8_shapes) *int8_shapes = i8_tuned_shapes();
     return 0;
 }
 

@@ -26,6 +26,9 @@
 #include <cmath>
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include "rcc_internal.h"
 
 namespace rcc {
@@ -403,27 +406,94 @@ static void launch_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_
                        (const i32x4*)B.planes, B.mx, C, ldc, alpha, beta);
 }
 
-int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, double2 alpha,
-             double2 beta, hipStream_t st) {
-    if (A.nks != B.nks || A.digits != B.digits) return -1;
-    const int m = A.rows, n = B.rows;
-    if (m <= 0 || n <= 0) return 0;
-    if (A.nks == 0) return -1;
-    static const int wm = [] {
-        const char* e = getenv("RCC_I8_WM");
-        return e ? atoi(e) : 4;
-    }();
-    // three LDS buffers of (2 WM + 4) S KiB: 144 KiB for (4, 4), 150 KiB for (5, 3)
-    if (A.digits == 4) {
+// Tile height WM (tile = 32 WM x 64): three LDS buffers of (2 WM + 4) S KiB,
+// 144 KiB for (4, 4), 150 KiB for (5, 3); rows_pad (a multiple of 384) is a
+// multiple of every 32 WM offered.
+static int launch_wm(int S, int wm, const I8Operand& A, const I8Operand& B, double2* C,
+                     int64_t ldc, double2 alpha, double2 beta, hipStream_t st) {
+    if (S == 4) {
         if (wm == 2) launch_i8<4, 2>(A, B, C, ldc, alpha, beta, st);
+        else if (wm == 3) launch_i8<4, 3>(A, B, C, ldc, alpha, beta, st);
         else launch_i8<4, 4>(A, B, C, ldc, alpha, beta, st);
-    } else if (A.digits == 5) {
+    } else if (S == 5) {
         if (wm == 2) launch_i8<5, 2>(A, B, C, ldc, alpha, beta, st);
         else launch_i8<5, 3>(A, B, C, ldc, alpha, beta, st);
     } else {
         return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// Tile-height autotuner: the first product of a shape (digits, m, n, k
+// steps) met outside graph capture times every tile height (WM 2, 3, 4 for
+// 4 digits; 2, 3 for 5) in place with alpha = 0, beta = 1 (C unchanged: the
+// first flush writes 1 C + 0, later flushes add 0) and keeps the fastest for
+// the process.  RCC_I8_WM fixes the height; with the autotuner off
+// (RCC_AUTOTUNE=0, zgemm_autotune(0)) it is 4 / 3.
+static std::mutex i8_tune_mu;
+static std::map<std::tuple<int, int, int, int>, int> i8_tuned;
+
+static int i8_choose_wm(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
+                        hipStream_t st) {
+    const int S = A.digits;
+    const int dflt = S == 4 ? 4 : 3;
+    const char* e = getenv("RCC_I8_WM");
+    if (e) {
+        const int w = atoi(e);
+        return (w == 2 || w == 3 || (w == 4 && S == 4)) ? w : dflt;
+    }
+    if (!zgemm_autotune_enabled()) return dflt;
+    std::lock_guard<std::mutex> lk(i8_tune_mu);
+    const auto key = std::make_tuple(S, A.rows, B.rows, A.nks);
+    auto it = i8_tuned.find(key);
+    if (it != i8_tuned.end()) return it->second;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return dflt;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return dflt;
+    int best = dflt;
+    float best_ms = 1e30f;
+    const int cands4[] = {2, 3, 4}, cands5[] = {2, 3};
+    const int* cands = S == 4 ? cands4 : cands5;
+    const int nc = S == 4 ? 3 : 2;
+    for (int c = 0; c < nc; ++c) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            (void)hipEventRecord(e0, st);
+            if (launch_wm(S, cands[c], A, B, C, ldc, make_double2(0, 0), make_double2(1, 0), st))
+                break;
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best = cands[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    i8_tuned[key] = best;
+    return best;
+}
+
+int64_t i8_tuned_shapes() {
+    std::lock_guard<std::mutex> lk(i8_tune_mu);
+    return (int64_t)i8_tuned.size();
+}
+
+int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, double2 alpha,
+             double2 beta, hipStream_t st) {
+    if (A.nks != B.nks || A.digits != B.digits) return -1;
+    const int m = A.rows, n = B.rows;
+    if (m <= 0 || n <= 0) return 0;
+    if (A.nks == 0) return -1;
+    if (A.digits != 4 && A.digits != 5) return -1;
+    const int wm = i8_choose_wm(A, B, C, ldc, st);
+    return launch_wm(A.digits, wm, A, B, C, ldc, alpha, beta, st);
 }
 
 }  // namespace rcc

@@ -97,6 +97,7 @@ def load():
     lib.rcc_probe_mfma.argtypes = [vp, dp]
     lib.rcc_set_graph.argtypes = [vp, ci]
     lib.rcc_iterate_batch.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
+    lib.rcc_iterate_batch_grouped.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
     lib.rcc_layout_block.argtypes = [vp, ci, ci, ctypes.POINTER(i64), ctypes.POINTER(i64),
                                      ctypes.POINTER(i64)]
     lib.rcc_zgemm_split_device.argtypes = [vp, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
@@ -105,6 +106,8 @@ def load():
     lib.rcc_probe_mfma_rate.argtypes = [vp, ci, ci, ci, dp]
     lib.rcc_synchronize.argtypes = [vp]
     lib.rcc_lambda.argtypes = [vp, ctypes.POINTER(ci), dp]
+    lib.rcc_set_autotune.argtypes = [vp, ci]
+    lib.rcc_autotune_entries.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     lib.rcc_set_lambda.argtypes = [vp, vp, vp]
     lib.rcc_get_lambda.argtypes = [vp, vp, vp]
     lib.rcc_lambda_vjp.argtypes = [vp, vp, vp, ci, vp, vp]
@@ -313,9 +316,10 @@ class RelCCSD:
         self._check(self.lib.rcc_set_graph(self.h, 1 if enable else 0))
 
     @staticmethod
-    def iterate_batch(solvers):
+    def iterate_batch(solvers, grouped=False):
         """One CCEQN iteration of every solver, their term phases in flight
-        together; returns a list of (nit, Ec, rms)."""
+        together (grouped: the four dominant products as one launch per term
+        over all solvers); returns a list of (nit, Ec, rms)."""
         n = len(solvers)
         if n == 0:
             return []
@@ -324,7 +328,8 @@ class RelCCSD:
         nit = (ctypes.c_int * n)()
         e = (ctypes.c_double * n)()
         r = (ctypes.c_double * n)()
-        rc = lib.rcc_iterate_batch(hs, n, nit, e, r)
+        fn = lib.rcc_iterate_batch_grouped if grouped else lib.rcc_iterate_batch
+        rc = fn(hs, n, nit, e, r)
         if rc != 0:
             msgs = [lib.rcc_last_error(s.h).decode() for s in solvers]
             raise RccError("; ".join(m for m in msgs if m) or "rcc_iterate_batch failed")
@@ -348,6 +353,15 @@ class RelCCSD:
         ms = ctypes.c_double()
         self._check(self.lib.rcc_last_iteration_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def set_autotune(self, enable: bool = True):
+        """Process-wide GEMM autotuner (K-split of FP64 products, INT8 tile height)."""
+        self._check(self.lib.rcc_set_autotune(self.h, 1 if enable else 0))
+
+    def autotune_entries(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.rcc_autotune_entries(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     # ------------------------------------------------------------ Lambda / density
     def lambda_solve(self):

@@ -79,3 +79,37 @@ def test_zgemm_conjugate_operand_against_numpy(opa, opb, m, n, k, three_mult):
     r.zgemm_device(opa, opb, m, n, k, alpha, tA, m, tB, k, beta, tC, m)
     got = tC.cpu().numpy().reshape(n, m).T
     assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
+
+
+@pytest.mark.parametrize("opa,opb,m,n,k,batch", [(2, 0, 7, 9, 40000, 1), (0, 0, 200, 20, 9000, 1),
+                                                  (1, 0, 64, 20, 30000, 1)])
+def test_autotuned_split_gemm_against_numpy(opa, opb, m, n, k, batch):
+    # first call of the shape: the autotuner times its K-splits in place
+    # (alpha 0, beta 1) before the real product, which must see C untouched
+    rng = np.random.default_rng(m + n + k)
+
+    def mat(r, c):
+        return rng.normal(size=(r, c)) + 1j * rng.normal(size=(r, c))
+
+    A = mat(m, k) if opa == 0 else mat(k, m)
+    B = mat(k, n)
+    C = mat(m, n)
+    alpha, beta = 0.3 + 0.1j, 0.9 - 0.4j
+
+    def op(x, o):
+        return x if o == 0 else (x.T if o == 1 else x.conj().T)
+
+    ref = alpha * op(A, opa) @ B + beta * C
+
+    def dev(x):
+        return torch.from_numpy(x.reshape(-1, order="F").copy()).to("cuda")
+
+    r = RelCCSD(2, 4)
+    r.set_autotune(True)
+    before = r.autotune_entries()[0]
+    for rep in range(2):            # tuned on the first call, cached on the second
+        tA, tB, tC = dev(A), dev(B), dev(C)
+        r.zgemm_device(opa, opb, m, n, k, alpha, tA, A.shape[0], tB, k, beta, tC, m)
+        got = tC.cpu().numpy().reshape(n, m).T
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max() * k ** 0.5
+    assert r.autotune_entries()[0] == before + 1
