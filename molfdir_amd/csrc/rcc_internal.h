@@ -73,7 +73,9 @@ int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 enum RecKind {
     RK_GETTAU, RK_EXPAND_E2, RK_EXPAND_F1, RK_EXPAND_RING, RK_PACK_PIJ, RK_PACK_PAB,
     RK_PACK_PABIJ, RK_PERM_AKCI, RK_TRACE_T1, RK_TRACE_GV, RK_DENOM1, RK_DENOM2, RK_AXPBY,
-    RK_CONJ, RK_COPY, RK_ZERO_DIAG, RK_BLOCKSET, RK_FILL, RK_SPLIT_REDUCE, RK_GEMM, RK_GROUPED
+    RK_CONJ, RK_COPY, RK_ZERO_DIAG, RK_BLOCKSET, RK_FILL, RK_SPLIT_REDUCE, RK_GEMM, RK_GROUPED,
+    // batched DIISX / CCENG stages (not recorded; parameters built per iteration)
+    RK_ZDOT_PART, RK_FINISH, RK_SUMNORM_PART, RK_LINCOMB, RK_DIIS_PREP
 };
 struct Rec {
     int kind = 0;
@@ -104,6 +106,28 @@ struct PairTab {
     int n;
     int64_t npair;
 };
+
+// Parameter blocks of the reductions and updates behind DIISX and CCENG, for
+// launching them over many handles at once (k_many).  Sizes by kind:
+// batch_param_size.  The single-handle k_* functions below use the same
+// kernels with the same partitioning, so batched and single results agree
+// bit for bit.
+int red_blocks(int64_t n, int nblk);
+void zdot_params(const double2* const* xs, int nx, const double2* y, int64_t n, double2* dst,
+                 double2* part, int nblk, void* zdot_part, void* finish);
+void sumnorm_params(const double2* x, int64_t n, double2* dst, double2* part, int nblk,
+                    void* sumnorm_part, void* finish);
+void lincomb_params(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
+                    void* lincomb);
+void diis_prep_params(const double2* S, const double2* T, double2* d, double2* amp, int64_t n,
+                      void* prep);
+void gettau_params(const double2* t1, const double2* t2, double2* tau, double s2, PairTab vv,
+                   PairTab oo, int nv, void* gettau);
+void copy_params(int64_t n, const double2* x, double2* y, void* copy);
+size_t batch_param_size(int kind);
+// gx: largest n (grid-stride kinds) or largest block count (reductions)
+int k_many(int kind, const void* dparams, int count, int64_t gx, hipStream_t st);
+
 
 void k_gettau(const double2* t1, const double2* t2, double2* tau, double s2,
               PairTab vv, PairTab oo, int nv, hipStream_t st);

@@ -506,18 +506,18 @@ struct CoefPack {
 };
 
 // Block b reduces elements [b*chunk, min(n,(b+1)*chunk)) of sum conj(x_v) * y for each v.
-__global__ void __launch_bounds__(TPB) zdotc_part_kernel(PtrPack xs, int nx,
-                                                         const double2* __restrict__ y, int64_t n,
-                                                         int64_t chunk, double2* __restrict__ part,
-                                                         int nblk) {
+struct ZdotPartP { PtrPack xs; int nx; const double2* y; int64_t n; int64_t chunk;
+                   double2* part; int nblk; };
+__device__ __forceinline__ void body(const ZdotPartP& p) {
     __shared__ double sh[2][TPB];
-    const int64_t beg = blockIdx.x * chunk;
-    const int64_t end = std::min<int64_t>(n, beg + chunk);
-    for (int v = 0; v < nx; ++v) {
-        const double2* __restrict__ x = xs.p[v];
+    if ((int)blockIdx.x >= p.nblk) return;      // whole workgroup (batched launches)
+    const int64_t beg = blockIdx.x * p.chunk;
+    const int64_t end = std::min<int64_t>(p.n, beg + p.chunk);
+    for (int v = 0; v < p.nx; ++v) {
+        const double2* __restrict__ x = p.xs.p[v];
         double sr = 0.0, si = 0.0;
         for (int64_t e = beg + threadIdx.x; e < end; e += TPB) {
-            const double2 a = x[e], b = y[e];
+            const double2 a = x[e], b = p.y[e];
             sr += a.x * b.x + a.y * b.y;
             si += a.x * b.y - a.y * b.x;
         }
@@ -531,21 +531,22 @@ __global__ void __launch_bounds__(TPB) zdotc_part_kernel(PtrPack xs, int nx,
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) part[v * nblk + blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
+        if (threadIdx.x == 0) p.part[v * p.nblk + blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
         __syncthreads();
     }
 }
 
 // One workgroup per reduced value: strided partial sums then a tree, both in a
 // fixed order, so the result does not depend on scheduling.
-__global__ void __launch_bounds__(TPB) finish_kernel(const double2* __restrict__ part, int nblk,
-                                                     int nv, double2* __restrict__ dst) {
+struct FinishP { const double2* part; int nblk; int nv; double2* dst; };
+__device__ __forceinline__ void body(const FinishP& p) {
     __shared__ double sh[2][TPB];
     const int v = blockIdx.x;
+    if (v >= p.nv) return;
     double sr = 0.0, si = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += TPB) {
-        sr += part[v * nblk + b].x;
-        si += part[v * nblk + b].y;
+    for (int b = threadIdx.x; b < p.nblk; b += TPB) {
+        sr += p.part[v * p.nblk + b].x;
+        si += p.part[v * p.nblk + b].y;
     }
     sh[0][threadIdx.x] = sr;
     sh[1][threadIdx.x] = si;
@@ -557,36 +558,48 @@ __global__ void __launch_bounds__(TPB) finish_kernel(const double2* __restrict__
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) dst[v] = make_double2(sh[0][0], sh[1][0]);
+    if (threadIdx.x == 0) p.dst[v] = make_double2(sh[0][0], sh[1][0]);
 }
 
 // partial-sum blocks for n elements: about 4096 elements each, at most nblk
-static inline int red_blocks(int64_t n, int nblk) {
+int red_blocks(int64_t n, int nblk) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(nblk, (n + 4095) / 4096));
+}
+
+void zdot_params(const double2* const* xs, int nx, const double2* y, int64_t n, double2* dst,
+                 double2* part, int nblk, void* zp, void* fp) {
+    ZdotPartP z{};
+    for (int v = 0; v < nx && v < MAXV; ++v) z.xs.p[v] = xs[v];
+    z.nx = nx;
+    z.y = y;
+    z.n = n;
+    z.nblk = red_blocks(n, nblk);
+    z.chunk = std::max<int64_t>((n + z.nblk - 1) / z.nblk, 1);
+    z.part = part;
+    *(ZdotPartP*)zp = z;
+    *(FinishP*)fp = FinishP{part, z.nblk, nx, dst};
 }
 
 void k_zdotc_multi(const double2* const* xs, int nx, const double2* y, int64_t n, double2* dst,
                    double2* part, int nblk, hipStream_t st) {
     if (nx <= 0) return;
-    PtrPack pk{};
-    for (int v = 0; v < nx && v < MAXV; ++v) pk.p[v] = xs[v];
-    nblk = red_blocks(n, nblk);
-    const int64_t chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(zdotc_part_kernel, dim3(nblk), dim3(TPB), 0, st, pk, nx, y, n,
-                       std::max<int64_t>(chunk, 1), part, nblk);
-    hipLaunchKernelGGL(finish_kernel, dim3(nx), dim3(TPB), 0, st, part, nblk, nx, dst);
+    ZdotPartP z;
+    FinishP f;
+    zdot_params(xs, nx, y, n, dst, part, nblk, &z, &f);
+    hipLaunchKernelGGL(ew_one<ZdotPartP>, dim3(z.nblk), dim3(TPB), 0, st, z);
+    hipLaunchKernelGGL(ew_one<FinishP>, dim3(nx), dim3(TPB), 0, st, f);
 }
 
 // dst[0] = sum x, dst[1] = (sum |x|^2, 0)
-__global__ void __launch_bounds__(TPB) sumnorm_part_kernel(const double2* __restrict__ x,
-                                                           int64_t n, int64_t chunk,
-                                                           double2* __restrict__ part, int nblk) {
+struct SumNormPartP { const double2* x; int64_t n; int64_t chunk; double2* part; int nblk; };
+__device__ __forceinline__ void body(const SumNormPartP& p) {
     __shared__ double sh[3][TPB];
-    const int64_t beg = blockIdx.x * chunk;
-    const int64_t end = std::min<int64_t>(n, beg + chunk);
+    if ((int)blockIdx.x >= p.nblk) return;
+    const int64_t beg = blockIdx.x * p.chunk;
+    const int64_t end = std::min<int64_t>(p.n, beg + p.chunk);
     double sr = 0.0, si = 0.0, nn = 0.0;
     for (int64_t e = beg + threadIdx.x; e < end; e += TPB) {
-        const double2 a = x[e];
+        const double2 a = p.x[e];
         sr += a.x;
         si += a.y;
         nn += a.x * a.x + a.y * a.y;
@@ -604,41 +617,127 @@ __global__ void __launch_bounds__(TPB) sumnorm_part_kernel(const double2* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
-        part[nblk + blockIdx.x] = make_double2(sh[2][0], 0.0);
+        p.part[blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
+        p.part[p.nblk + blockIdx.x] = make_double2(sh[2][0], 0.0);
     }
+}
+
+void sumnorm_params(const double2* x, int64_t n, double2* dst, double2* part, int nblk, void* sp,
+                    void* fp) {
+    SumNormPartP s{x, n, 1, part, red_blocks(n, nblk)};
+    s.chunk = std::max<int64_t>((n + s.nblk - 1) / s.nblk, 1);
+    *(SumNormPartP*)sp = s;
+    *(FinishP*)fp = FinishP{part, s.nblk, 2, dst};
 }
 
 void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, int nblk,
                     hipStream_t st) {
-    nblk = red_blocks(n, nblk);
-    const int64_t chunk = (n + nblk - 1) / nblk;
-    hipLaunchKernelGGL(sumnorm_part_kernel, dim3(nblk), dim3(TPB), 0, st, x, n,
-                       std::max<int64_t>(chunk, 1), part, nblk);
-    hipLaunchKernelGGL(finish_kernel, dim3(2), dim3(TPB), 0, st, part, nblk, 2, dst);
+    SumNormPartP s;
+    FinishP f;
+    sumnorm_params(x, n, dst, part, nblk, &s, &f);
+    hipLaunchKernelGGL(ew_one<SumNormPartP>, dim3(s.nblk), dim3(TPB), 0, st, s);
+    hipLaunchKernelGGL(ew_one<FinishP>, dim3(2), dim3(TPB), 0, st, f);
 }
 
-__global__ void lincomb_kernel(CoefPack cp, int nx, double2* __restrict__ y, int64_t n) {
-    GRID_LOOP(e, n) {
+struct LincombP { CoefPack cp; int nx; double2* y; int64_t n; };
+__device__ __forceinline__ void body(const LincombP& p) {
+    GRID_LOOP(e, p.n) {
         double sr = 0.0, si = 0.0;
-        for (int v = 0; v < nx; ++v) {
-            const double2 a = cp.p[v][e];
-            sr += cp.c[v] * a.x;
-            si += cp.c[v] * a.y;
+        for (int v = 0; v < p.nx; ++v) {
+            const double2 a = p.cp.p[v][e];
+            sr += p.cp.c[v] * a.x;
+            si += p.cp.c[v] * a.y;
         }
-        y[e] = make_double2(sr, si);
+        p.y[e] = make_double2(sr, si);
     }
+}
+
+void lincomb_params(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
+                    void* lp) {
+    LincombP l{};
+    for (int v = 0; v < nx && v < MAXV; ++v) {
+        l.cp.p[v] = xs[v];
+        l.cp.c[v] = coef[v];
+    }
+    l.nx = nx;
+    l.y = y;
+    l.n = n;
+    *(LincombP*)lp = l;
 }
 
 void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
                hipStream_t st) {
-    CoefPack cp{};
-    for (int v = 0; v < nx && v < MAXV; ++v) {
-        cp.p[v] = xs[v];
-        cp.c[v] = coef[v];
-    }
     if (n <= 0) return;
-    hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks(n)), dim3(TPB), 0, st, cp, nx, y, n);
+    LincombP l;
+    lincomb_params(xs, coef, nx, y, n, &l);
+    hipLaunchKernelGGL(ew_one<LincombP>, dim3(nblocks(n)), dim3(TPB), 0, st, l);
+}
+
+// DIISX's error vector and store in one pass: d = S - T (as y = 1 S + (-1) T),
+// amp = S.
+struct DiisPrepP { const double2* S; const double2* T; double2* d; double2* amp; int64_t n; };
+__device__ __forceinline__ void body(const DiisPrepP& p) {
+    GRID_LOOP(e, p.n) {
+        const double2 s = p.S[e];
+        p.d[e] = cadd(cmul(make_double2(1.0, 0.0), s), cmul(make_double2(-1.0, 0.0), p.T[e]));
+        p.amp[e] = s;
+    }
+}
+
+void diis_prep_params(const double2* S, const double2* T, double2* d, double2* amp, int64_t n,
+                      void* pp) {
+    *(DiisPrepP*)pp = DiisPrepP{S, T, d, amp, n};
+}
+
+void gettau_params(const double2* t1, const double2* t2, double2* tau, double s2, PairTab vv,
+                   PairTab oo, int nv, void* gp) {
+    *(GettauP*)gp = GettauP{t1, t2, tau, s2, vv.tab, vv.npair, oo.tab, oo.npair, nv};
+}
+
+void copy_params(int64_t n, const double2* x, double2* y, void* cp) {
+    *(CopyP*)cp = CopyP{n, x, y};
+}
+
+size_t batch_param_size(int kind) {
+    switch (kind) {
+        case RK_ZDOT_PART: return sizeof(ZdotPartP);
+        case RK_FINISH: return sizeof(FinishP);
+        case RK_SUMNORM_PART: return sizeof(SumNormPartP);
+        case RK_LINCOMB: return sizeof(LincombP);
+        case RK_DIIS_PREP: return sizeof(DiisPrepP);
+        case RK_GETTAU: return sizeof(GettauP);
+        case RK_COPY: return sizeof(CopyP);
+        default: return 0;
+    }
+}
+
+// One launch of a finish-phase kind over `count` handles (parameters at dp,
+// device): grid-stride kinds get a capped grid for the largest n, the
+// reductions exactly gx partial blocks (each handle uses its own number).
+int k_many(int kind, const void* dp, int count, int64_t gx, hipStream_t st) {
+    if (count <= 0 || gx <= 0) return 0;
+    const bool stride = kind == RK_LINCOMB || kind == RK_DIIS_PREP || kind == RK_GETTAU ||
+                        kind == RK_COPY;
+    const int64_t x = stride ? std::min<int64_t>(nblocks(gx),
+                                                 std::max<int64_t>(1, 65536 / count))
+                             : gx;
+    const dim3 grid((unsigned)x, 1, count);
+#define RCC_M(K, P)                                                                        \
+    case K:                                                                                \
+        hipLaunchKernelGGL(ew_many<P>, grid, dim3(TPB), 0, st, (const P*)dp);              \
+        break;
+    switch (kind) {
+        RCC_M(RK_ZDOT_PART, ZdotPartP)
+        RCC_M(RK_FINISH, FinishP)
+        RCC_M(RK_SUMNORM_PART, SumNormPartP)
+        RCC_M(RK_LINCOMB, LincombP)
+        RCC_M(RK_DIIS_PREP, DiisPrepP)
+        RCC_M(RK_GETTAU, GettauP)
+        RCC_M(RK_COPY, CopyP)
+        default: return -1;
+    }
+#undef RCC_M
+    return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 // ---- symmetry blocks <-> C1 layout ----

@@ -840,16 +840,21 @@ double flin(double* A, int idim, int n, int nr) {
 // asynchronous copy of the results to pinned host memory; diis_end (after the
 // stream has drained) forms the B matrix, solves it (FLIN) and enqueues the
 // extrapolation.
+// The circular store position of this DIISX call (cceqns.F:2022-2026).
+int diis_advance(rcc_t* h) {
+    const int md = h->maxdim;
+    h->it = std::min(h->it + 1, md);
+    h->itc = h->itc + 1;
+    if (h->itc > md) h->itc = 1;
+    return h->itc - 1;
+}
+
 int diis_begin(rcc_t* h, const double2* cur = nullptr, const double2* nw = nullptr) {
     if (!cur) cur = h->T;
     if (!nw) nw = h->S;
     const int64_t n = h->nt1 + h->nt2;
     hipStream_t st = h->st;
-    const int md = h->maxdim;
-    h->it = std::min(h->it + 1, md);
-    h->itc = h->itc + 1;
-    if (h->itc > md) h->itc = 1;
-    const int c = h->itc - 1;
+    const int c = diis_advance(h);
     double2* d = h->diis_err[c];
     // d = S - T ; store S
     HIPCHK(h, hipMemcpyAsync(d, nw, n * 16, hipMemcpyDeviceToDevice, st));
@@ -866,10 +871,9 @@ int diis_begin(rcc_t* h, const double2* cur = nullptr, const double2* nw = nullp
     return 0;
 }
 
-int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr) {
-    if (!nw) nw = h->S;
-    const int64_t n = h->nt1 + h->nt2;
-    hipStream_t st = h->st;
+// DIISX's host part on the reductions in hred: B row, scaling, bordered
+// system, FLIN; the extrapolation coefficients.
+int diis_coefs(rcc_t* h, double* rms_out, std::vector<double>& coef) {
     const int md = h->maxdim;
     const int c = h->itc - 1;
     const double2* sn = h->hred;
@@ -901,12 +905,20 @@ int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr) {
     for (int i = 0; i < it; ++i) BB(i, it + 1) = 0.0;
     BB(it, it + 1) = -1.0;
     flin(bb.data(), ld, it + 1, 1);
-    std::vector<double> coef(it);
+    coef.assign(it, 0.0);
     for (int i = 0; i < it; ++i) coef[i] = BB(i, it + 1);
-    std::vector<const double2*> amps;
-    for (int k = 0; k < it; ++k) amps.push_back(h->diis_amp[k]);
-    k_lincomb(amps.data(), coef.data(), it, nw, n, st);
     if (rms_out) *rms_out = rms;
+    return 0;
+}
+
+int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr) {
+    if (!nw) nw = h->S;
+    const int64_t n = h->nt1 + h->nt2;
+    std::vector<double> coef;
+    CK(diis_coefs(h, rms_out, coef));
+    std::vector<const double2*> amps;
+    for (int k = 0; k < h->it; ++k) amps.push_back(h->diis_amp[k]);
+    k_lincomb(amps.data(), coef.data(), h->it, nw, n, h->st);
     return 0;
 }
 
@@ -1846,6 +1858,13 @@ struct BatchPlan {
     int runs = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // batched DIISX / CCENG: parameters (pinned host staging and device copy)
+    // and every handle's reduction results gathered for one transfer
+    unsigned char* fin_host = nullptr;
+    void* fin_dev = nullptr;
+    size_t fin_bytes = 0;
+    double2* dslots = nullptr;
+    double2* hslots = nullptr;
 };
 std::vector<BatchPlan*> g_plans;
 
@@ -1854,6 +1873,10 @@ void free_plan(BatchPlan* p) {
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     batch_free(p->prog);
+    if (p->fin_host) (void)hipHostFree(p->fin_host);
+    if (p->fin_dev) (void)hipFree(p->fin_dev);
+    if (p->dslots) (void)hipFree(p->dslots);
+    if (p->hslots) (void)hipHostFree(p->hslots);
     for (int k = 0; k < 4; ++k)
         if (p->dG[k]) (void)hipFree(p->dG[k]);
     for (hipEvent_t e : p->evh) (void)hipEventDestroy(e);
@@ -2013,6 +2036,175 @@ int batch_program_phase(BatchPlan* p, bool use_graph) {
     for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
     return 0;
 }
+// DIISX, CCENG and S -> T of every handle as batched launches on the batch
+// stream, in the order finish_a / finish_b run them per handle: error vector
+// and store, |d|^2 and sum d, the B-row dots (one gather and one transfer of
+// all handles' results), the host FLIN per handle, then the extrapolation,
+// tau(S), the two energy dots and S -> T (one more gather and transfer).
+// Same kernels and partitioning as the per-handle path: same results.
+struct FinBuilder {
+    std::vector<unsigned char> host;
+    struct Seg { int kind; size_t off; int64_t gx; };
+    std::vector<Seg> segs;
+    // appends one parameter block per handle of `kind`
+    void add(int kind, const std::vector<std::vector<unsigned char>>& per, int64_t gx) {
+        const size_t ps = batch_param_size(kind);
+        const size_t off = (host.size() + 255) / 256 * 256;
+        host.resize(off + ps * per.size());
+        for (size_t i = 0; i < per.size(); ++i) memcpy(host.data() + off + i * ps, per[i].data(), ps);
+        segs.push_back({kind, off, gx});
+    }
+};
+
+int fin_run(BatchPlan* p, FinBuilder& fb, size_t slot_elems) {
+    rcc_t* h0 = p->hs[0];
+    const int nh = (int)p->hs.size();
+    if (fb.host.size() > p->fin_bytes) {
+        if (p->fin_host) (void)hipHostFree(p->fin_host);
+        if (p->fin_dev) (void)hipFree(p->fin_dev);
+        p->fin_host = nullptr;
+        p->fin_dev = nullptr;
+        p->fin_bytes = fb.host.size() * 2;
+        HIPCHK(h0, hipHostMalloc((void**)&p->fin_host, p->fin_bytes, hipHostMallocDefault));
+        HIPCHK(h0, hipMalloc(&p->fin_dev, p->fin_bytes));
+    }
+    memcpy(p->fin_host, fb.host.data(), fb.host.size());
+    HIPCHK(h0, hipMemcpyAsync(p->fin_dev, p->fin_host, fb.host.size(), hipMemcpyHostToDevice,
+                              p->st));
+    for (const FinBuilder::Seg& sg : fb.segs)
+        if (k_many(sg.kind, (const unsigned char*)p->fin_dev + sg.off, nh, sg.gx, p->st) != 0) {
+            h0->err = "batched DIIS / energy launch failed";
+            return -1;
+        }
+    HIPCHK(h0, hipMemcpyAsync(p->hslots, p->dslots, slot_elems * sizeof(double2),
+                              hipMemcpyDeviceToHost, p->st));
+    return 0;
+}
+
+template <class F>
+std::vector<std::vector<unsigned char>> per_handle(BatchPlan* p, int kind, F fill) {
+    std::vector<std::vector<unsigned char>> v(p->hs.size(),
+                                              std::vector<unsigned char>(batch_param_size(kind)));
+    for (size_t i = 0; i < p->hs.size(); ++i) fill(p->hs[i], (int)i, (void*)v[i].data());
+    return v;
+}
+
+int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
+    rcc_t* h0 = p->hs[0];
+    const int nh = (int)p->hs.size();
+    const size_t slots = (size_t)nh * RED_SLOTS;
+    if (!p->dslots) {
+        HIPCHK(h0, hipMalloc((void**)&p->dslots, slots * sizeof(double2)));
+        HIPCHK(h0, hipHostMalloc((void**)&p->hslots, slots * sizeof(double2), hipHostMallocDefault));
+    }
+    std::vector<int> cpos(nh);
+    int64_t nmax = 0, itmax = 0;
+    int bmax = 1;
+    for (int i = 0; i < nh; ++i) {
+        rcc_t* h = p->hs[i];
+        cpos[i] = diis_advance(h);
+        nmax = std::max<int64_t>(nmax, h->nt1 + h->nt2);
+        itmax = std::max<int64_t>(itmax, h->it);
+        bmax = std::max(bmax, red_blocks(h->nt1 + h->nt2, RED_BLOCKS));
+    }
+    // ---- DIISX reductions
+    {
+        FinBuilder fb;
+        std::vector<std::vector<unsigned char>> sn_part(nh), sn_fin(nh), zd_part(nh), zd_fin(nh);
+        for (int i = 0; i < nh; ++i) {
+            rcc_t* h = p->hs[i];
+            const int64_t n = h->nt1 + h->nt2;
+            double2* d = h->diis_err[cpos[i]];
+            sn_part[i].resize(batch_param_size(RK_SUMNORM_PART));
+            sn_fin[i].resize(batch_param_size(RK_FINISH));
+            sumnorm_params(d, n, h->red_dst, h->red_part, RED_BLOCKS, sn_part[i].data(),
+                           sn_fin[i].data());
+            std::vector<const double2*> xs(h->diis_err.begin(), h->diis_err.begin() + h->it);
+            zd_part[i].resize(batch_param_size(RK_ZDOT_PART));
+            zd_fin[i].resize(batch_param_size(RK_FINISH));
+            zdot_params(xs.data(), h->it, d, n, h->red_dst + 2, h->red_part, RED_BLOCKS,
+                        zd_part[i].data(), zd_fin[i].data());
+        }
+        fb.add(RK_DIIS_PREP, per_handle(p, RK_DIIS_PREP, [&](rcc_t* h, int i, void* q) {
+                   diis_prep_params(h->S, h->T, h->diis_err[cpos[i]], h->diis_amp[cpos[i]],
+                                    h->nt1 + h->nt2, q);
+               }), nmax);
+        fb.add(RK_SUMNORM_PART, sn_part, bmax);
+        fb.add(RK_FINISH, sn_fin, 2);
+        fb.add(RK_ZDOT_PART, zd_part, bmax);
+        fb.add(RK_FINISH, zd_fin, itmax);
+        fb.add(RK_COPY, per_handle(p, RK_COPY, [&](rcc_t* h, int i, void* q) {
+                   copy_params(2 + h->it, h->red_dst, p->dslots + (size_t)i * RED_SLOTS, q);
+               }), 2 + itmax);
+        CK(fin_run(p, fb, slots));
+        HIPCHK(h0, hipStreamSynchronize(p->st));
+    }
+    // ---- host FLIN, then extrapolation, CCENG on S and S -> T
+    std::vector<std::vector<double>> coef(nh);
+    for (int i = 0; i < nh; ++i) {
+        rcc_t* h = p->hs[i];
+        memcpy(h->hred, p->hslots + (size_t)i * RED_SLOTS, (2 + h->it) * sizeof(double2));
+        CK(diis_coefs(h, rms ? rms + i : nullptr, coef[i]));
+    }
+    {
+        FinBuilder fb;
+        std::vector<std::vector<unsigned char>> e1p(nh), e1f(nh), e2p(nh), e2f(nh);
+        int b1 = 1, b2 = 1;
+        int64_t n2 = 0;
+        for (int i = 0; i < nh; ++i) {
+            rcc_t* h = p->hs[i];
+            const double2* fvo = h->FVO;
+            const double2* vvoo = h->cls[RCC_VVOO];
+            for (auto* v : {&e1p[i], &e2p[i]}) v->resize(batch_param_size(RK_ZDOT_PART));
+            for (auto* v : {&e1f[i], &e2f[i]}) v->resize(batch_param_size(RK_FINISH));
+            zdot_params(&fvo, 1, h->S1(), h->nt1, h->red_dst + RED_E, h->red_part, RED_BLOCKS,
+                        e1p[i].data(), e1f[i].data());
+            zdot_params(&vvoo, 1, h->TAU, h->nt2, h->red_dst + RED_E + 1, h->red_part,
+                        RED_BLOCKS, e2p[i].data(), e2f[i].data());
+            b1 = std::max(b1, red_blocks(h->nt1, RED_BLOCKS));
+            b2 = std::max(b2, red_blocks(h->nt2, RED_BLOCKS));
+            n2 = std::max<int64_t>(n2, h->nt2);
+            h->tau_ok = false;     // TAU will hold tau(S), not tau(T)
+        }
+        fb.add(RK_LINCOMB, per_handle(p, RK_LINCOMB, [&](rcc_t* h, int i, void* q) {
+                   std::vector<const double2*> amps(h->diis_amp.begin(),
+                                                    h->diis_amp.begin() + h->it);
+                   lincomb_params(amps.data(), coef[i].data(), h->it, h->S, h->nt1 + h->nt2, q);
+               }), nmax);
+        fb.add(RK_GETTAU, per_handle(p, RK_GETTAU, [&](rcc_t* h, int, void* q) {
+                   gettau_params(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, q);
+               }), n2);
+        fb.add(RK_ZDOT_PART, e1p, b1);
+        fb.add(RK_FINISH, e1f, 1);
+        fb.add(RK_ZDOT_PART, e2p, b2);
+        fb.add(RK_FINISH, e2f, 1);
+        fb.add(RK_COPY, per_handle(p, RK_COPY, [&](rcc_t* h, int, void* q) {
+                   copy_params(h->nt1 + h->nt2, h->S, h->T, q);
+               }), nmax);
+        fb.add(RK_COPY, per_handle(p, RK_COPY, [&](rcc_t* h, int i, void* q) {
+                   copy_params(2, h->red_dst + RED_E, p->dslots + (size_t)i * RED_SLOTS + RED_E, q);
+               }), 2);
+        CK(fin_run(p, fb, slots));
+        for (rcc_t* h : p->hs) HIPCHK(h, hipEventRecord(h->ev1, p->st));
+        HIPCHK(h0, hipStreamSynchronize(p->st));
+    }
+    for (int i = 0; i < nh; ++i) {
+        rcc_t* h = p->hs[i];
+        memcpy(h->hred + RED_E, p->hslots + (size_t)i * RED_SLOTS + RED_E, 2 * sizeof(double2));
+        energy_end(h, ecc ? ecc + i : nullptr, nullptr);
+        invalidate_t(h);
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_iter_ms = ms;
+        h->nit += 1;
+        if (nit) nit[i] = h->nit;
+    }
+    // later per-handle work on the handles' own streams follows this batch
+    HIPCHK(h0, hipEventRecord(p->evb, p->st));
+    for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
+    return 0;
+}
+
 
 // One grouped product over all handles: the batch stream waits for every
 // handle's stream, runs the launch, and every handle's stream waits for it.
@@ -2490,8 +2682,11 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
     CK(get_plan(hs, n, &p));
     bool graph = true;
     for (int i = 0; i < n; ++i) graph = graph && hs[i]->use_graph;
-    if (p->prog) CK(batch_program_phase(p, graph));
-    else CK(batch_terms_grouped(p));
+    if (p->prog) {
+        CK(batch_program_phase(p, graph));
+        return batch_finish(p, nit, ecc, rms);
+    }
+    CK(batch_terms_grouped(p));
     for (int i = 0; i < n; ++i) CK(finish_a(hs[i]));
     for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
     for (int i = 0; i < n; ++i) CK(finish_b(hs[i], rms ? rms + i : nullptr));
