@@ -269,12 +269,18 @@ __device__ __forceinline__ void i8_copy(i32x4 (*dst)[64], const i32x4* ga, const
 // the stage barrier.  Stages run chunk by chunk, and within a chunk variant
 // by variant, so only one variant's S int32 accumulators are live; after each
 // (chunk, variant) they are scaled and added into C (FP64).
+//
+// Tile order: each XCD (workgroups dealt round-robin) takes a contiguous
+// range of tile numbers; gm > 1 numbers the tiles in bands of gm tile rows,
+// column by column inside a band, so the workgroups an XCD runs together
+// cover a gm x (32 / gm) block of tiles and share A and B panels through that
+// XCD's L2 (gm = 1: row-major, one A panel and 32 B panels).
 template <int S, int WM>
 __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
     int m, int n, int nks, int nch, const i32x4* __restrict__ PA,
     const unsigned long long* __restrict__ MA, const i32x4* __restrict__ PB,
     const unsigned long long* __restrict__ MB, double2* __restrict__ C, int64_t ldc,
-    double2 alpha, double2 beta) {
+    double2 alpha, double2 beta, int gm) {
     constexpr int NA = 2 * WM;
     __shared__ i32x4 lds[3][(NA + 4) * S][64];
     const int mt = (m + 32 * WM - 1) / (32 * WM), nt = (n + 63) / 64;
@@ -289,8 +295,14 @@ __global__ void __launch_bounds__(128 * WM) zgemm_i8_kernel(
     t.n = n;
     t.nks = nks;
     t.nch = nch;
-    t.tm = tile / nt;
-    t.tn = tile % nt;
+    {
+        const int band = gm * nt;                 // tiles per band of gm tile rows
+        const int first = (tile / band) * gm;
+        const int rows = min(mt - first, gm);
+        const int r = tile % band;
+        t.tm = first + r % rows;
+        t.tn = r / rows;
+    }
     t.lane = threadIdx.x & 63;
     t.wave = threadIdx.x >> 6;
     t.wm = t.wave % WM;
@@ -398,49 +410,54 @@ int i8_split(const I8Operand& S, const double2* X, int64_t ld, bool k_contiguous
 
 template <int S, int WM>
 static void launch_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
-                      double2 alpha, double2 beta, hipStream_t st) {
+                      double2 alpha, double2 beta, int gm, hipStream_t st) {
     // must match the kernel's tile count; planes cover whole 128-row tiles
     const int64_t tiles = (int64_t)((A.rows + 32 * WM - 1) / (32 * WM)) * ((B.rows + 63) / 64);
     hipLaunchKernelGGL((zgemm_i8_kernel<S, WM>), dim3((unsigned)tiles), dim3(128 * WM), 0, st,
                        A.rows, B.rows, A.nks, A.nch, (const i32x4*)A.planes, A.mx,
-                       (const i32x4*)B.planes, B.mx, C, ldc, alpha, beta);
+                       (const i32x4*)B.planes, B.mx, C, ldc, alpha, beta, std::max(gm, 1));
 }
 
 // Tile height WM (tile = 32 WM x 64): three LDS buffers of (2 WM + 4) S KiB,
 // 144 KiB for (4, 4), 150 KiB for (5, 3); rows_pad (a multiple of 384) is a
 // multiple of every 32 WM offered.
-static int launch_wm(int S, int wm, const I8Operand& A, const I8Operand& B, double2* C,
-                     int64_t ldc, double2 alpha, double2 beta, hipStream_t st) {
+static int launch_wm(int S, int wm, int gm, const I8Operand& A, const I8Operand& B,
+                     double2* C, int64_t ldc, double2 alpha, double2 beta, hipStream_t st) {
     if (S == 4) {
-        if (wm == 2) launch_i8<4, 2>(A, B, C, ldc, alpha, beta, st);
-        else if (wm == 3) launch_i8<4, 3>(A, B, C, ldc, alpha, beta, st);
-        else launch_i8<4, 4>(A, B, C, ldc, alpha, beta, st);
+        if (wm == 2) launch_i8<4, 2>(A, B, C, ldc, alpha, beta, gm, st);
+        else if (wm == 3) launch_i8<4, 3>(A, B, C, ldc, alpha, beta, gm, st);
+        else launch_i8<4, 4>(A, B, C, ldc, alpha, beta, gm, st);
     } else if (S == 5) {
-        if (wm == 2) launch_i8<5, 2>(A, B, C, ldc, alpha, beta, st);
-        else launch_i8<5, 3>(A, B, C, ldc, alpha, beta, st);
+        if (wm == 2) launch_i8<5, 2>(A, B, C, ldc, alpha, beta, gm, st);
+        else launch_i8<5, 3>(A, B, C, ldc, alpha, beta, gm, st);
     } else {
         return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// Tile-height autotuner: the first product of a shape (digits, m, n, k
-// steps) met outside graph capture times every tile height (WM 2, 3, 4 for
-// 4 digits; 2, 3 for 5) in place with alpha = 0, beta = 1 (C unchanged: the
-// first flush writes 1 C + 0, later flushes add 0) and keeps the fastest for
-// the process.  RCC_I8_WM fixes the height; with the autotuner off
-// (RCC_AUTOTUNE=0, zgemm_autotune(0)) it is 4 / 3.
+// Tile autotuner: the first product of a shape (digits, m, n, k steps) met
+// outside graph capture times every tile height WM (2, 3, 4 for 4 digits; 2,
+// 3 for 5) with tile-row bands gm of 1, 4 and 8, in place with alpha = 0,
+// beta = 1 (C unchanged: the first flush writes 1 C + 0, later flushes add
+// 0), and keeps the fastest for the process.  RCC_I8_WM fixes the height
+// (band 1); with the autotuner off (RCC_AUTOTUNE=0, zgemm_autotune(0)) it is
+// 4 / 3 with band 1.  The choice does not change the result: every tile sums
+// the same products in the same order.
+struct I8Choice {
+    int wm, gm;
+};
 static std::mutex i8_tune_mu;
-static std::map<std::tuple<int, int, int, int>, int> i8_tuned;
+static std::map<std::tuple<int, int, int, int>, I8Choice> i8_tuned;
 
-static int i8_choose_wm(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
-                        hipStream_t st) {
+static I8Choice i8_choose(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
+                          hipStream_t st) {
     const int S = A.digits;
-    const int dflt = S == 4 ? 4 : 3;
+    const I8Choice dflt{S == 4 ? 4 : 3, 1};
     const char* e = getenv("RCC_I8_WM");
     if (e) {
         const int w = atoi(e);
-        return (w == 2 || w == 3 || (w == 4 && S == 4)) ? w : dflt;
+        return (w == 2 || w == 3 || (w == 4 && S == 4)) ? I8Choice{w, 1} : dflt;
     }
     if (!zgemm_autotune_enabled()) return dflt;
     std::lock_guard<std::mutex> lk(i8_tune_mu);
@@ -452,28 +469,31 @@ static int i8_choose_wm(const I8Operand& A, const I8Operand& B, double2* C, int6
         return dflt;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return dflt;
-    int best = dflt;
+    I8Choice best = dflt;
     float best_ms = 1e30f;
-    const int cands4[] = {2, 3, 4}, cands5[] = {2, 3};
-    const int* cands = S == 4 ? cands4 : cands5;
-    const int nc = S == 4 ? 3 : 2;
-    for (int c = 0; c < nc; ++c) {
-        float ms_min = 1e30f;
-        for (int rep = 0; rep < 2; ++rep) {
-            (void)hipEventRecord(e0, st);
-            if (launch_wm(S, cands[c], A, B, C, ldc, make_double2(0, 0), make_double2(1, 0), st))
-                break;
-            (void)hipEventRecord(e1, st);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (rep > 0) ms_min = std::min(ms_min, ms);
+    const int wms4[] = {2, 3, 4}, wms5[] = {2, 3};
+    const int* wms = S == 4 ? wms4 : wms5;
+    const int nw = S == 4 ? 3 : 2;
+    const int gms[] = {1, 4, 8};
+    for (int c = 0; c < nw; ++c)
+        for (int g : gms) {
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                (void)hipEventRecord(e0, st);
+                if (launch_wm(S, wms[c], g, A, B, C, ldc, make_double2(0, 0), make_double2(1, 0),
+                              st))
+                    break;
+                (void)hipEventRecord(e1, st);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rep > 0) ms_min = std::min(ms_min, ms);
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best = I8Choice{wms[c], g};
+            }
         }
-        if (ms_min < best_ms) {
-            best_ms = ms_min;
-            best = cands[c];
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     i8_tuned[key] = best;
@@ -492,8 +512,8 @@ int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, do
     if (m <= 0 || n <= 0) return 0;
     if (A.nks == 0) return -1;
     if (A.digits != 4 && A.digits != 5) return -1;
-    const int wm = i8_choose_wm(A, B, C, ldc, st);
-    return launch_wm(A.digits, wm, A, B, C, ldc, alpha, beta, st);
+    const I8Choice ch = i8_choose(A, B, C, ldc, st);
+    return launch_wm(A.digits, ch.wm, ch.gm, A, B, C, ldc, alpha, beta, st);
 }
 
 }  // namespace rcc
