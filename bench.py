@@ -288,6 +288,7 @@ def bench_batch(args):
     for _ in range(2):              # the grouped plan's replay, then captured
         RelCCSD.iterate_batch(solvers, grouped=True)
     t_ggraph = timed(lambda: [RelCCSD.iterate_batch(solvers, grouped=True) for _ in range(K)])
+    plan = RelCCSD.batch_plan_info(solvers)
     rate = {k: nsys * K / t for k, t in (("loop", t_loop), ("batch", t_batch),
                                         ("batch_grouped", t_grouped),
                                         ("batch_graph", t_graph),
@@ -302,7 +303,7 @@ def bench_batch(args):
         "data": f"synthetic (seeded sizes seed {seed}; systems seeds 1000+; factor std {args.scale})",
         "config": {"workload": f"{nsys} independent systems, Dirichlet(2) irrep proportions",
                    "systems": [[o, v, no_l, nv_l] for o, v, no_l, nv_l, _ in sizes]},
-        "rates": rate, "speedup_vs_loop": rate[best] / rate["loop"],
+        "rates": rate, "speedup_vs_loop": rate[best] / rate["loop"], "grouped_plan": plan,
         "seconds": {"loop": t_loop, "batch": t_batch, "batch_grouped": t_grouped,
                     "batch_graph": t_graph, "batch_grouped_graph": t_ggraph},
     }

@@ -57,10 +57,11 @@ int zgemm_grouped(int opb, int tag, int gauss, const ZGroup* dG, int ngroups,
 // sequence uses: NN, NC, NT, TN, CN); tag as GemmArgs::tag for NN / NC.
 int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, int ngroups,
                       int64_t blocks, hipStream_t st, const int* gmap = nullptr);
-// The 32 x 32-tile grouped GEMM (same op combinations), its tile offsets.
+// The 32 x 32-tile grouped GEMM (same op combinations), its tile offsets;
+// persistent > 0: a grid of that many workgroups striding over the tiles.
 int64_t zgroup_offsets_small(ZGroup* g, int ngroups);
 int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
-                        int64_t blocks, hipStream_t st, const int* gmap);
+                        int64_t blocks, hipStream_t st, const int* gmap, int persistent = 0);
 int mfma_probe(double* dout, hipStream_t st);
 int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 
@@ -95,8 +96,9 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
 int batch_run(const BatchProgram* p, hipStream_t st);
 // times the 64 x 64 and 32 x 32 tiles of every grouped product, keeps the faster
 int batch_tune(BatchProgram* p, hipStream_t st);
-// grouped products that use the 32 x 32 tile (and, in *grouped, how many there are)
-int batch_small_tiles(const BatchProgram* p, int* grouped);
+// positions (launch sites) of the program, grouped products among them, and
+// how many of those use each tile variant (64 x 64, 32 x 32, 32 x 32 persistent)
+void batch_info(const BatchProgram* p, int* positions, int* grouped, int* variants);
 void batch_free(BatchProgram* p);
 
 // ---- element-wise / permutation kernels (rcc_kernels.hip) ----

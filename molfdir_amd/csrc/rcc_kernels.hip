@@ -1121,26 +1121,32 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
     return 0;
 }
 
+// Tile variants of a grouped product: 0 the 64 x 64 tile, 1 the 32 x 32 tile
+// (one workgroup per tile), 2 the 32 x 32 tile on a persistent grid of 1024
+// workgroups (four per CU) striding over the tiles.
+constexpr int TILE_VARIANTS = 3;
+
 static int grouped_launch(const BatchOp& op, int small, hipStream_t st) {
     if (op.count <= 0) return 0;
     if (small)
         return zgemm_grouped_small(op.opa, op.opb, op.gauss, (const ZGroup*)op.dp32, op.count,
-                                   op.blocks32, st, (const int*)op.dmap32);
+                                   op.blocks32, st, (const int*)op.dmap32,
+                                   small == 2 ? 1024 : 0);
     return zgemm_grouped_ops(op.opa, op.opb, op.tag, op.gauss, (const ZGroup*)op.dp, op.count,
                              op.blocks, st, (const int*)op.dmap);
 }
 
 // Tile autotuner of a compiled batch: every grouped product is timed with
-// the 64 x 64 and the 32 x 32 tile (HIP events, best of two runs after a
-// warm-up) and keeps the faster.  Run before the first replay: the products
+// each tile variant (HIP events, best of two runs after a warm-up) and keeps
+// the fastest.  Run before the first replay: the products
 // then write only intermediates that the term sequence recomputes from T.
 int batch_tune(BatchProgram* p, hipStream_t st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
     for (BatchOp& op : p->ops) {
         if (op.kind != RK_GROUPED || op.count <= 0) continue;
-        float best[2] = {1e30f, 1e30f};
-        for (int v = 0; v < 2; ++v)
+        float best[TILE_VARIANTS] = {1e30f, 1e30f, 1e30f};
+        for (int v = 0; v < TILE_VARIANTS; ++v)
             for (int rep = 0; rep < 3; ++rep) {
                 (void)hipEventRecord(e0, st);
                 if (grouped_launch(op, v, st) != 0) {
@@ -1153,22 +1159,26 @@ int batch_tune(BatchProgram* p, hipStream_t st) {
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 if (rep > 0) best[v] = std::min(best[v], ms);
             }
-        op.small = best[1] < best[0] ? 1 : 0;
+        int pick = 0;
+        for (int v = 1; v < TILE_VARIANTS; ++v)
+            if (best[v] < best[pick]) pick = v;
+        op.small = pick;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return 0;
 }
 
-int batch_small_tiles(const BatchProgram* p, int* grouped) {
-    int s = 0, g = 0;
+void batch_info(const BatchProgram* p, int* positions, int* grouped, int* variants) {
+    int g = 0;
+    for (int v = 0; v < 3; ++v) variants[v] = 0;
     for (const BatchOp& op : p->ops)
         if (op.kind == RK_GROUPED && op.count > 0) {
             ++g;
-            s += op.small;
+            variants[std::min(std::max(op.small, 0), 2)] += 1;
         }
-    if (grouped) *grouped = g;
-    return s;
+    *positions = (int)p->ops.size();
+    *grouped = g;
 }
 
 int batch_run(const BatchProgram* p, hipStream_t st) {

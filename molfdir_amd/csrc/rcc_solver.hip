@@ -2698,6 +2698,20 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
     return 0;
 }
 
+int rcc_batch_plan_info(rcc_t* const* hs, int n, int* info) {
+    if (!hs || n <= 0 || !info) return -1;
+    for (int i = 0; i < 6; ++i) info[i] = 0;
+    std::vector<rcc_t*> key(hs, hs + n);
+    for (BatchPlan* p : g_plans)
+        if (p->hs == key) {
+            info[0] = p->prog ? 1 : 0;
+            if (p->prog) batch_info(p->prog, info + 1, info + 2, info + 3);
+            return 0;
+        }
+    hs[0]->err = "rcc_batch_plan_info: no plan for these handles (run rcc_iterate_batch_grouped)";
+    return -1;
+}
+
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms) {
     if (!h) return -1;
     const double conv = std::pow(10.0, -h->ntol);

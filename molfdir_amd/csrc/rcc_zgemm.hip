@@ -398,15 +398,15 @@ zgemm_mfma_kernel(int m, int n, int k,
 constexpr int SB = 32;
 
 template <int OPA, int OPB, int GAUSS>
-__global__ void __launch_bounds__(256)
-zgemm_small_grouped_kernel(const ZGroup* __restrict__ G, int ngroups,
-                           const int* __restrict__ gmap) {
-    __shared__ double As_re[BK][SB + 1], As_im[BK][SB + 1];
-    __shared__ double Bs_re[BK][SB + 1], Bs_im[BK][SB + 1];
+__device__ __forceinline__ void small_tile(int b, const ZGroup* __restrict__ G, int ngroups,
+                                           const int* __restrict__ gmap,
+                                           double (&As_re)[BK][SB + 1],
+                                           double (&As_im)[BK][SB + 1],
+                                           double (&Bs_re)[BK][SB + 1],
+                                           double (&Bs_im)[BK][SB + 1]) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    int b = blockIdx.x;
     int lo = 0;
     if (gmap) {
         lo = gmap[b];
@@ -502,6 +502,21 @@ zgemm_small_grouped_kernel(const ZGroup* __restrict__ G, int ngroups,
     }
 }
 
+// One tile per workgroup, or (persistent) a grid of at most a few workgroups
+// per CU that strides over the tiles: tile b, b + gridDim.x, ...  (gridDim.x
+// a multiple of 8, so a workgroup stays on its XCD's tiles).
+template <int OPA, int OPB, int GAUSS>
+__global__ void __launch_bounds__(256)
+zgemm_small_grouped_kernel(const ZGroup* __restrict__ G, int ngroups,
+                           const int* __restrict__ gmap, int64_t blocks) {
+    __shared__ double As_re[BK][SB + 1], As_im[BK][SB + 1];
+    __shared__ double Bs_re[BK][SB + 1], Bs_im[BK][SB + 1];
+    for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
+        small_tile<OPA, OPB, GAUSS>((int)b, G, ngroups, gmap, As_re, As_im, Bs_re, Bs_im);
+        __syncthreads();     // the next tile's LDS writes after every wave's last reads
+    }
+}
+
 int64_t zgroup_offsets_small(ZGroup* g, int ngroups) {
     int64_t off = 0;
     for (int i = 0; i < ngroups; ++i) {
@@ -512,16 +527,18 @@ int64_t zgroup_offsets_small(ZGroup* g, int ngroups) {
 }
 
 int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
-                        int64_t blocks, hipStream_t st, const int* gmap) {
+                        int64_t blocks, hipStream_t st, const int* gmap, int persistent) {
     if (ngroups <= 0 || blocks <= 0) return 0;
+    const unsigned grid =
+        (unsigned)(persistent ? std::min<int64_t>(blocks, (int64_t)persistent) : blocks);
 #define RCC_S(a, b)                                                                            \
     if (opa == a && opb == b) {                                                                \
         if (gauss)                                                                             \
-            hipLaunchKernelGGL((zgemm_small_grouped_kernel<a, b, 1>), dim3((unsigned)blocks),   \
-                               dim3(256), 0, st, dG, ngroups, gmap);                           \
+            hipLaunchKernelGGL((zgemm_small_grouped_kernel<a, b, 1>), dim3(grid), dim3(256), 0, \
+                               st, dG, ngroups, gmap, blocks);                                 \
         else                                                                                   \
-            hipLaunchKernelGGL((zgemm_small_grouped_kernel<a, b, 0>), dim3((unsigned)blocks),   \
-                               dim3(256), 0, st, dG, ngroups, gmap);                           \
+            hipLaunchKernelGGL((zgemm_small_grouped_kernel<a, b, 0>), dim3(grid), dim3(256), 0, \
+                               st, dG, ngroups, gmap, blocks);                                 \
         return hipGetLastError() == hipSuccess ? 0 : -2;                                        \
     }
     RCC_S(OP_N, OP_N)

@@ -98,6 +98,7 @@ def load():
     lib.rcc_set_graph.argtypes = [vp, ci]
     lib.rcc_iterate_batch.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
     lib.rcc_iterate_batch_grouped.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
+    lib.rcc_batch_plan_info.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci)]
     lib.rcc_layout_block.argtypes = [vp, ci, ci, ctypes.POINTER(i64), ctypes.POINTER(i64),
                                      ctypes.POINTER(i64)]
     lib.rcc_zgemm_split_device.argtypes = [vp, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
@@ -334,6 +335,20 @@ class RelCCSD:
             msgs = [lib.rcc_last_error(s.h).decode() for s in solvers]
             raise RccError("; ".join(m for m in msgs if m) or "rcc_iterate_batch failed")
         return [(nit[i], e[i], r[i]) for i in range(n)]
+
+    @staticmethod
+    def batch_plan_info(solvers):
+        """The grouped batch plan of these solvers: compiled, launch positions,
+        grouped products, and their tile variants (64, 32, 32 persistent)."""
+        n = len(solvers)
+        lib = solvers[0].lib
+        hs = (ctypes.c_void_p * n)(*[s.h.value for s in solvers])
+        info = (ctypes.c_int * 6)()
+        if lib.rcc_batch_plan_info(hs, n, info) != 0:
+            raise RccError(lib.rcc_last_error(solvers[0].h).decode())
+        return {"compiled": bool(info[0]), "launch_positions": info[1],
+                "grouped_products": info[2],
+                "tile_variants": {"64x64": info[3], "32x32": info[4], "32x32_persistent": info[5]}}
 
     def cceqn(self):
         """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
