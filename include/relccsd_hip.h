@@ -200,8 +200,10 @@ int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);
  * S the pre-DENOM right-hand side of one CCEQN pass, D the DENOM
  * denominators, E CCENG's complex sum:  Lam = (dE/dT + Lam . dS/dT) / D,
  * iterated with DENOM, ZCORE and DIISX; Lam . dS/dT is the adjoint sweep of
- * the iteration's own contractions at the handle's T.  C1 handles with
- * precision 0.  Lam is in the T1 / T2 layouts. */
+ * the iteration's own contractions at the handle's T (with symmetry, the
+ * ladder's adjoint per compound irrep block in one grouped launch).
+ * Precision 0.  Lam is in the T1 / T2 layouts (symmetry-blocked for
+ * nrep > 1). */
 /* Solve for Lam at the current T (normally after rcc_cceqn) with the
  * handle's MAXIT / MAXDIM / NTOL; nit and rms of the last step.  Clears the
  * DIIS history (a later rcc_cceqn restarts it). */
@@ -213,7 +215,8 @@ int rcc_get_lambda(rcc_t* h, double* l1, double* l2);
 int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy, double* g1,
                    double* g2);
 /* One-particle response density of the current (T, Lam): dens is n x n
- * complex (n = no + nv, occupied then virtual, column-major), Hermitian, with
+ * complex (n = no + nv, occupied then virtual in the order of the orbital
+ * energies of rcc_set_fock, column-major), Hermitian, with
  * d Re(Ec) / d lam = tr(dens V) for the one-electron perturbation
  * f -> f + lam V (FOO, FVO, FVV and the orbital energies shifted by the
  * blocks and the diagonal of the Hermitian V). */

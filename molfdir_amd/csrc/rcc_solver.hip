@@ -160,6 +160,12 @@ struct rcc_handle {
             *LB2 = nullptr, *bPT2 = nullptr;
     bool lam_alloc = false, lam_set = false;
     double last_lambda_ms = 0.0;
+    // symmetric handles: the ladder adjoint TAUb(c) = VVVV(c)^T S2b(c) as one
+    // grouped launch over the compound irreps, on block copies
+    double2 *S2b_s = nullptr, *bTAU_s = nullptr;
+    ZGroup* g_ladder_adj = nullptr;
+    int ng_ladder_adj = 0;
+    int64_t nb_ladder_adj = 0;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -1462,8 +1468,8 @@ int c1_to_sym_host(rcc_t* h, int lay, const double2* src, double* out) {
 // (ai) <-> (ck)) and dE/dT2 = conj(VVOO).
 
 int lam_check(rcc_t* h) {
-    if (h->sym.nrep > 1 || h->precision != 0) {
-        h->err = "Lambda equations: C1 handles (nrep = 1) with precision 0 only";
+    if (h->precision != 0) {
+        h->err = "Lambda equations: precision 0 (FP64) only";
         return -1;
     }
     return 0;
@@ -1500,6 +1506,39 @@ int lam_alloc(rcc_t* h) {
     CK(dalloc(h, &h->LW, nvo));
     CK(dalloc(h, &h->LB2, (int64_t)o * o));
     CK(dalloc(h, &h->bPT2, nvvt + noot));
+    if (h->sym.nrep > 1) {
+        const int64_t nt2s = lay_total(h, RCC_LAY_T2);
+        CK(dalloc(h, &h->S2b_s, nt2s));
+        CK(dalloc(h, &h->bTAU_s, nt2s));
+        // per compound irrep: (NVVT_c x NOOT_c) = VVVV_c^T (NVVT_c x NVVT_c) S2b_c
+        std::vector<ZGroup> g;
+        for (int b = 0; b < h->sym.nrep; ++b) {
+            int64_t vr, vc, vo, tr, tc, to;
+            rcc_layout_block(h, RCC_VVVV, b, &vr, &vc, &vo);
+            rcc_layout_block(h, RCC_LAY_T2, b, &tr, &tc, &to);
+            if (vr == 0 || tc == 0) continue;
+            ZGroup z{};
+            z.m = (int)vc;
+            z.n = (int)tc;
+            z.k = (int)vr;
+            z.A = h->VVVV_s + vo;
+            z.lda = std::max<int64_t>(vr, 1);
+            z.B = h->S2b_s + to;
+            z.ldb = std::max<int64_t>(tr, 1);
+            z.C = h->bTAU_s + to;
+            z.ldc = std::max<int64_t>(tr, 1);
+            z.alpha = cplx(1, 0);
+            z.beta = cplx(0, 0);
+            g.push_back(z);
+        }
+        h->ng_ladder_adj = (int)g.size();
+        h->nb_ladder_adj = zgroup_offsets(g.data(), h->ng_ladder_adj);
+        CK(dalloc_raw(h, &h->g_ladder_adj, std::max<int64_t>(1, (int64_t)g.size()) *
+                                                (int64_t)sizeof(ZGroup)));
+        if (!g.empty())
+            HIPCHK(h, hipMemcpy(h->g_ladder_adj, g.data(), g.size() * sizeof(ZGroup),
+                                hipMemcpyHostToDevice));
+    }
     h->lam_alloc = true;
     return 0;
 }
@@ -1541,10 +1580,22 @@ int lam_sweep(rcc_t* h, const double2* Lin, double2* out, bool with_energy) {
         k_fill_zero(out, nt1 + nt2, st);
     }
     // ---- adjoint of the S2 terms
-    // BINTM ladder S2 += VVVV TAU (ccintm.F:700-722):  TAUb = VVVV^T S2b
+    // BINTM ladder S2 += VVVV TAU (ccintm.F:700-722):  TAUb = VVVV^T S2b (with
+    // symmetry per compound irrep block; TAUb is zero off the blocks)
     ktick(h, 0, 0);
-    CK(gemm(h, OP_T, OP_N, (int)nvvt, (int)noot, (int)nvvt, one, h->cls[RCC_VVVV], nvvt, s2b,
-            nvvt, zero, h->bTAU, nvvt, 1, 0, 0, 0, 1));
+    if (h->sym.nrep > 1) {
+        c1_to_sym_dev(h, RCC_LAY_T2, s2b, h->S2b_s);
+        if (zgemm_grouped_ops(OP_T, OP_N, 0, h->gemm_gauss, h->g_ladder_adj, h->ng_ladder_adj,
+                              h->nb_ladder_adj, st) != 0) {
+            h->err = "grouped ladder adjoint launch failed";
+            return -1;
+        }
+        k_fill_zero(h->bTAU, nt2, st);
+        sym_to_c1_dev(h, RCC_LAY_T2, h->bTAU_s, h->bTAU);
+    } else {
+        CK(gemm(h, OP_T, OP_N, (int)nvvt, (int)noot, (int)nvvt, one, h->cls[RCC_VVVV], nvvt, s2b,
+                nvvt, zero, h->bTAU, nvvt, 1, 0, 0, 0, 1));
+    }
     ktick(h, 0, 1);
     // T2EQNS TAU A (cceqns.F:1013-1026):  TAUb += S2b A^T,  Ab = TAU^T S2b
     CK(gemm(h, OP_N, OP_T, (int)nvvt, (int)noot, (int)noot, one, s2b, nvvt, h->AZ, noot, one,
@@ -2746,12 +2797,36 @@ int rcc_lambda(rcc_t* h, int* nit, double* rms) {
     return lam_solve(h, nit, rms);
 }
 
+// Lambda in / out in the handle's layouts (symmetry-blocked for nrep > 1,
+// like the amplitudes).
+static int lam_upload(rcc_t* h, const double* l1, const double* l2, double2* dst) {
+    if (h->sym.nrep > 1) {
+        k_fill_zero(dst, h->nt1 + h->nt2, h->st);
+        CK(sym_to_c1(h, RCC_LAY_T1, l1, hipMemcpyHostToDevice, dst, h->nt1));
+        CK(sym_to_c1(h, RCC_LAY_T2, l2, hipMemcpyHostToDevice, dst + h->nt1, h->nt2));
+        return 0;
+    }
+    CK(upload(h, dst, l1, h->nt1, hipMemcpyHostToDevice));
+    return upload(h, dst + h->nt1, l2, h->nt2, hipMemcpyHostToDevice);
+}
+
+static int lam_download(rcc_t* h, const double2* src, double* l1, double* l2) {
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (h->sym.nrep > 1) {
+        CK(c1_to_sym_host(h, RCC_LAY_T1, src, l1));
+        return c1_to_sym_host(h, RCC_LAY_T2, src + h->nt1, l2);
+    }
+    HIPCHK(h, hipMemcpy(l1, src, (size_t)h->nt1 * 16, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(l2, src + h->nt1, (size_t)h->nt2 * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int rcc_set_lambda(rcc_t* h, const double* l1, const double* l2) {
     if (!h || !l1 || !l2) return -1;
     CK(lam_check(h));
+    CK(prep(h));
     CK(lam_alloc(h));
-    CK(upload(h, h->LAM, l1, h->nt1, hipMemcpyHostToDevice));
-    CK(upload(h, h->LAM + h->nt1, l2, h->nt2, hipMemcpyHostToDevice));
+    CK(lam_upload(h, l1, l2, h->LAM));
     h->lam_set = true;
     return 0;
 }
@@ -2762,9 +2837,7 @@ int rcc_get_lambda(rcc_t* h, double* l1, double* l2) {
         h->err = "rcc_get_lambda: no Lambda (call rcc_lambda or rcc_set_lambda)";
         return -1;
     }
-    HIPCHK(h, hipMemcpy(l1, h->LAM, (size_t)h->nt1 * 16, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(l2, h->LAM + h->nt1, (size_t)h->nt2 * 16, hipMemcpyDeviceToHost));
-    return 0;
+    return lam_download(h, h->LAM, l1, l2);
 }
 
 int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy, double* g1,
@@ -2773,13 +2846,9 @@ int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy
     CK(lam_setup(h));
     double2* in = h->LAMS;
     double2* out = h->S;        // scratch once the forward pass is done
-    CK(upload(h, in, l1, h->nt1, hipMemcpyHostToDevice));
-    CK(upload(h, in + h->nt1, l2, h->nt2, hipMemcpyHostToDevice));
+    CK(lam_upload(h, l1, l2, in));
     CK(lam_sweep(h, in, out, with_energy != 0));
-    HIPCHK(h, hipStreamSynchronize(h->st));
-    HIPCHK(h, hipMemcpy(g1, out, (size_t)h->nt1 * 16, hipMemcpyDeviceToHost));
-    HIPCHK(h, hipMemcpy(g2, out + h->nt1, (size_t)h->nt2 * 16, hipMemcpyDeviceToHost));
-    return 0;
+    return lam_download(h, out, g1, g2);
 }
 
 int rcc_density(rcc_t* h, double* dens) {

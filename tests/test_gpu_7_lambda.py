@@ -186,3 +186,33 @@ def test_frozen_core_gradient_matches_gpu_central_differences():
     h = 1e-4
     fd = (prob.energy(h * V) - prob.energy(-h * V)) / (2 * h)
     assert abs(d - fd) <= 2e-6 * abs(fd) + 2e-11, (d, fd)
+
+
+@pytest.mark.parametrize("name", ["s4_5o12v", "s8_8o24v"])
+def test_symmetric_handle_lambda_and_density_equal_c1_handle(name):
+    # the symmetry-blocked handle (grouped transposed ladder over the irrep
+    # blocks) and a C1 handle on the same integrals give the same Lambda and
+    # density
+    from _golden import layout_for
+    g = load("symccsd", name)
+    s = system_for(g)
+    lay = layout_for(g)
+    no, nv = int(g["params"][0]), int(g["params"][1])
+    c1 = RelCCSD(no, nv, maxit=100, maxdim=8, ntol=12)
+    c1.set_system(s)
+    sy = RelCCSD(no, nv, maxit=100, maxdim=8, ntol=12,
+                 sym=(g["sym_no"].tolist(), g["sym_nv"].tolist()))
+    sy.set_fock(lay.to_sym("FOO", s["FOO"].numpy()), lay.to_sym("FVO", s["FVO"].numpy()),
+                lay.to_sym("FVV", s["FVV"].numpy()), s["eps"])
+    for c in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        sy.set_ints(c, lay.to_sym(c, s[c].numpy()))
+    for r in (c1, sy):
+        out = r.cceqn()
+        assert out["rows"][-1][2] < 1e-12
+        nit, rms = r.lambda_solve()
+        assert rms < 1e-12, (nit, rms)
+    a1, a2 = c1.get_lambda()
+    b1, b2 = sy.get_lambda()
+    assert np.abs(lay.to_c1("T1", b1) - a1).max() < 1e-9
+    assert np.abs(lay.to_c1("T2", b2) - a2).max() < 1e-9
+    assert np.abs(sy.density() - c1.density()).max() < 1e-10
