@@ -224,6 +224,30 @@ int rcc_density(rcc_t* h, double* dens);
 /* Device time of the last rcc_lambda iteration, ms. */
 int rcc_last_lambda_ms(rcc_t* h, double* ms);
 
+/* ---- Non-iterative triples: T3CORR (cceqns.F:1336-1863), called by
+ * CCENER after the CCSD equations when DOCCSDT is set (ccdriv.F:1799-1837).
+ * On the handle's current T1 / T2 (normally after rcc_cceqn), the set Fock
+ * FVO, orbital energies and the VOOO, VVOO, VOVV classes:
+ *   et[0] = ET1, the fourth-order +T correction     ('4th order triples')
+ *   et[1] = ET2, the fifth-order (T) term          (CCSD(T) = Ec + ET1 + ET2)
+ *   et[2] = ET3, the fifth-order -T term           (CCSD-T = Ec + ET1 + ET3)
+ * as T3CORR writes them to COMMON /ENERG/ (cceqns.F:1799-1812).  FP64 on the
+ * matrix cores for every precision mode; symmetric handles run the C1 sums
+ * (symmetry-forbidden elements are zero).  Occupied triples i > j > k are
+ * numbered in the reference's loop order (k outer, then j, then i);
+ * rcc_t3corr_part sums the range [first, first + count) only (count < 0:
+ * to the end), so a set of processes can split the triples and add their
+ * three numbers.  Returns 0 and zeros for fewer than 3 occupied or 3
+ * virtual spinors. */
+int rcc_t3corr(rcc_t* h, double* et);
+int rcc_t3corr_part(rcc_t* h, int64_t first, int64_t count, double* et);
+/* Number of occupied triples (o (o-1) (o-2) / 6). */
+int64_t rcc_t3_triples(rcc_t* h);
+/* Device time (ms) of the last rcc_t3corr call; parts (may be null, 4
+ * doubles): [0] its triples GEMMs, [1] its energy kernels (with their
+ * reductions), [2] occupied triples per launch, [3] launches. */
+double rcc_last_t3_ms(rcc_t* h, double* parts);
+
 /* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms. */
 int rcc_last_iteration_ms(rcc_t* h, double* ms);
 /* HIP-event durations (ms, on the handle's stream) of the last iteration's

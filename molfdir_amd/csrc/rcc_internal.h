@@ -62,6 +62,28 @@ int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, in
 int64_t zgroup_offsets_small(ZGroup* g, int ngroups);
 int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
                         int64_t blocks, hipStream_t st, const int* gmap, int persistent = 0);
+// ---- non-iterative triples (rcc_triples.hip) ----
+// Inputs of T3CORR in the C1 layouts (device pointers); gauss as GemmArgs.
+struct T3Input {
+    int no = 0, nv = 0, gauss = 1;
+    const double2 *T1 = nullptr, *T2 = nullptr, *FVO = nullptr;
+    const double2 *VOOO = nullptr, *VVOO = nullptr, *VOVV = nullptr;
+    const double* eps = nullptr;
+};
+struct T3Timing {
+    double gemm_ms = 0, energy_ms = 0, total_ms = 0;   // GEMMs, energy + reduce, whole call
+    int64_t chunks = 0;
+    int chunk = 0;                                      // occupied triples per launch
+};
+struct T3State;
+T3State* t3_new();
+void t3_free(T3State* s);
+int64_t t3_triples(int no);
+int t3_chunk(const T3State* s);
+// ET1, ET2, ET3 of the occupied triples [first, first + count) (reference
+// loop order); work buffers are kept in s between calls.
+int t3_run(T3State* s, const T3Input& in, int64_t first, int64_t count, hipStream_t st,
+           double* et, T3Timing* tm, std::string& err);
 int mfma_probe(double* dout, hipStream_t st);
 int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 

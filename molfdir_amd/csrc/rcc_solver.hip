@@ -166,6 +166,9 @@ struct rcc_handle {
     ZGroup* g_ladder_adj = nullptr;
     int ng_ladder_adj = 0;
     int64_t nb_ladder_adj = 0;
+    // non-iterative triples: work buffers and the last call's timing
+    T3State* t3 = nullptr;
+    T3Timing t3_tm;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -2395,6 +2398,7 @@ void rcc_destroy(rcc_t* h) {
     drop_plans_of(h);
     if (h->st) (void)hipStreamSynchronize(h->st);
     drop_graph(h);
+    if (h->t3) t3_free(h->t3);
     if (h->hred) (void)hipHostFree(h->hred);
     for (void* p : h->allocs) (void)hipFree(p);
     for (int w = 0; w < 4; ++w)
@@ -2790,6 +2794,44 @@ int rcc_autotune_entries(rcc_t* h, int64_t* fp64_shapes, int64_t* int8_shapes) {
     if (fp64_shapes) *fp64_shapes = zgemm_tuned_shapes();
     if (int8_shapes) *int8_shapes = i8_tuned_shapes();
     return 0;
+}
+
+int64_t rcc_t3_triples(rcc_t* h) { return h ? t3_triples(h->no) : -1; }
+
+int rcc_t3corr_part(rcc_t* h, int64_t first, int64_t count, double* et) {
+    if (!h || !et) return -1;
+    for (int c : {RCC_VOOO, RCC_VVOO, RCC_VOVV})
+        if (!h->cls[c]) {
+            h->err = "rcc_t3corr: integral class " + std::to_string(c) + " not set";
+            return -1;
+        }
+    if (!h->t3) h->t3 = t3_new();
+    T3Input in;
+    in.no = h->no;
+    in.nv = h->nv;
+    in.gauss = h->gemm_gauss;
+    in.T1 = h->T1();
+    in.T2 = h->T2();
+    in.FVO = h->FVO;
+    in.VOOO = h->cls[RCC_VOOO];
+    in.VVOO = h->cls[RCC_VVOO];
+    in.VOVV = h->cls[RCC_VOVV];
+    in.eps = h->eps;
+    if (count < 0) count = t3_triples(h->no) - first;
+    return t3_run(h->t3, in, first, count, h->st, et, &h->t3_tm, h->err);
+}
+
+int rcc_t3corr(rcc_t* h, double* et) { return rcc_t3corr_part(h, 0, -1, et); }
+
+double rcc_last_t3_ms(rcc_t* h, double* parts) {
+    if (!h) return -1.0;
+    if (parts) {
+        parts[0] = h->t3_tm.gemm_ms;
+        parts[1] = h->t3_tm.energy_ms;
+        parts[2] = (double)h->t3_tm.chunk;
+        parts[3] = (double)h->t3_tm.chunks;
+    }
+    return h->t3_tm.total_ms;
 }
 
 int rcc_lambda(rcc_t* h, int* nit, double* rms) {

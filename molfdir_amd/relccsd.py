@@ -114,6 +114,12 @@ def load():
     lib.rcc_lambda_vjp.argtypes = [vp, vp, vp, ci, vp, vp]
     lib.rcc_density.argtypes = [vp, vp]
     lib.rcc_last_lambda_ms.argtypes = [vp, dp]
+    lib.rcc_t3corr.argtypes = [vp, dp]
+    lib.rcc_t3corr_part.argtypes = [vp, i64, i64, dp]
+    lib.rcc_t3_triples.argtypes = [vp]
+    lib.rcc_t3_triples.restype = i64
+    lib.rcc_last_t3_ms.argtypes = [vp, dp]
+    lib.rcc_last_t3_ms.restype = ctypes.c_double
     _lib = lib
     return lib
 
@@ -421,6 +427,28 @@ class RelCCSD:
         ms = ctypes.c_double()
         self._check(self.lib.rcc_last_lambda_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def t3corr(self, first=0, count=-1):
+        """Non-iterative triples of T3CORR on the current amplitudes:
+        {'et1': +T (4th order), 'et2': (T) 5th-order term, 'et3': -T 5th-order
+        term}; CCSD(T) = Ec + et1 + et2.  first / count select a range of the
+        occupied triples i > j > k (reference loop order)."""
+        et = (ctypes.c_double * 3)()
+        if first == 0 and count < 0:
+            self._check(self.lib.rcc_t3corr(self.h, et))
+        else:
+            self._check(self.lib.rcc_t3corr_part(self.h, int(first), int(count), et))
+        return {"et1": et[0], "et2": et[1], "et3": et[2]}
+
+    def t3_triples(self):
+        return int(self.lib.rcc_t3_triples(self.h))
+
+    def last_t3_ms(self):
+        """(total ms, {'gemm': ms, 'energy': ms, 'chunk': triples per launch,
+        'launches': n}) of the last t3corr call."""
+        p = (ctypes.c_double * 4)()
+        tot = self.lib.rcc_last_t3_ms(self.h, p)
+        return tot, {"gemm": p[0], "energy": p[1], "chunk": int(p[2]), "launches": int(p[3])}
 
     def kernel_times(self):
         """HIP-event ms of the last iteration's ladder, two rings and VOVV.tau."""

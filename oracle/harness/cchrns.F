@@ -17,6 +17,9 @@ C  Mode 1 then runs CCENER (ccdriv.F:1574): MP2 + CCSD, and writes the
 C  final amplitudes.  Mode 2 reads given T1/T2 and calls the per-term
 C  routines in the order CCEQN uses them (cceqns.F:313-546), writing
 C  every intermediate, so each GPU term can be checked on its own.
+C  Mode 3 reads given T1/T2 and calls T3CORR (cceqns.F:1336) with the
+C  arrays CCENER allocates for it (ccdriv.F:1805-1816), writing the
+C  triples corrections ET1, ET2, ET3 (COMMON /ENERG/).
 C
 C  Input  : ccin.bin  (stream; header of 8 int32, then float64 data)
 C  Output : ccout.bin (stream; float64 data), plus the reference log
@@ -167,6 +170,31 @@ C
          CALL GETAMPT (T1,T2)
          WRITE (32) EMP2,ECCSD,DBLE(NIT)
          WRITE (32) T1,T2
+C
+      ELSEIF (MODE.EQ.3) THEN
+C
+C        Non-iterative triples on given amplitudes
+C
+         ALLOCATE (T1(2*NDIMT1),T2(2*NDIMT2))
+         READ (30) T1,T2
+         ALLOCATE (CC(2*NV2 + 2*IVVOOT(NREP+1) + 2*LOVVOT(NREP+1)
+     &             + 2*NDIMT2 + 2*KVVVOT(NREP+1) + 6*IVVVTMX
+     &             + 2*MAX0(NVVVOTMX,NDIMT2) + 16))
+         CC = 0.0D0
+         EQNS = 'AMPLTD'
+         PVOOO = 1
+         PT2S = PVOOO + 2*NV2
+         PT2T = PT2S + 2*IVVOOT(NREP+1)
+         PVVOO = PT2T + 2*LOVVOT(NREP+1)
+         PVVVO = PVVOO + 2*NDIMT2
+         PB1 = PVVVO + 2*KVVVOT(NREP+1)
+         PB2 = PB1 + 2*IVVVTMX
+         PB3 = PB2 + 2*IVVVTMX
+         PB4 = PB3 + 2*IVVVTMX
+         CALL T3CORR (FVO,EPS,T1,T2,CC(PT2S),CC(PT2T),
+     &                CC(PB1),CC(PB2),CC(PB3),CC(PB4),
+     &                CC(PVVVO),CC(PVVOO),CC(PVOOO))
+         WRITE (32) ET1,ET2,ET3
 C
       ELSE
 C

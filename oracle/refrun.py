@@ -10,6 +10,9 @@ the per-iteration table CCEQN prints (cceqns.F:270-272, 550-551).
 Mode 2 ("terms") evaluates every CCEQN term once on given T1/T2 and returns
 each intermediate (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before and after BINTM,
 S1/S2 after DENOM, and CCENG's energy).
+Mode 3 ("t3") evaluates the non-iterative triples corrections of T3CORR
+(cceqns.F:1336-1863) on given T1/T2: ET1 (fourth-order +T), ET2 ((T)) and
+ET3 (-T).
 """
 from __future__ import annotations
 
@@ -66,7 +69,7 @@ def _write_input(path, sysd, mode, maxit, maxdim, ntol, t1=None, t2=None):
             f.write(multb_full(nrep).T.astype(np.int32).tobytes())   # column-major
         for name in CLASS_ORDER:
             f.write(_c(lay.to_sym(name, _c(sysd[name]))).tobytes())
-        if mode == 2:
+        if mode in (2, 3):
             f.write(_c(lay.to_sym("T1", _c(t1))).tobytes())
             f.write(_c(lay.to_sym("T2", _c(t2))).tobytes())
 
@@ -98,7 +101,7 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
     that is the C1 layout."""
     lay = layout_of(sysd)
     nt1, nt2 = lay.size("T1"), lay.size("T2")
-    m = 1 if mode == "ccsd" else 2
+    m = {"ccsd": 1, "terms": 2, "t3": 3}[mode]
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         _write_input(os.path.join(d, "ccin.bin"), sysd, m, maxit, maxdim, ntol, t1, t2)
         p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
@@ -114,6 +117,11 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
         out["iterations"] = parse_iterations(log)
         mt = re.search(r"Total time in CCEQN :\s+([\d.]+)", log)
         out["cceqn_cpu_s"] = float(mt.group(1)) if mt else None
+        return out
+    if m == 3:
+        out["ET1"], out["ET2"], out["ET3"] = raw[0], raw[1], raw[2]
+        mt = re.search(r"Total time in T3CORR :\s+([\d.]+)", log)
+        out["t3corr_cpu_s"] = float(mt.group(1)) if mt else None
         return out
     nfoo, nfvo, nfvv = lay.size("FOO"), lay.size("FVO"), lay.size("FVV")
     nv1, nv4 = lay.size("OOOO"), lay.size("VOVO")

@@ -18,6 +18,9 @@ change in the generator is caught.
   symmetry of an abelian double group (2, 4 or 8 fermion irreps, Z2^k
   products); every array is in relccsd's symmetry-blocked layout (SYMTAB,
   ccdriv.F:746-1425), the per-irrep occupied / virtual counts are stored.
+* t3_<name>.npz: the triples corrections ET1, ET2, ET3 of relccsd's T3CORR
+  (cceqns.F:1336-1863) on seeded random amplitudes (C1 and symmetry-blocked)
+  or on the converged amplitudes of a ccsd_ fixture.
 * lambda_<name>.npz: converged CCSD energies of the reference at f and at
   f +- h V for a seeded Hermitian one-electron perturbation V (stored), the
   finite-field gradient d Ec / d lam the CPU path gives (relccsd's own
@@ -69,6 +72,43 @@ LAMBDA_CASES = [
     ("lam_5o16v", 5, 16, 2, 0.02, 0.0, 7, 1e-4, 80, 8, 13),
     ("lam_6o40v", 6, 40, 1, 0.02, 0.01, 8, 1e-4, 80, 8, 13),
 ]
+
+
+T3_CASES = [
+    # name, (no per irrep), (nv per irrep), seed, scale, fock_perturb, amplitudes
+    # (int: seed of random amplitudes; str: converged T1/T2 of that ccsd fixture)
+    ("r_4o10v", (4,), (10,), 3, 0.05, 0.05, 7),
+    ("r_6o20v", (6,), (20,), 4, 0.05, 0.03, 8),
+    ("r_7o70v", (7,), (70,), 5, 0.05, 0.02, 9),
+    ("r_5o3v", (5,), (3,), 6, 0.05, 0.05, 10),
+    ("r_2o6v", (2,), (6,), 7, 0.05, 0.05, 11),
+    ("conv_6o40v", (6,), (40,), 1, 0.05, 0.0, "cfg1_6o40v"),
+    ("s4_5o12v", (2, 1, 1, 1), (4, 3, 3, 2), 24, 0.1, 0.03, 14),
+    ("s8_8o24v", (2, 0, 1, 1, 2, 0, 1, 1), (4, 2, 3, 3, 4, 2, 3, 3), 25, 0.1, 0.03, 15),
+]
+
+
+def t3_case(name, sno, snv, seed, scale, fp, amp):
+    no, nv = sum(sno), sum(snv)
+    sym = (sno, snv) if len(sno) > 1 else None
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=sym)
+    if isinstance(amp, str):
+        z = np.load(os.path.join(HERE, f"ccsd_{amp}.npz"), allow_pickle=False)
+        t1, t2, tseed = z["T1"], z["T2"], -1
+    elif sym:
+        t1, t2 = symmetric_amplitudes(SymLayout(sno, snv), amp)
+        tseed = amp
+    else:
+        t1, t2 = random_amplitudes(no, nv, amp)
+        tseed = amp
+    r = refrun.run(s, "t3", t1=t1, t2=t2)
+    extra = {"sym_no": np.array(sno), "sym_nv": np.array(snv)} if sym else {}
+    np.savez_compressed(
+        os.path.join(HERE, f"t3_{name}.npz"),
+        params=np.array([no, nv, seed, tseed], dtype=np.int64),
+        amplitudes=str(amp), scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+        ET1=r["ET1"], ET2=r["ET2"], ET3=r["ET3"], **extra)
+    print(name, r["ET1"], r["ET2"], r["ET3"])
 
 
 def symmetric_amplitudes(lay, seed):
@@ -129,6 +169,12 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--t3-only" in sys.argv:
+        for case in T3_CASES:
+            t3_case(*case)
+        return
+    for case in T3_CASES:
+        t3_case(*case)
     for case in LAMBDA_CASES:
         lambda_case(*case)
     if only_lam:
