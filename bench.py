@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg3]
     python bench.py --config cfg4        # 64 small symmetric systems, batched
     python bench.py --config cfg5        # Lambda equations + response density
+    python bench.py --config cfg3t       # CCSD(T) triples (T3CORR) at 30o/300v
 
 A step is one CCEQN iteration (HOINTM .. BINTM, DENOM, DIIS, CCENG, S->T)
 over the whole synthetic system with every integral class resident in HBM.
@@ -31,6 +32,8 @@ CONFIGS = {
 }
 BATCH = {"cfg4": (64, 8, 4)}   # systems, irreps, seed of the size / proportion draws
 LAMBDA = {"cfg5": (20, 200, 5, "C1 20o/200v complex128")}
+TRIPLES = {"cfg3t": (30, 300, 3, "C1 30o/300v complex128")}
+T3_CPU_SAMPLE = (12, 70)  # relccsd T3CORR sample size (o, v)
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
@@ -157,6 +160,124 @@ def bench_lambda(args):
                              "CPU gradient is finite-field CCSD (tests/golden/lambda_*.npz)",
     }
     print(json.dumps(out), flush=True)
+
+
+def triples_flops(no: int, nv: int) -> float:
+    """Algorithmic flops of T3CORR's six GEMMs (cceqns.F:1471-1561): per
+    occupied triple an (v x nvvt) block over K = 3 v (particle) + 3 o (hole),
+    8 real flops per complex multiply-add."""
+    ntri = no * (no - 1) * (no - 2) // 6
+    nvvt = nv * (nv - 1) // 2
+    return 8.0 * ntri * nv * nvvt * 3 * (nv + no)
+
+
+def t3_cpu_baseline(workload_flops: float):
+    """relccsd's own CPU T3CORR (oracle/_ref, unmodified sources) on a bounded
+    sample with seeded amplitudes, extrapolated by the GEMM flop count."""
+    import numpy as np
+    from molfdir_amd import synth
+    from oracle import refrun
+    no, nv = T3_CPU_SAMPLE
+    if not refrun.available():
+        return None
+    s = synth.make_system(no, nv, seed=3)
+    rng = np.random.default_rng(3)
+    nt1, nt2 = no * nv, no * (no - 1) // 2 * nv * (nv - 1) // 2
+    t1 = 0.01 * (rng.normal(size=nt1) + 1j * rng.normal(size=nt1))
+    t2 = 0.01 * (rng.normal(size=nt2) + 1j * rng.normal(size=nt2))
+    t0 = time.perf_counter()
+    r = refrun.run(s, "t3", t1=t1, t2=t2)
+    wall = time.perf_counter() - t0
+    cpu = r["t3corr_cpu_s"] or wall
+    rate = triples_flops(no, nv) / cpu
+    return {
+        "value": rate / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+        "s_extrapolated": workload_flops / rate,
+        "sample": (f"relccsd T3CORR (unmodified reference sources, single-threaded netlib "
+                   f"BLAS) at C1 {no}o/{nv}v on seeded amplitudes, {cpu:.2f} s CPU; rate "
+                   f"extrapolated to the workload by the GEMM flop count"),
+    }
+
+
+def bench_triples(args):
+    """CCSD to convergence (NTOL 8) from the MP2 guess, then the triples
+    corrections of T3CORR on the converged amplitudes; one untimed warm-up
+    call on the first launch's triples (allocates the work buffers)."""
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+    no, nv, seed, desc = TRIPLES[args.config]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    tried = [args.t3_scale] + [x for x in (0.005, 0.0025) if x < args.t3_scale]
+    r = None
+    for sc in tried:
+        sysd = synth.make_system(no, nv, seed=seed, scale=sc, device=dev)
+        r = RelCCSD(no, nv, maxdim=8, maxit=60, ntol=8)
+        r.set_complex_product(args.complex_product == 3)
+        r.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
+        for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+            r.set_ints(name, sysd[name])
+            sysd[name] = None
+        del sysd
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        out = r.cceqn()
+        t_cc = time.perf_counter() - t0
+        rms = out["rows"][-1][2]
+        print(f"[bench] scale {sc}: CCSD Ec {out['eccsd']:.12f} nit {out['nit']} rms {rms:.2e} "
+              f"({t_cc:.1f} s)", file=sys.stderr, flush=True)
+        if rms < 1e-8:
+            scale = sc
+            break
+        r.close()
+        r = None
+        torch.cuda.empty_cache()
+    if r is None:
+        raise SystemExit("bench.py: CCSD did not converge at any factor scale")
+    r.synchronize()
+    r.t3corr(0, 1)
+    _, warm = r.last_t3_ms()
+    r.t3corr(0, min(warm["chunk"], r.t3_triples()))
+    r.synchronize()
+    t0 = time.perf_counter()
+    e = r.t3corr()
+    wall = time.perf_counter() - t0
+    tot_ms, parts = r.last_t3_ms()
+    print(f"[bench] (T) ET1 {e['et1']:.12f} ET2 {e['et2']:.12f} ET3 {e['et3']:.12f} "
+          f"{tot_ms:.1f} ms (GEMM {parts['gemm']:.1f}, energy {parts['energy']:.1f}; "
+          f"{parts['launches']} launches of {parts['chunk']} triples)", file=sys.stderr, flush=True)
+    F = triples_flops(no, nv)
+    ntri = no * (no - 1) * (no - 2) // 6
+    nvvt = nv * (nv - 1) // 2
+    ach = F / (parts["gemm"] * 1e-3) / 1e12
+    # algorithmic bytes of one GEMM launch: U' written once, the A panels
+    # read once, the three VOVV row blocks and T2 columns of every triple
+    kp = 3 * (nv + no)
+    res = {
+        "metric": f"CCSD(T) triples s + TFLOP/s at {no}o/{nv}v cplx",
+        "value": tot_ms * 1e-3, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": tot_ms, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": (f"synthetic (seed {seed}; factor Re/Im N(0,{scale}^2), the largest of "
+                 f"{', '.join(str(x) for x in tried)} at which CCSD converges); amplitudes "
+                 f"from CCSD converged to rms < 1e-8 on the GPU"),
+        "config": {"workload": f"T3CORR triples corrections, {desc}", "occupied_triples": ntri,
+                   "triples_per_launch": parts["chunk"], "launches": parts["launches"]},
+        "tflops": F / (tot_ms * 1e-3) / 1e12, "flops": F, "wall_s": wall,
+        "energies": {"ecc": out["eccsd"], "et1": e["et1"], "et2": e["et2"], "et3": e["et3"],
+                     "ccsd_t": out["eccsd"] + e["et1"] + e["et2"]},
+        "ccsd": {"nit": out["nit"], "rms": rms, "total_s": t_cc},
+        "roofline": {"bound": "mfma", "kernel": "t3_gemm_kernel (segmented-K, all launches)",
+                     "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP64_MFMA, "traffic": None,
+                     "algorithmic_flops_per_triple": 8.0 * nv * nvvt * kp},
+        "kernels_ms": {"gemm_total": parts["gemm"], "energy_total": parts["energy"],
+                       "gemm_per_triple": parts["gemm"] / ntri,
+                       "energy_per_triple": parts["energy"] / ntri},
+        "cpu_baseline": None if args.no_cpu_baseline else t3_cpu_baseline(F),
+    }
+    print(json.dumps(res), flush=True)
 
 
 def cpu_baseline(workload_flops: float):
@@ -375,11 +496,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg3",
-                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA))
+                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA) + sorted(TRIPLES))
     ap.add_argument("--lambda-maxit", type=int, default=80,
                     help="cfg5: MAXIT of the CCSD and Lambda solves")
     ap.add_argument("--scale", type=float, default=0.05,
                     help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
+    ap.add_argument("--t3-scale", type=float, default=0.01,
+                    help="cfg3t: std of the factor B (smaller ones are tried if CCSD diverges)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--complex-product", type=int, default=3, choices=(3, 4),
                     help="real MFMA products per complex product in the FP64 GEMMs")
@@ -413,6 +536,9 @@ def main():
         return
     if args.config in LAMBDA:
         bench_lambda(args)
+        return
+    if args.config in TRIPLES:
+        bench_triples(args)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8

@@ -2,8 +2,9 @@
 ! call sequence of relccsd (ccdriv.F:1715-1792, cceqns.F:141-643) with every
 ! term handed to librelccsd_hip.so through ISO_C_BINDING.
 !
-! Input  ccin.bin (stream): 8 int32 header (NO, NV, unused, MAXIT, MAXDIM,
-!        NTOL, NREP (0 = C1), LAMBDA), orbital energies, for NREP > 1 the fermion
+! Input  ccin.bin (stream): 8 int32 header (NO, NV, JOB, MAXIT, MAXDIM,
+!        NTOL, NREP (0 = C1), LAMBDA; JOB 2 adds the triples corrections
+!        after CCSD, as CCENER does with DOCCSDT, ccdriv.F:1799-1837), orbital energies, for NREP > 1 the fermion
 !        irrep of every spinor, the occupied count per irrep and MULTB(.,.,1)
 !        of the 2 NREP irreps (int32, as in MRCONEE), then FOO, FVO, FVV and
 !        the classes OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd's
@@ -11,7 +12,9 @@
 !        device in column batches, as GETVOVV / GETVVVV read them
 !        (ccgetv.F:225-402).
 ! Output the CCEQN iteration table on unit 6 (format of cceqns.F:6001) and
-!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout); with LAMBDA = 1
+!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout); with JOB = 2
+!        ET1, ET2, ET3 follow (and the T3CORR summary lines of cceqns.F:
+!        1799-1812 are printed); with LAMBDA = 1
 !        (C1) the Lambda equations are solved after CCSD and NIT, RMS of that
 !        solve and the response density (n x n complex, column-major) follow.
 program ccdrv_hip
@@ -27,7 +30,7 @@ program ccdrv_hip
   integer(c_int), allocatable :: irrsp(:), mult(:, :)
   type(rcc_sym_t) :: sym
   type(c_ptr) :: h
-  real(c_double) :: emp2, ecc, rms, diag, conv
+  real(c_double) :: emp2, ecc, rms, diag, conv, et(3)
 
   open (30, file='ccin.bin', access='stream', form='unformatted', status='old')
   read (30) hdr
@@ -136,6 +139,19 @@ program ccdrv_hip
   open (32, file='ccout.bin', access='stream', form='unformatted', status='replace')
   write (32) emp2, ecc, dble(nit)
   write (32) t1, t2
+  if (hdr(3) == 2) then
+    ! triples corrections on the converged amplitudes (T3CORR)
+    call rcc_check(h, rcc_t3corr(h, et), 'T3CORR')
+    write (*, '(//a/)') '  Triple excitation estimates'
+    write (*, '(a,t40,f25.15)') ' CCSD correlation energy :', ecc
+    write (*, '(a,t40,f25.15)') ' 4th order triples correction :', et(1)
+    write (*, '(a,t40,f25.15)') ' 5th order triples (T) correction :', et(2)
+    write (*, '(a,t40,f25.15)') ' 5th order triples -T  correction :', et(3)
+    write (*, '(a,t40,f25.15)') ' Total CCSD+T  energy :', ecc + et(1)
+    write (*, '(a,t40,f25.15)') ' Total CCSD(T) energy :', ecc + et(1) + et(2)
+    write (*, '(a,t40,f25.15)') ' Total CCSD-T  energy :', ecc + et(1) + et(3)
+    write (32) et
+  end if
   if (hdr(8) /= 0) then
     ! Lambda equations and the one-particle response density
     call rcc_check(h, rcc_lambda(h, lnit, lrms), 'LAMBDA')

@@ -91,6 +91,12 @@ module relccsd_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: dens(*)
     end function
+    ! non-iterative triples of T3CORR (cceqns.F:1336-1863): ET1, ET2, ET3
+    integer(c_int) function rcc_t3corr(h, et) bind(c, name="rcc_t3corr")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: et(3)
+    end function
     integer(c_int) function rcc_term(h) bind(c)
       import :: c_int, c_ptr
       type(c_ptr), value :: h

@@ -64,3 +64,26 @@ def test_fortran_driver_lambda_density_gives_reference_gradient(tmp_path):
     grad = float(np.real(np.trace(D @ g["V"])))
     fd = float(g["fd4"])
     assert abs(grad - fd) <= 2e-6 * abs(fd) + 2e-11, (grad, fd)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_triples_match_reference(tmp_path):
+    # JOB 2: CCSD, then T3CORR on the converged amplitudes (DOCCSDT)
+    g = load("ccsd", "cfg1_6o40v")
+    t = load("t3", "conv_6o40v")
+    no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    s = system_for(g)
+    path = tmp_path / "ccin.bin"
+    refrun._write_input(str(path), s, 1, maxit, maxdim, ntol)
+    raw = bytearray(path.read_bytes())
+    raw[8:12] = np.array([2], dtype=np.int32).tobytes()      # header word 3: JOB
+    path.write_bytes(bytes(raw))
+    p = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "Total CCSD(T) energy" in p.stdout
+    out = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
+    nt1, nt2 = g["T1"].size, g["T2"].size
+    et = out[3 + 2 * (nt1 + nt2):]
+    assert et.size == 3
+    for x, key in zip(et, ("ET1", "ET2", "ET3")):
+        assert abs(x - float(t[key])) < 1e-10, (key, x, float(t[key]))
