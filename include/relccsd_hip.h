@@ -40,7 +40,12 @@ enum {
 enum {
     RCC_ARR_HO = 0, RCC_ARR_HV = 1, RCC_ARR_HOV = 2, RCC_ARR_GO = 3, RCC_ARR_GV = 4,
     RCC_ARR_AZ = 5, RCC_ARR_H = 6, RCC_ARR_S1 = 7, RCC_ARR_S2 = 8, RCC_ARR_T1 = 9,
-    RCC_ARR_T2 = 10, RCC_ARR_TAU = 11
+    RCC_ARR_T2 = 10, RCC_ARR_TAU = 11,
+    /* the Fock blocks and the six classes as the handle holds them (C1
+     * handles; for checking the integral front end) */
+    RCC_ARR_FOO = 12, RCC_ARR_FVO = 13, RCC_ARR_FVV = 14, RCC_ARR_OOOO = 15,
+    RCC_ARR_VOOO = 16, RCC_ARR_VVOO = 17, RCC_ARR_VOVO = 18, RCC_ARR_VOVV = 19,
+    RCC_ARR_VVVV = 20
 };
 
 /* Symmetry description, as RDSYMI / SYMTAB hold it (ccdriv.F:2934, :746).
@@ -223,6 +228,35 @@ int rcc_lambda_vjp(rcc_t* h, const double* l1, const double* l2, int with_energy
 int rcc_density(rcc_t* h, double* dens);
 /* Device time of the last rcc_lambda iteration, ms. */
 int rcc_last_lambda_ms(rcc_t* h, double* ms);
+
+/* ---- Integral front end: RDINTS (ccints.F:33-569) with MKINDEX (:1421),
+ * GET2INTS (:765) and GETFM / MAKEFM (:1031-1419), called by CCSETI
+ * (ccdriv.F:1427-1570) when DOSORT.  Reads the MOLFDIR files MRCONEE
+ * (records 1-6: dimensions, irreps and orbital energies of the spinors,
+ * the one-electron matrix) and MDCINT (the Kramers-reduced two-electron
+ * list), expands every canonical integral into its 16 time-reversal /
+ * permutation / conjugation images on the GPU, antisymmetrises them into
+ * the six classes of the handle (replacing what rcc_set_ints would set),
+ * and builds FOO, FVO, FVV and the orbital energies (Re diag F) as GETFM
+ * does with USEOE = NORECMP = F.  nelec[nnelec]: occupied spinors per
+ * irrep of MRCONEE record 2 (the &RELCCSD NELEC input); the handle's no / nv
+ * must match.  e0e1 (may be null, 3 doubles): E0, E1 and ECORE of RDINTS's
+ * summary (ESCF = E0 + E1 + ECORE).  When two images land on one element,
+ * the later one in file order is kept, as relccsd's sequential stores do.
+ * C1 handles and files (NREP = 1). */
+int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
+                     int nnelec, double* e0e1);
+/* The expansion and sort alone, from a canonical list already in device
+ * memory: d_idx m x (IKR, JKR, KKR, LKR) int32, d_val m complex; kr the
+ * MDCINT header's (KR(i), KR(-i)) pairs (1-based spinors), spinor_pos[s]
+ * the handle position of spinor s (>= 0 occupied, -(virtual position + 1)).
+ * ms (may be null): device time of the sort. */
+int rcc_sort_ints_device(rcc_t* h, const int* spinor_pos, int nsp, const int* kr, int nkr,
+                         const void* d_idx, const void* d_val, int64_t m, double* ms);
+/* Device time (ms) of the last integral sort. */
+double rcc_last_sort_ms(rcc_t* h);
+/* The handle's orbital energies (no + nv doubles, occupied then virtual). */
+int rcc_get_eps(rcc_t* h, double* eps);
 
 /* ---- Non-iterative triples: T3CORR (cceqns.F:1336-1863), called by
  * CCENER after the CCSD equations when DOCCSDT is set (ccdriv.F:1799-1837).

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librelccsd_hip.so")
 SOURCES = ["rcc_zgemm.hip", "rcc_kernels.hip", "rcc_split.hip", "rcc_split_i8.hip",
-           "rcc_triples.hip", "rcc_solver.hip"]
+           "rcc_triples.hip", "rcc_frontend.hip", "rcc_solver.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RCC_OFFLOAD_ARCH", "gfx950")
 

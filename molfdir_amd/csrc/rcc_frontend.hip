@@ -1,0 +1,515 @@
+// Integral front end: MOLFDIR MRCONEE + MDCINT -> the six antisymmetrised
+// classes and the Fock blocks, on the GPU.  Replaces RDINTS (ccints.F:33-569)
+// with MKINDEX (:1421), GET2INTS (:765) and GETFM / MAKEFM (:1031-1419) for
+// the single-reference (non Fock-space) case.
+//
+// The host parses the two Fortran unformatted files (4-byte record markers).
+// The Kramers-reduced list is uploaded as one array of canonical integrals
+// (IKR JKR | KKR LKR) and expanded on the device, one thread per canonical
+// integral, into its 16 index quadruples in GET2INTS's loop order: time
+// reversal of the (IJ) and (KL) pairs, particle interchange and complex
+// conjugation.  Each (ij|kl) = <ik|jl> belongs to the class of the (ik; jl)
+// pair types -- the direct part when (i, k) is a stored pair, the exchange
+// part at (k, i) otherwise.
+//
+// relccsd stores every expanded value by assignment while it reads the file,
+// so when two expansions land on one element the later one in file order
+// is kept.  The device reproduces that exactly in two passes: the first
+// takes, per element and part, the maximum of a 64-bit order key
+// (canonical index x 16 + expansion index) with atomicMax, the second lets
+// only the holder of that key write.  The two surviving writes of an
+// element (direct, exchange) are added into a zeroed array, which is
+// exact in either order: class = direct - exchange, as RDINTS's XAXPY.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "rcc_internal.h"
+
+namespace rcc {
+
+namespace {
+
+struct Reader {
+    FILE* f = nullptr;
+    std::string path;
+    ~Reader() {
+        if (f) fclose(f);
+    }
+    // next record into buf; false at end of file
+    bool next(std::vector<unsigned char>& buf, std::string& err) {
+        int32_t n = 0, m = 0;
+        if (fread(&n, 4, 1, f) != 1) return false;
+        if (n < 0) {
+            err = path + ": records longer than 2 GiB (split records) are not supported";
+            return false;
+        }
+        buf.resize((size_t)n);
+        if (n && fread(buf.data(), 1, (size_t)n, f) != (size_t)n) {
+            err = path + ": truncated record";
+            return false;
+        }
+        if (fread(&m, 4, 1, f) != 1 || m != n) {
+            err = path + ": bad record marker";
+            return false;
+        }
+        return true;
+    }
+};
+
+template <class T>
+T get(const std::vector<unsigned char>& b, size_t& p) {
+    T v;
+    std::memcpy(&v, b.data() + p, sizeof(T));
+    p += sizeof(T);
+    return v;
+}
+
+}  // namespace
+
+int read_mrconee(const char* path, MolfdirOne& out, std::string& err) {
+    Reader r;
+    r.path = path;
+    r.f = fopen(path, "rb");
+    if (!r.f) {
+        err = std::string("cannot open ") + path;
+        return -1;
+    }
+    std::vector<unsigned char> b;
+    size_t p = 0;
+    // 1: NSP, BREIT, ECORE
+    if (!r.next(b, err) || b.size() < 16) {
+        if (err.empty()) err = std::string(path) + ": record 1";
+        return -1;
+    }
+    out.nsp = get<int32_t>(b, p);
+    out.breit = get<int32_t>(b, p) != 0;
+    out.ecore = get<double>(b, p);
+    // 2: NSYMRP + names, 3: NREP + abelian names, 4: MULTB
+    if (!r.next(b, err)) return -1;
+    p = 0;
+    out.nsymrp = get<int32_t>(b, p);
+    if (!r.next(b, err)) return -1;
+    p = 0;
+    out.nrep = get<int32_t>(b, p);
+    if (!r.next(b, err)) return -1;
+    // 5: per spinor (higher-group irrep, abelian irrep, orbital energy)
+    if (!r.next(b, err) || b.size() != (size_t)out.nsp * 16) {
+        if (err.empty()) err = std::string(path) + ": record 5 size";
+        return -1;
+    }
+    p = 0;
+    out.irrep_higher.resize(out.nsp);
+    out.irrep.resize(out.nsp);
+    out.eps.resize(out.nsp);
+    for (int s = 0; s < out.nsp; ++s) {
+        out.irrep_higher[s] = get<int32_t>(b, p);
+        out.irrep[s] = get<int32_t>(b, p);
+        out.eps[s] = get<double>(b, p);
+    }
+    // 6: the one-electron matrix, (Re, Im), column-major
+    if (!r.next(b, err) || b.size() != (size_t)out.nsp * out.nsp * 16) {
+        if (err.empty()) err = std::string(path) + ": record 6 (one-electron matrix) missing or of wrong size";
+        return -1;
+    }
+    out.h1.resize((size_t)out.nsp * out.nsp);
+    std::memcpy(out.h1.data(), b.data(), b.size());
+    return 0;
+}
+
+int read_mdcint(const char* path, MdcintList& out, std::string& err) {
+    Reader r;
+    r.path = path;
+    r.f = fopen(path, "rb");
+    if (!r.f) {
+        err = std::string("cannot open ") + path;
+        return -1;
+    }
+    std::vector<unsigned char> b;
+    if (!r.next(b, err) || b.size() < 22) {
+        if (err.empty()) err = std::string(path) + ": header record";
+        return -1;
+    }
+    size_t p = 18;     // DATEX (10) + TIMEX (8)
+    out.nkr = get<int32_t>(b, p);
+    if (out.nkr <= 0 || b.size() != 22 + (size_t)out.nkr * 8) {
+        err = std::string(path) + ": header record size does not match NKR";
+        return -1;
+    }
+    out.kr.resize(2 * (size_t)out.nkr);
+    std::memcpy(out.kr.data(), b.data() + 22, out.kr.size() * 4);
+    out.idx.clear();
+    out.val.clear();
+    while (true) {
+        if (!r.next(b, err)) {
+            if (err.empty()) err = std::string(path) + ": no end record (IKR = JKR = 0)";
+            return -1;
+        }
+        if (b.size() < 12) {
+            err = std::string(path) + ": short integral record";
+            return -1;
+        }
+        p = 0;
+        const int ikr = get<int32_t>(b, p), jkr = get<int32_t>(b, p), nz = get<int32_t>(b, p);
+        if (ikr == 0) {
+            if (jkr != 0) {
+                err = std::string(path) + ": continuation files (IKR = 0, JKR != 0) are not supported";
+                return -1;
+            }
+            break;
+        }
+        if (nz < 0 || b.size() != 12 + (size_t)nz * 24) {
+            err = std::string(path) + ": integral record size does not match NZ";
+            return -1;
+        }
+        const size_t base = out.idx.size();
+        out.idx.resize(base + nz);
+        out.val.resize(base + nz);
+        for (int z = 0; z < nz; ++z) {
+            int32_t kl[2];
+            std::memcpy(kl, b.data() + 12 + (size_t)z * 8, 8);
+            out.idx[base + z] = make_int4(ikr, jkr, kl[0], kl[1]);
+        }
+        std::memcpy(out.val.data() + base, b.data() + 12 + (size_t)nz * 8, (size_t)nz * 16);
+    }
+    return 0;
+}
+
+namespace {
+
+struct SortDev {
+    int no, nv;
+    int64_t rows[6];
+    double2* cls[6];
+    unsigned long long* key[2][6];   // [direct / exchange][class]
+};
+
+// MKINDEX for 0-based spinors s: spos[s] >= 0 occupied position, else
+// virtual position -spos - 1
+__device__ __forceinline__ int ptype(int pi, int pj) {   // 1 oo, 2 vo, 3 ov, 4 vv
+    return (pi >= 0 ? 1 : 2) + (pj >= 0 ? 0 : 2);
+}
+
+__device__ __forceinline__ int64_t tri(int p, int q, int n) {
+    return (int64_t)q * n - ((int64_t)q * (q + 1)) / 2 + (p - q - 1);
+}
+
+// packed pair index + 1 of (I, J), 0 when not a stored pair
+__device__ __forceinline__ int64_t pindex(int pi, int pj, int no, int nv) {
+    const int t = ptype(pi, pj);
+    if (t == 1) return pi > pj ? tri(pi, pj, no) + 1 : 0;
+    if (t == 4) {
+        const int a = -pi - 1, b = -pj - 1;
+        return a > b ? tri(a, b, nv) + 1 : 0;
+    }
+    if (t == 2) return (int64_t)pj * nv + (-pi - 1) + 1;
+    return 0;
+}
+
+__device__ __forceinline__ int class_of(int t) {
+    switch (t) {
+        case 1: return 0;    // OOOO
+        case 2: return 1;    // VOOO
+        case 4: return 2;    // VVOO
+        case 6: return 3;    // VOVO
+        case 14: return 4;   // VOVV
+        case 16: return 5;   // VVVV
+        default: return -1;
+    }
+}
+
+// The 16 expansions of one canonical integral in GET2INTS's loop order
+// (ccints.F:936-1021); f(v, class, part, element, re sign, im sign).
+template <class F>
+__device__ __forceinline__ void expand16(int4 q, const int* __restrict__ kr, int nkr,
+                                         const int* __restrict__ spos, const SortDev& d, F f) {
+    const int IK = q.x, JK = q.y, KK = q.z, LK = q.w;
+    const int ijsign = (IK > 0) == (JK > 0) ? 1 : -1;
+    int i0 = kr[nkr + IK], j0 = kr[nkr + JK];
+    int ijsg = 1;
+    int v = 0;
+    for (int t1 = 0; t1 < 2; ++t1) {
+        int k0 = kr[nkr + KK], l0 = kr[nkr + LK];
+        int klsg = 1;
+        for (int t2 = 0; t2 < 2; ++t2) {
+            int i1 = i0, j1 = j0, k1 = k0, l1 = l0;
+            for (int pi = 0; pi < 2; ++pi) {
+                int i = i1, j = j1, k = k1, l = l1;
+                for (int cc = 0; cc < 2; ++cc, ++v) {
+                    const int sj = spos[j], sl = spos[l];
+                    const int64_t jl = pindex(sj, sl, d.no, d.nv);
+                    if (jl != 0 && i != k) {
+                        const int si = spos[i], sk = spos[k];
+                        const int64_t ik = pindex(si, sk, d.no, d.nv);
+                        int part, t;
+                        int64_t row;
+                        if (ik != 0) {
+                            part = 0;
+                            t = (ptype(sj, sl) - 1) * 4 + ptype(si, sk);
+                            row = ik - 1;
+                        } else {
+                            part = 1;
+                            t = (ptype(sj, sl) - 1) * 4 + ptype(sk, si);
+                            row = pindex(sk, si, d.no, d.nv) - 1;
+                        }
+                        const int c = class_of(t);
+                        if (c >= 0) {
+                            const int sg = ijsg * klsg;
+                            f(v, c, part, row + (jl - 1) * d.rows[c], sg, cc == 0 ? sg : -sg);
+                        }
+                    }
+                    i = j1;
+                    j = i1;
+                    k = l1;
+                    l = k1;
+                }
+                i1 = k0;
+                j1 = l0;
+                k1 = i0;
+                l1 = j0;
+            }
+            k0 = kr[nkr - LK];
+            l0 = kr[nkr - KK];
+            klsg = (KK > 0) == (LK > 0) ? 1 : -1;
+        }
+        i0 = kr[nkr - JK];
+        j0 = kr[nkr - IK];
+        ijsg = ijsign;
+    }
+}
+
+__device__ __forceinline__ bool in_range(int x, int nkr) { return x != 0 && x >= -nkr && x <= nkr; }
+
+__device__ __forceinline__ bool valid(int4 q, int nkr) {
+    return in_range(q.x, nkr) && in_range(q.y, nkr) && in_range(q.z, nkr) && in_range(q.w, nkr);
+}
+
+// entries with a Kramers index outside +-NKR are skipped and flagged
+__global__ void sort_claim_kernel(SortDev d, const int4* __restrict__ idx, int64_t m, int64_t n0,
+                                  const int* __restrict__ kr, int nkr,
+                                  const int* __restrict__ spos, int* __restrict__ bad) {
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long base = (unsigned long long)(n0 + n) * 16ull + 1ull;
+        const int4 q = idx[n];
+        if (!valid(q, nkr)) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int, int) {
+            atomicMax(d.key[part][c] + e, base + (unsigned long long)v);
+        });
+    }
+}
+
+__global__ void sort_write_kernel(SortDev d, const int4* __restrict__ idx,
+                                  const double2* __restrict__ val, int64_t m, int64_t n0,
+                                  const int* __restrict__ kr, int nkr,
+                                  const int* __restrict__ spos) {
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long base = (unsigned long long)(n0 + n) * 16ull + 1ull;
+        const double2 x = val[n];
+        const int4 q = idx[n];
+        if (!valid(q, nkr)) continue;
+        expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int sre, int sim) {
+            if (d.key[part][c][e] == base + (unsigned long long)v) {
+                // direct - exchange, added into zeroed memory (exact in any order)
+                const double s = part == 0 ? 1.0 : -1.0;
+                atomicAdd(&d.cls[c][e].x, s * sre * x.x);
+                atomicAdd(&d.cls[c][e].y, s * sim * x.y);
+            }
+        });
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x)
+        p[x] = 0ull;
+}
+
+__global__ void fill_z_kernel(double2* p, int64_t n) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x)
+        p[x] = make_double2(0.0, 0.0);
+}
+
+int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1 << 16)); }
+
+// GETFM + MAKEFM (ccints.F:1031-1419) on packed C1 classes:
+//   F(i,j) = conj h(I,J) + sum_k <ik||jk>,  F(a,i) = conj h + sum_k <ak||ik>,
+//   F(a,c) = conj h + sum_k <ak||ck>
+// h1 is indexed by spinor; occ / vir hold the spinor of each position.
+__global__ void fock_kernel(int no, int nv, int nsp, const double2* __restrict__ h1,
+                            const int* __restrict__ occ, const int* __restrict__ vir,
+                            const double2* __restrict__ OOOO, const double2* __restrict__ VOOO,
+                            const double2* __restrict__ VOVO, double2* __restrict__ FOO,
+                            double2* __restrict__ FVO, double2* __restrict__ FVV) {
+    const int64_t noot = (int64_t)no * (no - 1) / 2, nvo = (int64_t)no * nv;
+    const int64_t total = (int64_t)no * no + nvo + (int64_t)nv * nv;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        double2 f;
+        if (x < (int64_t)no * no) {
+            const int i = (int)(x % no), j = (int)(x / no);
+            f = h1[occ[i] + (int64_t)occ[j] * nsp];
+            f.y = -f.y;
+            for (int k = 0; k < no; ++k) {
+                if (k == i || k == j) continue;
+                // <ik||jk>: bra pair (i,k), ket pair (j,k), one sign per reversed pair
+                const int64_t b = i > k ? tri(i, k, no) : tri(k, i, no);
+                const int64_t c = j > k ? tri(j, k, no) : tri(k, j, no);
+                const double s = (i > k) == (j > k) ? 1.0 : -1.0;
+                const double2 v = OOOO[b + c * noot];
+                f.x += s * v.x;
+                f.y += s * v.y;
+            }
+            FOO[x] = f;
+        } else if (x < (int64_t)no * no + nvo) {
+            const int64_t y = x - (int64_t)no * no;
+            const int a = (int)(y % nv), i = (int)(y / nv);
+            f = h1[vir[a] + (int64_t)occ[i] * nsp];
+            f.y = -f.y;
+            for (int k = 0; k < no; ++k) {
+                if (k == i) continue;
+                const int64_t c = i > k ? tri(i, k, no) : tri(k, i, no);
+                const double s = i > k ? 1.0 : -1.0;
+                const double2 v = VOOO[(int64_t)k * nv + a + c * nvo];
+                f.x += s * v.x;
+                f.y += s * v.y;
+            }
+            FVO[y] = f;
+        } else {
+            const int64_t y = x - (int64_t)no * no - nvo;
+            const int a = (int)(y % nv), c = (int)(y / nv);
+            f = h1[vir[a] + (int64_t)vir[c] * nsp];
+            f.y = -f.y;
+            for (int k = 0; k < no; ++k) {
+                const double2 v = VOVO[(int64_t)k * nv + a + ((int64_t)k * nv + c) * nvo];
+                f.x += v.x;
+                f.y += v.y;
+            }
+            FVV[y] = f;
+        }
+    }
+}
+
+// E0 = sum_i Re F(i,i), E1 = -1/2 sum_ik Re <ik||ik>, eps = Re diag F (one
+// workgroup, fixed order)
+__global__ void __launch_bounds__(256)
+fock_energy_kernel(int no, int nv, const double2* __restrict__ OOOO,
+                   const double2* __restrict__ FOO, const double2* __restrict__ FVV,
+                   double* __restrict__ eps, double* __restrict__ e01) {
+    __shared__ double sh[2][256];
+    const int tid = threadIdx.x;
+    const int64_t noot = (int64_t)no * (no - 1) / 2;
+    double e0 = 0.0, e1 = 0.0;
+    for (int i = tid; i < no; i += 256) {
+        e0 += FOO[i + (int64_t)i * no].x;
+        eps[i] = FOO[i + (int64_t)i * no].x;
+    }
+    for (int a = tid; a < nv; a += 256) eps[no + a] = FVV[a + (int64_t)a * nv].x;
+    for (int64_t p = tid; p < noot; p += 256) e1 -= OOOO[p + p * noot].x;
+    sh[0][tid] = e0;
+    sh[1][tid] = e1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            sh[0][tid] += sh[0][tid + s];
+            sh[1][tid] += sh[1][tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        e01[0] = sh[0][0];
+        e01[1] = sh[1][0];
+    }
+}
+
+}  // namespace
+
+#define FECK(call)                                                              \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            return -1;                                                          \
+        }                                                                       \
+    } while (0)
+
+int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos, int nsp,
+                          const int* d_kr, int nkr, const int4* d_idx, const double2* d_val,
+                          int64_t m, hipStream_t st, double* ms, std::string& err) {
+    (void)nsp;
+    SortDev d;
+    d.no = no;
+    d.nv = nv;
+    const int64_t noot = (int64_t)no * (no - 1) / 2, nvvt = (int64_t)nv * (nv - 1) / 2,
+                  nvo = (int64_t)no * nv;
+    const int64_t rows[6] = {noot, nvo, nvvt, nvo, nvo, nvvt};
+    const int64_t cols[6] = {noot, noot, noot, nvo, nvvt, nvvt};
+    int64_t nkeys = 0;
+    for (int c = 0; c < 6; ++c) {
+        d.rows[c] = rows[c];
+        d.cls[c] = cls[c];
+        nkeys += 2 * rows[c] * cols[c];
+    }
+    unsigned long long* keys = nullptr;
+    int* bad = nullptr;
+    FECK(hipMalloc((void**)&keys, (size_t)std::max<int64_t>(nkeys, 1) * 8));
+    FECK(hipMalloc((void**)&bad, sizeof(int)));
+    FECK(hipMemsetAsync(bad, 0, sizeof(int), st));
+    int64_t off = 0;
+    for (int part = 0; part < 2; ++part)
+        for (int c = 0; c < 6; ++c) {
+            d.key[part][c] = keys + off;
+            off += rows[c] * cols[c];
+        }
+    hipEvent_t e0, e1;
+    FECK(hipEventCreate(&e0));
+    FECK(hipEventCreate(&e1));
+    FECK(hipEventRecord(e0, st));
+    fill_u64_kernel<<<grid_for(nkeys), 256, 0, st>>>(keys, nkeys);
+    for (int c = 0; c < 6; ++c)
+        fill_z_kernel<<<grid_for(rows[c] * cols[c]), 256, 0, st>>>(cls[c], rows[c] * cols[c]);
+    const int g = grid_for(m);
+    if (m > 0) {
+        sort_claim_kernel<<<g, 256, 0, st>>>(d, d_idx, m, 0, d_kr, nkr, d_spos, bad);
+        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos);
+    }
+    FECK(hipGetLastError());
+    FECK(hipEventRecord(e1, st));
+    FECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    FECK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    int hbad = 0;
+    FECK(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FECK(hipFree(keys));
+    FECK(hipFree(bad));
+    if (hbad) {
+        err = "integral sort: a Kramers index of the list is 0 or outside +-NKR (entries skipped)";
+        return -1;
+    }
+    return 0;
+}
+
+int build_fock_device(int no, int nv, int nsp, const double2* d_h1, const int* d_occ,
+                      const int* d_vir, double2* const* cls, double2* FOO, double2* FVO,
+                      double2* FVV, double* eps, double* d_e01, hipStream_t st, std::string& err) {
+    const int64_t total = (int64_t)no * no + (int64_t)no * nv + (int64_t)nv * nv;
+    fock_kernel<<<grid_for(total), 256, 0, st>>>(no, nv, nsp, d_h1, d_occ, d_vir, cls[0], cls[1],
+                                                 cls[3], FOO, FVO, FVV);
+    FECK(hipGetLastError());
+    fock_energy_kernel<<<1, 256, 0, st>>>(no, nv, cls[0], FOO, FVV, eps, d_e01);
+    FECK(hipGetLastError());
+    return 0;
+}
+
+}  // namespace rcc

@@ -84,6 +84,34 @@ int t3_chunk(const T3State* s);
 // loop order); work buffers are kept in s between calls.
 int t3_run(T3State* s, const T3Input& in, int64_t first, int64_t count, hipStream_t st,
            double* et, T3Timing* tm, std::string& err);
+// ---- integral front end (rcc_frontend.hip) ----
+struct MolfdirOne {          // MRCONEE records 1-6
+    int nsp = 0, nsymrp = 0, nrep = 0;
+    bool breit = false;
+    double ecore = 0.0;
+    std::vector<int> irrep_higher, irrep;   // 1-based, per spinor
+    std::vector<double> eps;
+    std::vector<double2> h1;                // nsp x nsp, column-major
+};
+struct MdcintList {          // MDCINT: KR table and the canonical integrals
+    int nkr = 0;
+    std::vector<int> kr;                    // (KR(i), KR(-i)), i = 1..nkr, 1-based spinors
+    std::vector<int4> idx;                  // (IKR, JKR, KKR, LKR)
+    std::vector<double2> val;
+};
+int read_mrconee(const char* path, MolfdirOne& out, std::string& err);
+int read_mdcint(const char* path, MdcintList& out, std::string& err);
+// Expansion + sort of m canonical integrals (device arrays) into the six
+// packed C1 classes cls[0..5] (OOOO, VOOO, VVOO, VOVO, VOVV, VVVV), which
+// are overwritten.  d_spos[s]: C1 position of spinor s (>= 0 occupied,
+// -(position + 1) virtual); d_kr[nkr + i] = spinor of KR(i), i in [-nkr, nkr].
+int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos, int nsp,
+                          const int* d_kr, int nkr, const int4* d_idx, const double2* d_val,
+                          int64_t m, hipStream_t st, double* ms, std::string& err);
+// GETFM + MAKEFM on the sorted classes: Fock blocks, orbital energies, E0, E1.
+int build_fock_device(int no, int nv, int nsp, const double2* d_h1, const int* d_occ,
+                      const int* d_vir, double2* const* cls, double2* FOO, double2* FVO,
+                      double2* FVV, double* eps, double* d_e01, hipStream_t st, std::string& err);
 int mfma_probe(double* dout, hipStream_t st);
 int mfma_rate(int chains, int iters, int nblocks, double* out, hipStream_t st);
 

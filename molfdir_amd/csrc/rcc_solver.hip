@@ -169,6 +169,7 @@ struct rcc_handle {
     // non-iterative triples: work buffers and the last call's timing
     T3State* t3 = nullptr;
     T3Timing t3_tm;
+    double last_sort_ms = 0.0;     // last integral sort (rcc_read_molfdir / rcc_sort_ints_device)
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -2581,7 +2582,25 @@ int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
         case RCC_ARR_T1: src = h->T1(); len = h->nt1; break;
         case RCC_ARR_T2: src = h->T2(); len = h->nt2; break;
         case RCC_ARR_TAU: src = h->TAU; len = h->nt2; break;
+        case RCC_ARR_FOO: src = h->FOO; len = (int64_t)o * o; break;
+        case RCC_ARR_FVO: src = h->FVO; len = h->nvo; break;
+        case RCC_ARR_FVV: src = h->FVV; len = (int64_t)v * v; break;
+        case RCC_ARR_OOOO: case RCC_ARR_VOOO: case RCC_ARR_VVOO: case RCC_ARR_VOVO:
+        case RCC_ARR_VOVV: case RCC_ARR_VVVV: {
+            const int c = which - RCC_ARR_OOOO;
+            src = h->cls[c];
+            len = h->cls_rows[c] * h->cls_cols[c];
+            if (h->sym.nrep > 1) {
+                h->err = "rcc_get_array: classes and Fock blocks are read back from C1 handles";
+                return -1;
+            }
+            break;
+        }
         default: h->err = "rcc_get_array: unknown array"; return -1;
+    }
+    if (which >= RCC_ARR_FOO && h->sym.nrep > 1) {
+        h->err = "rcc_get_array: classes and Fock blocks are read back from C1 handles";
+        return -1;
     }
     if (h->sym.nrep > 1) {
         int lay = RCC_LAY_T2;
@@ -2793,6 +2812,178 @@ int rcc_autotune_entries(rcc_t* h, int64_t* fp64_shapes, int64_t* int8_shapes) {
     if (!h) return -1;
     if (fp64_shapes) *fp64_shapes = zgemm_tuned_shapes();
     if (int8_shapes) *int8_shapes = i8_tuned_shapes();
+    return 0;
+}
+
+// ---- integral front end: MRCONEE + MDCINT -> classes and Fock (RDINTS) ----
+
+// Spinor classification of SYMTAB (ccdriv.F:832-858): within each
+// higher-group irrep, in file order, the first NELEC spinors are occupied.
+// C1 working position: occupied irrep-major then virtual irrep-major.
+static int molfdir_positions(rcc_t* h, const MolfdirOne& one, const int* nelec, int nnelec,
+                             std::vector<int>& spos, std::vector<int>& occ_sp,
+                             std::vector<int>& vir_sp) {
+    if (nnelec != one.nsymrp) {
+        h->err = "rcc_read_molfdir: " + std::to_string(nnelec) + " NELEC values for " +
+                 std::to_string(one.nsymrp) + " irreps in MRCONEE";
+        return -1;
+    }
+    if (one.nrep != h->sym.nrep || h->sym.nrep != 1) {
+        h->err = "rcc_read_molfdir: C1 handles and files only (NREP = 1)";
+        return -1;
+    }
+    std::vector<int> seen(one.nsymrp + 1, 0);
+    int no = 0, nv = 0;
+    spos.assign(one.nsp, 0);
+    occ_sp.clear();
+    vir_sp.clear();
+    for (int s = 0; s < one.nsp; ++s) {
+        const int g = one.irrep_higher[s];
+        if (g < 1 || g > one.nsymrp) {
+            h->err = "rcc_read_molfdir: spinor irrep out of range in MRCONEE";
+            return -1;
+        }
+        if (++seen[g] <= nelec[g - 1]) {
+            spos[s] = no++;
+            occ_sp.push_back(s);
+        } else {
+            spos[s] = -(nv++) - 1;
+            vir_sp.push_back(s);
+        }
+    }
+    if (no != h->no || nv != h->nv) {
+        h->err = "rcc_read_molfdir: the files give " + std::to_string(no) + " occupied / " +
+                 std::to_string(nv) + " virtual spinors, the handle has " +
+                 std::to_string(h->no) + " / " + std::to_string(h->nv);
+        return -1;
+    }
+    return 0;
+}
+
+static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int* kr, int nkr,
+                               const int4* d_idx, const double2* d_val, int64_t m, double* ms) {
+    const int nsp = (int)spos.size();
+    if (2 * nkr != nsp) {
+        h->err = "rcc_sort: INCONSISTENT MRCONEE AND MDCINT FILES (2 NKR != NSP)";
+        return -1;
+    }
+    std::vector<int> krt(2 * (size_t)nkr + 1, 0);
+    for (int i = 1; i <= nkr; ++i) {
+        const int a = kr[2 * (i - 1)], b = kr[2 * (i - 1) + 1];
+        if (a < 1 || a > nsp || b < 1 || b > nsp) {
+            h->err = "rcc_sort: KR table entry outside 1..NSP";
+            return -1;
+        }
+        krt[nkr + i] = a - 1;
+        krt[nkr - i] = b - 1;
+    }
+    int *d_spos = nullptr, *d_kr = nullptr;
+    HIPCHK(h, hipMalloc((void**)&d_spos, spos.size() * 4));
+    HIPCHK(h, hipMalloc((void**)&d_kr, krt.size() * 4));
+    HIPCHK(h, hipMemcpy(d_spos, spos.data(), spos.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_kr, krt.data(), krt.size() * 4, hipMemcpyHostToDevice));
+    const int rc = sort_integrals_device(h->no, h->nv, h->cls, d_spos, nsp, d_kr, nkr, d_idx,
+                                         d_val, m, h->st, ms, h->err);
+    (void)hipFree(d_spos);
+    (void)hipFree(d_kr);
+    if (rc) return -1;
+    h->derived_ok = false;
+    h->split_const_ok = false;
+    h->tau_ok = h->x_ok = h->ring_ok = false;
+    if (h->graph_ok || h->graph_warm) drop_graph(h);
+    return 0;
+}
+
+int rcc_sort_ints_device(rcc_t* h, const int* spinor_pos, int nsp, const int* kr, int nkr,
+                         const void* d_idx, const void* d_val, int64_t m, double* ms) {
+    if (!h || !spinor_pos || !kr) return -1;
+    if (h->sym.nrep != 1) {
+        h->err = "rcc_sort_ints_device: C1 handles only";
+        return -1;
+    }
+    std::vector<int> spos(spinor_pos, spinor_pos + nsp);
+    int no = 0, nv = 0;
+    for (int s = 0; s < nsp; ++s) {
+        if (spos[s] >= 0) ++no;
+        else ++nv;
+    }
+    std::vector<char> seen_o(h->no, 0), seen_v(h->nv, 0);
+    bool ok = no == h->no && nv == h->nv;
+    for (int s = 0; ok && s < nsp; ++s) {
+        const int p = spos[s] >= 0 ? spos[s] : -spos[s] - 1;
+        std::vector<char>& seen = spos[s] >= 0 ? seen_o : seen_v;
+        ok = p < (int)seen.size() && !seen[p];
+        if (ok) seen[p] = 1;
+    }
+    if (!ok) {
+        h->err = "rcc_sort_ints_device: spinor positions are not one occupied position 0..no-1 "
+                 "and one virtual position 0..nv-1 each";
+        return -1;
+    }
+    return sort_upload_and_run(h, spos, kr, nkr, (const int4*)d_idx, (const double2*)d_val, m, ms);
+}
+
+int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
+                     int nnelec, double* e0e1) {
+    if (!h || !mrconee || !mdcint || !nelec) return -1;
+    MolfdirOne one;
+    if (read_mrconee(mrconee, one, h->err)) return -1;
+    std::vector<int> spos, occ_sp, vir_sp;
+    CK(molfdir_positions(h, one, nelec, nnelec, spos, occ_sp, vir_sp));
+    MdcintList lst;
+    if (read_mdcint(mdcint, lst, h->err)) return -1;
+    const int64_t m = (int64_t)lst.idx.size();
+    int4* d_idx = nullptr;
+    double2* d_val = nullptr;
+    HIPCHK(h, hipMalloc((void**)&d_idx, (size_t)std::max<int64_t>(m, 1) * sizeof(int4)));
+    HIPCHK(h, hipMalloc((void**)&d_val, (size_t)std::max<int64_t>(m, 1) * sizeof(double2)));
+    if (m) {
+        HIPCHK(h, hipMemcpy(d_idx, lst.idx.data(), (size_t)m * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(d_val, lst.val.data(), (size_t)m * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    double ms = 0.0;
+    const int rc = sort_upload_and_run(h, spos, lst.kr.data(), lst.nkr, d_idx, d_val, m, &ms);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_val);
+    if (rc) return -1;
+    // Fock blocks and orbital energies (GETFM / MAKEFM; USEOE = F)
+    double2* d_h1 = nullptr;
+    int *d_occ = nullptr, *d_vir = nullptr;
+    double* d_e01 = nullptr;
+    const size_t nh = (size_t)one.nsp * one.nsp;
+    HIPCHK(h, hipMalloc((void**)&d_h1, nh * sizeof(double2)));
+    HIPCHK(h, hipMalloc((void**)&d_occ, occ_sp.size() * 4 + 4));
+    HIPCHK(h, hipMalloc((void**)&d_vir, vir_sp.size() * 4 + 4));
+    HIPCHK(h, hipMalloc((void**)&d_e01, 2 * sizeof(double)));
+    HIPCHK(h, hipMemcpy(d_h1, one.h1.data(), nh * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_occ, occ_sp.data(), occ_sp.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(d_vir, vir_sp.data(), vir_sp.size() * 4, hipMemcpyHostToDevice));
+    const int fr = build_fock_device(h->no, h->nv, one.nsp, d_h1, d_occ, d_vir, h->cls, h->FOO,
+                                     h->FVO, h->FVV, h->eps, d_e01, h->st, h->err);
+    double e01[2] = {0.0, 0.0};
+    if (!fr) HIPCHK(h, hipMemcpyAsync(e01, d_e01, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    (void)hipFree(d_h1);
+    (void)hipFree(d_occ);
+    (void)hipFree(d_vir);
+    (void)hipFree(d_e01);
+    if (fr) return -1;
+    h->derived_ok = false;
+    h->last_sort_ms = ms;
+    if (e0e1) {
+        e0e1[0] = e01[0];
+        e0e1[1] = e01[1];
+        e0e1[2] = one.ecore;
+    }
+    return 0;
+}
+
+double rcc_last_sort_ms(rcc_t* h) { return h ? h->last_sort_ms : -1.0; }
+
+int rcc_get_eps(rcc_t* h, double* eps) {
+    if (!h || !eps) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(eps, h->eps, (size_t)(h->no + h->nv) * 8, hipMemcpyDeviceToHost));
     return 0;
 }
 

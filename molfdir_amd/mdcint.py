@@ -1,0 +1,129 @@
+"""MOLFDIR integral files: writers for MRCONEE and MDCINT and a seeded
+synthetic Kramers-reduced integral list.
+
+Record layouts (Fortran unformatted sequential, 4-byte record markers) as
+relccsd reads them:
+
+MRCONEE (RDSYMI ccdriv.F:2978-2988, GETFM ccints.F:1091-1096)
+  1  NSP, BREIT (logical*4), ECORE (real*8)
+  2  NSYMRP, names (character*14 each), the occupied count per irrep (not read)
+  3  NREP, names of the 2 NREP abelian irreps (character*4 each)
+  4  MULTB(1:2NREP, 1:2NREP, 1), column-major
+  5  (higher-group irrep, abelian irrep, orbital energy) per spinor
+  6  the one-electron (Fock) matrix, (Re, Im) per element, NSP x NSP column-major
+
+MDCINT (RDINTS ccints.F:117-118, GET2INTS ccints.F:819-835)
+  header  DATEX (character*10), TIMEX (character*8), NKR, (KR(i), KR(-i), i = 1..NKR)
+  records IKR, JKR, NZ, (INDK, INDL) x NZ, (Re, Im) x NZ  -- the integrals
+          (IKR JKR | INDK INDL) over Kramers pairs (negative: the barred
+          partner), Mulliken order
+  end     IKR = JKR = 0
+"""
+from __future__ import annotations
+
+import struct
+
+import numpy as np
+
+
+def _record(f, payload: bytes):
+    n = len(payload)
+    f.write(struct.pack("<i", n))
+    f.write(payload)
+    f.write(struct.pack("<i", n))
+
+
+def write_mrconee(path, eps, irreps, nelec, h1, multb=None, ecore=0.0, breit=False):
+    """irreps: 1-based fermion irrep of each spinor; nelec: occupied count per
+    irrep; h1: NSP x NSP complex one-electron matrix; multb: 2NREP x 2NREP
+    table (default the C1 one)."""
+    eps = np.asarray(eps, dtype=np.float64)
+    nsp = eps.size
+    irreps = np.asarray(irreps, dtype=np.int32)
+    nelec = np.asarray(nelec, dtype=np.int32)
+    nrep = nelec.size
+    if multb is None:
+        multb = np.array([[2, 1], [1, 2]], dtype=np.int32)
+    multb = np.asarray(multb, dtype=np.int32)
+    with open(path, "wb") as f:
+        _record(f, struct.pack("<iid", nsp, 1 if breit else 0, ecore))
+        names = b"".join(f"A{i + 1:02d}".ljust(14).encode() for i in range(nrep))
+        _record(f, struct.pack("<i", nrep) + names + nelec.tobytes())
+        rn = b"".join(f" A{i + 1:02d}".encode() for i in range(nrep)) + \
+            b"".join(f" a{i + 1:02d}".encode() for i in range(nrep))
+        _record(f, struct.pack("<i", nrep) + rn)
+        _record(f, multb.T.astype(np.int32).tobytes())            # column-major
+        rec5 = np.zeros(nsp, dtype=[("g", "<i4"), ("a", "<i4"), ("e", "<f8")])
+        rec5["g"], rec5["a"], rec5["e"] = irreps, irreps, eps
+        _record(f, rec5.tobytes())
+        h = np.asarray(h1, dtype=np.complex128).reshape(nsp, nsp)
+        _record(f, np.ascontiguousarray(h.T).reshape(-1).view(np.float64).tobytes())
+
+
+def write_mdcint(path, kr, ikr, jkr, kkr, lkr, val):
+    """kr: (NKR, 2) spinor indices (1-based) of the unbarred / barred member
+    of each Kramers pair.  The canonical integrals (ikr, jkr | kkr, lkr) are
+    written one record per run of equal (ikr, jkr), in the given order."""
+    kr = np.asarray(kr, dtype=np.int32)
+    ikr, jkr = np.asarray(ikr, np.int32), np.asarray(jkr, np.int32)
+    kkr, lkr = np.asarray(kkr, np.int32), np.asarray(lkr, np.int32)
+    val = np.asarray(val, dtype=np.complex128)
+    with open(path, "wb") as f:
+        _record(f, b"01Jan2026 " + b"00:00:00" + struct.pack("<i", kr.shape[0]) +
+                kr.astype("<i4").tobytes())
+        n = ikr.size
+        s = 0
+        while s < n:
+            e = s + 1
+            while e < n and ikr[e] == ikr[s] and jkr[e] == jkr[s]:
+                e += 1
+            kl = np.stack([kkr[s:e], lkr[s:e]], axis=1).astype("<i4")
+            _record(f, struct.pack("<iii", int(ikr[s]), int(jkr[s]), e - s) + kl.tobytes() +
+                    val[s:e].view(np.float64).astype("<f8").tobytes())
+            s = e
+        _record(f, struct.pack("<iii", 0, 0, 0))
+
+
+def synthetic_list(nkr: int, seed: int, keep: float = 0.7, scale: float = 0.05):
+    """A seeded Kramers-reduced list: records (IKR, JKR) for IKR = 1..NKR and
+    0 < |JKR| <= IKR; in each, (KKR, LKR) with 0 < KKR <= IKR, 0 < |LKR| <= KKR,
+    each kept with probability `keep`; complex values N(0, scale^2).  The
+    list is not restricted to one representative per symmetry orbit: the
+    expansion then writes some sorted elements more than once, and the
+    last write (in file order) is the one relccsd keeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ik, jk, kk, lk = [], [], [], []
+    for i in range(1, nkr + 1):
+        for j in list(range(-i, 0)) + list(range(1, i + 1)):
+            for k in range(1, i + 1):
+                for l in list(range(-k, 0)) + list(range(1, k + 1)):
+                    ik.append(i)
+                    jk.append(j)
+                    kk.append(k)
+                    lk.append(l)
+    ik, jk, kk, lk = (np.asarray(x, np.int32) for x in (ik, jk, kk, lk))
+    m = rng.random(ik.size) < keep
+    ik, jk, kk, lk = ik[m], jk[m], kk[m], lk[m]
+    val = scale * (rng.normal(size=ik.size) + 1j * rng.normal(size=ik.size))
+    return ik, jk, kk, lk, val
+
+
+def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7):
+    """C1 MRCONEE + MDCINT in dirpath: NSP = 2 NKR spinors with shuffled
+    Kramers pairs, the first nocc spinors (input order) occupied, orbital
+    energies ascending, a Hermitian one-electron matrix (diagonal eps,
+    off-diagonal N(0, 0.02^2)).  Returns the generated arrays."""
+    import os
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    nsp = 2 * nkr
+    perm = rng.permutation(nsp) + 1
+    kr = perm.reshape(nkr, 2)
+    eps = np.sort(np.concatenate([rng.uniform(-2.0, -0.3, nocc), rng.uniform(0.2, 5.0, nsp - nocc)]))
+    h1 = 0.02 * (rng.normal(size=(nsp, nsp)) + 1j * rng.normal(size=(nsp, nsp)))
+    h1 = 0.5 * (h1 + h1.conj().T)
+    h1[np.arange(nsp), np.arange(nsp)] = eps
+    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
+    write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, np.ones(nsp, np.int32), [nocc], h1)
+    write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val)
+    return {"nsp": nsp, "nocc": nocc, "kr": kr, "eps": eps, "h1": h1, "ikr": ik, "jkr": jk,
+            "kkr": kk, "lkr": lk, "val": val}

@@ -24,7 +24,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "relccsd_hip.h")
 
 CLASSES = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5}
 ARRAYS = {"HO": 0, "HV": 1, "HOV": 2, "GO": 3, "GV": 4, "AZ": 5, "H": 6, "S1": 7,
-          "S2": 8, "T1": 9, "T2": 10, "TAU": 11}
+          "S2": 8, "T1": 9, "T2": 10, "TAU": 11, "FOO": 12, "FVO": 13, "FVV": 14, "OOOO": 15,
+          "VOOO": 16, "VVOO": 17, "VOVO": 18, "VOVV": 19, "VVVV": 20}
 TERMS = ("hointm", "hvintm", "hovintm", "t1eqns", "gointm", "gvintm", "aintm", "hintm",
          "t2eqns", "bintm", "denom", "zcore", "accept")
 
@@ -114,6 +115,13 @@ def load():
     lib.rcc_lambda_vjp.argtypes = [vp, vp, vp, ci, vp, vp]
     lib.rcc_density.argtypes = [vp, vp]
     lib.rcc_last_lambda_ms.argtypes = [vp, dp]
+    lib.rcc_read_molfdir.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.POINTER(ci), ci, dp]
+    lib.rcc_sort_ints_device.argtypes = [vp, ctypes.POINTER(ci), ci, ctypes.POINTER(ci), ci, vp,
+                                         vp, i64, dp]
+    lib.rcc_last_sort_ms.argtypes = [vp]
+    lib.rcc_last_sort_ms.restype = ctypes.c_double
+    lib.rcc_get_eps.argtypes = [vp, dp]
     lib.rcc_t3corr.argtypes = [vp, dp]
     lib.rcc_t3corr_part.argtypes = [vp, i64, i64, dp]
     lib.rcc_t3_triples.argtypes = [vp]
@@ -280,9 +288,12 @@ class RelCCSD:
         if self.nrep > 1:
             n = self.layout_size(ARRAY_LAYOUT[name])
         else:
-            n = {"HO": self.no ** 2, "HV": self.nv ** 2, "HOV": self.nt1, "GO": self.no ** 2,
-                 "GV": self.nv ** 2, "AZ": self.noot ** 2, "H": self.nt1 ** 2, "S1": self.nt1,
-                 "S2": self.nt2, "T1": self.nt1, "T2": self.nt2, "TAU": self.nt2}[name]
+            o, v, oo, vv, vo = self.no, self.nv, self.noot, self.nvvt, self.nt1
+            n = {"HO": o ** 2, "HV": v ** 2, "HOV": vo, "GO": o ** 2, "GV": v ** 2, "AZ": oo ** 2,
+                 "H": vo ** 2, "S1": vo, "S2": self.nt2, "T1": vo, "T2": self.nt2,
+                 "TAU": self.nt2, "FOO": o ** 2, "FVO": vo, "FVV": v ** 2, "OOOO": oo ** 2,
+                 "VOOO": vo * oo, "VVOO": vv * oo, "VOVO": vo ** 2, "VOVV": vo * vv,
+                 "VVVV": vv ** 2}[name]
         out = np.empty(n, np.complex128)
         self._check(self.lib.rcc_get_array(self.h, ARRAYS[name], _ptr(out), n))
         return out
@@ -427,6 +438,40 @@ class RelCCSD:
         ms = ctypes.c_double()
         self._check(self.lib.rcc_last_lambda_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def read_molfdir(self, directory, nelec, mrconee="MRCONEE", mdcint="MDCINT"):
+        """Integral front end (RDINTS): the six classes, the Fock blocks and
+        the orbital energies from the MOLFDIR files in `directory`; nelec the
+        occupied spinors per irrep of MRCONEE.  Returns E0, E1, ECORE and
+        ESCF = E0 + E1 + ECORE."""
+        import os
+        ne = (ctypes.c_int * len(nelec))(*[int(x) for x in nelec])
+        out = (ctypes.c_double * 3)()
+        self._check(self.lib.rcc_read_molfdir(
+            self.h, os.path.join(directory, mrconee).encode(), os.path.join(directory, mdcint).encode(),
+            ne, len(nelec), out))
+        return {"e0": out[0], "e1": out[1], "ecore": out[2], "escf": out[0] + out[1] + out[2]}
+
+    def sort_ints_device(self, spinor_pos, kr, idx, val):
+        """Expansion + sort of a canonical list in device memory (torch
+        tensors: idx int32 (m, 4), val complex128 (m,)); returns device ms."""
+        sp = np.ascontiguousarray(spinor_pos, dtype=np.int32)
+        k = np.ascontiguousarray(kr, dtype=np.int32).reshape(-1)
+        ms = ctypes.c_double()
+        self._check(self.lib.rcc_sort_ints_device(
+            self.h, sp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), sp.size,
+            k.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), k.size // 2,
+            ctypes.c_void_p(idx.data_ptr()), ctypes.c_void_p(val.data_ptr()), int(idx.shape[0]),
+            ctypes.byref(ms)))
+        return ms.value
+
+    def last_sort_ms(self):
+        return float(self.lib.rcc_last_sort_ms(self.h))
+
+    def get_eps(self):
+        out = np.empty(self.no + self.nv, np.float64)
+        self._check(self.lib.rcc_get_eps(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
 
     def t3corr(self, first=0, count=-1):
         """Non-iterative triples of T3CORR on the current amplitudes:

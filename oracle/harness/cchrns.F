@@ -20,6 +20,12 @@ C  every intermediate, so each GPU term can be checked on its own.
 C  Mode 3 reads given T1/T2 and calls T3CORR (cceqns.F:1336) with the
 C  arrays CCENER allocates for it (ccdriv.F:1805-1816), writing the
 C  triples corrections ET1, ET2, ET3 (COMMON /ENERG/).
+C  Mode 4 takes MRCONEE (records 1-6) and MDCINT as written by the
+C  caller into the working directory and runs the integral sort CCSETI
+C  (ccdriv.F:1427) -> RDINTS (ccints.F:33) exactly as CCMAIN does with
+C  DOSORT (ccdriv.F:555, 638); it writes the Fock blocks (FMTOFILE), the
+C  orbital energies after GETFM and the six sorted classes (GETOOOO ..
+C  GETVVVV, every irrep).
 C
 C  Input  : ccin.bin  (stream; header of 8 int32, then float64 data)
 C  Output : ccout.bin (stream; float64 data), plus the reference log
@@ -51,7 +57,7 @@ C
       CHARACTER*14 SNAM(16)
       INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16)
       CHARACTER*200 LINE
-      LOGICAL BRT
+      LOGICAL BRT,DONE
 C
       MASTER = 0
       MYPROC = 0
@@ -92,6 +98,7 @@ C
 C     MRCONEE records 1-5 exactly as RDSYMI reads them (ccdriv.F:2978-
 C     2988); the higher point group is taken equal to the abelian one.
 C
+      IF (MODE.EQ.4) GOTO 5
       OPEN (MRCONEE,FILE='MRCONEE',FORM='UNFORMATTED',
      &      STATUS='REPLACE')
       BRT = .FALSE.
@@ -109,6 +116,7 @@ C
 C
 C     Namelist input read by CCINPT (ccdriv.F:126-373)
 C
+    5 CONTINUE
       OPEN (31,FILE='MOLFDIR.INP',FORM='FORMATTED',STATUS='REPLACE')
       WRITE (LINE,'(16(I4,A1))') (NELRP(I),',',I=1,NREPI)
       WRITE (31,'(A,A,A)') ' &RELCCSD NELEC=',TRIM(LINE),
@@ -135,6 +143,7 @@ C
 C     Read the Fock blocks and the six classes (reference layout)
 C
       ALLOCATE (FOO(2*NFOO),FVO(2*NFVO),FVV(2*NFVV))
+      IF (MODE.EQ.4) GOTO 7
       ALLOCATE (V1(2*NV1),V2(2*NV2),V3(2*NV3),V4(2*NV4),V5(2*NV5),
      &          V6(2*NV6))
       READ (30) FOO,FVO,FVV,V1,V2,V3,V4,V5,V6
@@ -151,10 +160,55 @@ C
       CALL RWRIT(ITAPT+6,V6,NV6*RCW*IRW,1)
       DEALLOCATE (V1,V2,V3,V4,V5,V6)
 C
-      OPEN (32,FILE='ccout.bin',ACCESS='STREAM',FORM='UNFORMATTED',
+    7 OPEN (32,FILE='ccout.bin',ACCESS='STREAM',FORM='UNFORMATTED',
      &      STATUS='REPLACE')
 C
-      IF (MODE.EQ.1) THEN
+      IF (MODE.EQ.4) THEN
+C
+C        Integral sort from MRCONEE + MDCINT (sizes, then the sort)
+C
+         KFREE = 1
+         LFREE = 2000000000
+         MXCORR = 0
+         CALL CCSETI (1,DUM,KFREE,LFREE,MXCORR,MREM,EPS,NSP,IREPSP,
+     &                IREPSPI)
+         ALLOCATE (CC(MXCORR+1024))
+         KFREE = 1
+         LFREE = MXCORR
+         MREM = 0
+         CALL CCSETI (2,CC,KFREE,LFREE,MXCORR,MREM,EPS,NSP,IREPSP,
+     &                IREPSPI)
+         CALL FMTOFILE (.FALSE.,FVO,FOO,FVV)
+         WRITE (32) FOO,FVO,FVV
+         WRITE (32) (EPS(I),I=1,NSP)
+         DEALLOCATE (CC)
+         ALLOCATE (V1(2*NV1),V2(2*NV2),V3(2*NV3),V4(2*NV4),V5(2*NV5),
+     &             V6(2*NV6))
+         CALL GETOOOO (V1)
+         CALL GETVOOO (V2)
+         CALL GETVVOO (V3)
+         CALL GETVOVO (V4)
+         WRITE (32) V1,V2,V3,V4
+         OFF = 1
+         DO IRP = 1, NREP
+            ISTART = 0
+            NINT = 0
+            MINT = 2000000000
+            CALL GETVOVV (IRP,ISTART,NINT,DONE,V5(OFF),MINT)
+            OFF = OFF + 2*NVO(IRP)*NINT
+         ENDDO
+         OFF = 1
+         DO IRP = 1, NREP
+            ISTART = 0
+            NINT = 0
+            MINT = 2000000000
+            CALL GETVVVV (IRP,ISTART,NINT,DONE,V6(OFF),MINT)
+            OFF = OFF + 2*NVVT(IRP)*NINT
+         ENDDO
+         WRITE (32) V5,V6
+         WRITE (32) ESCF
+C
+      ELSEIF (MODE.EQ.1) THEN
 C
 C        MP2 + CCSD through the reference energy driver
 C

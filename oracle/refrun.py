@@ -10,6 +10,9 @@ the per-iteration table CCEQN prints (cceqns.F:270-272, 550-551).
 Mode 2 ("terms") evaluates every CCEQN term once on given T1/T2 and returns
 each intermediate (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before and after BINTM,
 S1/S2 after DENOM, and CCENG's energy).
+Mode 4 (run_sort) sorts a MOLFDIR MRCONEE + MDCINT pair with relccsd's own
+CCSETI -> RDINTS (ccints.F:33-569): the six classes, the Fock blocks after
+GETFM / MAKEFM, the orbital energies and ESCF.
 Mode 3 ("t3") evaluates the non-iterative triples corrections of T3CORR
 (cceqns.F:1336-1863) on given T1/T2: ET1 (fourth-order +T), ET2 ((T)) and
 ET3 (-T).
@@ -135,4 +138,43 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
         pos += n
     assert pos == z.size, (pos, z.size)
     out["ECC"] = raw[-1]
+    return out
+
+
+def run_sort(files_dir, no, nv, workdir=None, timeout=3600):
+    """Integral sort of files_dir/MRCONEE + files_dir/MDCINT (C1)."""
+    import shutil
+    from oracle.symtab import SymLayout
+    lay = SymLayout([no], [nv])
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        for fn in ("MRCONEE", "MDCINT"):
+            shutil.copy(os.path.join(files_dir, fn), os.path.join(d, fn))
+        hdr = np.array([no, nv, 4, 1, 1, 12, 0, 0], dtype=np.int32)
+        with open(os.path.join(d, "ccin.bin"), "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(np.zeros(no + nv).tobytes())
+        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
+            raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
+        raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+    out = {"log": log}
+    pos = 0
+    for name in ("FOO", "FVO", "FVV"):
+        n = lay.size(name)
+        out[name] = raw[pos:pos + 2 * n].view(np.complex128).copy()
+        pos += 2 * n
+    out["eps"] = raw[pos:pos + no + nv].copy()
+    pos += no + nv
+    for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        n = lay.size(name)
+        out[name] = raw[pos:pos + 2 * n].view(np.complex128).copy()
+        pos += 2 * n
+    out["ESCF"] = raw[pos]
+    assert pos + 1 == raw.size, (pos, raw.size)
+    for key, pat in (("E0", r"Zero order electronic energy :\s+(-?[\d.]+)"),
+                     ("E1", r"First order electronic energy :\s+(-?[\d.]+)")):
+        m = re.search(pat, log)
+        out[key] = float(m.group(1)) if m else None
+    m = re.search(r"Total time in .*?sort.*?:\s+([\d.]+)", log, re.I)
     return out

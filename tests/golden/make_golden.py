@@ -21,6 +21,10 @@ change in the generator is caught.
 * t3_<name>.npz: the triples corrections ET1, ET2, ET3 of relccsd's T3CORR
   (cceqns.F:1336-1863) on seeded random amplitudes (C1 and symmetry-blocked)
   or on the converged amplitudes of a ccsd_ fixture.
+* sort_<name>.npz: relccsd's integral sort (CCSETI -> RDINTS, ccints.F:33-569)
+  of seeded synthetic MOLFDIR files (molfdir_amd.mdcint.synthetic_files):
+  the six classes, the Fock blocks, the orbital energies, E0, E1, ESCF, and
+  the SHA-256 of the MDCINT file the reference read.
 * lambda_<name>.npz: converged CCSD energies of the reference at f and at
   f +- h V for a seeded Hermitian one-electron perturbation V (stored), the
   finite-field gradient d Ec / d lam the CPU path gives (relccsd's own
@@ -111,6 +115,32 @@ def t3_case(name, sno, snv, seed, scale, fp, amp):
     print(name, r["ET1"], r["ET2"], r["ET3"])
 
 
+SORT_CASES = [
+    # name, NKR, occupied spinors, seed, fraction of the canonical list kept
+    ("k5_2o", 5, 2, 2, 1.0),
+    ("k6_4o", 6, 4, 1, 0.7),
+    ("k7_6o", 7, 6, 3, 0.5),
+    ("k9_5o", 9, 5, 4, 0.8),
+    ("k12_8o", 12, 8, 6, 0.7),
+]
+
+
+def sort_case(name, nkr, nocc, seed, keep):
+    import hashlib
+    import tempfile
+    from molfdir_amd import mdcint
+    with tempfile.TemporaryDirectory() as d:
+        mdcint.synthetic_files(d, nkr, nocc, seed, keep)
+        digest = hashlib.sha256(open(os.path.join(d, "MDCINT"), "rb").read()).hexdigest()
+        r = refrun.run_sort(d, nocc, 2 * nkr - nocc)
+    keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    np.savez_compressed(
+        os.path.join(HERE, f"sort_{name}.npz"),
+        params=np.array([nkr, nocc, seed], dtype=np.int64), keep=keep, mdcint_sha256=digest,
+        E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
+    print(name, r["E0"], r["E1"], r["ESCF"])
+
+
 def symmetric_amplitudes(lay, seed):
     """Random T1/T2 with the symmetry of the layout, returned in C1 layout."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -169,6 +199,12 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--sort-only" in sys.argv:
+        for case in SORT_CASES:
+            sort_case(*case)
+        return
+    for case in SORT_CASES:
+        sort_case(*case)
     if "--t3-only" in sys.argv:
         for case in T3_CASES:
             t3_case(*case)

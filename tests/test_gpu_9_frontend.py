@@ -1,0 +1,64 @@
+"""GPU integral front end (rcc_read_molfdir, RDINTS ccints.F:33-569) against
+relccsd's own sort of the same seeded MOLFDIR files (tests/golden/sort_*.npz).
+
+Bar: the six classes bit for bit (values are moved and sign-flipped only;
+the last write of an element in file order wins on both sides); Fock blocks
+and orbital energies to 1e-14; E0 / E1 to 1e-12.  The device-list entry
+point (rcc_sort_ints_device) gives the same classes.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from molfdir_amd.relccsd import RelCCSD, RccError  # noqa: E402
+
+from _golden import load, names  # noqa: E402
+from test_oracle_rdints import CLASSES, regenerate  # noqa: E402
+
+
+@pytest.mark.parametrize("name", names("sort"))
+def test_read_molfdir_matches_reference_sort(name, tmp_path):
+    g = load("sort", name)
+    print(f"sort fixture {name}", flush=True)
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    r = RelCCSD(nocc, 2 * nkr - nocc)
+    e = r.read_molfdir(str(tmp_path), [nocc])
+    for c in CLASSES:
+        assert np.array_equal(r.get(c), g[c]), c
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(r.get(k) - g[k]).max() < 1e-14, k
+    assert np.abs(r.get_eps() - g["eps"]).max() < 1e-14
+    assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+    assert abs(e["escf"] - float(g["ESCF"])) < 1e-12
+    assert r.last_sort_ms() >= 0.0
+
+
+def test_device_list_entry_point_gives_the_same_classes(tmp_path):
+    g = load("sort", "k9_5o")
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    nsp = 2 * nkr
+    r = RelCCSD(nocc, nsp - nocc)
+    pos = np.concatenate([np.arange(nocc), -(np.arange(nsp - nocc) + 1)]).astype(np.int32)
+    idx = torch.tensor(np.stack([gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"]], 1),
+                       dtype=torch.int32, device="cuda")
+    val = torch.tensor(gen["val"], dtype=torch.complex128, device="cuda")
+    ms = r.sort_ints_device(pos, gen["kr"], idx, val)
+    assert ms >= 0.0
+    for c in CLASSES:
+        assert np.array_equal(r.get(c), g[c]), c
+
+
+def test_inconsistent_inputs_are_rejected(tmp_path):
+    g = load("sort", "k6_4o")
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    r = RelCCSD(nocc + 1, 2 * nkr - nocc - 1)
+    with pytest.raises(RccError):
+        r.read_molfdir(str(tmp_path), [nocc])          # handle of another occupation
+    r2 = RelCCSD(nocc, 2 * nkr - nocc)
+    with pytest.raises(RccError):
+        r2.read_molfdir(str(tmp_path), [nocc, 0])      # two NELEC values, one irrep
+    with pytest.raises(RccError):
+        r2.read_molfdir(str(tmp_path / "missing"), [nocc])

@@ -1,0 +1,71 @@
+"""The integral front-end oracle (oracle/rdints_oracle.py) against relccsd's
+own sort of seeded MOLFDIR files (CCSETI -> RDINTS, tests/golden/sort_*.npz).
+
+* The six classes agree bit for bit: the expansion moves values and flips
+  signs only, and both sides keep the last write of an element in file order.
+* The Fock blocks and orbital energies after GETFM / MAKEFM agree to 1e-14,
+  E0 and E1 to the 15 decimals relccsd prints.
+* A planted mistake -- keeping the first write of an element instead of the
+  last -- fails on the lists with repeated images (the k5 list keeps every
+  canonical integral, so many elements are written more than once).
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from molfdir_amd import mdcint
+from oracle import rdints_oracle as ro
+
+from _golden import load, names
+
+CLASSES = ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+
+
+def regenerate(g, d):
+    nkr, nocc, seed = (int(x) for x in g["params"])
+    gen = mdcint.synthetic_files(str(d), nkr, nocc, seed, float(g["keep"]))
+    digest = hashlib.sha256(open(os.path.join(str(d), "MDCINT"), "rb").read()).hexdigest()
+    assert digest == str(g["mdcint_sha256"]), "synthetic MDCINT generator changed"
+    return gen, nkr, nocc
+
+
+@pytest.mark.parametrize("name", names("sort"))
+def test_oracle_sort_matches_reference(name, tmp_path):
+    g = load("sort", name)
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    sp = ro.Spinors(np.ones(2 * nkr, int), [nocc])
+    kr = ro.kr_table(gen["kr"])
+    cl = ro.sort_classes(sp, kr, gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"], gen["val"])
+    for c in CLASSES:
+        assert np.array_equal(cl[c], g[c]), c
+    f = ro.fock(sp, gen["h1"], cl)
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(f[k] - g[k]).max() < 1e-14, k
+    assert np.abs(f["eps"] - g["eps"]).max() < 1e-14
+    assert abs(f["E0"] - float(g["E0"])) < 1e-13 and abs(f["E1"] - float(g["E1"])) < 1e-13
+    assert abs(f["E0"] + f["E1"] - float(g["ESCF"])) < 1e-13
+
+
+def test_planted_first_write_wins_is_caught(tmp_path, monkeypatch):
+    g = load("sort", "k5_2o")
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    sp = ro.Spinors(np.ones(2 * nkr, int), [nocc])
+    kr = ro.kr_table(gen["kr"])
+    seen = set()
+    orig = ro.expand
+
+    def first_wins(*a):
+        out = []
+        for e in orig(*a):
+            key = (e[1] > 0, ro.CLASS_OF_TYPE.get(abs(e[1])), e[2], e[3])
+            if key in seen:
+                continue
+            seen.add(key)
+            out.append(e)
+        return out
+
+    monkeypatch.setattr(ro, "expand", first_wins)
+    cl = ro.sort_classes(sp, kr, gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"], gen["val"])
+    assert any(not np.array_equal(cl[c], g[c]) for c in CLASSES)
