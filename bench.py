@@ -4,6 +4,7 @@
     python bench.py --config cfg4        # 64 small symmetric systems, batched
     python bench.py --config cfg5        # Lambda equations + response density
     python bench.py --config cfg3t       # CCSD(T) triples (T3CORR) at 30o/300v
+    python bench.py --config sort2       # integral front end (RDINTS) at 20o/200v
 
 A step is one CCEQN iteration (HOINTM .. BINTM, DENOM, DIIS, CCENG, S->T)
 over the whole synthetic system with every integral class resident in HBM.
@@ -34,6 +35,8 @@ BATCH = {"cfg4": (64, 8, 4)}   # systems, irreps, seed of the size / proportion 
 LAMBDA = {"cfg5": (20, 200, 5, "C1 20o/200v complex128")}
 TRIPLES = {"cfg3t": (30, 300, 3, "C1 30o/300v complex128")}
 T3_CPU_SAMPLE = (12, 70)  # relccsd T3CORR sample size (o, v)
+SORT = {"sort2": (20, 200, 2, "C1 20o/200v (110 Kramers pairs)")}
+SORT_CPU_SAMPLE = (64, 32)   # relccsd RDINTS sample: Kramers pairs, occupied spinors
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
@@ -280,6 +283,100 @@ def bench_triples(args):
     print(json.dumps(res), flush=True)
 
 
+def sort_cpu_baseline(m_workload: int):
+    """relccsd's own CCSETI -> RDINTS (oracle/_ref) on a seeded MOLFDIR file
+    pair written here; canonical integrals per second of its wall time."""
+    import tempfile
+    from molfdir_amd import mdcint
+    from oracle import refrun
+    nkr, nocc = SORT_CPU_SAMPLE
+    if not refrun.available():
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        gen = mdcint.synthetic_files(d, nkr, nocc, seed=3, keep=1.0)
+        t0 = time.perf_counter()
+        refrun.run_sort(d, nocc, 2 * nkr - nocc)
+        wall = time.perf_counter() - t0
+    m = len(gen["ikr"])
+    rate = m / wall
+    return {
+        "value": rate, "unit": "canonical integrals/s", "cores": 1, "kind": "reference",
+        "s_extrapolated": m_workload / rate,
+        "sample": (f"relccsd CCSETI -> RDINTS -> GETFM (unmodified reference sources) on a "
+                   f"seeded C1 MRCONEE + MDCINT pair, {nkr} Kramers pairs, {m} canonical "
+                   f"integrals, {wall:.2f} s wall for the whole process (file reads, three "
+                   f"sort passes, Fock build); rate extrapolated by the canonical count"),
+    }
+
+
+def bench_sort(args):
+    """The integral front end's expansion + six-class sort on a canonical
+    list already in HBM (rcc_sort_ints_device); one untimed warm-up sort."""
+    import numpy as np
+    import torch
+    from molfdir_amd.relccsd import RelCCSD
+    no, nv, seed, desc = SORT[args.config]
+    nsp = no + nv
+    nkr = nsp // 2
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    m = int(0.35 * nkr ** 4)
+    # canonical (IKR JKR | KKR LKR): 0 < |JKR| <= IKR, 0 < KKR <= IKR, 0 < |LKR| <= KKR,
+    # grouped by (IKR, JKR) as the file records are
+    i = torch.randint(1, nkr + 1, (m,), generator=g, device=dev)
+    j = (torch.rand(m, generator=g, device=dev) * i).long() + 1
+    j = torch.where(torch.rand(m, generator=g, device=dev) < 0.5, -j, j)
+    k = (torch.rand(m, generator=g, device=dev) * i).long() + 1
+    l = (torch.rand(m, generator=g, device=dev) * k).long() + 1
+    l = torch.where(torch.rand(m, generator=g, device=dev) < 0.5, -l, l)
+    order = torch.argsort(i * (2 * nkr + 1) + (j + nkr))
+    idx = torch.stack([i, j, k, l], 1)[order].to(torch.int32).contiguous()
+    del i, j, k, l, order
+    val = 0.05 * torch.randn(m, dtype=torch.complex128, generator=g, device=dev)
+    kr = np.stack([np.arange(1, nsp, 2), np.arange(2, nsp + 1, 2)], 1).astype(np.int32)
+    pos = np.concatenate([np.arange(no), -(np.arange(nv) + 1)]).astype(np.int32)
+    r = RelCCSD(no, nv)
+    r.sort_ints_device(pos, kr, idx, val)          # warm-up
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        times.append(r.sort_ints_device(pos, kr, idx, val))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = sum(times) / len(times)
+    noot, nvvt, nvo = no * (no - 1) // 2, nv * (nv - 1) // 2, no * nv
+    elements = noot * noot + nvo * noot + nvvt * noot + nvo * nvo + nvo * nvvt + nvvt * nvvt
+    alg_bytes = 32.0 * m + 16.0 * elements
+    ach = alg_bytes / (ms * 1e-3) / 1e9
+    print(f"[bench] sort of {m} canonical integrals ({16 * m} images) into {elements} class "
+          f"elements: {ms:.1f} ms per sort ({args.steps} sorts, {wall:.2f} s wall)",
+          file=sys.stderr, flush=True)
+    res = {
+        "metric": f"integral sort s + canonical integrals/s at {no}o/{nv}v (MDCINT -> 6 classes)",
+        "value": ms * 1e-3, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32 index / f64 values",
+        "data": (f"synthetic canonical Kramers-reduced list (seed {seed}): {m} integrals "
+                 f"(IKR JKR | KKR LKR) drawn in the canonical ranges, grouped by (IKR, JKR), "
+                 f"values N(0, 0.05^2); already in HBM"),
+        "config": {"workload": f"RDINTS expansion + six-class sort, {desc}", "canonical": m,
+                   "images": 16 * m, "class_elements": elements},
+        "canonical_per_s": m / (ms * 1e-3), "images_per_s": 16 * m / (ms * 1e-3),
+        "roofline": {"bound": "hbm", "kernel": "sort_claim + sort_write (whole sort)",
+                     "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                     "traffic": None,
+                     "algorithmic_bytes": alg_bytes,
+                     "note": "algorithmic bytes = the canonical list read once (32 B each) + "
+                             "every class element written once (16 B); the 64-bit order keys "
+                             "(two atomic passes over 16 images) are the real traffic"},
+        "cpu_baseline": None if args.no_cpu_baseline else sort_cpu_baseline(m),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline(workload_flops: float):
     """relccsd's own CPU CCEQN (built from the reference sources by
     oracle/Makefile.ref) on a bounded sample, extrapolated by flop count."""
@@ -496,7 +593,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg3",
-                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA) + sorted(TRIPLES))
+                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA) + sorted(TRIPLES) + sorted(SORT))
     ap.add_argument("--lambda-maxit", type=int, default=80,
                     help="cfg5: MAXIT of the CCSD and Lambda solves")
     ap.add_argument("--scale", type=float, default=0.05,
@@ -539,6 +636,9 @@ def main():
         return
     if args.config in TRIPLES:
         bench_triples(args)
+        return
+    if args.config in SORT:
+        bench_sort(args)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8

@@ -4,7 +4,11 @@
 !
 ! Input  ccin.bin (stream): 8 int32 header (NO, NV, JOB, MAXIT, MAXDIM,
 !        NTOL, NREP (0 = C1), LAMBDA; JOB 2 adds the triples corrections
-!        after CCSD, as CCENER does with DOCCSDT, ccdriv.F:1799-1837), orbital energies, for NREP > 1 the fermion
+!        after CCSD, as CCENER does with DOCCSDT, ccdriv.F:1799-1837; JOB 4
+!        sorts the MOLFDIR files MRCONEE + MDCINT of the working directory
+!        instead (CCSETI -> RDINTS, C1, NELEC = NO) and writes E0, E1, ECORE,
+!        FOO, FVO, FVV, the six classes and the orbital energies to
+!        ccout.bin, then stops), orbital energies, for NREP > 1 the fermion
 !        irrep of every spinor, the occupied count per irrep and MULTB(.,.,1)
 !        of the 2 NREP irreps (int32, as in MRCONEE), then FOO, FVO, FVV and
 !        the classes OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd's
@@ -30,7 +34,9 @@ program ccdrv_hip
   integer(c_int), allocatable :: irrsp(:), mult(:, :)
   type(rcc_sym_t) :: sym
   type(c_ptr) :: h
-  real(c_double) :: emp2, ecc, rms, diag, conv, et(3)
+  real(c_double) :: emp2, ecc, rms, diag, conv, et(3), e01(3)
+  integer(c_int) :: nel(1), which
+  integer(c_int64_t) :: narr(12:20)
 
   open (30, file='ccin.bin', access='stream', form='unformatted', status='old')
   read (30) hdr
@@ -68,6 +74,40 @@ program ccdrv_hip
   rc = rcc_create(sym, h)
   call rcc_check(h, rc, 'rcc_create')
   call rcc_check(h, rcc_set_options(h, maxdim, maxit, ntol, 0_c_int), 'rcc_set_options')
+
+  if (hdr(3) == 4) then
+    ! integral sort from the MOLFDIR files (CCSETI -> RDINTS -> GETFM)
+    close (30)
+    nel(1) = no
+    call rcc_check(h, rcc_read_molfdir(h, 'MRCONEE'//c_null_char, 'MDCINT'//c_null_char, &
+                                       nel, 1_c_int, e01), 'RDINTS')
+    write (*, '(a,t40,f25.15)') ' Nuclear repulsion + core energy :', e01(3)
+    write (*, '(a,t40,f25.15)') ' Zero order electronic energy :', e01(1)
+    write (*, '(a,t40,f25.15)') ' First order electronic energy :', e01(2)
+    write (*, '(a,t40,f25.15)') ' SCF energy :', e01(1) + e01(2) + e01(3)
+    narr(12) = int(no, c_int64_t) ** 2
+    narr(13) = int(nv, c_int64_t) * no
+    narr(14) = int(nv, c_int64_t) ** 2
+    narr(15) = (int(no, c_int64_t) * (no - 1) / 2) ** 2
+    narr(16) = int(nv, c_int64_t) * no * (int(no, c_int64_t) * (no - 1) / 2)
+    narr(17) = (int(nv, c_int64_t) * (nv - 1) / 2) * (int(no, c_int64_t) * (no - 1) / 2)
+    narr(18) = (int(nv, c_int64_t) * no) ** 2
+    narr(19) = int(nv, c_int64_t) * no * (int(nv, c_int64_t) * (nv - 1) / 2)
+    narr(20) = (int(nv, c_int64_t) * (nv - 1) / 2) ** 2
+    open (32, file='ccout.bin', access='stream', form='unformatted', status='replace')
+    write (32) e01
+    do which = 12, 20
+      if (allocated(buf)) deallocate (buf)
+      allocate (buf(2 * max(1_c_int64_t, narr(which))))
+      call rcc_check(h, rcc_get_array(h, which, buf, narr(which)), 'rcc_get_array')
+      write (32) buf(1:2 * narr(which))
+    end do
+    call rcc_check(h, rcc_get_eps(h, eps), 'rcc_get_eps')
+    write (32) eps
+    close (32)
+    call rcc_destroy(h)
+    stop
+  end if
 
   ! block sizes from the library's SYMTAB restatement (IOO/IVO/IVV totals)
   call rcc_check(h, rcc_layout_block(h, RCC_LAY_FOO, nrep, rows, cols, nfoo), 'layout')

@@ -91,6 +91,29 @@ module relccsd_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: dens(*)
     end function
+    ! integral front end (RDINTS ccints.F:33-569): classes and Fock from the
+    ! MOLFDIR files MRCONEE + MDCINT (C strings: end them with c_null_char)
+    integer(c_int) function rcc_read_molfdir(h, mrconee, mdcint, nelec, nnelec, e0e1) &
+        bind(c, name="rcc_read_molfdir")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: mrconee(*), mdcint(*)
+      integer(c_int), intent(in) :: nelec(*)
+      integer(c_int), value :: nnelec
+      real(c_double), intent(out) :: e0e1(3)
+    end function
+    integer(c_int) function rcc_get_array(h, which, out, n) bind(c, name="rcc_get_array")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int), value :: which
+      real(c_double), intent(out) :: out(*)
+      integer(c_int64_t), value :: n
+    end function
+    integer(c_int) function rcc_get_eps(h, eps) bind(c, name="rcc_get_eps")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: eps(*)
+    end function
     ! non-iterative triples of T3CORR (cceqns.F:1336-1863): ET1, ET2, ET3
     integer(c_int) function rcc_t3corr(h, et) bind(c, name="rcc_t3corr")
       import :: c_int, c_ptr, c_double

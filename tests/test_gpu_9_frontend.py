@@ -62,3 +62,34 @@ def test_inconsistent_inputs_are_rejected(tmp_path):
         r2.read_molfdir(str(tmp_path), [nocc, 0])      # two NELEC values, one irrep
     with pytest.raises(RccError):
         r2.read_molfdir(str(tmp_path / "missing"), [nocc])
+
+
+def test_fortran_driver_sorts_molfdir_files(tmp_path):
+    # JOB 4 of the Fortran driver: rcc_read_molfdir through ISO_C_BINDING
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "molfdir_amd", "fortran", "ccdrv_hip.x")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    g = load("sort", "k7_6o")
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    nv = 2 * nkr - nocc
+    hdr = np.array([nocc, nv, 4, 1, 1, 12, 0, 0], dtype=np.int32)
+    with open(tmp_path / "ccin.bin", "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.zeros(nocc + nv).tobytes())
+    p = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    raw = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
+    assert abs(raw[0] - float(g["E0"])) < 1e-12 and abs(raw[1] - float(g["E1"])) < 1e-12
+    pos = 3
+    for k in ("FOO", "FVO", "FVV") + CLASSES:
+        n = g[k].size
+        x = raw[pos:pos + 2 * n].view(np.complex128)
+        pos += 2 * n
+        if k in CLASSES:
+            assert np.array_equal(x, g[k]), k
+        else:
+            assert np.abs(x - g[k]).max() < 1e-14, k
+    assert np.abs(raw[pos:] - g["eps"]).max() < 1e-14
