@@ -258,6 +258,23 @@ double rcc_last_sort_ms(rcc_t* h);
 /* The handle's orbital energies (no + nv doubles, occupied then virtual). */
 int rcc_get_eps(rcc_t* h, double* eps);
 
+/* ---- MCCRES restart file: PUTAMPT / GETAMPT (ccgetv.F:577-709) and the
+ * status / energy records of CCDUMP (ccdriv.F:2767-2816).  Direct access,
+ * 8192-byte records (RSRCLEN = 1024 REAL*8, IRECLEN for LINUX): record 1
+ * unused, 2 IMSTAT(1:30) (int32), 3 ETOT ESCF EMP2 ECCSD ET1 ET2 ET3, from 4
+ * T1 and from 4 + (2 NDIMT1 / 1024 + 1) T2, 512 complex per record, in
+ * relccsd's (symmetry-blocked) amplitude layout.  The handle forms read /
+ * write the handle's current T1, T2; imstat (30 ints) and energies (7
+ * doubles) may be null (written as zeros).  The host forms need no GPU and
+ * report errors through rcc_file_error(). */
+int rcc_write_mccres(rcc_t* h, const char* path, const int* imstat, const double* energies);
+int rcc_read_mccres(rcc_t* h, const char* path, int* imstat, double* energies);
+int rcc_mccres_write(const char* path, int64_t ndimt1, int64_t ndimt2, const double* t1,
+                     const double* t2, const int* imstat, const double* energies);
+int rcc_mccres_read(const char* path, int64_t ndimt1, int64_t ndimt2, double* t1, double* t2,
+                    int* imstat, double* energies);
+const char* rcc_file_error(void);
+
 /* ---- Non-iterative triples: T3CORR (cceqns.F:1336-1863), called by
  * CCENER after the CCSD equations when DOCCSDT is set (ccdriv.F:1799-1837).
  * On the handle's current T1 / T2 (normally after rcc_cceqn), the set Fock

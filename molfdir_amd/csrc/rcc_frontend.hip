@@ -513,3 +513,96 @@ int build_fock_device(int no, int nv, int nsp, const double2* d_h1, const int* d
 }
 
 }  // namespace rcc
+
+// ---- MCCRES restart file (PUTAMPT / GETAMPT ccgetv.F:577-709, CCDUMP
+// ccdriv.F:2767-2816, CCDINI ccdriv.F:2744-2751) ----
+// Direct access, records of RSRCLEN = 1024 REAL*8 (8192 bytes; IRECLEN for
+// LINUX, gpluuk.F:99-128): record 1 input (unused, zero), 2 IMSTAT(30),
+// 3 ETOT ESCF EMP2 ECCSD ET1 ET2 ET3, from 4 the T1 amplitudes and from
+// 4 + (2 NDIMT1 / 1024 + 1) the T2 amplitudes, 512 complex per record.
+namespace {
+
+thread_local std::string g_file_err;
+constexpr int64_t RSRCLEN = 1024;
+
+int64_t nrec(int64_t ndim) { return ndim * 2 / RSRCLEN + 1; }
+
+}  // namespace
+
+extern "C" const char* rcc_file_error(void) { return g_file_err.c_str(); }
+
+extern "C" int rcc_mccres_write(const char* path, int64_t ndimt1, int64_t ndimt2, const double* t1,
+                                const double* t2, const int* imstat, const double* energies) {
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        g_file_err = std::string("cannot create ") + path;
+        return -1;
+    }
+    std::vector<double> rec(RSRCLEN, 0.0);
+    bool ok = fwrite(rec.data(), 8, RSRCLEN, f) == (size_t)RSRCLEN;          // 1
+    std::vector<int32_t> st(2 * RSRCLEN, 0);
+    for (int i = 0; i < 30 && imstat; ++i) st[i] = imstat[i];
+    ok = ok && fwrite(st.data(), 4, 2 * RSRCLEN, f) == (size_t)(2 * RSRCLEN);  // 2
+    for (int i = 0; i < 7; ++i) rec[i] = energies ? energies[i] : 0.0;
+    ok = ok && fwrite(rec.data(), 8, RSRCLEN, f) == (size_t)RSRCLEN;          // 3
+    for (int part = 0; part < 2 && ok; ++part) {
+        const int64_t n = part == 0 ? ndimt1 : ndimt2;
+        const double* src = part == 0 ? t1 : t2;
+        for (int64_t r = 0; r < nrec(n) && ok; ++r) {
+            std::fill(rec.begin(), rec.end(), 0.0);
+            const int64_t c0 = r * (RSRCLEN / 2), nc = std::max<int64_t>(0, std::min<int64_t>(n - c0, RSRCLEN / 2));
+            if (nc > 0) std::memcpy(rec.data(), src + 2 * c0, (size_t)nc * 16);
+            ok = fwrite(rec.data(), 8, RSRCLEN, f) == (size_t)RSRCLEN;
+        }
+    }
+    if (fclose(f) != 0) ok = false;
+    if (!ok) {
+        g_file_err = std::string("write error on ") + path;
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int rcc_mccres_read(const char* path, int64_t ndimt1, int64_t ndimt2, double* t1,
+                               double* t2, int* imstat, double* energies) {
+    FILE* f = fopen(path, "rb");
+    if (!f) {
+        g_file_err = std::string("cannot open ") + path;
+        return -1;
+    }
+    fseek(f, 0, SEEK_END);
+    const long long size = ftell(f);
+    const long long need = (3 + nrec(ndimt1) + nrec(ndimt2)) * RSRCLEN * 8;
+    if (size < need) {
+        fclose(f);
+        g_file_err = std::string(path) + ": " + std::to_string(size) + " bytes, the amplitudes of " +
+                     "this layout need " + std::to_string(need);
+        return -1;
+    }
+    std::vector<double> rec(RSRCLEN);
+    bool ok = fseek(f, RSRCLEN * 8, SEEK_SET) == 0;
+    std::vector<int32_t> st(2 * RSRCLEN);
+    ok = ok && fread(st.data(), 4, 2 * RSRCLEN, f) == (size_t)(2 * RSRCLEN);
+    if (imstat)
+        for (int i = 0; i < 30; ++i) imstat[i] = st[i];
+    ok = ok && fread(rec.data(), 8, RSRCLEN, f) == (size_t)RSRCLEN;
+    if (energies)
+        for (int i = 0; i < 7; ++i) energies[i] = rec[i];
+    for (int part = 0; part < 2 && ok; ++part) {
+        const int64_t n = part == 0 ? ndimt1 : ndimt2;
+        double* dst = part == 0 ? t1 : t2;
+        const int64_t first = part == 0 ? 3 : 3 + nrec(ndimt1);       // 0-based record
+        ok = fseek(f, (long)(first * RSRCLEN * 8), SEEK_SET) == 0;
+        for (int64_t r = 0; r < nrec(n) && ok; ++r) {
+            ok = fread(rec.data(), 8, RSRCLEN, f) == (size_t)RSRCLEN;
+            const int64_t c0 = r * (RSRCLEN / 2), nc = std::max<int64_t>(0, std::min<int64_t>(n - c0, RSRCLEN / 2));
+            if (ok && nc > 0 && dst) std::memcpy(dst + 2 * c0, rec.data(), (size_t)nc * 16);
+        }
+    }
+    fclose(f);
+    if (!ok) {
+        g_file_err = std::string("read error on ") + path;
+        return -1;
+    }
+    return 0;
+}

@@ -2987,6 +2987,38 @@ int rcc_get_eps(rcc_t* h, double* eps) {
     return 0;
 }
 
+// ---- MCCRES restart file of the handle's amplitudes (PUTAMPT / GETAMPT) ----
+extern "C" int rcc_mccres_write(const char* path, int64_t ndimt1, int64_t ndimt2, const double* t1,
+                                const double* t2, const int* imstat, const double* energies);
+extern "C" int rcc_mccres_read(const char* path, int64_t ndimt1, int64_t ndimt2, double* t1,
+                               double* t2, int* imstat, double* energies);
+extern "C" const char* rcc_file_error(void);
+
+int rcc_write_mccres(rcc_t* h, const char* path, const int* imstat, const double* energies) {
+    if (!h || !path) return -1;
+    const int64_t n1 = h->sym.nrep > 1 ? lay_total(h, RCC_LAY_T1) : h->nt1;
+    const int64_t n2 = h->sym.nrep > 1 ? lay_total(h, RCC_LAY_T2) : h->nt2;
+    std::vector<double> t1(2 * (size_t)n1), t2(2 * (size_t)n2);
+    CK(rcc_get_amplitudes(h, t1.data(), t2.data()));
+    if (rcc_mccres_write(path, n1, n2, t1.data(), t2.data(), imstat, energies)) {
+        h->err = rcc_file_error();
+        return -1;
+    }
+    return 0;
+}
+
+int rcc_read_mccres(rcc_t* h, const char* path, int* imstat, double* energies) {
+    if (!h || !path) return -1;
+    const int64_t n1 = h->sym.nrep > 1 ? lay_total(h, RCC_LAY_T1) : h->nt1;
+    const int64_t n2 = h->sym.nrep > 1 ? lay_total(h, RCC_LAY_T2) : h->nt2;
+    std::vector<double> t1(2 * (size_t)n1), t2(2 * (size_t)n2);
+    if (rcc_mccres_read(path, n1, n2, t1.data(), t2.data(), imstat, energies)) {
+        h->err = rcc_file_error();
+        return -1;
+    }
+    return rcc_set_amplitudes(h, t1.data(), t2.data());
+}
+
 int64_t rcc_t3_triples(rcc_t* h) { return h ? t3_triples(h->no) : -1; }
 
 int rcc_t3corr_part(rcc_t* h, int64_t first, int64_t count, double* et) {

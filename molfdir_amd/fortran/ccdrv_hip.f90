@@ -16,7 +16,8 @@
 !        device in column batches, as GETVOVV / GETVVVV read them
 !        (ccgetv.F:225-402).
 ! Output the CCEQN iteration table on unit 6 (format of cceqns.F:6001) and
-!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout); with JOB = 2
+!        ccout.bin: EMP2, ECCSD, NIT, T1, T2 (same layout), and the restart
+!        file MCCRES (PUTAMPT / CCDUMP layout); with JOB = 2
 !        ET1, ET2, ET3 follow (and the T3CORR summary lines of cceqns.F:
 !        1799-1812 are printed); with LAMBDA = 1
 !        (C1) the Lambda equations are solved after CCSD and NIT, RMS of that
@@ -35,7 +36,8 @@ program ccdrv_hip
   type(rcc_sym_t) :: sym
   type(c_ptr) :: h
   real(c_double) :: emp2, ecc, rms, diag, conv, et(3), e01(3)
-  integer(c_int) :: nel(1), which
+  integer(c_int) :: nel(1), which, imstat(30)
+  real(c_double) :: energ(7)
   integer(c_int64_t) :: narr(12:20)
 
   open (30, file='ccin.bin', access='stream', form='unformatted', status='old')
@@ -192,6 +194,21 @@ program ccdrv_hip
     write (*, '(a,t40,f25.15)') ' Total CCSD-T  energy :', ecc + et(1) + et(3)
     write (32) et
   end if
+  ! restart file as CCENER leaves it (PUTAMPT + CCDUMP): IMSTAT(4:6) and
+  ! ETOT ESCF EMP2 ECCSD ET1 ET2 ET3 (no SCF energy here: ESCF = 0)
+  imstat = 0
+  imstat(4) = 3
+  imstat(5) = 3      ! set after CCEQN whether or not it converged (ccdriv.F:1792)
+  energ = 0.0d0
+  energ(1) = ecc
+  energ(3) = emp2
+  energ(4) = ecc
+  if (hdr(3) == 2) then
+    imstat(6) = 3
+    energ(5:7) = et
+    energ(1) = ecc + et(1) + et(2)
+  end if
+  call rcc_check(h, rcc_write_mccres(h, 'MCCRES'//c_null_char, imstat, energ), 'PUTAMPT')
   if (hdr(8) /= 0) then
     ! Lambda equations and the one-particle response density
     call rcc_check(h, rcc_lambda(h, lnit, lrms), 'LAMBDA')

@@ -114,6 +114,23 @@ module relccsd_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: eps(*)
     end function
+    ! MCCRES restart file of the handle's amplitudes (PUTAMPT / CCDUMP)
+    integer(c_int) function rcc_write_mccres(h, path, imstat, energies) &
+        bind(c, name="rcc_write_mccres")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(in) :: imstat(30)
+      real(c_double), intent(in) :: energies(7)
+    end function
+    integer(c_int) function rcc_read_mccres(h, path, imstat, energies) &
+        bind(c, name="rcc_read_mccres")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: imstat(30)
+      real(c_double), intent(out) :: energies(7)
+    end function
     ! non-iterative triples of T3CORR (cceqns.F:1336-1863): ET1, ET2, ET3
     integer(c_int) function rcc_t3corr(h, et) bind(c, name="rcc_t3corr")
       import :: c_int, c_ptr, c_double

@@ -32,6 +32,38 @@ TERMS = ("hointm", "hvintm", "hovintm", "t1eqns", "gointm", "gvintm", "aintm", "
 _lib = None
 
 
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def mccres_write(path, t1, t2, imstat=None, energies=None):
+    """relccsd restart file (PUTAMPT / CCDUMP layout) from host amplitudes in
+    relccsd's layout; imstat 30 ints, energies ETOT ESCF EMP2 ECCSD ET1 ET2 ET3.
+    No GPU needed."""
+    lib = load()
+    t1 = np.ascontiguousarray(t1, dtype=np.complex128)
+    t2 = np.ascontiguousarray(t2, dtype=np.complex128)
+    st = None if imstat is None else (ctypes.c_int * 30)(*[int(x) for x in imstat])
+    en = None if energies is None else np.ascontiguousarray(energies, dtype=np.float64)
+    if lib.rcc_mccres_write(str(path).encode(), t1.size, t2.size, _dptr(t1.view(np.float64)),
+                            _dptr(t2.view(np.float64)), st,
+                            None if en is None else _dptr(en)) != 0:
+        raise RccError(lib.rcc_file_error().decode())
+
+
+def mccres_read(path, nt1, nt2):
+    """(T1, T2, IMSTAT, energies) of a relccsd restart file (GETAMPT layout)."""
+    lib = load()
+    t1 = np.empty(nt1, np.complex128)
+    t2 = np.empty(nt2, np.complex128)
+    st = (ctypes.c_int * 30)()
+    en = np.empty(7, np.float64)
+    if lib.rcc_mccres_read(str(path).encode(), nt1, nt2, _dptr(t1.view(np.float64)),
+                           _dptr(t2.view(np.float64)), st, _dptr(en)) != 0:
+        raise RccError(lib.rcc_file_error().decode())
+    return t1, t2, np.array(list(st)), en
+
+
 class RccError(RuntimeError):
     pass
 
@@ -115,6 +147,11 @@ def load():
     lib.rcc_lambda_vjp.argtypes = [vp, vp, vp, ci, vp, vp]
     lib.rcc_density.argtypes = [vp, vp]
     lib.rcc_last_lambda_ms.argtypes = [vp, dp]
+    lib.rcc_mccres_write.argtypes = [ctypes.c_char_p, i64, i64, dp, dp, ctypes.POINTER(ci), dp]
+    lib.rcc_mccres_read.argtypes = [ctypes.c_char_p, i64, i64, dp, dp, ctypes.POINTER(ci), dp]
+    lib.rcc_file_error.restype = ctypes.c_char_p
+    lib.rcc_write_mccres.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci), dp]
+    lib.rcc_read_mccres.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci), dp]
     lib.rcc_read_molfdir.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.POINTER(ci), ci, dp]
     lib.rcc_sort_ints_device.argtypes = [vp, ctypes.POINTER(ci), ci, ctypes.POINTER(ci), ci, vp,
@@ -472,6 +509,21 @@ class RelCCSD:
         out = np.empty(self.no + self.nv, np.float64)
         self._check(self.lib.rcc_get_eps(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def write_mccres(self, path, imstat=None, energies=None):
+        """The handle's T1, T2 as a relccsd restart file (PUTAMPT + CCDUMP)."""
+        st = None if imstat is None else (ctypes.c_int * 30)(*[int(x) for x in imstat])
+        en = None if energies is None else np.ascontiguousarray(energies, dtype=np.float64)
+        self._check(self.lib.rcc_write_mccres(self.h, str(path).encode(), st,
+                                              None if en is None else _dptr(en)))
+
+    def read_mccres(self, path):
+        """Sets T1, T2 from a relccsd restart file (GETAMPT); returns
+        (IMSTAT, energies)."""
+        st = (ctypes.c_int * 30)()
+        en = np.empty(7, np.float64)
+        self._check(self.lib.rcc_read_mccres(self.h, str(path).encode(), st, _dptr(en)))
+        return np.array(list(st)), en
 
     def t3corr(self, first=0, count=-1):
         """Non-iterative triples of T3CORR on the current amplitudes:

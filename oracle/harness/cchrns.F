@@ -20,6 +20,10 @@ C  every intermediate, so each GPU term can be checked on its own.
 C  Mode 3 reads given T1/T2 and calls T3CORR (cceqns.F:1336) with the
 C  arrays CCENER allocates for it (ccdriv.F:1805-1816), writing the
 C  triples corrections ET1, ET2, ET3 (COMMON /ENERG/).
+C  Mode 5 copies the records of MCCRES.in (a restart file written by the
+C  caller) into MCCRES after CCDINI and reads it back with GETAMPT
+C  (ccgetv.F:643-709) and the status / energy records CCDUMP writes;
+C  mode 1 leaves the MCCRES of its CCENER run in the working directory.
 C  Mode 4 takes MRCONEE (records 1-6) and MDCINT as written by the
 C  caller into the working directory and runs the integral sort CCSETI
 C  (ccdriv.F:1427) -> RDINTS (ccints.F:33) exactly as CCMAIN does with
@@ -52,7 +56,8 @@ C
       REAL*8, ALLOCATABLE :: T1(:),T2(:),S1(:),S2(:),AZ(:),H(:),
      &                       HO(:),HV(:),HOV(:),GOO(:),GVV(:),
      &                       BUF1(:),BUF2(:),BUF3(:)
-      REAL*8 DUM(4),ECC
+      REAL*8 DUM(4),ECC,RBUF(RSRCLEN)
+      INTEGER ISZ
       CHARACTER*4 RNAM(32)
       CHARACTER*14 SNAM(16)
       INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16)
@@ -207,6 +212,34 @@ C
          ENDDO
          WRITE (32) V5,V6
          WRITE (32) ESCF
+C
+      ELSEIF (MODE.EQ.5) THEN
+C
+C        GETAMPT on a restart file written by the caller
+C
+         ALLOCATE (T1(2*NDIMT1),T2(2*NDIMT2))
+         IRL = IRECLEN (RSRCLEN,0,0)
+         INQUIRE (FILE='MCCRES.in',SIZE=ISZ)
+         NRS = ISZ / (8*RSRCLEN)
+         OPEN (33,FILE='MCCRES.in',ACCESS='DIRECT',STATUS='OLD',
+     &         RECL=IRL)
+         OPEN (MCCRES,FILE='MCCRES',ACCESS='DIRECT',STATUS='OLD',
+     &         RECL=IRL)
+         DO IR = 1, NRS
+            READ (33,REC=IR) RBUF
+            WRITE (MCCRES,REC=IR) RBUF
+         ENDDO
+         CLOSE (33)
+         CLOSE (MCCRES)
+         CALL GETAMPT (T1,T2)
+         OPEN (MCCRES,FILE='MCCRES',ACCESS='DIRECT',STATUS='OLD',
+     &         RECL=IRL)
+         READ (MCCRES,REC=2) IMSTAT
+         READ (MCCRES,REC=3) RBUF
+         CLOSE (MCCRES)
+         WRITE (32) T1,T2
+         WRITE (32) (DBLE(IMSTAT(I)),I=1,30)
+         WRITE (32) (RBUF(I),I=1,7)
 C
       ELSEIF (MODE.EQ.1) THEN
 C

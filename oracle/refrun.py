@@ -99,12 +99,12 @@ def parse_iterations(log: str):
 
 
 def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
-        workdir=None, timeout=3600):
+        workdir=None, timeout=3600, keep_mccres=None):
     """Outputs are in the reference's (symmetry) layout; for a C1 system
     that is the C1 layout."""
     lay = layout_of(sysd)
     nt1, nt2 = lay.size("T1"), lay.size("T2")
-    m = {"ccsd": 1, "terms": 2, "t3": 3}[mode]
+    m = {"ccsd": 1, "terms": 2, "t3": 3, "restart": 5}[mode]
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         _write_input(os.path.join(d, "ccin.bin"), sysd, m, maxit, maxdim, ntol, t1, t2)
         p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
@@ -112,6 +112,9 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
         if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
             raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
         raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+        if keep_mccres:
+            import shutil
+            shutil.copy(os.path.join(d, "MCCRES"), keep_mccres)
     out = {"log": log}
     if m == 1:
         out["emp2"], out["eccsd"], out["nit"] = raw[0], raw[1], int(raw[2])
@@ -178,3 +181,23 @@ def run_sort(files_dir, no, nv, workdir=None, timeout=3600):
         out[key] = float(m.group(1)) if m else None
     m = re.search(r"Total time in .*?sort.*?:\s+([\d.]+)", log, re.I)
     return out
+
+
+def read_restart(sysd, mccres_path, workdir=None, timeout=600):
+    """Mode 5: relccsd's GETAMPT (and the CCDUMP records) on a given MCCRES
+    file; returns T1, T2 (reference layout), IMSTAT and the 7 energies."""
+    import shutil
+    lay = layout_of(sysd)
+    nt1, nt2 = lay.size("T1"), lay.size("T2")
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        _write_input(os.path.join(d, "ccin.bin"), sysd, 5, 1, 1, 12)
+        shutil.copy(mccres_path, os.path.join(d, "MCCRES.in"))
+        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
+            raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
+        raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+    z = raw[:2 * (nt1 + nt2)].view(np.complex128)
+    return {"T1": z[:nt1].copy(), "T2": z[nt1:].copy(),
+            "IMSTAT": raw[2 * (nt1 + nt2):2 * (nt1 + nt2) + 30].astype(np.int64),
+            "energies": raw[2 * (nt1 + nt2) + 30:].copy()}

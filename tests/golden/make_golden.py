@@ -25,6 +25,10 @@ change in the generator is caught.
   of seeded synthetic MOLFDIR files (molfdir_amd.mdcint.synthetic_files):
   the six classes, the Fock blocks, the orbital energies, E0, E1, ESCF, and
   the SHA-256 of the MDCINT file the reference read.
+* mccres_<name>.bin / .npz: the MCCRES restart file relccsd's CCENER wrote
+  for a ccsd_ / symccsd_ fixture run (PUTAMPT + CCDUMP), with the T1 / T2,
+  IMSTAT and energy records GETAMPT reads back; the generator also checks
+  that relccsd's GETAMPT reads a file written by rcc_mccres_write exactly.
 * lambda_<name>.npz: converged CCSD energies of the reference at f and at
   f +- h V for a seeded Hermitian one-electron perturbation V (stored), the
   finite-field gradient d Ec / d lam the CPU path gives (relccsd's own
@@ -141,6 +145,40 @@ def sort_case(name, nkr, nocc, seed, keep):
     print(name, r["E0"], r["E1"], r["ESCF"])
 
 
+MCCRES_CASES = [("ccsd", "cfg1_6o40v"), ("symccsd", "s4_5o12v")]
+
+
+def mccres_case(kind, name):
+    import shutil
+    import tempfile
+    from molfdir_amd.relccsd import mccres_write
+    z = np.load(os.path.join(HERE, f"{kind}_{name}.npz"), allow_pickle=False)
+    no, nv, seed, maxit, maxdim, ntol = (int(x) for x in z["params"])
+    sym = (z["sym_no"].tolist(), z["sym_nv"].tolist()) if "sym_no" in z.files else None
+    s = synth.make_system(no, nv, seed=seed, scale=float(z["scale"]),
+                          fock_perturb=float(z["fock_perturb"]), sym=sym)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "MCCRES")
+        r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol, keep_mccres=path)
+        back = refrun.read_restart(s, path)
+        assert np.array_equal(back["T1"], r["T1"]) and np.array_equal(back["T2"], r["T2"])
+        shutil.copy(path, os.path.join(HERE, f"mccres_{name}.bin"))
+        # relccsd's GETAMPT on a file written by this library
+        rng = np.random.default_rng(5)
+        a1 = rng.normal(size=r["T1"].size) + 1j * rng.normal(size=r["T1"].size)
+        a2 = rng.normal(size=r["T2"].size) + 1j * rng.normal(size=r["T2"].size)
+        mine = os.path.join(d, "MCCRES.mine")
+        mccres_write(mine, a1, a2, imstat=back["IMSTAT"], energies=back["energies"])
+        chk = refrun.read_restart(s, mine)
+        assert np.array_equal(chk["T1"], a1) and np.array_equal(chk["T2"], a2)
+        assert np.array_equal(chk["IMSTAT"], back["IMSTAT"])
+        assert np.array_equal(chk["energies"], back["energies"])
+    np.savez_compressed(os.path.join(HERE, f"mccres_{name}.npz"), source=f"{kind}_{name}",
+                        T1=back["T1"], T2=back["T2"], IMSTAT=back["IMSTAT"],
+                        energies=back["energies"])
+    print(name, back["IMSTAT"][:6], back["energies"][:4])
+
+
 def symmetric_amplitudes(lay, seed):
     """Random T1/T2 with the symmetry of the layout, returned in C1 layout."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -199,6 +237,10 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--mccres-only" in sys.argv:
+        for case in MCCRES_CASES:
+            mccres_case(*case)
+        return
     if "--sort-only" in sys.argv:
         for case in SORT_CASES:
             sort_case(*case)

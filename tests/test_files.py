@@ -1,0 +1,59 @@
+"""The MCCRES restart file (PUTAMPT / GETAMPT ccgetv.F:577-709, CCDUMP
+ccdriv.F:2767-2816) through the library's host functions, against files
+relccsd's CCENER wrote (tests/golden/mccres_*.bin, C1 and 4 irreps).
+
+* Reading a reference file gives exactly the amplitudes, IMSTAT and energy
+  records relccsd's own GETAMPT / CCDUMP records hold.
+* Writing the same contents gives the same bytes wherever relccsd defines
+  them (the status and energy prefixes, the T1 and T2 blocks); relccsd's
+  GETAMPT was run on files of this writer when the fixtures were made.
+* A file too short for the layout is refused.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from molfdir_amd.relccsd import RccError, mccres_read, mccres_write
+
+from _golden import GOLDEN, load, names
+
+REC = 8192
+
+
+def regions(raw, nt1, nt2):
+    nrec1 = nt1 * 2 // 1024 + 1
+    t1 = raw[3 * REC:3 * REC + 16 * nt1]
+    t2 = raw[(3 + nrec1) * REC:(3 + nrec1) * REC + 16 * nt2]
+    return raw[REC:REC + 120], raw[2 * REC:2 * REC + 56], t1, t2
+
+
+@pytest.mark.parametrize("name", names("mccres"))
+def test_reader_gives_getampt_records(name):
+    g = load("mccres", name)
+    t1, t2, st, en = mccres_read(os.path.join(GOLDEN, f"mccres_{name}.bin"), g["T1"].size,
+                                 g["T2"].size)
+    assert np.array_equal(t1, g["T1"]) and np.array_equal(t2, g["T2"])
+    assert np.array_equal(st, g["IMSTAT"]) and np.array_equal(en, g["energies"])
+    assert st[4] == 3          # IMSTAT(5): CCSD done
+
+
+@pytest.mark.parametrize("name", names("mccres"))
+def test_writer_reproduces_reference_bytes(name, tmp_path):
+    g = load("mccres", name)
+    ref = open(os.path.join(GOLDEN, f"mccres_{name}.bin"), "rb").read()
+    out = tmp_path / "MCCRES"
+    mccres_write(out, g["T1"], g["T2"], imstat=g["IMSTAT"], energies=g["energies"])
+    mine = out.read_bytes()
+    assert len(mine) == len(ref)
+    for a, b in zip(regions(mine, g["T1"].size, g["T2"].size),
+                    regions(ref, g["T1"].size, g["T2"].size)):
+        assert a == b
+
+
+def test_short_file_is_refused(tmp_path):
+    g = load("mccres", "s4_5o12v")
+    p = tmp_path / "short"
+    p.write_bytes(open(os.path.join(GOLDEN, "mccres_s4_5o12v.bin"), "rb").read()[:3 * REC])
+    with pytest.raises(RccError):
+        mccres_read(p, g["T1"].size, g["T2"].size)

@@ -8,7 +8,7 @@ import pytest
 
 from oracle import refrun
 
-from _golden import load, system_for
+from _golden import load, names, system_for
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +40,14 @@ def test_fortran_driver_matches_reference(kind, name, tmp_path):
     assert abs(raw[0] - float(g["emp2"])) < 1e-10
     assert np.abs(z[:nt1] - g["T1"]).max() < 1e-9
     assert np.abs(z[nt1:] - g["T2"]).max() < 1e-9
+    # the restart file the driver leaves, as relccsd's CCENER leaves MCCRES
+    from molfdir_amd.relccsd import mccres_read
+    t1, t2, st, en = mccres_read(tmp_path / "MCCRES", nt1, g["T2"].size)
+    assert np.array_equal(t1, z[:nt1]) and np.array_equal(t2, z[nt1:])
+    if name in names("mccres"):
+        m = load("mccres", name)
+        assert np.array_equal(st, m["IMSTAT"])
+        assert np.abs(en - m["energies"]).max() < 1e-10
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")

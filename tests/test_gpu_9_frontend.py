@@ -93,3 +93,25 @@ def test_fortran_driver_sorts_molfdir_files(tmp_path):
         else:
             assert np.abs(x - g[k]).max() < 1e-14, k
     assert np.abs(raw[pos:] - g["eps"]).max() < 1e-14
+
+
+@pytest.mark.parametrize("name", ["cfg1_6o40v", "s4_5o12v"])
+def test_handle_restart_file_round_trip(name, tmp_path):
+    # the handle reads relccsd's MCCRES into T and writes it back unchanged
+    import os
+    from _golden import GOLDEN
+    m = load("mccres", name)
+    kind = "ccsd" if name.startswith("cfg") else "symccsd"
+    g = load(kind, name)
+    no, nv = int(g["params"][0]), int(g["params"][1])
+    sym = (g["sym_no"].tolist(), g["sym_nv"].tolist()) if "sym_no" in g else None
+    r = RelCCSD(no, nv, sym=sym)
+    st, en = r.read_mccres(os.path.join(GOLDEN, f"mccres_{name}.bin"))
+    t1, t2 = r.get_amplitudes()
+    assert np.array_equal(t1, m["T1"]) and np.array_equal(t2, m["T2"])
+    assert np.array_equal(st, m["IMSTAT"]) and np.array_equal(en, m["energies"])
+    out = tmp_path / "MCCRES"
+    r.write_mccres(out, imstat=st, energies=en)
+    from molfdir_amd.relccsd import mccres_read
+    a1, a2, st2, en2 = mccres_read(out, m["T1"].size, m["T2"].size)
+    assert np.array_equal(a1, m["T1"]) and np.array_equal(a2, m["T2"])
