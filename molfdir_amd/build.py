@@ -13,7 +13,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librelccsd_hip.so")
 SOURCES = ["rcc_zgemm.hip", "rcc_kernels.hip", "rcc_split.hip", "rcc_split_i8.hip",
-           "rcc_triples.hip", "rcc_frontend.hip", "rcc_solver.hip"]
+           "rcc_split_i8w.hip", "rcc_triples.hip", "rcc_frontend.hip", "rcc_solver.hip"]
+# sources compiled without the VGPR form of the MFMA accumulators (none now)
+NO_VGPR_FORM = set()
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RCC_OFFLOAD_ARCH", "gfx950")
 
@@ -48,7 +50,10 @@ def build(verbose: bool = False) -> str:
     for s in SOURCES:
         src = os.path.join(CSRC, s)
         obj = os.path.join(CSRC, s.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+        flags = FLAGS
+        if s in NO_VGPR_FORM:
+            flags = [f for f in FLAGS if f not in ("-mllvm", "-amdgpu-mfma-vgpr-form")]
+        cmd = [HIPCC, *flags, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

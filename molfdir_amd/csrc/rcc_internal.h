@@ -294,5 +294,8 @@ int zgemm_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc, do
              double2 beta, hipStream_t st);
 // shapes whose tile height the INT8 autotuner has picked so far
 int64_t i8_tuned_shapes();
+// the 128 x 128-tile form of zgemm_i8 (rcc_split_i8w.hip), four digits
+int launch_i8_wide(int S, int gm, const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
+                   double2 alpha, double2 beta, hipStream_t st);
 
 }  // namespace rcc

@@ -423,6 +423,7 @@ static void launch_i8(const I8Operand& A, const I8Operand& B, double2* C, int64_
 // multiple of every 32 WM offered.
 static int launch_wm(int S, int wm, int gm, const I8Operand& A, const I8Operand& B,
                      double2* C, int64_t ldc, double2 alpha, double2 beta, hipStream_t st) {
+    if (wm == 8) return launch_i8_wide(S, gm, A, B, C, ldc, alpha, beta, st);
     if (S == 4) {
         if (wm == 2) launch_i8<4, 2>(A, B, C, ldc, alpha, beta, gm, st);
         else if (wm == 3) launch_i8<4, 3>(A, B, C, ldc, alpha, beta, gm, st);
@@ -437,8 +438,9 @@ static int launch_wm(int S, int wm, int gm, const I8Operand& A, const I8Operand&
 }
 
 // Tile autotuner: the first product of a shape (digits, m, n, k steps) met
-// outside graph capture times every tile height WM (2, 3, 4 for 4 digits; 2,
-// 3 for 5) with tile-row bands gm of 1, 4 and 8, in place with alpha = 0,
+// outside graph capture times every tile height WM (2, 3, 4 and the 128 x 128
+// tile of rcc_split_i8w.hip, "8", for 4 digits; 2, 3 for 5) with tile-row
+// bands gm of 1, 4 and 8, in place with alpha = 0,
 // beta = 1 (C unchanged: the first flush writes 1 C + 0, later flushes add
 // 0), and keeps the fastest for the process.  RCC_I8_WM fixes the height
 // (band 1); with the autotuner off (RCC_AUTOTUNE=0, zgemm_autotune(0)) it is
@@ -457,7 +459,7 @@ static I8Choice i8_choose(const I8Operand& A, const I8Operand& B, double2* C, in
     const char* e = getenv("RCC_I8_WM");
     if (e) {
         const int w = atoi(e);
-        return (w == 2 || w == 3 || (w == 4 && S == 4)) ? I8Choice{w, 1} : dflt;
+        return (w == 2 || w == 3 || ((w == 4 || w == 8) && S == 4)) ? I8Choice{w, 1} : dflt;
     }
     if (!zgemm_autotune_enabled()) return dflt;
     std::lock_guard<std::mutex> lk(i8_tune_mu);
@@ -471,9 +473,9 @@ static I8Choice i8_choose(const I8Operand& A, const I8Operand& B, double2* C, in
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return dflt;
     I8Choice best = dflt;
     float best_ms = 1e30f;
-    const int wms4[] = {2, 3, 4}, wms5[] = {2, 3};
+    const int wms4[] = {2, 3, 4, 8}, wms5[] = {2, 3};
     const int* wms = S == 4 ? wms4 : wms5;
-    const int nw = S == 4 ? 3 : 2;
+    const int nw = S == 4 ? 4 : 2;
     const int gms[] = {1, 4, 8};
     for (int c = 0; c < nw; ++c)
         for (int g : gms) {

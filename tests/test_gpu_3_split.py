@@ -247,3 +247,19 @@ def test_split_i8_trajectory_matches_fp64(name, digits, tol):
         t1, t2 = r.get_amplitudes()
         assert np.abs(t1 - g["T1"]).max() < tol
         assert np.abs(t2 - g["T2"]).max() < tol
+
+
+@pytest.mark.parametrize("m,n,k", [(200, 300, 700), (130, 129, 8300), (64, 64, 64), (300, 140, 1000)])
+def test_i8_tile_shapes_give_identical_results(m, n, k, monkeypatch):
+    # every tile shape (32 WM x 64 for WM = 2, 3, 4, and the 128 x 128 tile of
+    # 8 waves, "8") sums the same products in the same order: bit-identical C
+    rng = np.random.default_rng(m + n + k)
+    A = rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+    C = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
+    outs = []
+    for wm in ("2", "3", "4", "8"):
+        monkeypatch.setenv("RCC_I8_WM", wm)
+        outs.append(i8_gemm(4, A, B, C, 0.7 - 0.2j, -0.3 + 0.5j))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
