@@ -41,8 +41,8 @@ enum {
     RCC_ARR_HO = 0, RCC_ARR_HV = 1, RCC_ARR_HOV = 2, RCC_ARR_GO = 3, RCC_ARR_GV = 4,
     RCC_ARR_AZ = 5, RCC_ARR_H = 6, RCC_ARR_S1 = 7, RCC_ARR_S2 = 8, RCC_ARR_T1 = 9,
     RCC_ARR_T2 = 10, RCC_ARR_TAU = 11,
-    /* the Fock blocks and the six classes as the handle holds them (C1
-     * handles; for checking the integral front end) */
+    /* the Fock blocks and the six classes as the handle holds them (for
+     * checking the integral front end; symmetry-blocked layout for nrep > 1) */
     RCC_ARR_FOO = 12, RCC_ARR_FVO = 13, RCC_ARR_FVV = 14, RCC_ARR_OOOO = 15,
     RCC_ARR_VOOO = 16, RCC_ARR_VVOO = 17, RCC_ARR_VOVO = 18, RCC_ARR_VOVV = 19,
     RCC_ARR_VVVV = 20
@@ -243,7 +243,8 @@ int rcc_last_lambda_ms(rcc_t* h, double* ms);
  * must match.  e0e1 (may be null, 3 doubles): E0, E1 and ECORE of RDINTS's
  * summary (ESCF = E0 + E1 + ECORE).  When two images land on one element,
  * the later one in file order is kept, as relccsd's sequential stores do.
- * C1 handles and files (NREP = 1). */
+ * For a symmetry-blocked handle the files' abelian irreps (MRCONEE records
+ * 3-5) must give the handle's occupied / virtual counts per irrep. */
 int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
                      int nnelec, double* e0e1);
 /* The expansion and sort alone, from a canonical list already in device

@@ -2590,17 +2590,27 @@ int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
             const int c = which - RCC_ARR_OOOO;
             src = h->cls[c];
             len = h->cls_rows[c] * h->cls_cols[c];
-            if (h->sym.nrep > 1) {
-                h->err = "rcc_get_array: classes and Fock blocks are read back from C1 handles";
-                return -1;
-            }
             break;
         }
         default: h->err = "rcc_get_array: unknown array"; return -1;
     }
     if (which >= RCC_ARR_FOO && h->sym.nrep > 1) {
-        h->err = "rcc_get_array: classes and Fock blocks are read back from C1 handles";
-        return -1;
+        // symmetry-blocked layout; VOVV and VVVV from their block copies
+        const int lay = which == RCC_ARR_FOO ? RCC_LAY_FOO
+                        : which == RCC_ARR_FVO ? RCC_LAY_T1
+                        : which == RCC_ARR_FVV ? RCC_LAY_FVV : which - RCC_ARR_OOOO;
+        const int64_t need = lay_total(h, lay);
+        if (n != need) {
+            h->err = "rcc_get_array: size mismatch (expected " + std::to_string(need) + ")";
+            return -1;
+        }
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        if (which == RCC_ARR_VOVV || which == RCC_ARR_VVVV) {
+            const double2* blk = which == RCC_ARR_VVVV ? h->VVVV_s : h->VOVV_s;
+            HIPCHK(h, hipMemcpy(out, blk, (size_t)need * 16, hipMemcpyDeviceToHost));
+            return 0;
+        }
+        return c1_to_sym_host(h, lay, src, out);
     }
     if (h->sym.nrep > 1) {
         int lay = RCC_LAY_T2;
@@ -2828,34 +2838,50 @@ static int molfdir_positions(rcc_t* h, const MolfdirOne& one, const int* nelec, 
                  std::to_string(one.nsymrp) + " irreps in MRCONEE";
         return -1;
     }
-    if (one.nrep != h->sym.nrep || h->sym.nrep != 1) {
-        h->err = "rcc_read_molfdir: C1 handles and files only (NREP = 1)";
+    const SymInfo& S = h->sym;
+    if (one.nrep != S.nrep) {
+        h->err = "rcc_read_molfdir: MRCONEE has " + std::to_string(one.nrep) +
+                 " abelian irreps, the handle " + std::to_string(S.nrep);
         return -1;
     }
-    std::vector<int> seen(one.nsymrp + 1, 0);
-    int no = 0, nv = 0;
-    spos.assign(one.nsp, 0);
-    occ_sp.clear();
-    vir_sp.clear();
+    const int nrep = S.nrep;
+    std::vector<int> seen(one.nsymrp + 1, 0), no_r(nrep, 0), nv_r(nrep, 0);
+    std::vector<char> occ(one.nsp, 0);
     for (int s = 0; s < one.nsp; ++s) {
-        const int g = one.irrep_higher[s];
-        if (g < 1 || g > one.nsymrp) {
+        const int g = one.irrep_higher[s], a = one.irrep[s];
+        if (g < 1 || g > one.nsymrp || a < 1 || a > nrep) {
             h->err = "rcc_read_molfdir: spinor irrep out of range in MRCONEE";
             return -1;
         }
-        if (++seen[g] <= nelec[g - 1]) {
-            spos[s] = no++;
-            occ_sp.push_back(s);
-        } else {
-            spos[s] = -(nv++) - 1;
-            vir_sp.push_back(s);
-        }
+        occ[s] = ++seen[g] <= nelec[g - 1];
+        if (occ[s]) ++no_r[a - 1];
+        else ++nv_r[a - 1];
     }
-    if (no != h->no || nv != h->nv) {
-        h->err = "rcc_read_molfdir: the files give " + std::to_string(no) + " occupied / " +
-                 std::to_string(nv) + " virtual spinors, the handle has " +
-                 std::to_string(h->no) + " / " + std::to_string(h->nv);
-        return -1;
+    for (int r = 0; r < nrep; ++r)
+        if (no_r[r] != S.no[r] || nv_r[r] != S.nv[r]) {
+            h->err = "rcc_read_molfdir: irrep " + std::to_string(r + 1) + " has " +
+                     std::to_string(no_r[r]) + " occupied / " + std::to_string(nv_r[r]) +
+                     " virtual spinors in the files, the handle " + std::to_string(S.no[r]) +
+                     " / " + std::to_string(S.nv[r]);
+            return -1;
+        }
+    // C1 working position: occupied irrep-major then virtual irrep-major, file
+    // order inside an irrep (SYMTAB's IREPSPI, ccdriv.F:832-858)
+    std::vector<int> fo(nrep, 0), fv(nrep, 0);
+    spos.assign(one.nsp, 0);
+    occ_sp.assign(h->no, 0);
+    vir_sp.assign(h->nv, 0);
+    for (int s = 0; s < one.nsp; ++s) {
+        const int r = one.irrep[s] - 1;
+        if (occ[s]) {
+            const int p = S.io[r] + fo[r]++;
+            spos[s] = p;
+            occ_sp[p] = s;
+        } else {
+            const int p = S.iv[r] + fv[r]++;
+            spos[s] = -p - 1;
+            vir_sp[p] = s;
+        }
     }
     return 0;
 }
@@ -2882,10 +2908,24 @@ static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int
     HIPCHK(h, hipMalloc((void**)&d_kr, krt.size() * 4));
     HIPCHK(h, hipMemcpy(d_spos, spos.data(), spos.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(d_kr, krt.data(), krt.size() * 4, hipMemcpyHostToDevice));
-    const int rc = sort_integrals_device(h->no, h->nv, h->cls, d_spos, nsp, d_kr, nkr, d_idx,
-                                         d_val, m, h->st, ms, h->err);
+    // symmetric handles keep VVVV only in its irrep blocks: sort it into a
+    // C1-sized scratch array and gather the blocks from there
+    double2* cls[6];
+    for (int c = 0; c < 6; ++c) cls[c] = h->cls[c];
+    double2* vvvv_c1 = nullptr;
+    if (h->sym.nrep > 1)
+        HIPCHK(h, hipMalloc((void**)&vvvv_c1, (size_t)h->nvvt * h->nvvt * sizeof(double2) + 16));
+    if (vvvv_c1) cls[RCC_VVVV] = vvvv_c1;
+    const int rc = sort_integrals_device(h->no, h->nv, cls, d_spos, nsp, d_kr, nkr, d_idx, d_val,
+                                         m, h->st, ms, h->err);
+    if (!rc && h->sym.nrep > 1) {
+        c1_to_sym_dev(h, RCC_VOVV, h->cls[RCC_VOVV], h->VOVV_s);
+        c1_to_sym_dev(h, RCC_VVVV, vvvv_c1, h->VVVV_s);
+        (void)hipStreamSynchronize(h->st);
+    }
     (void)hipFree(d_spos);
     (void)hipFree(d_kr);
+    if (vvvv_c1) (void)hipFree(vvvv_c1);
     if (rc) return -1;
     h->derived_ok = false;
     h->split_const_ok = false;

@@ -127,3 +127,59 @@ def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7):
     write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val)
     return {"nsp": nsp, "nocc": nocc, "kr": kr, "eps": eps, "h1": h1, "ikr": ik, "jkr": jk,
             "kkr": kk, "lkr": lk, "val": val}
+
+
+def abelian_multb(nrep: int) -> np.ndarray:
+    """MULTB(1:2NREP, 1:2NREP, 1) of the abelian double group Z2^k x
+    {fermion, boson} (fermion irreps 1..NREP, boson irreps NREP+1..2NREP):
+    irreps multiply by xor of their 0-based index within each half, and two
+    fermion (or two boson) irreps give a boson irrep."""
+    out = np.zeros((2 * nrep, 2 * nrep), dtype=np.int32)
+    for i in range(2 * nrep):
+        for j in range(2 * nrep):
+            g = (i % nrep) ^ (j % nrep)
+            fermion = (i < nrep) != (j < nrep)
+            out[i, j] = (1 + g) if fermion else (nrep + 1 + g)
+    return out
+
+
+def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float = 0.7):
+    """MRCONEE + MDCINT with the symmetry of Z2^k x {fermion, boson}
+    (len(nkr_irrep) = 2, 4 or 8 fermion irreps).  Both members of a Kramers
+    pair lie in the pair's irrep; spinors are written in a shuffled order, and
+    within each irrep the first nocc_irrep[r] of them (file order) are
+    occupied; the one-electron matrix is irrep-diagonal; the canonical list
+    keeps only integrals whose four Kramers pairs have irreps that multiply
+    to the totally symmetric one (so every image is allowed)."""
+    import os
+    nrep = len(nkr_irrep)
+    rng = np.random.Generator(np.random.PCG64(seed + 104729))
+    pair_irrep = np.concatenate([np.full(n, r, dtype=np.int64) for r, n in enumerate(nkr_irrep)])
+    nkr = pair_irrep.size
+    nsp = 2 * nkr
+    perm = rng.permutation(nsp) + 1
+    kr = perm.reshape(nkr, 2)
+    irreps = np.zeros(nsp, dtype=np.int32)
+    irreps[kr[:, 0] - 1] = pair_irrep + 1
+    irreps[kr[:, 1] - 1] = pair_irrep + 1
+    occ = np.zeros(nsp, dtype=bool)
+    for r in range(nrep):
+        members = np.nonzero(irreps == r + 1)[0]
+        occ[members[:nocc_irrep[r]]] = True
+    eps = np.where(occ, rng.uniform(-2.0, -0.3, nsp), rng.uniform(0.2, 5.0, nsp))
+    h1 = 0.02 * (rng.normal(size=(nsp, nsp)) + 1j * rng.normal(size=(nsp, nsp)))
+    h1 = 0.5 * (h1 + h1.conj().T)
+    h1[irreps[:, None] != irreps[None, :]] = 0.0
+    h1[np.arange(nsp), np.arange(nsp)] = eps
+    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
+    g = pair_irrep
+    ok = (g[np.abs(ik) - 1] ^ g[np.abs(jk) - 1] ^ g[np.abs(kk) - 1] ^ g[np.abs(lk) - 1]) == 0
+    ik, jk, kk, lk, val = ik[ok], jk[ok], kk[ok], lk[ok], val[ok]
+    nelec = np.array(nocc_irrep, dtype=np.int32)
+    write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, irreps, nelec, h1,
+                  multb=abelian_multb(nrep))
+    write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val)
+    no_i = [int(x) for x in nelec]
+    nv_i = [int(np.sum(irreps == r + 1)) - no_i[r] for r in range(nrep)]
+    return {"nsp": nsp, "irreps": irreps, "nelec": nelec, "no": no_i, "nv": nv_i, "kr": kr,
+            "eps": eps, "h1": h1, "ikr": ik, "jkr": jk, "kkr": kk, "lkr": lk, "val": val}

@@ -78,7 +78,9 @@ LAYOUTS = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5,
            "T1": 6, "T2": 7, "FOO": 8, "FVV": 9}
 # layout of each readable array
 ARRAY_LAYOUT = {"HO": "FOO", "GO": "FOO", "HV": "FVV", "GV": "FVV", "HOV": "T1", "S1": "T1",
-                "T1": "T1", "AZ": "OOOO", "H": "VOVO", "S2": "T2", "T2": "T2", "TAU": "T2"}
+                "T1": "T1", "AZ": "OOOO", "H": "VOVO", "S2": "T2", "T2": "T2", "TAU": "T2",
+                "FOO": "FOO", "FVO": "T1", "FVV": "FVV", "OOOO": "OOOO", "VOOO": "VOOO",
+                "VVOO": "VVOO", "VOVO": "VOVO", "VOVV": "VOVV", "VVVV": "VVVV"}
 
 
 def xor_multb(nrep: int) -> np.ndarray:
@@ -91,7 +93,8 @@ def xor_multb(nrep: int) -> np.ndarray:
 def declared_symbols() -> list[str]:
     """Names of every function include/relccsd_hip.h declares."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(rcc_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|int64_t|const char\*)\s+(rcc_\w+)\s*\(",
+                                 text, re.M)))
 
 
 def load():

@@ -1,12 +1,15 @@
-"""CPU restatement of relccsd's integral front end for the C1 double group
-(test infrastructure only): MKINDEX (ccints.F:1421-1580), GET2INTS
+"""CPU restatement of relccsd's integral front end, single reference, C1 and
+abelian double groups (test infrastructure only): MKINDEX (ccints.F:1421-1580), GET2INTS
 (ccints.F:765-1029, the non-Fock-space branch), the class sort of RDINTS
 (ccints.F:33-569) and the Fock build of GETFM / MAKEFM
 (ccints.F:1031-1419).
 
 * Spinors: within each irrep, in input (MRCONEE) order, the first NELEC are
-  occupied, the rest virtual (SYMTAB ccdriv.F:832-858).  C1 positions: the
-  order among the occupied / among the virtual spinors.
+  occupied, the rest virtual (SYMTAB ccdriv.F:832-858).  Positions are those
+  of the C1 working order: occupied irrep-major (file order inside an irrep),
+  then virtual irrep-major -- for one irrep, the order among the occupied /
+  virtual spinors.  With symmetry the classes come out in that C1 order and
+  SymLayout.to_sym gives relccsd's symmetry-blocked layout.
 * MKINDEX: type of a spinor pair (I, J) = 1 oo, 2 vo, 3 ov, 4 vv (I, J
   occupied or virtual); packed pair index of oo / vv pairs for I > J in
   position order, of vo pairs (A fastest) for all.
@@ -23,7 +26,8 @@
   E1 = -1/2 sum_ik Re <ik||ik>, and the orbital energies become Re diag F.
 
 Pinned by tests/test_oracle_rdints.py against the reference's own sort of
-seeded MRCONEE / MDCINT files (tests/golden/sort_*.npz).
+seeded MRCONEE / MDCINT files (tests/golden/sort_*.npz and symsort_*.npz,
+the latter with 2, 4 and 8 irreps).
 """
 from __future__ import annotations
 
@@ -49,8 +53,10 @@ class Spinors:
             seen[r] = seen.get(r, 0) + 1
             self.occ[s] = seen[r] <= int(nelec[r - 1])
         self.pos = np.zeros(self.nsp, dtype=np.int64)
-        self.pos[self.occ] = np.arange(self.occ.sum())
-        self.pos[~self.occ] = np.arange((~self.occ).sum())
+        for flag in (True, False):
+            sel = np.nonzero(self.occ == flag)[0]
+            order = sel[np.argsort(irreps[sel], kind="stable")]   # irrep-major, file order
+            self.pos[order] = np.arange(order.size)
         self.no, self.nv = int(self.occ.sum()), int((~self.occ).sum())
 
     def ptype(self, i, j):
@@ -145,8 +151,8 @@ def fock(sp: Spinors, h1, classes):
     """GETFM + MAKEFM: FOO, FVO, FVV (packed column-major), E0, E1, eps."""
     from .ccsd_oracle import Layout
     no, nv = sp.no, sp.nv
-    occ_s = np.nonzero(sp.occ)[0]
-    vir_s = np.nonzero(~sp.occ)[0]
+    occ_s = np.nonzero(sp.occ)[0][np.argsort(sp.pos[sp.occ])]
+    vir_s = np.nonzero(~sp.occ)[0][np.argsort(sp.pos[~sp.occ])]
     h = np.asarray(h1)
     foo = np.conj(h[np.ix_(occ_s, occ_s)])
     fvo = np.conj(h[np.ix_(vir_s, occ_s)])

@@ -144,18 +144,28 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
     return out
 
 
-def run_sort(files_dir, no, nv, workdir=None, timeout=3600):
-    """Integral sort of files_dir/MRCONEE + files_dir/MDCINT (C1)."""
+def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None):
+    """Integral sort of files_dir/MRCONEE + files_dir/MDCINT.  sym (for
+    NREP > 1): (file-order irrep of every spinor (1-based), occupied count
+    per irrep, nv per irrep); outputs are then in the symmetry-blocked layout."""
     import shutil
-    from oracle.symtab import SymLayout
-    lay = SymLayout([no], [nv])
+    from oracle.symtab import SymLayout, multb_full
+    if sym is None:
+        lay = SymLayout([no], [nv])
+    else:
+        lay = SymLayout(list(sym[1]), list(sym[2]))
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         for fn in ("MRCONEE", "MDCINT"):
             shutil.copy(os.path.join(files_dir, fn), os.path.join(d, fn))
-        hdr = np.array([no, nv, 4, 1, 1, 12, 0, 0], dtype=np.int32)
+        nrep = lay.nrep
+        hdr = np.array([no, nv, 4, 1, 1, 12, nrep if nrep > 1 else 0, 0], dtype=np.int32)
         with open(os.path.join(d, "ccin.bin"), "wb") as f:
             f.write(hdr.tobytes())
             f.write(np.zeros(no + nv).tobytes())
+            if nrep > 1:
+                f.write(np.asarray(sym[0], dtype=np.int32).tobytes())
+                f.write(np.asarray(sym[1], dtype=np.int32).tobytes())
+                f.write(multb_full(nrep).T.astype(np.int32).tobytes())
         p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
         log = p.stdout + p.stderr
         if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):

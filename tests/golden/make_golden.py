@@ -129,6 +129,32 @@ SORT_CASES = [
 ]
 
 
+SYM_SORT_CASES = [
+    # name, Kramers pairs per irrep, occupied spinors per irrep, seed, kept fraction
+    ("s2_k5", (2, 3), (2, 1), 2, 1.0),
+    ("s4_k8", (3, 2, 2, 1), (2, 2, 1, 1), 1, 0.8),
+    ("s8_k12", (2, 1, 1, 2, 1, 1, 2, 2), (2, 0, 1, 1, 2, 0, 1, 1), 3, 0.7),
+]
+
+
+def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
+    import hashlib
+    import tempfile
+    from molfdir_amd import mdcint
+    with tempfile.TemporaryDirectory() as d:
+        gen = mdcint.synthetic_files_sym(d, nkr_irrep, nocc_irrep, seed, keep)
+        digest = hashlib.sha256(open(os.path.join(d, "MDCINT"), "rb").read()).hexdigest()
+        r = refrun.run_sort(d, sum(gen["no"]), sum(gen["nv"]),
+                            sym=(gen["irreps"], gen["no"], gen["nv"]))
+    keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    np.savez_compressed(
+        os.path.join(HERE, f"symsort_{name}.npz"), nkr_irrep=np.array(nkr_irrep),
+        nocc_irrep=np.array(nocc_irrep), seed=seed, keep=keep, mdcint_sha256=digest,
+        sym_no=np.array(gen["no"]), sym_nv=np.array(gen["nv"]),
+        E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
+    print(name, r["E0"], r["E1"], r["ESCF"])
+
+
 def sort_case(name, nkr, nocc, seed, keep):
     import hashlib
     import tempfile
@@ -244,9 +270,13 @@ def main():
     if "--sort-only" in sys.argv:
         for case in SORT_CASES:
             sort_case(*case)
+        for case in SYM_SORT_CASES:
+            sym_sort_case(*case)
         return
     for case in SORT_CASES:
         sort_case(*case)
+    for case in SYM_SORT_CASES:
+        sym_sort_case(*case)
     if "--t3-only" in sys.argv:
         for case in T3_CASES:
             t3_case(*case)

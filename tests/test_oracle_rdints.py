@@ -69,3 +69,31 @@ def test_planted_first_write_wins_is_caught(tmp_path, monkeypatch):
     monkeypatch.setattr(ro, "expand", first_wins)
     cl = ro.sort_classes(sp, kr, gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"], gen["val"])
     assert any(not np.array_equal(cl[c], g[c]) for c in CLASSES)
+
+
+def regenerate_sym(g, d):
+    gen = mdcint.synthetic_files_sym(str(d), g["nkr_irrep"].tolist(), g["nocc_irrep"].tolist(),
+                                     int(g["seed"]), float(g["keep"]))
+    digest = hashlib.sha256(open(os.path.join(str(d), "MDCINT"), "rb").read()).hexdigest()
+    assert digest == str(g["mdcint_sha256"]), "synthetic MDCINT generator changed"
+    return gen
+
+
+@pytest.mark.parametrize("name", names("symsort"))
+def test_oracle_symmetric_sort_matches_reference(name, tmp_path):
+    # classes in the C1 working order (irrep-major spinors), read out through
+    # SYMTAB's block maps, against relccsd's symmetry-blocked sort
+    from oracle.symtab import SymLayout
+    g = load("symsort", name)
+    gen = regenerate_sym(g, tmp_path)
+    sp = ro.Spinors(gen["irreps"], gen["nelec"])
+    kr = ro.kr_table(gen["kr"])
+    cl = ro.sort_classes(sp, kr, gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"], gen["val"])
+    lay = SymLayout(g["sym_no"].tolist(), g["sym_nv"].tolist())
+    for c in CLASSES:
+        assert np.array_equal(lay.to_sym(c, cl[c]), g[c]), c
+    f = ro.fock(sp, gen["h1"], cl)
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(lay.to_sym(k, f[k]) - g[k]).max() < 1e-14, k
+    assert np.abs(f["eps"] - g["eps"]).max() < 1e-14
+    assert abs(f["E0"] - float(g["E0"])) < 1e-13 and abs(f["E1"] - float(g["E1"])) < 1e-13
