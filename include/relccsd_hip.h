@@ -256,6 +256,21 @@ int rcc_sort_ints_device(rcc_t* h, const int* spinor_pos, int nsp, const int* kr
                          const void* d_idx, const void* d_val, int64_t m, double* ms);
 /* Device time (ms) of the last integral sort. */
 double rcc_last_sort_ms(rcc_t* h);
+/* ---- One-electron properties.  rcc_read_property: RDPROP (ccints.F:569-764)
+ * -- operator `name` from an MDPROP file, returned as the (no + nv) x (no + nv)
+ * complex matrix in the handle's spinor order (occupied, then virtual;
+ * column-major), with the spinor classification of MRCONEE + nelec as in
+ * rcc_read_molfdir.  rcc_expectation: EXPVAL (ccgrad.F:261-347) -- level 0,
+ * the reference-determinant value sum_i P(i,i) (CCGRAD's SCF line,
+ * ccdriv.F:2028-2039); level 1, the unrelaxed CCSD value sum_i P(i,i) +
+ * tr(D P) with D the Lambda response density of rcc_density (needs
+ * rcc_lambda first), i.e. d(E_ref + Ec)/d lam for f -> f + lam P at fixed
+ * orbitals.  relccsd's own correlated property path is the MP2 relaxed one
+ * (Z-vector, RPAA ccrpa.F:34); orbital relaxation is not included here.
+ * value: 2 doubles (Re, Im). */
+int rcc_read_property(rcc_t* h, const char* mrconee, const char* mdprop, const char* name,
+                      const int* nelec, int nnelec, double* prop);
+int rcc_expectation(rcc_t* h, const double* prop, int level, double* value);
 /* The handle's orbital energies (no + nv doubles, occupied then virtual). */
 int rcc_get_eps(rcc_t* h, double* eps);
 

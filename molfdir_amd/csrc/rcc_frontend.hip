@@ -178,6 +178,35 @@ int read_mdcint(const char* path, MdcintList& out, std::string& err) {
     return 0;
 }
 
+int read_mdprop(const char* path, const char* name, int nsp, std::vector<double2>& prop,
+                std::string& err) {
+    Reader r;
+    r.path = path;
+    r.f = fopen(path, "rb");
+    if (!r.f) {
+        err = std::string("cannot open ") + path;
+        return -1;
+    }
+    std::string want(name);
+    want.resize(8, ' ');
+    std::vector<unsigned char> b;
+    while (r.next(b, err)) {
+        if (b.size() != 32 || std::memcmp(b.data(), "********", 8) != 0 ||
+            std::memcmp(b.data() + 24, want.data(), 8) != 0)
+            continue;
+        if (!r.next(b, err) || b.size() != (size_t)nsp * nsp * 16) {
+            if (err.empty())
+                err = std::string(path) + ": operator " + name + " is not an NSP x NSP matrix";
+            return -1;
+        }
+        prop.resize((size_t)nsp * nsp);
+        std::memcpy(prop.data(), b.data(), b.size());
+        return 0;
+    }
+    if (err.empty()) err = std::string("property ") + name + " not found on " + path;
+    return -1;
+}
+
 namespace {
 
 struct SortDev {

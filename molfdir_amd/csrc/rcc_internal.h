@@ -101,6 +101,9 @@ struct MdcintList {          // MDCINT: KR table and the canonical integrals
 };
 int read_mrconee(const char* path, MolfdirOne& out, std::string& err);
 int read_mdcint(const char* path, MdcintList& out, std::string& err);
+// RDPROP: the NSP x NSP matrix of operator `name` on an MDPROP file
+int read_mdprop(const char* path, const char* name, int nsp, std::vector<double2>& prop,
+                std::string& err);
 // Expansion + sort of m canonical integrals (device arrays) into the six
 // packed C1 classes cls[0..5] (OOOO, VOOO, VVOO, VOVO, VOVV, VVVV), which
 // are overwritten.  d_spos[s]: C1 position of spinor s (>= 0 occupied,

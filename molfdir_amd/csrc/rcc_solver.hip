@@ -3019,6 +3019,55 @@ int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const in
 }
 
 double rcc_last_sort_ms(rcc_t* h) { return h ? h->last_sort_ms : -1.0; }
+int rcc_read_property(rcc_t* h, const char* mrconee, const char* mdprop, const char* name,
+                      const int* nelec, int nnelec, double* prop) {
+    if (!h || !mrconee || !mdprop || !name || !nelec || !prop) return -1;
+    MolfdirOne one;
+    if (read_mrconee(mrconee, one, h->err)) return -1;
+    std::vector<int> spos, occ_sp, vir_sp;
+    CK(molfdir_positions(h, one, nelec, nnelec, spos, occ_sp, vir_sp));
+    std::vector<double2> p;
+    if (read_mdprop(mdprop, name, one.nsp, p, h->err)) return -1;
+    // the operator in the handle's spinor order: occupied positions, then virtual
+    const int n = h->no + h->nv;
+    std::vector<int> sp(n);
+    for (int i = 0; i < h->no; ++i) sp[i] = occ_sp[i];
+    for (int a = 0; a < h->nv; ++a) sp[h->no + a] = vir_sp[a];
+    double2* out = (double2*)prop;
+    for (int q = 0; q < n; ++q)
+        for (int r = 0; r < n; ++r) out[r + (int64_t)q * n] = p[sp[r] + (int64_t)sp[q] * one.nsp];
+    return 0;
+}
+
+int rcc_expectation(rcc_t* h, const double* prop, int level, double* value) {
+    if (!h || !prop || !value) return -1;
+    if (level != 0 && level != 1) {
+        h->err = "rcc_expectation: level 0 (reference determinant) or 1 (CCSD, unrelaxed)";
+        return -1;
+    }
+    const int n = h->no + h->nv;
+    const double2* P = (const double2*)prop;
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < h->no; ++i) {
+        re += P[i + (int64_t)i * n].x;
+        im += P[i + (int64_t)i * n].y;
+    }
+    if (level == 1) {
+        std::vector<double2> D((size_t)n * n);
+        CK(rcc_density(h, (double*)D.data()));
+        // tr(D P) = sum_pq D(p,q) P(q,p)
+        for (int q = 0; q < n; ++q)
+            for (int p = 0; p < n; ++p) {
+                const double2 d = D[p + (int64_t)q * n], a = P[q + (int64_t)p * n];
+                re += d.x * a.x - d.y * a.y;
+                im += d.x * a.y + d.y * a.x;
+            }
+    }
+    value[0] = re;
+    value[1] = im;
+    return 0;
+}
+
 
 int rcc_get_eps(rcc_t* h, double* eps) {
     if (!h || !eps) return -1;

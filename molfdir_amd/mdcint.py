@@ -12,6 +12,10 @@ MRCONEE (RDSYMI ccdriv.F:2978-2988, GETFM ccints.F:1091-1096)
   5  (higher-group irrep, abelian irrep, orbital energy) per spinor
   6  the one-electron (Fock) matrix, (Re, Im) per element, NSP x NSP column-major
 
+MDPROP (RDPROP ccints.F:569-764): per operator a label record (character*32:
+'********', date (8), storage info (8), name (8)) followed by the operator
+matrix, (Re, Im) per element, NSP x NSP column-major.
+
 MDCINT (RDINTS ccints.F:117-118, GET2INTS ccints.F:819-835)
   header  DATEX (character*10), TIMEX (character*8), NKR, (KR(i), KR(-i), i = 1..NKR)
   records IKR, JKR, NZ, (INDK, INDL) x NZ, (Re, Im) x NZ  -- the integrals
@@ -58,6 +62,16 @@ def write_mrconee(path, eps, irreps, nelec, h1, multb=None, ecore=0.0, breit=Fal
         _record(f, rec5.tobytes())
         h = np.asarray(h1, dtype=np.complex128).reshape(nsp, nsp)
         _record(f, np.ascontiguousarray(h.T).reshape(-1).view(np.float64).tobytes())
+
+
+def write_mdprop(path, operators):
+    """operators: {name (at most 8 characters): NSP x NSP complex matrix}."""
+    with open(path, "wb") as f:
+        for name, mat in operators.items():
+            label = ("********" + "01Jan26 " + "SQUARE  " + name.ljust(8)[:8]).encode()
+            _record(f, label)
+            m = np.asarray(mat, dtype=np.complex128)
+            _record(f, np.ascontiguousarray(m.T).reshape(-1).view(np.float64).tobytes())
 
 
 def write_mdcint(path, kr, ikr, jkr, kkr, lkr, val):

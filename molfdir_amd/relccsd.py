@@ -162,6 +162,9 @@ def load():
     lib.rcc_last_sort_ms.argtypes = [vp]
     lib.rcc_last_sort_ms.restype = ctypes.c_double
     lib.rcc_get_eps.argtypes = [vp, dp]
+    lib.rcc_read_property.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                      ctypes.POINTER(ci), ci, vp]
+    lib.rcc_expectation.argtypes = [vp, vp, ci, dp]
     lib.rcc_t3corr.argtypes = [vp, dp]
     lib.rcc_t3corr_part.argtypes = [vp, i64, i64, dp]
     lib.rcc_t3_triples.argtypes = [vp]
@@ -507,6 +510,23 @@ class RelCCSD:
 
     def last_sort_ms(self):
         return float(self.lib.rcc_last_sort_ms(self.h))
+
+    def read_property(self, mrconee, mdprop, name, nelec):
+        """RDPROP: operator `name` of an MDPROP file as an (no+nv) x (no+nv)
+        complex matrix in the handle's spinor order."""
+        n = self.no + self.nv
+        out = np.empty((n, n), np.complex128, order="F")
+        ne = (ctypes.c_int * len(nelec))(*[int(x) for x in nelec])
+        self._check(self.lib.rcc_read_property(self.h, str(mrconee).encode(), str(mdprop).encode(),
+                                               name.encode(), ne, len(nelec), _ptr(out)))
+        return out
+
+    def expectation(self, prop, level=1):
+        """EXPVAL: level 0 reference determinant, 1 unrelaxed CCSD (Lambda)."""
+        p = np.asfortranarray(prop, dtype=np.complex128)
+        v = (ctypes.c_double * 2)()
+        self._check(self.lib.rcc_expectation(self.h, _ptr(p), int(level), v))
+        return complex(v[0], v[1])
 
     def get_eps(self):
         out = np.empty(self.no + self.nv, np.float64)

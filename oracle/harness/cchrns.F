@@ -20,6 +20,10 @@ C  every intermediate, so each GPU term can be checked on its own.
 C  Mode 3 reads given T1/T2 and calls T3CORR (cceqns.F:1336) with the
 C  arrays CCENER allocates for it (ccdriv.F:1805-1816), writing the
 C  triples corrections ET1, ET2, ET3 (COMMON /ENERG/).
+C  Mode 6 reads operator 'PROPTEST' from an MDPROP file written by the
+C  caller with RDPROP (ccints.F:569) and evaluates its reference-determinant
+C  expectation value with EXPVAL (ccgrad.F:261) on zero densities, as
+C  CCGRAD does for its SCF line (ccdriv.F:2028-2039).
 C  Mode 5 copies the records of MCCRES.in (a restart file written by the
 C  caller) into MCCRES after CCDINI and reads it back with GETAMPT
 C  (ccgetv.F:643-709) and the status / energy records CCDUMP writes;
@@ -57,6 +61,10 @@ C
      &                       HO(:),HV(:),HOV(:),GOO(:),GVV(:),
      &                       BUF1(:),BUF2(:),BUF3(:)
       REAL*8 DUM(4),ECC,RBUF(RSRCLEN)
+      REAL*8, ALLOCATABLE :: PB(:),AVO(:),AOV(:),AOO(:),AVV(:),
+     &                       ZD(:)
+      COMPLEX*16 EPH,CEXPA
+      CHARACTER*8 PNAME
       INTEGER ISZ
       CHARACTER*4 RNAM(32)
       CHARACTER*14 SNAM(16)
@@ -212,6 +220,21 @@ C
          ENDDO
          WRITE (32) V5,V6
          WRITE (32) ESCF
+C
+      ELSEIF (MODE.EQ.6) THEN
+C
+C        RDPROP + EXPVAL (reference determinant) on a caller's MDPROP
+C
+         ALLOCATE (PB(2*NSP*NSP),AVO(2*NSP*NSP),AOV(2*NSP*NSP),
+     &             AOO(2*NFOO),AVV(2*NFVV),ZD(2*NSP*NSP))
+         ZD = 0.0D0
+         PNAME = 'PROPTEST'
+         CALL RDPROP (1,PNAME,NSP,.TRUE.,PB,IREPSPI,EPH,AVO,AOV,
+     &                AOO,AVV)
+         CALL EXPVAL (ZD,ZD,ZD,ZD,AOO,AVV,AVO,AOV,EPH,CEXPA)
+         WRITE (32) (AOO(I),I=1,2*NFOO),(AVV(I),I=1,2*NFVV),
+     &              (AVO(I),I=1,2*NFVO),(AOV(I),I=1,2*NFVO)
+         WRITE (32) DBLE(CEXPA),DIMAG(CEXPA)
 C
       ELSEIF (MODE.EQ.5) THEN
 C

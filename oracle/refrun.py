@@ -211,3 +211,27 @@ def read_restart(sysd, mccres_path, workdir=None, timeout=600):
     return {"T1": z[:nt1].copy(), "T2": z[nt1:].copy(),
             "IMSTAT": raw[2 * (nt1 + nt2):2 * (nt1 + nt2) + 30].astype(np.int64),
             "energies": raw[2 * (nt1 + nt2) + 30:].copy()}
+
+
+def read_property(sysd, mdprop_path, workdir=None, timeout=600):
+    """Mode 6: relccsd's RDPROP of operator 'PROPTEST' (blocks AOO, AVV, AVO,
+    AOV in the reference layout) and EXPVAL on zero densities (the
+    reference-determinant expectation value)."""
+    import shutil
+    lay = layout_of(sysd)
+    nfoo, nfvv, nfvo = lay.size("FOO"), lay.size("FVV"), lay.size("FVO")
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        _write_input(os.path.join(d, "ccin.bin"), sysd, 6, 1, 1, 12)
+        shutil.copy(mdprop_path, os.path.join(d, "MDPROP"))
+        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
+            raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
+        raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+    z = raw[:-2].view(np.complex128)
+    out, pos = {}, 0
+    for name, n in (("AOO", nfoo), ("AVV", nfvv), ("AVO", nfvo), ("AOV", nfvo)):
+        out[name] = z[pos:pos + n].copy()
+        pos += n
+    out["expval"] = complex(raw[-2], raw[-1])
+    return out

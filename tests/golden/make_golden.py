@@ -171,6 +171,31 @@ def sort_case(name, nkr, nocc, seed, keep):
     print(name, r["E0"], r["E1"], r["ESCF"])
 
 
+PROP_CASES = [
+    # name, no, nv, seed, scale, fock_perturb, operator seed
+    ("p_4o10v", 4, 10, 3, 0.05, 0.05, 21),
+    ("p_6o20v", 6, 20, 4, 0.05, 0.03, 22),
+]
+
+
+def prop_case(name, no, nv, seed, scale, fp, pseed):
+    """relccsd's RDPROP + EXPVAL (reference determinant) of a seeded Hermitian
+    operator written as MDPROP 'PROPTEST' (harness mode 6)."""
+    import tempfile
+    from molfdir_amd import mdcint
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+    P = synth.hermitian_perturbation(no + nv, pseed, 0.1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "MDPROP")
+        mdcint.write_mdprop(path, {"OTHER": 2.0 * P, "PROPTEST": P})
+        r = refrun.read_property(s, path)
+    np.savez_compressed(os.path.join(HERE, f"prop_{name}.npz"),
+                        params=np.array([no, nv, seed, pseed], dtype=np.int64), scale=scale,
+                        fock_perturb=fp, AOO=r["AOO"], AVV=r["AVV"], AVO=r["AVO"], AOV=r["AOV"],
+                        expval=r["expval"])
+    print(name, r["expval"])
+
+
 MCCRES_CASES = [("ccsd", "cfg1_6o40v"), ("symccsd", "s4_5o12v")]
 
 
@@ -263,6 +288,10 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--prop-only" in sys.argv:
+        for case in PROP_CASES:
+            prop_case(*case)
+        return
     if "--mccres-only" in sys.argv:
         for case in MCCRES_CASES:
             mccres_case(*case)

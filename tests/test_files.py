@@ -8,6 +8,8 @@ relccsd's CCENER wrote (tests/golden/mccres_*.bin, C1 and 4 irreps).
   them (the status and energy prefixes, the T1 and T2 blocks); relccsd's
   GETAMPT was run on files of this writer when the fixtures were made.
 * A file too short for the layout is refused.
+* The MDPROP writer's files are read by relccsd's RDPROP as the operator
+  written (tests/golden/prop_*.npz).
 """
 import os
 
@@ -57,3 +59,20 @@ def test_short_file_is_refused(tmp_path):
     p.write_bytes(open(os.path.join(GOLDEN, "mccres_s4_5o12v.bin"), "rb").read()[:3 * REC])
     with pytest.raises(RccError):
         mccres_read(p, g["T1"].size, g["T2"].size)
+
+
+@pytest.mark.parametrize("name", names("prop"))
+def test_reference_rdprop_reads_the_mdprop_writer(name):
+    """relccsd's RDPROP (harness mode 6) read 'PROPTEST' -- the second
+    operator of an MDPROP file written by molfdir_amd.mdcint.write_mdprop --
+    into AOO(i,j) = P(i,j), AVV(a,b) = P(a,b), AVO(a,i) = P(a,i) and
+    AOV(a,i) = P(i,a), and EXPVAL on zero densities gave sum_i P(i,i)."""
+    from molfdir_amd import synth
+    g = load("prop", name)
+    no, nv, _, pseed = (int(x) for x in g["params"])
+    P = synth.hermitian_perturbation(no + nv, pseed, 0.1)
+    assert np.array_equal(g["AOO"], P[:no, :no].T.reshape(-1))
+    assert np.array_equal(g["AVV"], P[no:, no:].T.reshape(-1))
+    assert np.array_equal(g["AVO"], P[no:, :no].T.reshape(-1))
+    assert np.array_equal(g["AOV"], P[:no, no:].reshape(-1))
+    assert abs(complex(g["expval"]) - np.trace(P[:no, :no])) < 1e-14
