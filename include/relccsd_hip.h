@@ -45,7 +45,13 @@ enum {
      * checking the integral front end; symmetry-blocked layout for nrep > 1) */
     RCC_ARR_FOO = 12, RCC_ARR_FVO = 13, RCC_ARR_FVV = 14, RCC_ARR_OOOO = 15,
     RCC_ARR_VOOO = 16, RCC_ARR_VVOO = 17, RCC_ARR_VOVO = 18, RCC_ARR_VOVV = 19,
-    RCC_ARR_VVVV = 20
+    RCC_ARR_VVVV = 20,
+    /* after rcc_mp2_density (C1): the density blocks DOO (o x o) and DVV
+     * (v x v), column-major, as CCDENSC(1) leaves them; the X intermediate as
+     * CCDENSZ stores it (-X, a VO vector, index i*nv + a); the Z-vector DVO
+     * and DOV (VO vectors) as RPAA leaves them; the MP2 amplitudes */
+    RCC_ARR_DOO = 21, RCC_ARR_DVV = 22, RCC_ARR_XVO = 23, RCC_ARR_DVO = 24,
+    RCC_ARR_DOV = 25, RCC_ARR_MP2T1 = 26, RCC_ARR_MP2T2 = 27
 };
 
 /* Symmetry description, as RDSYMI / SYMTAB hold it (ccdriv.F:2934, :746).
@@ -265,12 +271,24 @@ double rcc_last_sort_ms(rcc_t* h);
  * ccdriv.F:2028-2039); level 1, the unrelaxed CCSD value sum_i P(i,i) +
  * tr(D P) with D the Lambda response density of rcc_density (needs
  * rcc_lambda first), i.e. d(E_ref + Ec)/d lam for f -> f + lam P at fixed
- * orbitals.  relccsd's own correlated property path is the MP2 relaxed one
- * (Z-vector, RPAA ccrpa.F:34); orbital relaxation is not included here.
+ * orbitals (no orbital relaxation); level 2, the MP2 relaxed value of
+ * relccsd's CCFOPR (ccdriv.F:2080-2087), after rcc_mp2_density.
  * value: 2 doubles (Re, Im). */
 int rcc_read_property(rcc_t* h, const char* mrconee, const char* mdprop, const char* name,
                       const int* nelec, int nnelec, double* prop);
 int rcc_expectation(rcc_t* h, const double* prop, int level, double* value);
+/* ---- MP2 relaxed density: the DOMP2G block of CCFOPR (ccdriv.F:2058-2090),
+ * replacing MP2EQN with EQNS = 'LAMBDA' (cceqns.F:33), CCDENSC(1)
+ * (ccgrad.F:34) and CCDENSZ(1) (ccgrad.F:141) with its Z-vector solve RPAA
+ * (ccrpa.F:34) at zero frequency.  C1 handles, on the handle's integrals and
+ * orbital energies (its amplitudes are not touched).  maxit / maxdim / ntol
+ * are RPAA's MXITGR / MXDIMGR / NTOLGR (converged when the DIIS RMS drops
+ * below 10^-ntol; maxdim <= 16).  nit, rms (may be null): iterations taken,
+ * last RMS.  Afterwards rcc_expectation level 2 gives the MP2 relaxed value
+ * of EXPVAL, and rcc_get_array RCC_ARR_DOO .. RCC_ARR_MP2T2 the pieces. */
+int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double* rms);
+/* Device time (ms) of the last rcc_mp2_density. */
+double rcc_last_mp2d_ms(rcc_t* h);
 /* The handle's orbital energies (no + nv doubles, occupied then virtual). */
 int rcc_get_eps(rcc_t* h, double* eps);
 

@@ -170,6 +170,18 @@ struct rcc_handle {
     T3State* t3 = nullptr;
     T3Timing t3_tm;
     double last_sort_ms = 0.0;     // last integral sort (rcc_read_molfdir / rcc_sort_ints_device)
+    // MP2 relaxed density (rcc_mp2_density): the MP2 amplitudes, the density
+    // blocks DOO (o x o), DVV (v x v), the X intermediate as CCDENSZ stores it
+    // (-X), DVO and DOV (VO vectors), the RPA preconditioner and the Z-vector
+    // work (DIIS history, partial sums)
+    double2* mp2t = nullptr;
+    double2* mp2d = nullptr;
+    double* zdg = nullptr;
+    double2* zwork = nullptr;
+    int64_t zwork_n = 0;
+    bool mp2d_ok = false;
+    int mp2_nit = 0;
+    double mp2_rms = 0.0, last_mp2d_ms = 0.0;
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -2592,6 +2604,31 @@ int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
             len = h->cls_rows[c] * h->cls_cols[c];
             break;
         }
+        case RCC_ARR_DOO: case RCC_ARR_DVV: case RCC_ARR_XVO: case RCC_ARR_DVO:
+        case RCC_ARR_DOV: case RCC_ARR_MP2T1: case RCC_ARR_MP2T2: {
+            if (!h->mp2d_ok) {
+                h->err = "rcc_get_array: no MP2 density (rcc_mp2_density first)";
+                return -1;
+            }
+            const double2* d = h->mp2d;
+            const int64_t oo = (int64_t)o * o, vv = (int64_t)v * v, nvo = h->nvo;
+            switch (which) {
+                case RCC_ARR_DOO: src = d; len = oo; break;
+                case RCC_ARR_DVV: src = d + oo; len = vv; break;
+                case RCC_ARR_XVO: src = d + oo + vv; len = nvo; break;
+                case RCC_ARR_DVO: src = d + oo + vv + nvo; len = nvo; break;
+                case RCC_ARR_DOV: src = d + oo + vv + 2 * nvo; len = nvo; break;
+                case RCC_ARR_MP2T1: src = h->mp2t; len = h->nt1; break;
+                default: src = h->mp2t + h->nt1; len = h->nt2; break;
+            }
+            if (n != len) {
+                h->err = "rcc_get_array: size mismatch (expected " + std::to_string(len) + ")";
+                return -1;
+            }
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            HIPCHK(h, hipMemcpy(out, src, (size_t)len * 16, hipMemcpyDeviceToHost));
+            return 0;
+        }
         default: h->err = "rcc_get_array: unknown array"; return -1;
     }
     if (which >= RCC_ARR_FOO && h->sym.nrep > 1) {
@@ -3041,8 +3078,13 @@ int rcc_read_property(rcc_t* h, const char* mrconee, const char* mdprop, const c
 
 int rcc_expectation(rcc_t* h, const double* prop, int level, double* value) {
     if (!h || !prop || !value) return -1;
-    if (level != 0 && level != 1) {
-        h->err = "rcc_expectation: level 0 (reference determinant) or 1 (CCSD, unrelaxed)";
+    if (level < 0 || level > 2) {
+        h->err = "rcc_expectation: level 0 (reference determinant), 1 (CCSD, unrelaxed) or 2 "
+                 "(MP2, relaxed)";
+        return -1;
+    }
+    if (level == 2 && !h->mp2d_ok) {
+        h->err = "rcc_expectation: level 2 needs rcc_mp2_density first";
         return -1;
     }
     const int n = h->no + h->nv;
@@ -3063,11 +3105,230 @@ int rcc_expectation(rcc_t* h, const double* prop, int level, double* value) {
                 im += d.x * a.y + d.y * a.x;
             }
     }
+    if (level == 2) {
+        // EXPVAL (ccgrad.F:300-333): unconjugated dot products of the operator
+        // blocks EOV(A,I) = P(I,A), EVO(A,I) = P(A,I), EOO, EVV with DVO, DOV,
+        // DOO, DVV
+        const int o = h->no, v = h->nv;
+        const int64_t nvo = h->nvo;
+        std::vector<double2> d((size_t)o * o + (size_t)v * v + 3 * nvo);
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        HIPCHK(h, hipMemcpy(d.data(), h->mp2d, d.size() * 16, hipMemcpyDeviceToHost));
+        const double2* doo = d.data();
+        const double2* dvv = doo + (size_t)o * o;
+        const double2* dvo = dvv + (size_t)v * v + nvo;
+        const double2* dov = dvo + nvo;
+        auto acc = [&](double2 p, double2 q) {
+            re += p.x * q.x - p.y * q.y;
+            im += p.x * q.y + p.y * q.x;
+        };
+        for (int i = 0; i < o; ++i)
+            for (int a = 0; a < v; ++a) {
+                const size_t ai = a + (size_t)v * i;
+                acc(P[i + (int64_t)(o + a) * n], dvo[ai]);
+                acc(P[(o + a) + (int64_t)i * n], dov[ai]);
+            }
+        for (int j = 0; j < o; ++j)
+            for (int i = 0; i < o; ++i) acc(P[i + (int64_t)j * n], doo[i + (size_t)o * j]);
+        for (int b = 0; b < v; ++b)
+            for (int a = 0; a < v; ++a)
+                acc(P[(o + a) + (int64_t)(o + b) * n], dvv[a + (size_t)v * b]);
+    }
     value[0] = re;
     value[1] = im;
     return 0;
 }
 
+
+// ---- MP2 relaxed density: the DOMP2G block of CCFOPR (ccdriv.F:2058-2090) ----
+namespace {
+
+int free_tracked(rcc_t* h, void* p) {
+    if (!p) return 0;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    HIPCHK(h, hipFree(p));
+    return 0;
+}
+
+// RPAA's DIISX step (cceqns.F:1985-2178) on the joint vector (X, Y) with
+// Y = conj X, which every iterate keeps (PRECD divides by a real DG and the
+// extrapolation coefficients are real): with d the error of X, the joint
+// sums are sum d + conj sum d = 2 Re sum d and 2 sum |d|^2, and the joint
+// B entries are Re + Im of <d_k|d> + conj <d_k|d> = 2 Re <d_k|d>.
+struct ZDiis {
+    int md = 8, it = 0, itc = 0;
+    std::vector<double> bb2;
+};
+
+int zdiis_step(rcc_t* h, ZDiis& z, double2** err, double2** amp, const double2* cur, double2* nw,
+               int64_t n, double* rms) {
+    hipStream_t st = h->st;
+    z.it = std::min(z.it + 1, z.md);
+    z.itc = z.itc + 1 > z.md ? 1 : z.itc + 1;
+    const int c = z.itc - 1;
+    double2* d = err[c];
+    k_copy(n, nw, d, st);
+    k_axpby(n, cplx(-1, 0), cur, cplx(1, 0), d, st);
+    k_copy(n, nw, amp[c], st);
+    k_sum_and_norm(d, n, h->red_dst, h->red_part, RED_BLOCKS, st);
+    std::vector<const double2*> xs(err, err + z.it);
+    k_zdotc_multi(xs.data(), z.it, d, n, h->red_dst + 2, h->red_part, RED_BLOCKS, st);
+    HIPCHK(h, hipMemcpyAsync(h->hred, h->red_dst, (2 + z.it) * sizeof(double2),
+                             hipMemcpyDeviceToHost, st));
+    HIPCHK(h, hipStreamSynchronize(st));
+    const double2 sum = h->hred[0];
+    const double sum2 = 2.0 * h->hred[1].x, csum = 2.0 * sum.x;
+    *rms = std::sqrt(std::fabs(sum2 - csum * csum));
+    const int md = z.md, ld = md + 1;
+    auto BB2 = [&](int i, int j) -> double& { return z.bb2[i + (int64_t)j * ld]; };
+    for (int k = 0; k < z.it; ++k) {
+        const double vv = 2.0 * h->hred[2 + k].x;
+        BB2(k, c) = vv;
+        BB2(c, k) = vv;
+    }
+    const int it = z.it;
+    std::vector<double> bb((size_t)ld * (md + 2), 0.0);
+    auto BB = [&](int i, int j) -> double& { return bb[i + (int64_t)j * ld]; };
+    double xm = 0.0;
+    for (int i = 0; i < it; ++i)
+        for (int j = 0; j < it; ++j) {
+            BB(i, j) = BB2(i, j);
+            xm = std::max(xm, std::fabs(BB(i, j)));
+        }
+    if (xm > 0.0)
+        for (int i = 0; i < it; ++i)
+            for (int j = 0; j < it; ++j) BB(i, j) /= xm;
+    for (int i = 0; i < it; ++i) {
+        BB(i, it) = -1.0;
+        BB(it, i) = -1.0;
+    }
+    BB(it, it) = 0.0;
+    BB(it, it + 1) = -1.0;
+    flin(bb.data(), ld, it + 1, 1);
+    std::vector<double> coef(it);
+    for (int i = 0; i < it; ++i) coef[i] = BB(i, it + 1);
+    std::vector<const double2*> as(amp, amp + it);
+    k_lincomb(as.data(), coef.data(), it, nw, n, st);
+    return 0;
+}
+
+}  // namespace
+
+int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double* rms) {
+    if (!h) return -1;
+    h->mp2d_ok = false;
+    if (h->sym.nrep > 1) {
+        h->err = "rcc_mp2_density: C1 handles only";
+        return -1;
+    }
+    if (maxit < 1 || maxdim < 1 || maxdim > 16 || ntol < 1 || ntol > 16) {
+        h->err = "rcc_mp2_density: maxit >= 1, 1 <= maxdim <= 16, 1 <= ntol <= 16";
+        return -1;
+    }
+    CK(prep(h));
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt, nt1 = h->nt1, nt2 = h->nt2;
+    hipStream_t st = h->st;
+    if (!h->mp2t) {
+        CK(dalloc(h, &h->mp2t, nt1 + nt2));
+        CK(dalloc(h, &h->mp2d, (int64_t)o * o + (int64_t)v * v + 3 * nvo));
+        CK(dalloc_raw(h, &h->zdg, std::max<int64_t>(nvo, 1) * 8));
+    }
+    const int nch = zvec_chunks(nvo);
+    const int64_t need = (2 * (int64_t)maxdim + nch + 1) * std::max<int64_t>(nvo, 1);
+    if (h->zwork_n < need) {
+        CK(free_tracked(h, h->zwork));
+        h->zwork = nullptr;
+        h->zwork_n = 0;
+        CK(dalloc(h, &h->zwork, need));
+        h->zwork_n = need;
+    }
+    HIPCHK(h, hipEventRecord(h->ev0, st));
+    double2* T1m = h->mp2t;
+    double2* T2m = h->mp2t + nt1;
+    double2* DOO = h->mp2d;
+    double2* DVV = DOO + (int64_t)o * o;
+    double2* XVO = DVV + (int64_t)v * v;
+    double2* DVO = XVO + nvo;
+    double2* DOV = DVO + nvo;
+    // MP2EQN 'LAMBDA' (cceqns.F:89-92, 116-117): T = (FVO, VVOO) / D and the
+    // lambda amplitudes equal to T -- L below is T itself
+    k_copy(nt1, h->FVO, T1m, st);
+    k_copy(nt2, h->cls[RCC_VVOO], T2m, st);
+    k_denom(T1m, T2m, h->eps, h->any_frozen ? h->frozen : nullptr, h->vv, h->oo, v, o, st);
+    // CCDENSC(1) (ccgrad.F:77-104)
+    // DOO = conj[ -T(CDL,I)^H L(CDL,J) - T(C,I)^H L(C,J) ]
+    k_expand_e2(T2m, h->E2, nvvt, h->oo, o, st);
+    {
+        const int64_t K = nvvt * o;
+        CK(gemm(h, OP_C, OP_N, o, o, (int)K, cplx(-1, 0), h->E2, K, h->E2, K, cplx(0, 0), DOO, o));
+        CK(gemm(h, OP_C, OP_N, o, o, v, cplx(-1, 0), T1m, v, T1m, v, cplx(1, 0), DOO, o));
+        k_conj((int64_t)o * o, DOO, DOO, st);
+    }
+    // DVV = conj[ T(A,DKL) L(B,DKL)^H + T(A,K) L(B,K)^H ]
+    k_expand_f1(T2m, h->F1, noot, h->vv, v, st);
+    {
+        const int64_t K = (int64_t)v * noot;
+        CK(gemm(h, OP_N, OP_C, v, v, (int)K, cplx(1, 0), h->F1, v, h->F1, v, cplx(0, 0), DVV, v));
+        CK(gemm(h, OP_N, OP_C, v, v, o, cplx(1, 0), T1m, v, T1m, v, cplx(1, 0), DVV, v));
+        k_conj((int64_t)v * v, DVV, DVV, st);
+    }
+    // CCDENSZ's X intermediate (ccgrad.F:199-229)
+    k_fill_zero(XVO, nvo, st);
+    // T1EQN1 with tau -> L: X(A,I) += V(K,L,C,I) L(A,C,K,L)   (F1 holds L expanded)
+    CK(gemm(h, OP_N, OP_N, v, o, (int)(v * noot), cplx(1, 0), h->F1, v, h->M1, v * noot,
+            cplx(1, 0), XVO, v));
+    // T1EQN2 with tau -> L: X(A,I) += V(A,K,C,D) L(C,D,I,K)
+    CK(gemm(h, OP_N, OP_N, (int)nvo, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VOVV], nvo, T2m,
+            nvvt, cplx(0, 0), h->X, nvo));
+    h->x_ok = false;     // h->X no longer holds VOVV tau
+    k_trace_t1eqn2(h->X, XVO, h->oo, v, o, st);
+    // X(A,I) += V(IK,AL) D(K,L)   (SRT1ST4 + XGEMV)
+    k_x_vooo_doo(h->cls[RCC_VOOO], DOO, XVO, v, o, st);
+    // X(A,I) += W(CI,DA) D(C,D)   (SRT20D, left-hand contraction)
+    k_x_vovv_dvv(h->cls[RCC_VOVV], DVV, XVO, v, o, st);
+    k_axpby(nvo, cplx(0, 0), XVO, cplx(-1, 0), XVO, st);
+    // RPAA at zero frequency (ccrpa.F:104-229): X1 = XVO / DG, then
+    // X2 = (XVO + A' X1 - B conj X1) / DG, DIIS, until RMS < 10^-ntol
+    k_zvec_dg(h->VOVO_R, h->eps, h->zdg, v, o, st);
+    k_zvec_precd(XVO, h->zdg, DVO, nvo, st);
+    std::vector<double2*> zerr(maxdim), zamp(maxdim);
+    for (int k = 0; k < maxdim; ++k) {
+        zerr[k] = h->zwork + (int64_t)k * nvo;
+        zamp[k] = h->zwork + (int64_t)(maxdim + k) * nvo;
+    }
+    double2* part = h->zwork + 2 * (int64_t)maxdim * nvo;
+    double2* X2 = part + (int64_t)nch * nvo;
+    ZDiis z;
+    z.md = maxdim;
+    z.bb2.assign((size_t)(maxdim + 1) * (maxdim + 2), 0.0);
+    const double tol = std::pow(10.0, -ntol);
+    double r = 1.0;
+    int it = 0;
+    while (nvo > 0 && it < maxit) {
+        ++it;
+        k_zvec_gemv(h->VOVO_R, h->VVOO_R, DVO, part, nvo, st);
+        k_zvec_step(part, XVO, h->VOVO_R, DVO, h->zdg, X2, nvo, st);
+        CK(zdiis_step(h, z, zerr.data(), zamp.data(), DVO, X2, nvo, &r));
+        k_copy(nvo, X2, DVO, st);
+        if (r < tol) break;
+    }
+    k_conj(nvo, DVO, DOV, st);
+    HIPCHK(h, hipEventRecord(h->ev1, st));
+    HIPCHK(h, hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_mp2d_ms = ms;
+    h->mp2_nit = it;
+    h->mp2_rms = r;
+    h->mp2d_ok = true;
+    if (nit) *nit = it;
+    if (rms) *rms = r;
+    return 0;
+}
+
+double rcc_last_mp2d_ms(rcc_t* h) { return h ? h->last_mp2d_ms : 0.0; }
 
 int rcc_get_eps(rcc_t* h, double* eps) {
     if (!h || !eps) return -1;

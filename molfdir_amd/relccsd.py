@@ -25,7 +25,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "relccsd_hip.h")
 CLASSES = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5}
 ARRAYS = {"HO": 0, "HV": 1, "HOV": 2, "GO": 3, "GV": 4, "AZ": 5, "H": 6, "S1": 7,
           "S2": 8, "T1": 9, "T2": 10, "TAU": 11, "FOO": 12, "FVO": 13, "FVV": 14, "OOOO": 15,
-          "VOOO": 16, "VVOO": 17, "VOVO": 18, "VOVV": 19, "VVVV": 20}
+          "VOOO": 16, "VVOO": 17, "VOVO": 18, "VOVV": 19, "VVVV": 20, "DOO": 21, "DVV": 22,
+          "XVO": 23, "DVO": 24, "DOV": 25, "MP2T1": 26, "MP2T2": 27}
 TERMS = ("hointm", "hvintm", "hovintm", "t1eqns", "gointm", "gvintm", "aintm", "hintm",
          "t2eqns", "bintm", "denom", "zcore", "accept")
 
@@ -165,6 +166,9 @@ def load():
     lib.rcc_read_property.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                       ctypes.POINTER(ci), ci, vp]
     lib.rcc_expectation.argtypes = [vp, vp, ci, dp]
+    lib.rcc_mp2_density.argtypes = [vp, ci, ci, ci, ctypes.POINTER(ci), dp]
+    lib.rcc_last_mp2d_ms.argtypes = [vp]
+    lib.rcc_last_mp2d_ms.restype = ctypes.c_double
     lib.rcc_t3corr.argtypes = [vp, dp]
     lib.rcc_t3corr_part.argtypes = [vp, i64, i64, dp]
     lib.rcc_t3_triples.argtypes = [vp]
@@ -336,7 +340,8 @@ class RelCCSD:
                  "H": vo ** 2, "S1": vo, "S2": self.nt2, "T1": vo, "T2": self.nt2,
                  "TAU": self.nt2, "FOO": o ** 2, "FVO": vo, "FVV": v ** 2, "OOOO": oo ** 2,
                  "VOOO": vo * oo, "VVOO": vv * oo, "VOVO": vo ** 2, "VOVV": vo * vv,
-                 "VVVV": vv ** 2}[name]
+                 "VVVV": vv ** 2, "DOO": o ** 2, "DVV": v ** 2, "XVO": vo, "DVO": vo,
+                 "DOV": vo, "MP2T1": vo, "MP2T2": self.nt2}[name]
         out = np.empty(n, np.complex128)
         self._check(self.lib.rcc_get_array(self.h, ARRAYS[name], _ptr(out), n))
         return out
@@ -521,8 +526,21 @@ class RelCCSD:
                                                name.encode(), ne, len(nelec), _ptr(out)))
         return out
 
+    def mp2_density(self, maxit=30, maxdim=8, ntol=12):
+        """CCFOPR's MP2 relaxed density (MP2EQN 'LAMBDA', CCDENSC(1), CCDENSZ(1)
+        with the RPAA Z-vector); returns (iterations, last RMS)."""
+        nit = ctypes.c_int()
+        rms = ctypes.c_double()
+        self._check(self.lib.rcc_mp2_density(self.h, int(maxit), int(maxdim), int(ntol),
+                                             ctypes.byref(nit), ctypes.byref(rms)))
+        return nit.value, rms.value
+
+    def last_mp2d_ms(self):
+        return float(self.lib.rcc_last_mp2d_ms(self.h))
+
     def expectation(self, prop, level=1):
-        """EXPVAL: level 0 reference determinant, 1 unrelaxed CCSD (Lambda)."""
+        """EXPVAL: level 0 reference determinant, 1 unrelaxed CCSD (Lambda),
+        2 MP2 relaxed (after mp2_density)."""
         p = np.asfortranarray(prop, dtype=np.complex128)
         v = (ctypes.c_double * 2)()
         self._check(self.lib.rcc_expectation(self.h, _ptr(p), int(level), v))

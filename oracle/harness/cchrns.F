@@ -24,6 +24,12 @@ C  Mode 6 reads operator 'PROPTEST' from an MDPROP file written by the
 C  caller with RDPROP (ccints.F:569) and evaluates its reference-determinant
 C  expectation value with EXPVAL (ccgrad.F:261) on zero densities, as
 C  CCGRAD does for its SCF line (ccdriv.F:2028-2039).
+C  Mode 7 runs the MP2 relaxed property block of CCFOPR (ccdriv.F:
+C  2058-2090) on the given classes: MP2EQN with EQNS = 'LAMBDA',
+C  CCDENSC(1), CCDENSZ(1) (the X intermediate and the RPAA Z-vector
+C  solve, ccgrad.F:141, ccrpa.F:34) and EXPVAL of operator 'PROPTEST';
+C  it writes the MP2 amplitudes, DOO, DVV, -X, the Z-vector (DVO, DOV),
+C  the RPA preconditioner DG and the expectation value.
 C  Mode 5 copies the records of MCCRES.in (a restart file written by the
 C  caller) into MCCRES after CCDINI and reads it back with GETAMPT
 C  (ccgetv.F:643-709) and the status / energy records CCDUMP writes;
@@ -235,6 +241,74 @@ C
          WRITE (32) (AOO(I),I=1,2*NFOO),(AVV(I),I=1,2*NFVV),
      &              (AVO(I),I=1,2*NFVO),(AOV(I),I=1,2*NFVO)
          WRITE (32) DBLE(CEXPA),DIMAG(CEXPA)
+C
+      ELSEIF (MODE.EQ.7) THEN
+C
+C        MP2 relaxed first-order property as CCFOPR's DOMP2G block
+C        (ccdriv.F:2028-2090): MP2 lambda = amplitudes, CCDENSC(1),
+C        CCDENSZ(1) (X intermediate + RPAA Z-vector), EXPVAL of the
+C        caller's MDPROP operator 'PROPTEST'.  HDR(4..6) give the
+C        Z-vector's MAXIT, MAXDIM and NTOL (MXITGR, MXDIMGR, NTOLGR).
+C
+         EQNS = 'LAMBDA'
+         MXITGR = HDR(4)
+         MXDIMGR = HDR(5)
+         NTOLGR = HDR(6)
+         MAXIT = MXITGR
+         MAXDIM = MXDIMGR
+         NEOPER = 1
+         NAMEE(1) = 'PROPTEST'
+         NBUF1 = MAX0(JOOOO(NREP+1),JVOOO(NREP+1),
+     &                JVVOO(NREP+1),JVOVO(NREP+1))
+         NBUF2 = NBUF1
+         NV5IRP = 1
+         NV6IRP = 1
+         DO IRP = 1, NREP
+            NV5IRP = MAX0(NV5IRP,NVO (IRP)*NVVT(IRP))
+            NV6IRP = MAX0(NV6IRP,NVVT(IRP)*NVVT(IRP))
+         ENDDO
+         NBUF3 = MAX0(NBUF1,NV5IRP,NV6IRP)
+         ISZ = MAX0(NBUF1,NBUF3,NDIMT2) + 1024
+         ALLOCATE (T1(2*NDIMT1),T2(2*NDIMT2),S1(2*NDIMT1),
+     &             S2(2*NDIMT2),BUF1(2*ISZ),BUF2(2*ISZ),BUF3(2*ISZ),
+     &             PB(2*NSP*NSP),AVO(2*NSP*NSP),AOV(2*NSP*NSP),
+     &             AOO(2*NFOO),AVV(2*NFVV),ZD(24*NDIMX+2*NFOO+2*NFVV
+     &             + 2*(MAXDIM+1)*(MAXDIM+2) + MAXDIM + 64))
+         ZD = 0.0D0
+         PDOO = 1
+         PDVV = PDOO + 2*NFOO
+         PXVO = PDVV + 2*NFVV
+         PXOV = PXVO + 2*NDIMX
+         PDVO = PXOV + 2*NDIMX
+         PDOV = PDVO + 2*NDIMX
+         PD2VO = PDOV + 2*NDIMX
+         PD2OV = PD2VO + 2*NDIMX
+         PDG = PD2OV + 2*NDIMX
+         PCC1 = PDG + 2*NDIMX
+         PBB1 = PCC1 + MAXDIM + 8
+         PBB2 = PBB1 + (MAXDIM+1)*(MAXDIM+2) + 8
+         PNAME = 'PROPTEST'
+         CALL FMTOFILE (.FALSE.,FVO,FOO,FVV)
+         CALL MP2EQN (EPS,FVO,T1,S1,T2,S2)
+         CALL CCDENSC (1,T1,S1,T2,S2,BUF1,BUF2,BUF3,ZD(PDOO),ZD(PDVV),
+     &                 ZD(PDVO),ZD(PDOV))
+         WRITE (32) T1,T2
+         WRITE (32) (ZD(I),I=PDOO,PDOO+2*NFOO-1),
+     &              (ZD(I),I=PDVV,PDVV+2*NFVV-1)
+         CALL CCDENSZ (1,NSP,IREPSPI,EPS,T1,S1,T2,S2,
+     &                 BUF1,BUF2,BUF3,NBUF2,NBUF3,PB,
+     &                 ZD(PXVO),ZD(PXOV),ZD(PDOO),ZD(PDVV),ZD(PDVO),
+     &                 ZD(PDOV),AVO,AOV,ZD(PD2VO),ZD(PD2OV),ZD(PDG),
+     &                 ZD(PCC1),ZD(PBB1),ZD(PBB2),10.0D0**(-NTOLGR))
+         WRITE (32) (ZD(I),I=PXVO,PXVO+2*NDIMX-1),
+     &              (ZD(I),I=PDVO,PDVO+2*NDIMX-1),
+     &              (ZD(I),I=PDOV,PDOV+2*NDIMX-1),
+     &              (ZD(I),I=PDG,PDG+NDIMX-1)
+         CALL RDPROP (1,PNAME,NSP,.TRUE.,PB,IREPSPI,EPH,AVO,AOV,
+     &                AOO,AVV)
+         CALL EXPVAL (ZD(PDOO),ZD(PDVV),ZD(PDVO),ZD(PDOV),AOO,AVV,AVO,
+     &                AOV,EPH,CEXPA)
+         WRITE (32) DBLE(CEXPA),DIMAG(CEXPA),DBLE(NDIMX)
 C
       ELSEIF (MODE.EQ.5) THEN
 C

@@ -235,3 +235,33 @@ def read_property(sysd, mdprop_path, workdir=None, timeout=600):
         pos += n
     out["expval"] = complex(raw[-2], raw[-1])
     return out
+
+
+def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, timeout=600):
+    """Mode 7: the MP2 relaxed property block of CCFOPR (ccdriv.F:2058-2090)
+    on operator 'PROPTEST' of an MDPROP file.  C1 systems.  Returns the MP2
+    amplitudes, DOO / DVV after CCDENSC(1) (packed, reference layout), the
+    X intermediate as CCDENSZ leaves it (-X), the Z-vector DVO / DOV that
+    RPAA converged, its preconditioner DG and EXPVAL's value."""
+    import shutil
+    no, nv = sysd["no"], sysd["nv"]
+    nt1, nt2 = no * nv, (nv * (nv - 1) // 2) * (no * (no - 1) // 2)
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        _write_input(os.path.join(d, "ccin.bin"), sysd, 7, maxit, maxdim, ntol)
+        shutil.copy(mdprop_path, os.path.join(d, "MDPROP"))
+        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
+            raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
+        raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
+    ndimx = int(round(raw[-1]))
+    out, pos = {}, 0
+    for name, n in (("T1", nt1), ("T2", nt2), ("DOO", no * no), ("DVV", nv * nv),
+                    ("XVO", ndimx), ("DVO", ndimx), ("DOV", ndimx)):
+        out[name] = raw[pos:pos + 2 * n].view(np.complex128).copy()
+        pos += 2 * n
+    out["DG"] = raw[pos:pos + ndimx].copy()
+    pos += ndimx
+    out["expval"] = complex(raw[pos], raw[pos + 1])
+    out["log"] = log
+    return out

@@ -33,6 +33,11 @@ change in the generator is caught.
   f +- h V for a seeded Hermitian one-electron perturbation V (stored), the
   finite-field gradient d Ec / d lam the CPU path gives (relccsd's own
   Lambda / density code is switched off, ccdriv.F:1918-1924).
+* prop_<name>.npz: relccsd's RDPROP blocks of a seeded operator written as
+  MDPROP 'PROPTEST', and EXPVAL on zero densities.
+* mp2g_<name>.npz: relccsd's MP2 relaxed property chain of CCFOPR (harness
+  mode 7): DOO / DVV of CCDENSC(1), the X intermediate and the RPAA Z-vector
+  (DVO, DOV, preconditioner DG) of CCDENSZ(1), and EXPVAL of the operator.
 """
 import os
 import sys
@@ -196,6 +201,34 @@ def prop_case(name, no, nv, seed, scale, fp, pseed):
     print(name, r["expval"])
 
 
+MP2G_CASES = [
+    # name, no, nv, seed, scale, fock_perturb, operator seed, MAXDIM, NTOL
+    ("g_4o10v", 4, 10, 3, 0.05, 0.05, 21, 8, 13),
+    ("g_6o20v", 6, 20, 4, 0.05, 0.03, 22, 6, 13),
+    ("g_8o30v", 8, 30, 5, 0.04, 0.0, 23, 8, 13),
+]
+
+
+def mp2g_case(name, no, nv, seed, scale, fp, pseed, maxdim, ntol):
+    """relccsd's MP2 relaxed property chain (harness mode 7): MP2 lambda,
+    CCDENSC(1), CCDENSZ(1) with the RPAA Z-vector, EXPVAL of a seeded
+    Hermitian operator written as MDPROP 'PROPTEST'."""
+    import tempfile
+    from molfdir_amd import mdcint
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+    P = synth.hermitian_perturbation(no + nv, pseed, 0.1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "MDPROP")
+        mdcint.write_mdprop(path, {"OTHER": 2.0 * P, "PROPTEST": P})
+        r = refrun.mp2_property(s, path, maxit=100, maxdim=maxdim, ntol=ntol)
+    np.savez_compressed(os.path.join(HERE, f"mp2g_{name}.npz"),
+                        params=np.array([no, nv, seed, pseed, maxdim, ntol], dtype=np.int64),
+                        scale=scale, fock_perturb=fp, T1=r["T1"], DOO=r["DOO"], DVV=r["DVV"],
+                        XVO=r["XVO"], DVO=r["DVO"], DOV=r["DOV"], DG=r["DG"],
+                        expval=r["expval"])
+    print(name, r["expval"])
+
+
 MCCRES_CASES = [("ccsd", "cfg1_6o40v"), ("symccsd", "s4_5o12v")]
 
 
@@ -288,6 +321,10 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--mp2g-only" in sys.argv:
+        for case in MP2G_CASES:
+            mp2g_case(*case)
+        return
     if "--prop-only" in sys.argv:
         for case in PROP_CASES:
             prop_case(*case)

@@ -1,0 +1,107 @@
+"""MP2 relaxed first-order property on the GPU: rcc_mp2_density (CCFOPR's
+DOMP2G block, ccdriv.F:2058-2090) and rcc_expectation level 2 (EXPVAL
+ccgrad.F:261-347).
+
+* Against relccsd's own chain (harness mode 7, tests/golden/mp2g_*.npz): the
+  MP2 amplitudes, DOO, DVV and the X intermediate to 1e-12 relative; the
+  Z-vector (DVO, DOV), converged to the same DIIS RMS as the reference, to
+  1e-10; the expectation value to 1e-11.
+* Against the oracle's direct solve of the Z-vector equations at 10o/60v
+  (600 VO pairs): X to 1e-12, the Z-vector to 1e-10.
+* Two calls give bit-identical densities; the CCSD amplitudes of the handle
+  are left alone; level 2 before the density, a symmetric handle and a
+  DIIS dimension beyond 16 are refused.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from molfdir_amd import synth  # noqa: E402
+from molfdir_amd.relccsd import RelCCSD, RccError  # noqa: E402
+from oracle import mp2grad_oracle as O  # noqa: E402
+from oracle.ccsd_oracle import Layout  # noqa: E402
+
+from _golden import load, names, system_for  # noqa: E402
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300)
+
+
+def solver_for(s, **kw):
+    r = RelCCSD(int(s["no"]), int(s["nv"]), **kw)
+    r.set_system(s)
+    return r
+
+
+@pytest.mark.parametrize("name", names("mp2g"))
+def test_mp2_density_matches_reference(name):
+    g = load("mp2g", name)
+    no, nv, _, pseed, maxdim, ntol = (int(x) for x in g["params"])
+    s = system_for(g)
+    P = synth.hermitian_perturbation(no + nv, pseed, 0.1)
+    r = solver_for(s)
+    print(f"mp2g {name}: no={no} nv={nv} maxdim={maxdim} ntol={ntol}", flush=True)
+    nit, rms = r.mp2_density(maxit=100, maxdim=maxdim, ntol=ntol)
+    assert rms < 10.0 ** -ntol and nit < 100
+    assert rel(r.get("MP2T1"), g["T1"]) < 1e-13
+    assert rel(r.get("DOO"), g["DOO"]) < 1e-12
+    assert rel(r.get("DVV"), g["DVV"]) < 1e-12
+    assert rel(r.get("XVO"), g["XVO"]) < 1e-12
+    assert rel(r.get("DVO"), g["DVO"]) < 1e-10
+    assert rel(r.get("DOV"), g["DOV"]) < 1e-10
+    e = r.expectation(P, level=2)
+    assert abs(e - complex(g["expval"])) < 1e-11, (e, g["expval"])
+    # the reference-determinant part is the same operator trace
+    assert abs(r.expectation(P, level=0) - np.trace(P[:no, :no])) < 1e-15
+
+
+def test_mp2_density_matches_oracle_10o60v():
+    no, nv = 10, 60
+    s = synth.make_system(no, nv, seed=11, scale=0.03, fock_perturb=0.02)
+    P = synth.hermitian_perturbation(no + nv, 31, 0.1)
+    o = O.mp2_relaxed(s, P)
+    L = Layout(no, nv)
+    r = solver_for(s)
+    print(f"mp2g oracle 10o60v", flush=True)
+    nit, rms = r.mp2_density(maxit=100, maxdim=8, ntol=13)
+    assert rms < 1e-13
+    assert rel(r.get("DOO"), L.pmat(o["doo"])) < 1e-12
+    assert rel(r.get("DVV"), L.pmat(o["dvv"])) < 1e-12
+    assert rel(r.get("XVO"), o["xvo"]) < 1e-12
+    assert rel(r.get("DVO"), o["dvo"]) < 1e-10
+    assert abs(r.expectation(P, level=2) - o["expval"]) < 1e-11
+
+
+def test_repeat_is_bit_identical_and_amplitudes_untouched():
+    g = load("mp2g", "g_6o20v")
+    s = system_for(g)
+    r = solver_for(s)
+    r.mp2()
+    t1, t2 = r.get("T1"), r.get("T2")
+    r.mp2_density(maxit=100, maxdim=6, ntol=13)
+    first = {k: r.get(k) for k in ("DOO", "DVV", "XVO", "DVO", "DOV")}
+    r.mp2_density(maxit=100, maxdim=6, ntol=13)
+    for k, v in first.items():
+        assert np.array_equal(r.get(k), v), k
+    assert np.array_equal(r.get("T1"), t1) and np.array_equal(r.get("T2"), t2)
+
+
+def test_refusals():
+    g = load("mp2g", "g_4o10v")
+    s = system_for(g)
+    r = solver_for(s)
+    P = np.eye(14, dtype=np.complex128)
+    with pytest.raises(RccError):
+        r.expectation(P, level=2)
+    with pytest.raises(RccError):
+        r.mp2_density(maxdim=17)
+    gs = load("symccsd", "s4_5o12v")
+    ss = system_for(gs)
+    rs = RelCCSD(int(ss["no"]), int(ss["nv"]), sym=(gs["sym_no"].tolist(), gs["sym_nv"].tolist()))
+    rs.set_system(ss)
+    with pytest.raises(RccError):
+        rs.mp2_density()
