@@ -5,6 +5,7 @@
     python bench.py --config cfg5        # Lambda equations + response density
     python bench.py --config cfg3t       # CCSD(T) triples (T3CORR) at 30o/300v
     python bench.py --config sort2       # integral front end (RDINTS) at 20o/200v
+    python bench.py --config mp2g        # MP2 relaxed density + Z-vector at 30o/300v
 
 A step is one CCEQN iteration (HOINTM .. BINTM, DENOM, DIIS, CCENG, S->T)
 over the whole synthetic system with every integral class resident in HBM.
@@ -37,6 +38,8 @@ TRIPLES = {"cfg3t": (30, 300, 3, "C1 30o/300v complex128")}
 T3_CPU_SAMPLE = (12, 70)  # relccsd T3CORR sample size (o, v)
 SORT = {"sort2": (20, 200, 2, "C1 20o/200v (110 Kramers pairs)")}
 SORT_CPU_SAMPLE = (64, 32)   # relccsd RDINTS sample: Kramers pairs, occupied spinors
+MP2G = {"mp2g": (30, 300, 3, "C1 30o/300v complex128")}
+MP2G_CPU_SAMPLE = (16, 120)  # relccsd CCFOPR MP2 block sample: occupied, virtual
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
@@ -377,6 +380,114 @@ def bench_sort(args):
     print(json.dumps(res), flush=True)
 
 
+def mp2g_flops(no: int, nv: int, nit: int):
+    """Algorithmic flops of CCFOPR's MP2 block: the four products of CCDENSC /
+    CCDENSZ (VOVV x lambda of T1EQN2, T1EQN1's expanded lambda x VOOO, the DVV
+    and DOO products; 8 real flops per complex multiply-add) and, per Z-vector
+    iteration, the two VO x VO matrix-vector products of DOAX / DOBY."""
+    nvo, noot, nvvt = no * nv, no * (no - 1) // 2, nv * (nv - 1) // 2
+    gemm = nvo * noot * nvvt + nv * no * nv * noot + nv * nv * nv * noot + no * no * nvvt * no
+    return 8.0 * gemm, 8.0 * 2 * nvo * nvo * nit
+
+
+def mp2g_cpu_baseline(workload_flops: float):
+    """relccsd's own CCFOPR MP2 block (oracle/_ref, harness mode 7: MP2EQN
+    'LAMBDA', CCDENSC(1), CCDENSZ(1) with RPAA, EXPVAL) on a bounded sample;
+    its CPU seconds, and the rate extrapolated by the flop count."""
+    import re
+    import tempfile
+    from molfdir_amd import mdcint, synth
+    from oracle import refrun
+    no, nv = MP2G_CPU_SAMPLE
+    if not refrun.available():
+        return None
+    s = synth.make_system(no, nv, seed=3, scale=0.02)
+    P = synth.hermitian_perturbation(no + nv, 21, 0.1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "MDPROP")
+        mdcint.write_mdprop(path, {"PROPTEST": P})
+        r = refrun.mp2_property(s, path, maxit=100, maxdim=8, ntol=10)
+    rows = re.findall(r"^\s+(\d+)\s+-?[\d.]+\s+-?[\d.]+\s+[\d.]*E[-+]\d+\s*$", r["log"], re.M)
+    nit = int(rows[-1]) if rows else 0
+    fg, fz = mp2g_flops(no, nv, nit)
+    rate = (fg + fz) / r["cpu_s"]
+    return {
+        "value": rate / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+        "s_extrapolated": workload_flops / rate,
+        "sample": (f"relccsd CCFOPR MP2 block (MP2EQN 'LAMBDA', CCDENSC(1), CCDENSZ(1) with "
+                   f"the RPAA Z-vector, {nit} iterations to RMS < 1e-10, EXPVAL; unmodified "
+                   f"reference sources, single-threaded netlib BLAS) at C1 {no}o/{nv}v, "
+                   f"{r['cpu_s']:.2f} s CPU; rate extrapolated to the workload by the flop "
+                   f"count"),
+    }
+
+
+def bench_mp2g(args):
+    """The MP2 relaxed density of CCFOPR on the GPU (rcc_mp2_density: MP2
+    lambda, CCDENSC(1) densities, CCDENSZ(1) X intermediate and Z-vector to
+    RMS < 1e-10) and EXPVAL of a one-electron operator; one untimed warm-up
+    call (allocations, derived integral layouts)."""
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+    no, nv, seed, desc = MP2G[args.config]
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    scale = args.t3_scale
+    sysd = synth.make_system(no, nv, seed=seed, scale=scale, device=dev)
+    r = RelCCSD(no, nv)
+    r.set_complex_product(args.complex_product == 3)
+    r.set_fock(sysd["FOO"], sysd["FVO"], sysd["FVV"], sysd["eps"])
+    for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
+        r.set_ints(name, sysd[name])
+        sysd[name] = None
+    del sysd
+    torch.cuda.empty_cache()
+    P = synth.hermitian_perturbation(no + nv, 77, 0.1)
+    r.mp2_density(maxit=100, maxdim=8, ntol=10)          # warm-up
+    r.synchronize()
+    parts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nit, rms = r.mp2_density(maxit=100, maxdim=8, ntol=10)
+        parts.append(r.last_mp2d_ms())
+    wall = time.perf_counter() - t0
+    e = r.expectation(P, level=2)
+    avg = {k: sum(p[k] for p in parts) / len(parts) for k in parts[0]}
+    fg, fz = mp2g_flops(no, nv, nit)
+    nvo, noot, nvvt = no * nv, no * (no - 1) // 2, nv * (nv - 1) // 2
+    xflops = 8.0 * nvo * noot * nvvt
+    x_ach = xflops / (avg["xgemm"] * 1e-3) / 1e12
+    zbytes = 2.0 * 16 * nvo * nvo
+    z_ach = zbytes * nit / (avg["zgemv"] * 1e-3) / 1e9
+    print(f"[bench] MP2 relaxed density {avg['total']:.1f} ms (densities {avg['densities']:.1f}, "
+          f"X {avg['x']:.1f} of which VOVV x lambda {avg['xgemm']:.1f}, Z-vector "
+          f"{avg['zvector']:.1f} over {nit} iterations of which matrix-vector "
+          f"{avg['zgemv']:.1f}); <P> = {e}", file=sys.stderr, flush=True)
+    res = {
+        "metric": f"MP2 relaxed density + Z-vector s at {no}o/{nv}v cplx",
+        "value": avg["total"] * 1e-3, "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": avg["total"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": f"synthetic (seed {seed}; factor Re/Im N(0,{scale}^2)); operator: seeded "
+                f"Hermitian matrix (norm 0.1)",
+        "config": {"workload": f"CCFOPR MP2 block (MP2 lambda, CCDENSC, CCDENSZ X + RPAA "
+                               f"Z-vector to RMS < 1e-10, EXPVAL), {desc}",
+                   "zvector_iterations": nit, "zvector_rms": rms},
+        "expectation_value": [e.real, e.imag], "phases_ms": avg, "wall_s": wall,
+        "tflops": (fg + fz) / (avg["total"] * 1e-3) / 1e12, "flops": fg + fz,
+        "roofline": {"bound": "mfma", "kernel": "zgemm VOVV x lambda (T1EQN2 product of X)",
+                     "achieved": x_ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
+                     "frac": x_ach / PEAK_FP64_MFMA, "traffic": None,
+                     "algorithmic_flops": xflops},
+        "roofline_zvector": {"bound": "hbm", "kernel": "zvec_gemv_kernel (A and B read once)",
+                             "achieved": z_ach, "peak": 8000.0, "unit": "GB/s",
+                             "frac": z_ach / 8000.0, "algorithmic_bytes_per_launch": zbytes},
+        "cpu_baseline": None if args.no_cpu_baseline else mp2g_cpu_baseline(fg + fz),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def cpu_baseline(workload_flops: float):
     """relccsd's own CPU CCEQN (built from the reference sources by
     oracle/Makefile.ref) on a bounded sample, extrapolated by flop count."""
@@ -593,7 +704,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg3",
-                    choices=sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA) + sorted(TRIPLES) + sorted(SORT))
+                    choices=(sorted(CONFIGS) + sorted(BATCH) + sorted(LAMBDA) + sorted(TRIPLES)
+                             + sorted(SORT) + sorted(MP2G)))
     ap.add_argument("--lambda-maxit", type=int, default=80,
                     help="cfg5: MAXIT of the CCSD and Lambda solves")
     ap.add_argument("--scale", type=float, default=0.05,
@@ -639,6 +751,9 @@ def main():
         return
     if args.config in SORT:
         bench_sort(args)
+        return
+    if args.config in MP2G:
+        bench_mp2g(args)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8

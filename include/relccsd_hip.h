@@ -287,8 +287,12 @@ int rcc_expectation(rcc_t* h, const double* prop, int level, double* value);
  * last RMS.  Afterwards rcc_expectation level 2 gives the MP2 relaxed value
  * of EXPVAL, and rcc_get_array RCC_ARR_DOO .. RCC_ARR_MP2T2 the pieces. */
 int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double* rms);
-/* Device time (ms) of the last rcc_mp2_density. */
+/* Device time (ms) of the last rcc_mp2_density; rcc_mp2d_timing: 6 doubles,
+ * total, MP2 amplitudes + densities, X intermediate, Z-vector loop, the sum
+ * over the Z-vector's matrix-vector launches, and the VOVV x lambda GEMM
+ * (part of the X intermediate). */
 double rcc_last_mp2d_ms(rcc_t* h);
+int rcc_mp2d_timing(rcc_t* h, double* ms);
 /* The handle's orbital energies (no + nv doubles, occupied then virtual). */
 int rcc_get_eps(rcc_t* h, double* eps);
 

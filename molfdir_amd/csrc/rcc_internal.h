@@ -252,12 +252,12 @@ void k_gather_block(const double2* src, int64_t ld, const int* rowmap, const int
 void k_fill_zero(double2* p, int64_t n, hipStream_t st);
 
 // MP2 relaxed property (rcc_props.hip).  X[a + nv*i] += sum_kl <ak||il> D[k + no*l]
-// over the packed VOOO class; X[a + nv*i] += sum_bc D[b + nv*c] conj <bi||ca> over
-// the packed VOVV class
-void k_x_vooo_doo(const double2* vooo, const double2* doo, double2* x, int nv, int no,
-                  hipStream_t st);
-void k_x_vovv_dvv(const double2* vovv, const double2* dvv, double2* x, int nv, int no,
-                  hipStream_t st);
+// over the packed VOOO class (part: 2 noot nv scratch); X[a + nv*i] += sum_bc
+// D[b + nv*c] conj <bi||ca> over the packed VOVV class (part: 2 nvvt no scratch)
+void k_x_vooo_doo(const double2* vooo, const double2* doo, double2* x, PairTab oo, double2* part,
+                  int nv, int no, hipStream_t st);
+void k_x_vovv_dvv(const double2* vovv, const double2* dvv, double2* x, PairTab vv, double2* part,
+                  int nv, int no, hipStream_t st);
 // Z-vector: dg = eps_a - eps_i - Re R[ai,ai]; part = chunked R x - B conj(x)
 // (zvec_chunks(n) chunks of n); xn = (rhs + sum part - R[r,r] x) / dg; y = x / dg
 void k_zvec_dg(const double2* R, const double* eps, double* dg, int nv, int no, hipStream_t st);

@@ -4,8 +4,9 @@
 // matrix-vector products DOAX / DOBY (ccrpa.F:467-603), the preconditioner
 // GETDG (ccrpa.F:1450-1528) and PRECD (ccrpa.F:695-754).
 //
-// All reductions run in a fixed order (one thread or one workgroup per
-// output, no atomics), so repeated calls give bit-identical results.
+// All reductions run in a fixed order (per-column partial sums, then one
+// thread per output; no atomics), so repeated calls give bit-identical
+// results.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "rcc_internal.h"
@@ -22,90 +23,118 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// X[a + nv*i] += sum_{k,l} <ak||il> D[k + no*l]; V is the packed VOOO class
-// [(a + nv*k), pair(p > q)] = <ak||pq>.  One thread per (a, i); consecutive
-// threads read consecutive rows of a column.
-__global__ void x_vooo_doo_kernel(const double2* __restrict__ V, const double2* __restrict__ D,
-                                  double2* __restrict__ X, int nv, int no) {
+// X[a + nv*i] += sum_{k,l} <ak||il> D[k + no*l] over the packed VOOO class
+// V[(a + nv*k), pair(x > y)] = <ak||xy>, in two passes.  Pass 1, one
+// workgroup per column pair(x > y), each element read once:
+//   part[2 (a + nv P)]     = sum_k V[(a,k),P] D(k,y)   (-> X(a,x) +=)
+//   part[2 (a + nv P) + 1] = sum_k V[(a,k),P] D(k,x)   (-> X(a,y) -=)
+// Pass 2, one thread per (a, m): X(a,m) += sum_{y<m} part1[pair(m,y)] -
+// sum_{x>m} part2[pair(x,m)], in a fixed order.
+__global__ void __launch_bounds__(256)
+x_vooo_cols_kernel(const double2* __restrict__ V, const double2* __restrict__ D,
+                   double2* __restrict__ part, PairTab oo, int nv, int no) {
     const int64_t nvo = (int64_t)nv * no;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nvo) return;
-    const int a = (int)(e % nv), i = (int)(e / nv);
-    double re = 0.0, im = 0.0;
-    for (int l = 0; l < no; ++l) {
-        if (l == i) continue;
-        const int64_t col = nvo * (i > l ? pidx(i, l, no) : pidx(l, i, no));
-        const double s = i > l ? 1.0 : -1.0;
+    const int64_t P = blockIdx.x;
+    const int2 xy = oo.tab[P];
+    const double2* col = V + nvo * P;
+    for (int a = threadIdx.x; a < nv; a += blockDim.x) {
+        double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
         for (int k = 0; k < no; ++k) {
-            const double2 p = cmul(V[col + a + (int64_t)nv * k], D[k + (int64_t)no * l]);
-            re += s * p.x;
-            im += s * p.y;
+            const double2 v = col[a + (int64_t)nv * k];
+            const double2 p1 = cmul(v, D[k + (int64_t)no * xy.y]);
+            const double2 p2 = cmul(v, D[k + (int64_t)no * xy.x]);
+            s1.x += p1.x; s1.y += p1.y;
+            s2.x += p2.x; s2.y += p2.y;
         }
+        part[2 * (a + (int64_t)nv * P)] = s1;
+        part[2 * (a + (int64_t)nv * P) + 1] = s2;
+    }
+}
+
+__global__ void x_vooo_finish_kernel(const double2* __restrict__ part, double2* __restrict__ X,
+                                     int nv, int no) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nv * no) return;
+    const int a = (int)(e % nv), m = (int)(e / nv);
+    double re = 0.0, im = 0.0;
+    for (int y = 0; y < m; ++y) {
+        const double2 p = part[2 * (a + (int64_t)nv * pidx(m, y, no))];
+        re += p.x;
+        im += p.y;
+    }
+    for (int x = m + 1; x < no; ++x) {
+        const double2 p = part[2 * (a + (int64_t)nv * pidx(x, m, no)) + 1];
+        re -= p.x;
+        im -= p.y;
     }
     X[e].x += re;
     X[e].y += im;
 }
 
-// X[a + nv*i] += sum_{b,c} D[b + nv*c] conj <bi||ca>; V is the packed VOVV
-// class [(b + nv*i), pair(p > q)] = <bi||pq>.  Workgroup (a, block of IC
-// occupied i): for every c != a it reads the IC*nv contiguous rows (b, i) of
-// column pair(c, a), i.e. each element of the class is read twice in all
-// (once for each index of its pair).  Fixed-order workgroup reduction.
-constexpr int XV_IC = 8;
-constexpr int XV_THREADS = 128;
+// X[a + nv*i] += sum_{b,c} D[b + nv*c] conj <bi||ca> over the packed VOVV
+// class V[(b + nv*i), pair(x > y)] = <bi||xy>, in two passes.  Pass 1, one
+// workgroup per column pair(x > y) (each element read once), one wave per
+// occupied i at a time, lanes over b, fixed-order butterfly reduction:
+//   part[2 (i + no P)]     = sum_b D(b,x) conj V[(b,i),P]   (-> X(y,i) +=)
+//   part[2 (i + no P) + 1] = sum_b D(b,y) conj V[(b,i),P]   (-> X(x,i) -=)
+// Pass 2, one thread per (m, i): X(m,i) += sum_{x>m} part1[pair(x,m)] -
+// sum_{y<m} part2[pair(m,y)].
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
-__global__ void __launch_bounds__(XV_THREADS)
-x_vovv_dvv_kernel(const double2* __restrict__ V, const double2* __restrict__ D,
-                  double2* __restrict__ X, int nv, int no) {
+__global__ void __launch_bounds__(256)
+x_vovv_cols_kernel(const double2* __restrict__ V, const double2* __restrict__ D,
+                   double2* __restrict__ part, PairTab vv, int nv, int no) {
     const int64_t nvo = (int64_t)nv * no;
-    const int a = blockIdx.x;
-    const int i0 = blockIdx.y * XV_IC;
-    const int ni = min(XV_IC, no - i0);
-    double re[XV_IC], im[XV_IC];
-#pragma unroll
-    for (int q = 0; q < XV_IC; ++q) re[q] = im[q] = 0.0;
-    for (int c = 0; c < nv; ++c) {
-        if (c == a) continue;
-        const double2* col = V + nvo * (c > a ? pidx(c, a, nv) : pidx(a, c, nv)) + (int64_t)nv * i0;
-        const double s = c > a ? 1.0 : -1.0;
-        const double2* dc = D + (int64_t)nv * c;
-        for (int b = threadIdx.x; b < nv; b += XV_THREADS) {
-            const double2 d = dc[b];
-            const double dr = s * d.x, di = s * d.y;
-#pragma unroll
-            for (int q = 0; q < XV_IC; ++q) {
-                if (q < ni) {
-                    const double2 v = col[b + (int64_t)nv * q];
-                    // d * conj(v)
-                    re[q] += dr * v.x + di * v.y;
-                    im[q] += di * v.x - dr * v.y;
-                }
-            }
+    const int64_t P = blockIdx.x;
+    const int2 xy = vv.tab[P];
+    const double2* col = V + nvo * P;
+    const double2* dx = D + (int64_t)nv * xy.x;
+    const double2* dy = D + (int64_t)nv * xy.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = w; i < no; i += 4) {
+        double r1 = 0.0, i1 = 0.0, r2 = 0.0, i2 = 0.0;
+        for (int b = lane; b < nv; b += 64) {
+            const double2 v = col[b + (int64_t)nv * i];
+            const double2 d1 = dx[b], d2 = dy[b];
+            r1 += d1.x * v.x + d1.y * v.y;
+            i1 += d1.y * v.x - d1.x * v.y;
+            r2 += d2.x * v.x + d2.y * v.y;
+            i2 += d2.y * v.x - d2.x * v.y;
+        }
+        r1 = wave_sum(r1);
+        i1 = wave_sum(i1);
+        r2 = wave_sum(r2);
+        i2 = wave_sum(i2);
+        if (lane == 0) {
+            part[2 * (i + (int64_t)no * P)] = make_double2(r1, i1);
+            part[2 * (i + (int64_t)no * P) + 1] = make_double2(r2, i2);
         }
     }
-    __shared__ double sr[XV_IC][XV_THREADS], si[XV_IC][XV_THREADS];
-#pragma unroll
-    for (int q = 0; q < XV_IC; ++q) {
-        sr[q][threadIdx.x] = re[q];
-        si[q][threadIdx.x] = im[q];
+}
+
+__global__ void x_vovv_finish_kernel(const double2* __restrict__ part, double2* __restrict__ X,
+                                     int nv, int no) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nv * no) return;
+    const int i = (int)(e % no), m = (int)(e / no);
+    double re = 0.0, im = 0.0;
+    for (int x = m + 1; x < nv; ++x) {
+        const double2 p = part[2 * (i + (int64_t)no * pidx(x, m, nv))];
+        re += p.x;
+        im += p.y;
     }
-    __syncthreads();
-    for (int w = XV_THREADS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-#pragma unroll
-            for (int q = 0; q < XV_IC; ++q) {
-                sr[q][threadIdx.x] += sr[q][threadIdx.x + w];
-                si[q][threadIdx.x] += si[q][threadIdx.x + w];
-            }
-        }
-        __syncthreads();
+    for (int y = 0; y < m; ++y) {
+        const double2 p = part[2 * (i + (int64_t)no * pidx(m, y, nv)) + 1];
+        re -= p.x;
+        im -= p.y;
     }
-    if ((int)threadIdx.x < ni) {
-        const int q = threadIdx.x;
-        double2* x = X + a + (int64_t)nv * (i0 + q);
-        x->x += sr[q][0];
-        x->y += si[q][0];
-    }
+    double2* xp = X + m + (int64_t)nv * i;
+    xp->x += re;
+    xp->y += im;
 }
 
 // dg[a + nv*i] = eps[no + a] - eps[i] - Re R[(ai), (ai)]   (GETDG)
@@ -173,19 +202,22 @@ inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
-void k_x_vooo_doo(const double2* vooo, const double2* doo, double2* x, int nv, int no,
-                  hipStream_t st) {
-    const int64_t n = (int64_t)nv * no;
-    if (n == 0 || no < 2) return;
-    hipLaunchKernelGGL(x_vooo_doo_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, vooo, doo, x, nv,
-                       no);
+void k_x_vooo_doo(const double2* vooo, const double2* doo, double2* x, PairTab oo, double2* part,
+                  int nv, int no, hipStream_t st) {
+    if (oo.npair == 0 || nv == 0) return;
+    hipLaunchKernelGGL(x_vooo_cols_kernel, dim3((unsigned)oo.npair), dim3(256), 0, st, vooo, doo,
+                       part, oo, nv, no);
+    hipLaunchKernelGGL(x_vooo_finish_kernel, dim3(nblk((int64_t)nv * no, 256)), dim3(256), 0, st,
+                       part, x, nv, no);
 }
 
-void k_x_vovv_dvv(const double2* vovv, const double2* dvv, double2* x, int nv, int no,
-                  hipStream_t st) {
-    if (nv < 2 || no < 1) return;
-    hipLaunchKernelGGL(x_vovv_dvv_kernel, dim3(nv, (no + XV_IC - 1) / XV_IC), dim3(XV_THREADS), 0,
-                       st, vovv, dvv, x, nv, no);
+void k_x_vovv_dvv(const double2* vovv, const double2* dvv, double2* x, PairTab vv, double2* part,
+                  int nv, int no, hipStream_t st) {
+    if (vv.npair == 0 || no == 0) return;
+    hipLaunchKernelGGL(x_vovv_cols_kernel, dim3((unsigned)vv.npair), dim3(256), 0, st, vovv, dvv,
+                       part, vv, nv, no);
+    hipLaunchKernelGGL(x_vovv_finish_kernel, dim3(nblk((int64_t)nv * no, 256)), dim3(256), 0, st,
+                       part, x, nv, no);
 }
 
 void k_zvec_dg(const double2* R, const double* eps, double* dg, int nv, int no, hipStream_t st) {

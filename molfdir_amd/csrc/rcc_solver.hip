@@ -182,6 +182,12 @@ struct rcc_handle {
     bool mp2d_ok = false;
     int mp2_nit = 0;
     double mp2_rms = 0.0, last_mp2d_ms = 0.0;
+    // phase timers: 0 start, 1 densities done, 2 X done, 3 end; 4/5 around
+    // each Z-vector matrix-vector launch, 6/7 around the VOVV x lambda GEMM.
+    // mp2_ms: total, densities, X, Z-vector loop, sum of the matrix-vector
+    // launches, the VOVV x lambda GEMM
+    hipEvent_t mev[8] = {};
+    double mp2_ms[6] = {0, 0, 0, 0, 0, 0};
 
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
@@ -2419,6 +2425,8 @@ void rcc_destroy(rcc_t* h) {
             if (h->kev[w][p]) (void)hipEventDestroy(h->kev[w][p]);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (hipEvent_t e : h->mev)
+        if (e) (void)hipEventDestroy(e);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
 }
@@ -3236,7 +3244,9 @@ int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double*
         CK(dalloc_raw(h, &h->zdg, std::max<int64_t>(nvo, 1) * 8));
     }
     const int nch = zvec_chunks(nvo);
-    const int64_t need = (2 * (int64_t)maxdim + nch + 1) * std::max<int64_t>(nvo, 1);
+    // the Z-vector work, and before it the partial sums of the X contractions
+    const int64_t need = std::max({(2 * (int64_t)maxdim + nch + 1) * std::max<int64_t>(nvo, 1),
+                                   2 * noot * v, 2 * nvvt * o, (int64_t)1});
     if (h->zwork_n < need) {
         CK(free_tracked(h, h->zwork));
         h->zwork = nullptr;
@@ -3244,7 +3254,9 @@ int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double*
         CK(dalloc(h, &h->zwork, need));
         h->zwork_n = need;
     }
-    HIPCHK(h, hipEventRecord(h->ev0, st));
+    for (int k = 0; k < 8; ++k)
+        if (!h->mev[k]) HIPCHK(h, hipEventCreate(&h->mev[k]));
+    HIPCHK(h, hipEventRecord(h->mev[0], st));
     double2* T1m = h->mp2t;
     double2* T2m = h->mp2t + nt1;
     double2* DOO = h->mp2d;
@@ -3274,21 +3286,25 @@ int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double*
         CK(gemm(h, OP_N, OP_C, v, v, o, cplx(1, 0), T1m, v, T1m, v, cplx(1, 0), DVV, v));
         k_conj((int64_t)v * v, DVV, DVV, st);
     }
+    HIPCHK(h, hipEventRecord(h->mev[1], st));
     // CCDENSZ's X intermediate (ccgrad.F:199-229)
     k_fill_zero(XVO, nvo, st);
     // T1EQN1 with tau -> L: X(A,I) += V(K,L,C,I) L(A,C,K,L)   (F1 holds L expanded)
     CK(gemm(h, OP_N, OP_N, v, o, (int)(v * noot), cplx(1, 0), h->F1, v, h->M1, v * noot,
             cplx(1, 0), XVO, v));
     // T1EQN2 with tau -> L: X(A,I) += V(A,K,C,D) L(C,D,I,K)
+    HIPCHK(h, hipEventRecord(h->mev[6], st));
     CK(gemm(h, OP_N, OP_N, (int)nvo, (int)noot, (int)nvvt, cplx(1, 0), h->cls[RCC_VOVV], nvo, T2m,
             nvvt, cplx(0, 0), h->X, nvo));
+    HIPCHK(h, hipEventRecord(h->mev[7], st));
     h->x_ok = false;     // h->X no longer holds VOVV tau
     k_trace_t1eqn2(h->X, XVO, h->oo, v, o, st);
     // X(A,I) += V(IK,AL) D(K,L)   (SRT1ST4 + XGEMV)
-    k_x_vooo_doo(h->cls[RCC_VOOO], DOO, XVO, v, o, st);
+    k_x_vooo_doo(h->cls[RCC_VOOO], DOO, XVO, h->oo, h->zwork, v, o, st);
     // X(A,I) += W(CI,DA) D(C,D)   (SRT20D, left-hand contraction)
-    k_x_vovv_dvv(h->cls[RCC_VOVV], DVV, XVO, v, o, st);
+    k_x_vovv_dvv(h->cls[RCC_VOVV], DVV, XVO, h->vv, h->zwork, v, o, st);
     k_axpby(nvo, cplx(0, 0), XVO, cplx(-1, 0), XVO, st);
+    HIPCHK(h, hipEventRecord(h->mev[2], st));
     // RPAA at zero frequency (ccrpa.F:104-229): X1 = XVO / DG, then
     // X2 = (XVO + A' X1 - B conj X1) / DG, DIIS, until RMS < 10^-ntol
     k_zvec_dg(h->VOVO_R, h->eps, h->zdg, v, o, st);
@@ -3304,22 +3320,34 @@ int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double*
     z.md = maxdim;
     z.bb2.assign((size_t)(maxdim + 1) * (maxdim + 2), 0.0);
     const double tol = std::pow(10.0, -ntol);
-    double r = 1.0;
+    double r = 1.0, gemv_ms = 0.0;
     int it = 0;
     while (nvo > 0 && it < maxit) {
         ++it;
+        HIPCHK(h, hipEventRecord(h->mev[4], st));
         k_zvec_gemv(h->VOVO_R, h->VVOO_R, DVO, part, nvo, st);
+        HIPCHK(h, hipEventRecord(h->mev[5], st));
         k_zvec_step(part, XVO, h->VOVO_R, DVO, h->zdg, X2, nvo, st);
-        CK(zdiis_step(h, z, zerr.data(), zamp.data(), DVO, X2, nvo, &r));
+        CK(zdiis_step(h, z, zerr.data(), zamp.data(), DVO, X2, nvo, &r));   // synchronises
+        float gms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&gms, h->mev[4], h->mev[5]));
+        gemv_ms += gms;
         k_copy(nvo, X2, DVO, st);
         if (r < tol) break;
     }
     k_conj(nvo, DVO, DOV, st);
-    HIPCHK(h, hipEventRecord(h->ev1, st));
-    HIPCHK(h, hipEventSynchronize(h->ev1));
-    float ms = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->last_mp2d_ms = ms;
+    HIPCHK(h, hipEventRecord(h->mev[3], st));
+    HIPCHK(h, hipEventSynchronize(h->mev[3]));
+    float ms[3] = {0.f, 0.f, 0.f}, tot = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&tot, h->mev[0], h->mev[3]));
+    for (int k = 0; k < 3; ++k) HIPCHK(h, hipEventElapsedTime(&ms[k], h->mev[k], h->mev[k + 1]));
+    h->mp2_ms[0] = tot;
+    for (int k = 0; k < 3; ++k) h->mp2_ms[k + 1] = ms[k];
+    h->mp2_ms[4] = gemv_ms;
+    float xg = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&xg, h->mev[6], h->mev[7]));
+    h->mp2_ms[5] = xg;
+    h->last_mp2d_ms = tot;
     h->mp2_nit = it;
     h->mp2_rms = r;
     h->mp2d_ok = true;
@@ -3329,6 +3357,12 @@ int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double*
 }
 
 double rcc_last_mp2d_ms(rcc_t* h) { return h ? h->last_mp2d_ms : 0.0; }
+
+int rcc_mp2d_timing(rcc_t* h, double* ms) {
+    if (!h || !ms) return -1;
+    for (int k = 0; k < 6; ++k) ms[k] = h->mp2_ms[k];
+    return 0;
+}
 
 int rcc_get_eps(rcc_t* h, double* eps) {
     if (!h || !eps) return -1;

@@ -8,7 +8,15 @@
 !        sorts the MOLFDIR files MRCONEE + MDCINT of the working directory
 !        instead (CCSETI -> RDINTS, C1, NELEC = NO) and writes E0, E1, ECORE,
 !        FOO, FVO, FVV, the six classes and the orbital energies to
-!        ccout.bin, then stops), orbital energies, for NREP > 1 the fermion
+!        ccout.bin, then stops; JOB 5 runs CCFOPR's MP2 block instead of
+!        CCSD, ccdriv.F:2028-2090: after MP2, the MP2 relaxed density with
+!        MAXIT / MAXDIM / NTOL as the Z-vector's MXITGR / MXDIMGR / NTOLGR,
+!        then for each operator named at the end of ccin.bin (int32 NEOPER,
+!        then NEOPER names of 8 characters) RDPROP from MDPROP (spinors
+!        classified by MRCONEE of the working directory, C1) and its SCF and
+!        MP2 expectation values, printed as CCFOPR prints them and written
+!        to ccout.bin (Re, Im SCF, Re, Im MP2 per operator, then the
+!        Z-vector's iterations and RMS)), orbital energies, for NREP > 1 the fermion
 !        irrep of every spinor, the occupied count per irrep and MULTB(.,.,1)
 !        of the 2 NREP irreps (int32, as in MRCONEE), then FOO, FVO, FVV and
 !        the classes OOOO, VOOO, VVOO, VOVO, VOVV, VVVV in relccsd's
@@ -39,6 +47,10 @@ program ccdrv_hip
   integer(c_int) :: nel(1), which, imstat(30)
   real(c_double) :: energ(7)
   integer(c_int64_t) :: narr(12:20)
+  integer(c_int) :: neoper, iop
+  character(len=8), allocatable :: names(:)
+  real(c_double), allocatable :: prop(:)
+  real(c_double) :: v0(2), v2(2)
 
   open (30, file='ccin.bin', access='stream', form='unformatted', status='old')
   read (30) hdr
@@ -137,11 +149,48 @@ program ccdrv_hip
       end do
     end do
   end do
+  if (hdr(3) == 5) then
+    read (30) neoper
+    allocate (names(neoper))
+    do iop = 1, neoper
+      read (30) names(iop)
+    end do
+  end if
   close (30)
 
   ! MP2EQN (cceqns.F:33-139)
   call rcc_check(h, rcc_mp2(h, emp2), 'MP2EQN')
   write (*, '(a,t40,f25.15)') ' MP2 correlation energy :', emp2
+
+  if (hdr(3) == 5) then
+    ! CCFOPR (ccdriv.F:2028-2090): SCF and MP2 relaxed expectation values
+    call rcc_check(h, rcc_mp2_density(h, int(maxit, c_int), int(maxdim, c_int), &
+                   int(ntol, c_int), lnit, lrms), 'CCDENSZ')
+    write (*, '(a,i4,a,es10.3)') ' Z-vector equations: iterations', lnit, '  RMS', lrms
+    allocate (prop(2 * nsp * nsp))
+    nel(1) = no
+    open (32, file='ccout.bin', access='stream', form='unformatted', status='replace')
+    write (*, 1002) '    SCF'
+    do iop = 1, neoper
+      call rcc_check(h, rcc_read_property(h, 'MRCONEE'//c_null_char, 'MDPROP'//c_null_char, &
+                     trim(names(iop))//c_null_char, nel, 1_c_int, prop), 'RDPROP')
+      call rcc_check(h, rcc_expectation(h, prop, 0_c_int, v0), 'EXPVAL')
+      write (*, 1004) names(iop), v0
+    end do
+    write (*, 1002) '    MP2'
+    do iop = 1, neoper
+      call rcc_check(h, rcc_read_property(h, 'MRCONEE'//c_null_char, 'MDPROP'//c_null_char, &
+                     trim(names(iop))//c_null_char, nel, 1_c_int, prop), 'RDPROP')
+      call rcc_check(h, rcc_expectation(h, prop, 0_c_int, v0), 'EXPVAL')
+      call rcc_check(h, rcc_expectation(h, prop, 2_c_int, v2), 'EXPVAL')
+      write (*, 1004) names(iop), v2
+      write (32) v0, v2
+    end do
+    write (32) dble(lnit), lrms
+    close (32)
+    call rcc_destroy(h)
+    stop
+  end if
 
   ! CCEQN (cceqns.F:141-643)
   conv = 10.0d0 ** (-ntol)
@@ -221,4 +270,7 @@ program ccdrv_hip
   close (32)
   call rcc_destroy(h)
 6001 format(2x, i3, 5x, 2f20.15, f10.5)
+1002 format(//, 1x, a7, ' Gradient results', //3x, 'Expectation value', 13x, &
+            'Real part', 8x, 'Imaginary part')
+1004 format('  <', a8, '> ', t30, 2f20.13)
 end program ccdrv_hip

@@ -131,6 +131,34 @@ module relccsd_hip
       integer(c_int), intent(out) :: imstat(30)
       real(c_double), intent(out) :: energies(7)
     end function
+    ! one-electron properties: RDPROP (ccints.F:569-764) of operator `name`
+    ! (C string) into an (no+nv)^2 complex matrix, EXPVAL (ccgrad.F:261-347)
+    ! at level 0 (SCF), 1 (unrelaxed CCSD) or 2 (MP2 relaxed), and CCFOPR's
+    ! MP2 relaxed density (ccdriv.F:2058-2090)
+    integer(c_int) function rcc_read_property(h, mrconee, mdprop, name, nelec, nnelec, prop) &
+        bind(c, name="rcc_read_property")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: mrconee(*), mdprop(*), name(*)
+      integer(c_int), intent(in) :: nelec(*)
+      integer(c_int), value :: nnelec
+      real(c_double), intent(out) :: prop(*)
+    end function
+    integer(c_int) function rcc_expectation(h, prop, level, val) bind(c, name="rcc_expectation")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: prop(*)
+      integer(c_int), value :: level
+      real(c_double), intent(out) :: val(2)
+    end function
+    integer(c_int) function rcc_mp2_density(h, maxit, maxdim, ntol, nit, rms) &
+        bind(c, name="rcc_mp2_density")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: maxit, maxdim, ntol
+      integer(c_int), intent(out) :: nit
+      real(c_double), intent(out) :: rms
+    end function
     ! non-iterative triples of T3CORR (cceqns.F:1336-1863): ET1, ET2, ET3
     integer(c_int) function rcc_t3corr(h, et) bind(c, name="rcc_t3corr")
       import :: c_int, c_ptr, c_double

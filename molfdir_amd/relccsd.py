@@ -169,6 +169,7 @@ def load():
     lib.rcc_mp2_density.argtypes = [vp, ci, ci, ci, ctypes.POINTER(ci), dp]
     lib.rcc_last_mp2d_ms.argtypes = [vp]
     lib.rcc_last_mp2d_ms.restype = ctypes.c_double
+    lib.rcc_mp2d_timing.argtypes = [vp, dp]
     lib.rcc_t3corr.argtypes = [vp, dp]
     lib.rcc_t3corr_part.argtypes = [vp, i64, i64, dp]
     lib.rcc_t3_triples.argtypes = [vp]
@@ -536,7 +537,11 @@ class RelCCSD:
         return nit.value, rms.value
 
     def last_mp2d_ms(self):
-        return float(self.lib.rcc_last_mp2d_ms(self.h))
+        """Device ms of the last mp2_density: total and its phases."""
+        v = (ctypes.c_double * 6)()
+        self._check(self.lib.rcc_mp2d_timing(self.h, v))
+        return {"total": v[0], "densities": v[1], "x": v[2], "zvector": v[3], "zgemv": v[4],
+                "xgemm": v[5]}
 
     def expectation(self, prop, level=1):
         """EXPVAL: level 0 reference determinant, 1 unrelaxed CCSD (Lambda),

@@ -29,7 +29,8 @@ C  2058-2090) on the given classes: MP2EQN with EQNS = 'LAMBDA',
 C  CCDENSC(1), CCDENSZ(1) (the X intermediate and the RPAA Z-vector
 C  solve, ccgrad.F:141, ccrpa.F:34) and EXPVAL of operator 'PROPTEST';
 C  it writes the MP2 amplitudes, DOO, DVV, -X, the Z-vector (DVO, DOV),
-C  the RPA preconditioner DG and the expectation value.
+C  the RPA preconditioner DG, the expectation value and the CPU time of
+C  the chain (FMTOFILE .. EXPVAL).
 C  Mode 5 copies the records of MCCRES.in (a restart file written by the
 C  caller) into MCCRES after CCDINI and reads it back with GETAMPT
 C  (ccgetv.F:643-709) and the status / energy records CCDUMP writes;
@@ -77,6 +78,7 @@ C
       INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16)
       CHARACTER*200 LINE
       LOGICAL BRT,DONE
+      REAL TC0,TC1
 C
       MASTER = 0
       MYPROC = 0
@@ -288,6 +290,7 @@ C
          PBB1 = PCC1 + MAXDIM + 8
          PBB2 = PBB1 + (MAXDIM+1)*(MAXDIM+2) + 8
          PNAME = 'PROPTEST'
+         CALL CPU_TIME(TC0)
          CALL FMTOFILE (.FALSE.,FVO,FOO,FVV)
          CALL MP2EQN (EPS,FVO,T1,S1,T2,S2)
          CALL CCDENSC (1,T1,S1,T2,S2,BUF1,BUF2,BUF3,ZD(PDOO),ZD(PDVV),
@@ -308,7 +311,8 @@ C
      &                AOO,AVV)
          CALL EXPVAL (ZD(PDOO),ZD(PDVV),ZD(PDVO),ZD(PDOV),AOO,AVV,AVO,
      &                AOV,EPH,CEXPA)
-         WRITE (32) DBLE(CEXPA),DIMAG(CEXPA),DBLE(NDIMX)
+         CALL CPU_TIME(TC1)
+         WRITE (32) DBLE(CEXPA),DIMAG(CEXPA),DBLE(NDIMX),DBLE(TC1-TC0)
 C
       ELSEIF (MODE.EQ.5) THEN
 C

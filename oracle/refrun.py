@@ -242,7 +242,8 @@ def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, t
     on operator 'PROPTEST' of an MDPROP file.  C1 systems.  Returns the MP2
     amplitudes, DOO / DVV after CCDENSC(1) (packed, reference layout), the
     X intermediate as CCDENSZ leaves it (-X), the Z-vector DVO / DOV that
-    RPAA converged, its preconditioner DG and EXPVAL's value."""
+    RPAA converged, its preconditioner DG, EXPVAL's value and the CPU seconds
+    of the chain (cpu_s)."""
     import shutil
     no, nv = sysd["no"], sysd["nv"]
     nt1, nt2 = no * nv, (nv * (nv - 1) // 2) * (no * (no - 1) // 2)
@@ -254,7 +255,7 @@ def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, t
         if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
             raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
         raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
-    ndimx = int(round(raw[-1]))
+    ndimx = int(round(raw[-2]))
     out, pos = {}, 0
     for name, n in (("T1", nt1), ("T2", nt2), ("DOO", no * no), ("DVV", nv * nv),
                     ("XVO", ndimx), ("DVO", ndimx), ("DOV", ndimx)):
@@ -263,5 +264,6 @@ def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, t
     out["DG"] = raw[pos:pos + ndimx].copy()
     pos += ndimx
     out["expval"] = complex(raw[pos], raw[pos + 1])
+    out["cpu_s"] = float(raw[-1])
     out["log"] = log
     return out
