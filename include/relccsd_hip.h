@@ -46,10 +46,12 @@ enum {
     RCC_ARR_FOO = 12, RCC_ARR_FVO = 13, RCC_ARR_FVV = 14, RCC_ARR_OOOO = 15,
     RCC_ARR_VOOO = 16, RCC_ARR_VVOO = 17, RCC_ARR_VOVO = 18, RCC_ARR_VOVV = 19,
     RCC_ARR_VVVV = 20,
-    /* after rcc_mp2_density (C1): the density blocks DOO (o x o) and DVV
-     * (v x v), column-major, as CCDENSC(1) leaves them; the X intermediate as
-     * CCDENSZ stores it (-X, a VO vector, index i*nv + a); the Z-vector DVO
-     * and DOV (VO vectors) as RPAA leaves them; the MP2 amplitudes */
+    /* after rcc_mp2_density: the density blocks DOO (o x o) and DVV (v x v),
+     * column-major, as CCDENSC(1) leaves them; the X intermediate as CCDENSZ
+     * stores it (-X, a VO vector, index i*nv + a); the Z-vector DVO and DOV
+     * (VO vectors) as RPAA leaves them (its totally symmetric block); the MP2
+     * amplitudes.  Symmetry-blocked handles: the FOO, FVV, T1 and T2
+     * layouts */
     RCC_ARR_DOO = 21, RCC_ARR_DVV = 22, RCC_ARR_XVO = 23, RCC_ARR_DVO = 24,
     RCC_ARR_DOV = 25, RCC_ARR_MP2T1 = 26, RCC_ARR_MP2T2 = 27
 };
@@ -280,8 +282,8 @@ int rcc_expectation(rcc_t* h, const double* prop, int level, double* value);
 /* ---- MP2 relaxed density: the DOMP2G block of CCFOPR (ccdriv.F:2058-2090),
  * replacing MP2EQN with EQNS = 'LAMBDA' (cceqns.F:33), CCDENSC(1)
  * (ccgrad.F:34) and CCDENSZ(1) (ccgrad.F:141) with its Z-vector solve RPAA
- * (ccrpa.F:34) at zero frequency.  C1 handles, on the handle's integrals and
- * orbital energies (its amplitudes are not touched).  maxit / maxdim / ntol
+ * (ccrpa.F:34) at zero frequency, on the handle's integrals and orbital
+ * energies (its amplitudes are not touched); C1 or symmetry-blocked.  maxit / maxdim / ntol
  * are RPAA's MXITGR / MXDIMGR / NTOLGR (converged when the DIIS RMS drops
  * below 10^-ntol; maxdim <= 16).  nit, rms (may be null): iterations taken,
  * last RMS.  Afterwards rcc_expectation level 2 gives the MP2 relaxed value

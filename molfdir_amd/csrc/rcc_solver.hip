@@ -2629,6 +2629,18 @@ int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
                 case RCC_ARR_MP2T1: src = h->mp2t; len = h->nt1; break;
                 default: src = h->mp2t + h->nt1; len = h->nt2; break;
             }
+            if (h->sym.nrep > 1) {
+                const int lay = which == RCC_ARR_DOO   ? RCC_LAY_FOO
+                                : which == RCC_ARR_DVV ? RCC_LAY_FVV
+                                : which == RCC_ARR_MP2T2 ? RCC_LAY_T2 : RCC_LAY_T1;
+                const int64_t need = lay_total(h, lay);
+                if (n != need) {
+                    h->err = "rcc_get_array: size mismatch (expected " + std::to_string(need) + ")";
+                    return -1;
+                }
+                HIPCHK(h, hipStreamSynchronize(h->st));
+                return c1_to_sym_host(h, lay, src, out);
+            }
             if (n != len) {
                 h->err = "rcc_get_array: size mismatch (expected " + std::to_string(len) + ")";
                 return -1;
@@ -3226,10 +3238,10 @@ int zdiis_step(rcc_t* h, ZDiis& z, double2** err, double2** amp, const double2* 
 int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double* rms) {
     if (!h) return -1;
     h->mp2d_ok = false;
-    if (h->sym.nrep > 1) {
-        h->err = "rcc_mp2_density: C1 handles only";
-        return -1;
-    }
+    // Symmetry-blocked handles run in the C1 working layout, where every
+    // class but VVVV is held with its symmetry-forbidden elements zero: the
+    // densities and X come out zero outside their totally symmetric blocks,
+    // and the Z-vector's right-hand side, hence its solution, too.
     if (maxit < 1 || maxdim < 1 || maxdim > 16 || ntol < 1 || ntol > 16) {
         h->err = "rcc_mp2_density: maxit >= 1, 1 <= maxdim <= 16, 1 <= ntol <= 16";
         return -1;

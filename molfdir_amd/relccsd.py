@@ -81,7 +81,9 @@ LAYOUTS = {"OOOO": 0, "VOOO": 1, "VVOO": 2, "VOVO": 3, "VOVV": 4, "VVVV": 5,
 ARRAY_LAYOUT = {"HO": "FOO", "GO": "FOO", "HV": "FVV", "GV": "FVV", "HOV": "T1", "S1": "T1",
                 "T1": "T1", "AZ": "OOOO", "H": "VOVO", "S2": "T2", "T2": "T2", "TAU": "T2",
                 "FOO": "FOO", "FVO": "T1", "FVV": "FVV", "OOOO": "OOOO", "VOOO": "VOOO",
-                "VVOO": "VVOO", "VOVO": "VOVO", "VOVV": "VOVV", "VVVV": "VVVV"}
+                "VVOO": "VVOO", "VOVO": "VOVO", "VOVV": "VOVV", "VVVV": "VVVV", "DOO": "FOO",
+                "DVV": "FVV", "XVO": "T1", "DVO": "T1", "DOV": "T1", "MP2T1": "T1",
+                "MP2T2": "T2"}
 
 
 def xor_multb(nrep: int) -> np.ndarray:

@@ -200,15 +200,21 @@ def make_system(no: int, nv: int, seed: int, rank: int = 120,
     return out
 
 
-def hermitian_perturbation(n: int, seed: int, sigma: float = 0.01) -> np.ndarray:
+def hermitian_perturbation(n: int, seed: int, sigma: float = 0.01, irreps=None) -> np.ndarray:
     """Seeded Hermitian one-electron perturbation V (n x n, spinors occupied
     then virtual): V_pq (p < q) with Re, Im ~ N(0, sigma^2), V_qp its
-    conjugate, V_pp ~ N(0, sigma^2)."""
+    conjugate, V_pp ~ N(0, sigma^2).  irreps (the irrep of each spinor):
+    V_pq = 0 between spinors of different irreps (a totally symmetric
+    operator)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     re = rng.normal(0.0, sigma, (n, n))
     im = rng.normal(0.0, sigma, (n, n))
     up = np.triu(re + 1j * im, 1)
-    return up + up.conj().T + np.diag(rng.normal(0.0, sigma, n))
+    v = up + up.conj().T + np.diag(rng.normal(0.0, sigma, n))
+    if irreps is not None:
+        g = np.asarray(irreps)
+        v[g[:, None] != g[None, :]] = 0.0
+    return v
 
 
 def _pair_c1(x, y, n):

@@ -239,14 +239,15 @@ def read_property(sysd, mdprop_path, workdir=None, timeout=600):
 
 def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, timeout=600):
     """Mode 7: the MP2 relaxed property block of CCFOPR (ccdriv.F:2058-2090)
-    on operator 'PROPTEST' of an MDPROP file.  C1 systems.  Returns the MP2
+    on operator 'PROPTEST' of an MDPROP file (reference layouts; the VO
+    vectors over all NDIMX pairs, totally symmetric block first).  Returns the MP2
     amplitudes, DOO / DVV after CCDENSC(1) (packed, reference layout), the
     X intermediate as CCDENSZ leaves it (-X), the Z-vector DVO / DOV that
     RPAA converged, its preconditioner DG, EXPVAL's value and the CPU seconds
     of the chain (cpu_s)."""
     import shutil
-    no, nv = sysd["no"], sysd["nv"]
-    nt1, nt2 = no * nv, (nv * (nv - 1) // 2) * (no * (no - 1) // 2)
+    lay = layout_of(sysd)
+    nt1, nt2, nfoo, nfvv = lay.size("T1"), lay.size("T2"), lay.size("FOO"), lay.size("FVV")
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         _write_input(os.path.join(d, "ccin.bin"), sysd, 7, maxit, maxdim, ntol)
         shutil.copy(mdprop_path, os.path.join(d, "MDPROP"))
@@ -257,7 +258,7 @@ def mp2_property(sysd, mdprop_path, maxit=60, maxdim=8, ntol=13, workdir=None, t
         raw = np.fromfile(os.path.join(d, "ccout.bin"), dtype=np.float64)
     ndimx = int(round(raw[-2]))
     out, pos = {}, 0
-    for name, n in (("T1", nt1), ("T2", nt2), ("DOO", no * no), ("DVV", nv * nv),
+    for name, n in (("T1", nt1), ("T2", nt2), ("DOO", nfoo), ("DVV", nfvv),
                     ("XVO", ndimx), ("DVO", ndimx), ("DOV", ndimx)):
         out[name] = raw[pos:pos + 2 * n].view(np.complex128).copy()
         pos += 2 * n

@@ -209,23 +209,34 @@ MP2G_CASES = [
 ]
 
 
-def mp2g_case(name, no, nv, seed, scale, fp, pseed, maxdim, ntol):
+MP2G_SYM_CASES = [
+    # name, occupied per irrep, virtual per irrep, seed, scale, fock_perturb, operator seed,
+    # MAXDIM, NTOL
+    ("s2_6o20v", [3, 3], [9, 11], 41, 0.05, 0.03, 24, 8, 13),
+    ("s4_7o24v", [3, 2, 1, 1], [7, 6, 6, 5], 42, 0.05, 0.02, 25, 8, 13),
+]
+
+
+def mp2g_case(name, no, nv, seed, scale, fp, pseed, maxdim, ntol, sym=None):
     """relccsd's MP2 relaxed property chain (harness mode 7): MP2 lambda,
     CCDENSC(1), CCDENSZ(1) with the RPAA Z-vector, EXPVAL of a seeded
-    Hermitian operator written as MDPROP 'PROPTEST'."""
+    Hermitian operator written as MDPROP 'PROPTEST' (totally symmetric for
+    a symmetric system)."""
     import tempfile
     from molfdir_amd import mdcint
-    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
-    P = synth.hermitian_perturbation(no + nv, pseed, 0.1)
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=sym)
+    P = synth.hermitian_perturbation(no + nv, pseed, 0.1,
+                                     irreps=None if sym is None else synth.spinor_irreps(sym))
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "MDPROP")
         mdcint.write_mdprop(path, {"OTHER": 2.0 * P, "PROPTEST": P})
         r = refrun.mp2_property(s, path, maxit=100, maxdim=maxdim, ntol=ntol)
+    extra = {} if sym is None else {"sym_no": np.array(sym[0]), "sym_nv": np.array(sym[1])}
     np.savez_compressed(os.path.join(HERE, f"mp2g_{name}.npz"),
                         params=np.array([no, nv, seed, pseed, maxdim, ntol], dtype=np.int64),
                         scale=scale, fock_perturb=fp, T1=r["T1"], DOO=r["DOO"], DVV=r["DVV"],
                         XVO=r["XVO"], DVO=r["DVO"], DOV=r["DOV"], DG=r["DG"],
-                        expval=r["expval"])
+                        expval=r["expval"], **extra)
     print(name, r["expval"])
 
 
@@ -324,6 +335,9 @@ def main():
     if "--mp2g-only" in sys.argv:
         for case in MP2G_CASES:
             mp2g_case(*case)
+        for name, sno, snv, seed, scale, fp, pseed, maxdim, ntol in MP2G_SYM_CASES:
+            mp2g_case(name, sum(sno), sum(snv), seed, scale, fp, pseed, maxdim, ntol,
+                      sym=(sno, snv))
         return
     if "--prop-only" in sys.argv:
         for case in PROP_CASES:
