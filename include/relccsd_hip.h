@@ -298,6 +298,20 @@ int rcc_mp2d_timing(rcc_t* h, double* ms);
 /* The handle's orbital energies (no + nv doubles, occupied then virtual). */
 int rcc_get_eps(rcc_t* h, double* eps);
 
+/* ---- MRCONEE writer (host only, no GPU): records 1-6 as RDSYMI
+ * (ccdriv.F:2978-2988) and GETFM (ccints.F:1091-1096) read them, i.e. the
+ * file tmoone's MFDTRAN writes for relccsd (tmoone/mfdtran.F:219-226): NSP,
+ * BREIT, ECORE; the irrep names and occupied count per irrep (nelec, nrep
+ * ints); the abelian irrep names; MULTB(1:2NREP, 1:2NREP, 1) (column-major;
+ * null: the C1 table, nrep = 1); per spinor its irrep (1..nrep, also as the
+ * higher-group irrep) and orbital energy; the NSP x NSP one-electron matrix
+ * h1 (complex), record 6 holding h1(p,q) at p + NSP q, which is how GETFM
+ * reads it (MFDTRAN writes ONE(i,j) with j fastest, so h1 = ONE^T gives its
+ * file).  Errors through rcc_file_error(). */
+int rcc_mrconee_write(const char* path, int nsp, int nrep, const int* irreps, const int* nelec,
+                      const int* multb, const double* eps, const double* h1, double ecore,
+                      int breit);
+
 /* ---- MCCRES restart file: PUTAMPT / GETAMPT (ccgetv.F:577-709) and the
  * status / energy records of CCDUMP (ccdriv.F:2767-2816).  Direct access,
  * 8192-byte records (RSRCLEN = 1024 REAL*8, IRECLEN for LINUX): record 1

@@ -560,6 +560,91 @@ int64_t nrec(int64_t ndim) { return ndim * 2 / RSRCLEN + 1; }
 
 extern "C" const char* rcc_file_error(void) { return g_file_err.c_str(); }
 
+// MRCONEE records 1-6 as RDSYMI (ccdriv.F:2978-2988) and GETFM (ccints.F:1091-1096)
+// read them; the higher point group is written equal to the abelian one
+// (names A01.. / a01..), as for the files the reference test inputs use.
+extern "C" int rcc_mrconee_write(const char* path, int nsp, int nrep, const int* irreps,
+                                 const int* nelec, const int* multb, const double* eps,
+                                 const double* h1, double ecore, int breit) {
+    if (!path || nsp < 1 || nrep < 1 || nrep > 16 || !irreps || !nelec || !eps || !h1 ||
+        (!multb && nrep != 1)) {
+        g_file_err = "rcc_mrconee_write: bad arguments (multb may be null only for nrep = 1)";
+        return -1;
+    }
+    for (int s = 0; s < nsp; ++s)
+        if (irreps[s] < 1 || irreps[s] > nrep) {
+            g_file_err = "rcc_mrconee_write: spinor irrep out of 1..nrep";
+            return -1;
+        }
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        g_file_err = std::string("cannot create ") + path;
+        return -1;
+    }
+    bool ok = true;
+    auto record = [&](const std::vector<unsigned char>& b) {
+        const int32_t n = (int32_t)b.size();
+        ok = ok && fwrite(&n, 4, 1, f) == 1 && (b.empty() || fwrite(b.data(), 1, b.size(), f) == b.size()) &&
+             fwrite(&n, 4, 1, f) == 1;
+    };
+    auto put = [](std::vector<unsigned char>& b, const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        b.insert(b.end(), c, c + n);
+    };
+    std::vector<unsigned char> b;
+    const int32_t nsp32 = nsp, brt = breit ? 1 : 0, nrep32 = nrep;
+    put(b, &nsp32, 4);
+    put(b, &brt, 4);
+    put(b, &ecore, 8);
+    record(b);
+    b.clear();
+    put(b, &nrep32, 4);
+    char name[32];
+    for (int i = 0; i < nrep; ++i) {
+        std::string nm(14, ' ');                                      // character*14
+        snprintf(name, sizeof name, "A%02d", i + 1);
+        nm.replace(0, std::strlen(name), name);
+        put(b, nm.data(), 14);
+    }
+    for (int i = 0; i < nrep; ++i) {
+        const int32_t ne = nelec[i];
+        put(b, &ne, 4);
+    }
+    record(b);
+    b.clear();
+    put(b, &nrep32, 4);
+    for (int half = 0; half < 2; ++half)
+        for (int i = 0; i < nrep; ++i) {
+            snprintf(name, sizeof name, " %c%02d", half ? 'a' : 'A', i + 1);  // character*4
+            put(b, name, 4);
+        }
+    record(b);
+    b.clear();
+    for (int j = 0; j < 2 * nrep; ++j)
+        for (int i = 0; i < 2 * nrep; ++i) {
+            const int32_t m = multb ? multb[i + 2 * nrep * j] : (i == j ? 2 : 1);
+            put(b, &m, 4);
+        }
+    record(b);
+    b.clear();
+    for (int sp = 0; sp < nsp; ++sp) {
+        const int32_t g = irreps[sp];
+        put(b, &g, 4);
+        put(b, &g, 4);
+        put(b, &eps[sp], 8);
+    }
+    record(b);
+    b.clear();
+    put(b, h1, (size_t)nsp * nsp * 16);
+    record(b);
+    if (fclose(f) != 0) ok = false;
+    if (!ok) {
+        g_file_err = std::string("write error on ") + path;
+        return -1;
+    }
+    return 0;
+}
+
 extern "C" int rcc_mccres_write(const char* path, int64_t ndimt1, int64_t ndimt2, const double* t1,
                                 const double* t2, const int* imstat, const double* energies) {
     FILE* f = fopen(path, "wb");

@@ -52,6 +52,25 @@ def mccres_write(path, t1, t2, imstat=None, energies=None):
         raise RccError(lib.rcc_file_error().decode())
 
 
+def mrconee_write(path, eps, irreps, nelec, h1, multb=None, ecore=0.0, breit=False):
+    """MRCONEE records 1-6 (RDSYMI / GETFM layout) through the library's
+    writer: irreps 1-based per spinor, nelec per irrep, h1 NSP x NSP complex,
+    multb the 2NREP x 2NREP table (None: C1).  No GPU needed."""
+    lib = load()
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    nsp = eps.size
+    irr = np.ascontiguousarray(irreps, dtype=np.int32)
+    ne = np.ascontiguousarray(nelec, dtype=np.int32)
+    h = np.asfortranarray(np.asarray(h1, dtype=np.complex128).reshape(nsp, nsp))
+    mb = None if multb is None else np.asfortranarray(multb, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    if lib.rcc_mrconee_write(str(path).encode(), nsp, ne.size, irr.ctypes.data_as(ip),
+                             ne.ctypes.data_as(ip), None if mb is None else mb.ctypes.data_as(ip),
+                             _dptr(eps), _dptr(h.reshape(-1, order="F").view(np.float64)),
+                             float(ecore), 1 if breit else 0) != 0:
+        raise RccError(lib.rcc_file_error().decode())
+
+
 def mccres_read(path, nt1, nt2):
     """(T1, T2, IMSTAT, energies) of a relccsd restart file (GETAMPT layout)."""
     lib = load()
@@ -156,6 +175,9 @@ def load():
     lib.rcc_mccres_write.argtypes = [ctypes.c_char_p, i64, i64, dp, dp, ctypes.POINTER(ci), dp]
     lib.rcc_mccres_read.argtypes = [ctypes.c_char_p, i64, i64, dp, dp, ctypes.POINTER(ci), dp]
     lib.rcc_file_error.restype = ctypes.c_char_p
+    lib.rcc_mrconee_write.argtypes = [ctypes.c_char_p, ci, ci, ctypes.POINTER(ci),
+                                      ctypes.POINTER(ci), ctypes.POINTER(ci), dp, dp,
+                                      ctypes.c_double, ci]
     lib.rcc_write_mccres.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci), dp]
     lib.rcc_read_mccres.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci), dp]
     lib.rcc_read_molfdir.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p,

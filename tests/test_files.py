@@ -8,6 +8,8 @@ relccsd's CCENER wrote (tests/golden/mccres_*.bin, C1 and 4 irreps).
   them (the status and energy prefixes, the T1 and T2 blocks); relccsd's
   GETAMPT was run on files of this writer when the fixtures were made.
 * A file too short for the layout is refused.
+* The library's MRCONEE writer reproduces the files relccsd reads in the
+  integral-sort fixtures, byte for byte.
 * The MDPROP writer's files are read by relccsd's RDPROP as the operator
   written (tests/golden/prop_*.npz).
 """
@@ -76,3 +78,27 @@ def test_reference_rdprop_reads_the_mdprop_writer(name):
     assert np.array_equal(g["AVO"], P[no:, :no].T.reshape(-1))
     assert np.array_equal(g["AOV"], P[:no, no:].reshape(-1))
     assert abs(complex(g["expval"]) - np.trace(P[:no, :no])) < 1e-14
+
+
+@pytest.mark.parametrize("nrep", [1, 2, 4])
+def test_mrconee_writer_matches_the_files_the_reference_reads(nrep, tmp_path):
+    """The library's MRCONEE writer gives, byte for byte, the files written by
+    molfdir_amd.mdcint.write_mrconee, which relccsd's RDSYMI / GETFM read in
+    the integral-sort fixtures (tests/golden/sort_*.npz, symsort_*.npz)."""
+    from molfdir_amd import mdcint
+    from molfdir_amd.relccsd import mrconee_write
+    rng = np.random.default_rng(nrep)
+    nsp = 12 * nrep
+    irreps = rng.integers(1, nrep + 1, nsp).astype(np.int32)
+    nelec = np.array([int(np.sum(irreps == r + 1)) // 3 for r in range(nrep)], np.int32)
+    eps = rng.uniform(-2, 5, nsp)
+    h1 = rng.normal(size=(nsp, nsp)) + 1j * rng.normal(size=(nsp, nsp))
+    h1 = h1 + h1.conj().T
+    multb = None if nrep == 1 else mdcint.abelian_multb(nrep)
+    mdcint.write_mrconee(str(tmp_path / "py"), eps, irreps, nelec, h1, multb=multb, ecore=-3.25,
+                         breit=nrep == 2)
+    mrconee_write(tmp_path / "lib", eps, irreps, nelec, h1, multb=multb, ecore=-3.25,
+                  breit=nrep == 2)
+    assert (tmp_path / "lib").read_bytes() == (tmp_path / "py").read_bytes()
+    with pytest.raises(RccError):
+        mrconee_write(tmp_path / "bad", eps, irreps + nrep, nelec, h1, multb=multb)
