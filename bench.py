@@ -41,6 +41,13 @@ SORT_CPU_SAMPLE = (64, 32)   # relccsd RDINTS sample: Kramers pairs, occupied sp
 MP2G = {"mp2g": (30, 300, 3, "C1 30o/300v complex128")}
 MP2G_CPU_SAMPLE = (16, 120)  # relccsd CCFOPR MP2 block sample: occupied, virtual
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
+# HBM bytes per launch (read + write) of the roofline kernels at 30o/300v, from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs scaled by what the same
+# counters read for a 1 GiB device copy (x2.000, x1.000): profiles/*_pmc.md
+TRAFFIC = {"ladder": 217.08e9, "mp2g_xgemm": 23.57e9, "mp2g_zgemv": 2.605e9}
+TRAFFIC_SRC = {"ladder": "profiles/zgemm_hbm_traffic_cfg3_30o300v_pmc.md",
+               "mp2g_xgemm": "profiles/mp2g_30o300v_pmc.md",
+               "mp2g_zgemv": "profiles/mp2g_30o300v_pmc.md"}
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
 CPU_SAMPLE = (10, 100)    # relccsd reference sample size (o, v)
@@ -478,11 +485,15 @@ def bench_mp2g(args):
         "tflops": (fg + fz) / (avg["total"] * 1e-3) / 1e12, "flops": fg + fz,
         "roofline": {"bound": "mfma", "kernel": "zgemm VOVV x lambda (T1EQN2 product of X)",
                      "achieved": x_ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": x_ach / PEAK_FP64_MFMA, "traffic": None,
+                     "frac": x_ach / PEAK_FP64_MFMA,
+                     "traffic": TRAFFIC["mp2g_xgemm"] if (no, nv) == (30, 300) else None,
+                     "traffic_source": TRAFFIC_SRC["mp2g_xgemm"],
                      "algorithmic_flops": xflops},
         "roofline_zvector": {"bound": "hbm", "kernel": "zvec_gemv_kernel (A and B read once)",
                              "achieved": z_ach, "peak": 8000.0, "unit": "GB/s",
-                             "frac": z_ach / 8000.0, "algorithmic_bytes_per_launch": zbytes},
+                             "frac": z_ach / 8000.0, "algorithmic_bytes_per_launch": zbytes,
+                             "traffic": TRAFFIC["mp2g_zgemv"] if (no, nv) == (30, 300) else None,
+                             "traffic_source": TRAFFIC_SRC["mp2g_zgemv"]},
         "cpu_baseline": None if args.no_cpu_baseline else mp2g_cpu_baseline(fg + fz),
     }
     print(json.dumps(res), flush=True)
@@ -844,7 +855,9 @@ def main():
                    "algorithmic_flops_per_iter": F},
         "roofline": {"bound": "mfma", "kernel": "VVVV ladder ZGEMM (BINTM)",
                      "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "frac": achieved / PEAK_FP64_MFMA,
+                     "traffic": TRAFFIC["ladder"] if (no, nv) == (30, 300) else None,
+                     "traffic_source": TRAFFIC_SRC["ladder"] if (no, nv) == (30, 300) else None,
                      "flops_per_launch": ladder_flops, "avg_launch_ms": ladder_ms},
         "kernels_ms": {k: sum(x[k] for x in kt) / len(kt) for k in kt[0]},
         "iteration_event_ms": sum(iter_ms) / len(iter_ms),
