@@ -3216,9 +3216,11 @@ int zdiis_step(rcc_t* h, ZDiis& z, double2** err, double2** amp, const double2* 
             BB(i, j) = BB2(i, j);
             xm = std::max(xm, std::fabs(BB(i, j)));
         }
-    if (xm > 0.0)
-        for (int i = 0; i < it; ++i)
-            for (int j = 0; j < it; ++j) BB(i, j) /= xm;
+    // every stored error is zero (e.g. a zero right-hand side): the bordered
+    // system is singular, and the plain step is already the fixed point
+    if (!(xm > 0.0)) return 0;
+    for (int i = 0; i < it; ++i)
+        for (int j = 0; j < it; ++j) BB(i, j) /= xm;
     for (int i = 0; i < it; ++i) {
         BB(i, it) = -1.0;
         BB(it, i) = -1.0;

@@ -107,6 +107,22 @@ def test_repeat_is_bit_identical_and_amplitudes_untouched():
     assert np.array_equal(r.get("T1"), t1) and np.array_equal(r.get("T2"), t2)
 
 
+def test_zero_right_hand_side_gives_zero_zvector():
+    """No VO coupling in the X intermediate (all integrals zero but the Fock
+    diagonal): X = 0, and the Z-vector stays exactly zero."""
+    g = load("mp2g", "g_4o10v")
+    s = system_for(g)
+    for k in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV", "FVO"):
+        s[k] = torch.zeros_like(s[k])
+    r = solver_for(s)
+    nit, rms = r.mp2_density(maxit=10, maxdim=4, ntol=12)
+    assert rms == 0.0 and nit == 1
+    for k in ("XVO", "DVO", "DOV", "DOO", "DVV"):
+        assert not np.any(r.get(k)), k
+    P = synth.hermitian_perturbation(14, 3, 0.1)
+    assert abs(r.expectation(P, level=2) - np.trace(P[:4, :4])) < 1e-15
+
+
 def test_refusals():
     g = load("mp2g", "g_4o10v")
     s = system_for(g)
