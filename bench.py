@@ -561,6 +561,62 @@ def converged_accuracy(no, nv, seed, scale, modes, maxit=40, ntol=9):
     return res
 
 
+CFG2_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "cfg2ref_cfg2_20o200v.npz")
+
+
+def cfg2_vs_cpu():
+    """The plain FP64 path against relccsd's CPU CCSD at C1 20o/200v: the same
+    seeded system (factor std 0.005, at which the iteration converges)
+    converged to NTOL 8 with DIIS 6 on the GPU, compared with the reference
+    run stored in tests/golden/cfg2ref_cfg2_20o200v.npz (energies, iteration
+    count, T1, the largest T2 elements, the T2 norm and seeded projections)."""
+    import numpy as np
+    import torch
+    from molfdir_amd import synth
+    from molfdir_amd.relccsd import RelCCSD
+    if not os.path.exists(CFG2_REF):
+        return None
+    g = np.load(CFG2_REF, allow_pickle=False)
+    no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    sysd = synth.make_system(no, nv, seed=seed, scale=float(g["scale"]), device="cuda")
+    r = RelCCSD(no, nv, maxdim=maxdim, maxit=maxit, ntol=ntol)
+    r.set_system(sysd)
+    del sysd
+    torch.cuda.empty_cache()
+    r.synchronize()
+    t0 = time.perf_counter()
+    o = r.cceqn()
+    wall = time.perf_counter() - t0
+    t1, t2 = r.get_amplitudes()
+    r.close()
+    torch.cuda.empty_cache()
+    rng = np.random.Generator(np.random.PCG64(2024))
+    w = rng.integers(0, 2, (g["T2_proj"].size, t2.size), dtype=np.int8) * 2 - 1
+    top = g["T2_top_index"]
+    res = {
+        "system": f"C1 {no}o/{nv}v, factor std {float(g['scale'])}, seed {seed}, NTOL {ntol}, "
+                  f"DIIS {maxdim}",
+        "gpu": {"emp2": o["emp2"], "ecc": o["eccsd"], "nit": o["nit"], "rms": o["rows"][-1][2],
+                "s_per_iter": wall / max(o["nit"], 1)},
+        "cpu": {"emp2": float(g["emp2"]), "ecc": float(g["eccsd"]), "nit": int(g["nit"]),
+                "s_per_iter": float(g["cceqn_cpu_s"]) / max(int(g["nit"]), 1),
+                "host": "the build container (8 cores; relccsd single-threaded)"},
+        "dEc": abs(o["eccsd"] - float(g["eccsd"])), "dEmp2": abs(o["emp2"] - float(g["emp2"])),
+        "max_dT1": float(np.abs(t1 - g["T1"]).max()),
+        "max_dT2_top": float(np.abs(t2[top] - g["T2_top"]).max()),
+        "T2_top_count": int(top.size),
+        "dT2_norm_rel": abs(np.linalg.norm(t2) - float(g["T2_norm"])) / float(g["T2_norm"]),
+        "max_dT2_proj": float(np.abs(np.array([np.dot(wk.astype(np.float64), t2) for wk in w])
+                                     - g["T2_proj"]).max()),
+    }
+    print(f"[bench] 20o/200v vs relccsd CPU: Ec {o['eccsd']:.12f} (ref {float(g['eccsd']):.12f}, "
+          f"dEc {res['dEc']:.2e}) nit {o['nit']} (ref {int(g['nit'])}) max|dT1| "
+          f"{res['max_dT1']:.2e} max|dT2| over the {top.size} largest {res['max_dT2_top']:.2e}",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def batch_systems(nsys, nrep, seed):
     """Sizes of the batched workload: occupied ~ U{4..16}, virtual ~ U{40..160},
     both split over the irreps with one Dirichlet(2) draw of proportions."""
@@ -912,6 +968,13 @@ def main():
             out["split_best_at_1uEh"] = {"mode": best,
                                          "speedup_vs_fp64": out["split"][best]["speedup_vs_fp64"],
                                          "converged_dEc": acc[best]["dEc"]}
+    if args.config == "cfg2":
+        try:
+            r.close()
+        except Exception:
+            pass
+        torch.cuda.empty_cache()
+        out["vs_cpu_converged"] = cfg2_vs_cpu()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(F)

@@ -35,6 +35,9 @@ change in the generator is caught.
   Lambda / density code is switched off, ccdriv.F:1918-1924).
 * prop_<name>.npz: relccsd's RDPROP blocks of a seeded operator written as
   MDPROP 'PROPTEST', and EXPVAL on zero densities.
+* cfg2ref_<name>.npz: relccsd's CCSD at C1 20o/200v (sigma 0.005) converged to
+  NTOL 8: MP2 and CCSD energies, the iteration table, T1, the largest T2
+  elements with their indices, the T2 norm and seeded +-1 projections of T2.
 * mp2g_<name>.npz: relccsd's MP2 relaxed property chain of CCFOPR (harness
   mode 7): DOO / DVV of CCDENSC(1), the X intermediate and the RPAA Z-vector
   (DVO, DOV, preconditioner DG) of CCDENSZ(1), and EXPVAL of the operator.
@@ -240,6 +243,42 @@ def mp2g_case(name, no, nv, seed, scale, fp, pseed, maxdim, ntol, sym=None):
     print(name, r["expval"])
 
 
+# The plain FP64 comparison at C1 20o/200v: relccsd's CPU CCSD converged to NTOL 8 (DIIS 6).
+# The generator's sigma of 0.05 makes the iteration diverge at this size; 0.005 converges.
+CFG2_CASE = ("cfg2_20o200v", 20, 200, 2, 0.005, 40, 6, 8)
+CFG2_TOPK = 4096        # largest |T2| elements kept, with their packed indices
+CFG2_NPROJ = 16         # seeded +-1 projections of T2
+
+
+def cfg2_projections(n, seed=2024):
+    """The CFG2_NPROJ seeded +-1 weight vectors (rows) over the packed T2."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.integers(0, 2, (CFG2_NPROJ, n), dtype=np.int8) * 2 - 1
+
+
+def cfg2_case(name, no, nv, seed, scale, maxit, maxdim, ntol, full_t2_path=None):
+    """relccsd's CCSD at 20o/200v (about 10^2 s of CPU per iteration).  T2 has
+    3.8 M elements, so the fixture keeps T1, the iteration table, the largest
+    elements of T2, its norm and seeded projections; full_t2_path, if given,
+    receives the whole T2 (not committed)."""
+    s = synth.make_system(no, nv, seed=seed, scale=scale)
+    r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol, timeout=6 * 3600)
+    t2 = r["T2"]
+    if full_t2_path:
+        np.save(full_t2_path, t2)
+    top = np.argsort(-np.abs(t2))[:CFG2_TOPK]
+    w = cfg2_projections(t2.size)
+    it = np.asarray(r["iterations"], dtype=np.float64)
+    np.savez_compressed(os.path.join(HERE, f"cfg2ref_{name}.npz"),
+                        params=np.array([no, nv, seed, maxit, maxdim, ntol], dtype=np.int64),
+                        scale=scale, fock_perturb=0.0, fingerprint=fingerprint(s),
+                        emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
+                        T1=r["T1"], T2_top_index=top.astype(np.int64), T2_top=t2[top],
+                        T2_norm=np.linalg.norm(t2), T2_proj=w @ t2,
+                        cceqn_cpu_s=r["cceqn_cpu_s"] or 0.0)
+    print(name, r["emp2"], r["eccsd"], r["nit"], r["cceqn_cpu_s"])
+
+
 MCCRES_CASES = [("ccsd", "cfg1_6o40v"), ("symccsd", "s4_5o12v")]
 
 
@@ -332,6 +371,9 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--cfg2-only" in sys.argv:
+        cfg2_case(*CFG2_CASE, full_t2_path=os.environ.get("CFG2_FULL_T2"))
+        return
     if "--mp2g-only" in sys.argv:
         for case in MP2G_CASES:
             mp2g_case(*case)
