@@ -143,26 +143,49 @@ int read_mdcint(const char* path, MdcintList& out, std::string& err) {
     std::memcpy(out.kr.data(), b.data() + 22, out.kr.size() * 4);
     out.idx.clear();
     out.val.clear();
+    int continuations = 0;
     while (true) {
         if (!r.next(b, err)) {
-            if (err.empty()) err = std::string(path) + ": no end record (IKR = JKR = 0)";
+            if (err.empty()) err = r.path + ": no end record (IKR = JKR = 0)";
             return -1;
         }
         if (b.size() < 12) {
-            err = std::string(path) + ": short integral record";
+            err = r.path + ": short integral record";
             return -1;
         }
         p = 0;
         const int ikr = get<int32_t>(b, p), jkr = get<int32_t>(b, p), nz = get<int32_t>(b, p);
         if (ikr == 0) {
-            if (jkr != 0) {
-                err = std::string(path) + ": continuation files (IKR = 0, JKR != 0) are not supported";
+            if (jkr == 0) break;
+            // continuation (GET2INTS ccints.F:832-838): the next record names
+            // the file (CHARACTER*8) whose integral records follow, without a
+            // header; it is looked for beside the file that names it
+            if (!r.next(b, err) || b.size() != 8) {
+                if (err.empty()) err = r.path + ": continuation record without a file name";
                 return -1;
             }
-            break;
+            std::string next((const char*)b.data(), 8);
+            next.erase(next.find_last_not_of(' ') + 1);
+            const std::string cur = r.path;
+            const size_t slash = cur.find_last_of('/');
+            const std::string full =
+                (next.empty() || next[0] == '/' || slash == std::string::npos)
+                    ? next : cur.substr(0, slash + 1) + next;
+            if (++continuations > 4096 || next.empty()) {
+                err = cur + ": bad continuation file name or too many continuation files";
+                return -1;
+            }
+            fclose(r.f);
+            r.path = full;
+            r.f = fopen(full.c_str(), "rb");
+            if (!r.f) {
+                err = "cannot open continuation file " + full + " (named in " + cur + ")";
+                return -1;
+            }
+            continue;
         }
         if (nz < 0 || b.size() != 12 + (size_t)nz * 24) {
-            err = std::string(path) + ": integral record size does not match NZ";
+            err = r.path + ": integral record size does not match NZ";
             return -1;
         }
         const size_t base = out.idx.size();

@@ -22,6 +22,9 @@ MDCINT (RDINTS ccints.F:117-118, GET2INTS ccints.F:819-835)
           (IKR JKR | INDK INDL) over Kramers pairs (negative: the barred
           partner), Mulliken order
   end     IKR = JKR = 0
+  continuation (GET2INTS ccints.F:832-838): IKR = 0, JKR != 0, then a record
+          naming the next file (CHARACTER*8), which holds integral records
+          only
 """
 from __future__ import annotations
 
@@ -74,28 +77,47 @@ def write_mdprop(path, operators):
             _record(f, np.ascontiguousarray(m.T).reshape(-1).view(np.float64).tobytes())
 
 
-def write_mdcint(path, kr, ikr, jkr, kkr, lkr, val):
+def write_mdcint(path, kr, ikr, jkr, kkr, lkr, val, parts: int = 1):
     """kr: (NKR, 2) spinor indices (1-based) of the unbarred / barred member
     of each Kramers pair.  The canonical integrals (ikr, jkr | kkr, lkr) are
-    written one record per run of equal (ikr, jkr), in the given order."""
+    written one record per run of equal (ikr, jkr), in the given order.
+    parts > 1 spreads the records over that many files, as GET2INTS reads
+    them (ccints.F:832-838): each file but the last ends with the
+    continuation record (IKR = 0, JKR = 1, NZ = 0) and a record naming the
+    next file (CHARACTER*8: MDCINT1, MDCINT2, .. beside path), which holds
+    integral records only."""
+    import os
     kr = np.asarray(kr, dtype=np.int32)
     ikr, jkr = np.asarray(ikr, np.int32), np.asarray(jkr, np.int32)
     kkr, lkr = np.asarray(kkr, np.int32), np.asarray(lkr, np.int32)
     val = np.asarray(val, dtype=np.complex128)
-    with open(path, "wb") as f:
-        _record(f, b"01Jan2026 " + b"00:00:00" + struct.pack("<i", kr.shape[0]) +
-                kr.astype("<i4").tobytes())
-        n = ikr.size
-        s = 0
-        while s < n:
-            e = s + 1
-            while e < n and ikr[e] == ikr[s] and jkr[e] == jkr[s]:
-                e += 1
-            kl = np.stack([kkr[s:e], lkr[s:e]], axis=1).astype("<i4")
-            _record(f, struct.pack("<iii", int(ikr[s]), int(jkr[s]), e - s) + kl.tobytes() +
+    recs = []
+    n = ikr.size
+    s = 0
+    while s < n:
+        e = s + 1
+        while e < n and ikr[e] == ikr[s] and jkr[e] == jkr[s]:
+            e += 1
+        kl = np.stack([kkr[s:e], lkr[s:e]], axis=1).astype("<i4")
+        recs.append(struct.pack("<iii", int(ikr[s]), int(jkr[s]), e - s) + kl.tobytes() +
                     val[s:e].view(np.float64).astype("<f8").tobytes())
-            s = e
-        _record(f, struct.pack("<iii", 0, 0, 0))
+        s = e
+    parts = max(1, min(int(parts), max(len(recs), 1)))
+    cuts = [round(i * len(recs) / parts) for i in range(parts + 1)]
+    folder = os.path.dirname(os.path.abspath(path))
+    for part in range(parts):
+        name = path if part == 0 else os.path.join(folder, f"MDCINT{part}")
+        with open(name, "wb") as f:
+            if part == 0:
+                _record(f, b"01Jan2026 " + b"00:00:00" + struct.pack("<i", kr.shape[0]) +
+                        kr.astype("<i4").tobytes())
+            for rec in recs[cuts[part]:cuts[part + 1]]:
+                _record(f, rec)
+            if part + 1 < parts:
+                _record(f, struct.pack("<iii", 0, 1, 0))
+                _record(f, f"MDCINT{part + 1}".ljust(8).encode())
+            else:
+                _record(f, struct.pack("<iii", 0, 0, 0))
 
 
 def synthetic_list(nkr: int, seed: int, keep: float = 0.7, scale: float = 0.05):
@@ -122,7 +144,7 @@ def synthetic_list(nkr: int, seed: int, keep: float = 0.7, scale: float = 0.05):
     return ik, jk, kk, lk, val
 
 
-def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7):
+def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, parts: int = 1):
     """C1 MRCONEE + MDCINT in dirpath: NSP = 2 NKR spinors with shuffled
     Kramers pairs, the first nocc spinors (input order) occupied, orbital
     energies ascending, a Hermitian one-electron matrix (diagonal eps,
@@ -138,7 +160,7 @@ def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7):
     h1[np.arange(nsp), np.arange(nsp)] = eps
     ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
     write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, np.ones(nsp, np.int32), [nocc], h1)
-    write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val)
+    write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val, parts=parts)
     return {"nsp": nsp, "nocc": nocc, "kr": kr, "eps": eps, "h1": h1, "ikr": ik, "jkr": jk,
             "kkr": kk, "lkr": lk, "val": val}
 

@@ -25,6 +25,8 @@ change in the generator is caught.
   of seeded synthetic MOLFDIR files (molfdir_amd.mdcint.synthetic_files):
   the six classes, the Fock blocks, the orbital energies, E0, E1, ESCF, and
   the SHA-256 of the MDCINT file the reference read.
+* sortcont_<name>.npz: the same for the list of sort_<name> spread over three
+  files joined by continuation records (MDCINT, MDCINT1, MDCINT2).
 * mccres_<name>.bin / .npz: the MCCRES restart file relccsd's CCENER wrote
   for a ccsd_ / symccsd_ fixture run (PUTAMPT + CCDUMP), with the T1 / T2,
   IMSTAT and energy records GETAMPT reads back; the generator also checks
@@ -35,7 +37,7 @@ change in the generator is caught.
   Lambda / density code is switched off, ccdriv.F:1918-1924).
 * prop_<name>.npz: relccsd's RDPROP blocks of a seeded operator written as
   MDPROP 'PROPTEST', and EXPVAL on zero densities.
-* cfg2ref_<name>.npz: relccsd's CCSD at C1 20o/200v (sigma 0.005) converged to
+* cfg2ref_<name>.npz: relccsd's CCSD at C1 20o/150v (sigma 0.005) converged to
   NTOL 8: MP2 and CCSD energies, the iteration table, T1, the largest T2
   elements with their indices, the T2 norm and seeded +-1 projections of T2.
 * mp2g_<name>.npz: relccsd's MP2 relaxed property chain of CCFOPR (harness
@@ -163,18 +165,24 @@ def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
     print(name, r["E0"], r["E1"], r["ESCF"])
 
 
-def sort_case(name, nkr, nocc, seed, keep):
+# the list of sort_k9_5o spread over 3 files (MDCINT, MDCINT1, MDCINT2), read by the
+# reference's continuation records
+SORT_CONT_CASES = [("k9_5o", 9, 5, 4, 0.8, 3)]
+
+
+def sort_case(name, nkr, nocc, seed, keep, parts=1):
     import hashlib
     import tempfile
     from molfdir_amd import mdcint
     with tempfile.TemporaryDirectory() as d:
-        mdcint.synthetic_files(d, nkr, nocc, seed, keep)
+        mdcint.synthetic_files(d, nkr, nocc, seed, keep, parts=parts)
         digest = hashlib.sha256(open(os.path.join(d, "MDCINT"), "rb").read()).hexdigest()
         r = refrun.run_sort(d, nocc, 2 * nkr - nocc)
     keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
     np.savez_compressed(
-        os.path.join(HERE, f"sort_{name}.npz"),
+        os.path.join(HERE, f"{'sort' if parts == 1 else 'sortcont'}_{name}.npz"),
         params=np.array([nkr, nocc, seed], dtype=np.int64), keep=keep, mdcint_sha256=digest,
+        parts=parts,
         E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
     print(name, r["E0"], r["E1"], r["ESCF"])
 
@@ -243,9 +251,12 @@ def mp2g_case(name, no, nv, seed, scale, fp, pseed, maxdim, ntol, sym=None):
     print(name, r["expval"])
 
 
-# The plain FP64 comparison at C1 20o/200v: relccsd's CPU CCSD converged to NTOL 8 (DIIS 6).
-# The generator's sigma of 0.05 makes the iteration diverge at this size; 0.005 converges.
-CFG2_CASE = ("cfg2_20o200v", 20, 200, 2, 0.005, 40, 6, 8)
+# The plain FP64 comparison against relccsd's CPU CCSD converged to NTOL 8 (DIIS 6), at the
+# largest 20-occupied C1 size its file layer can hold: ccunix.c's wrabsf / rdabsf compute
+# byte offsets in 32-bit ints, so no class file may exceed 2 GiB (VVVV at 20o/200v is
+# 6.3 GB; at 20o/150v 2.0 GB).  The generator's sigma of 0.05 makes the iteration diverge
+# at this size; 0.005 converges.
+CFG2_CASE = ("cfg2_20o150v", 20, 150, 2, 0.005, 40, 6, 8)
 CFG2_TOPK = 4096        # largest |T2| elements kept, with their packed indices
 CFG2_NPROJ = 16         # seeded +-1 projections of T2
 
@@ -257,8 +268,8 @@ def cfg2_projections(n, seed=2024):
 
 
 def cfg2_case(name, no, nv, seed, scale, maxit, maxdim, ntol, full_t2_path=None):
-    """relccsd's CCSD at 20o/200v (about 10^2 s of CPU per iteration).  T2 has
-    3.8 M elements, so the fixture keeps T1, the iteration table, the largest
+    """relccsd's CCSD at 20o/150v (about 10^2 s of CPU per iteration).  T2 has
+    2.1 M elements, so the fixture keeps T1, the iteration table, the largest
     elements of T2, its norm and seeded projections; full_t2_path, if given,
     receives the whole T2 (not committed)."""
     s = synth.make_system(no, nv, seed=seed, scale=scale)
@@ -390,12 +401,12 @@ def main():
             mccres_case(*case)
         return
     if "--sort-only" in sys.argv:
-        for case in SORT_CASES:
+        for case in SORT_CASES + SORT_CONT_CASES:
             sort_case(*case)
         for case in SYM_SORT_CASES:
             sym_sort_case(*case)
         return
-    for case in SORT_CASES:
+    for case in SORT_CASES + SORT_CONT_CASES:
         sort_case(*case)
     for case in SYM_SORT_CASES:
         sym_sort_case(*case)

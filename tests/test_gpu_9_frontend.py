@@ -36,6 +36,31 @@ def test_read_molfdir_matches_reference_sort(name, tmp_path):
     assert r.last_sort_ms() >= 0.0
 
 
+@pytest.mark.parametrize("name", names("sortcont"))
+def test_read_molfdir_follows_continuation_files(name, tmp_path):
+    """The list spread over MDCINT, MDCINT1, MDCINT2 (continuation records)
+    gives the single-file reference sort bit for bit, every class; the
+    classes relccsd's first pass sorts also equal its own continuation run
+    (its VOVV / VVVV passes re-read only the last file; see
+    test_oracle_rdints.test_reference_reads_continuation_files)."""
+    from molfdir_amd import mdcint
+    g = load("sortcont", name)
+    single = load("sort", name)
+    nkr, nocc, seed = (int(x) for x in g["params"])
+    print(f"sortcont fixture {name}", flush=True)
+    mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]), parts=int(g["parts"]))
+    r = RelCCSD(nocc, 2 * nkr - nocc)
+    e = r.read_molfdir(str(tmp_path), [nocc])
+    for c in CLASSES:
+        assert np.array_equal(r.get(c), single[c]), c
+    for c in ("OOOO", "VOOO", "VVOO", "VOVO"):
+        assert np.array_equal(r.get(c), g[c]), c
+    assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+    (tmp_path / "MDCINT2").unlink()
+    with pytest.raises(RccError):
+        r.read_molfdir(str(tmp_path), [nocc])
+
+
 def test_device_list_entry_point_gives_the_same_classes(tmp_path):
     g = load("sort", "k9_5o")
     gen, nkr, nocc = regenerate(g, tmp_path)

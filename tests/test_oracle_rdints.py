@@ -97,3 +97,36 @@ def test_oracle_symmetric_sort_matches_reference(name, tmp_path):
         assert np.abs(lay.to_sym(k, f[k]) - g[k]).max() < 1e-14, k
     assert np.abs(f["eps"] - g["eps"]).max() < 1e-14
     assert abs(f["E0"] - float(g["E0"])) < 1e-13 and abs(f["E1"] - float(g["E1"])) < 1e-13
+
+
+@pytest.mark.parametrize("name", names("sortcont"))
+def test_reference_reads_continuation_files(name, tmp_path):
+    """The list of sort_<name> spread over MDCINT, MDCINT1, MDCINT2 (continuation
+    records, GET2INTS ccints.F:832-838).  relccsd's first pass follows them:
+    OOOO, VOOO, VVOO, VOVO, the Fock blocks and E0 / E1 equal the single-file
+    sort bit for bit.  Its VOVV and VVVV passes restart with REWIND + one
+    skipped record on the unit, which is then bound to the last file
+    (ccints.F:291-292, 395-396), so those two classes hold exactly that file's
+    records after its first one -- the library reads the whole list instead."""
+    g = load("sortcont", name)
+    single = load("sort", name)
+    nkr, nocc, seed = (int(x) for x in g["params"])
+    gen = mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]),
+                                 parts=int(g["parts"]))
+    digest = hashlib.sha256(open(os.path.join(str(tmp_path), "MDCINT"), "rb").read()).hexdigest()
+    assert digest == str(g["mdcint_sha256"])
+    for c in ("OOOO", "VOOO", "VVOO", "VOVO", "FOO", "FVO", "FVV", "eps"):
+        assert np.array_equal(g[c], single[c]), c
+    assert float(g["E0"]) == float(single["E0"]) and float(g["E1"]) == float(single["E1"])
+    ik, jk = gen["ikr"], gen["jkr"]
+    starts = [0] + [i for i in range(1, ik.size) if (ik[i], jk[i]) != (ik[i - 1], jk[i - 1])]
+    parts = int(g["parts"])
+    cuts = [round(i * len(starts) / parts) for i in range(parts + 1)]
+    lo = starts[cuts[parts - 1] + 1]
+    sel = slice(lo, ik.size)
+    sp = ro.Spinors(np.ones(2 * nkr, int), [nocc])
+    cl = ro.sort_classes(sp, ro.kr_table(gen["kr"]), ik[sel], jk[sel], gen["kkr"][sel],
+                         gen["lkr"][sel], gen["val"][sel])
+    for c in ("VOVV", "VVVV"):
+        assert np.array_equal(cl[c], g[c]), c
+        assert not np.array_equal(g[c], single[c]), c
