@@ -120,7 +120,7 @@ int read_mrconee(const char* path, MolfdirOne& out, std::string& err) {
     return 0;
 }
 
-int read_mdcint(const char* path, MdcintList& out, std::string& err) {
+int read_mdcint(const char* path, MdcintList& out, std::string& err, bool header_only_ok) {
     Reader r;
     r.path = path;
     r.f = fopen(path, "rb");
@@ -135,12 +135,13 @@ int read_mdcint(const char* path, MdcintList& out, std::string& err) {
     }
     size_t p = 18;     // DATEX (10) + TIMEX (8)
     out.nkr = get<int32_t>(b, p);
-    if (out.nkr <= 0 || b.size() != 22 + (size_t)out.nkr * 8) {
+    const bool bare = b.size() == 22;     // MDBINT: RDINTS reads DATEX, TIMEX, NKR only
+    if (out.nkr <= 0 || (b.size() != 22 + (size_t)out.nkr * 8 && !(bare && header_only_ok))) {
         err = std::string(path) + ": header record size does not match NKR";
         return -1;
     }
-    out.kr.resize(2 * (size_t)out.nkr);
-    std::memcpy(out.kr.data(), b.data() + 22, out.kr.size() * 4);
+    out.kr.assign(2 * (size_t)out.nkr, 0);
+    if (!bare) std::memcpy(out.kr.data(), b.data() + 22, out.kr.size() * 4);
     out.idx.clear();
     out.val.clear();
     int continuations = 0;
@@ -234,6 +235,7 @@ namespace {
 
 struct SortDev {
     int no, nv;
+    int itrsgn;                      // GET2INTS's ITRSGN: 1 Coulomb, -1 Breit
     int64_t rows[6];
     double2* cls[6];
     unsigned long long* key[2][6];   // [direct / exchange][class]
@@ -279,7 +281,7 @@ template <class F>
 __device__ __forceinline__ void expand16(int4 q, const int* __restrict__ kr, int nkr,
                                          const int* __restrict__ spos, const SortDev& d, F f) {
     const int IK = q.x, JK = q.y, KK = q.z, LK = q.w;
-    const int ijsign = (IK > 0) == (JK > 0) ? 1 : -1;
+    const int ijsign = ((IK > 0) == (JK > 0) ? 1 : -1) * d.itrsgn;     // ccints.F:855
     int i0 = kr[nkr + IK], j0 = kr[nkr + JK];
     int ijsg = 1;
     int v = 0;
@@ -325,7 +327,7 @@ __device__ __forceinline__ void expand16(int4 q, const int* __restrict__ kr, int
             }
             k0 = kr[nkr - LK];
             l0 = kr[nkr - KK];
-            klsg = (KK > 0) == (LK > 0) ? 1 : -1;
+            klsg = ((KK > 0) == (LK > 0) ? 1 : -1) * d.itrsgn;      // ccints.F:939
         }
         i0 = kr[nkr - JK];
         j0 = kr[nkr - IK];
@@ -495,11 +497,12 @@ fock_energy_kernel(int no, int nv, const double2* __restrict__ OOOO,
 
 int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos, int nsp,
                           const int* d_kr, int nkr, const int4* d_idx, const double2* d_val,
-                          int64_t m, hipStream_t st, double* ms, std::string& err) {
+                          int64_t m, hipStream_t st, double* ms, std::string& err, int itrsgn) {
     (void)nsp;
     SortDev d;
     d.no = no;
     d.nv = nv;
+    d.itrsgn = itrsgn;
     const int64_t noot = (int64_t)no * (no - 1) / 2, nvvt = (int64_t)nv * (nv - 1) / 2,
                   nvo = (int64_t)no * nv;
     const int64_t rows[6] = {noot, nvo, nvvt, nvo, nvo, nvvt};

@@ -100,7 +100,9 @@ struct MdcintList {          // MDCINT: KR table and the canonical integrals
     std::vector<double2> val;
 };
 int read_mrconee(const char* path, MolfdirOne& out, std::string& err);
-int read_mdcint(const char* path, MdcintList& out, std::string& err);
+// header_only_ok: a header record without the KR table is accepted (MDBINT)
+int read_mdcint(const char* path, MdcintList& out, std::string& err,
+                bool header_only_ok = false);
 // RDPROP: the NSP x NSP matrix of operator `name` on an MDPROP file
 int read_mdprop(const char* path, const char* name, int nsp, std::vector<double2>& prop,
                 std::string& err);
@@ -108,9 +110,12 @@ int read_mdprop(const char* path, const char* name, int nsp, std::vector<double2
 // packed C1 classes cls[0..5] (OOOO, VOOO, VVOO, VOVO, VOVV, VVVV), which
 // are overwritten.  d_spos[s]: C1 position of spinor s (>= 0 occupied,
 // -(position + 1) virtual); d_kr[nkr + i] = spinor of KR(i), i in [-nkr, nkr].
+// itrsgn: GET2INTS's sign of the time-reversed pair images (1 Coulomb list,
+// -1 Breit list).
 int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos, int nsp,
                           const int* d_kr, int nkr, const int4* d_idx, const double2* d_val,
-                          int64_t m, hipStream_t st, double* ms, std::string& err);
+                          int64_t m, hipStream_t st, double* ms, std::string& err,
+                          int itrsgn = 1);
 // GETFM + MAKEFM on the sorted classes: Fock blocks, orbital energies, E0, E1.
 int build_fock_device(int no, int nv, int nsp, const double2* d_h1, const int* d_occ,
                       const int* d_vir, double2* const* cls, double2* FOO, double2* FVO,

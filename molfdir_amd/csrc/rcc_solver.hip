@@ -2943,8 +2943,12 @@ static int molfdir_positions(rcc_t* h, const MolfdirOne& one, const int* nelec, 
     return 0;
 }
 
+// breit (may be null): the MDBINT list, sorted with ITRSGN = -1 into zeroed
+// classes, to which the Coulomb classes are then added (RDINTS ccints.F:
+// 230-238, 354-358, 457-461: the Breit pass ends with XAXPY(+1, Coulomb)).
 static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int* kr, int nkr,
-                               const int4* d_idx, const double2* d_val, int64_t m, double* ms) {
+                               const int4* d_idx, const double2* d_val, int64_t m, double* ms,
+                               const MdcintList* breit = nullptr) {
     const int nsp = (int)spos.size();
     if (2 * nkr != nsp) {
         h->err = "rcc_sort: INCONSISTENT MRCONEE AND MDCINT FILES (2 NKR != NSP)";
@@ -2973,8 +2977,51 @@ static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int
     if (h->sym.nrep > 1)
         HIPCHK(h, hipMalloc((void**)&vvvv_c1, (size_t)h->nvvt * h->nvvt * sizeof(double2) + 16));
     if (vvvv_c1) cls[RCC_VVVV] = vvvv_c1;
-    const int rc = sort_integrals_device(h->no, h->nv, cls, d_spos, nsp, d_kr, nkr, d_idx, d_val,
-                                         m, h->st, ms, h->err);
+    int rc = sort_integrals_device(h->no, h->nv, cls, d_spos, nsp, d_kr, nkr, d_idx, d_val,
+                                   m, h->st, ms, h->err);
+    if (!rc && breit) {
+        int64_t sizes[6], total = 0;
+        for (int c = 0; c < 6; ++c) {
+            sizes[c] = h->cls_rows[c] * h->cls_cols[c];
+            total += sizes[c];
+        }
+        const int64_t mb = (int64_t)breit->idx.size();
+        double2 *saved = nullptr, *b_val = nullptr;
+        int4* b_idx = nullptr;
+        hipError_t e = hipMalloc((void**)&saved, (size_t)std::max<int64_t>(total, 1) * 16);
+        if (e == hipSuccess) e = hipMalloc((void**)&b_idx, (size_t)std::max<int64_t>(mb, 1) * 16);
+        if (e == hipSuccess) e = hipMalloc((void**)&b_val, (size_t)std::max<int64_t>(mb, 1) * 16);
+        if (e == hipSuccess && mb) {
+            e = hipMemcpy(b_idx, breit->idx.data(), (size_t)mb * 16, hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(b_val, breit->val.data(), (size_t)mb * 16, hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) {
+            h->err = std::string("Breit list: ") + hipGetErrorString(e);
+            rc = -1;
+        } else {
+            int64_t off = 0;
+            for (int c = 0; c < 6; ++c) {
+                k_copy(sizes[c], cls[c], saved + off, h->st);
+                off += sizes[c];
+            }
+            double ms_b = 0.0;
+            rc = sort_integrals_device(h->no, h->nv, cls, d_spos, nsp, d_kr, nkr, b_idx, b_val, mb,
+                                       h->st, &ms_b, h->err, -1);
+            if (!rc) {
+                off = 0;
+                for (int c = 0; c < 6; ++c) {
+                    k_axpby(sizes[c], cplx(1, 0), saved + off, cplx(1, 0), cls[c], h->st);
+                    off += sizes[c];
+                }
+                if (hipStreamSynchronize(h->st) != hipSuccess) rc = -1;
+                if (ms) *ms += ms_b;
+            }
+        }
+        (void)hipFree(saved);
+        (void)hipFree(b_idx);
+        (void)hipFree(b_val);
+    }
     if (!rc && h->sym.nrep > 1) {
         c1_to_sym_dev(h, RCC_VOVV, h->cls[RCC_VOVV], h->VOVV_s);
         c1_to_sym_dev(h, RCC_VVVV, vvvv_c1, h->VVVV_s);
@@ -3038,8 +3085,28 @@ int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const in
         HIPCHK(h, hipMemcpy(d_idx, lst.idx.data(), (size_t)m * sizeof(int4), hipMemcpyHostToDevice));
         HIPCHK(h, hipMemcpy(d_val, lst.val.data(), (size_t)m * sizeof(double2), hipMemcpyHostToDevice));
     }
+    // BREIT in MRCONEE: the Breit (Gaunt) list MDBINT beside MDCINT is sorted
+    // too and added (RDINTS ccints.F:496-505)
+    MdcintList blst;
+    if (one.breit) {
+        std::string bp(mdcint);
+        const size_t slash = bp.find_last_of('/');
+        bp = (slash == std::string::npos ? std::string() : bp.substr(0, slash + 1)) + "MDBINT";
+        if (read_mdcint(bp.c_str(), blst, h->err, true)) {
+            (void)hipFree(d_idx);
+            (void)hipFree(d_val);
+            return -1;
+        }
+        if (blst.nkr != lst.nkr) {
+            (void)hipFree(d_idx);
+            (void)hipFree(d_val);
+            h->err = "MDCINT AND MDBINT FILES ARE INCOMPATIBLE (NKR differs)";
+            return -1;
+        }
+    }
     double ms = 0.0;
-    const int rc = sort_upload_and_run(h, spos, lst.kr.data(), lst.nkr, d_idx, d_val, m, &ms);
+    const int rc = sort_upload_and_run(h, spos, lst.kr.data(), lst.nkr, d_idx, d_val, m, &ms,
+                                       one.breit ? &blst : nullptr);
     (void)hipFree(d_idx);
     (void)hipFree(d_val);
     if (rc) return -1;

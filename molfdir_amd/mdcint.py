@@ -144,11 +144,14 @@ def synthetic_list(nkr: int, seed: int, keep: float = 0.7, scale: float = 0.05):
     return ik, jk, kk, lk, val
 
 
-def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, parts: int = 1):
+def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, parts: int = 1,
+                    breit: bool = False):
     """C1 MRCONEE + MDCINT in dirpath: NSP = 2 NKR spinors with shuffled
     Kramers pairs, the first nocc spinors (input order) occupied, orbital
     energies ascending, a Hermitian one-electron matrix (diagonal eps,
-    off-diagonal N(0, 0.02^2)).  Returns the generated arrays."""
+    off-diagonal N(0, 0.02^2)).  breit: MRCONEE's BREIT flag set and a
+    second seeded list (values N(0, 0.01^2)) written as MDBINT.  Returns the
+    generated arrays."""
     import os
     rng = np.random.Generator(np.random.PCG64(seed + 7919))
     nsp = 2 * nkr
@@ -159,10 +162,16 @@ def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, 
     h1 = 0.5 * (h1 + h1.conj().T)
     h1[np.arange(nsp), np.arange(nsp)] = eps
     ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
-    write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, np.ones(nsp, np.int32), [nocc], h1)
+    write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, np.ones(nsp, np.int32), [nocc], h1,
+                  breit=breit)
     write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val, parts=parts)
-    return {"nsp": nsp, "nocc": nocc, "kr": kr, "eps": eps, "h1": h1, "ikr": ik, "jkr": jk,
-            "kkr": kk, "lkr": lk, "val": val}
+    out = {"nsp": nsp, "nocc": nocc, "kr": kr, "eps": eps, "h1": h1, "ikr": ik, "jkr": jk,
+           "kkr": kk, "lkr": lk, "val": val}
+    if breit:
+        bl = synthetic_list(nkr, seed + 1000, keep, scale=0.01)
+        write_mdcint(os.path.join(dirpath, "MDBINT"), kr, *bl)
+        out["breit"] = dict(zip(("ikr", "jkr", "kkr", "lkr", "val"), bl))
+    return out
 
 
 def abelian_multb(nrep: int) -> np.ndarray:

@@ -21,6 +21,9 @@ abelian double groups (test infrastructure only): MKINDEX (ccints.F:1421-1580), 
   (i, k) is a canonical pair, the exchange part at (k, i) otherwise.
 * RDINTS stores every expanded value by assignment, in file order, so the
   last write of an element wins; class = direct - exchange.
+* With BREIT, the Breit list (MDBINT) is expanded with ITRSGN = -1 (the
+  signs of the time-reversed pair images flipped), sorted the same way, and
+  the Coulomb classes are added to it.
 * GETFM: F = conj(h) of MRCONEE record 6 on the oo / vo / vv blocks, then
   MAKEFM adds the occupied two-electron sums, E0 = sum Re F_ii,
   E1 = -1/2 sum_ik Re <ik||ik>, and the orbital energies become Re diag F.
@@ -79,14 +82,16 @@ class Spinors:
 CLASS_OF_TYPE = {1: "OOOO", 2: "VOOO", 4: "VVOO", 6: "VOVO", 14: "VOVV", 16: "VVVV"}
 
 
-def expand(sp: Spinors, kr, ikr, jkr, kkr, lkr):
+def expand(sp: Spinors, kr, ikr, jkr, kkr, lkr, itrsgn=1):
     """GET2INTS: for each canonical integral, the list of (class type, row,
     col, re sign, im sign) of its stored expansions, in the order RDINTS
-    assigns them.  kr[(i)] / kr[(-i)]: 0-based spinor of Kramers pair i."""
+    assigns them.  kr[(i)] / kr[(-i)]: 0-based spinor of Kramers pair i.
+    itrsgn: ITRSGN, multiplying the signs of the time-reversed pair images
+    (ccints.F:855, 939): 1 for the Coulomb list, -1 for the Breit list."""
     out = []
     for n in range(len(ikr)):
         IK, JK, KK, LK = int(ikr[n]), int(jkr[n]), int(kkr[n]), int(lkr[n])
-        ijsign = int(np.sign(IK * JK))
+        ijsign = int(np.sign(IK * JK)) * itrsgn
         i0, j0 = kr[IK], kr[JK]
         ijsg = 1
         for _ in range(2):
@@ -110,7 +115,7 @@ def expand(sp: Spinors, kr, ikr, jkr, kkr, lkr):
                         i, j, k, l = j1, i1, l1, k1
                     i1, j1, k1, l1 = k0, l0, i0, j0
                 k0, l0 = kr[-LK], kr[-KK]
-                klsg = int(np.sign(KK * LK))
+                klsg = int(np.sign(KK * LK)) * itrsgn
             i0, j0 = kr[-JK], kr[-IK]
             ijsg = ijsign
     return out
@@ -122,12 +127,12 @@ def class_shapes(no, nv):
             "VOVO": (nvo, nvo), "VOVV": (nvo, nvvt), "VVVV": (nvvt, nvvt)}
 
 
-def sort_classes(sp: Spinors, kr, ikr, jkr, kkr, lkr, val):
+def sort_classes(sp: Spinors, kr, ikr, jkr, kkr, lkr, val, itrsgn=1):
     """RDINTS's six classes (packed C1 layout, rows fastest)."""
     shapes = class_shapes(sp.no, sp.nv)
     direct = {c: np.zeros(r * q, np.complex128) for c, (r, q) in shapes.items()}
     exch = {c: np.zeros(r * q, np.complex128) for c, (r, q) in shapes.items()}
-    for n, t, row, col, s_re, s_im in expand(sp, kr, ikr, jkr, kkr, lkr):
+    for n, t, row, col, s_re, s_im in expand(sp, kr, ikr, jkr, kkr, lkr, itrsgn):
         c = CLASS_OF_TYPE.get(abs(t))
         if c is None:
             continue
@@ -135,6 +140,15 @@ def sort_classes(sp: Spinors, kr, ikr, jkr, kkr, lkr, val):
         v = val[n]
         (direct if t > 0 else exch)[c][row + col * rows] = complex(v.real * s_re, v.imag * s_im)
     return {c: direct[c] - exch[c] for c in shapes}
+
+
+def sort_classes_breit(sp: Spinors, kr, coulomb, breit):
+    """RDINTS with BREIT (ccints.F:230-238, 354-358, 457-461, 496-505): the
+    Breit list sorted like the Coulomb one with ITRSGN = -1, then the Coulomb
+    classes added to it.  coulomb, breit: (ikr, jkr, kkr, lkr, val)."""
+    c = sort_classes(sp, kr, *coulomb)
+    b = sort_classes(sp, kr, *breit, itrsgn=-1)
+    return {k: b[k] + c[k] for k in c}
 
 
 def kr_table(kr_pairs):

@@ -155,9 +155,10 @@ def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None):
     else:
         lay = SymLayout(list(sym[1]), list(sym[2]))
     with tempfile.TemporaryDirectory(dir=workdir) as d:
-        # MRCONEE, MDCINT and any continuation files MDCINT1, MDCINT2, ..
+        # MRCONEE, MDCINT, any continuation files MDCINT1, MDCINT2, .. and the
+        # Breit list MDBINT
         for fn in sorted(os.listdir(files_dir)):
-            if fn == "MRCONEE" or fn.startswith("MDCINT"):
+            if fn == "MRCONEE" or fn.startswith("MDCINT") or fn.startswith("MDBINT"):
                 shutil.copy(os.path.join(files_dir, fn), os.path.join(d, fn))
         nrep = lay.nrep
         hdr = np.array([no, nv, 4, 1, 1, 12, nrep if nrep > 1 else 0, 0], dtype=np.int32)

@@ -27,6 +27,8 @@ change in the generator is caught.
   the SHA-256 of the MDCINT file the reference read.
 * sortcont_<name>.npz: the same for the list of sort_<name> spread over three
   files joined by continuation records (MDCINT, MDCINT1, MDCINT2).
+* sortbreit_<name>.npz: the same with MRCONEE's BREIT flag set and a second
+  seeded list written as MDBINT (Coulomb + Breit classes).
 * mccres_<name>.bin / .npz: the MCCRES restart file relccsd's CCENER wrote
   for a ccsd_ / symccsd_ fixture run (PUTAMPT + CCDUMP), with the T1 / T2,
   IMSTAT and energy records GETAMPT reads back; the generator also checks
@@ -168,6 +170,22 @@ def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
 # the list of sort_k9_5o spread over 3 files (MDCINT, MDCINT1, MDCINT2), read by the
 # reference's continuation records
 SORT_CONT_CASES = [("k9_5o", 9, 5, 4, 0.8, 3)]
+# Coulomb + Breit lists (MRCONEE's BREIT flag set, MDBINT beside MDCINT)
+SORT_BREIT_CASES = [("k7_6o", 7, 6, 3, 0.5), ("k9_5o", 9, 5, 4, 0.8)]
+
+
+def sort_breit_case(name, nkr, nocc, seed, keep):
+    import tempfile
+    from molfdir_amd import mdcint
+    with tempfile.TemporaryDirectory() as d:
+        mdcint.synthetic_files(d, nkr, nocc, seed, keep, breit=True)
+        r = refrun.run_sort(d, nocc, 2 * nkr - nocc)
+    keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    np.savez_compressed(
+        os.path.join(HERE, f"sortbreit_{name}.npz"),
+        params=np.array([nkr, nocc, seed], dtype=np.int64), keep=keep,
+        E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
+    print(name, "breit", r["E0"], r["E1"], r["ESCF"])
 
 
 def sort_case(name, nkr, nocc, seed, keep, parts=1):
@@ -403,11 +421,15 @@ def main():
     if "--sort-only" in sys.argv:
         for case in SORT_CASES + SORT_CONT_CASES:
             sort_case(*case)
+        for case in SORT_BREIT_CASES:
+            sort_breit_case(*case)
         for case in SYM_SORT_CASES:
             sym_sort_case(*case)
         return
     for case in SORT_CASES + SORT_CONT_CASES:
         sort_case(*case)
+    for case in SORT_BREIT_CASES:
+        sort_breit_case(*case)
     for case in SYM_SORT_CASES:
         sym_sort_case(*case)
     if "--t3-only" in sys.argv:

@@ -61,6 +61,28 @@ def test_read_molfdir_follows_continuation_files(name, tmp_path):
         r.read_molfdir(str(tmp_path), [nocc])
 
 
+@pytest.mark.parametrize("name", names("sortbreit"))
+def test_read_molfdir_adds_the_breit_list(name, tmp_path):
+    """BREIT in MRCONEE: the MDBINT list is sorted with ITRSGN = -1 and the
+    Coulomb classes are added, bit for bit as relccsd's RDINTS; a missing
+    MDBINT is an error."""
+    from molfdir_amd import mdcint
+    g = load("sortbreit", name)
+    nkr, nocc, seed = (int(x) for x in g["params"])
+    print(f"sortbreit fixture {name}", flush=True)
+    mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]), breit=True)
+    r = RelCCSD(nocc, 2 * nkr - nocc)
+    e = r.read_molfdir(str(tmp_path), [nocc])
+    for c in CLASSES:
+        assert np.array_equal(r.get(c), g[c]), c
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(r.get(k) - g[k]).max() < 1e-14, k
+    assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+    (tmp_path / "MDBINT").unlink()
+    with pytest.raises(RccError):
+        r.read_molfdir(str(tmp_path), [nocc])
+
+
 def test_device_list_entry_point_gives_the_same_classes(tmp_path):
     g = load("sort", "k9_5o")
     gen, nkr, nocc = regenerate(g, tmp_path)

@@ -130,3 +130,27 @@ def test_reference_reads_continuation_files(name, tmp_path):
     for c in ("VOVV", "VVVV"):
         assert np.array_equal(cl[c], g[c]), c
         assert not np.array_equal(g[c], single[c]), c
+
+
+@pytest.mark.parametrize("name", names("sortbreit"))
+def test_oracle_breit_sort_matches_reference(name, tmp_path):
+    """MRCONEE's BREIT flag set and a Breit list in MDBINT: relccsd sorts it
+    with ITRSGN = -1 and adds the Coulomb classes (ccints.F:230-238, 496-505).
+    The oracle agrees bit for bit; the Breit list sorted with ITRSGN = +1
+    (a planted mistake) does not."""
+    g = load("sortbreit", name)
+    nkr, nocc, seed = (int(x) for x in g["params"])
+    gen = mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]), breit=True)
+    sp = ro.Spinors(np.ones(2 * nkr, int), [nocc])
+    kr = ro.kr_table(gen["kr"])
+    coul = (gen["ikr"], gen["jkr"], gen["kkr"], gen["lkr"], gen["val"])
+    b = gen["breit"]
+    brt = (b["ikr"], b["jkr"], b["kkr"], b["lkr"], b["val"])
+    cl = ro.sort_classes_breit(sp, kr, coul, brt)
+    for c in CLASSES:
+        assert np.array_equal(cl[c], g[c]), c
+    f = ro.fock(sp, gen["h1"], cl)
+    assert abs(f["E0"] - float(g["E0"])) < 1e-13 and abs(f["E1"] - float(g["E1"])) < 1e-13
+    wrong_b = ro.sort_classes(sp, kr, *brt)
+    wrong_c = ro.sort_classes(sp, kr, *coul)
+    assert any(not np.array_equal(wrong_b[c] + wrong_c[c], g[c]) for c in CLASSES)
