@@ -562,15 +562,18 @@ def converged_accuracy(no, nv, seed, scale, modes, maxit=40, ntol=9):
 
 
 CFG2_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
-                        "cfg2ref_cfg2_20o200v.npz")
+                        "cfg2ref_cfg2_20o150v.npz")
 
 
 def cfg2_vs_cpu():
-    """The plain FP64 path against relccsd's CPU CCSD at C1 20o/200v: the same
-    seeded system (factor std 0.005, at which the iteration converges)
-    converged to NTOL 8 with DIIS 6 on the GPU, compared with the reference
-    run stored in tests/golden/cfg2ref_cfg2_20o200v.npz (energies, iteration
-    count, T1, the largest T2 elements, the T2 norm and seeded projections)."""
+    """The plain FP64 path against relccsd's CPU CCSD: the same seeded system
+    (factor std 0.005, at which the iteration converges) converged to NTOL 8
+    with DIIS 6 on the GPU, compared with the reference run stored in
+    tests/golden/cfg2ref_cfg2_20o150v.npz (energies, iteration count, T1, the
+    largest T2 elements, the T2 norm and seeded projections).  20o/150v is the
+    largest 20-occupied C1 size relccsd's file layer can hold (ccunix.c
+    wrabsf / rdabsf: 32-bit byte offsets, 2 GiB per class file; VVVV at
+    20o/200v is 6.3 GB)."""
     import numpy as np
     import torch
     from molfdir_amd import synth
@@ -610,7 +613,7 @@ def cfg2_vs_cpu():
         "max_dT2_proj": float(np.abs(np.array([np.dot(wk.astype(np.float64), t2) for wk in w])
                                      - g["T2_proj"]).max()),
     }
-    print(f"[bench] 20o/200v vs relccsd CPU: Ec {o['eccsd']:.12f} (ref {float(g['eccsd']):.12f}, "
+    print(f"[bench] {no}o/{nv}v vs relccsd CPU: Ec {o['eccsd']:.12f} (ref {float(g['eccsd']):.12f}, "
           f"dEc {res['dEc']:.2e}) nit {o['nit']} (ref {int(g['nit'])}) max|dT1| "
           f"{res['max_dT1']:.2e} max|dT2| over the {top.size} largest {res['max_dT2_top']:.2e}",
           file=sys.stderr, flush=True)
