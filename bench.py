@@ -779,7 +779,7 @@ def main():
     ap.add_argument("--lambda-maxit", type=int, default=80,
                     help="cfg5: MAXIT of the CCSD and Lambda solves")
     ap.add_argument("--scale", type=float, default=0.05,
-                    help="std of the Re/Im parts of the factor B (BASELINE recipe: 0.05)")
+                    help="std of the Re/Im parts of the factor B (the seeded recipe's value: 0.05)")
     ap.add_argument("--t3-scale", type=float, default=0.01,
                     help="cfg3t: std of the factor B (smaller ones are tried if CCSD diverges)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -3,7 +3,7 @@ reference fixtures and the CPU oracle.
 
 Bars: per-term intermediates to 1e-12 relative; whole trajectories with the
 correlation energy to 1e-10 Eh and amplitudes to 1e-9 max-abs (the FP64
-tolerance the north-star sets for this path).
+parity tolerance of this path).
 """
 import numpy as np
 import pytest
