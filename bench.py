@@ -802,7 +802,7 @@ def main():
                          "reference point that scales rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
+        print("bench.py: measures one GPU (one process, no collectives); --gpus must be 1",
               file=sys.stderr)
         sys.exit(2)
 

@@ -40,7 +40,7 @@ def test_fortran_driver_matches_reference(kind, name, tmp_path):
     assert abs(raw[0] - float(g["emp2"])) < 1e-10
     assert np.abs(z[:nt1] - g["T1"]).max() < 1e-9
     assert np.abs(z[nt1:] - g["T2"]).max() < 1e-9
-    # the restart file the driver leaves, as relccsd's CCENER leaves MCCRES
+    # the restart file the Fortran driver leaves, as relccsd's CCENER leaves MCCRES
     from molfdir_amd.relccsd import mccres_read
     t1, t2, st, en = mccres_read(tmp_path / "MCCRES", nt1, g["T2"].size)
     assert np.array_equal(t1, z[:nt1]) and np.array_equal(t2, z[nt1:])
