@@ -255,6 +255,13 @@ int rcc_last_lambda_ms(rcc_t* h, double* ms);
  * 3-5) must give the handle's occupied / virtual counts per irrep. */
 int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
                      int nnelec, double* e0e1);
+/* rcc_read_molfdir with namelist CCSORT's options (ccdriv.F:155, 342-343;
+ * GETFM ccints.F:1080, 1150-1198, 1260, 1287): useoe keeps MRCONEE's orbital
+ * energies instead of Re diag F; norecmp takes F = diag of those energies
+ * (FVO = 0) and E0 = their occupied sum instead of rebuilding F (E1 still
+ * from the integrals).  rcc_read_molfdir is this with both 0. */
+int rcc_read_molfdir_opts(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
+                          int nnelec, int useoe, int norecmp, double* e0e1);
 /* The expansion and sort alone, from a canonical list already in device
  * memory: d_idx m x (IKR, JKR, KKR, LKR) int32, d_val m complex; kr the
  * MDCINT header's (KR(i), KR(-i)) pairs (1-based spinors), spinor_pos[s]

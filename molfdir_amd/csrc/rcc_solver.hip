@@ -3069,6 +3069,11 @@ int rcc_sort_ints_device(rcc_t* h, const int* spinor_pos, int nsp, const int* kr
 
 int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
                      int nnelec, double* e0e1) {
+    return rcc_read_molfdir_opts(h, mrconee, mdcint, nelec, nnelec, 0, 0, e0e1);
+}
+
+int rcc_read_molfdir_opts(rcc_t* h, const char* mrconee, const char* mdcint, const int* nelec,
+                          int nnelec, int useoe, int norecmp, double* e0e1) {
     if (!h || !mrconee || !mdcint || !nelec) return -1;
     MolfdirOne one;
     if (read_mrconee(mrconee, one, h->err)) return -1;
@@ -3132,6 +3137,30 @@ int rcc_read_molfdir(rcc_t* h, const char* mrconee, const char* mdcint, const in
     (void)hipFree(d_vir);
     (void)hipFree(d_e01);
     if (fr) return -1;
+    // namelist CCSORT (GETFM ccints.F:1080, 1150-1198, 1260, 1287): USEOE keeps
+    // MRCONEE's orbital energies; NORECMP takes F = diag of them, FVO = 0 and
+    // E0 = their occupied sum (E1 comes from the integrals either way)
+    if (useoe || norecmp) {
+        const int o = h->no, v = h->nv;
+        std::vector<double> ew((size_t)o + v);
+        for (int p = 0; p < o; ++p) ew[p] = one.eps[occ_sp[p]];
+        for (int p = 0; p < v; ++p) ew[o + p] = one.eps[vir_sp[p]];
+        HIPCHK(h, hipMemcpy(h->eps, ew.data(), ew.size() * 8, hipMemcpyHostToDevice));
+        if (norecmp) {
+            std::vector<double2> foo((size_t)o * o, make_double2(0.0, 0.0)),
+                fvo((size_t)v * o, make_double2(0.0, 0.0)), fvv((size_t)v * v, make_double2(0.0, 0.0));
+            double e0 = 0.0;
+            for (int p = 0; p < o; ++p) {
+                foo[p + (size_t)o * p].x = ew[p];
+                e0 += ew[p];
+            }
+            for (int p = 0; p < v; ++p) fvv[p + (size_t)v * p].x = ew[o + p];
+            HIPCHK(h, hipMemcpy(h->FOO, foo.data(), foo.size() * 16, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->FVO, fvo.data(), fvo.size() * 16, hipMemcpyHostToDevice));
+            HIPCHK(h, hipMemcpy(h->FVV, fvv.data(), fvv.size() * 16, hipMemcpyHostToDevice));
+            e01[0] = e0;
+        }
+    }
     h->derived_ok = false;
     h->last_sort_ms = ms;
     if (e0e1) {

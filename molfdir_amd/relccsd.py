@@ -182,6 +182,8 @@ def load():
     lib.rcc_read_mccres.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ci), dp]
     lib.rcc_read_molfdir.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.POINTER(ci), ci, dp]
+    lib.rcc_read_molfdir_opts.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.POINTER(ci), ci, ci, ci, dp]
     lib.rcc_sort_ints_device.argtypes = [vp, ctypes.POINTER(ci), ci, ctypes.POINTER(ci), ci, vp,
                                          vp, i64, dp]
     lib.rcc_last_sort_ms.argtypes = [vp]
@@ -512,17 +514,18 @@ class RelCCSD:
         self._check(self.lib.rcc_last_lambda_ms(self.h, ctypes.byref(ms)))
         return ms.value
 
-    def read_molfdir(self, directory, nelec, mrconee="MRCONEE", mdcint="MDCINT"):
+    def read_molfdir(self, directory, nelec, mrconee="MRCONEE", mdcint="MDCINT", useoe=False,
+                     norecmp=False):
         """Integral front end (RDINTS): the six classes, the Fock blocks and
         the orbital energies from the MOLFDIR files in `directory`; nelec the
-        occupied spinors per irrep of MRCONEE.  Returns E0, E1, ECORE and
-        ESCF = E0 + E1 + ECORE."""
+        occupied spinors per irrep of MRCONEE; useoe / norecmp namelist
+        CCSORT's options.  Returns E0, E1, ECORE and ESCF = E0 + E1 + ECORE."""
         import os
         ne = (ctypes.c_int * len(nelec))(*[int(x) for x in nelec])
         out = (ctypes.c_double * 3)()
-        self._check(self.lib.rcc_read_molfdir(
+        self._check(self.lib.rcc_read_molfdir_opts(
             self.h, os.path.join(directory, mrconee).encode(), os.path.join(directory, mdcint).encode(),
-            ne, len(nelec), out))
+            ne, len(nelec), int(bool(useoe)), int(bool(norecmp)), out))
         return {"e0": out[0], "e1": out[1], "ecore": out[2], "escf": out[0] + out[1] + out[2]}
 
     def sort_ints_device(self, spinor_pos, kr, idx, val):

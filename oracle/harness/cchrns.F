@@ -87,7 +87,8 @@ C
       IW = 6
       INTERFACE = 'MOLFDIR   '
 C
-C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, NREP (0 = C1), unused.
+C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, NREP (0 = C1), and for
+C     mode 4 the CCSORT options (1 USEOE + 2 NORECMP), else unused.
 C     For NREP > 1 the orbital energies are followed by the fermion irrep
 C     (1..NREP) of every spinor, the occupied count per irrep and the
 C     2NREP x 2NREP table MULTB(.,.,1), all int32.
@@ -145,6 +146,10 @@ C
       WRITE (31,'(A,I4,A,I4,A,I4,A)') ' &CCENER DOMP2=T, DOCCSD=T,'//
      &   ' DOCCSDT=F, MAXIT=',HDR(4),', MAXDIM=',HDR(5),
      &   ', NTOL=',HDR(6),' /'
+C     HDR(8) (mode 4): bit 1 USEOE, bit 2 NORECMP of namelist CCSORT
+      IF (MODE.EQ.4.AND.HDR(8).NE.0) WRITE (31,'(A,L1,A,L1,A)')
+     &   ' &CCSORT USEOE=',IAND(HDR(8),1).NE.0,', NORECMP=',
+     &   IAND(HDR(8),2).NE.0,' /'
       CLOSE (31)
 C
 C     Set-up sequence of CCMAIN (ccdriv.F:426-529)

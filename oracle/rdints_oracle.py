@@ -161,12 +161,28 @@ def kr_table(kr_pairs):
     return kr
 
 
-def fock(sp: Spinors, h1, classes):
-    """GETFM + MAKEFM: FOO, FVO, FVV (packed column-major), E0, E1, eps."""
+def fock(sp: Spinors, h1, classes, eps_file=None, useoe=False, norecmp=False):
+    """GETFM + MAKEFM: FOO, FVO, FVV (packed column-major), E0, E1, eps.
+    eps_file: MRCONEE record 5's orbital energy of every spinor (file order),
+    used with namelist CCSORT's USEOE (eps kept, ccints.F:1260, 1287) and
+    NORECMP (F = diag(eps_file), FVO = 0, E0 = sum of the occupied ones, E1
+    from the integrals as always, ccints.F:1161-1198)."""
     from .ccsd_oracle import Layout
     no, nv = sp.no, sp.nv
     occ_s = np.nonzero(sp.occ)[0][np.argsort(sp.pos[sp.occ])]
     vir_s = np.nonzero(~sp.occ)[0][np.argsort(sp.pos[~sp.occ])]
+    if useoe or norecmp:
+        ef = np.asarray(eps_file, dtype=np.float64)
+        eps_w = np.concatenate([ef[occ_s], ef[vir_s]])
+    if norecmp:
+        L = Layout(no, nv)
+        oooo = L.unpack_pp(classes["OOOO"], no, no)
+        foo = np.diag(eps_w[:no]).astype(np.complex128)
+        fvv = np.diag(eps_w[no:]).astype(np.complex128)
+        fvo = np.zeros((nv, no), np.complex128)
+        return {"FOO": foo.T.reshape(-1), "FVO": fvo.T.reshape(-1), "FVV": fvv.T.reshape(-1),
+                "E0": float(np.sum(eps_w[:no])),
+                "E1": -0.5 * float(np.real(np.einsum("ikik->", oooo))), "eps": eps_w}
     h = np.asarray(h1)
     foo = np.conj(h[np.ix_(occ_s, occ_s)])
     fvo = np.conj(h[np.ix_(vir_s, occ_s)])
@@ -181,5 +197,7 @@ def fock(sp: Spinors, h1, classes):
     e1 = -0.5 * float(np.real(np.einsum("ikik->", oooo)))
     e0 = float(np.real(np.trace(foo)))
     eps = np.concatenate([np.real(np.diag(foo)), np.real(np.diag(fvv))])
+    if useoe:
+        eps = eps_w
     return {"FOO": foo.T.reshape(-1), "FVO": fvo.T.reshape(-1), "FVV": fvv.T.reshape(-1),
             "E0": e0, "E1": e1, "eps": eps}

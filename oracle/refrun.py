@@ -144,10 +144,13 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
     return out
 
 
-def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None):
+def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None, useoe=False,
+             norecmp=False):
     """Integral sort of files_dir/MRCONEE + files_dir/MDCINT.  sym (for
     NREP > 1): (file-order irrep of every spinor (1-based), occupied count
-    per irrep, nv per irrep); outputs are then in the symmetry-blocked layout."""
+    per irrep, nv per irrep); outputs are then in the symmetry-blocked layout.
+    useoe / norecmp: namelist CCSORT's USEOE and NORECMP (GETFM ccints.F:
+    1080, 1150-1198, 1260, 1287)."""
     import shutil
     from oracle.symtab import SymLayout, multb_full
     if sym is None:
@@ -161,7 +164,8 @@ def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None):
             if fn == "MRCONEE" or fn.startswith("MDCINT") or fn.startswith("MDBINT"):
                 shutil.copy(os.path.join(files_dir, fn), os.path.join(d, fn))
         nrep = lay.nrep
-        hdr = np.array([no, nv, 4, 1, 1, 12, nrep if nrep > 1 else 0, 0], dtype=np.int32)
+        opts = (1 if useoe else 0) + (2 if norecmp else 0)
+        hdr = np.array([no, nv, 4, 1, 1, 12, nrep if nrep > 1 else 0, opts], dtype=np.int32)
         with open(os.path.join(d, "ccin.bin"), "wb") as f:
             f.write(hdr.tobytes())
             f.write(np.zeros(no + nv).tobytes())

@@ -29,6 +29,8 @@ change in the generator is caught.
   files joined by continuation records (MDCINT, MDCINT1, MDCINT2).
 * sortbreit_<name>.npz: the same with MRCONEE's BREIT flag set and a second
   seeded list written as MDBINT (Coulomb + Breit classes).
+* sortopt_<name>_<useoe|norecmp>.npz: the same with namelist CCSORT's USEOE
+  or NORECMP set (the orbital energies / Fock blocks GETFM then leaves).
 * mccres_<name>.bin / .npz: the MCCRES restart file relccsd's CCENER wrote
   for a ccsd_ / symccsd_ fixture run (PUTAMPT + CCDUMP), with the T1 / T2,
   IMSTAT and energy records GETAMPT reads back; the generator also checks
@@ -170,6 +172,26 @@ def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
 # the list of sort_k9_5o spread over 3 files (MDCINT, MDCINT1, MDCINT2), read by the
 # reference's continuation records
 SORT_CONT_CASES = [("k9_5o", 9, 5, 4, 0.8, 3)]
+# namelist CCSORT options: USEOE (keep MRCONEE's orbital energies) and NORECMP
+# (Fock matrix = diag of those energies, not recomputed)
+SORT_OPT_CASES = [("k7_6o", 7, 6, 3, 0.5, True, False), ("k7_6o", 7, 6, 3, 0.5, False, True)]
+
+
+def sort_opt_case(name, nkr, nocc, seed, keep, useoe, norecmp):
+    import tempfile
+    from molfdir_amd import mdcint
+    with tempfile.TemporaryDirectory() as d:
+        mdcint.synthetic_files(d, nkr, nocc, seed, keep)
+        r = refrun.run_sort(d, nocc, 2 * nkr - nocc, useoe=useoe, norecmp=norecmp)
+    tag = "useoe" if useoe else "norecmp"
+    keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
+    np.savez_compressed(
+        os.path.join(HERE, f"sortopt_{name}_{tag}.npz"),
+        params=np.array([nkr, nocc, seed, int(useoe), int(norecmp)], dtype=np.int64), keep=keep,
+        E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
+    print(name, tag, r["E0"], r["E1"], r["ESCF"])
+
+
 # Coulomb + Breit lists (MRCONEE's BREIT flag set, MDBINT beside MDCINT)
 SORT_BREIT_CASES = [("k7_6o", 7, 6, 3, 0.5), ("k9_5o", 9, 5, 4, 0.8)]
 
@@ -423,6 +445,8 @@ def main():
             sort_case(*case)
         for case in SORT_BREIT_CASES:
             sort_breit_case(*case)
+        for case in SORT_OPT_CASES:
+            sort_opt_case(*case)
         for case in SYM_SORT_CASES:
             sym_sort_case(*case)
         return
@@ -430,6 +454,8 @@ def main():
         sort_case(*case)
     for case in SORT_BREIT_CASES:
         sort_breit_case(*case)
+    for case in SORT_OPT_CASES:
+        sort_opt_case(*case)
     for case in SYM_SORT_CASES:
         sym_sort_case(*case)
     if "--t3-only" in sys.argv:

@@ -83,6 +83,25 @@ def test_read_molfdir_adds_the_breit_list(name, tmp_path):
         r.read_molfdir(str(tmp_path), [nocc])
 
 
+@pytest.mark.parametrize("name", names("sortopt"))
+def test_read_molfdir_ccsort_options(name, tmp_path):
+    """Namelist CCSORT's USEOE / NORECMP through rcc_read_molfdir_opts: the
+    Fock blocks, orbital energies and E0 / E1 relccsd's GETFM leaves."""
+    from molfdir_amd import mdcint
+    g = load("sortopt", name)
+    nkr, nocc, seed, useoe, norecmp = (int(x) for x in g["params"])
+    print(f"sortopt fixture {name}", flush=True)
+    mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]))
+    r = RelCCSD(nocc, 2 * nkr - nocc)
+    e = r.read_molfdir(str(tmp_path), [nocc], useoe=bool(useoe), norecmp=bool(norecmp))
+    for c in CLASSES:
+        assert np.array_equal(r.get(c), g[c]), c
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(r.get(k) - g[k]).max() < 1e-14, k
+    assert np.abs(r.get_eps() - g["eps"]).max() < 1e-14
+    assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+
+
 def test_device_list_entry_point_gives_the_same_classes(tmp_path):
     g = load("sort", "k9_5o")
     gen, nkr, nocc = regenerate(g, tmp_path)

@@ -154,3 +154,25 @@ def test_oracle_breit_sort_matches_reference(name, tmp_path):
     wrong_b = ro.sort_classes(sp, kr, *brt)
     wrong_c = ro.sort_classes(sp, kr, *coul)
     assert any(not np.array_equal(wrong_b[c] + wrong_c[c], g[c]) for c in CLASSES)
+
+
+@pytest.mark.parametrize("name", names("sortopt"))
+def test_oracle_ccsort_options_match_reference(name, tmp_path):
+    """Namelist CCSORT: USEOE keeps MRCONEE's orbital energies; NORECMP takes
+    F = diag of them (FVO = 0), E0 = their occupied sum, E1 from the integrals
+    (GETFM ccints.F:1080, 1150-1198, 1260, 1287).  The classes do not change."""
+    g = load("sortopt", name)
+    nkr, nocc, seed, useoe, norecmp = (int(x) for x in g["params"])
+    gen = mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]))
+    sp = ro.Spinors(np.ones(2 * nkr, int), [nocc])
+    cl = ro.sort_classes(sp, ro.kr_table(gen["kr"]), gen["ikr"], gen["jkr"], gen["kkr"],
+                         gen["lkr"], gen["val"])
+    for c in CLASSES:
+        assert np.array_equal(cl[c], g[c]), c
+    f = ro.fock(sp, gen["h1"], cl, eps_file=gen["eps"], useoe=bool(useoe), norecmp=bool(norecmp))
+    for k in ("FOO", "FVO", "FVV"):
+        assert np.abs(f[k] - g[k]).max() < 1e-14, k
+    assert np.abs(f["eps"] - g["eps"]).max() < 1e-14
+    assert abs(f["E0"] - float(g["E0"])) < 1e-13 and abs(f["E1"] - float(g["E1"])) < 1e-13
+    plain = ro.fock(sp, gen["h1"], cl)
+    assert np.abs(plain["eps"] - g["eps"]).max() > 1e-6
