@@ -188,14 +188,17 @@ def abelian_multb(nrep: int) -> np.ndarray:
     return out
 
 
-def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float = 0.7):
+def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float = 0.7,
+                        breit: bool = False):
     """MRCONEE + MDCINT with the symmetry of Z2^k x {fermion, boson}
     (len(nkr_irrep) = 2, 4 or 8 fermion irreps).  Both members of a Kramers
     pair lie in the pair's irrep; spinors are written in a shuffled order, and
     within each irrep the first nocc_irrep[r] of them (file order) are
     occupied; the one-electron matrix is irrep-diagonal; the canonical list
     keeps only integrals whose four Kramers pairs have irreps that multiply
-    to the totally symmetric one (so every image is allowed)."""
+    to the totally symmetric one (so every image is allowed).  breit:
+    MRCONEE's BREIT flag set and a second list, filtered the same way, as
+    MDBINT."""
     import os
     nrep = len(nkr_irrep)
     rng = np.random.Generator(np.random.PCG64(seed + 104729))
@@ -222,8 +225,14 @@ def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float =
     ik, jk, kk, lk, val = ik[ok], jk[ok], kk[ok], lk[ok], val[ok]
     nelec = np.array(nocc_irrep, dtype=np.int32)
     write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, irreps, nelec, h1,
-                  multb=abelian_multb(nrep))
+                  multb=abelian_multb(nrep), breit=breit)
     write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val)
+    if breit:
+        # a second list with the same symmetry filter, as MDBINT
+        bl = synthetic_list(nkr, seed + 1000, keep, scale=0.01)
+        okb = (g[np.abs(bl[0]) - 1] ^ g[np.abs(bl[1]) - 1] ^ g[np.abs(bl[2]) - 1] ^
+               g[np.abs(bl[3]) - 1]) == 0
+        write_mdcint(os.path.join(dirpath, "MDBINT"), kr, *(x[okb] for x in bl))
     no_i = [int(x) for x in nelec]
     nv_i = [int(np.sum(irreps == r + 1)) - no_i[r] for r in range(nrep)]
     return {"nsp": nsp, "irreps": irreps, "nelec": nelec, "no": no_i, "nv": nv_i, "kr": kr,

@@ -29,6 +29,8 @@ change in the generator is caught.
   files joined by continuation records (MDCINT, MDCINT1, MDCINT2).
 * sortbreit_<name>.npz: the same with MRCONEE's BREIT flag set and a second
   seeded list written as MDBINT (Coulomb + Breit classes).
+* symsortbreit_<name>.npz: a symsort_ case with MRCONEE's BREIT flag and a
+  symmetry-filtered MDBINT.
 * sortopt_<name>_<useoe|norecmp>.npz: the same with namelist CCSORT's USEOE
   or NORECMP set (the orbital energies / Fock blocks GETFM then leaves).
 * mccres_<name>.bin / .npz: the MCCRES restart file relccsd's CCENER wrote
@@ -151,18 +153,19 @@ SYM_SORT_CASES = [
 ]
 
 
-def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
+def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep, breit=False):
     import hashlib
     import tempfile
     from molfdir_amd import mdcint
     with tempfile.TemporaryDirectory() as d:
-        gen = mdcint.synthetic_files_sym(d, nkr_irrep, nocc_irrep, seed, keep)
+        gen = mdcint.synthetic_files_sym(d, nkr_irrep, nocc_irrep, seed, keep, breit=breit)
         digest = hashlib.sha256(open(os.path.join(d, "MDCINT"), "rb").read()).hexdigest()
         r = refrun.run_sort(d, sum(gen["no"]), sum(gen["nv"]),
                             sym=(gen["irreps"], gen["no"], gen["nv"]))
     keys = ("FOO", "FVO", "FVV", "eps", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
     np.savez_compressed(
-        os.path.join(HERE, f"symsort_{name}.npz"), nkr_irrep=np.array(nkr_irrep),
+        os.path.join(HERE, f"{'symsortbreit' if breit else 'symsort'}_{name}.npz"),
+        nkr_irrep=np.array(nkr_irrep),
         nocc_irrep=np.array(nocc_irrep), seed=seed, keep=keep, mdcint_sha256=digest,
         sym_no=np.array(gen["no"]), sym_nv=np.array(gen["nv"]),
         E0=r["E0"], E1=r["E1"], ESCF=r["ESCF"], **{k: r[k] for k in keys})
@@ -174,6 +177,7 @@ def sym_sort_case(name, nkr_irrep, nocc_irrep, seed, keep):
 SORT_CONT_CASES = [("k9_5o", 9, 5, 4, 0.8, 3)]
 # namelist CCSORT options: USEOE (keep MRCONEE's orbital energies) and NORECMP
 # (Fock matrix = diag of those energies, not recomputed)
+SYM_SORT_BREIT_CASES = [("s4_k8", (3, 2, 2, 1), (2, 2, 1, 1), 1, 0.8)]
 SORT_OPT_CASES = [("k7_6o", 7, 6, 3, 0.5, True, False), ("k7_6o", 7, 6, 3, 0.5, False, True)]
 
 
@@ -449,6 +453,8 @@ def main():
             sort_opt_case(*case)
         for case in SYM_SORT_CASES:
             sym_sort_case(*case)
+        for case in SYM_SORT_BREIT_CASES:
+            sym_sort_case(*case, breit=True)
         return
     for case in SORT_CASES + SORT_CONT_CASES:
         sort_case(*case)
@@ -458,6 +464,8 @@ def main():
         sort_opt_case(*case)
     for case in SYM_SORT_CASES:
         sym_sort_case(*case)
+    for case in SYM_SORT_BREIT_CASES:
+        sym_sort_case(*case, breit=True)
     if "--t3-only" in sys.argv:
         for case in T3_CASES:
             t3_case(*case)

@@ -183,6 +183,25 @@ def test_handle_restart_file_round_trip(name, tmp_path):
     assert np.array_equal(a1, m["T1"]) and np.array_equal(a2, m["T2"])
 
 
+def test_read_molfdir_symmetric_breit_matches_reference_sort(tmp_path):
+    # a symmetry-blocked handle with the Breit list (BREIT flag, MDBINT): the
+    # Breit pass also sorts into the C1-sized VVVV scratch before the irrep
+    # blocks are gathered
+    from test_oracle_rdints import regenerate_sym
+    for name in names("symsortbreit"):
+        g = load("symsortbreit", name)
+        print(f"symsortbreit fixture {name}", flush=True)
+        gen = regenerate_sym(g, tmp_path, breit=True)
+        no_l, nv_l = g["sym_no"].tolist(), g["sym_nv"].tolist()
+        r = RelCCSD(sum(no_l), sum(nv_l), sym=(no_l, nv_l))
+        e = r.read_molfdir(str(tmp_path), gen["nelec"].tolist())
+        for c in CLASSES:
+            assert np.array_equal(r.get(c), g[c]), c
+        for k in ("FOO", "FVO", "FVV"):
+            assert np.abs(r.get(k) - g[k]).max() < 1e-14, k
+        assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+
+
 @pytest.mark.parametrize("name", names("symsort"))
 def test_read_molfdir_symmetric_matches_reference_sort(name, tmp_path):
     # a symmetry-blocked handle: the classes (VOVV / VVVV in their irrep

@@ -71,9 +71,9 @@ def test_planted_first_write_wins_is_caught(tmp_path, monkeypatch):
     assert any(not np.array_equal(cl[c], g[c]) for c in CLASSES)
 
 
-def regenerate_sym(g, d):
+def regenerate_sym(g, d, breit=False):
     gen = mdcint.synthetic_files_sym(str(d), g["nkr_irrep"].tolist(), g["nocc_irrep"].tolist(),
-                                     int(g["seed"]), float(g["keep"]))
+                                     int(g["seed"]), float(g["keep"]), breit=breit)
     digest = hashlib.sha256(open(os.path.join(str(d), "MDCINT"), "rb").read()).hexdigest()
     assert digest == str(g["mdcint_sha256"]), "synthetic MDCINT generator changed"
     return gen
