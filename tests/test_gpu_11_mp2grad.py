@@ -10,6 +10,9 @@ ccgrad.F:261-347).
   (600 VO pairs): X to 1e-12, the Z-vector to 1e-10.
 * Symmetry-blocked handles (2 and 4 irreps) against the reference's
   symmetric runs, in SYMTAB's layouts.
+* At 20o/200v (4000 VO pairs) the Z-vector satisfies the response
+  equations built from the handle's own classes, to the convergence
+  threshold.
 * Two calls give bit-identical densities; the CCSD amplitudes of the handle
   are left alone; level 2 or the density arrays before the density, and a
   DIIS dimension beyond 16, are refused.
@@ -134,3 +137,40 @@ def test_refusals():
         r.mp2_density(maxdim=17)
     with pytest.raises(RccError):
         r.get("DOO")
+
+
+def test_zvector_solves_the_response_equations_at_20o200v():
+    """Size-independent check at 20o/200v (4000 VO pairs): the Z-vector the
+    GPU converged satisfies RPAA's fixed point (eps_a - eps_i) X - A X +
+    B conj X = XVO, with A(ai,ck) = <ak||ci>, B(ai,ck) = <ac||ik> unpacked
+    from the handle's own VOVO / VVOO classes (torch on the GPU), to the
+    convergence threshold; Y = conj X."""
+    no, nv = 20, 200
+    s = synth.make_system(no, nv, seed=5, scale=0.01, device="cuda")
+    r = solver_for(s)
+    nit, rms = r.mp2_density(maxit=100, maxdim=8, ntol=11)
+    assert rms < 1e-11
+    dev = torch.device("cuda")
+    nvo = no * nv
+    vovo = torch.from_numpy(r.get("VOVO")).to(dev).reshape(no, nv, no, nv)   # [i][c][k][a]
+    A = vovo.permute(0, 3, 2, 1).reshape(nvo, nvo)                          # rows (i,a), cols (k,c)
+    del vovo
+    p, q = synth.pair_index(nv)
+    u, w = synth.pair_index(no)
+    vv = torch.from_numpy(r.get("VVOO")).to(dev).reshape(len(u), len(p)).T   # [(a>c), (i>k)]
+    D = torch.zeros(nv, nv, no, no, dtype=torch.complex128, device=dev)
+    P, Q = torch.from_numpy(p).to(dev)[:, None], torch.from_numpy(q).to(dev)[:, None]
+    U, W = torch.from_numpy(u).to(dev)[None, :], torch.from_numpy(w).to(dev)[None, :]
+    D[P, Q, U, W] = vv
+    D[Q, P, U, W] = -vv
+    D[P, Q, W, U] = -vv
+    D[Q, P, W, U] = vv
+    B = D.permute(2, 0, 3, 1).reshape(nvo, nvo)                              # <ac||ik>
+    del D, vv
+    eps = torch.from_numpy(r.get_eps()).to(dev)
+    de = (eps[no:][None, :] - eps[:no][:, None]).reshape(nvo).to(torch.complex128)
+    X = torch.from_numpy(r.get("DVO")).to(dev)
+    rhs = torch.from_numpy(r.get("XVO")).to(dev)
+    res = de * X - A @ X + B @ torch.conj(X) - rhs
+    assert float(res.abs().max()) < 1e-9 * float(rhs.abs().max()), float(res.abs().max())
+    assert np.array_equal(r.get("DOV"), np.conj(r.get("DVO")))
