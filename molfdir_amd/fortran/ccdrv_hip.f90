@@ -6,7 +6,9 @@
 !        NTOL, NREP (0 = C1), LAMBDA; JOB 2 adds the triples corrections
 !        after CCSD, as CCENER does with DOCCSDT, ccdriv.F:1799-1837; JOB 4
 !        sorts the MOLFDIR files MRCONEE + MDCINT of the working directory
-!        instead (CCSETI -> RDINTS, C1, NELEC = NO) and writes E0, E1, ECORE,
+!        instead (CCSETI -> RDINTS, C1, NELEC = NO; header word 8 holds
+!        namelist CCSORT's options, 1 USEOE + 2 NORECMP; with MRCONEE's BREIT
+!        flag MDBINT is sorted and added) and writes E0, E1, ECORE,
 !        FOO, FVO, FVV, the six classes and the orbital energies to
 !        ccout.bin, then stops; JOB 5 runs CCFOPR's MP2 block instead of
 !        CCSD, ccdriv.F:2028-2090: after MP2, the MP2 relaxed density with
@@ -93,8 +95,10 @@ program ccdrv_hip
     ! integral sort from the MOLFDIR files (CCSETI -> RDINTS -> GETFM)
     close (30)
     nel(1) = no
-    call rcc_check(h, rcc_read_molfdir(h, 'MRCONEE'//c_null_char, 'MDCINT'//c_null_char, &
-                                       nel, 1_c_int, e01), 'RDINTS')
+    ! header word 8: namelist CCSORT's options (1 USEOE + 2 NORECMP)
+    call rcc_check(h, rcc_read_molfdir_opts(h, 'MRCONEE'//c_null_char, 'MDCINT'//c_null_char, &
+                   nel, 1_c_int, iand(hdr(8), 1_c_int), iand(hdr(8), 2_c_int) / 2_c_int, e01), &
+                   'RDINTS')
     write (*, '(a,t40,f25.15)') ' Nuclear repulsion + core energy :', e01(3)
     write (*, '(a,t40,f25.15)') ' Zero order electronic energy :', e01(1)
     write (*, '(a,t40,f25.15)') ' First order electronic energy :', e01(2)

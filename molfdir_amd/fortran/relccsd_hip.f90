@@ -102,6 +102,16 @@ module relccsd_hip
       integer(c_int), value :: nnelec
       real(c_double), intent(out) :: e0e1(3)
     end function
+    ! the same with namelist CCSORT's USEOE / NORECMP (GETFM ccints.F:1080-1287)
+    integer(c_int) function rcc_read_molfdir_opts(h, mrconee, mdcint, nelec, nnelec, useoe, &
+        norecmp, e0e1) bind(c, name="rcc_read_molfdir_opts")
+      import :: c_int, c_ptr, c_double, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: mrconee(*), mdcint(*)
+      integer(c_int), intent(in) :: nelec(*)
+      integer(c_int), value :: nnelec, useoe, norecmp
+      real(c_double), intent(out) :: e0e1(3)
+    end function
     integer(c_int) function rcc_get_array(h, which, out, n) bind(c, name="rcc_get_array")
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h

@@ -130,18 +130,24 @@ def test_inconsistent_inputs_are_rejected(tmp_path):
         r2.read_molfdir(str(tmp_path / "missing"), [nocc])
 
 
-def test_fortran_driver_sorts_molfdir_files(tmp_path):
-    # JOB 4 of the Fortran driver: rcc_read_molfdir through ISO_C_BINDING
+@pytest.mark.parametrize("kind,name,opts", [("sort", "k7_6o", 0),
+                                             ("sortopt", "k7_6o_norecmp", 2),
+                                             ("sortopt", "k7_6o_useoe", 1)])
+def test_fortran_driver_sorts_molfdir_files(kind, name, opts, tmp_path):
+    # JOB 4 of the Fortran driver: rcc_read_molfdir_opts through ISO_C_BINDING,
+    # header word 8 carrying namelist CCSORT's options
     import os
     import subprocess
+    from molfdir_amd import mdcint
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                        "molfdir_amd", "fortran", "ccdrv_hip.x")
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built")
-    g = load("sort", "k7_6o")
-    gen, nkr, nocc = regenerate(g, tmp_path)
+    g = load(kind, name)
+    nkr, nocc, seed = (int(x) for x in g["params"][:3])
+    mdcint.synthetic_files(str(tmp_path), nkr, nocc, seed, float(g["keep"]))
     nv = 2 * nkr - nocc
-    hdr = np.array([nocc, nv, 4, 1, 1, 12, 0, 0], dtype=np.int32)
+    hdr = np.array([nocc, nv, 4, 1, 1, 12, 0, opts], dtype=np.int32)
     with open(tmp_path / "ccin.bin", "wb") as f:
         f.write(hdr.tobytes())
         f.write(np.zeros(nocc + nv).tobytes())
