@@ -359,22 +359,29 @@ __global__ void sort_claim_kernel(SortDev d, const int4* __restrict__ idx, int64
     }
 }
 
+// The key holder of each (class element, part) writes its value: the direct
+// pass (which = 0) stores d into the zeroed class, the exchange pass (which =
+// 1, launched after it) replaces the element by fl(d - e), RDINTS's XAXPY of
+// the exchange part (ccints.F:221-225).  One holder per element and part:
+// plain stores, no atomics.
 __global__ void sort_write_kernel(SortDev d, const int4* __restrict__ idx,
                                   const double2* __restrict__ val, int64_t m, int64_t n0,
                                   const int* __restrict__ kr, int nkr,
-                                  const int* __restrict__ spos) {
+                                  const int* __restrict__ spos, int which) {
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
          n += (int64_t)gridDim.x * blockDim.x) {
         const unsigned long long base = (unsigned long long)(n0 + n) * 16ull + 1ull;
-        const double2 x = val[n];
         const int4 q = idx[n];
         if (!valid(q, nkr)) continue;
+        const double2 x = val[n];
         expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int sre, int sim) {
-            if (d.key[part][c][e] == base + (unsigned long long)v) {
-                // direct - exchange, added into zeroed memory (exact in any order)
-                const double s = part == 0 ? 1.0 : -1.0;
-                atomicAdd(&d.cls[c][e].x, s * sre * x.x);
-                atomicAdd(&d.cls[c][e].y, s * sim * x.y);
+            if (part != which || d.key[part][c][e] != base + (unsigned long long)v) return;
+            double2* p = d.cls[c] + e;
+            if (which == 0) {
+                *p = make_double2(sre * x.x, sim * x.y);
+            } else {
+                const double2 o = *p;
+                *p = make_double2(o.x - sre * x.x, o.y - sim * x.y);
             }
         });
     }
@@ -534,7 +541,8 @@ int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos
     const int g = grid_for(m);
     if (m > 0) {
         sort_claim_kernel<<<g, 256, 0, st>>>(d, d_idx, m, 0, d_kr, nkr, d_spos, bad);
-        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos);
+        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos, 0);
+        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos, 1);
     }
     FECK(hipGetLastError());
     FECK(hipEventRecord(e1, st));
