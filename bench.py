@@ -44,10 +44,13 @@ PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 # HBM bytes per launch (read + write) of the roofline kernels at 30o/300v, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs scaled by what the same
 # counters read for a 1 GiB device copy (x2.000, x1.000): profiles/*_pmc.md
-TRAFFIC = {"ladder": 217.08e9, "mp2g_xgemm": 23.57e9, "mp2g_zgemv": 2.605e9}
+TRAFFIC = {"ladder": 217.08e9, "mp2g_xgemm": 23.57e9, "mp2g_zgemv": 2.605e9,
+           "t3_gemm": 30.39e9, "sort": 67.16e9}
 TRAFFIC_SRC = {"ladder": "profiles/zgemm_hbm_traffic_cfg3_30o300v_pmc.md",
                "mp2g_xgemm": "profiles/mp2g_30o300v_pmc.md",
-               "mp2g_zgemv": "profiles/mp2g_30o300v_pmc.md"}
+               "mp2g_zgemv": "profiles/mp2g_30o300v_pmc.md",
+               "t3_gemm": "profiles/t3_sort_hbm_traffic_pmc.md",
+               "sort": "profiles/t3_sort_hbm_traffic_pmc.md"}
 PEAK_BF16_MFMA = 2500.0   # TFLOP/s, MI355X dense BF16 matrix
 PEAK_INT8_MFMA = 5000.0   # TOP/s, MI355X dense INT8 matrix
 CPU_SAMPLE = (10, 100)    # relccsd reference sample size (o, v)
@@ -283,7 +286,11 @@ def bench_triples(args):
         "ccsd": {"nit": out["nit"], "rms": rms, "total_s": t_cc},
         "roofline": {"bound": "mfma", "kernel": "t3_gemm_kernel (segmented-K, all launches)",
                      "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP64_MFMA, "traffic": None,
+                     "frac": ach / PEAK_FP64_MFMA,
+                     "traffic": TRAFFIC["t3_gemm"] if (no, nv, parts["chunk"]) == (30, 300, 19)
+                     else None,
+                     "traffic_unit": "bytes per launch of 19 triples",
+                     "traffic_source": TRAFFIC_SRC["t3_gemm"],
                      "algorithmic_flops_per_triple": 8.0 * nv * nvvt * kp},
         "kernels_ms": {"gemm_total": parts["gemm"], "energy_total": parts["energy"],
                        "gemm_per_triple": parts["gemm"] / ntri,
@@ -377,7 +384,8 @@ def bench_sort(args):
         "canonical_per_s": m / (ms * 1e-3), "images_per_s": 16 * m / (ms * 1e-3),
         "roofline": {"bound": "hbm", "kernel": "sort_claim + sort_write (whole sort)",
                      "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
-                     "traffic": None,
+                     "traffic": TRAFFIC["sort"] if (no, nv) == (20, 200) else None,
+                     "traffic_unit": "bytes per sort", "traffic_source": TRAFFIC_SRC["sort"],
                      "algorithmic_bytes": alg_bytes,
                      "note": "algorithmic bytes = the canonical list read once (32 B each) + "
                              "every class element written once (16 B); the 64-bit order keys "
