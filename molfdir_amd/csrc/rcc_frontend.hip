@@ -14,12 +14,13 @@
 //
 // relccsd stores every expanded value by assignment while it reads the file,
 // so when two expansions land on one element the later one in file order
-// is kept.  The device reproduces that exactly in two passes: the first
-// takes, per element and part, the maximum of a 64-bit order key
-// (canonical index x 16 + expansion index) with atomicMax, the second lets
-// only the holder of that key write.  The two surviving writes of an
-// element (direct, exchange) are added into a zeroed array, which is
-// exact in either order: class = direct - exchange, as RDINTS's XAXPY.
+// is kept.  The device reproduces that exactly: a claim pass takes, per
+// element and part, the maximum of a 64-bit order key (canonical index x 16
+// + expansion index) with atomicMax, and only the holder of that key writes.
+// The writes come in two ordered passes with plain stores: the direct pass
+// stores d into the zeroed class, and the exchange pass, launched after it,
+// replaces the element by fl(d - e) -- RDINTS's sequential stores followed
+// by its direct - exchange XAXPY.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -89,10 +90,16 @@ int read_mrconee(const char* path, MolfdirOne& out, std::string& err) {
     out.breit = get<int32_t>(b, p) != 0;
     out.ecore = get<double>(b, p);
     // 2: NSYMRP + names, 3: NREP + abelian names, 4: MULTB
-    if (!r.next(b, err)) return -1;
+    if (!r.next(b, err) || b.size() < 4) {
+        if (err.empty()) err = std::string(path) + ": record 2";
+        return -1;
+    }
     p = 0;
     out.nsymrp = get<int32_t>(b, p);
-    if (!r.next(b, err)) return -1;
+    if (!r.next(b, err) || b.size() < 4) {
+        if (err.empty()) err = std::string(path) + ": record 3";
+        return -1;
+    }
     p = 0;
     out.nrep = get<int32_t>(b, p);
     if (!r.next(b, err)) return -1;

@@ -219,6 +219,7 @@ int dalloc(rcc_t* h, double2** p, int64_t n) {
     if (e != hipSuccess) {
         h->err = "hipMalloc of " + std::to_string((double)n * 16.0 / 1e9) + " GB failed: " +
                  hipGetErrorString(e);
+        (void)hipGetLastError();     // reported here, not again by a later launch check
         *p = nullptr;
         return -1;
     }
@@ -231,6 +232,7 @@ int dalloc_raw(rcc_t* h, T** p, int64_t bytes) {
     hipError_t e = hipMalloc((void**)p, (size_t)std::max<int64_t>(bytes, 16));
     if (e != hipSuccess) {
         h->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+        (void)hipGetLastError();
         return -1;
     }
     h->allocs.push_back(*p);
@@ -2476,7 +2478,14 @@ int rcc_set_split_digits(rcc_t* h, int digits) {
 
 int rcc_set_complex_product(rcc_t* h, int three_mult) {
     if (!h) return -1;
-    h->gemm_gauss = three_mult ? 1 : 0;
+    const int g = three_mult ? 1 : 0;
+    if (g != h->gemm_gauss) {
+        // a captured iteration and any compiled batch plan hold the old
+        // product's kernels: rebuild them on the next call
+        drop_graph(h);
+        drop_plans_of(h);
+        h->gemm_gauss = g;
+    }
     return 0;
 }
 

@@ -465,6 +465,7 @@ int round16(int x) { return (x + 15) / 16 * 16; }
 }  // namespace
 
 struct T3State {
+    bool ready = false;            // every buffer below allocated for (no, nv, budget)
     int no = 0, nv = 0, zmax = 0;
     int64_t ntri = 0, nblk = 0, budget = 0;
     int4* tri = nullptr;
@@ -486,6 +487,7 @@ static void t3_release(T3State* s) {
     s->blk = nullptr;
     s->T2F = s->Apk = s->U = nullptr;
     s->part = s->sums = nullptr;
+    s->ready = false;
 }
 
 void t3_free(T3State* s) {
@@ -499,17 +501,22 @@ int64_t t3_default_budget() {
     return std::max<int64_t>(mb, 1) << 20;
 }
 
+// On failure every buffer is released, so that a later call re-allocates
+// instead of finding a half-built state, and the runtime's last-error state
+// is cleared, so that the failure is not reported again by a later check.
 #define T3CK(call)                                                              \
     do {                                                                        \
         hipError_t e_ = (call);                                                 \
         if (e_ != hipSuccess) {                                                 \
             err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+            (void)hipGetLastError();                                            \
+            t3_release(s);                                                      \
             return -1;                                                          \
         }                                                                       \
     } while (0)
 
 static int t3_prepare(T3State* s, int no, int nv, std::string& err) {
-    if (s->tri && s->no == no && s->nv == nv && s->budget == t3_default_budget()) return 0;
+    if (s->ready && s->no == no && s->nv == nv && s->budget == t3_default_budget()) return 0;
     t3_release(s);
     s->no = no;
     s->nv = nv;
@@ -531,7 +538,10 @@ static int t3_prepare(T3State* s, int no, int nv, std::string& err) {
     // grid.y holds the triples of a launch
     s->zmax = (int)std::max<int64_t>(
         1, std::min<int64_t>({s->ntri, s->budget / std::max<int64_t>(per, 1), 16384}));
-    if (s->ntri == 0 || s->nblk == 0) return 0;
+    if (s->ntri == 0 || s->nblk == 0) {
+        s->ready = true;
+        return 0;
+    }
     T3CK(hipMalloc((void**)&s->tri, tri.size() * sizeof(int4)));
     T3CK(hipMemcpy(s->tri, tri.data(), tri.size() * sizeof(int4), hipMemcpyHostToDevice));
     T3CK(hipMalloc((void**)&s->blk, blk.size() * sizeof(int4)));
@@ -543,6 +553,7 @@ static int t3_prepare(T3State* s, int no, int nv, std::string& err) {
     T3CK(hipMalloc((void**)&s->part, (size_t)s->zmax * s->nblk * 3 * sizeof(double)));
     s->nsums = (s->ntri + s->zmax - 1) / s->zmax;
     T3CK(hipMalloc((void**)&s->sums, (size_t)s->nsums * 3 * sizeof(double)));
+    s->ready = true;
     return 0;
 }
 

@@ -3,7 +3,10 @@
 ! term handed to librelccsd_hip.so through ISO_C_BINDING.
 !
 ! Input  ccin.bin (stream): 8 int32 header (NO, NV, JOB, MAXIT, MAXDIM,
-!        NTOL, NREP (0 = C1), LAMBDA; JOB 2 adds the triples corrections
+!        NTOL, NREP (0 = C1), FLAGS: 1 LAMBDA, 2 NFROZ (namelist RELCCSD's
+!        frozen occupied spinors per irrep, int32, follow the symmetry data;
+!        the lowest NFROZ(irp) occupied spinors of each irrep are frozen, as
+!        SYMTAB sets IFROZ, ccdriv.F:838-849); JOB 2 adds the triples corrections
 !        after CCSD, as CCENER does with DOCCSDT, ccdriv.F:1799-1837; JOB 4
 !        sorts the MOLFDIR files MRCONEE + MDCINT of the working directory
 !        instead (CCSETI -> RDINTS, C1, NELEC = NO; header word 8 holds
@@ -41,7 +44,8 @@ program ccdrv_hip
   real(c_double), allocatable :: eps(:), foo(:), fvo(:), fvv(:), buf(:), t1(:), t2(:), dens(:)
   integer(c_int) :: lnit
   real(c_double) :: lrms
-  integer(c_int), allocatable, target :: nos(:), nvs(:), multb(:, :)
+  integer(c_int), allocatable, target :: nos(:), nvs(:), multb(:, :), frz(:)
+  integer(c_int), allocatable :: nfroz(:)
   integer(c_int), allocatable :: irrsp(:), mult(:, :)
   type(rcc_sym_t) :: sym
   type(c_ptr) :: h
@@ -80,11 +84,23 @@ program ccdrv_hip
     nos(1) = no
     nvs(1) = nv
   end if
+  allocate (nfroz(nrep), frz(max(1, no)))
+  nfroz = 0
+  if (hdr(3) /= 4 .and. iand(hdr(8), 2_c_int) /= 0) read (30) nfroz
+  ! IFROZ: the first NFROZ(irp) occupied spinors of every irrep (irrep-major order)
+  frz = 0
+  j = 0
+  do i = 1, nrep
+    if (nfroz(i) < 0 .or. nfroz(i) > nos(i)) stop 'NFROZ exceeds the occupied spinors'
+    frz(j + 1:j + nfroz(i)) = 1
+    j = j + nos(i)
+  end do
 
   sym%nrep = nrep
   sym%no = c_loc(nos)
   sym%nv = c_loc(nvs)
   sym%frozen = c_null_ptr
+  if (any(nfroz > 0)) sym%frozen = c_loc(frz)
   sym%multb = c_null_ptr
   if (nrep > 1) sym%multb = c_loc(multb)
   rc = rcc_create(sym, h)
@@ -262,7 +278,7 @@ program ccdrv_hip
     energ(1) = ecc + et(1) + et(2)
   end if
   call rcc_check(h, rcc_write_mccres(h, 'MCCRES'//c_null_char, imstat, energ), 'PUTAMPT')
-  if (hdr(8) /= 0) then
+  if (iand(hdr(8), 1_c_int) /= 0) then
     ! Lambda equations and the one-particle response density
     call rcc_check(h, rcc_lambda(h, lnit, lrms), 'LAMBDA')
     write (*, '(a,i4,a,es10.3)') ' Lambda equations: iterations', lnit, '  RMS', lrms

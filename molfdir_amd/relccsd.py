@@ -444,17 +444,19 @@ class RelCCSD:
 
     def cceqn(self):
         """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
-        Returns dict(emp2, rows=[(nit, Ec, rms)], eccsd, nit)."""
+        Returns dict(emp2, rows=[(nit, Ec, rms)], t1diag=[T1DIAG of every
+        iteration, as CCEQN prints it beside Ec], eccsd, nit)."""
         emp2 = self.mp2()
-        rows = []
+        rows, diag = [], []
         conv = 10.0 ** (-self.ntol)
         for _ in range(self.maxit):
             nit, e, r = self.iterate()
             rows.append((nit, e, r))
+            diag.append(self.t1diag())
             if r < conv:
                 break
-        return {"emp2": emp2, "rows": rows, "eccsd": rows[-1][1] if rows else 0.0,
-                "nit": len(rows)}
+        return {"emp2": emp2, "rows": rows, "t1diag": diag,
+                "eccsd": rows[-1][1] if rows else 0.0, "nit": len(rows)}
 
     def last_iteration_ms(self):
         ms = ctypes.c_double()

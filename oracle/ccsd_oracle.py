@@ -288,8 +288,42 @@ def cceng(S, t1, t2):
 
 
 def t1diag(S, t1):
-    """T1DIAG cceqns.F:2453-2519."""
+    """T1DIAG cceqns.F:2453-2519: sqrt(||T1||^2 / N), N = sum of NE over the
+    irreps, NE = NO (ccdriv.F:865): every occupied spinor, frozen ones
+    included."""
     return float(np.sqrt(np.real(np.vdot(t1, t1)) / S.no))
+
+
+def frozen_mask(sysd):
+    """IFROZ of SYMTAB (ccdriv.F:838-849) as a mask over the occupied
+    spinors in C1 working order (irrep-major): the first NFROZ(irp)
+    occupied spinors of each irrep, in file order, are frozen.  None when
+    the system has no frozen spinor."""
+    nfroz = sysd.get("nfroz")
+    if nfroz is None or not any(int(x) for x in nfroz):
+        return None
+    no_l = list(map(int, sysd["sym"][0])) if sysd.get("sym") is not None else [int(sysd["no"])]
+    if len(nfroz) != len(no_l):
+        raise ValueError("nfroz needs one count per irrep")
+    mask = []
+    for n, f in zip(no_l, nfroz):
+        if not 0 <= int(f) <= n:
+            raise ValueError("nfroz exceeds the occupied count of an irrep")
+        mask += [True] * int(f) + [False] * (n - int(f))
+    return np.asarray(mask)
+
+
+def zcore(t1, t2, mask):
+    """ZCORE cceqns.F:1864-1922: zero T1(A,I) for frozen I and T2(AB,IJ)
+    when I or J is frozen (dense t1[a,i], t2[a,b,i,j])."""
+    if mask is None:
+        return t1, t2
+    t1 = t1.copy()
+    t2 = t2.copy()
+    t1[:, mask] = 0
+    t2[:, :, mask, :] = 0
+    t2[:, :, :, mask] = 0
+    return t1, t2
 
 
 # --------------------------------------------------------------------------
@@ -320,9 +354,10 @@ def terms(sysd, T1, T2):
     out["S2_T2EQNS"] = L.pt2(s2)
     s2 = bintm(S, t1, t2, s2)
     out["S2_BINTM"] = L.pt2(s2)
-    s1d, s2d = denom(S, s1, s2)
+    s1d, s2d = zcore(*denom(S, s1, s2), frozen_mask(sysd))
     out["S1_DENOM"], out["S2_DENOM"] = L.pt1(s1d), L.pt2(s2d)
     out["ECC"] = cceng(S, s1d, s2d)[0]
+    out["T1DIAG"] = t1diag(S, s1d)
     return out
 
 
@@ -350,34 +385,38 @@ def mp2eng(S, t1, t2):
 
 
 def mp2(sysd):
-    """MP2EQN cceqns.F:33-139: T = (FVO, VVOO) / D, energy by MP2ENG."""
+    """MP2EQN cceqns.F:33-139: T = (FVO, VVOO) / D, ZCORE (cceqns.F:94),
+    energy by MP2ENG."""
     S = System(sysd)
-    t1, t2 = denom(S, S.fvo, S.vvoo)
+    t1, t2 = zcore(*denom(S, S.fvo, S.vvoo), frozen_mask(sysd))
     return S, t1, t2, mp2eng(S, t1, t2)
 
 
 def cceqn(sysd, maxit=20, maxdim=6, ntol=12):
     """CCEQN cceqns.F:141-643 from the MP2 guess (CCENER ccdriv.F:1753-1782).
-    Returns dict with the per-iteration rows (nit, Ec, rms), final packed
-    T1/T2, the MP2 energy and the iteration count."""
+    Returns dict with the per-iteration rows (nit, Ec, rms), the T1
+    diagnostic of every iteration (t1diag), final packed T1/T2, the MP2
+    energy and the iteration count."""
     S, t1, t2, emp2 = mp2(sysd)
+    mask = frozen_mask(sysd)
     L = S.L
     T1, T2 = L.pt1(t1), L.pt2(t2)
     st = _diis.DiisState(maxdim)
     conv = 10.0 ** (-ntol)
-    rows = []
+    rows, diag = [], []
     nit = 0
     while True:
         nit += 1
         if nit > maxit:
             nit -= 1
             break
-        s1, s2 = update(S, L.t1(T1), L.t2(T2))
+        s1, s2 = zcore(*update(S, L.t1(T1), L.t2(T2)), mask)   # ZCORE cceqns.F:483
         S1, S2, rms = _diis.diisx(st, T1, L.pt1(s1), T2, L.pt2(s2))
         ec = cceng(S, L.t1(S1), L.t2(S2))[0]
         rows.append((nit, ec, rms))
+        diag.append(t1diag(S, L.t1(S1)))
         T1, T2 = S1, S2
         if rms < conv:
             break
-    return {"rows": rows, "T1": T1, "T2": T2, "emp2": emp2, "nit": nit,
+    return {"rows": rows, "t1diag": diag, "T1": T1, "T2": T2, "emp2": emp2, "nit": nit,
             "eccsd": rows[-1][1] if rows else 0.0}

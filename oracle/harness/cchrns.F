@@ -16,7 +16,8 @@ C
 C  Mode 1 then runs CCENER (ccdriv.F:1574): MP2 + CCSD, and writes the
 C  final amplitudes.  Mode 2 reads given T1/T2 and calls the per-term
 C  routines in the order CCEQN uses them (cceqns.F:313-546), writing
-C  every intermediate, so each GPU term can be checked on its own.
+C  every intermediate, so each GPU term can be checked on its own, and
+C  ends with CCENG's energy and T1DIAG (cceqns.F:2453) of the new T1.
 C  Mode 3 reads given T1/T2 and calls T3CORR (cceqns.F:1336) with the
 C  arrays CCENER allocates for it (ccdriv.F:1805-1816), writing the
 C  triples corrections ET1, ET2, ET3 (COMMON /ENERG/).
@@ -67,7 +68,7 @@ C
       REAL*8, ALLOCATABLE :: T1(:),T2(:),S1(:),S2(:),AZ(:),H(:),
      &                       HO(:),HV(:),HOV(:),GOO(:),GVV(:),
      &                       BUF1(:),BUF2(:),BUF3(:)
-      REAL*8 DUM(4),ECC,RBUF(RSRCLEN)
+      REAL*8 DUM(4),ECC,DIAG,RBUF(RSRCLEN)
       REAL*8, ALLOCATABLE :: PB(:),AVO(:),AOV(:),AOO(:),AVV(:),
      &                       ZD(:)
       COMPLEX*16 EPH,CEXPA
@@ -75,7 +76,8 @@ C
       INTEGER ISZ
       CHARACTER*4 RNAM(32)
       CHARACTER*14 SNAM(16)
-      INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16)
+      INTEGER MULT(32,32),IRRSP(MAXSP),NELRP(16),NFRZ(16)
+      CHARACTER*100 FLINE
       CHARACTER*200 LINE
       LOGICAL BRT,DONE
       REAL TC0,TC1
@@ -88,7 +90,9 @@ C
       INTERFACE = 'MOLFDIR   '
 C
 C     Header: NO, NV, MODE, MAXIT, MAXDIM, NTOL, NREP (0 = C1), and for
-C     mode 4 the CCSORT options (1 USEOE + 2 NORECMP), else unused.
+C     mode 4 the CCSORT options (1 USEOE + 2 NORECMP); in the other modes
+C     bit 2 of word 8 says that NFROZ (frozen occupied spinors per irrep,
+C     namelist RELCCSD) follows the symmetry data.
 C     For NREP > 1 the orbital energies are followed by the fermion irrep
 C     (1..NREP) of every spinor, the occupied count per irrep and the
 C     2NREP x 2NREP table MULTB(.,.,1), all int32.
@@ -116,6 +120,11 @@ C
          MULT(1,2) = 1
          MULT(2,2) = 2
       ENDIF
+      DO I = 1, 16
+         NFRZ(I) = 0
+      ENDDO
+      IF (MODE.NE.4.AND.IAND(HDR(8),2).NE.0)
+     &   READ (30) (NFRZ(I),I=1,NREPI)
 C
 C     MRCONEE records 1-5 exactly as RDSYMI reads them (ccdriv.F:2978-
 C     2988); the higher point group is taken equal to the abelian one.
@@ -141,7 +150,9 @@ C
     5 CONTINUE
       OPEN (31,FILE='MOLFDIR.INP',FORM='FORMATTED',STATUS='REPLACE')
       WRITE (LINE,'(16(I4,A1))') (NELRP(I),',',I=1,NREPI)
-      WRITE (31,'(A,A,A)') ' &RELCCSD NELEC=',TRIM(LINE),
+      WRITE (FLINE,'(16(I4,A1))') (NFRZ(I),',',I=1,NREPI)
+      WRITE (31,'(A,A,A,A,A)') ' &RELCCSD NELEC=',TRIM(LINE),
+     &   ' NFROZ=',TRIM(FLINE),
      &   ' CARITH=T, DOSORT=F, DOENER=T, IPRNT=1, TIMING=T /'
       WRITE (31,'(A,I4,A,I4,A,I4,A)') ' &CCENER DOMP2=T, DOCCSD=T,'//
      &   ' DOCCSDT=F, MAXIT=',HDR(4),', MAXDIM=',HDR(5),
@@ -442,7 +453,8 @@ C
          CALL ZCORE (S1,S2)
          WRITE (32) S1,S2
          CALL CCENG (S1,S2,FVO,BUF1,BUF2,ECC)
-         WRITE (32) ECC
+         CALL T1DIAG (DIAG,S1,BUF1)
+         WRITE (32) ECC,DIAG
       ENDIF
       CLOSE (32)
       CLOSE (30)

@@ -9,7 +9,10 @@ with the given MAXIT / MAXDIM / NTOL and returns the final amplitudes and
 the per-iteration table CCEQN prints (cceqns.F:270-272, 550-551).
 Mode 2 ("terms") evaluates every CCEQN term once on given T1/T2 and returns
 each intermediate (HO, HV, HOV, S1, GO, GV, AZ, H, S2 before and after BINTM,
-S1/S2 after DENOM, and CCENG's energy).
+S1/S2 after DENOM and ZCORE, CCENG's energy and T1DIAG of the new T1).
+A system dict may carry "nfroz": frozen occupied spinors per irrep (namelist
+RELCCSD NFROZ, SYMTAB ccdriv.F:838-849), the lowest of each irrep in file
+order.
 Mode 4 (run_sort) sorts a MOLFDIR MRCONEE + MDCINT pair with relccsd's own
 CCSETI -> RDINTS (ccints.F:33-569): the six classes, the Fock blocks after
 GETFM / MAKEFM, the orbital energies and ESCF.
@@ -58,8 +61,9 @@ def _write_input(path, sysd, mode, maxit, maxdim, ntol, t1=None, t2=None):
     lay = layout_of(sysd)
     no, nv = sysd["no"], sysd["nv"]
     nrep = lay.nrep
-    hdr = np.array([no, nv, mode, maxit, maxdim, ntol, nrep if nrep > 1 else 0, 0],
-                   dtype=np.int32)
+    nfroz = sysd.get("nfroz")
+    hdr = np.array([no, nv, mode, maxit, maxdim, ntol, nrep if nrep > 1 else 0,
+                    2 if nfroz is not None and any(nfroz) else 0], dtype=np.int32)
     eps = np.asarray(sysd["eps"].cpu().numpy() if hasattr(sysd["eps"], "cpu") else sysd["eps"],
                      dtype=np.float64)
     with open(path, "wb") as f:
@@ -70,6 +74,8 @@ def _write_input(path, sysd, mode, maxit, maxdim, ntol, t1=None, t2=None):
             f.write(irr.astype(np.int32).tobytes())
             f.write(lay.no.astype(np.int32).tobytes())
             f.write(multb_full(nrep).T.astype(np.int32).tobytes())   # column-major
+        if hdr[7] & 2:
+            f.write(np.asarray(nfroz, dtype=np.int32).reshape(nrep).tobytes())
         for name in CLASS_ORDER:
             f.write(_c(lay.to_sym(name, _c(sysd[name]))).tobytes())
         if mode in (2, 3):
@@ -134,13 +140,13 @@ def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
     sizes = [("HO", nfoo), ("HV", nfvv), ("HOV", nfvo), ("S1", nt1), ("GO", nfoo),
              ("GV", nfvv), ("AZ", nv1), ("H", nv4), ("S2_T2EQNS", nt2),
              ("S2_BINTM", nt2), ("S1_DENOM", nt1), ("S2_DENOM", nt2)]
-    z = raw[:-1].view(np.complex128)
+    z = raw[:-2].view(np.complex128)
     pos = 0
     for name, n in sizes:
         out[name] = z[pos:pos + n].copy()
         pos += n
     assert pos == z.size, (pos, z.size)
-    out["ECC"] = raw[-1]
+    out["ECC"], out["T1DIAG"] = raw[-2], raw[-1]
     return out
 
 

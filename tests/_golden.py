@@ -26,8 +26,32 @@ def system_for(g, device="cpu"):
     sym = None
     if "sym_no" in g:
         sym = (g["sym_no"].tolist(), g["sym_nv"].tolist())
-    return synth.make_system(no, nv, seed=seed, scale=float(g["scale"]),
-                             fock_perturb=float(g["fock_perturb"]), device=device, sym=sym)
+    s = synth.make_system(no, nv, seed=seed, scale=float(g["scale"]),
+                          fock_perturb=float(g["fock_perturb"]), device=device, sym=sym)
+    if "nfroz" in g:
+        s["nfroz"] = g["nfroz"].tolist()
+    return s
+
+
+def frozen_mask_for(g):
+    """Frozen mask over the occupied spinors (C1 working order, irrep-major)
+    of a fixture with NFROZ, else None."""
+    if "nfroz" not in g:
+        return None
+    no_l = g["sym_no"].tolist() if "sym_no" in g else [int(g["params"][0])]
+    mask = []
+    for n, f in zip(no_l, g["nfroz"].tolist()):
+        mask += [1] * f + [0] * (n - f)
+    return mask
+
+
+def fixture_amplitudes(g):
+    """The random T1/T2 (C1 layout) a terms / symterms / frzterms fixture was
+    evaluated on."""
+    no, nv, tseed = int(g["params"][0]), int(g["params"][1]), int(g["params"][3])
+    if "sym_no" in g:
+        return symmetric_amplitudes(layout_for(g), tseed)
+    return random_amplitudes(no, nv, tseed)
 
 
 def layout_for(g):

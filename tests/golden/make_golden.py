@@ -46,6 +46,12 @@ change in the generator is caught.
 * cfg2ref_<name>.npz: relccsd's CCSD at C1 20o/150v (sigma 0.005) converged to
   NTOL 8: MP2 and CCSD energies, the iteration table, T1, the largest T2
   elements with their indices, the T2 norm and seeded +-1 projections of T2.
+* frzccsd_<name>.npz / frzterms_<name>.npz: ccsd_ / terms_ runs with frozen
+  occupied spinors (namelist RELCCSD NFROZ: the lowest NFROZ(irp) occupied
+  spinors of each irrep, IFROZ of SYMTAB ccdriv.F:838-849, whose amplitudes
+  ZCORE cceqns.F:1864 zeroes), C1 and symmetry-blocked; nfroz is stored.
+  terms_ / symterms_ / frzterms_ fixtures also hold T1DIAG (cceqns.F:2453)
+  of the new T1 at full precision.
 * mp2g_<name>.npz: relccsd's MP2 relaxed property chain of CCFOPR (harness
   mode 7): DOO / DVV of CCDENSC(1), the X intermediate and the RPAA Z-vector
   (DVO, DOV, preconditioner DG) of CCDENSZ(1), and EXPVAL of the operator.
@@ -87,6 +93,86 @@ SYM_TERM_CASES = [
     ("s4_t_5o12v", (2, 1, 1, 1), (4, 3, 3, 2), 24, 0.1, 0.03, 14),
     ("s8_t_8o24v", (2, 0, 1, 1, 2, 0, 1, 1), (4, 2, 3, 3, 4, 2, 3, 3), 25, 0.1, 0.03, 15),
 ]
+
+
+FROZEN_CCSD_CASES = [
+    # name, (no per irrep), (nv per irrep), nfroz per irrep, seed, scale, fock_perturb,
+    # maxit, maxdim, ntol
+    ("c1_5o16v", (5,), (16,), (2,), 2, 0.02, 0.0, 40, 8, 11),
+    ("c1_6o40v", (6,), (40,), (1,), 1, 0.05, 0.0, 20, 6, 12),
+    ("c1_4o12v_fock", (4,), (12,), (1,), 5, 0.03, 0.03, 30, 6, 10),
+    ("s4_7o16v", (3, 2, 1, 1), (5, 4, 4, 3), (1, 1, 0, 1), 26, 0.1, 0.02, 30, 6, 11),
+    ("s8_8o24v", (2, 0, 1, 1, 2, 0, 1, 1), (4, 2, 3, 3, 4, 2, 3, 3), (1, 0, 0, 1, 1, 0, 0, 0),
+     23, 0.1, 0.0, 30, 6, 11),
+]
+FROZEN_TERM_CASES = [
+    # name, (no per irrep), (nv per irrep), nfroz, seed, scale, fock_perturb, amplitude seed
+    ("c1_5o14v", (5,), (14,), (2,), 4, 0.05, 0.05, 12),
+    ("s4_7o16v", (3, 2, 1, 1), (5, 4, 4, 3), (1, 1, 0, 1), 27, 0.1, 0.03, 16),
+]
+TERM_KEYS = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
+             "S2_BINTM", "S1_DENOM", "S2_DENOM"]
+
+
+def frozen_ccsd_case(name, sno, snv, nfroz, seed, scale, fp, maxit, maxdim, ntol):
+    no, nv = sum(sno), sum(snv)
+    sym = (sno, snv) if len(sno) > 1 else None
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=sym)
+    s["nfroz"] = list(nfroz)
+    r = refrun.run(s, "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol)
+    it = np.asarray(r["iterations"], dtype=np.float64)
+    extra = {"sym_no": np.array(sno), "sym_nv": np.array(snv)} if sym else {}
+    np.savez_compressed(
+        os.path.join(HERE, f"frzccsd_{name}.npz"),
+        params=np.array([no, nv, seed, maxit, maxdim, ntol], dtype=np.int64),
+        nfroz=np.array(nfroz), scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+        emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
+        T1=r["T1"], T2=r["T2"], **extra)
+    print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
+
+
+def frozen_terms_case(name, sno, snv, nfroz, seed, scale, fp, tseed):
+    no, nv = sum(sno), sum(snv)
+    sym = (sno, snv) if len(sno) > 1 else None
+    s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=sym)
+    s["nfroz"] = list(nfroz)
+    if sym:
+        t1, t2 = symmetric_amplitudes(SymLayout(sno, snv), tseed)
+    else:
+        t1, t2 = random_amplitudes(no, nv, tseed)
+    r = refrun.run(s, "terms", t1=t1, t2=t2)
+    extra = {"sym_no": np.array(sno), "sym_nv": np.array(snv)} if sym else {}
+    np.savez_compressed(
+        os.path.join(HERE, f"frzterms_{name}.npz"),
+        params=np.array([no, nv, seed, tseed], dtype=np.int64), nfroz=np.array(nfroz),
+        scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+        ECC=r["ECC"], T1DIAG=r["T1DIAG"], **{k: r[k] for k in TERM_KEYS}, **extra)
+    print(name, r["ECC"], r["T1DIAG"])
+
+
+def terms_cases():
+    for name, no, nv, seed, scale, fp, tseed in TERM_CASES:
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
+        t1, t2 = random_amplitudes(no, nv, tseed)
+        r = refrun.run(s, "terms", t1=t1, t2=t2)
+        np.savez_compressed(
+            os.path.join(HERE, f"terms_{name}.npz"),
+            params=np.array([no, nv, seed, tseed], dtype=np.int64),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            ECC=r["ECC"], T1DIAG=r["T1DIAG"], **{k: r[k] for k in TERM_KEYS})
+        print(name, r["ECC"], r["T1DIAG"])
+    for name, sno, snv, seed, scale, fp, tseed in SYM_TERM_CASES:
+        no, nv = sum(sno), sum(snv)
+        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=(sno, snv))
+        t1, t2 = symmetric_amplitudes(SymLayout(sno, snv), tseed)
+        r = refrun.run(s, "terms", t1=t1, t2=t2)
+        np.savez_compressed(
+            os.path.join(HERE, f"symterms_{name}.npz"),
+            params=np.array([no, nv, seed, tseed], dtype=np.int64),
+            sym_no=np.array(sno), sym_nv=np.array(snv),
+            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
+            ECC=r["ECC"], T1DIAG=r["T1DIAG"], **{k: r[k] for k in TERM_KEYS})
+        print(name, r["ECC"], r["T1DIAG"])
 
 
 LAMBDA_CASES = [
@@ -426,6 +512,15 @@ def main():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
+    if "--terms-only" in sys.argv:
+        terms_cases()
+        return
+    if "--frozen-only" in sys.argv:
+        for case in FROZEN_CCSD_CASES:
+            frozen_ccsd_case(*case)
+        for case in FROZEN_TERM_CASES:
+            frozen_terms_case(*case)
+        return
     if "--cfg2-only" in sys.argv:
         cfg2_case(*CFG2_CASE, full_t2_path=os.environ.get("CFG2_FULL_T2"))
         return
@@ -487,18 +582,6 @@ def main():
             emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
             T1=r["T1"], T2=r["T2"])
         print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
-    for name, no, nv, seed, scale, fp, tseed in ([] if only_sym else TERM_CASES):
-        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp)
-        t1, t2 = random_amplitudes(no, nv, tseed)
-        r = refrun.run(s, "terms", t1=t1, t2=t2)
-        keys = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
-                "S2_BINTM", "S1_DENOM", "S2_DENOM"]
-        np.savez_compressed(
-            os.path.join(HERE, f"terms_{name}.npz"),
-            params=np.array([no, nv, seed, tseed], dtype=np.int64),
-            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
-            ECC=r["ECC"], **{k: r[k] for k in keys})
-        print(name, r["ECC"])
     for name, sno, snv, seed, scale, fp, maxit, maxdim, ntol in SYM_CCSD_CASES:
         no, nv = sum(sno), sum(snv)
         s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=(sno, snv))
@@ -512,20 +595,11 @@ def main():
             emp2=r["emp2"], eccsd=r["eccsd"], nit=r["nit"], iterations=it,
             T1=r["T1"], T2=r["T2"])
         print(name, r["emp2"], r["eccsd"], r["nit"], len(it))
-    for name, sno, snv, seed, scale, fp, tseed in SYM_TERM_CASES:
-        no, nv = sum(sno), sum(snv)
-        s = synth.make_system(no, nv, seed=seed, scale=scale, fock_perturb=fp, sym=(sno, snv))
-        t1, t2 = symmetric_amplitudes(SymLayout(sno, snv), tseed)
-        r = refrun.run(s, "terms", t1=t1, t2=t2)
-        keys = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
-                "S2_BINTM", "S1_DENOM", "S2_DENOM"]
-        np.savez_compressed(
-            os.path.join(HERE, f"symterms_{name}.npz"),
-            params=np.array([no, nv, seed, tseed], dtype=np.int64),
-            sym_no=np.array(sno), sym_nv=np.array(snv),
-            scale=scale, fock_perturb=fp, fingerprint=fingerprint(s),
-            ECC=r["ECC"], **{k: r[k] for k in keys})
-        print(name, r["ECC"])
+    terms_cases()
+    for case in FROZEN_CCSD_CASES:
+        frozen_ccsd_case(*case)
+    for case in FROZEN_TERM_CASES:
+        frozen_terms_case(*case)
 
 
 if __name__ == "__main__":

@@ -56,6 +56,7 @@ def test_terms_match_reference(name):
     np.testing.assert_allclose(r.get("S1"), g["S1_DENOM"], atol=1e-12, rtol=0)
     np.testing.assert_allclose(r.get("S2"), g["S2_DENOM"], atol=1e-12, rtol=0)
     assert abs(r.energy() - float(g["ECC"])) < 1e-12
+    assert abs(r.t1diag() - float(g["T1DIAG"])) < 1e-13 * max(1.0, float(g["T1DIAG"]))
 
 
 @pytest.mark.parametrize("three_mult", [True, False])
@@ -70,9 +71,11 @@ def test_trajectory_matches_reference(name, three_mult):
     ref_rows = g["iterations"][1:]
     assert abs(out["emp2"] - float(g["emp2"])) < 1e-10
     assert out["nit"] == int(g["nit"]) == len(ref_rows)
-    for (nit, e, rms), ref in zip(out["rows"], ref_rows):
+    for (nit, e, rms), diag, ref in zip(out["rows"], out["t1diag"], ref_rows):
         assert abs(e - ref[1]) < 1e-10, (nit, e, ref[1])
         assert abs(rms - ref[2]) < 1e-9 * max(1.0, ref[2]), (nit, rms, ref[2])
+        # T1-DIAGN column of the reference table (F10.5, cceqns.F:619)
+        assert abs(diag - ref[3]) <= 5e-6 + 1e-12, (nit, diag, ref[3])
     t1, t2 = r.get_amplitudes()
     assert np.abs(t1 - g["T1"]).max() < 1e-9
     assert np.abs(t2 - g["T2"]).max() < 1e-9
@@ -89,6 +92,8 @@ def test_iterations_match_oracle_beyond_fixture_sizes(no, nv, seed):
     assert abs(out["emp2"] - ref["emp2"]) < 1e-10
     for a, b in zip(out["rows"], ref["rows"]):
         assert abs(a[1] - b[1]) < 1e-10 and abs(a[2] - b[2]) < 1e-9 * max(1, b[2])
+    for a, b in zip(out["t1diag"], ref["t1diag"]):
+        assert abs(a - b) < 1e-12 * max(1.0, b)
     t1, t2 = r.get_amplitudes()
     assert np.abs(t1 - ref["T1"]).max() < 1e-9
     assert t2.size == ref["T2"].size

@@ -70,6 +70,7 @@ def test_symmetric_terms_match_reference(name):
     np.testing.assert_allclose(r.get("S1"), g["S1_DENOM"], atol=1e-12, rtol=0)
     np.testing.assert_allclose(r.get("S2"), g["S2_DENOM"], atol=1e-12, rtol=0)
     assert abs(r.energy() - float(g["ECC"])) < 1e-12
+    assert abs(r.t1diag() - float(g["T1DIAG"])) < 1e-13 * max(1.0, float(g["T1DIAG"]))
 
 
 @pytest.mark.parametrize("device_ints", [False, True])
@@ -82,9 +83,10 @@ def test_symmetric_trajectory_matches_reference(name, device_ints):
     ref_rows = g["iterations"][1:]
     assert abs(out["emp2"] - float(g["emp2"])) < 1e-10
     assert out["nit"] == int(g["nit"]) == len(ref_rows)
-    for (nit, e, rms), ref in zip(out["rows"], ref_rows):
+    for (nit, e, rms), diag, ref in zip(out["rows"], out["t1diag"], ref_rows):
         assert abs(e - ref[1]) < 1e-10, (nit, e, ref[1])
         assert abs(rms - ref[2]) < 1e-9 * max(1.0, ref[2]), (nit, rms, ref[2])
+        assert abs(diag - ref[3]) <= 5e-6 + 1e-12, (nit, diag, ref[3])
     t1, t2 = r.get_amplitudes()
     assert t1.shape == g["T1"].shape and t2.shape == g["T2"].shape
     assert np.abs(t1 - g["T1"]).max() < 1e-9

@@ -132,3 +132,24 @@ def test_oracle_on_larger_random_cases(no, nv):
     r.set_amplitudes(t1, t2)
     e = r.t3corr()
     check(e, to.t3corr(s, t1, t2))
+
+
+def test_failed_work_allocation_is_reported_again_not_launched(monkeypatch):
+    """Regression: when a work buffer of the triples cannot be allocated the
+    call fails, and a second call with the same sizes fails the same way
+    (the partially built work state is released, never launched on).  A
+    budget of 1 PB asks for 16384 triples per launch, whose U' blocks
+    (16384 x 140 x 9730 x 16 B = 357 GB at 48o/140v) exceed the device."""
+    from molfdir_amd.relccsd import RccError
+    no, nv = 48, 140
+    need = 16384 * nv * (nv * (nv - 1) // 2) * 16
+    if torch.cuda.get_device_properties(0).total_memory >= need:
+        pytest.skip("the device holds the oversized work buffer")
+    r = RelCCSD(no, nv)
+    monkeypatch.setenv("RCC_T3_BUDGET_MB", str(10 ** 9))
+    for _ in range(2):
+        with pytest.raises(RccError, match="hipMalloc"):
+            r.t3corr()
+    monkeypatch.delenv("RCC_T3_BUDGET_MB")
+    g = load("t3", "r_4o10v")
+    check(handle(g)[0].t3corr(), ref_of(g))

@@ -130,6 +130,28 @@ def test_inconsistent_inputs_are_rejected(tmp_path):
         r2.read_molfdir(str(tmp_path / "missing"), [nocc])
 
 
+@pytest.mark.parametrize("record", [2, 3])
+def test_truncated_mrconee_records_are_rejected(record, tmp_path):
+    """An MRCONEE whose record 2 (NSYMRP) or 3 (NREP) is shorter than the
+    4-byte count RDSYMI reads first (ccdriv.F:2978-2988) is refused with an
+    error, not read past its end."""
+    import struct
+    g = load("sort", "k6_4o")
+    gen, nkr, nocc = regenerate(g, tmp_path)
+    raw = (tmp_path / "MRCONEE").read_bytes()
+    recs, p = [], 0
+    while p < len(raw):
+        n = struct.unpack_from("<i", raw, p)[0]
+        recs.append(raw[p + 4:p + 4 + n])
+        p += n + 8
+    recs[record - 1] = b"\x07\x00"
+    (tmp_path / "MRCONEE").write_bytes(b"".join(
+        struct.pack("<i", len(x)) + x + struct.pack("<i", len(x)) for x in recs))
+    r = RelCCSD(nocc, 2 * nkr - nocc)
+    with pytest.raises(RccError, match=f"record {record}"):
+        r.read_molfdir(str(tmp_path), [nocc])
+
+
 @pytest.mark.parametrize("kind,name,opts", [("sort", "k7_6o", 0),
                                              ("sortopt", "k7_6o_norecmp", 2),
                                              ("sortopt", "k7_6o_useoe", 1)])

@@ -16,13 +16,14 @@ from oracle import diis as D
 from oracle import textbook as TB
 
 from _golden import (load, names, system_for, fingerprint, random_amplitudes, layout_for,
-                     symmetric_amplitudes, TERM_LAYOUT)
+                     symmetric_amplitudes, TERM_LAYOUT, fixture_amplitudes)
 
 TERM_KEYS = ["HO", "HV", "HOV", "S1", "GO", "GV", "AZ", "H", "S2_T2EQNS",
              "S2_BINTM", "S1_DENOM", "S2_DENOM"]
 
 
-@pytest.mark.parametrize("kind,name", [(k, n) for k in ("ccsd", "terms", "symccsd", "symterms")
+@pytest.mark.parametrize("kind,name", [(k, n) for k in ("ccsd", "terms", "symccsd", "symterms",
+                                                         "frzccsd", "frzterms")
                                        for n in names(k)])
 def test_generator_matches_fixture_inputs(kind, name):
     g = load(kind, name)
@@ -41,6 +42,8 @@ def test_terms_match_reference(name):
         ref = g[k]
         assert np.abs(o[k] - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), k
     assert abs(o["ECC"] - float(g["ECC"])) < 1e-13
+    # T1DIAG (cceqns.F:2453) of the new T1, written by the reference at full precision
+    assert abs(o["T1DIAG"] - float(g["T1DIAG"])) < 1e-14 * max(1.0, float(g["T1DIAG"]))
 
 
 @pytest.mark.parametrize("name", names("ccsd"))
@@ -55,11 +58,13 @@ def test_trajectory_matches_reference(name):
     ref_rows = it[1:]
     assert len(r["rows"]) == len(ref_rows) == int(g["nit"])
     assert abs(r["emp2"] - float(g["emp2"])) < 1e-12 * max(1, abs(float(g["emp2"])))
-    for (nit, ec, rms), ref in zip(r["rows"], ref_rows):
+    for (nit, ec, rms), diag, ref in zip(r["rows"], r["t1diag"], ref_rows):
         assert nit == int(ref[0])
         # the printed table has 15 decimals
         assert abs(ec - ref[1]) < 1e-10, (nit, ec, ref[1])
         assert abs(rms - ref[2]) < 1e-10 * max(1.0, ref[2]), (nit, rms, ref[2])
+        # T1-DIAGN column, printed F10.5 (cceqns.F:619): equal after rounding
+        assert abs(diag - ref[3]) <= 5e-6 + 1e-12, (nit, diag, ref[3])
     assert np.abs(r["T1"] - g["T1"]).max() < 1e-9
     assert np.abs(r["T2"] - g["T2"]).max() < 1e-9
 
@@ -82,6 +87,7 @@ def test_terms_match_symmetric_reference(name):
         # and nothing outside the symmetry blocks
         assert np.array_equal(lay.to_c1(TERM_LAYOUT[k], got), o[k]), k
     assert abs(o["ECC"] - float(g["ECC"])) < 1e-13
+    assert abs(o["T1DIAG"] - float(g["T1DIAG"])) < 1e-14 * max(1.0, float(g["T1DIAG"]))
 
 
 @pytest.mark.parametrize("name", names("symccsd"))
@@ -94,11 +100,105 @@ def test_trajectory_matches_symmetric_reference(name):
     ref_rows = g["iterations"][1:]
     assert len(r["rows"]) == len(ref_rows) == int(g["nit"])
     assert abs(r["emp2"] - float(g["emp2"])) < 1e-12
-    for (nit, ec, rms), ref in zip(r["rows"], ref_rows):
+    for (nit, ec, rms), diag, ref in zip(r["rows"], r["t1diag"], ref_rows):
         assert abs(ec - ref[1]) < 1e-10, (nit, ec, ref[1])
         assert abs(rms - ref[2]) < 1e-10 * max(1.0, ref[2]), (nit, rms, ref[2])
+        assert abs(diag - ref[3]) <= 5e-6 + 1e-12, (nit, diag, ref[3])
     assert np.abs(lay.to_sym("T1", r["T1"]) - g["T1"]).max() < 1e-9
     assert np.abs(lay.to_sym("T2", r["T2"]) - g["T2"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("name", names("frzterms"))
+def test_terms_with_frozen_core_match_reference(name):
+    """NFROZ != 0: every intermediate is unchanged by frozen spinors; after
+    DENOM, ZCORE (cceqns.F:1864) zeroes T1(A,I) of frozen I and T2(AB,IJ) with
+    I or J frozen.  C1 and symmetry-blocked (frozen spinors in several irreps)."""
+    g = load("frzterms", name)
+    s = system_for(g)
+    t1, t2 = fixture_amplitudes(g)
+    o = O.terms(s, t1, t2)
+    lay = layout_for(g) if "sym_no" in g else None
+    for k in TERM_KEYS:
+        got = lay.to_sym(TERM_LAYOUT[k], o[k]) if lay else o[k]
+        ref = g[k]
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), k
+    assert abs(o["ECC"] - float(g["ECC"])) < 1e-13
+    assert abs(o["T1DIAG"] - float(g["T1DIAG"])) < 1e-14 * max(1.0, float(g["T1DIAG"]))
+    # the frozen rows really are zero in the reference, and only those
+    mask = np.asarray(O.frozen_mask(s))
+    S1 = (lay.to_c1("T1", g["S1_DENOM"]) if lay else g["S1_DENOM"]).reshape(len(mask), -1)
+    assert np.all(S1[mask] == 0) and np.all(np.abs(S1[~mask]).max(axis=1) > 0)
+
+
+@pytest.mark.parametrize("name", names("frzccsd"))
+def test_frozen_core_trajectory_matches_reference(name):
+    """MP2EQN and CCEQN with NFROZ != 0 (ZCORE on the MP2 guess, cceqns.F:94,
+    and on every update, :483): energies, RMS, T1-diagnostic and the final
+    amplitudes against relccsd's CCENER."""
+    g = load("frzccsd", name)
+    maxit, maxdim, ntol = (int(x) for x in g["params"][3:6])
+    s = system_for(g)
+    lay = layout_for(g) if "sym_no" in g else None
+    r = O.cceqn(s, maxit=maxit, maxdim=maxdim, ntol=ntol)
+    ref_rows = g["iterations"][1:]
+    assert len(r["rows"]) == len(ref_rows) == int(g["nit"])
+    assert abs(r["emp2"] - float(g["emp2"])) < 1e-12 * max(1, abs(float(g["emp2"])))
+    for (nit, ec, rms), diag, ref in zip(r["rows"], r["t1diag"], ref_rows):
+        assert nit == int(ref[0])
+        assert abs(ec - ref[1]) < 1e-10, (nit, ec, ref[1])
+        assert abs(rms - ref[2]) < 1e-10 * max(1.0, ref[2]), (nit, rms, ref[2])
+        assert abs(diag - ref[3]) <= 5e-6 + 1e-12, (nit, diag, ref[3])
+    t1 = lay.to_sym("T1", r["T1"]) if lay else r["T1"]
+    t2 = lay.to_sym("T2", r["T2"]) if lay else r["T2"]
+    assert np.abs(t1 - g["T1"]).max() < 1e-9
+    assert np.abs(t2 - g["T2"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("mistake", ["last_spinor", "t2_first_index_only", "no_mp2_zcore"])
+def test_planted_frozen_core_mistakes_are_caught(monkeypatch, mistake):
+    """Plausible ZCORE mistakes in the oracle fail the frozen fixtures:
+    freezing the highest instead of the lowest occupied spinor, zeroing T2
+    only where I (not J) is frozen, and skipping MP2EQN's ZCORE."""
+    g = load("frzccsd", "c1_5o16v")
+    maxit, maxdim, ntol = (int(x) for x in g["params"][3:6])
+    s = system_for(g)
+    good_mask, good_zcore = O.frozen_mask, O.zcore
+    if mistake == "last_spinor":
+        monkeypatch.setattr(O, "frozen_mask", lambda sd: good_mask(sd)[::-1].copy())
+    elif mistake == "t2_first_index_only":
+        def bad(t1, t2, mask):
+            if mask is None:
+                return t1, t2
+            t1, t2 = t1.copy(), t2.copy()
+            t1[:, mask] = 0
+            t2[:, :, mask, :] = 0
+            return t1, t2
+        monkeypatch.setattr(O, "zcore", bad)
+    else:
+        calls = {"n": 0}
+
+        def skip_first(t1, t2, mask):
+            calls["n"] += 1
+            return (t1, t2) if calls["n"] == 1 else good_zcore(t1, t2, mask)
+        monkeypatch.setattr(O, "zcore", skip_first)
+    r = O.cceqn(s, maxit=maxit, maxdim=maxdim, ntol=ntol)
+    bad_e = abs(r["emp2"] - float(g["emp2"])) > 1e-8 or \
+        abs(r["rows"][-1][1] - g["iterations"][-1][1]) > 1e-8 or \
+        np.abs(r["T2"] - g["T2"]).max() > 1e-8
+    assert bad_e
+
+
+def test_planted_t1diag_mistake_is_caught(monkeypatch):
+    """T1DIAG divided by the unfrozen occupied count (a plausible misreading
+    of NE) fails the frozen terms fixture."""
+    g = load("frzterms", "c1_5o14v")
+    s = system_for(g)
+    t1, t2 = fixture_amplitudes(g)
+    nact = int(g["params"][0]) - int(g["nfroz"].sum())
+    monkeypatch.setattr(O, "t1diag",
+                        lambda S, t: float(np.sqrt(np.real(np.vdot(t, t)) / nact)))
+    o = O.terms(s, t1, t2)
+    assert abs(o["T1DIAG"] - float(g["T1DIAG"])) > 1e-6
 
 
 def test_symmetry_layout_sizes_and_round_trip():
