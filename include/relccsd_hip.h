@@ -373,6 +373,19 @@ int rcc_kernel_times(rcc_t* h, double* ms4);
 int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
                      const void* A, int64_t lda, const void* B, int64_t ldb,
                      const double* beta, void* C, int64_t ldc);
+/* Allocation-free form of rcc_zgemm_device.  No counterpart in relccsd: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same product and conventions; it calls no
+ * hipMalloc / hipFree and does not synchronise: the product is enqueued on
+ * the handle's stream (rcc_synchronize waits for it).  The K-split partial
+ * sums go to a fixed 32 MiB scratch array per device, shared by every call
+ * on that device: calls that overlap on one GPU race.  ksplit = number of k
+ * chunks (0: the library's heuristic split that fits the scratch); a split
+ * whose partial sums (ksplit x m x n x 16 bytes) exceed 32 MiB is refused
+ * with an error.  No autotuning. */
+int rcc_zgemm_device_noalloc(rcc_t* h, int opa, int opb, int m, int n, int k,
+                             const double* alpha, const void* A, int64_t lda, const void* B,
+                             int64_t ldb, const double* beta, void* C, int64_t ldc, int ksplit);
 /* The split-BF16 GEMM on device pointers: C = alpha A B + beta C with A (m x k)
  * and B (k x n) column-major complex (diagnostic; allocates its own planes). */
 int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, const void* A,

@@ -28,6 +28,15 @@ struct GemmArgs {
 
 // ws may be null (no K-split); ws_elems = capacity in complex elements.
 int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
+// The allocation-free form of zgemm: K-split partial sums in a fixed
+// file-scope device array of zgemm_scratch_elems() complex elements per
+// device.  zgemm_scratch_init(device) (device current) takes the array's
+// address once per device; zgemm_noalloc then splits k into ksplit chunks
+// (0: the heuristic split that fits the array) and fails with err set when
+// the partial sums would not fit.  No allocation, no autotuning.
+int zgemm_scratch_init(int device);
+int64_t zgemm_scratch_elems();
+int zgemm_noalloc(const GemmArgs& g, int device, int ksplit, hipStream_t st, std::string& err);
 // K-split autotuner of zgemm (process-wide): on / off, and the number of
 // shapes tuned so far.
 void zgemm_autotune(int enable);

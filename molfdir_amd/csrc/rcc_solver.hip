@@ -73,6 +73,7 @@ struct SOp {
 };
 
 struct rcc_handle {
+    int device = 0;        // the device current at rcc_create (its stream's device)
     int no = 0, nv = 0;
     int64_t noot = 0, nvvt = 0, nvo = 0, nt1 = 0, nt2 = 0;
     int maxdim = 8, maxit = 30, ntol = 12, precision = 0;
@@ -2374,6 +2375,12 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
     h->nvo = (int64_t)h->no * h->nv;
     h->nt1 = h->nvo;
     h->nt2 = h->nvvt * h->noot;
+    HIPCHK(h, hipGetDevice(&h->device));
+    // address of this device's allocation-free scratch (rcc_zgemm_device_noalloc)
+    if (zgemm_scratch_init(h->device) != 0) {
+        h->err = "rcc_create: no address for the device's product scratch";
+        return -1;
+    }
     HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
@@ -3658,6 +3665,36 @@ int rcc_zgemm_device(rcc_t* h, int opa, int opb, int m, int n, int k, const doub
     CK(gemm(h, opa, opb, m, n, k, cplx(alpha[0], alpha[1]), (const double2*)A, lda,
             (const double2*)B, ldb, cplx(beta[0], beta[1]), (double2*)C, ldc));
     HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int rcc_zgemm_device_noalloc(rcc_t* h, int opa, int opb, int m, int n, int k, const double* alpha,
+                             const void* A, int64_t lda, const void* B, int64_t ldb,
+                             const double* beta, void* C, int64_t ldc, int ksplit) {
+    if (!h) return -1;
+    const bool conj_pair = (opa == OP_R && opb == OP_N) || (opa == OP_N && opb == OP_R);
+    if (opa < 0 || opb < 0 || ((opa > 2 || opb > 2) && !conj_pair)) {
+        h->err = "rcc_zgemm_device_noalloc: op must be 0, 1 or 2 (3 only with 0 on the other "
+                 "operand)";
+        return -1;
+    }
+    if (ksplit < 0) {
+        h->err = "rcc_zgemm_device_noalloc: ksplit must be >= 0";
+        return -1;
+    }
+    GemmArgs g;
+    g.gauss = h->gemm_gauss;
+    g.opA = opa; g.opB = opb; g.m = m; g.n = n; g.k = k;
+    g.A = (const double2*)A; g.lda = lda;
+    g.B = (const double2*)B; g.ldb = ldb;
+    g.C = (double2*)C; g.ldc = ldc;
+    g.alpha = cplx(alpha[0], alpha[1]);
+    g.beta = cplx(beta[0], beta[1]);
+    std::string err;
+    if (zgemm_noalloc(g, h->device, ksplit, h->st, err) != 0) {
+        h->err = "rcc_zgemm_device_noalloc: " + err;
+        return -1;
+    }
     return 0;
 }
 

@@ -713,6 +713,68 @@ int zgemm(const GemmArgs& g0, double2* ws, int64_t ws_elems, hipStream_t st) {
     return launch_split(g, nsplit, kchunk, ws, st);
 }
 
+// ---- allocation-free products (rcc_zgemm_device_noalloc)
+// The K-split partial sums of these products live in this file-scope array
+// instead of a workspace the library allocates: 32 MiB (2 Mi complex
+// elements) on every GPU the library runs on, held until the process ends.
+// Host code never uses the name `scratch` as an address (on the host it is
+// only a stand-in); zgemm_scratch_init takes the array's device address once
+// per device with that device current, and every launch uses that address.
+// All products on one GPU share the array, and it keeps what the last call
+// left: calls that overlap on one GPU race.  Nothing needs clearing, since
+// every split writes every element of its partial-sum slice before the
+// reduction reads it.
+namespace {
+constexpr int64_t kScratchElems = (int64_t)(32 << 20) / (int64_t)sizeof(double2);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double2 scratch[kScratchElems];
+
+int64_t zgemm_scratch_elems() { return kScratchElems; }
+
+int zgemm_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    g_scratch_ptr[device] = (double2*)ptr;
+    return 0;
+}
+
+int zgemm_noalloc(const GemmArgs& g, int device, int ksplit, hipStream_t st, std::string& err) {
+    double2* ws = nullptr;
+    if (device >= 0 && device < kMaxDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ws = g_scratch_ptr[device];
+    }
+    if (!ws) {
+        err = "no scratch address for this device (the handle was not created on it)";
+        return -1;
+    }
+    if (g.m <= 0 || g.n <= 0 || g.batch <= 0) return 0;
+    if (g.k <= 0) return zgemm(g, nullptr, 0, st);      // C = beta C, no partial sums
+    const int s = ksplit > 0 ? ksplit : choose_split(g, kScratchElems);
+    int nsplit, kchunk;
+    plan_split(g, s, nsplit, kchunk);
+    const int64_t need = nsplit > 1 ? (int64_t)nsplit * g.m * g.n * g.batch : 0;
+    if (need > kScratchElems) {
+        err = "the K split needs " + std::to_string(need * (int64_t)sizeof(double2)) +
+              " bytes of partial sums; the scratch holds " +
+              std::to_string(kScratchElems * (int64_t)sizeof(double2)) + " bytes";
+        return -1;
+    }
+    const int rc = launch_split(g, nsplit, kchunk, ws, st);
+    if (rc != 0) err = "zgemm launch failed (" + std::to_string(rc) + ")";
+    return rc;
+}
+
 // Layout probe: one v_mfma_f64_16x16x4_f64 with A[i][k] = i + 16 k and
 // B[k][j] = (k == j % 4) ? 1 : 0... the host checks D against the assumed
 // lane mapping.  out[64 * 4]: lane-major D registers.

@@ -152,6 +152,8 @@ def load():
     lib.rcc_last_iteration_ms.argtypes = [vp, dp]
     lib.rcc_kernel_times.argtypes = [vp, dp]
     lib.rcc_zgemm_device.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp, vp, i64]
+    lib.rcc_zgemm_device_noalloc.argtypes = [vp, ci, ci, ci, ci, ci, dp, vp, i64, vp, i64, dp,
+                                             vp, i64, ci]
     lib.rcc_probe_mfma.argtypes = [vp, dp]
     lib.rcc_set_graph.argtypes = [vp, ci]
     lib.rcc_iterate_batch.argtypes = [ctypes.POINTER(vp), ci, ctypes.POINTER(ci), dp, dp]
@@ -638,6 +640,18 @@ class RelCCSD:
         self._check(self.lib.rcc_zgemm_device(
             self.h, opa, opb, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
             ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc))
+
+    def zgemm_device_noalloc(self, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                             ksplit=0):
+        """rcc_zgemm_device without allocation: enqueued on the handle's
+        stream (call synchronize()), K-split partial sums in the library's
+        fixed 32 MiB scratch; ksplit 0 = heuristic split."""
+        al = (ctypes.c_double * 2)(alpha.real, alpha.imag)
+        be = (ctypes.c_double * 2)(beta.real, beta.imag)
+        self._check(self.lib.rcc_zgemm_device_noalloc(
+            self.h, opa, opb, m, n, k, al, ctypes.c_void_p(A.data_ptr()), lda,
+            ctypes.c_void_p(B.data_ptr()), ldb, be, ctypes.c_void_p(C.data_ptr()), ldc,
+            int(ksplit)))
 
     def mfma_rate(self, chains=8, iters=20000, nblocks=2048):
         """Measured FP64 MFMA throughput (TFLOP/s) of a register-only loop."""
