@@ -1145,8 +1145,12 @@ int alloc_all(rcc_t* h) {
     CK(dalloc(h, &h->Y1, nvo * o * o));
     CK(dalloc(h, &h->YAZ, noot * o * o));
     CK(dalloc(h, &h->SMALL, (int64_t)o * o + nvo));
-    h->ws_elems = std::max<int64_t>(1 << 22, std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
-                                                               64 * (int64_t)v * v + (int64_t)256 * o * o));
+    // K-split partial sums: room for a two-way split of the ladder's output
+    // (nvvt x noot), which the tuner may pick to fill the last round of tiles
+    h->ws_elems = std::max<int64_t>({(int64_t)1 << 22,
+                                     std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
+                                                       64 * (int64_t)v * v + (int64_t)256 * o * o),
+                                     2 * nvvt * noot});
     CK(dalloc(h, &h->ws, h->ws_elems));
     CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
     CK(dalloc(h, &h->red_dst, RED_SLOTS));

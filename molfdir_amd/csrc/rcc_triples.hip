@@ -13,18 +13,18 @@
 // The six GEMMs of the reference per triple (cceqns.F:1471-1561) are one
 // GEMM here, over a K dimension made of six segments: the A operand of a
 // triple (v x K) is packed once per triple from T2 and VOOO (signs folded
-// in, every segment zero-padded to a multiple of the 16-deep K-step), and
-// the B operand is read in place:
+// in, every segment zero-padded to a multiple of 16, so that no 8-deep
+// stage of the GEMM core straddles two segments), and the B operand is read
+// in place:
 //   - particle segments: rows (occ * v + e) of the VOVV class itself, which
 //     relccsd stores as ((E,K), C>D) with E fastest (the reference sorts and
 //     conjugates it first, SRT23 + CONJUGA, cceqns.F:1410-1416; here the
 //     conjugate is taken on load);
 //   - hole segments: T(bc; occ, l) from T2 expanded once to all ordered
 //     occupied pairs (nvvt x o^2, the role of SRT22, cceqns.F:1408).
-// Each K-step lies in one segment, so the B loader switches source per
-// step, not per element.  Tiles are ordered M-fastest inside each XCD's
-// range: the v/64 tiles that share a B panel run together and read it
-// through one L2.
+// Each stage lies in one segment, so the B copies switch source per stage,
+// not per element.  Tiles are ordered M-fastest inside each XCD's range: the
+// v/64 tiles that share a B panel run together and read it through one L2.
 //
 // The energy kernel takes 8 x 8 x 8 blocks of (a, b, c) with block indices
 // A >= B >= C; the pair rows of VVOO, T2 and TAU3 and the T1 / FVO values its
@@ -40,7 +40,7 @@
 #include <string>
 #include <vector>
 #include "rcc_internal.h"
-#include "rcc_mfma_tile.h"
+#include "rcc_zgemm_core.h"
 
 namespace rcc {
 
@@ -116,20 +116,24 @@ __global__ void t3_pack_a_kernel(int nv, int no, int nvp, int nop, int64_t nvvt,
     }
 }
 
-// U'_z (v x nvvt, ld v) = A_z (v x Kp) . B_z (Kp x nvvt): one 64 x 64 tile per
-// workgroup, four waves of 32 x 32, K-steps of 16 through two LDS stages
-// (the structure of zgemm_mfma_kernel, with the segmented B operand).
+// U'_z (v x nvvt, ld v) = A_z (v x Kp) . B_z (Kp x nvvt) on the GEMM core of
+// rcc_zgemm_core.h: a 64 x 96 tile per workgroup (2 x 2 waves of 32 x 48),
+// 8-deep stages copied global -> LDS.  A_z is m-fastest in memory.  The B
+// operand changes source with the segment a stage lies in: particle stages
+// read VOVV rows (occ v + e) in place, k-fastest in memory, conjugated after
+// the LDS read; hole stages read T2F rows (occ o + l), n-fastest.  Rows past
+// a segment's end (its zero padding in A_z) are clamped to its last row.
+using TLT = ZTileT3;
+
 template <int GAUSS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 t3_gemm_kernel(int nv, int no, int nvp, int nop, int nvvt, int64_t nvo,
                const int4* __restrict__ tri, const double2* __restrict__ Apk,
                const double2* __restrict__ VOVV, const double2* __restrict__ T2F,
                double2* __restrict__ U) {
-    __shared__ double As_re[2][BK * LDA_S];
-    __shared__ double As_im[2][BK * LDA_S];
-    __shared__ double Bs_re[2][BK * LDB_S];
-    __shared__ double Bs_im[2][BK * LDB_S];
-
+    __shared__ double2 lds0[TLT::SA + TLT::SB];
+    __shared__ double2 lds1[TLT::SA + TLT::SB];
+    constexpr int BM = TLT::BM, BN = TLT::BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -148,150 +152,70 @@ t3_gemm_kernel(int nv, int no, int nvp, int nop, int nvvt, int64_t nvo,
     const int4 t = tri[z];
     const int occ[3] = {t.z, t.x, t.y};   // odd occupied index of the pairs ij, jk, ik
 
-    v4d acc0[2][2], acc1[2][2], acc2[2][2];
+    const double2* Az = Apk + (int64_t)z * m * Kp;
+    int64_t offa[TLT::LA];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            acc0[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc1[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc2[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-        }
-
-    // B element loads of one K-step (4 per thread) and their LDS slots
-    const int pk = tid & 15, pc = tid >> 4;     // particle: k fastest across lanes
-    const int hc = tid & 63, hk = tid >> 6;     // hole: column fastest across lanes
-    auto load_b = [&](int k0, double2 (&rb)[4]) {
-        if (k0 < 3 * nvp) {
-            const int s = k0 / nvp;
-            const int e = min(k0 - s * nvp + pk, nv - 1);
-            const double2* base = VOVV + (int64_t)occ[s] * nv + e;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t gn = min(n0 + pc + 16 * j, n - 1);
-                double2 v = base[gn * nvo];
-                v.y = -v.y;
-                rb[j] = v;
-            }
-        } else {
-            const int c2 = k0 - 3 * nvp;
-            const int s = c2 / nop;
-            const int r0 = c2 - s * nop;
-            const int64_t gn = min(n0 + hc, n - 1);
-            const double2* base = T2F + (int64_t)occ[s] * no * n + gn;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int l = min(r0 + hk + 4 * j, no - 1);
-                rb[j] = base[(int64_t)l * n];
-            }
-        }
-    };
-    auto store_bs = [&](int k0, double* __restrict__ sre, double* __restrict__ sim,
-                        const double2 (&rb)[4]) {
-        if (k0 < 3 * nvp) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                sre[pk * LDB_S + swz<true>(pk, pc + 16 * j)] = rb[j].x;
-                sim[pk * LDB_S + swz<true>(pk, pc + 16 * j)] = rb[j].y;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int kr = hk + 4 * j;
-                sre[kr * LDB_S + swz<true>(kr, hc)] = rb[j].x;
-                sim[kr * LDB_S + swz<true>(kr, hc)] = rb[j].y;
-            }
-        }
-    };
-
-    double2 ra[4], rb[4];
-    const int lr = lane & 15;
-    const int lk = lane >> 4;
-    Loader LA = make_loader_a<OP_N>(Apk + (int64_t)z * m * Kp, m, m, m0, 0, tid);
-    load_step<OP_N>(LA, 0, Kp, true, ra);
-    load_b(0, rb);
-    store_a<OP_N>(As_re[0], As_im[0], tid, ra);
-    store_bs(0, Bs_re[0], Bs_im[0], rb);
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = 0; k0 < Kp; k0 += BK) {
-        const bool more = k0 + BK < Kp;
-        if (more) {
-            advance(LA);
-            load_step<OP_N>(LA, k0 + BK, Kp, true, ra);
-            load_b(k0 + BK, rb);
-        }
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double are[2], aim[2], bre[2], bim[2], bnim[2];
-            const int kr = kk + lk;
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = wm * 32 + i * 16 + lr;
-                are[i] = As_re[cur][kr * LDA_S + r];
-                aim[i] = As_im[cur][kr * LDA_S + r];
-            }
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int c = swz<true>(kr, wn * 32 + j * 16 + lr);
-                bre[j] = Bs_re[cur][kr * LDB_S + c];
-                bim[j] = Bs_im[cur][kr * LDB_S + c];
-                bnim[j] = -bim[j];
-            }
-            if (GAUSS) {
-                double asum[2], bsum[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) asum[i] = are[i] + aim[i];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) bsum[j] = bre[j] + bim[j];
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(asum[i], bsum[j], acc1[i][j], 0, 0, 0);
-                        acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], acc2[i][j], 0, 0, 0);
-                    }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc1[i][j], 0, 0, 0);
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bnim[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc1[i][j], 0, 0, 0);
-                    }
-            }
-        }
-        if (more) {
-            store_a<OP_N>(As_re[cur ^ 1], As_im[cur ^ 1], tid, ra);
-            store_bs(k0 + BK, Bs_re[cur ^ 1], Bs_im[cur ^ 1], rb);
-        }
-        __syncthreads();
-        cur ^= 1;
+    for (int j = 0; j < TLT::LA; ++j) {
+        int r, kq;
+        zslot<true, BM>((j * 4 + wave) * 64 + lane, r, kq);
+        offa[j] = min(m0 + r, m - 1) + (int64_t)kq * m;
     }
+    // B slots in both layouts: (k, column offset) of each copy
+    int kp[TLT::LB], kh[TLT::LB];
+    int64_t cp[TLT::LB], ch[TLT::LB];
+#pragma unroll
+    for (int j = 0; j < TLT::LB; ++j) {
+        const int p = (j * 4 + wave) * 64 + lane;
+        int r;
+        zslot<false, BN>(p, r, kp[j]);
+        cp[j] = (int64_t)min(n0 + r, n - 1) * nvo;
+        zslot<true, BN>(p, r, kh[j]);
+        ch[j] = min(n0 + r, n - 1);
+    }
+
+    ZAcc<TLT> acc;
+    acc.zero();
+    zloop<TLT, GAUSS>(
+        acc, Kp / ZBK, lds0, lds1,
+        [&](int s, double2* buf) {
+            const int k0 = s * ZBK;
+#pragma unroll
+            for (int j = 0; j < TLT::LA; ++j)
+                zcopy(Az + (int64_t)k0 * m + offa[j], buf + (j * 4 + wave) * 64);
+            double2* lb = buf + TLT::SA;
+            if (k0 < 3 * nvp) {
+                const int sg = k0 / nvp, e0 = k0 - sg * nvp;
+                const double2* base = VOVV + (int64_t)occ[sg] * nv;
+#pragma unroll
+                for (int j = 0; j < TLT::LB; ++j)
+                    zcopy(base + min(e0 + kp[j], nv - 1) + cp[j], lb + (j * 4 + wave) * 64);
+            } else {
+                const int c2 = k0 - 3 * nvp;
+                const int sg = c2 / nop, l0 = c2 - sg * nop;
+                const double2* base = T2F + (int64_t)occ[sg] * no * n;
+#pragma unroll
+                for (int j = 0; j < TLT::LB; ++j)
+                    zcopy(base + (int64_t)min(l0 + kh[j], no - 1) * n + ch[j],
+                          lb + (j * 4 + wave) * 64);
+            }
+        },
+        [&](int s, double2* buf, ZOps<TLT>& o) {
+            if (s * ZBK < 3 * nvp)
+                zread<TLT, true, false>(o, buf, buf + TLT::SA, wm, wn, lane, false, true);
+            else
+                zread<TLT, true, true>(o, buf, buf + TLT::SA, wm, wn, lane, false, false);
+        });
 
     double2* Uz = U + (int64_t)z * m * n;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TLT::TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TLT::TJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * 32 + i * 16 + lk + 4 * r;
-                const int gn = n0 + wn * 32 + j * 16 + lr;
-                if (gm < m && gn < n) {
-                    double xr, xi;
-                    if (GAUSS) {
-                        xr = acc0[i][j][r] - acc2[i][j][r];
-                        xi = acc1[i][j][r] - acc0[i][j][r] - acc2[i][j][r];
-                    } else {
-                        xr = acc0[i][j][r];
-                        xi = acc1[i][j][r];
-                    }
-                    Uz[gm + (int64_t)gn * m] = make_double2(xr, xi);
-                }
+                const int gm = m0 + zrow<TLT>(wm, i, lane, r);
+                const int gn = n0 + zcol<TLT>(wn, j, lane);
+                if (gm < m && gn < n) Uz[gm + (int64_t)gn * m] = acc.template value<GAUSS>(i, j, r);
             }
 }
 
@@ -596,7 +520,7 @@ int t3_run(T3State* s, const T3Input& in, int64_t first, int64_t count, hipStrea
         t3_expand_t2_kernel<<<grid, 256, 0, st>>>(no, nvvt, in.T2, s->T2F);
         T3CK(hipGetLastError());
     }
-    const int mt = (nv + BM - 1) / BM, nt = (int)((nvvt + BN - 1) / BN);
+    const int mt = (nv + TLT::BM - 1) / TLT::BM, nt = (int)((nvvt + TLT::BN - 1) / TLT::BN);
     for (int64_t c = 0; c < nchunk; ++c) {
         const int64_t z0 = first + c * s->zmax;
         const int zn = (int)std::min<int64_t>(s->zmax, first + count - z0);

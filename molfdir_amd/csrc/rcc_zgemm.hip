@@ -3,32 +3,23 @@
 //   C(m x n) = alpha * op(A)(m x k) * op(B)(k x n) + beta * C      (column-major)
 //
 // op is N (as stored), T (transposed), C (conjugate-transposed) or R
-// (conjugated, not transposed; with the other operand N).  Operands
-// are interleaved complex (re, im) doubles, the layout relccsd's REAL*8 pairs
-// use (RCW = 2).  Tiles are staged through LDS de-interleaved into planar
-// real / imaginary arrays.  Each complex product is formed on
-// v_mfma_f64_16x16x4_f64 in FP64 either
-//   - with four real products: re = Ar Br - Ai Bi, im = Ar Bi + Ai Br (the
-//     imaginary part of B is negated in a register, no accumulator is ever
-//     subtracted), or
-//   - with Gauss's three-multiplication form: P1 = Ar Br, P2 = Ai Bi,
-//     P3 = (Ar + Ai)(Br + Bi), re = P1 - P2, im = P3 - P1 - P2, which does 3/4
-//     of the matrix-core work (the "3M" product of ZGEMM3M libraries; its
-//     rounding error bound is normwise, see Higham, "Stability of a method for
-//     multiplying complex matrices with three real matrix multiplications",
-//     SIAM J. Matrix Anal. Appl. 13 (1992) 681).
+// (conjugated, not transposed; with the other operand N).  Operands are
+// interleaved complex (re, im) doubles, the layout relccsd's REAL*8 pairs use
+// (RCW = 2).  The tile machinery -- direct global -> LDS copies of 8-deep K
+// stages into two LDS buffers, interleaved complex in LDS, the 3M or 4M
+// complex product on v_mfma_f64_16x16x4_f64 -- is in rcc_zgemm_core.h and is
+// shared with the triples GEMM.
 //
-// Work decomposition: a 256-thread workgroup (4 wave64) owns a 64 x 64
-// complex C tile, each wave a 32 x 32 sub-tile (2 x 2 MFMA blocks of 16 x 16).
-// K is walked in steps of 16 through two LDS stages: the next global tile is
-// loaded into registers and written to the idle stage while the current stage
-// feeds the MFMAs, so one barrier per K-step suffices.  Tiles are numbered
-// so that each of the 8 XCDs (workgroups are dealt to XCDs round-robin) owns a
-// contiguous range of tiles ordered N-fastest: the workgroups sharing one
-// M-panel of A run together on one XCD and read it through that XCD's L2
-// once.  grid.z enumerates (batch, K-split).  K-splits write raw partial sums
-// to a workspace that a second kernel reduces in a fixed order, so results
-// are deterministic.
+// Work decomposition: a 256-thread workgroup (2 x 2 wave64) owns a 96 x 64
+// complex C tile, each wave 48 x 32 (3 x 2 MFMA blocks of 16 x 16); 40 KB
+// of LDS, so two workgroups share a CU and each one's barrier waits are
+// covered by the other's MFMAs.  Tiles are numbered so that each of the 8
+// XCDs (workgroups are dealt to XCDs round-robin) owns a contiguous range of
+// tiles ordered N-fastest: the workgroups sharing one M-panel of A run
+// together on one XCD and read it through that XCD's L2 once.  grid.z
+// enumerates (batch, K-split).  K-splits write raw partial sums to a
+// workspace that a second kernel reduces in a fixed order, so results are
+// deterministic.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <algorithm>
@@ -37,19 +28,47 @@
 #include <mutex>
 #include <vector>
 #include "rcc_internal.h"
-#include "rcc_mfma_tile.h"
+#include "rcc_zgemm_core.h"
 
 namespace rcc {
 
-// One workgroup: 64 x 64 tile of C for batch index / K-split given by blockIdx.z.
+// The default tile, and a skinny one (128 x 32, waves of 64 x 16) for
+// products with n <= 32: the occupied-index products (n = o) would leave
+// half of a 64-wide tile idle.
+using TLS = ZTile<4, 1>;
+constexpr int BM = ZTileGemm::BM;
+constexpr int BN = ZTileGemm::BN;
+
+// the tile of a product: skinny for untagged products with n <= 32 (the
+// tagged ladder / ring launches and the grouped launches use the default)
+constexpr bool skinny_n(int n) { return n <= TLS::BN; }
+int64_t tile_count(const GemmArgs& g) {
+    const bool sk = g.tag == 0 && skinny_n(g.n);
+    const int bm = sk ? TLS::BM : BM, bn = sk ? TLS::BN : BN;
+    return (int64_t)((g.m + bm - 1) / bm) * ((g.n + bn - 1) / bn) * g.batch;
+}
+
+template <int OP>
+__device__ __forceinline__ double2 cj(double2 v) {
+    if (OP == OP_C || OP == OP_R) v.y = -v.y;
+    return v;
+}
+
+// N and R read the operand as stored (R conjugates), T and C transposed
+template <int OP>
+constexpr bool as_stored() { return OP == OP_N || OP == OP_R; }
+template <int OP>
+constexpr bool conj_op() { return OP == OP_C || OP == OP_R; }
+
+// One workgroup: one TL tile of C for batch index / K-split given by blockIdx.z.
 // split == 1: C = alpha * acc + beta * C.  split > 1: raw acc goes to
 // W[(z) * m * n + i + j * m] and zgemm_reduce finishes the epilogue.
 
 // GROUPED: the problem (m, n, k, operands, alpha, beta) is read from the
 // group descriptor that owns this block (binary search on tile offsets); no
 // batch, no K-split.
-template <int OPA, int OPB, int TAG, int GAUSS, int GROUPED = 0>
-__global__ void __launch_bounds__(256)
+template <int OPA, int OPB, int TAG, int GAUSS, int GROUPED = 0, class TL = ZTileGemm>
+__global__ void __launch_bounds__(256, 2)     // two workgroups (two waves per SIMD) per CU
 zgemm_mfma_kernel(int m, int n, int k,
                   const double2* __restrict__ A, int64_t lda, int64_t sA,
                   const double2* __restrict__ B, int64_t ldb, int64_t sB,
@@ -57,11 +76,12 @@ zgemm_mfma_kernel(int m, int n, int k,
                   double2 alpha, double2 beta, int nsplit, int kchunk,
                   double2* __restrict__ W, const ZGroup* __restrict__ G = nullptr,
                   int ngroups = 0, const int* __restrict__ gmap = nullptr) {
-    // two LDS stages: the next K-tile is written while the current one feeds the MFMAs
-    __shared__ double As_re[2][BK * LDA_S];
-    __shared__ double As_im[2][BK * LDA_S];
-    __shared__ double Bs_re[2][BK * LDB_S];
-    __shared__ double Bs_im[2][BK * LDB_S];
+    // two stage buffers, distinct arrays (see rcc_zgemm_core.h)
+    __shared__ double2 lds0[TL::SA + TL::SB];
+    __shared__ double2 lds1[TL::SA + TL::SB];
+    constexpr int BM = TL::BM, BN = TL::BN;
+    constexpr bool AR = as_stored<OPA>();     // A: m fastest in memory
+    constexpr bool BR = !as_stored<OPB>();    // B (k x n as used): n fastest in memory
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -108,117 +128,89 @@ zgemm_mfma_kernel(int m, int n, int k,
     const int kbeg = split * kchunk;
     const int kend = min(k, kbeg + kchunk);
 
-    // 4M: acc0 = re, acc1 = im.  3M: acc0 = P1, acc1 = P3, acc2 = P2.
-    v4d acc0[2][2], acc1[2][2], acc2[2][2];
+    // per-lane element offsets of this thread's copies, relative to the
+    // stage base (constant over k); rows past the matrix edge are clamped to
+    // the last one: they only feed outputs that are never stored
+    int64_t offa[TL::LA], offb[TL::LB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            acc0[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc1[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc2[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-        }
-
-    double2 ra[4], rb[4];
-    const int lr = lane & 15;        // row (A) / column (B) inside a 16-block
-    const int lk = lane >> 4;        // k inside a 4-step
-    Loader LA = make_loader_a<OPA>(A, lda, m, m0, kbeg, tid);
-    Loader LB = make_loader_b<OPB>(B, ldb, n, n0, kbeg, tid);
-    if (kbeg < kend) {
-        const bool full = kbeg + BK <= kend;
-        load_step<OPA>(LA, kbeg, kend, full, ra);
-        load_step<OPB>(LB, kbeg, kend, full, rb);
-        store_a<OPA>(As_re[0], As_im[0], tid, ra);
-        store_b<OPB>(Bs_re[0], Bs_im[0], tid, rb);
+    for (int j = 0; j < TL::LA; ++j) {
+        int r, kq;
+        zslot<AR, BM>((j * 4 + wave) * 64 + lane, r, kq);
+        const int64_t g = min(m0 + r, m - 1);
+        offa[j] = AR ? g + (int64_t)kq * lda : (int64_t)kq + g * lda;
     }
-    __syncthreads();
-    int cur = 0;
-    for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        const bool more = k0 + BK < kend;
-        if (more) {
-            advance(LA);
-            advance(LB);
-            const bool full = k0 + 2 * BK <= kend;
-            load_step<OPA>(LA, k0 + BK, kend, full, ra);
-            load_step<OPB>(LB, k0 + BK, kend, full, rb);
+#pragma unroll
+    for (int j = 0; j < TL::LB; ++j) {
+        int r, kq;
+        zslot<BR, BN>((j * 4 + wave) * 64 + lane, r, kq);
+        const int64_t g = min(n0 + r, n - 1);
+        offb[j] = BR ? g + (int64_t)kq * ldb : (int64_t)kq + g * ldb;
+    }
+    const int64_t ka = AR ? lda : 1, kb = BR ? ldb : 1;
+    const double2* A0 = A + (int64_t)kbeg * ka;
+    const double2* B0 = B + (int64_t)kbeg * kb;
+    const int nst = kend > kbeg ? (kend - kbeg) / ZBK : 0;
+
+    ZAcc<TL> acc;
+    acc.zero();
+    zloop<TL, GAUSS>(
+        acc, nst, lds0, lds1,
+        [&](int s, double2* buf) {
+            const double2* ga = A0 + (int64_t)s * ZBK * ka;
+            const double2* gb = B0 + (int64_t)s * ZBK * kb;
+#pragma unroll
+            for (int j = 0; j < TL::LA; ++j) zcopy(ga + offa[j], buf + (j * 4 + wave) * 64);
+#pragma unroll
+            for (int j = 0; j < TL::LB; ++j)
+                zcopy(gb + offb[j], buf + TL::SA + (j * 4 + wave) * 64);
+        },
+        [&](int, double2* buf, ZOps<TL>& o) {
+            zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
+        });
+    // the last, partial stage: register copies, zero past kend
+    const int kt = kbeg + nst * ZBK;
+    if (kt < kend) {
+        __syncthreads();
+        double2* buf = (nst & 1) ? lds1 : lds0;
+        for (int p = tid; p < TL::SA; p += ZNT) {
+            int r, kq;
+            zslot<AR, BM>(p, r, kq);
+            const int64_t g = min(m0 + r, m - 1);
+            double2 v = make_double2(0.0, 0.0);
+            if (kt + kq < kend) v = AR ? A[g + (int64_t)(kt + kq) * lda] : A[(kt + kq) + g * lda];
+            buf[p] = v;
         }
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            double are[2], aim[2], bre[2], bim[2], bnim[2];
-            const int kr = kk + lk;
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = swz<!as_stored<OPA>()>(kr, wm * 32 + i * 16 + lr);
-                are[i] = As_re[cur][kr * LDA_S + r];
-                aim[i] = As_im[cur][kr * LDA_S + r];
-            }
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int c = swz<as_stored<OPB>()>(kr, wn * 32 + j * 16 + lr);
-                bre[j] = Bs_re[cur][kr * LDB_S + c];
-                bim[j] = Bs_im[cur][kr * LDB_S + c];
-                bnim[j] = -bim[j];
-            }
-            if (GAUSS) {
-                double asum[2], bsum[2];
-#pragma unroll
-                for (int i = 0; i < 2; ++i) asum[i] = are[i] + aim[i];
-#pragma unroll
-                for (int j = 0; j < 2; ++j) bsum[j] = bre[j] + bim[j];
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(asum[i], bsum[j], acc1[i][j], 0, 0, 0);
-                        acc2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], acc2[i][j], 0, 0, 0);
-                    }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], acc1[i][j], 0, 0, 0);
-                        acc0[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bnim[j], acc0[i][j], 0, 0, 0);
-                        acc1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], acc1[i][j], 0, 0, 0);
-                    }
-            }
-        }
-        if (more) {
-            store_a<OPA>(As_re[cur ^ 1], As_im[cur ^ 1], tid, ra);
-            store_b<OPB>(Bs_re[cur ^ 1], Bs_im[cur ^ 1], tid, rb);
+        for (int p = tid; p < TL::SB; p += ZNT) {
+            int r, kq;
+            zslot<BR, BN>(p, r, kq);
+            const int64_t g = min(n0 + r, n - 1);
+            double2 v = make_double2(0.0, 0.0);
+            if (kt + kq < kend) v = BR ? B[g + (int64_t)(kt + kq) * ldb] : B[(kt + kq) + g * ldb];
+            buf[TL::SA + p] = v;
         }
         __syncthreads();
-        cur ^= 1;
+        ZOps<TL> o;
+        zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
+        zmma<TL, GAUSS>(acc, o);
     }
 
-    // D layout of v_mfma_f64_16x16x4_f64 on gfx950 (pinned by the lane-layout
-    // probe test): lane l, element r -> row (l/16) + 4 r, column l%16
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TL::TI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TL::TJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm * 32 + i * 16 + lk + 4 * r;
-                const int gn = n0 + wn * 32 + j * 16 + lr;
+                const int gm = m0 + zrow<TL>(wm, i, lane, r);
+                const int gn = n0 + zcol<TL>(wn, j, lane);
                 if (gm < m && gn < n) {
-                    double xr, xi;
-                    if (GAUSS) {
-                        xr = acc0[i][j][r] - acc2[i][j][r];
-                        xi = acc1[i][j][r] - acc0[i][j][r] - acc2[i][j][r];
-                    } else {
-                        xr = acc0[i][j][r];
-                        xi = acc1[i][j][r];
-                    }
+                    const double2 x = acc.template value<GAUSS>(i, j, r);
                     if (nsplit > 1) {
-                        W[(int64_t)z * m * n + gm + (int64_t)gn * m] = make_double2(xr, xi);
+                        W[(int64_t)z * m * n + gm + (int64_t)gn * m] = x;
                     } else {
                         double2* cp = C + (int64_t)batch * sC + gm + (int64_t)gn * ldc;
                         double2 o;
-                        o.x = alpha.x * xr - alpha.y * xi;
-                        o.y = alpha.x * xi + alpha.y * xr;
+                        o.x = alpha.x * x.x - alpha.y * x.y;
+                        o.y = alpha.x * x.y + alpha.y * x.x;
                         if (beta.x != 0.0 || beta.y != 0.0) {
                             const double2 c = *cp;
                             o.x += beta.x * c.x - beta.y * c.y;
@@ -236,14 +228,15 @@ zgemm_mfma_kernel(int m, int n, int k,
 // irrep), where the large tile would mostly compute padding.  Tiles of a
 // group are numbered from tile_off as 32 x 32 tiles (zgroup_offsets_small).
 constexpr int SB = 32;
+constexpr int SBK = 16;     // k per step of the small-tile kernel
 
 template <int OPA, int OPB, int GAUSS>
 __device__ __forceinline__ void small_tile(int b, const ZGroup* __restrict__ G, int ngroups,
                                            const int* __restrict__ gmap,
-                                           double (&As_re)[BK][SB + 1],
-                                           double (&As_im)[BK][SB + 1],
-                                           double (&Bs_re)[BK][SB + 1],
-                                           double (&Bs_im)[BK][SB + 1]) {
+                                           double (&As_re)[SBK][SB + 1],
+                                           double (&As_im)[SBK][SB + 1],
+                                           double (&Bs_re)[SBK][SB + 1],
+                                           double (&Bs_im)[SBK][SB + 1]) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -268,7 +261,7 @@ __device__ __forceinline__ void small_tile(int b, const ZGroup* __restrict__ G, 
     const double2* __restrict__ Bm = gg.B;
     v4d acc0 = (v4d){0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0;
     const int lr = lane & 15, lk = lane >> 4;
-    for (int k0 = 0; k0 < k; k0 += BK) {
+    for (int k0 = 0; k0 < k; k0 += SBK) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             int r, kk;
@@ -297,7 +290,7 @@ __device__ __forceinline__ void small_tile(int b, const ZGroup* __restrict__ G, 
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
+        for (int kk = 0; kk < SBK; kk += 4) {
             const int kr = kk + lk;
             const double are = As_re[kr][wm * 16 + lr], aim = As_im[kr][wm * 16 + lr];
             const double bre = Bs_re[kr][wn * 16 + lr], bim = Bs_im[kr][wn * 16 + lr];
@@ -349,8 +342,8 @@ template <int OPA, int OPB, int GAUSS>
 __global__ void __launch_bounds__(256)
 zgemm_small_grouped_kernel(const ZGroup* __restrict__ G, int ngroups,
                            const int* __restrict__ gmap, int64_t blocks) {
-    __shared__ double As_re[BK][SB + 1], As_im[BK][SB + 1];
-    __shared__ double Bs_re[BK][SB + 1], Bs_im[BK][SB + 1];
+    __shared__ double As_re[SBK][SB + 1], As_im[SBK][SB + 1];
+    __shared__ double Bs_re[SBK][SB + 1], Bs_im[SBK][SB + 1];
     for (int64_t b = blockIdx.x; b < blocks; b += gridDim.x) {
         small_tile<OPA, OPB, GAUSS>((int)b, G, ngroups, gmap, As_re, As_im, Bs_re, Bs_im);
         __syncthreads();     // the next tile's LDS writes after every wave's last reads
@@ -438,18 +431,24 @@ __global__ void zscale_kernel(int m, int n, int batch, double2* __restrict__ C, 
     }
 }
 
-template <int OPA, int OPB, int TAG>
-static void launch_t(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
-    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
+template <int OPA, int OPB, int TAG, class TLX>
+static void launch_tl(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
+    const int64_t tiles = (int64_t)((g.m + TLX::BM - 1) / TLX::BM) * ((g.n + TLX::BN - 1) / TLX::BN);
     dim3 grid((unsigned)tiles, 1, g.batch * nsplit);
     if (g.gauss)
-        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 1>), grid, dim3(256), 0, st,
+        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 1, 0, TLX>), grid, dim3(256), 0, st,
                            g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
                            g.alpha, g.beta, nsplit, kchunk, W);
     else
-        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 0>), grid, dim3(256), 0, st,
+        hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 0, 0, TLX>), grid, dim3(256), 0, st,
                            g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
                            g.alpha, g.beta, nsplit, kchunk, W);
+}
+
+template <int OPA, int OPB, int TAG>
+static void launch_t(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
+    if (TAG == 0 && skinny_n(g.n)) launch_tl<OPA, OPB, TAG, TLS>(g, nsplit, kchunk, W, st);
+    else launch_tl<OPA, OPB, TAG, ZTileGemm>(g, nsplit, kchunk, W, st);
 }
 
 template <int OPA, int OPB>
@@ -533,7 +532,7 @@ int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, in
 }
 
 int choose_split(const GemmArgs& g, int64_t ws_elems) {
-    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN) * g.batch;
+    const int64_t tiles = tile_count(g);
     if (tiles >= 512 || g.k < 512) return 1;
     int64_t s = std::min<int64_t>((1024 + tiles - 1) / tiles, (g.k + 255) / 256);
     s = std::min<int64_t>(s, 256);
@@ -542,13 +541,13 @@ int choose_split(const GemmArgs& g, int64_t ws_elems) {
     return (int)std::max<int64_t>(1, s);
 }
 
-// (nsplit, kchunk) for a requested split count: k chunks a multiple of BK.
+// (nsplit, kchunk) for a requested split count: k chunks a multiple of ZBK.
 static void plan_split(const GemmArgs& g, int s, int& nsplit, int& kchunk) {
     nsplit = 1;
     kchunk = g.k;
     if (s <= 1) return;
     kchunk = (g.k + s - 1) / s;
-    kchunk = (kchunk + BK - 1) / BK * BK;
+    kchunk = (kchunk + ZBK - 1) / ZBK * ZBK;
     nsplit = (g.k + kchunk - 1) / kchunk;
     if (nsplit <= 1) {
         nsplit = 1;
@@ -576,14 +575,18 @@ static int launch_split(const GemmArgs& g, int nsplit, int kchunk, double2* ws, 
 
 // ---- K-split autotuner
 // A product whose output tiles cannot fill the chip (fewer than 512 tile x
-// batch workgroups) with a long k is split over k; how finely is a trade of
-// per-workgroup latency (k steps in a chain) against partial sums written
-// and reduced.  The first time a shape (ops, tag, complex form, m, n, k,
-// batch) is met outside graph capture, every split with k chunks of
-// 64 .. 4096 (and no split) is timed on the stream with HIP events -- run
-// in place with alpha = 0, beta = 1, which leaves C bit-for-bit unchanged --
-// and the fastest is kept for the process.  RCC_AUTOTUNE=0 in the
-// environment (or zgemm_autotune(0)) falls back to choose_split.
+// batch workgroups), or whose tiles leave the last round of workgroups
+// poorly filled (less than 95 % of the workgroups resident at once: two
+// per CU), and whose k is long, is split over k; how finely is a trade of
+// per-workgroup latency (k stages in a chain), of the round fill, and of
+// the partial sums written and reduced.  The first time a shape (ops, tag,
+// complex form, m, n, k, batch) is met outside graph capture, every split
+// with k chunks of 64 .. 4096, and 2 .. 8 equal chunks (and no split), is
+// timed on the stream with HIP events.  The trial products write a scratch
+// C of the product's size, allocated for the trial and freed after it, so
+// the live C is never touched; the fastest split is kept for the process.
+// RCC_AUTOTUNE=0 in the environment (or zgemm_autotune(0)) falls back to
+// choose_split.
 namespace {
 struct TuneKey {
     int opa, opb, tag, gauss, m, n, k, batch;
@@ -611,24 +614,58 @@ bool capturing(hipStream_t st) {
     return cs != hipStreamCaptureStatusNone;
 }
 
+// workgroups of the engine resident at once on the current device (two per CU)
+int64_t resident_slots() {
+    static int64_t slots[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 512;
+    if (!slots[dev]) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cu <= 0)
+            cu = 256;
+        slots[dev] = 2 * (int64_t)cu;
+    }
+    return slots[dev];
+}
+
+// fraction of the workgroup slots the tiles keep busy over their rounds
+double round_fill(int64_t tiles) {
+    const int64_t slots = resident_slots();
+    return (double)tiles / (double)(((tiles + slots - 1) / slots) * slots);
+}
+
 // Times the candidate splits; returns the fastest split count.
 int tune_split(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st, int fallback) {
-    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN) * g.batch;
+    const int64_t tiles = tile_count(g);
     const int64_t per = (int64_t)g.m * g.n * g.batch;
     std::vector<int> cand{1};
-    for (int c = 64; c <= 4096; c *= 2) {
-        const int64_t s = (g.k + c - 1) / c;
-        if (s < 2 || s > 4096 || per * s > ws_elems || tiles * s > 65536) continue;
+    auto add = [&](int64_t s) {
+        if (s < 2 || s > 4096 || per * s > ws_elems || tiles * s > 65536) return;
         int ns, kc;
         plan_split(g, (int)s, ns, kc);
-        if (std::find(cand.begin(), cand.end(), ns) == cand.end()) cand.push_back(ns);
-    }
+        if (ns > 1 && std::find(cand.begin(), cand.end(), ns) == cand.end()) cand.push_back(ns);
+    };
+    for (int c = 64; c <= 4096; c *= 2) add((g.k + c - 1) / c);
+    for (int e = 2; e <= 8; ++e) add(e);
     if (cand.size() == 1) return fallback;
+    // trial products on a zeroed scratch C (same shape, tightly packed)
     GemmArgs t = g;
-    t.alpha = make_double2(0.0, 0.0);
-    t.beta = make_double2(1.0, 0.0);
+    double2* ct = nullptr;
+    if (hipMalloc((void**)&ct, (size_t)per * sizeof(double2)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fallback;
+    }
+    (void)hipMemsetAsync(ct, 0, (size_t)per * sizeof(double2), st);
+    t.C = ct;
+    t.ldc = g.m;
+    t.sC = (int64_t)g.m * g.n;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fallback;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(ct);
+        return fallback;
+    }
     int best = fallback;
     float best_ms = 1e30f;
     for (int s : cand) {
@@ -651,6 +688,8 @@ int tune_split(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st,
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(ct);
     return best;
 }
 }  // namespace
@@ -698,8 +737,10 @@ int zgemm(const GemmArgs& g0, double2* ws, int64_t ws_elems, hipStream_t st) {
         return 0;
     }
     int s = ws ? choose_split(g, ws_elems) : 1;
-    const int64_t tiles = (int64_t)((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN) * g.batch;
-    if (ws && tiles < 512 && g.k >= 256) {
+    const int64_t tiles = tile_count(g);
+    const bool few = tiles < 512 && g.k >= 256;
+    const bool ragged = g.k >= 2048 && round_fill(tiles) < 0.95;
+    if (ws && (few || ragged)) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         if (autotune_on()) {
             const TuneKey key{g.opA, g.opB, g.tag, g.gauss, g.m, g.n, g.k, g.batch};

@@ -27,7 +27,8 @@ def test_mfma_f64_lane_layout():
 @pytest.mark.parametrize("opa", [0, 1, 2])
 @pytest.mark.parametrize("opb", [0, 1, 2])
 @pytest.mark.parametrize("m,n,k", [(64, 64, 16), (37, 91, 53), (5, 3, 3000), (130, 70, 1),
-                                   (257, 129, 77)])
+                                   (257, 129, 77), (300, 30, 301), (129, 32, 40), (257, 33, 9),
+                                   (1000, 1, 500)])
 def test_zgemm_against_numpy(opa, opb, m, n, k, three_mult):
     rng = np.random.default_rng(m * 7 + n * 3 + k + 10 * opa + opb)
 
@@ -84,8 +85,8 @@ def test_zgemm_conjugate_operand_against_numpy(opa, opb, m, n, k, three_mult):
 @pytest.mark.parametrize("opa,opb,m,n,k,batch", [(2, 0, 7, 9, 40000, 1), (0, 0, 200, 20, 9000, 1),
                                                   (1, 0, 64, 20, 30000, 1)])
 def test_autotuned_split_gemm_against_numpy(opa, opb, m, n, k, batch):
-    # first call of the shape: the autotuner times its K-splits in place
-    # (alpha 0, beta 1) before the real product, which must see C untouched
+    # first call of the shape: the autotuner times its K-splits on a scratch
+    # C before the real product, which must see the live C untouched
     rng = np.random.default_rng(m + n + k)
 
     def mat(r, c):
