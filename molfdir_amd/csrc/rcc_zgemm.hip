@@ -38,6 +38,7 @@ namespace rcc {
 using TLS = ZTile<4, 1>;
 constexpr int BM = ZTileGemm::BM;
 constexpr int BN = ZTileGemm::BN;
+constexpr int ZBAND = 8;      // tile rows per band of the tile order
 
 // the tile of a product: skinny for untagged products with n <= 32 (the
 // tagged ladder / ring launches and the grouped launches use the default)
@@ -75,7 +76,7 @@ zgemm_mfma_kernel(int m, int n, int k,
                   double2* __restrict__ C, int64_t ldc, int64_t sC,
                   double2 alpha, double2 beta, int nsplit, int kchunk,
                   double2* __restrict__ W, const ZGroup* __restrict__ G = nullptr,
-                  int ngroups = 0, const int* __restrict__ gmap = nullptr) {
+                  int ngroups = 0, const int* __restrict__ gmap = nullptr, int band_rows = ZBAND) {
     // two stage buffers, distinct arrays (see rcc_zgemm_core.h)
     __shared__ double2 lds0[TL::SA + TL::SB];
     __shared__ double2 lds1[TL::SA + TL::SB];
@@ -117,9 +118,6 @@ zgemm_mfma_kernel(int m, int n, int k,
     const int xcd = b % NXCD, idx = b / NXCD;
     const int q = T / NXCD, rr = T % NXCD;
     const int start = xcd < rr ? xcd * (q + 1) : rr * (q + 1) + (xcd - rr) * q;
-    const int tile = start + idx;
-    const int m0 = (tile / nt) * BM;
-    const int n0 = (tile % nt) * BN;
     const int z = blockIdx.z;
     const int batch = z / nsplit;
     const int split = z - batch * nsplit;
@@ -127,99 +125,117 @@ zgemm_mfma_kernel(int m, int n, int k,
     B += (int64_t)batch * sB;
     const int kbeg = split * kchunk;
     const int kend = min(k, kbeg + kchunk);
-
-    // per-lane element offsets of this thread's copies, relative to the
-    // stage base (constant over k); rows past the matrix edge are clamped to
-    // the last one: they only feed outputs that are never stored
-    int64_t offa[TL::LA], offb[TL::LB];
-#pragma unroll
-    for (int j = 0; j < TL::LA; ++j) {
-        int r, kq;
-        zslot<AR, BM>((j * 4 + wave) * 64 + lane, r, kq);
-        const int64_t g = min(m0 + r, m - 1);
-        offa[j] = AR ? g + (int64_t)kq * lda : (int64_t)kq + g * lda;
-    }
-#pragma unroll
-    for (int j = 0; j < TL::LB; ++j) {
-        int r, kq;
-        zslot<BR, BN>((j * 4 + wave) * 64 + lane, r, kq);
-        const int64_t g = min(n0 + r, n - 1);
-        offb[j] = BR ? g + (int64_t)kq * ldb : (int64_t)kq + g * ldb;
-    }
     const int64_t ka = AR ? lda : 1, kb = BR ? ldb : 1;
     const double2* A0 = A + (int64_t)kbeg * ka;
     const double2* B0 = B + (int64_t)kbeg * kb;
     const int nst = kend > kbeg ? (kend - kbeg) / ZBK : 0;
 
-    ZAcc<TL> acc;
-    acc.zero();
-    zloop<TL, GAUSS>(
-        acc, nst, lds0, lds1,
-        [&](int s, double2* buf) {
-            const double2* ga = A0 + (int64_t)s * ZBK * ka;
-            const double2* gb = B0 + (int64_t)s * ZBK * kb;
+    {
+        // bands of band_rows tile rows, column by column inside a band: the
+        // workgroups an XCD runs at once cover a near-square block of tiles
+        // and share its A and B panels through the XCD's L2 (band_rows = 1:
+        // N-fastest rows)
+        int tm, tn;
+        {
+            const int t = start + idx;
+            const int band = band_rows * nt;
+            const int first = (t / band) * band_rows;
+            const int rows = min(mt - first, band_rows);
+            const int r = t - (t / band) * band;
+            tm = first + r % rows;
+            tn = r / rows;
+        }
+        const int m0 = tm * BM;
+        const int n0 = tn * BN;
+        // per-lane element offsets of this thread's copies, relative to the
+        // stage base (constant over k); rows past the matrix edge are clamped to
+        // the last one: they only feed outputs that are never stored
+        int64_t offa[TL::LA], offb[TL::LB];
 #pragma unroll
-            for (int j = 0; j < TL::LA; ++j) zcopy(ga + offa[j], buf + (j * 4 + wave) * 64);
-#pragma unroll
-            for (int j = 0; j < TL::LB; ++j)
-                zcopy(gb + offb[j], buf + TL::SA + (j * 4 + wave) * 64);
-        },
-        [&](int, double2* buf, ZOps<TL>& o) {
-            zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
-        });
-    // the last, partial stage: register copies, zero past kend
-    const int kt = kbeg + nst * ZBK;
-    if (kt < kend) {
-        __syncthreads();
-        double2* buf = (nst & 1) ? lds1 : lds0;
-        for (int p = tid; p < TL::SA; p += ZNT) {
+        for (int j = 0; j < TL::LA; ++j) {
             int r, kq;
-            zslot<AR, BM>(p, r, kq);
+            zslot<AR, BM>((j * 4 + wave) * 64 + lane, r, kq);
             const int64_t g = min(m0 + r, m - 1);
-            double2 v = make_double2(0.0, 0.0);
-            if (kt + kq < kend) v = AR ? A[g + (int64_t)(kt + kq) * lda] : A[(kt + kq) + g * lda];
-            buf[p] = v;
+            offa[j] = AR ? g + (int64_t)kq * lda : (int64_t)kq + g * lda;
         }
-        for (int p = tid; p < TL::SB; p += ZNT) {
+#pragma unroll
+        for (int j = 0; j < TL::LB; ++j) {
             int r, kq;
-            zslot<BR, BN>(p, r, kq);
+            zslot<BR, BN>((j * 4 + wave) * 64 + lane, r, kq);
             const int64_t g = min(n0 + r, n - 1);
-            double2 v = make_double2(0.0, 0.0);
-            if (kt + kq < kend) v = BR ? B[g + (int64_t)(kt + kq) * ldb] : B[(kt + kq) + g * ldb];
-            buf[TL::SA + p] = v;
+            offb[j] = BR ? g + (int64_t)kq * ldb : (int64_t)kq + g * ldb;
         }
-        __syncthreads();
-        ZOps<TL> o;
-        zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
-        zmma<TL, GAUSS>(acc, o);
-    }
+
+        ZAcc<TL> acc;
+        acc.zero();
+        zloop<TL, GAUSS>(
+            acc, nst, lds0, lds1,
+            [&](int s, double2* buf) {
+                const double2* ga = A0 + (int64_t)s * ZBK * ka;
+                const double2* gb = B0 + (int64_t)s * ZBK * kb;
+#pragma unroll
+                for (int j = 0; j < TL::LA; ++j) zcopy(ga + offa[j], buf + (j * 4 + wave) * 64);
+#pragma unroll
+                for (int j = 0; j < TL::LB; ++j)
+                    zcopy(gb + offb[j], buf + TL::SA + (j * 4 + wave) * 64);
+            },
+            [&](int, double2* buf, ZOps<TL>& o) {
+                zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
+            });
+        // the last, partial stage: register copies, zero past kend
+        const int kt = kbeg + nst * ZBK;
+        if (kt < kend) {
+            __syncthreads();
+            double2* buf = (nst & 1) ? lds1 : lds0;
+            for (int p = tid; p < TL::SA; p += ZNT) {
+                int r, kq;
+                zslot<AR, BM>(p, r, kq);
+                const int64_t g = min(m0 + r, m - 1);
+                double2 v = make_double2(0.0, 0.0);
+                if (kt + kq < kend) v = AR ? A[g + (int64_t)(kt + kq) * lda] : A[(kt + kq) + g * lda];
+                buf[p] = v;
+            }
+            for (int p = tid; p < TL::SB; p += ZNT) {
+                int r, kq;
+                zslot<BR, BN>(p, r, kq);
+                const int64_t g = min(n0 + r, n - 1);
+                double2 v = make_double2(0.0, 0.0);
+                if (kt + kq < kend) v = BR ? B[g + (int64_t)(kt + kq) * ldb] : B[(kt + kq) + g * ldb];
+                buf[TL::SA + p] = v;
+            }
+            __syncthreads();
+            ZOps<TL> o;
+            zread<TL, AR, BR>(o, buf, buf + TL::SA, wm, wn, lane, conj_op<OPA>(), conj_op<OPB>());
+            zmma<TL, GAUSS>(acc, o);
+        }
 
 #pragma unroll
-    for (int i = 0; i < TL::TI; ++i)
+        for (int i = 0; i < TL::TI; ++i)
 #pragma unroll
-        for (int j = 0; j < TL::TJ; ++j)
+            for (int j = 0; j < TL::TJ; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + zrow<TL>(wm, i, lane, r);
-                const int gn = n0 + zcol<TL>(wn, j, lane);
-                if (gm < m && gn < n) {
-                    const double2 x = acc.template value<GAUSS>(i, j, r);
-                    if (nsplit > 1) {
-                        W[(int64_t)z * m * n + gm + (int64_t)gn * m] = x;
-                    } else {
-                        double2* cp = C + (int64_t)batch * sC + gm + (int64_t)gn * ldc;
-                        double2 o;
-                        o.x = alpha.x * x.x - alpha.y * x.y;
-                        o.y = alpha.x * x.y + alpha.y * x.x;
-                        if (beta.x != 0.0 || beta.y != 0.0) {
-                            const double2 c = *cp;
-                            o.x += beta.x * c.x - beta.y * c.y;
-                            o.y += beta.x * c.y + beta.y * c.x;
+                for (int r = 0; r < 4; ++r) {
+                    const int gm = m0 + zrow<TL>(wm, i, lane, r);
+                    const int gn = n0 + zcol<TL>(wn, j, lane);
+                    if (gm < m && gn < n) {
+                        const double2 x = acc.template value<GAUSS>(i, j, r);
+                        if (nsplit > 1) {
+                            W[(int64_t)z * m * n + gm + (int64_t)gn * m] = x;
+                        } else {
+                            double2* cp = C + (int64_t)batch * sC + gm + (int64_t)gn * ldc;
+                            double2 o;
+                            o.x = alpha.x * x.x - alpha.y * x.y;
+                            o.y = alpha.x * x.y + alpha.y * x.x;
+                            if (beta.x != 0.0 || beta.y != 0.0) {
+                                const double2 c = *cp;
+                                o.x += beta.x * c.x - beta.y * c.y;
+                                o.y += beta.x * c.y + beta.y * c.x;
+                            }
+                            *cp = o;
                         }
-                        *cp = o;
                     }
                 }
-            }
+    }
 }
 
 // Small-tile grouped GEMM: a 32 x 32 C tile per workgroup (four waves, one
@@ -431,18 +447,37 @@ __global__ void zscale_kernel(int m, int n, int batch, double2* __restrict__ C, 
     }
 }
 
+// workgroups of the engine resident at once on the current device (two per CU)
+int64_t resident_slots() {
+    static int64_t slots[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 512;
+    if (!slots[dev]) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cu <= 0)
+            cu = 256;
+        slots[dev] = 2 * (int64_t)cu;
+    }
+    return slots[dev];
+}
+
+
 template <int OPA, int OPB, int TAG, class TLX>
 static void launch_tl(const GemmArgs& g, int nsplit, int kchunk, double2* W, hipStream_t st) {
     const int64_t tiles = (int64_t)((g.m + TLX::BM - 1) / TLX::BM) * ((g.n + TLX::BN - 1) / TLX::BN);
     dim3 grid((unsigned)tiles, 1, g.batch * nsplit);
+    // the ladder's few tile columns share its A panels best row by row
+    // (measured: fewer fetched bytes); the rest in bands of 8 tile rows
+    const int band = TAG == 1 ? 1 : ZBAND;
     if (g.gauss)
         hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 1, 0, TLX>), grid, dim3(256), 0, st,
                            g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
-                           g.alpha, g.beta, nsplit, kchunk, W);
+                           g.alpha, g.beta, nsplit, kchunk, W, nullptr, 0, nullptr, band);
     else
         hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, TAG, 0, 0, TLX>), grid, dim3(256), 0, st,
                            g.m, g.n, g.k, g.A, g.lda, g.sA, g.B, g.ldb, g.sB, g.C, g.ldc, g.sC,
-                           g.alpha, g.beta, nsplit, kchunk, W);
+                           g.alpha, g.beta, nsplit, kchunk, W, nullptr, 0, nullptr, band);
 }
 
 template <int OPA, int OPB, int TAG>
@@ -612,21 +647,6 @@ bool capturing(hipStream_t st) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess) return true;
     return cs != hipStreamCaptureStatusNone;
-}
-
-// workgroups of the engine resident at once on the current device (two per CU)
-int64_t resident_slots() {
-    static int64_t slots[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 512;
-    if (!slots[dev]) {
-        int cu = 0;
-        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            cu <= 0)
-            cu = 256;
-        slots[dev] = 2 * (int64_t)cu;
-    }
-    return slots[dev];
 }
 
 // fraction of the workgroup slots the tiles keep busy over their rounds
