@@ -617,20 +617,6 @@ def cfg2_vs_cpu():
     return res
 
 
-def batch_systems(nsys, nrep, seed):
-    """Sizes of the batched workload: occupied ~ U{4..16}, virtual ~ U{40..160},
-    both split over the irreps with one Dirichlet(2) draw of proportions."""
-    import numpy as np
-    from molfdir_amd import synth
-    rng = np.random.Generator(np.random.PCG64(seed))
-    out = []
-    for k in range(nsys):
-        o, v = int(rng.integers(4, 17)), int(rng.integers(40, 161))
-        p = rng.dirichlet([2.0] * nrep)
-        out.append((o, v, synth.dirichlet_split(o, p), synth.dirichlet_split(v, p), 1000 + k))
-    return out
-
-
 def bench_batch(args):
     """Many small symmetric systems: a per-system loop of eager iterations
     against the batched iterations -- rcc_iterate_batch (each system's term
@@ -642,7 +628,7 @@ def bench_batch(args):
     from molfdir_amd import synth
     from molfdir_amd.relccsd import RelCCSD
     nsys, nrep, seed = BATCH[args.config]
-    sizes = batch_systems(nsys, nrep, seed)
+    sizes = synth.batch_sizes(nsys, nrep, seed)
     dev = torch.device("cuda:0")
     solvers = []
     t0 = time.perf_counter()

@@ -294,6 +294,20 @@ def to_blocks(name, x, sym, maps=None):
     return np.concatenate(parts) if parts else np.zeros(0, np.asarray(x).dtype)
 
 
+def batch_sizes(nsys: int, nrep: int, seed: int) -> list:
+    """Sizes of the batched many-systems workload: per system, occupied ~
+    U{4..16} and virtual ~ U{40..160}, both split over nrep irreps with one
+    Dirichlet(2) draw of proportions; returns (o, v, o per irrep, v per
+    irrep, generator seed) tuples."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(nsys):
+        o, v = int(rng.integers(4, 17)), int(rng.integers(40, 161))
+        p = rng.dirichlet([2.0] * nrep)
+        out.append((o, v, dirichlet_split(o, p), dirichlet_split(v, p), 1000 + k))
+    return out
+
+
 def dirichlet_split(total: int, props) -> list:
     """Integer counts summing to `total` in the given proportions (largest
     remainders)."""
