@@ -137,7 +137,9 @@ int rcc_set_ints(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols, cons
 int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncols,
                         const void* dbuf);
 
-/* Amplitudes in / out (PUTAMPT / GETAMPT ccgetv.F:577-709). */
+/* Amplitudes in / out (PUTAMPT / GETAMPT ccgetv.F:577-709).  Setting them
+ * restarts the iteration from them as CCEQN's entry does (cceqns.F:255-260):
+ * iteration count and DIIS history start afresh. */
 int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2);
 int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2);
 /* Copy of a device-resident array (RCC_ARR_*) to host memory. */

@@ -2582,12 +2582,16 @@ int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
     if (h->sym.nrep > 1) {
         if (t1) CK(sym_to_c1(h, RCC_LAY_T1, t1, hipMemcpyHostToDevice, h->T1(), h->nt1));
         if (t2) CK(sym_to_c1(h, RCC_LAY_T2, t2, hipMemcpyHostToDevice, h->T2(), h->nt2));
-        invalidate_t(h);
-        return 0;
+    } else {
+        if (t1) CK(upload(h, h->T1(), t1, h->nt1, hipMemcpyHostToDevice));
+        if (t2) CK(upload(h, h->T2(), t2, h->nt2, hipMemcpyHostToDevice));
     }
-    if (t1) CK(upload(h, h->T1(), t1, h->nt1, hipMemcpyHostToDevice));
-    if (t2) CK(upload(h, h->T2(), t2, h->nt2, hipMemcpyHostToDevice));
     invalidate_t(h);
+    // a new start, as CCEQN's entry (NIT = ITC = IT = 0, cceqns.F:255-260)
+    h->nit = 0;
+    h->it = 0;
+    h->itc = 0;
+    std::fill(h->bb2.begin(), h->bb2.end(), 0.0);
     return 0;
 }
 

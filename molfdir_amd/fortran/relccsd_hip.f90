@@ -61,6 +61,19 @@ module relccsd_hip
       type(c_ptr), value :: h
       real(c_double), intent(out) :: t1(*), t2(*)
     end function
+    ! start the iterations from these amplitudes (CCEQN's entry, cceqns.F:255-260)
+    integer(c_int) function rcc_set_amplitudes(h, t1, t2) bind(c, name="rcc_set_amplitudes")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: t1(*), t2(*)
+    end function
+    ! one CCEQN iteration (cceqns.F:293-569): terms, DENOM, ZCORE, DIISX, CCENG, S -> T
+    integer(c_int) function rcc_iterate(h, nit, ecc, rms) bind(c, name="rcc_iterate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: nit
+      real(c_double), intent(out) :: ecc, rms
+    end function
     integer(c_int) function rcc_layout_block(h, lay, block, rows, cols, offset) &
         bind(c, name="rcc_layout_block")
       import :: c_int, c_ptr, c_int64_t

@@ -145,13 +145,13 @@ def synthetic_list(nkr: int, seed: int, keep: float = 0.7, scale: float = 0.05):
 
 
 def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, parts: int = 1,
-                    breit: bool = False):
+                    breit: bool = False, scale: float = 0.05):
     """C1 MRCONEE + MDCINT in dirpath: NSP = 2 NKR spinors with shuffled
     Kramers pairs, the first nocc spinors (input order) occupied, orbital
     energies ascending, a Hermitian one-electron matrix (diagonal eps,
-    off-diagonal N(0, 0.02^2)).  breit: MRCONEE's BREIT flag set and a
-    second seeded list (values N(0, 0.01^2)) written as MDBINT.  Returns the
-    generated arrays."""
+    off-diagonal N(0, 0.02^2)), integral values N(0, scale^2).  breit:
+    MRCONEE's BREIT flag set and a second seeded list (values N(0, 0.01^2))
+    written as MDBINT.  Returns the generated arrays."""
     import os
     rng = np.random.Generator(np.random.PCG64(seed + 7919))
     nsp = 2 * nkr
@@ -161,7 +161,7 @@ def synthetic_files(dirpath, nkr: int, nocc: int, seed: int, keep: float = 0.7, 
     h1 = 0.02 * (rng.normal(size=(nsp, nsp)) + 1j * rng.normal(size=(nsp, nsp)))
     h1 = 0.5 * (h1 + h1.conj().T)
     h1[np.arange(nsp), np.arange(nsp)] = eps
-    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
+    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep, scale)
     write_mrconee(os.path.join(dirpath, "MRCONEE"), eps, np.ones(nsp, np.int32), [nocc], h1,
                   breit=breit)
     write_mdcint(os.path.join(dirpath, "MDCINT"), kr, ik, jk, kk, lk, val, parts=parts)
@@ -189,7 +189,7 @@ def abelian_multb(nrep: int) -> np.ndarray:
 
 
 def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float = 0.7,
-                        breit: bool = False):
+                        breit: bool = False, scale: float = 0.05):
     """MRCONEE + MDCINT with the symmetry of Z2^k x {fermion, boson}
     (len(nkr_irrep) = 2, 4 or 8 fermion irreps).  Both members of a Kramers
     pair lie in the pair's irrep; spinors are written in a shuffled order, and
@@ -198,7 +198,7 @@ def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float =
     keeps only integrals whose four Kramers pairs have irreps that multiply
     to the totally symmetric one (so every image is allowed).  breit:
     MRCONEE's BREIT flag set and a second list, filtered the same way, as
-    MDBINT."""
+    MDBINT.  Integral values N(0, scale^2)."""
     import os
     nrep = len(nkr_irrep)
     rng = np.random.Generator(np.random.PCG64(seed + 104729))
@@ -219,7 +219,7 @@ def synthetic_files_sym(dirpath, nkr_irrep, nocc_irrep, seed: int, keep: float =
     h1 = 0.5 * (h1 + h1.conj().T)
     h1[irreps[:, None] != irreps[None, :]] = 0.0
     h1[np.arange(nsp), np.arange(nsp)] = eps
-    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep)
+    ik, jk, kk, lk, val = synthetic_list(nkr, seed, keep, scale)
     g = pair_irrep
     ok = (g[np.abs(ik) - 1] ^ g[np.abs(jk) - 1] ^ g[np.abs(kk) - 1] ^ g[np.abs(lk) - 1]) == 0
     ik, jk, kk, lk, val = ik[ok], jk[ok], kk[ok], lk[ok], val[ok]

@@ -31,6 +31,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "_ref", "cchrns.x")
+# relccsd's own PROGRAM RELCCSD (ccmain.F), CPU
+PROGRAM = os.path.join(HERE, "_ref", "relccsd.x")
+# the same two programs with the GPU solver linked in place of cceqns.F's
+# MP2EQN and CCEQN (molfdir_amd/fortran/cceqn_hip.F): relccsd's CCENER is
+# then the caller of the GPU solver
+EXE_GPU = os.path.join(HERE, "_ref", "cchrns_gpu.x")
+PROGRAM_GPU = os.path.join(HERE, "_ref", "relccsd_gpu.x")
 
 CLASS_ORDER = ("FOO", "FVO", "FVV", "OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV")
 
@@ -105,15 +112,16 @@ def parse_iterations(log: str):
 
 
 def run(sysd, mode="ccsd", maxit=20, maxdim=6, ntol=12, t1=None, t2=None,
-        workdir=None, timeout=3600, keep_mccres=None):
+        workdir=None, timeout=3600, keep_mccres=None, exe=None):
     """Outputs are in the reference's (symmetry) layout; for a C1 system
-    that is the C1 layout."""
+    that is the C1 layout.  exe: the harness binary (default the CPU one)."""
     lay = layout_of(sysd)
     nt1, nt2 = lay.size("T1"), lay.size("T2")
     m = {"ccsd": 1, "terms": 2, "t3": 3, "restart": 5}[mode]
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         _write_input(os.path.join(d, "ccin.bin"), sysd, m, maxit, maxdim, ntol, t1, t2)
-        p = subprocess.run([EXE], cwd=d, capture_output=True, text=True, timeout=timeout)
+        p = subprocess.run([exe or EXE], cwd=d, capture_output=True, text=True,
+                           timeout=timeout)
         log = p.stdout + p.stderr
         if p.returncode != 0 or not os.path.exists(os.path.join(d, "ccout.bin")):
             raise RuntimeError(f"reference harness failed (rc={p.returncode}):\n{log[-4000:]}")
@@ -203,6 +211,44 @@ def run_sort(files_dir, no, nv, workdir=None, timeout=3600, sym=None, useoe=Fals
         m = re.search(pat, log)
         out[key] = float(m.group(1)) if m else None
     m = re.search(r"Total time in .*?sort.*?:\s+([\d.]+)", log, re.I)
+    return out
+
+
+def run_program(files_dir, nelec, maxit=20, maxdim=6, ntol=10, nfroz=None, exe=None,
+                timeout=3600, keep_dir=None):
+    """relccsd itself (PROGRAM RELCCSD, ccmain.F) on files_dir/MRCONEE +
+    MDCINT: the integral sort CCSETI -> RDINTS, then CCENER's MP2 and CCSD,
+    from a MOLFDIR.INP with namelists RELCCSD (NELEC and NFROZ per irrep,
+    CARITH, DOSORT, DOENER, IPRNT = 1, TIMING) and CCENER (MAXIT, MAXDIM,
+    NTOL; ccdriv.F:155-225).  Returns the log, the printed SCF / MP2 / CCSD
+    energies, the CCEQN table and the MCCRES restart file's bytes.  exe:
+    the program binary (default the CPU one)."""
+    import shutil
+    line = ",".join(str(int(x)) for x in nelec)
+    frz = "" if nfroz is None else " NFROZ=" + ",".join(str(int(x)) for x in nfroz) + ","
+    with tempfile.TemporaryDirectory() as d:
+        for fn in sorted(os.listdir(files_dir)):
+            if fn == "MRCONEE" or fn.startswith("MDCINT") or fn.startswith("MDBINT"):
+                shutil.copy(os.path.join(files_dir, fn), os.path.join(d, fn))
+        with open(os.path.join(d, "MOLFDIR.INP"), "w") as f:
+            f.write(f" &RELCCSD NELEC={line},{frz} CARITH=T, DOSORT=T, DOENER=T, IPRNT=1,"
+                    f" TIMING=T, MWORD=15 /\n")
+            f.write(f" &CCENER DOMP2=T, DOCCSD=T, MAXIT={maxit}, MAXDIM={maxdim},"
+                    f" NTOL={ntol} /\n")
+        p = subprocess.run([exe or PROGRAM], cwd=d, capture_output=True, text=True,
+                           timeout=timeout)
+        log = p.stdout + p.stderr
+        if p.returncode != 0 or "Normal end of RELCCSD" not in log:
+            raise RuntimeError(f"relccsd failed (rc={p.returncode}):\n{log[-4000:]}")
+        mccres = open(os.path.join(d, "MCCRES"), "rb").read()
+        if keep_dir:
+            shutil.copy(os.path.join(d, "MCCRES"), os.path.join(keep_dir, "MCCRES"))
+    out = {"log": log, "iterations": parse_iterations(log), "mccres": mccres}
+    for key, pat in (("escf", r"^ SCF energy :\s+(-?[\d.]+)"),
+                     ("emp2", r"^ MP2 correlation energy :\s+(-?[\d.]+)"),
+                     ("eccsd", r"^ CCSD correlation energy :\s+(-?[\d.]+)")):
+        m = re.findall(pat, log, re.M)
+        out[key] = float(m[-1]) if m else None
     return out
 
 

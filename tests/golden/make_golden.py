@@ -507,9 +507,55 @@ def lambda_case(name, no, nv, seed, scale, fp, vseed, h, maxit, maxdim, ntol):
     print(name, out["e0"], fd, fd4, out["nit_e0"])
 
 
+# relccsd itself (PROGRAM RELCCSD) on synthetic MOLFDIR files: name, Kramers
+# pairs per irrep, occupied spinors per irrep, seed, keep, integral scale,
+# MAXIT, MAXDIM, NTOL, NFROZ per irrep (or None)
+PROGRAM_CASES = [
+    ("c1_k14", (14,), (6,), 11, 0.7, 0.01, 20, 6, 10, None),
+    ("c1frz_k14", (14,), (6,), 11, 0.7, 0.01, 20, 6, 10, (2,)),
+    ("s4_k14", (5, 3, 3, 3), (2, 2, 1, 1), 5, 0.8, 0.01, 20, 6, 10, (1, 0, 1, 0)),
+    ("s8_k16", (3, 2, 2, 2, 2, 2, 2, 1), (2, 1, 1, 1, 1, 1, 1, 0), 7, 0.8, 0.01, 20, 6, 10,
+     None),
+]
+
+
+def program_files(d, nkr_irrep, nocc_irrep, seed, keep, scale):
+    """The MOLFDIR files of a PROGRAM_CASES entry; returns (no, nv) per irrep."""
+    from molfdir_amd import mdcint
+    if len(nkr_irrep) == 1:
+        mdcint.synthetic_files(d, nkr_irrep[0], nocc_irrep[0], seed, keep, scale=scale)
+        return [nocc_irrep[0]], [2 * nkr_irrep[0] - nocc_irrep[0]]
+    o = mdcint.synthetic_files_sym(d, nkr_irrep, nocc_irrep, seed, keep, scale=scale)
+    return o["no"], o["nv"]
+
+
+def program_case(name, nkr_irrep, nocc_irrep, seed, keep, scale, maxit, maxdim, ntol, nfroz):
+    import tempfile
+    from molfdir_amd.relccsd import mccres_read
+    from oracle.symtab import SymLayout
+    with tempfile.TemporaryDirectory() as d:
+        no_l, nv_l = program_files(d, nkr_irrep, nocc_irrep, seed, keep, scale)
+        r = refrun.run_program(d, nocc_irrep, maxit, maxdim, ntol, nfroz=nfroz, keep_dir=d)
+        lay = SymLayout(list(no_l), list(nv_l))
+        t1, t2, st, en = mccres_read(os.path.join(d, "MCCRES"), lay.size("T1"), lay.size("T2"))
+    np.savez_compressed(
+        os.path.join(HERE, f"prog_{name}.npz"),
+        params=np.array([seed, maxit, maxdim, ntol], dtype=np.int64), keep=keep, scale=scale,
+        nkr=np.array(nkr_irrep), nelec=np.array(nocc_irrep),
+        nfroz=np.array(nfroz if nfroz is not None else []), sym_no=np.array(no_l),
+        sym_nv=np.array(nv_l), escf=r["escf"], emp2=r["emp2"], eccsd=r["eccsd"],
+        iterations=np.asarray(r["iterations"], dtype=np.float64), T1=t1, T2=t2, IMSTAT=st,
+        energies=en)
+    print(name, r["escf"], r["emp2"], r["eccsd"], len(r["iterations"]))
+
+
 def main():
     if not refrun.available():
         raise SystemExit("build oracle/_ref first: make -C oracle -f Makefile.ref")
+    if "--program-only" in sys.argv:
+        for case in PROGRAM_CASES:
+            program_case(*case)
+        return
     only_sym = "--sym-only" in sys.argv
     only_lam = "--lambda-only" in sys.argv
     if "--terms-only" in sys.argv:
@@ -600,6 +646,8 @@ def main():
         frozen_ccsd_case(*case)
     for case in FROZEN_TERM_CASES:
         frozen_terms_case(*case)
+    for case in PROGRAM_CASES:
+        program_case(*case)
 
 
 if __name__ == "__main__":

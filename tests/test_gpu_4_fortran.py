@@ -1,5 +1,14 @@
-"""The Fortran driver (ISO_C_BINDING over the C ABI, molfdir_amd/fortran)
-reproduces the relccsd reference trajectory on the GPU."""
+"""The Fortran side on the GPU.
+
+* The Fortran driver (ISO_C_BINDING over the C ABI, molfdir_amd/fortran/
+  ccdrv_hip.f90) reproduces the relccsd reference trajectory.
+* relccsd's own driver as the caller: the relccsd objects built from the
+  reference sources with molfdir_amd/fortran/cceqn_hip.F linked in place of
+  cceqns.F's MP2EQN and CCEQN (oracle/_ref/cchrns_gpu.x, whose CCENER runs
+  MP2 + CCSD; oracle/_ref/relccsd_gpu.x, the whole PROGRAM RELCCSD with its
+  own integral sort of MOLFDIR files) print the CPU reference's MP2 energy,
+  iteration table and CCSD energy and leave its amplitudes and MCCRES
+  records."""
 import os
 import subprocess
 
@@ -14,12 +23,23 @@ pytestmark = pytest.mark.gpu
 
 EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                    "molfdir_amd", "fortran", "ccdrv_hip.x")
+CASES = [("ccsd", "cfg1_6o40v"), ("ccsd", "conv_5o16v"), ("symccsd", "s4_5o12v"),
+         ("symccsd", "s8_8o24v"), ("frzccsd", "c1_6o40v"), ("frzccsd", "s4_7o16v")]
+
+
+def check_rows(rows, ref):
+    assert len(rows) == len(ref)
+    for a, b in zip(rows[1:], ref[1:]):
+        assert int(a[0]) == int(b[0])
+        assert abs(a[1] - b[1]) < 1e-10, (a, b)
+        assert abs(a[2] - b[2]) < 1e-9 * max(1.0, b[2]), (a, b)
+        # T1-DIAGN: both tables print it F10.5; one unit of the last digit
+        # covers a value that sits on a rounding boundary
+        assert abs(a[3] - b[3]) <= 1.0000001e-5, (a, b)
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
-@pytest.mark.parametrize("kind,name", [("ccsd", "cfg1_6o40v"), ("ccsd", "conv_5o16v"),
-                                       ("symccsd", "s4_5o12v"), ("symccsd", "s8_8o24v"),
-                                       ("frzccsd", "c1_6o40v"), ("frzccsd", "s4_7o16v")])
+@pytest.mark.parametrize("kind,name", CASES)
 def test_fortran_driver_matches_reference(kind, name, tmp_path):
     g = load(kind, name)
     no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
@@ -27,16 +47,7 @@ def test_fortran_driver_matches_reference(kind, name, tmp_path):
     refrun._write_input(str(tmp_path / "ccin.bin"), s, 1, maxit, maxdim, ntol)
     p = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
-    rows = refrun.parse_iterations(p.stdout)
-    ref = g["iterations"]
-    assert len(rows) == len(ref)
-    for a, b in zip(rows[1:], ref[1:]):
-        assert int(a[0]) == int(b[0])
-        assert abs(a[1] - b[1]) < 1e-10
-        assert abs(a[2] - b[2]) < 1e-9 * max(1.0, b[2])
-        # T1-DIAGN: both tables print it F10.5; one unit of the last digit
-        # covers a value that sits on a rounding boundary
-        assert abs(a[3] - b[3]) <= 1.0000001e-5, (a, b)
+    check_rows(refrun.parse_iterations(p.stdout), g["iterations"])
     raw = np.fromfile(tmp_path / "ccout.bin", dtype=np.float64)
     z = raw[3:].view(np.complex128)
     nt1 = g["T1"].size
@@ -132,3 +143,55 @@ def test_fortran_driver_mp2_relaxed_property_matches_reference(tmp_path):
     assert abs(complex(out[4], out[5]) - 2 * scf) < 1e-13
     assert abs(complex(out[6], out[7]) - 2 * mp2) < 1e-11
     assert out[-1] < 10.0 ** -ntol
+
+
+@pytest.mark.skipif(not os.path.exists(refrun.EXE_GPU), reason="GPU-linked relccsd not built")
+@pytest.mark.parametrize("kind,name", CASES)
+def test_relccsd_ccener_drives_the_gpu_solver(kind, name, tmp_path):
+    """relccsd's CCENER (ccdriv.F:1715-1792) calling the GPU MP2EQN / CCEQN."""
+    g = load(kind, name)
+    no, nv, seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    print(f"CCENER -> GPU solver: {kind} {name}", flush=True)
+    r = refrun.run(system_for(g), "ccsd", maxit=maxit, maxdim=maxdim, ntol=ntol,
+                   exe=refrun.EXE_GPU, workdir=str(tmp_path), timeout=300)
+    assert "Timing of routine CCEQN (GPU solver" in r["log"]
+    check_rows(r["iterations"], g["iterations"])
+    assert abs(r["emp2"] - float(g["emp2"])) < 1e-10
+    assert abs(r["eccsd"] - float(g["eccsd"])) < 1e-10
+    assert r["nit"] == int(g["nit"])
+    # what CCENER's PUTAMPT wrote to MCCRES after CCEQN, read back by GETAMPT
+    assert np.abs(r["T1"] - g["T1"]).max() < 1e-9
+    assert np.abs(r["T2"] - g["T2"]).max() < 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists(refrun.PROGRAM_GPU), reason="GPU-linked relccsd not built")
+@pytest.mark.parametrize("name", names("prog"))
+def test_relccsd_program_with_the_gpu_solver(name, tmp_path):
+    """PROGRAM RELCCSD: its own integral sort of MOLFDIR files (CCSETI ->
+    RDINTS), then CCENER's MP2 and CCSD on the GPU solver."""
+    from molfdir_amd import mdcint
+    from molfdir_amd.relccsd import mccres_read
+    g = load("prog", name)
+    seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    nkr, nelec = g["nkr"].tolist(), g["nelec"].tolist()
+    nfroz = g["nfroz"].tolist() or None
+    print(f"RELCCSD -> GPU solver: {name}", flush=True)
+    if len(nkr) == 1:
+        mdcint.synthetic_files(str(tmp_path), nkr[0], nelec[0], seed, float(g["keep"]),
+                               scale=float(g["scale"]))
+    else:
+        mdcint.synthetic_files_sym(str(tmp_path), nkr, nelec, seed, float(g["keep"]),
+                                   scale=float(g["scale"]))
+    r = refrun.run_program(str(tmp_path), nelec, maxit, maxdim, ntol, nfroz=nfroz,
+                           exe=refrun.PROGRAM_GPU, timeout=300, keep_dir=str(tmp_path))
+    assert "Timing of routine CCEQN (GPU solver" in r["log"]
+    # the sort is relccsd's own in both runs
+    assert abs(r["escf"] - float(g["escf"])) < 1e-12
+    assert abs(r["emp2"] - float(g["emp2"])) < 1e-10
+    assert abs(r["eccsd"] - float(g["eccsd"])) < 1e-10
+    check_rows(r["iterations"], g["iterations"])
+    t1, t2, st, en = mccres_read(tmp_path / "MCCRES", g["T1"].size, g["T2"].size)
+    assert np.abs(t1 - g["T1"]).max() < 1e-9
+    assert np.abs(t2 - g["T2"]).max() < 1e-9
+    assert np.array_equal(st, g["IMSTAT"])
+    assert np.abs(en - g["energies"]).max() < 1e-10

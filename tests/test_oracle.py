@@ -7,11 +7,14 @@
 * against closed forms (FLIN vs a library solve, MP2 energy, packed-pair
   enumeration), and by planting mistakes that the fixtures must catch.
 """
+import os
+
 import numpy as np
 import pytest
 
 from molfdir_amd import synth
 from oracle import ccsd_oracle as O
+from oracle import refrun
 from oracle import diis as D
 from oracle import textbook as TB
 
@@ -340,3 +343,25 @@ def test_product_block_conversion_matches_symtab(sym):
         want = lay.to_sym(n, s[n].numpy())
         assert np.array_equal(synth.to_blocks(n, s[n], sym).numpy(), want), n
         assert np.array_equal(synth.to_blocks(n, s[n].numpy(), sym), want), n
+
+
+@pytest.mark.skipif(not os.path.exists(refrun.PROGRAM), reason="relccsd program not built")
+@pytest.mark.parametrize("name", names("prog"))
+def test_relccsd_program_fixtures_reproduce(name, tmp_path):
+    """The committed PROGRAM RELCCSD runs (tests/golden/prog_*.npz) come back
+    from the CPU program on regenerated MOLFDIR files: the generator and
+    the runner the GPU test uses are the ones that made them."""
+    from molfdir_amd import mdcint
+    g = load("prog", name)
+    seed, maxit, maxdim, ntol = (int(x) for x in g["params"])
+    nkr, nelec = g["nkr"].tolist(), g["nelec"].tolist()
+    if len(nkr) == 1:
+        mdcint.synthetic_files(str(tmp_path), nkr[0], nelec[0], seed, float(g["keep"]),
+                               scale=float(g["scale"]))
+    else:
+        mdcint.synthetic_files_sym(str(tmp_path), nkr, nelec, seed, float(g["keep"]),
+                                   scale=float(g["scale"]))
+    r = refrun.run_program(str(tmp_path), nelec, maxit, maxdim, ntol,
+                           nfroz=g["nfroz"].tolist() or None, timeout=300)
+    assert r["escf"] == float(g["escf"]) and r["eccsd"] == float(g["eccsd"])
+    assert np.array_equal(np.asarray(r["iterations"]), g["iterations"])
