@@ -161,10 +161,10 @@ struct BatchProgram;
 int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out,
                   std::string& err);
 int batch_run(const BatchProgram* p, hipStream_t st);
-// times the 64 x 64 and 32 x 32 tiles of every grouped product, keeps the faster
+// times the 96 x 64 and 32 x 32 tiles of every grouped product, keeps the faster
 int batch_tune(BatchProgram* p, hipStream_t st);
 // positions (launch sites) of the program, grouped products among them, and
-// how many of those use each tile variant (64 x 64, 32 x 32, 32 x 32 persistent)
+// how many of those use each tile variant (96 x 64, 32 x 32, 32 x 32 persistent)
 void batch_info(const BatchProgram* p, int* positions, int* grouped, int* variants);
 void batch_free(BatchProgram* p);
 

@@ -1121,7 +1121,7 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
     return 0;
 }
 
-// Tile variants of a grouped product: 0 the 64 x 64 tile, 1 the 32 x 32 tile
+// Tile variants of a grouped product: 0 the 96 x 64 tile, 1 the 32 x 32 tile
 // (one workgroup per tile), 2 the 32 x 32 tile on a persistent grid of 1024
 // workgroups (four per CU) striding over the tiles.
 constexpr int TILE_VARIANTS = 3;

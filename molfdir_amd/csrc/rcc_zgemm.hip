@@ -240,7 +240,7 @@ zgemm_mfma_kernel(int m, int n, int k,
 
 // Small-tile grouped GEMM: a 32 x 32 C tile per workgroup (four waves, one
 // 16 x 16 MFMA block each), one LDS stage.  For groups much smaller than the
-// 64 x 64 tile (the per-irrep blocks of small systems: NOOT, o or v by
+// 96 x 64 tile (the per-irrep blocks of small systems: NOOT, o or v by
 // irrep), where the large tile would mostly compute padding.  Tiles of a
 // group are numbered from tile_off as 32 x 32 tiles (zgroup_offsets_small).
 constexpr int SB = 32;

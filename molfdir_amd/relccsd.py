@@ -433,7 +433,7 @@ class RelCCSD:
     @staticmethod
     def batch_plan_info(solvers):
         """The grouped batch plan of these solvers: compiled, launch positions,
-        grouped products, and their tile variants (64, 32, 32 persistent)."""
+        grouped products, and their tile variants (96 x 64, 32 x 32, 32 x 32 persistent)."""
         n = len(solvers)
         lib = solvers[0].lib
         hs = (ctypes.c_void_p * n)(*[s.h.value for s in solvers])
@@ -442,7 +442,7 @@ class RelCCSD:
             raise RccError(lib.rcc_last_error(solvers[0].h).decode())
         return {"compiled": bool(info[0]), "launch_positions": info[1],
                 "grouped_products": info[2],
-                "tile_variants": {"64x64": info[3], "32x32": info[4], "32x32_persistent": info[5]}}
+                "tile_variants": {"96x64": info[3], "32x32": info[4], "32x32_persistent": info[5]}}
 
     def cceqn(self):
         """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
