@@ -14,16 +14,33 @@
 //
 // relccsd stores every expanded value by assignment while it reads the file,
 // so when two expansions land on one element the later one in file order
-// is kept.  The device reproduces that exactly: a claim pass takes, per
-// element and part, the maximum of a 64-bit order key (canonical index x 16
-// + expansion index) with atomicMax, and only the holder of that key writes.
-// The writes come in two ordered passes with plain stores: the direct pass
-// stores d into the zeroed class, and the exchange pass, launched after it,
-// replaces the element by fl(d - e) -- RDINTS's sequential stores followed
-// by its direct - exchange XAXPY.
+// is kept.  Two expansions land on one (element, part) only if they are the
+// same index quadruple, and the 16 expansions of a list entry are its whole
+// orbit under the pair time reversals, particle interchange and conjugation
+// -- the same set of quadruples for every entry of that orbit.  So the last
+// entry of each orbit in file order writes every element its orbit reaches,
+// and within that entry the last expansion wins.  The device finds those
+// entries without ordering the images: a claim pass takes, per orbit, the
+// largest entry index (one 32-bit atomicMax per entry).  The orbit's key is
+// its canonical member's index: every orbit of Kramers tuples (a, b | c, d)
+// under the 16 operations holds tuples with a > 0, |b| <= a, 0 < |c| <= a,
+// |d| <= |c|, and the smallest enumeration index among them names the orbit.
+// The keys of one a (the largest Kramers index of the orbit, constant on
+// it) fill 4 a^2 (a + 1) consecutive words, so a list written in IKR order
+// walks them through the caches.  Where that array would exceed
+// SORT_CANON_MAX bytes the key is the orbit's anchor instead -- the smallest
+// (class, part, element) among its stored expansions -- kept in a word of
+// the still-zero class element, and a flag pass marks the holders and
+// clears the keys before any class write.  Only the holders write, each
+// element and part then from one thread, in two ordered passes of plain
+// stores: the direct pass stores d into the zeroed class, and the exchange
+// pass, launched after it, replaces the element by fl(d - e) -- RDINTS's
+// sequential stores followed by its direct - exchange XAXPY.
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <string>
@@ -245,8 +262,64 @@ struct SortDev {
     int itrsgn;                      // GET2INTS's ITRSGN: 1 Coulomb, -1 Breit
     int64_t rows[6];
     double2* cls[6];
-    unsigned long long* key[2][6];   // [direct / exchange][class]
+    unsigned* ckey;                  // orbit keys by canonical index, or null (anchor keys)
+    int all_last;                    // every entry is the last of its orbit (no duplicates)
 };
+
+// canonical orbit keys only up to this many bytes of key array
+constexpr int64_t SORT_CANON_MAX = 4ll << 30;
+
+// number of canonical tuples with largest index below a: sum_{x<a} 4 x^2 (x+1)
+__host__ __device__ __forceinline__ int64_t canon_base(int64_t a) {
+    const int64_t n = a - 1;
+    const int64_t s3 = (n * (n + 1) / 2) * (n * (n + 1) / 2);
+    const int64_t s2 = n * (n + 1) * (2 * n + 1) / 6;
+    return 4 * (s3 + s2);
+}
+
+// enumeration index of a canonical tuple: a, then b (2a values), then c in
+// the order 1, -1, 2, -2, .., then d (2|c| values), negative before positive
+__device__ __forceinline__ int64_t canon_index(int a, int b, int c, int d) {
+    const int ac = c < 0 ? -c : c;
+    const int bi = b < 0 ? b + a : b + a - 1;
+    const int ci = 2 * ac * (ac - 1) + (c > 0 ? 0 : 2 * ac);
+    const int di = d < 0 ? d + ac : d + ac - 1;
+    return canon_base(a) + (int64_t)bi * 2 * a * (a + 1) + ci + di;
+}
+
+__device__ __forceinline__ bool canon_form(int a, int b, int c, int d) {
+    const int ab = b < 0 ? -b : b, ac = c < 0 ? -c : c, ad = d < 0 ? -d : d;
+    return a > 0 && ab <= a && ac <= a && ad <= ac;
+}
+
+// the orbit key of a list entry: the smallest canonical index among its 16
+// tuples, in the order of GET2INTS's loops (ccints.F:936-1021)
+__device__ __forceinline__ int64_t orbit_key(int4 q) {
+    int64_t best = INT64_MAX;
+#pragma unroll
+    for (int t1 = 0; t1 < 2; ++t1) {
+        const int i0 = t1 ? -q.y : q.x, j0 = t1 ? -q.x : q.y;
+#pragma unroll
+        for (int t2 = 0; t2 < 2; ++t2) {
+            const int k0 = t2 ? -q.w : q.z, l0 = t2 ? -q.z : q.w;
+#pragma unroll
+            for (int pi = 0; pi < 2; ++pi) {
+                const int i1 = pi ? k0 : i0, j1 = pi ? l0 : j0;
+                const int k1 = pi ? i0 : k0, l1 = pi ? j0 : l0;
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const int a = cc ? j1 : i1, b = cc ? i1 : j1;
+                    const int c = cc ? l1 : k1, d = cc ? k1 : l1;
+                    if (canon_form(a, b, c, d)) {
+                        const int64_t x = canon_index(a, b, c, d);
+                        best = x < best ? x : best;
+                    }
+                }
+            }
+        }
+    }
+    return best;
+}
 
 // MKINDEX for 0-based spinors s: spos[s] >= 0 occupied position, else
 // virtual position -spos - 1
@@ -292,13 +365,17 @@ __device__ __forceinline__ void expand16(int4 q, const int* __restrict__ kr, int
     int i0 = kr[nkr + IK], j0 = kr[nkr + JK];
     int ijsg = 1;
     int v = 0;
+#pragma unroll
     for (int t1 = 0; t1 < 2; ++t1) {
         int k0 = kr[nkr + KK], l0 = kr[nkr + LK];
         int klsg = 1;
+#pragma unroll
         for (int t2 = 0; t2 < 2; ++t2) {
             int i1 = i0, j1 = j0, k1 = k0, l1 = l0;
+#pragma unroll
             for (int pi = 0; pi < 2; ++pi) {
                 int i = i1, j = j1, k = k1, l = l1;
+#pragma unroll
                 for (int cc = 0; cc < 2; ++cc, ++v) {
                     const int sj = spos[j], sl = spos[l];
                     const int64_t jl = pindex(sj, sl, d.no, d.nv);
@@ -348,56 +425,204 @@ __device__ __forceinline__ bool valid(int4 q, int nkr) {
     return in_range(q.x, nkr) && in_range(q.y, nkr) && in_range(q.z, nkr) && in_range(q.w, nkr);
 }
 
-// entries with a Kramers index outside +-NKR are skipped and flagged
-__global__ void sort_claim_kernel(SortDev d, const int4* __restrict__ idx, int64_t m, int64_t n0,
+// An expansion as one 64-bit code: class (bits 61-63), part (bit 60: 0
+// direct, 1 exchange), element (bits 0-59); NOIMG for one that is not stored.
+constexpr unsigned long long NOIMG = ~0ull;
+
+__device__ __forceinline__ unsigned long long img_code(int c, int part, int64_t e) {
+    return ((unsigned long long)c << 61) | ((unsigned long long)part << 60) |
+           (unsigned long long)e;
+}
+
+struct Images {
+    unsigned long long code[16];
+    signed char sre[16], sim[16];
+};
+
+// the 16 expansions of an entry in GET2INTS's order (registers: every index
+// is a constant once the expansion loops are unrolled)
+__device__ __forceinline__ void images16(int4 q, const int* __restrict__ kr, int nkr,
+                                         const int* __restrict__ spos, const SortDev& d,
+                                         Images& im) {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        im.code[v] = NOIMG;
+        im.sre[v] = 1;
+        im.sim[v] = 1;
+    }
+    expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int sre, int sim) {
+        im.code[v] = img_code(c, part, e);
+        im.sre[v] = (signed char)sre;
+        im.sim[v] = (signed char)sim;
+    });
+}
+
+__device__ __forceinline__ unsigned long long anchor_of(const Images& im) {
+    unsigned long long a = NOIMG;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) a = im.code[v] < a ? im.code[v] : a;
+    return a;
+}
+
+// the 32-bit key of an anchor: word `part` of its (zero until the writes)
+// class element
+__device__ __forceinline__ unsigned* anchor_key(const SortDev& d, unsigned long long a) {
+    const int c = (int)(a >> 61), part = (int)((a >> 60) & 1ull);
+    const int64_t e = (int64_t)(a & ((1ull << 60) - 1ull));
+    return reinterpret_cast<unsigned*>(d.cls[c] + e) + part;
+}
+
+// Claim: per anchor, the largest 1-based entry index.  Entries with a
+// Kramers index outside +-NKR are skipped and flagged.
+__global__ void sort_claim_kernel(SortDev d, const int4* __restrict__ idx, int64_t m,
                                   const int* __restrict__ kr, int nkr,
                                   const int* __restrict__ spos, int* __restrict__ bad) {
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
          n += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long base = (unsigned long long)(n0 + n) * 16ull + 1ull;
         const int4 q = idx[n];
         if (!valid(q, nkr)) {
             atomicOr(bad, 1);
             continue;
         }
-        expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int, int) {
-            atomicMax(d.key[part][c] + e, base + (unsigned long long)v);
-        });
+        if (d.ckey) {
+            const int64_t k = orbit_key(q);
+            if (k == INT64_MAX) atomicOr(bad, 2);      // no canonical member (never seen)
+            else atomicMax(d.ckey + k, (unsigned)(n + 1));
+            continue;
+        }
+        Images im;
+        images16(q, kr, nkr, spos, d, im);
+        const unsigned long long a = anchor_of(im);
+        if (a != NOIMG) atomicMax(anchor_key(d, a), (unsigned)(n + 1));
     }
 }
 
-// The key holder of each (class element, part) writes its value: the direct
-// pass (which = 0) stores d into the zeroed class, the exchange pass (which =
-// 1, launched after it) replaces the element by fl(d - e), RDINTS's XAXPY of
-// the exchange part (ccints.F:221-225).  One holder per element and part:
-// plain stores, no atomics.
-__global__ void sort_write_kernel(SortDev d, const int4* __restrict__ idx,
-                                  const double2* __restrict__ val, int64_t m, int64_t n0,
-                                  const int* __restrict__ kr, int nkr,
-                                  const int* __restrict__ spos, int which) {
+// Canonical keys, first pass: every entry stores its index into its orbit's
+// key (plain stores: with two entries of one orbit either may stay).  When
+// as many keys are set as there are entries, no orbit has two entries and
+// every entry writes; otherwise the claim pass (atomicMax) settles the
+// last one.
+__global__ void sort_mark_kernel(SortDev d, const int4* __restrict__ idx, int64_t m, int nkr,
+                                 int* __restrict__ bad) {
     for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
          n += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long base = (unsigned long long)(n0 + n) * 16ull + 1ull;
         const int4 q = idx[n];
-        if (!valid(q, nkr)) continue;
-        const double2 x = val[n];
-        expand16(q, kr, nkr, spos, d, [&](int v, int c, int part, int64_t e, int sre, int sim) {
-            if (part != which || d.key[part][c][e] != base + (unsigned long long)v) return;
-            double2* p = d.cls[c] + e;
-            if (which == 0) {
-                *p = make_double2(sre * x.x, sim * x.y);
-            } else {
-                const double2 o = *p;
-                *p = make_double2(o.x - sre * x.x, o.y - sim * x.y);
+        if (!valid(q, nkr)) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        const int64_t k = orbit_key(q);
+        if (k == INT64_MAX) atomicOr(bad, 2);      // no canonical member (never seen)
+        else d.ckey[k] = (unsigned)(n + 1);
+    }
+}
+
+// number of set keys: per-wave sums by lane shuffles, one atomic per wave
+__global__ void count_set_kernel(const unsigned* __restrict__ p, int64_t n,
+                                 unsigned long long* __restrict__ out) {
+    unsigned long long c = 0;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x)
+        c += p[x] != 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// whether entry n writes: every valid entry when no orbit has two, else the
+// holder of its orbit's key (canonical keys) or flagged by the flag pass
+// (anchor keys)
+__device__ __forceinline__ bool holds(const SortDev& d, int4 q, int64_t n, int nkr,
+                                      const unsigned char* __restrict__ last) {
+    if (d.all_last) return valid(q, nkr);
+    if (d.ckey) {
+        if (!valid(q, nkr)) return false;
+        const int64_t k = orbit_key(q);
+        return k != INT64_MAX && d.ckey[k] == (unsigned)(n + 1);
+    }
+    return last[n] != 0;
+}
+
+// Flags the last entry of every orbit (the holder of its anchor's key) and
+// clears the key, so the classes are zero again; an entry that is not the
+// holder reads either the holder's index or 0, neither its own.
+__global__ void sort_flag_kernel(SortDev d, const int4* __restrict__ idx, int64_t m,
+                                 const int* __restrict__ kr, int nkr,
+                                 const int* __restrict__ spos,
+                                 unsigned char* __restrict__ last) {
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = idx[n];
+        unsigned char w = 0;
+        if (valid(q, nkr)) {
+            Images im;
+            images16(q, kr, nkr, spos, d, im);
+            const unsigned long long a = anchor_of(im);
+            if (a != NOIMG) {
+                unsigned* key = anchor_key(d, a);
+                if (__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    (unsigned)(n + 1)) {
+                    w = 1;
+                    __hip_atomic_store(key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
+        }
+        last[n] = w;
+    }
+}
+
+// The direct pass: every flagged entry stores its direct expansions, in
+// expansion order (a later one on the same element overwrites).
+__global__ void sort_direct_kernel(SortDev d, const int4* __restrict__ idx,
+                                   const double2* __restrict__ val, int64_t m,
+                                   const int* __restrict__ kr, int nkr,
+                                   const int* __restrict__ spos,
+                                   const unsigned char* __restrict__ last) {
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = idx[n];
+        if (!holds(d, q, n, nkr, last)) continue;
+        const double2 x = val[n];
+        expand16(q, kr, nkr, spos, d, [&](int, int c, int part, int64_t e, int sre, int sim) {
+            if (part == 0) d.cls[c][e] = make_double2(sre * x.x, sim * x.y);
         });
     }
 }
 
-__global__ void fill_u64_kernel(unsigned long long* p, int64_t n) {
+// The exchange pass, after the direct one: every flagged entry replaces each
+// element its exchange expansions reach by fl(d - e), with e the last of its
+// expansions on that element.
+__global__ void sort_exchange_kernel(SortDev d, const int4* __restrict__ idx,
+                                     const double2* __restrict__ val, int64_t m,
+                                     const int* __restrict__ kr, int nkr,
+                                     const int* __restrict__ spos,
+                                     const unsigned char* __restrict__ last) {
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < m;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = idx[n];
+        if (!holds(d, q, n, nkr, last)) continue;
+        const double2 x = val[n];
+        Images im;
+        images16(q, kr, nkr, spos, d, im);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const unsigned long long cv = im.code[v];
+            if (cv == NOIMG || ((cv >> 60) & 1ull) == 0) continue;
+            bool later = false;
+#pragma unroll
+            for (int w = v + 1; w < 16; ++w) later = later || im.code[w] == cv;
+            if (later) continue;
+            double2* p = d.cls[(int)(cv >> 61)] + (int64_t)(cv & ((1ull << 60) - 1ull));
+            const double2 o = *p;
+            *p = make_double2(o.x - im.sre[v] * x.x, o.y - im.sim[v] * x.y);
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(unsigned* p, int64_t n) {
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
          x += (int64_t)gridDim.x * blockDim.x)
-        p[x] = 0ull;
+        p[x] = 0u;
 }
 
 __global__ void fill_z_kernel(double2* p, int64_t n) {
@@ -521,35 +746,56 @@ int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos
                   nvo = (int64_t)no * nv;
     const int64_t rows[6] = {noot, nvo, nvvt, nvo, nvo, nvvt};
     const int64_t cols[6] = {noot, noot, noot, nvo, nvvt, nvvt};
-    int64_t nkeys = 0;
     for (int c = 0; c < 6; ++c) {
         d.rows[c] = rows[c];
         d.cls[c] = cls[c];
-        nkeys += 2 * rows[c] * cols[c];
     }
-    unsigned long long* keys = nullptr;
-    int* bad = nullptr;
-    FECK(hipMalloc((void**)&keys, (size_t)std::max<int64_t>(nkeys, 1) * 8));
-    FECK(hipMalloc((void**)&bad, sizeof(int)));
-    FECK(hipMemsetAsync(bad, 0, sizeof(int), st));
-    int64_t off = 0;
-    for (int part = 0; part < 2; ++part)
-        for (int c = 0; c < 6; ++c) {
-            d.key[part][c] = keys + off;
-            off += rows[c] * cols[c];
+    if (m >= 0xffffffffll) {
+        err = "integral sort: more than 2^32 - 2 canonical integrals";
+        return -1;
+    }
+    // canonical orbit keys when the array is small enough (and allocatable);
+    // RCC_SORT_KEYS=anchor in the environment takes the anchor keys always
+    const int64_t nckey = canon_base((int64_t)nkr + 1);
+    const char* ek = std::getenv("RCC_SORT_KEYS");
+    const bool anchor_only = ek && std::strcmp(ek, "anchor") == 0;
+    d.ckey = nullptr;
+    if (!anchor_only && nckey * 4 <= SORT_CANON_MAX) {
+        if (hipMalloc((void**)&d.ckey, (size_t)std::max<int64_t>(nckey, 1) * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            d.ckey = nullptr;
         }
+    }
+    d.all_last = 0;
+    unsigned char* last = nullptr;
+    // bad flags (int) and the set-key count (u64) in one small buffer
+    struct SortStat { int bad; int pad; unsigned long long set; };
+    SortStat* stat = nullptr;
+    if (!d.ckey) FECK(hipMalloc((void**)&last, (size_t)std::max<int64_t>(m, 1)));
+    FECK(hipMalloc((void**)&stat, sizeof(SortStat)));
+    FECK(hipMemsetAsync(stat, 0, sizeof(SortStat), st));
+    int* bad = &stat->bad;
     hipEvent_t e0, e1;
     FECK(hipEventCreate(&e0));
     FECK(hipEventCreate(&e1));
     FECK(hipEventRecord(e0, st));
-    fill_u64_kernel<<<grid_for(nkeys), 256, 0, st>>>(keys, nkeys);
     for (int c = 0; c < 6; ++c)
         fill_z_kernel<<<grid_for(rows[c] * cols[c]), 256, 0, st>>>(cls[c], rows[c] * cols[c]);
+    if (d.ckey) fill_u32_kernel<<<grid_for(nckey), 256, 0, st>>>(d.ckey, nckey);
     const int g = grid_for(m);
-    if (m > 0) {
-        sort_claim_kernel<<<g, 256, 0, st>>>(d, d_idx, m, 0, d_kr, nkr, d_spos, bad);
-        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos, 0);
-        sort_write_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, 0, d_kr, nkr, d_spos, 1);
+    SortStat hs{0, 0, 0};
+    if (m > 0 && d.ckey) {
+        sort_mark_kernel<<<g, 256, 0, st>>>(d, d_idx, m, nkr, bad);
+        count_set_kernel<<<1024, 256, 0, st>>>(d.ckey, nckey, &stat->set);
+        FECK(hipMemcpyAsync(&hs, stat, sizeof(SortStat), hipMemcpyDeviceToHost, st));
+        FECK(hipStreamSynchronize(st));
+        d.all_last = hs.bad == 0 && hs.set == (unsigned long long)m;
+    }
+    if (m > 0 && !hs.bad) {
+        if (!d.all_last) sort_claim_kernel<<<g, 256, 0, st>>>(d, d_idx, m, d_kr, nkr, d_spos, bad);
+        if (!d.ckey) sort_flag_kernel<<<g, 256, 0, st>>>(d, d_idx, m, d_kr, nkr, d_spos, last);
+        sort_direct_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, d_kr, nkr, d_spos, last);
+        sort_exchange_kernel<<<g, 256, 0, st>>>(d, d_idx, d_val, m, d_kr, nkr, d_spos, last);
     }
     FECK(hipGetLastError());
     FECK(hipEventRecord(e1, st));
@@ -561,10 +807,15 @@ int sort_integrals_device(int no, int nv, double2* const* cls, const int* d_spos
     FECK(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    FECK(hipFree(keys));
-    FECK(hipFree(bad));
-    if (hbad) {
+    if (last) FECK(hipFree(last));
+    if (d.ckey) FECK(hipFree(d.ckey));
+    FECK(hipFree(stat));
+    if (hbad & 1) {
         err = "integral sort: a Kramers index of the list is 0 or outside +-NKR (entries skipped)";
+        return -1;
+    }
+    if (hbad) {
+        err = "integral sort: an entry without a canonical form (entries skipped)";
         return -1;
     }
     return 0;

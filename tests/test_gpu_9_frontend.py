@@ -247,3 +247,33 @@ def test_read_molfdir_symmetric_matches_reference_sort(name, tmp_path):
         assert np.abs(r.get(k) - g[k]).max() < 1e-14, k
     assert np.abs(r.get_eps() - g["eps"]).max() < 1e-14
     assert abs(e["e0"] - float(g["E0"])) < 1e-12 and abs(e["e1"] - float(g["E1"])) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["sort", "sortbreit", "symsort"])
+def test_anchor_keyed_sort_matches_reference_sort(kind, tmp_path, monkeypatch):
+    """The sort's second key form (RCC_SORT_KEYS=anchor: each orbit keyed by
+    its smallest stored expansion, in the class memory, with a flag pass),
+    which lists whose canonical key array would pass 4 GiB use: the same
+    classes bit for bit on every fixture of the kind."""
+    from test_oracle_rdints import regenerate_sym
+    monkeypatch.setenv("RCC_SORT_KEYS", "anchor")
+    for name in names(kind):
+        g = load(kind, name)
+        print(f"anchor-keyed sort, {kind} fixture {name}", flush=True)
+        d = tmp_path / name
+        d.mkdir()
+        if kind == "symsort":
+            gen = regenerate_sym(g, d)
+            no_l, nv_l = g["sym_no"].tolist(), g["sym_nv"].tolist()
+            r = RelCCSD(sum(no_l), sum(nv_l), sym=(no_l, nv_l))
+            r.read_molfdir(str(d), gen["nelec"].tolist())
+        else:
+            from molfdir_amd import mdcint
+            nkr, nocc, seed = (int(x) for x in g["params"])
+            mdcint.synthetic_files(str(d), nkr, nocc, seed, float(g["keep"]),
+                                   breit=(kind == "sortbreit"))
+            r = RelCCSD(nocc, 2 * nkr - nocc)
+            r.read_molfdir(str(d), [nocc])
+        for c in CLASSES:
+            assert np.array_equal(r.get(c), g[c]), (name, c)
+        r.close()
