@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(256)
 fock_energy_kernel(int no, int nv, const double2* __restrict__ OOOO,
                    const double2* __restrict__ FOO, const double2* __restrict__ FVV,
                    double* __restrict__ eps, double* __restrict__ e01) {
-    __shared__ double sh[2][256];
+    __shared__ double sh[2][4];
     const int tid = threadIdx.x;
     const int64_t noot = (int64_t)no * (no - 1) / 2;
     double e0 = 0.0, e1 = 0.0;
@@ -707,19 +707,20 @@ fock_energy_kernel(int no, int nv, const double2* __restrict__ OOOO,
     }
     for (int a = tid; a < nv; a += 256) eps[no + a] = FVV[a + (int64_t)a * nv].x;
     for (int64_t p = tid; p < noot; p += 256) e1 -= OOOO[p + p * noot].x;
-    sh[0][tid] = e0;
-    sh[1][tid] = e1;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-            sh[0][tid] += sh[0][tid + s];
-            sh[1][tid] += sh[1][tid + s];
-        }
-        __syncthreads();
+    // wave sums by shuffles (fixed order), then the four waves in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        e0 += __shfl_down(e0, off, 64);
+        e1 += __shfl_down(e1, off, 64);
     }
+    if ((tid & 63) == 0) {
+        sh[0][tid >> 6] = e0;
+        sh[1][tid >> 6] = e1;
+    }
+    __syncthreads();
     if (tid == 0) {
-        e01[0] = sh[0][0];
-        e01[1] = sh[1][0];
+        e01[0] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        e01[1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     }
 }
 

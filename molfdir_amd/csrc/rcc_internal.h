@@ -332,4 +332,29 @@ int64_t i8_tuned_shapes();
 int launch_i8_wide(int S, int gm, const I8Operand& A, const I8Operand& B, double2* C, int64_t ldc,
                    double2 alpha, double2 beta, hipStream_t st);
 
+// ---- K-split reduction (shared by zgemm_reduce_kernel and the batched
+// split reduction, so the two give the same bits) ----
+// Lanes per output: lane q of an output's group adds the splits q, q + ways,
+// .. in order, then the group combines by a fixed shuffle butterfly.
+__host__ __device__ inline int split_ways(int nsplit) {
+    return nsplit >= 128 ? 16 : nsplit >= 32 ? 8 : nsplit >= 8 ? 4 : 1;
+}
+
+// sum over the nsplit partials w[s * stride] of one output; every lane of
+// the group calls it, lane q == 0 gets the sum
+__device__ inline double2 split_sum(const double2* __restrict__ w, int nsplit, int64_t stride,
+                                    int q, int ways) {
+    double sr = 0.0, si = 0.0;
+    for (int s = q; s < nsplit; s += ways) {
+        const double2 v = w[(int64_t)s * stride];
+        sr += v.x;
+        si += v.y;
+    }
+    for (int off = ways >> 1; off > 0; off >>= 1) {
+        sr += __shfl_down(sr, off, ways);
+        si += __shfl_down(si, off, ways);
+    }
+    return make_double2(sr, si);
+}
+
 }  // namespace rcc

@@ -50,6 +50,19 @@ __device__ __forceinline__ double2 cscale(double s, double2 a) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
          i += (int64_t)gridDim.x * blockDim.x)
 
+// 2-D element loops over a column-major array: columns over blockIdx.y,
+// rows (contiguous in memory) over blockIdx.x.  The row index is a 32-bit
+// value and a column's own indices are decoded once per column, so the
+// bodies do no 64-bit division per element.
+#define COL_LOOP(c, ncol) for (int64_t c = blockIdx.y; c < (ncol); c += gridDim.y)
+#define ROW_LOOP(r, nrow)                                                     \
+    for (int r = (int)(blockIdx.x * blockDim.x + threadIdx.x); r < (int)(nrow); \
+         r += (int)(gridDim.x * blockDim.x))
+// y-grid of a 2-D launch: the columns, at most 2048 (the loop strides)
+static inline int col_grid(int64_t ncol) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ncol, 2048));
+}
+
 // ---------------------------------------------------------------- launching
 // Every element-wise kernel of the term sequence is a parameter struct P and
 // a device body(const P&) with a grid-stride loop over blockIdx.x; ew_one
@@ -86,8 +99,8 @@ struct GettauP {
     const int2* vv; int64_t nvvt; const int2* oo; int64_t noot; int nv;
 };
 __device__ __forceinline__ void body(const GettauP& p) {
-    GRID_LOOP(e, p.nvvt * p.noot) {
-        const int64_t ab = e % p.nvvt, ij = e / p.nvvt;
+    COL_LOOP(ij, p.noot) ROW_LOOP(ab, p.nvvt) {
+        const int64_t e = ab + p.nvvt * ij;
         const int2 q1 = p.vv[ab], q2 = p.oo[ij];
         const int a = q1.x, b = q1.y, i = q2.x, j = q2.y, nv = p.nv;
         double2 v = cscale(p.s2, p.t2[e]);
@@ -100,47 +113,46 @@ __device__ __forceinline__ void body(const GettauP& p) {
 void k_gettau(const double2* t1, const double2* t2, double2* tau, double s2, PairTab vv,
               PairTab oo, int nv, hipStream_t st) {
     const GettauP p{t1, t2, tau, s2, vv.tab, vv.npair, oo.tab, oo.npair, nv};
-    ew_launch(RK_GETTAU, p, vv.npair * oo.npair, 1, st);
+    ew_launch(RK_GETTAU, p, vv.npair, col_grid(oo.npair), st);
 }
 
 // ---------------------------------------------------------------- expansions
 struct ExpandE2P { const double2* P; double2* out; int64_t R; int no; };
 __device__ __forceinline__ void body(const ExpandE2P& p) {
     const int no = p.no;
-    GRID_LOOP(o, p.R * no * no) {
-        const int64_t r = o % p.R;
-        const int64_t q = o / p.R;
+    COL_LOOP(q, (int64_t)no * no) {
         const int y = (int)(q % no), x = (int)(q / no);
-        double2 v = make_double2(0.0, 0.0);
-        if (x > y) v = p.P[r + p.R * pair_of(x, y, no)];
-        else if (x < y) { v = p.P[r + p.R * pair_of(y, x, no)]; v.x = -v.x; v.y = -v.y; }
-        p.out[o] = v;
+        ROW_LOOP(r, p.R) {
+            double2 v = make_double2(0.0, 0.0);
+            if (x > y) v = p.P[r + p.R * pair_of(x, y, no)];
+            else if (x < y) { v = p.P[r + p.R * pair_of(y, x, no)]; v.x = -v.x; v.y = -v.y; }
+            p.out[r + p.R * q] = v;
+        }
     }
 }
 
 void k_expand_e2(const double2* P, double2* out, int64_t R, PairTab oo, int no, hipStream_t st) {
     (void)oo;
-    ew_launch(RK_EXPAND_E2, ExpandE2P{P, out, R, no}, R * no * no, 1, st);
+    ew_launch(RK_EXPAND_E2, ExpandE2P{P, out, R, no}, R, col_grid((int64_t)no * no), st);
 }
 
 struct ExpandF1P { const double2* P; double2* out; int64_t ncol; int nv; int64_t nvvt; };
 __device__ __forceinline__ void body(const ExpandF1P& p) {
-    const int64_t nv2 = (int64_t)p.nv * p.nv;
-    GRID_LOOP(o, nv2 * p.ncol) {
-        const int a = (int)(o % p.nv);
-        const int d = (int)((o / p.nv) % p.nv);
-        const int64_t col = o / nv2;
+    const int nv = p.nv;
+    const int64_t nv2 = (int64_t)nv * nv;
+    COL_LOOP(col, p.ncol) ROW_LOOP(r, nv2) {
+        const int a = r % nv, d = r / nv;
         double2 v = make_double2(0.0, 0.0);
-        if (a > d) v = p.P[pair_of(a, d, p.nv) + p.nvvt * col];
-        else if (a < d) { v = p.P[pair_of(d, a, p.nv) + p.nvvt * col]; v.x = -v.x; v.y = -v.y; }
-        p.out[o] = v;
+        if (a > d) v = p.P[pair_of(a, d, nv) + p.nvvt * col];
+        else if (a < d) { v = p.P[pair_of(d, a, nv) + p.nvvt * col]; v.x = -v.x; v.y = -v.y; }
+        p.out[r + nv2 * col] = v;
     }
 }
 
 void k_expand_f1(const double2* P, double2* out, int64_t ncol, PairTab vv, int nv,
                  hipStream_t st) {
-    ew_launch(RK_EXPAND_F1, ExpandF1P{P, out, ncol, nv, vv.npair}, (int64_t)nv * nv * ncol, 1,
-              st);
+    ew_launch(RK_EXPAND_F1, ExpandF1P{P, out, ncol, nv, vv.npair}, (int64_t)nv * nv,
+              col_grid(ncol), st);
 }
 
 struct ExpandRingP {
@@ -149,12 +161,10 @@ struct ExpandRingP {
 __device__ __forceinline__ void body(const ExpandRingP& p) {
     const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
-    GRID_LOOP(o, nvo * nvo) {
-        const int a = (int)(o % nv);
-        const int i = (int)((o / nv) % no);
-        const int64_t cc = o / nvo;
-        const int c = (int)(cc % nv);
-        const int k = (int)(cc / nv);
+    COL_LOOP(cc, nvo) ROW_LOOP(r, nvo) {
+        const int c = (int)(cc % nv), k = (int)(cc / nv);
+        const int a = r % nv, i = r / nv;
+        const int64_t o = r + nvo * cc;
         double2 v = make_double2(0.0, 0.0);
         if (a != c && i != k) {
             const int64_t ab = a > c ? pair_of(a, c, nv) : pair_of(c, a, nv);
@@ -172,7 +182,8 @@ void k_expand_ring(const double2* T2, const double2* t1, double s, double2* out,
                    PairTab oo, int nv, int no, hipStream_t st) {
     (void)oo;
     const int64_t nvo = (int64_t)nv * no;
-    ew_launch(RK_EXPAND_RING, ExpandRingP{T2, t1, s, out, nv, no, vv.npair}, nvo * nvo, 1, st);
+    ew_launch(RK_EXPAND_RING, ExpandRingP{T2, t1, s, out, nv, no, vv.npair}, nvo, col_grid(nvo),
+              st);
 }
 
 __global__ void expand_cols_kernel(const double2* __restrict__ P, double2* __restrict__ out,
@@ -201,8 +212,8 @@ void k_expand_cols(const double2* P, double2* out, int64_t R, PairTab vv, int nv
 struct PackPijP { const double2* Y; double2* S; int64_t NR; double alpha; const int2* oo;
                   int64_t noot; int no; };
 __device__ __forceinline__ void body(const PackPijP& p) {
-    GRID_LOOP(e, p.NR * p.noot) {
-        const int64_t r = e % p.NR, ij = e / p.NR;
+    COL_LOOP(ij, p.noot) ROW_LOOP(r, p.NR) {
+        const int64_t e = r + p.NR * ij;
         const int2 q = p.oo[ij];
         const double2 y1 = p.Y[r + p.NR * (q.x + (int64_t)p.no * q.y)];
         const double2 y2 = p.Y[r + p.NR * (q.y + (int64_t)p.no * q.x)];
@@ -215,15 +226,16 @@ __device__ __forceinline__ void body(const PackPijP& p) {
 
 void k_pack_pij(const double2* Y, double2* S, int64_t NR, double alpha, PairTab oo, int no,
                 hipStream_t st) {
-    ew_launch(RK_PACK_PIJ, PackPijP{Y, S, NR, alpha, oo.tab, oo.npair, no}, NR * oo.npair, 1, st);
+    ew_launch(RK_PACK_PIJ, PackPijP{Y, S, NR, alpha, oo.tab, oo.npair, no}, NR,
+              col_grid(oo.npair), st);
 }
 
 struct PackPabP { const double2* Y; double2* S; int64_t ncol; double alpha; const int2* vv;
                   int64_t nvvt; int nv; };
 __device__ __forceinline__ void body(const PackPabP& p) {
     const int64_t nv2 = (int64_t)p.nv * p.nv;
-    GRID_LOOP(e, p.nvvt * p.ncol) {
-        const int64_t ab = e % p.nvvt, col = e / p.nvvt;
+    COL_LOOP(col, p.ncol) ROW_LOOP(ab, p.nvvt) {
+        const int64_t e = ab + p.nvvt * col;
         const int2 q = p.vv[ab];
         const double2 y1 = p.Y[q.x + (int64_t)p.nv * q.y + nv2 * col];
         const double2 y2 = p.Y[q.y + (int64_t)p.nv * q.x + nv2 * col];
@@ -236,16 +248,17 @@ __device__ __forceinline__ void body(const PackPabP& p) {
 
 void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTab vv, int nv,
                 hipStream_t st) {
-    ew_launch(RK_PACK_PAB, PackPabP{Y, S, ncol, alpha, vv.tab, vv.npair, nv}, vv.npair * ncol, 1,
-              st);
+    ew_launch(RK_PACK_PAB, PackPabP{Y, S, ncol, alpha, vv.tab, vv.npair, nv}, vv.npair,
+              col_grid(ncol), st);
 }
 
 struct PackPabijP { const double2* Q; double2* S; const int2* vv; int64_t nvvt; const int2* oo;
                     int64_t noot; int nv; int no; double alpha; };
 __device__ __forceinline__ void body(const PackPabijP& p) {
     const int64_t nvo = (int64_t)p.nv * p.no;
-    GRID_LOOP(e, p.nvvt * p.noot) {
-        const int2 q1 = p.vv[e % p.nvvt], q2 = p.oo[e / p.nvvt];
+    COL_LOOP(ij, p.noot) ROW_LOOP(ab, p.nvvt) {
+        const int64_t e = ab + p.nvvt * ij;
+        const int2 q1 = p.vv[ab], q2 = p.oo[ij];
         const int64_t ai = q1.x + (int64_t)p.nv * q2.x, bj = q1.y + (int64_t)p.nv * q2.y;
         const int64_t bi = q1.y + (int64_t)p.nv * q2.x, aj = q1.x + (int64_t)p.nv * q2.y;
         const double2 v1 = p.Q[ai + nvo * bj], v2 = p.Q[bi + nvo * aj];
@@ -260,7 +273,7 @@ __device__ __forceinline__ void body(const PackPabijP& p) {
 void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
                   hipStream_t st, double alpha) {
     ew_launch(RK_PACK_PABIJ, PackPabijP{Q, S, vv.tab, vv.npair, oo.tab, oo.npair, nv, no, alpha},
-              vv.npair * oo.npair, 1, st);
+              vv.npair, col_grid(oo.npair), st);
 }
 
 // ---------------------------------------------------------------- ring permutation
@@ -268,11 +281,10 @@ struct PermAkciP { const double2* X; double2* H; int nv; int no; int accumulate;
 __device__ __forceinline__ void body(const PermAkciP& p) {
     const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
-    GRID_LOOP(o, nvo * nvo) {
-        const int a = (int)(o % nv);
-        const int i = (int)((o / nv) % no);
-        const int64_t cc = o / nvo;
+    COL_LOOP(cc, nvo) ROW_LOOP(r, nvo) {
         const int c = (int)(cc % nv), k = (int)(cc / nv);
+        const int a = r % nv, i = r / nv;
+        const int64_t o = r + nvo * cc;
         const double2 x = p.X[(a + (int64_t)nv * k) + nvo * (c + (int64_t)nv * i)];
         p.H[o] = p.accumulate ? cadd(p.H[o], x) : x;
     }
@@ -281,7 +293,7 @@ __device__ __forceinline__ void body(const PermAkciP& p) {
 void k_perm_akci(const double2* X, double2* H, int nv, int no, bool accumulate,
                  hipStream_t st) {
     const int64_t nvo = (int64_t)nv * no;
-    ew_launch(RK_PERM_AKCI, PermAkciP{X, H, nv, no, accumulate ? 1 : 0}, nvo * nvo, 1, st);
+    ew_launch(RK_PERM_AKCI, PermAkciP{X, H, nv, no, accumulate ? 1 : 0}, nvo, col_grid(nvo), st);
 }
 
 // ---------------------------------------------------------------- traces
@@ -410,8 +422,9 @@ __device__ __forceinline__ void body(const Denom1P& p) {
 struct Denom2P { double2* S2; const double* eps; const int* frozen; const int2* vv; int64_t nvvt;
                  const int2* oo; int64_t noot; int no; };
 __device__ __forceinline__ void body(const Denom2P& p) {
-    GRID_LOOP(e, p.nvvt * p.noot) {
-        const int2 q1 = p.vv[e % p.nvvt], q2 = p.oo[e / p.nvvt];
+    COL_LOOP(ij, p.noot) ROW_LOOP(ab, p.nvvt) {
+        const int64_t e = ab + p.nvvt * ij;
+        const int2 q1 = p.vv[ab], q2 = p.oo[ij];
         double2 s = p.S2[e];
         if (p.eps) {
             const double d = p.eps[q2.x] + p.eps[q2.y] - p.eps[p.no + q1.x] - p.eps[p.no + q1.y];
@@ -428,7 +441,7 @@ void k_denom(double2* S1, double2* S2, const double* eps, const int* frozen, Pai
     if (S1) ew_launch(RK_DENOM1, Denom1P{S1, eps, frozen, nv, no}, (int64_t)nv * no, 1, st);
     if (S2)
         ew_launch(RK_DENOM2, Denom2P{S2, eps, frozen, vv.tab, vv.npair, oo.tab, oo.npair, no},
-                  vv.npair * oo.npair, 1, st);
+                  vv.npair, col_grid(oo.npair), st);
 }
 
 // ---------------------------------------------------------------- small BLAS-1
@@ -505,42 +518,67 @@ struct CoefPack {
     double c[MAXV];
 };
 
-// Block b reduces elements [b*chunk, min(n,(b+1)*chunk)) of sum conj(x_v) * y for each v.
+// Reductions: each thread accumulates its strided elements, a wave adds its
+// 64 lanes with shuffles (fixed butterfly order), the waves' sums meet in LDS
+// and one lane adds them in wave order -- so the result does not depend on
+// scheduling.
+constexpr int NWAVE = TPB / 64;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    return x;     // lane 0 holds the sum
+}
+
+// Block b reduces elements [b*chunk, min(n,(b+1)*chunk)) of sum conj(x_v) * y
+// for every v, reading y once for all of them.
 struct ZdotPartP { PtrPack xs; int nx; const double2* y; int64_t n; int64_t chunk;
                    double2* part; int nblk; };
 __device__ __forceinline__ void body(const ZdotPartP& p) {
-    __shared__ double sh[2][TPB];
+    __shared__ double sh[2][MAXV][NWAVE];
     if ((int)blockIdx.x >= p.nblk) return;      // whole workgroup (batched launches)
     const int64_t beg = blockIdx.x * p.chunk;
     const int64_t end = std::min<int64_t>(p.n, beg + p.chunk);
-    for (int v = 0; v < p.nx; ++v) {
-        const double2* __restrict__ x = p.xs.p[v];
-        double sr = 0.0, si = 0.0;
-        for (int64_t e = beg + threadIdx.x; e < end; e += TPB) {
-            const double2 a = x[e], b = p.y[e];
-            sr += a.x * b.x + a.y * b.y;
-            si += a.x * b.y - a.y * b.x;
-        }
-        sh[0][threadIdx.x] = sr;
-        sh[1][threadIdx.x] = si;
-        __syncthreads();
-        for (int s = TPB / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-                sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    double sr[MAXV], si[MAXV];
+#pragma unroll
+    for (int v = 0; v < MAXV; ++v) sr[v] = si[v] = 0.0;
+    for (int64_t e = beg + threadIdx.x; e < end; e += TPB) {
+        const double2 b = p.y[e];
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v)
+            if (v < p.nx) {
+                const double2 a = p.xs.p[v][e];
+                sr[v] += a.x * b.x + a.y * b.y;
+                si[v] += a.x * b.y - a.y * b.x;
             }
-            __syncthreads();
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < MAXV; ++v)
+        if (v < p.nx) {
+            const double r = wave_sum(sr[v]), i = wave_sum(si[v]);
+            if (lane == 0) {
+                sh[0][v][w] = r;
+                sh[1][v][w] = i;
+            }
         }
-        if (threadIdx.x == 0) p.part[v * p.nblk + blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
-        __syncthreads();
+    __syncthreads();
+    if ((int)threadIdx.x < p.nx) {
+        double r = 0.0, i = 0.0;
+#pragma unroll
+        for (int q = 0; q < NWAVE; ++q) {
+            r += sh[0][threadIdx.x][q];
+            i += sh[1][threadIdx.x][q];
+        }
+        p.part[threadIdx.x * p.nblk + blockIdx.x] = make_double2(r, i);
     }
 }
 
-// One workgroup per reduced value: strided partial sums then a tree, both in a
-// fixed order, so the result does not depend on scheduling.
+// One workgroup per reduced value: strided partial sums, wave shuffles, then
+// the waves in order.
 struct FinishP { const double2* part; int nblk; int nv; double2* dst; };
 __device__ __forceinline__ void body(const FinishP& p) {
-    __shared__ double sh[2][TPB];
+    __shared__ double sh[2][NWAVE];
     const int v = blockIdx.x;
     if (v >= p.nv) return;
     double sr = 0.0, si = 0.0;
@@ -548,17 +586,23 @@ __device__ __forceinline__ void body(const FinishP& p) {
         sr += p.part[v * p.nblk + b].x;
         si += p.part[v * p.nblk + b].y;
     }
-    sh[0][threadIdx.x] = sr;
-    sh[1][threadIdx.x] = si;
-    __syncthreads();
-    for (int s = TPB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-        }
-        __syncthreads();
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = sr;
+        sh[1][w] = si;
     }
-    if (threadIdx.x == 0) p.dst[v] = make_double2(sh[0][0], sh[1][0]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0, i = 0.0;
+#pragma unroll
+        for (int q = 0; q < NWAVE; ++q) {
+            r += sh[0][q];
+            i += sh[1][q];
+        }
+        p.dst[v] = make_double2(r, i);
+    }
 }
 
 // partial-sum blocks for n elements: about 4096 elements each, at most nblk
@@ -593,7 +637,7 @@ void k_zdotc_multi(const double2* const* xs, int nx, const double2* y, int64_t n
 // dst[0] = sum x, dst[1] = (sum |x|^2, 0)
 struct SumNormPartP { const double2* x; int64_t n; int64_t chunk; double2* part; int nblk; };
 __device__ __forceinline__ void body(const SumNormPartP& p) {
-    __shared__ double sh[3][TPB];
+    __shared__ double sh[3][NWAVE];
     if ((int)blockIdx.x >= p.nblk) return;
     const int64_t beg = blockIdx.x * p.chunk;
     const int64_t end = std::min<int64_t>(p.n, beg + p.chunk);
@@ -604,21 +648,26 @@ __device__ __forceinline__ void body(const SumNormPartP& p) {
         si += a.y;
         nn += a.x * a.x + a.y * a.y;
     }
-    sh[0][threadIdx.x] = sr;
-    sh[1][threadIdx.x] = si;
-    sh[2][threadIdx.x] = nn;
-    __syncthreads();
-    for (int s = TPB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-            sh[2][threadIdx.x] += sh[2][threadIdx.x + s];
-        }
-        __syncthreads();
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    nn = wave_sum(nn);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = sr;
+        sh[1][w] = si;
+        sh[2][w] = nn;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        p.part[blockIdx.x] = make_double2(sh[0][0], sh[1][0]);
-        p.part[p.nblk + blockIdx.x] = make_double2(sh[2][0], 0.0);
+        double r = 0.0, i = 0.0, q2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NWAVE; ++q) {
+            r += sh[0][q];
+            i += sh[1][q];
+            q2 += sh[2][q];
+        }
+        p.part[blockIdx.x] = make_double2(r, i);
+        p.part[p.nblk + blockIdx.x] = make_double2(q2, 0.0);
     }
 }
 
@@ -817,16 +866,15 @@ struct SplitReduceP { int m; int n; int nsplit; int batch; const double2* W; dou
                       int64_t ldc; int64_t sC; double2 alpha; double2 beta; };
 __device__ __forceinline__ void body(const SplitReduceP& p) {
     const int64_t mn = (int64_t)p.m * p.n;
-    GRID_LOOP(t, mn * p.batch) {
-        const int64_t b = t / mn, e = t - b * mn;
+    const int ways = split_ways(p.nsplit);
+    GRID_LOOP(t, mn * p.batch * ways) {
+        const int64_t oi = t / ways;
+        const int q = (int)(t - oi * ways);
+        const int64_t b = oi / mn, e = oi - b * mn;
         const int64_t i = e % p.m, j = e / p.m;
-        double sr = 0.0, si = 0.0;
-        const double2* w = p.W + b * p.nsplit * mn + e;
-        for (int s = 0; s < p.nsplit; ++s) {
-            const double2 v = w[(int64_t)s * mn];
-            sr += v.x;
-            si += v.y;
-        }
+        const double2 sum = split_sum(p.W + b * p.nsplit * mn + e, p.nsplit, mn, q, ways);
+        if (q != 0) continue;
+        const double sr = sum.x, si = sum.y;
         double2* cp = p.C + b * p.sC + i + j * p.ldc;
         double2 o;
         o.x = p.alpha.x * sr - p.alpha.y * si;

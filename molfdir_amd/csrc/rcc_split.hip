@@ -59,16 +59,16 @@ __global__ void __launch_bounds__(256) split_exp_kernel(const double2* __restric
             mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
         }
     }
-    __shared__ double sh[256];
-    sh[threadIdx.x] = mx;
+    // wave maxima by shuffles, then the four waves' in LDS
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
         int e = 0;
-        if (sh[0] > 0.0) (void)frexp(sh[0], &e);
+        if (m > 0.0) (void)frexp(m, &e);
         ex[(int64_t)rb * nkc + kc] = (signed char)std::max(-127, std::min(127, e));
     }
 }

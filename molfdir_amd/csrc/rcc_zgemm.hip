@@ -399,25 +399,25 @@ int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngrou
     return -1;
 }
 
-// Sums the K-split partials in split order and applies alpha / beta.
+// Sums the K-split partials (split_sum: split_ways(nsplit) lanes per output)
+// and applies alpha / beta.
 __global__ void zgemm_reduce_kernel(int m, int n, int nsplit, int batch,
                                     const double2* __restrict__ W,
                                     double2* __restrict__ C, int64_t ldc, int64_t sC,
                                     double2 alpha, double2 beta) {
     const int64_t mn = (int64_t)m * n;
-    const int64_t total = mn * batch;
+    const int ways = split_ways(nsplit);
+    const int64_t total = mn * batch * ways;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = t / mn;
-        const int64_t e = t - b * mn;
+        const int64_t oi = t / ways;
+        const int q = (int)(t - oi * ways);
+        const int64_t b = oi / mn;
+        const int64_t e = oi - b * mn;
         const int64_t i = e % m, j = e / m;
-        double sr = 0.0, si = 0.0;
-        const double2* w = W + b * nsplit * mn + e;
-        for (int s = 0; s < nsplit; ++s) {
-            const double2 v = w[(int64_t)s * mn];
-            sr += v.x;
-            si += v.y;
-        }
+        const double2 sum = split_sum(W + b * nsplit * mn + e, nsplit, mn, q, ways);
+        if (q != 0) continue;
+        const double sr = sum.x, si = sum.y;
         double2* cp = C + b * sC + i + j * ldc;
         double2 o;
         o.x = alpha.x * sr - alpha.y * si;
@@ -600,7 +600,7 @@ static int launch_split(const GemmArgs& g, int nsplit, int kchunk, double2* ws, 
     else return -1;
 #undef RCC_L
     if (nsplit > 1) {
-        const int64_t tot = (int64_t)g.m * g.n * g.batch;
+        const int64_t tot = (int64_t)g.m * g.n * g.batch * split_ways(nsplit);
         const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65535);
         hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(blocks), dim3(256), 0, st, g.m, g.n, nsplit,
                            g.batch, Wm, g.C, g.ldc, g.sC, g.alpha, g.beta);

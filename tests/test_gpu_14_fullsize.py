@@ -187,7 +187,9 @@ def test_cfg4_64_system_batch_paths_against_per_system_iterations():
         for (n0, e0, r0), (n1, e1, r1) in zip(eager[0][i], grouped[0][i]):
             assert n0 == n1
             assert abs(e0 - e1) < 1e-12, (i, n0, e0, e1)
-            assert abs(r0 - r1) < 1e-9 * r0, (i, n0, r0, r1)
+            # near convergence the rms is a difference of nearly equal
+            # amplitudes: rounding of order 1e-18 Eh, hence the absolute floor
+            assert abs(r0 - r1) < 1e-9 * r0 + 1e-16, (i, n0, r0, r1)
         for x, y in zip(eager[1][i], grouped[1][i]):
             assert np.abs(x - y).max() < 1e-12, i
 
