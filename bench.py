@@ -998,11 +998,11 @@ def main():
     for mode in modes:
         set_mode(r, mode, maxit, args.ntol)
         if mode == "bf16":
-            # 4 real products x 6 slice products per complex product
-            ops, peak = 24 * 2.0 * nvvt * nvvt * noot, PEAK_BF16_MFMA
-            scheme = ("3 BF16 slices per FP64 value, 6 slice products, FP32 accumulation over "
-                      "128-term chunks, FP64 across chunks")
-            counts = "executed BF16 MFMA flops (24 real products per complex product)"
+            # 3 real products (Re, Im, Re+Im) x 6 slice products per complex product
+            ops, peak = 18 * 2.0 * nvvt * nvvt * noot, PEAK_BF16_MFMA
+            scheme = ("Re, Im, Re+Im as 3 BF16 slices each (per row and 128-k chunk), 6 slice "
+                      "products, FP32 accumulation over 128-term chunks, FP64 across chunks")
+            counts = "BF16 MFMA flops of the unpadded product (3 real products x 6 slice products)"
         else:
             d = int(mode[-1])
             pairs = d * (d + 1) // 2

@@ -96,8 +96,9 @@ const char* rcc_last_error(const rcc_t* h);
  * (ccdriv.F:208-225).  precision: 0 = FP64 everywhere; 1 and 2 run the VVVV
  * ladder, the VOVV.tau product and both (ov)^3 ring contractions as split
  * matrix-core products while denominators, residuals, DIIS and the energy stay
- * FP64.  1 = split-BF16: three BF16 slices per value, FP32 accumulation over
- * 128-term chunks, FP64 across chunks.  2 = split-INT8: Re, Im and Re+Im as
+ * FP64.  1 = split-BF16: Re, Im and Re+Im as three BF16 slices each (scale
+ * per row and 128-term chunk), FP32 accumulation over 128-term chunks, FP64
+ * across chunks.  2 = split-INT8: Re, Im and Re+Im as
  * signed 7-bit digits (rcc_set_split_digits; fixed point per row and 8192-term
  * chunk), exact int32 accumulation, FP64 across chunks.  No counterpart in the
  * reference (FP64 only). */
@@ -389,7 +390,8 @@ int rcc_zgemm_device_noalloc(rcc_t* h, int opa, int opb, int m, int n, int k,
                              const double* alpha, const void* A, int64_t lda, const void* B,
                              int64_t ldb, const double* beta, void* C, int64_t ldc, int ksplit);
 /* The split-BF16 GEMM on device pointers: C = alpha A B + beta C with A (m x k)
- * and B (k x n) column-major complex (diagnostic; allocates its own planes). */
+ * and B (k x n) column-major complex (diagnostic; allocates its own planes and
+ * room for an 8-way K-split's partial sums). */
 int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, const void* A,
                            int64_t lda, const void* B, int64_t ldb, const double* beta, void* C,
                            int64_t ldc);

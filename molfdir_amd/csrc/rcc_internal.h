@@ -294,13 +294,14 @@ void k_blockset(const double2* src, const BlockSet& bs, double2* dst, int64_t ld
                 hipStream_t st);
 
 // ---- split-BF16 operands and GEMM (rcc_split.hip) ----
-// Six BF16 planes (Re slices 1-3, Im slices 1-3), K-major, rows padded to 64
-// and k to 128, plus one power-of-two exponent per (64-row, 128-k) block.
+// Re, Im and Re+Im as three BF16 slices each, fragment-major, rows padded to
+// 128 and k to 128, plus one power-of-two exponent per (row, 128-k chunk,
+// variant).
 struct SplitOperand {
-    int rows = 0, k = 0, rows_pad = 0, nkc = 0;
-    int64_t k_pad = 0, plane = 0;
-    void* planes = nullptr;          // 6 * plane bf16 values
-    signed char* ex = nullptr;       // (rows_pad / 64) * nkc exponents
+    int rows = 0, k = 0, rows_pad = 0, nkc = 0, nks = 0;
+    int64_t k_pad = 0, plane_bytes = 0, ex_bytes = 0;
+    void* planes = nullptr;          // plane_bytes of slice fragments
+    signed char* ex = nullptr;       // ex_bytes exponents
 };
 void split_layout(SplitOperand& S, int rows, int k);
 // X: logical rows x k complex operand; k_contiguous: element (r, kk) at
@@ -308,9 +309,14 @@ void split_layout(SplitOperand& S, int rows, int k);
 int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_contiguous,
                   bool conj, hipStream_t st);
 // C(m x n) = alpha * A . B^T + beta * C with A (m rows) and B (n rows) split
-// operands over the same k.
+// operands over the same k.  ws (ws_elems complex, may be null) holds the
+// partial sums of a K-split.
 int zgemm_split(const SplitOperand& A, const SplitOperand& B, double2* C, int64_t ldc,
-                double2 alpha, double2 beta, hipStream_t st);
+                double2 alpha, double2 beta, double2* ws, int64_t ws_elems, hipStream_t st);
+// C = alpha sum_s W[s] + beta C over nsplit m x n partials W (ld m, stride m n),
+// summed as zgemm's K-split reduction
+void launch_split_reduce(int m, int n, int nsplit, const double2* W, double2* C, int64_t ldc,
+                         double2 alpha, double2 beta, hipStream_t st);
 
 // ---- split-INT8 operands and GEMM (rcc_split_i8.hip) ----
 // Re, Im and Re+Im as `digits` signed 7-bit digits per value (fixed point per

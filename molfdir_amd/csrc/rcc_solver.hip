@@ -414,8 +414,8 @@ int do_split(rcc_t* h, SOp& S, int rows, int64_t k, const double2* X, int64_t ld
     if (h->precision == 1) {
         if (!S.bf.planes) {
             split_layout(S.bf, rows, (int)k);
-            CK(dalloc_raw(h, (char**)&S.bf.planes, 6 * S.bf.plane * 2));
-            CK(dalloc_raw(h, &S.bf.ex, (int64_t)(S.bf.rows_pad / 64) * S.bf.nkc));
+            CK(dalloc_raw(h, (char**)&S.bf.planes, std::max<int64_t>(1, S.bf.plane_bytes)));
+            CK(dalloc_raw(h, &S.bf.ex, std::max<int64_t>(1, S.bf.ex_bytes)));
         }
         if (split_operand(S.bf, X, ld, kcont, conj, h->st) != 0) {
             h->err = "split_operand launch failed";
@@ -438,7 +438,8 @@ int do_split(rcc_t* h, SOp& S, int rows, int64_t k, const double2* X, int64_t ld
 
 int sgemm(rcc_t* h, const SOp& A, const SOp& B, double2* C, int64_t ldc, double2 alpha,
           double2 beta) {
-    const int rc = h->precision == 1 ? zgemm_split(A.bf, B.bf, C, ldc, alpha, beta, h->st)
+    const int rc = h->precision == 1
+                       ? zgemm_split(A.bf, B.bf, C, ldc, alpha, beta, h->ws, h->ws_elems, h->st)
                                      : zgemm_i8(A.i8, B.i8, C, ldc, alpha, beta, h->st);
     if (rc != 0) {
         h->err = "split GEMM launch failed";
@@ -3713,11 +3714,14 @@ int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, c
     SplitOperand a, b;
     split_layout(a, m, k);
     split_layout(b, n, k);
-    void* mem[4] = {nullptr, nullptr, nullptr, nullptr};
-    HIPCHK(h, hipMalloc(&mem[0], 6 * a.plane * 2));
-    HIPCHK(h, hipMalloc(&mem[1], std::max<int64_t>(1, (int64_t)(a.rows_pad / 64) * a.nkc)));
-    HIPCHK(h, hipMalloc(&mem[2], 6 * b.plane * 2));
-    HIPCHK(h, hipMalloc(&mem[3], std::max<int64_t>(1, (int64_t)(b.rows_pad / 64) * b.nkc)));
+    void* mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // room for an 8-way K-split's partial sums
+    const int64_t ws_elems = 8 * (int64_t)std::max(m, 1) * std::max(n, 1);
+    HIPCHK(h, hipMalloc(&mem[0], std::max<int64_t>(1, a.plane_bytes)));
+    HIPCHK(h, hipMalloc(&mem[1], std::max<int64_t>(1, a.ex_bytes)));
+    HIPCHK(h, hipMalloc(&mem[2], std::max<int64_t>(1, b.plane_bytes)));
+    HIPCHK(h, hipMalloc(&mem[3], std::max<int64_t>(1, b.ex_bytes)));
+    HIPCHK(h, hipMalloc(&mem[4], ws_elems * sizeof(double2)));
     a.planes = mem[0];
     a.ex = (signed char*)mem[1];
     b.planes = mem[2];
@@ -3727,7 +3731,7 @@ int rcc_zgemm_split_device(rcc_t* h, int m, int n, int k, const double* alpha, c
     if (rc == 0) rc = split_operand(b, (const double2*)B, ldb, true, false, h->st);
     if (rc == 0)
         rc = zgemm_split(a, b, (double2*)C, ldc, cplx(alpha[0], alpha[1]), cplx(beta[0], beta[1]),
-                         h->st);
+                         (double2*)mem[4], ws_elems, h->st);
     if (rc != 0) h->err = "rcc_zgemm_split_device: launch failed";
     HIPCHK(h, hipStreamSynchronize(h->st));
     for (void* p : mem) HIPCHK(h, hipFree(p));

@@ -590,6 +590,14 @@ static void plan_split(const GemmArgs& g, int s, int& nsplit, int& kchunk) {
     }
 }
 
+void launch_split_reduce(int m, int n, int nsplit, const double2* W, double2* C, int64_t ldc,
+                         double2 alpha, double2 beta, hipStream_t st) {
+    const int64_t tot = (int64_t)m * n * split_ways(nsplit);
+    const int blocks = (int)std::min<int64_t>((tot + 255) / 256, 65535);
+    hipLaunchKernelGGL(zgemm_reduce_kernel, dim3(blocks), dim3(256), 0, st, m, n, nsplit, 1, W, C,
+                       ldc, (int64_t)0, alpha, beta);
+}
+
 static int launch_split(const GemmArgs& g, int nsplit, int kchunk, double2* ws, hipStream_t st) {
     double2* Wm = nsplit > 1 ? ws : nullptr;
 #define RCC_L(a, b) if (g.opA == a && g.opB == b) { launch<a, b>(g, nsplit, kchunk, Wm, st); }

@@ -1,5 +1,5 @@
 """GPU checks of the split complex GEMMs and of the CCSD iteration run with
-them: split-BF16 (three BF16 slices per FP64 operand on
+them: split-BF16 (Re, Im and Re+Im as three BF16 slices each on
 v_mfma_f32_16x16x32_bf16, FP32 accumulation over 128-term chunks; precision
 1) and split-INT8 (4 or 5 signed 7-bit digits of Re, Im and Re+Im on
 v_mfma_i32_16x16x64_i8, exact int32 accumulation; precision 2).
@@ -8,12 +8,12 @@ Exact checks: integer operands small enough that every slice product and
 every 128-term FP32 sum is exact must give numpy's product bit for bit (pins
 the A/B/D lane layouts, chunk loop and exponent handling); power-of-two
 rescaling of the operands must rescale the result bit for bit (pins the
-per-block exponents).  Rounding checks: random operands to an rms-relative
+per-row exponents).  Rounding checks: random operands to an rms-relative
 error of 2^-17 (FP32 accumulation sets the floor), which a missing
 first-order slice product exceeds by orders of magnitude; k = 1 products of
-values built from exactly one, two or three BF16 slices must come out exact,
-which pins every retained slice product.  The CCSD trajectories are compared
-with the FP64 reference fixtures.
+values whose Re, Im and Re+Im are each built from exactly one, two or three
+BF16 slices must come out exact, which pins every retained slice product.
+The CCSD trajectories are compared with the FP64 reference fixtures.
 """
 import numpy as np
 import pytest
@@ -77,9 +77,10 @@ def test_split_gemm_slice_products_exact(case):
     mk = {"A3xB1": (3, 1), "A1xB3": (1, 3), "A2xB2": (2, 2)}[case]
 
     def op(shape, terms):
-        if terms == 1:
-            return _pow2(rng, shape) + 1j * _pow2(rng, shape)
-        return _sliced(rng, shape, terms) + 1j * _sliced(rng, shape, terms)
+        # Im = s Re with s in {-1, 0, 1}: Re, Im and Re + Im (0, x or 2x) are
+        # all sums of exactly `terms` BF16 slices
+        x = _pow2(rng, shape) if terms == 1 else _sliced(rng, shape, terms)
+        return x * (1 + 1j * rng.integers(-1, 2, shape))
 
     A = op((m, 1), mk[0])
     B = op((1, n), mk[1])
@@ -122,6 +123,24 @@ def test_split_gemm_block_scaling_keeps_small_blocks_accurate():
     for rows in (slice(0, 64), slice(64, 128)):
         s = np.sqrt(256 * np.mean(np.abs(A[rows]) ** 2) * np.mean(np.abs(B) ** 2))
         assert np.abs(got[rows] - ref[rows]).max() / s < 2.0 ** -17
+
+
+@pytest.mark.parametrize("tile,kparts,gm", [(128, 3, 1), (64, 8, 4), (64, 2, 2), (64, 1, 1),
+                                             (128, 1, 8)])
+def test_split_gemm_tiles_ksplit_and_band_exact(monkeypatch, tile, kparts, gm):
+    # both tiles (128 x 128 with three LDS buffers, 128 x 64 with two), the
+    # K-split (partial sums reduced in a fixed order) and the tile-row bands
+    # keep exact integer products exact, alpha and beta included
+    monkeypatch.setenv("RCC_BF_TILE", str(tile))
+    monkeypatch.setenv("RCC_BF_KPARTS", str(kparts))
+    monkeypatch.setenv("RCC_BF_GM", str(gm))
+    rng = np.random.default_rng(kparts + 10 * gm + tile)
+    m, n, k = 300, 270, 1100
+    A = rng.integers(-64, 65, (m, k)) + 1j * rng.integers(-64, 65, (m, k))
+    B = rng.integers(-64, 65, (k, n)) + 1j * rng.integers(-64, 65, (k, n))
+    C = rng.integers(-64, 65, (m, n)) + 1j * rng.integers(-64, 65, (m, n))
+    got = split_gemm(A, B, C, 2.0 - 1.0j, -1.0 + 0.5j)
+    assert np.array_equal(got, (2.0 - 1.0j) * (A @ B) + (-1.0 + 0.5j) * C)
 
 
 @pytest.mark.parametrize("name", [x for x in names("ccsd") if x != "cfg1_6o40v"])
