@@ -747,9 +747,12 @@ def bench_batch(args):
 
 
 def bench_symmetric(args, no, nv, seed, desc):
-    """One system with abelian double-group symmetry: the symmetry-blocked
-    handle (ladder, VOVV.tau and rings as grouped GEMMs over the irrep
-    blocks) against a C1 handle fed the same integrals."""
+    """One system with abelian double-group symmetry, three handles on the
+    same integrals: the symmetry-blocked engine (every array stored as its
+    irrep blocks, every contraction one grouped GEMM over them), the earlier
+    symmetric form (C1 working layout, the ladder, VOVV.tau and rings grouped;
+    RCC_BLOCKED=0) and a C1 handle.  Device memory per handle from
+    hipMemGetInfo around its set-up."""
     import numpy as np
     import torch
     from molfdir_amd import synth
@@ -763,9 +766,15 @@ def bench_symmetric(args, no, nv, seed, desc):
     maps = synth.block_maps(sym)
     K, W = args.steps, args.warmup
     res = {}
-    for mode in ("symmetric", "c1"):
-        if mode == "symmetric":
+    for mode in args.sym_modes.split(","):
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        free0 = torch.cuda.mem_get_info()[0]
+        t_setup = time.perf_counter()
+        if mode in ("blocked", "c1_layout"):
+            os.environ["RCC_BLOCKED"] = "1" if mode == "blocked" else "0"
             r = RelCCSD(no, nv, sym=sym, maxdim=6, maxit=K + W, ntol=12)
+            os.environ.pop("RCC_BLOCKED", None)
             r.set_fock(*(synth.to_blocks(n, sysd[n], sym, maps).cpu()
                          for n in ("FOO", "FVO", "FVV")), sysd["eps"])
             for name in ("OOOO", "VOOO", "VVOO", "VOVO", "VOVV", "VVVV"):
@@ -774,6 +783,10 @@ def bench_symmetric(args, no, nv, seed, desc):
             r = RelCCSD(no, nv, maxdim=6, maxit=K + W, ntol=12)
             r.set_system(sysd)
         r.mp2()
+        r.synchronize()
+        t_setup = time.perf_counter() - t_setup
+        torch.cuda.empty_cache()     # upload staging held by torch's cache is not the handle's
+        mem = free0 - torch.cuda.mem_get_info()[0]
         for _ in range(W):
             r.iterate()
         r.synchronize()
@@ -781,26 +794,30 @@ def bench_symmetric(args, no, nv, seed, desc):
         rows = [r.iterate() for _ in range(K)]
         r.synchronize()
         res[mode] = {"s_per_iter": (time.perf_counter() - t0) / K, "energies": [x[1] for x in rows],
-                     "kernels_ms": r.kernel_times()}
-        print(f"[bench] {mode}: {res[mode]['s_per_iter'] * 1e3:.1f} ms/iter", file=sys.stderr,
-              flush=True)
+                     "device_gb": mem / 1e9, "setup_s": t_setup}
+        print(f"[bench] {mode}: {res[mode]['s_per_iter'] * 1e3:.2f} ms/iter, "
+              f"{mem / 1e9:.1f} GB, set-up {t_setup:.1f} s", file=sys.stderr, flush=True)
         r.close()
         del r
         torch.cuda.empty_cache()
-    de = max(abs(a - b) for a, b in zip(res["symmetric"]["energies"], res["c1"]["energies"]))
+    ref = res.get("c1") or res.get("c1_layout") or res["blocked"]
+    de = max(abs(a - b) for m in res for a, b in zip(res[m]["energies"], ref["energies"]))
     F = ccsd_flops(no, nv)
     out = {
-        "metric": f"CCSD s/iter at {no}o/{nv}v with {nrep}-irrep symmetry (grouped vs C1)",
-        "value": res["symmetric"]["s_per_iter"], "unit": "s/iter", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": res["symmetric"]["s_per_iter"] * 1e3,
+        "metric": f"CCSD s/iter at {no}o/{nv}v with {nrep}-irrep symmetry (blocked vs C1)",
+        "value": res["blocked"]["s_per_iter"], "unit": "s/iter", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": res["blocked"]["s_per_iter"] * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": f"synthetic (seed {seed}, factor std {args.scale}, Z2^k symmetry)",
         "config": {"workload": f"CCSD iteration, {desc}, {nrep} irreps", "no_per_irrep": sym[0],
                    "nv_per_irrep": sym[1]},
-        "speedup_vs_c1_handle": res["c1"]["s_per_iter"] / res["symmetric"]["s_per_iter"],
-        "c1_s_per_iter": res["c1"]["s_per_iter"], "max_dEc_vs_c1": de,
-        "c1_equivalent_tflops": F / res["symmetric"]["s_per_iter"] / 1e12,
-        "kernels_ms": {m: res[m]["kernels_ms"] for m in res},
+        "speedup_vs_c1_handle": (res["c1"]["s_per_iter"] / res["blocked"]["s_per_iter"]
+                                 if "c1" in res else None),
+        "c1_layout_s_per_iter": res.get("c1_layout", {}).get("s_per_iter"),
+        "c1_s_per_iter": res.get("c1", {}).get("s_per_iter"), "max_dEc_vs_c1": de,
+        "device_gb": {m: res[m]["device_gb"] for m in res},
+        "setup_s": {m: res[m]["setup_s"] for m in res},
+        "c1_equivalent_tflops": F / res["blocked"]["s_per_iter"] / 1e12,
     }
     print(json.dumps(out), flush=True)
 
@@ -828,6 +845,8 @@ def main():
     ap.add_argument("--t3-scale", type=float, default=0.01,
                     help="cfg3t: std of the factor B (smaller ones are tried if CCSD diverges)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sym-modes", default="blocked,c1_layout,c1",
+                    help="--sym-irreps: the handle forms to time (blocked, c1_layout, c1)")
     ap.add_argument("--complex-product", type=int, default=3, choices=(3, 4),
                     help="real MFMA products per complex product in the FP64 GEMMs")
     ap.add_argument("--split", default="all", choices=("none", "bf16", "i8x4", "i8x5", "all"),

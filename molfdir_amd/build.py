@@ -14,6 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librelccsd_hip.so")
 SOURCES = ["rcc_zgemm.hip", "rcc_kernels.hip", "rcc_split.hip", "rcc_split_i8.hip",
            "rcc_split_i8w.hip", "rcc_triples.hip", "rcc_frontend.hip", "rcc_props.hip",
+           "rcc_blocked.hip",
            "rcc_solver.hip"]
 # sources compiled without the VGPR form of the MFMA accumulators (none now)
 NO_VGPR_FORM = set()

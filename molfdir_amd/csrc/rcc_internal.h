@@ -143,7 +143,9 @@ enum RecKind {
     RK_PACK_PABIJ, RK_PERM_AKCI, RK_TRACE_T1, RK_TRACE_GV, RK_DENOM1, RK_DENOM2, RK_AXPBY,
     RK_CONJ, RK_COPY, RK_ZERO_DIAG, RK_BLOCKSET, RK_FILL, RK_SPLIT_REDUCE, RK_GEMM, RK_GROUPED,
     // batched DIISX / CCENG stages (not recorded; parameters built per iteration)
-    RK_ZDOT_PART, RK_FINISH, RK_SUMNORM_PART, RK_LINCOMB, RK_DIIS_PREP
+    RK_ZDOT_PART, RK_FINISH, RK_SUMNORM_PART, RK_LINCOMB, RK_DIIS_PREP,
+    // the symmetry-blocked engine's element-wise work (rcc_blocked.hip)
+    RK_MAPSUM, RK_MAPBIL, RK_BDENOM, RK_GRED
 };
 struct Rec {
     int kind = 0;
@@ -153,8 +155,8 @@ struct Rec {
     GemmArgs g;                       // RK_GEMM: the product, its K split and workspace
     int nsplit = 1, kchunk = 0;
     double2* ws = nullptr;
-    std::vector<ZGroup> groups;       // RK_GROUPED: descriptors, op(B), tag, complex form
-    int opb = 0, tag = 0, gauss = 0;
+    std::vector<ZGroup> groups;       // RK_GROUPED: descriptors, op(A), op(B), tag, complex form
+    int opa = 0, opb = 0, tag = 0, gauss = 0;
 };
 extern thread_local std::vector<Rec>* g_record;
 struct BatchProgram;
@@ -256,6 +258,35 @@ void k_sum_and_norm(const double2* x, int64_t n, double2* dst, double2* part, in
                     hipStream_t st);
 void k_lincomb(const double2* const* xs, const double* coef, int nx, double2* y, int64_t n,
                hipStream_t st);
+
+// ---- symmetry-blocked engine kernels ----
+// out[e] = beta out[e] + alpha sum_t f_t(src[slot[t]][map[t n + e] >> 2]), map
+// entries (index << 2) | (negate << 1) | conjugate, -1 for a zero term,
+// term-major; T > 4 terms all read src[0].
+void k_map_sum(double2* out, const double2* const* src, const int* slot, int nsrc,
+               const int32_t* map, int64_t n, int T, double alpha, double beta, hipStream_t st);
+// out[e] = c0 f(src0[m0]) + sg0 A[p0] B[q0] + sg1 A[p1] B[q1] (np terms, -1: absent)
+void k_map_bil(double2* out, const double2* src0, const double2* A, const double2* B,
+               const int32_t* m0, const int32_t* p0, const int32_t* q0, const int32_t* p1,
+               const int32_t* q1, int64_t n, int np, double c0, double sg0, double sg1,
+               hipStream_t st);
+void mapbil_params(double2* out, const double2* src0, const double2* A, const double2* B,
+                   const int32_t* m0, const int32_t* p0, const int32_t* q0, const int32_t* p1,
+                   const int32_t* q1, int64_t n, int np, double c0, double sg0, double sg1,
+                   void* q);
+// S[e] /= D[e] (divide), S[e] = 0 where F[e] != 0 (mask)
+void k_bdenom(double2* S, const double* D, const double* F, int64_t n, int divide, int mask,
+              hipStream_t st);
+// K-split partial sums of grouped products: C = alpha sum_s W[s m n + e] + beta C
+struct GRedDesc {
+    double2* C;
+    const double2* W;
+    int64_t ldc;
+    int m, n, nsplit, pad;
+    double alpha, beta;
+};
+// maxn: the largest m n split_ways(nsplit) over the descriptors
+void k_gred(const GRedDesc* d, int count, int64_t maxn, hipStream_t st);
 
 // symmetry blocks (packed, column-major, rows x ncols) <-> C1 arrays:
 // C1 element of block element (r, c) = rowmap[r] + ld * colmap[c]

@@ -738,6 +738,144 @@ void diis_prep_params(const double2* S, const double2* T, double2* d, double2* a
     *(DiisPrepP*)pp = DiisPrepP{S, T, d, amp, n};
 }
 
+// ---------------------------------------------------------------- symmetry-blocked engine
+// The element-wise work of the blocked iteration (rcc_blocked.hip): gathers
+// through precomputed index maps, the bilinear gather of tau and the ring
+// intermediate, denominators per blocked element and the K-split reductions
+// of grouped products.  Map entries are (index << 2) | (negate << 1) | conj,
+// -1 for a zero term, stored term-major.
+struct MapSumP {
+    double2* out; const double2* src[4]; int slot[4]; const int32_t* map; int64_t n; int T;
+    double alpha, beta;
+};
+__device__ __forceinline__ void body(const MapSumP& p) {
+    GRID_LOOP(e, p.n) {
+        double sr = 0.0, si = 0.0;
+        for (int t = 0; t < p.T; ++t) {
+            const int32_t m = p.map[(int64_t)t * p.n + e];
+            if (m < 0) continue;
+            const double2 v = p.src[p.T <= 4 ? p.slot[t] : 0][m >> 2];
+            const double vi = (m & 1) ? -v.y : v.y;
+            if (m & 2) { sr -= v.x; si -= vi; }
+            else { sr += v.x; si += vi; }
+        }
+        double2 o = make_double2(p.alpha * sr, p.alpha * si);
+        if (p.beta != 0.0) {
+            const double2 c = p.out[e];
+            o.x += p.beta * c.x;
+            o.y += p.beta * c.y;
+        }
+        p.out[e] = o;
+    }
+}
+
+void k_map_sum(double2* out, const double2* const* src, const int* slot, int nsrc,
+               const int32_t* map, int64_t n, int T, double alpha, double beta, hipStream_t st) {
+    MapSumP p{};
+    p.out = out;
+    for (int i = 0; i < 4; ++i) {
+        p.src[i] = i < nsrc ? src[i] : nullptr;
+        p.slot[i] = i < T && T <= 4 ? slot[i] : 0;
+    }
+    p.map = map;
+    p.n = n;
+    p.T = T;
+    p.alpha = alpha;
+    p.beta = beta;
+    ew_launch(RK_MAPSUM, p, n, 1, st);
+}
+
+struct MapBilP {
+    double2* out; const double2* src0; const double2* A; const double2* B;
+    const int32_t* m0; const int32_t* p0; const int32_t* q0; const int32_t* p1; const int32_t* q1;
+    int64_t n; int np; double c0, sg0, sg1;
+};
+__device__ __forceinline__ void body(const MapBilP& p) {
+    GRID_LOOP(e, p.n) {
+        double2 v = make_double2(0.0, 0.0);
+        const int32_t m = p.m0[e];
+        if (m >= 0) {
+            double2 w = p.src0[m >> 2];
+            if (m & 1) w.y = -w.y;
+            if (m & 2) { w.x = -w.x; w.y = -w.y; }
+            v = cscale(p.c0, w);
+        }
+        if (p.np > 0 && p.p0[e] >= 0) {
+            const double2 q = cmul(p.A[p.p0[e]], p.B[p.q0[e]]);
+            v.x += p.sg0 * q.x;
+            v.y += p.sg0 * q.y;
+        }
+        if (p.np > 1 && p.p1[e] >= 0) {
+            const double2 q = cmul(p.A[p.p1[e]], p.B[p.q1[e]]);
+            v.x += p.sg1 * q.x;
+            v.y += p.sg1 * q.y;
+        }
+        p.out[e] = v;
+    }
+}
+
+void mapbil_params(double2* out, const double2* src0, const double2* A, const double2* B,
+                   const int32_t* m0, const int32_t* p0, const int32_t* q0, const int32_t* p1,
+                   const int32_t* q1, int64_t n, int np, double c0, double sg0, double sg1,
+                   void* q) {
+    *(MapBilP*)q = MapBilP{out, src0, A, B, m0, p0, q0, p1, q1, n, np, c0, sg0, sg1};
+}
+
+void k_map_bil(double2* out, const double2* src0, const double2* A, const double2* B,
+               const int32_t* m0, const int32_t* p0, const int32_t* q0, const int32_t* p1,
+               const int32_t* q1, int64_t n, int np, double c0, double sg0, double sg1,
+               hipStream_t st) {
+    ew_launch(RK_MAPBIL, MapBilP{out, src0, A, B, m0, p0, q0, p1, q1, n, np, c0, sg0, sg1}, n,
+              1, st);
+}
+
+struct BDenomP { double2* S; const double* D; const double* F; int64_t n; int divide; int mask; };
+__device__ __forceinline__ void body(const BDenomP& p) {
+    GRID_LOOP(e, p.n) {
+        double2 s = p.S[e];
+        if (p.divide) {
+            const double d = p.D[e];
+            s.x /= d;
+            s.y /= d;
+        }
+        if (p.mask && p.F[e] != 0.0) s = make_double2(0.0, 0.0);
+        p.S[e] = s;
+    }
+}
+
+void k_bdenom(double2* S, const double* D, const double* F, int64_t n, int divide, int mask,
+              hipStream_t st) {
+    ew_launch(RK_BDENOM, BDenomP{S, D, F, n, divide, mask}, n, 1, st);
+}
+
+struct GRedP { const GRedDesc* d; int count; };
+__device__ __forceinline__ void body(const GRedP& p) {
+    for (int q = blockIdx.y; q < p.count; q += gridDim.y) {
+        const GRedDesc d = p.d[q];
+        const int64_t mn = (int64_t)d.m * d.n;
+        const int ways = split_ways(d.nsplit);
+        GRID_LOOP(t, mn * ways) {
+            const int64_t e = t / ways;
+            const int qq = (int)(t - e * ways);
+            const double2 sum = split_sum(d.W + e, d.nsplit, mn, qq, ways);
+            if (qq != 0) continue;
+            const int64_t i = e % d.m, j = e / d.m;
+            double2* cp = d.C + i + j * d.ldc;
+            double2 o = make_double2(d.alpha * sum.x, d.alpha * sum.y);
+            if (d.beta != 0.0) {
+                const double2 c = *cp;
+                o.x += d.beta * c.x;
+                o.y += d.beta * c.y;
+            }
+            *cp = o;
+        }
+    }
+}
+
+void k_gred(const GRedDesc* d, int count, int64_t maxn, hipStream_t st) {
+    ew_launch(RK_GRED, GRedP{d, count}, maxn, std::max(1, std::min(count, 64)), st);
+}
+
 void gettau_params(const double2* t1, const double2* t2, double2* tau, double s2, PairTab vv,
                    PairTab oo, int nv, void* gp) {
     *(GettauP*)gp = GettauP{t1, t2, tau, s2, vv.tab, vv.npair, oo.tab, oo.npair, nv};
@@ -756,6 +894,7 @@ size_t batch_param_size(int kind) {
         case RK_DIIS_PREP: return sizeof(DiisPrepP);
         case RK_GETTAU: return sizeof(GettauP);
         case RK_COPY: return sizeof(CopyP);
+        case RK_MAPBIL: return sizeof(MapBilP);
         default: return 0;
     }
 }
@@ -766,7 +905,7 @@ size_t batch_param_size(int kind) {
 int k_many(int kind, const void* dp, int count, int64_t gx, hipStream_t st) {
     if (count <= 0 || gx <= 0) return 0;
     const bool stride = kind == RK_LINCOMB || kind == RK_DIIS_PREP || kind == RK_GETTAU ||
-                        kind == RK_COPY;
+                        kind == RK_COPY || kind == RK_MAPBIL;
     const int64_t x = stride ? std::min<int64_t>(nblocks(gx),
                                                  std::max<int64_t>(1, 65536 / count))
                              : gx;
@@ -783,6 +922,7 @@ int k_many(int kind, const void* dp, int count, int64_t gx, hipStream_t st) {
         RCC_M(RK_DIIS_PREP, DiisPrepP)
         RCC_M(RK_GETTAU, GettauP)
         RCC_M(RK_COPY, CopyP)
+        RCC_M(RK_MAPBIL, MapBilP)
         default: return -1;
     }
 #undef RCC_M
@@ -909,6 +1049,10 @@ static size_t param_size(int kind) {
         case RK_BLOCKSET: return sizeof(BlockSetP);
         case RK_FILL: return sizeof(FillP);
         case RK_SPLIT_REDUCE: return sizeof(SplitReduceP);
+        case RK_MAPSUM: return sizeof(MapSumP);
+        case RK_MAPBIL: return sizeof(MapBilP);
+        case RK_BDENOM: return sizeof(BDenomP);
+        case RK_GRED: return sizeof(GRedP);
         default: return 0;
     }
 }
@@ -946,6 +1090,10 @@ static int ew_many_launch(int kind, const void* dp, int count, int64_t maxn, int
         RCC_EW(RK_BLOCKSET, BlockSetP)
         RCC_EW(RK_FILL, FillP)
         RCC_EW(RK_SPLIT_REDUCE, SplitReduceP)
+        RCC_EW(RK_MAPSUM, MapSumP)
+        RCC_EW(RK_MAPBIL, MapBilP)
+        RCC_EW(RK_BDENOM, BDenomP)
+        RCC_EW(RK_GRED, GRedP)
         default: return -1;
     }
 #undef RCC_EW
@@ -1016,7 +1164,8 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
                 same = r.g.opA == r0.g.opA && r.g.opB == r0.g.opB && r.g.tag == r0.g.tag &&
                        r.g.gauss == r0.g.gauss;
             if (same && r0.kind == RK_GROUPED)
-                same = r.opb == r0.opb && r.tag == r0.tag && r.gauss == r0.gauss;
+                same = r.opa == r0.opa && r.opb == r0.opb && r.tag == r0.tag &&
+                       r.gauss == r0.gauss;
             if (!same) {
                 err = "batch: handles record different term sequences (position " +
                       std::to_string(j) + ")";
@@ -1099,7 +1248,7 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
                 }
             }
             op.kind = RK_GROUPED;
-            op.opa = r0.kind == RK_GEMM ? r0.g.opA : OP_N;
+            op.opa = r0.kind == RK_GEMM ? r0.g.opA : r0.opa;
             op.opb = r0.kind == RK_GEMM ? r0.g.opB : r0.opb;
             op.tag = r0.kind == RK_GEMM ? r0.g.tag : r0.tag;
             op.gauss = r0.kind == RK_GEMM ? r0.g.gauss : r0.gauss;

@@ -24,12 +24,14 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 #include <algorithm>
 #include "rcc_internal.h"
+#include "rcc_blocked.h"
 #include "../../include/relccsd_hip.h"
 
 using namespace rcc;
@@ -190,6 +192,19 @@ struct rcc_handle {
     hipEvent_t mev[8] = {};
     double mp2_ms[6] = {0, 0, 0, 0, 0, 0};
 
+    // symmetry-blocked engine (symmetric handles whose irreps are their own
+    // inverse): the iteration's state and work live in its blocks; the C1
+    // arrays above serve the paths that run in the C1 layout (Lambda, (T),
+    // MP2 density), which take T from the engine first (c1t_stale)
+    BlkEngine* bk = nullptr;
+    bool bk_const_ok = false;
+    bool c1t_stale = false;
+    // the C1 working set (classes, their re-orderings, amplitudes and
+    // intermediates in the C1 layout) exists; blocked handles allocate it only
+    // for a path that runs in the C1 layout
+    bool c1_ok = false;
+    int64_t diis_len = 0;      // elements per DIIS vector allocated
+
     double2* T1() { return T; }
     double2* T2() { return T + nt1; }
     double2* S1() { return S; }
@@ -273,6 +288,24 @@ void ktick(rcc_t* h, int which, int phase) {
     } while (0)
 
 inline int64_t pair_index(int x, int y, int n) { return (int64_t)y * (2 * n - y - 1) / 2 + (x - y - 1); }
+
+// The amplitude vectors [T1 | T2] / [S1 | S2] the iteration works on: the
+// blocked engine's (symmetry-blocked layout) or the C1 arrays.
+inline int64_t amp_n1(const rcc_t* h) { return h->bk ? blk_size(h->bk, blk::A_T1) : h->nt1; }
+inline int64_t amp_n2(const rcc_t* h) { return h->bk ? blk_size(h->bk, blk::A_T2) : h->nt2; }
+inline int64_t amp_n(const rcc_t* h) { return amp_n1(h) + amp_n2(h); }
+inline double2* amp_T(rcc_t* h) { return h->bk ? blk_ptr(h->bk, blk::A_T) : h->T; }
+inline double2* amp_S(rcc_t* h) { return h->bk ? blk_ptr(h->bk, blk::A_S) : h->S; }
+inline double2* amp_TAU(rcc_t* h) { return h->bk ? blk_ptr(h->bk, blk::A_TAU) : h->TAU; }
+inline const double2* amp_FVO(rcc_t* h) { return h->bk ? blk_ptr(h->bk, blk::A_FVO) : h->FVO; }
+inline const double2* amp_VVOO(rcc_t* h) {
+    return h->bk ? blk_ptr(h->bk, blk::A_VVOO) : h->cls[RCC_VVOO];
+}
+
+// one program of the blocked engine on the handle's stream
+int brun(rcc_t* h, int prog) {
+    return blk_run(h->bk, prog, h->gemm_gauss, h->st, h->err) == 0 ? 0 : -1;
+}
 
 // Builds the derived constant layouts once all classes are present.
 int build_derived(rcc_t* h) {
@@ -376,6 +409,7 @@ int ensure_ring(rcc_t* h) {
 // BINTM's X(AK,IJ) = sum_{C>D} VOVV(AK,CD) TAU(CD,IJ)  (ccintm.F:772-788); also the
 // sum that T1EQN2 (cceqns.F:2601-2714) takes the (I,K) diagonal of.
 void invalidate_t(rcc_t* h) {
+    if (h->bk) blk_invalidate(h->bk);
     h->tau_s_ok = false;
     h->ring_s_ok = false;
     h->tau_ok = false;
@@ -881,10 +915,11 @@ int diis_advance(rcc_t* h) {
     return h->itc - 1;
 }
 
-int diis_begin(rcc_t* h, const double2* cur = nullptr, const double2* nw = nullptr) {
-    if (!cur) cur = h->T;
-    if (!nw) nw = h->S;
-    const int64_t n = h->nt1 + h->nt2;
+int diis_begin(rcc_t* h, const double2* cur = nullptr, const double2* nw = nullptr,
+               int64_t n = -1) {
+    if (!cur) cur = amp_T(h);
+    if (!nw) nw = amp_S(h);
+    if (n < 0) n = amp_n(h);
     hipStream_t st = h->st;
     const int c = diis_advance(h);
     double2* d = h->diis_err[c];
@@ -943,9 +978,9 @@ int diis_coefs(rcc_t* h, double* rms_out, std::vector<double>& coef) {
     return 0;
 }
 
-int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr) {
-    if (!nw) nw = h->S;
-    const int64_t n = h->nt1 + h->nt2;
+int diis_end(rcc_t* h, double* rms_out, double2* nw = nullptr, int64_t n = -1) {
+    if (!nw) nw = amp_S(h);
+    if (n < 0) n = amp_n(h);
     std::vector<double> coef;
     CK(diis_coefs(h, rms_out, coef));
     std::vector<const double2*> amps;
@@ -963,6 +998,18 @@ int do_diis(rcc_t* h, double* rms_out) {
 // CCENG (cceqns.F:2180-2270) on S: Re[zdotc(FVO,S1) + zdotc(VVOO,tau(S))]
 // CCENG (cceqns.F:2180-2270) on S, split like DIISX around one synchronisation.
 int energy_begin(rcc_t* h) {
+    if (h->bk) {
+        CK(brun(h, blk::P_TAUS));
+        const double2* fvo = amp_FVO(h);
+        const double2* vvoo = amp_VVOO(h);
+        k_zdotc_multi(&fvo, 1, amp_S(h), amp_n1(h), h->red_dst + RED_E, h->red_part, RED_BLOCKS,
+                      h->st);
+        k_zdotc_multi(&vvoo, 1, amp_TAU(h), amp_n2(h), h->red_dst + RED_E + 1, h->red_part,
+                      RED_BLOCKS, h->st);
+        HIPCHK(h, hipMemcpyAsync(h->hred + RED_E, h->red_dst + RED_E, 2 * sizeof(double2),
+                                 hipMemcpyDeviceToHost, h->st));
+        return 0;
+    }
     k_gettau(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, h->st);
     h->tau_ok = false;  // TAU now holds tau(S), not tau(T)
     const double2* fvo = h->FVO;
@@ -988,15 +1035,61 @@ int do_energy(rcc_t* h, double* ecc, double* eimag) {
     return 0;
 }
 
-int do_t1diag(rcc_t* h, const double2* t1, double* diag) {
+int do_t1diag(rcc_t* h, const double2* t1, int64_t n1, double* diag) {
     std::vector<double2> a;
-    CK(zdot_multi(h, {t1}, t1, h->nt1, a));
+    CK(zdot_multi(h, {t1}, t1, n1, a));
     *diag = std::sqrt(a[0].x / std::max(1, h->no));
     return 0;
 }
 
+// Engine constants: the class and Fock blocks, the orbital-energy
+// denominators and the integral re-orderings of the program, from the
+// handle's arrays (again after any of them changes).
+int bprep(rcc_t* h) {
+    if (h->bk_const_ok) return 0;
+    BlkEngine* E = h->bk;
+    c1_to_sym_dev(h, RCC_LAY_FOO, h->FOO, blk_ptr(E, blk::A_FOO));
+    c1_to_sym_dev(h, RCC_LAY_T1, h->FVO, blk_ptr(E, blk::A_FVO));
+    c1_to_sym_dev(h, RCC_LAY_FVV, h->FVV, blk_ptr(E, blk::A_FVV));
+    std::vector<double> eps(h->no + h->nv);
+    HIPCHK(h, hipMemcpyAsync(eps.data(), h->eps, eps.size() * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (blk_set_eps(E, eps.data(), h->err) != 0) return -1;
+    CK(brun(h, blk::P_CONST));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->bk_const_ok = true;
+    return 0;
+}
+
+// The C1 amplitudes from the engine's, for the paths that run in the C1
+// layout (Lambda, (T)).
+int c1_sync_t(rcc_t* h) {
+    if (!h->bk || !h->c1t_stale) return 0;
+    k_fill_zero(h->T, h->nt1 + h->nt2, h->st);
+    sym_to_c1_dev(h, RCC_LAY_T1, blk_ptr(h->bk, blk::A_T1), h->T1());
+    sym_to_c1_dev(h, RCC_LAY_T2, blk_ptr(h->bk, blk::A_T2), h->T2());
+    h->tau_ok = h->x_ok = h->ring_ok = false;
+    h->tau_s_ok = h->ring_s_ok = false;
+    h->c1t_stale = false;
+    return 0;
+}
+
+// the blocked term sequence, HOINTM .. BINTM, DENOM + ZCORE
+int blk_terms(rcc_t* h) {
+    CK(bprep(h));
+    for (int p : {blk::P_HOINTM, blk::P_HVINTM, blk::P_HOVINTM, blk::P_T1EQNS, blk::P_GOINTM,
+                  blk::P_GVINTM, blk::P_AINTM, blk::P_HINTM, blk::P_T2EQNS, blk::P_BINTM,
+                  blk::P_DENOM})
+        CK(brun(h, p));
+    return 0;
+}
+
+int run_terms_c1(rcc_t* h);
+
 // HOINTM .. BINTM, DENOM: device work only, no host synchronisation.
-int run_terms(rcc_t* h) {
+int run_terms(rcc_t* h) { return h->bk ? blk_terms(h) : run_terms_c1(h); }
+
+int run_terms_c1(rcc_t* h) {
     CK(do_hointm(h));
     CK(do_hvintm(h));
     CK(do_hovintm(h));
@@ -1027,7 +1120,7 @@ void drop_graph(rcc_t* h) {
 // call after that replays it with one launch.  The sequence is the same each
 // iteration: it starts from invalidated intermediates and fixed buffers.
 int terms_phase(rcc_t* h) {
-    CK(prep(h));
+    CK(h->bk ? bprep(h) : prep(h));
     if (!h->use_graph) return run_terms(h);
     if (h->graph_ok) {
         HIPCHK(h, hipGraphLaunch(h->graph_exec, h->st));
@@ -1065,7 +1158,8 @@ int finish_a(rcc_t* h) { return diis_begin(h); }
 int finish_b(rcc_t* h, double* rms) {
     CK(diis_end(h, rms));
     CK(energy_begin(h));
-    HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(amp_T(h), amp_S(h), amp_n(h) * 16, hipMemcpyDeviceToDevice, h->st));
+    if (h->bk) h->c1t_stale = true;
     HIPCHK(h, hipEventRecord(h->ev1, h->st));
     return 0;
 }
@@ -1096,6 +1190,11 @@ int do_iteration(rcc_t* h, double* ecc, double* rms) {
     return finish_c(h, ecc);
 }
 
+int alloc_c1_work(rcc_t* h);
+
+// Arrays every handle needs: Fock blocks, orbital energies, frozen flags,
+// the product workspace and the reduction buffers; the C1 working set too
+// unless the blocked engine runs the iteration.
 int alloc_all(rcc_t* h) {
     const int o = h->no, v = h->nv;
     const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
@@ -1110,10 +1209,32 @@ int alloc_all(rcc_t* h) {
     h->cls_rows[RCC_VOVO] = nvo;  h->cls_cols[RCC_VOVO] = nvo;
     h->cls_rows[RCC_VOVV] = nvo;  h->cls_cols[RCC_VOVV] = nvvt;
     h->cls_rows[RCC_VVVV] = nvvt; h->cls_cols[RCC_VVVV] = nvvt;
+    // K-split partial sums: room for a two-way split of the ladder's output
+    // (nvvt x noot), which the tuner may pick to fill the last round of tiles
+    h->ws_elems = std::max<int64_t>({(int64_t)1 << 22,
+                                     std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
+                                                       64 * (int64_t)v * v + (int64_t)256 * o * o),
+                                     2 * nvvt * noot});
+    CK(dalloc(h, &h->ws, h->ws_elems));
+    CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
+    CK(dalloc(h, &h->red_dst, RED_SLOTS));
+    HIPCHK(h, hipHostMalloc((void**)&h->hred, RED_SLOTS * sizeof(double2), hipHostMallocDefault));
+    return h->bk ? 0 : alloc_c1_work(h);
+}
+
+// The C1 working set: the classes (VVVV of a symmetric handle only as its
+// blocks), their derived layouts, T, S and the intermediates.
+int alloc_c1_work(rcc_t* h) {
+    if (h->c1_ok) return 0;
+    const int o = h->no, v = h->nv;
+    const int64_t nvo = h->nvo, noot = h->noot, nvvt = h->nvvt;
     for (int c = 0; c < 6; ++c) {
         // with symmetry the VVVV ladder runs on the blocks only (grouped GEMM)
         const bool blocks_only = c == RCC_VVVV && h->sym.nrep > 1;
         CK(dalloc(h, &h->cls[c], blocks_only ? 1 : h->cls_rows[c] * h->cls_cols[c]));
+        // off-symmetry blocks of the C1 arrays are never written
+        if (h->sym.nrep > 1 && !blocks_only)
+            k_fill_zero(h->cls[c], h->cls_rows[c] * h->cls_cols[c], h->st);
     }
     CK(dalloc(h, &h->VVOO_E2, nvvt * o * o));
     CK(dalloc(h, &h->VVOO_F1, (int64_t)v * v * noot));
@@ -1146,25 +1267,26 @@ int alloc_all(rcc_t* h) {
     CK(dalloc(h, &h->Y1, nvo * o * o));
     CK(dalloc(h, &h->YAZ, noot * o * o));
     CK(dalloc(h, &h->SMALL, (int64_t)o * o + nvo));
-    // K-split partial sums: room for a two-way split of the ladder's output
-    // (nvvt x noot), which the tuner may pick to fill the last round of tiles
-    h->ws_elems = std::max<int64_t>({(int64_t)1 << 22,
-                                     std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
-                                                       64 * (int64_t)v * v + (int64_t)256 * o * o),
-                                     2 * nvvt * noot});
-    CK(dalloc(h, &h->ws, h->ws_elems));
-    CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
-    CK(dalloc(h, &h->red_dst, RED_SLOTS));
-    HIPCHK(h, hipHostMalloc((void**)&h->hred, RED_SLOTS * sizeof(double2), hipHostMallocDefault));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->c1_ok = true;
     return 0;
 }
 
-int alloc_diis(rcc_t* h) {
-    const int64_t namp = h->nt1 + h->nt2;
+// DIIS store of MAXDIM error and amplitude vectors of `need` elements (the
+// iteration's amplitude vector by default; the Lambda solve asks for C1 ones)
+int alloc_diis(rcc_t* h, int64_t need = -1) {
+    if (need < 0) need = h->bk ? amp_n(h) : h->nt1 + h->nt2;
+    if (need > h->diis_len) {
+        for (double2* p : h->diis_err) dfree(h, p);
+        for (double2* p : h->diis_amp) dfree(h, p);
+        h->diis_err.clear();
+        h->diis_amp.clear();
+        h->diis_len = need;
+    }
     while ((int)h->diis_err.size() < h->maxdim) {
         double2 *e = nullptr, *a = nullptr;
-        CK(dalloc(h, &e, namp));
-        CK(dalloc(h, &a, namp));
+        CK(dalloc(h, &e, h->diis_len));
+        CK(dalloc(h, &a, h->diis_len));
         h->diis_err.push_back(e);
         h->diis_amp.push_back(a);
     }
@@ -1423,19 +1545,26 @@ int run_groups(rcc_t* h, int opb, int tag, const ZGroup* dG, int ng, int64_t nbl
     return 0;
 }
 
-// Block arrays and grouped-GEMM descriptors for nrep > 1 (after alloc_all).
-int setup_grouped(rcc_t* h) {
+// The VVVV and VOVV irrep blocks of a symmetric handle (nrep > 1).
+int alloc_blocks(rcc_t* h) {
     const SymInfo& S = h->sym;
     bool selfinv = true;
     for (int i = 0; i < S.nrep; ++i) selfinv = selfinv && S.ptab[i + S.nrep * i] == 0;
     h->ring_grouped = selfinv;
     CK(dalloc(h, &h->VVVV_s, lay_total(h, RCC_VVVV)));
     CK(dalloc(h, &h->VOVV_s, lay_total(h, RCC_VOVV)));
+    k_fill_zero(h->VVVV_s, lay_total(h, RCC_VVVV), h->st);
+    k_fill_zero(h->VOVV_s, lay_total(h, RCC_VOVV), h->st);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// Block copies and grouped-GEMM descriptors of the C1 working layout of a
+// symmetric handle (after alloc_blocks and alloc_c1_work).
+int setup_grouped(rcc_t* h) {
     CK(dalloc(h, &h->TAU_s, lay_total(h, RCC_LAY_T2)));
     CK(dalloc(h, &h->S2_s, lay_total(h, RCC_LAY_T2)));
     CK(dalloc(h, &h->X_s, lay_total(h, RCC_VOOO)));
-    k_fill_zero(h->VVVV_s, lay_total(h, RCC_VVVV), h->st);
-    k_fill_zero(h->VOVV_s, lay_total(h, RCC_VOVV), h->st);
     CK(make_groups(h, RCC_VVVV, h->VVVV_s, RCC_LAY_T2, h->TAU_s, RCC_LAY_T2, h->S2_s, false,
                    cplx(1, 0), cplx(1, 0), &h->g_ladder, &h->ng_ladder, &h->nb_ladder, &h->hg[0]));
     CK(make_groups(h, RCC_VOVV, h->VOVV_s, RCC_LAY_T2, h->TAU_s, RCC_VOOO, h->X_s, false,
@@ -1454,6 +1583,23 @@ int setup_grouped(rcc_t* h) {
                        &h->hg[3]));
     }
     HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+// The C1 working set of a blocked handle, for a path that runs in the C1
+// layout: allocated on first use, classes filled from the engine's blocks,
+// T taken from the engine (c1_sync_t) before use.
+int ensure_c1(rcc_t* h) {
+    if (h->c1_ok) return 0;
+    CK(alloc_c1_work(h));
+    if (h->sym.nrep > 1) CK(setup_grouped(h));
+    if (h->bk) {
+        for (int c : {RCC_OOOO, RCC_VOOO, RCC_VVOO, RCC_VOVO, RCC_VOVV})
+            sym_to_c1_dev(h, c, blk_ptr(h->bk, blk::A_OOOO + c), h->cls[c]);
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        h->c1t_stale = true;
+    }
+    h->derived_ok = false;
     return 0;
 }
 
@@ -1502,7 +1648,7 @@ int lam_check(rcc_t* h) {
         h->err = "Lambda equations: precision 0 (FP64) only";
         return -1;
     }
-    return 0;
+    return ensure_c1(h);
 }
 
 int lam_alloc(rcc_t* h) {
@@ -1580,8 +1726,9 @@ int lam_setup(rcc_t* h) {
     CK(lam_check(h));
     CK(prep(h));
     CK(lam_alloc(h));
+    CK(c1_sync_t(h));
     invalidate_t(h);
-    CK(run_terms(h));
+    CK(run_terms_c1(h));
     const int o = h->no, v = h->nv;
     k_conj(h->nvo, h->FVO, h->LW, h->st);
     k_axpby(h->nvo, cplx(1, 0), h->HOV, cplx(-2, 0), h->LW, h->st);
@@ -1790,7 +1937,7 @@ void reset_diis(rcc_t* h) {
 // for a later CCEQN.
 int lam_solve(rcc_t* h, int* nit, double* rms_out) {
     CK(lam_setup(h));
-    CK(alloc_diis(h));
+    CK(alloc_diis(h, h->nt1 + h->nt2));
     const int64_t nt1 = h->nt1, nt2 = h->nt2;
     hipStream_t st = h->st;
     const int* frz = h->any_frozen ? h->frozen : nullptr;
@@ -1806,9 +1953,9 @@ int lam_solve(rcc_t* h, int* nit, double* rms_out) {
         HIPCHK(h, hipEventRecord(h->ev0, st));
         CK(lam_sweep(h, h->LAM, h->LAMS, true));
         k_denom(h->LAMS, h->LAMS + nt1, h->eps, frz, h->vv, h->oo, h->nv, h->no, st);
-        CK(diis_begin(h, h->LAM, h->LAMS));
+        CK(diis_begin(h, h->LAM, h->LAMS, nt1 + nt2));
         HIPCHK(h, hipStreamSynchronize(st));
-        CK(diis_end(h, &r, h->LAMS));
+        CK(diis_end(h, &r, h->LAMS, nt1 + nt2));
         HIPCHK(h, hipMemcpyAsync(h->LAM, h->LAMS, (size_t)(nt1 + nt2) * 16,
                                  hipMemcpyDeviceToDevice, st));
         HIPCHK(h, hipEventRecord(h->ev1, st));
@@ -2011,7 +2158,7 @@ bool batch_groupable(rcc_t* const* hs, int n) {
     for (int i = 0; i < n; ++i) {
         rcc_t* h = hs[i];
         if (h->precision != 0 || h->gemm_gauss != hs[0]->gemm_gauss) return false;
-        if (h->sym.nrep > 1 && !h->ring_grouped) return false;
+        if (!h->bk && h->sym.nrep > 1 && !h->ring_grouped) return false;
     }
     return n > 0;
 }
@@ -2059,7 +2206,7 @@ int get_plan(rcc_t* const* hs, int n, BatchPlan** out) {
     std::vector<std::vector<Rec>> recs((size_t)n);
     for (int i = 0; i < n; ++i) {
         rcc_t* h = hs[i];
-        if (prep(h) != 0) return fail(h->err);
+        if ((h->bk ? bprep(h) : prep(h)) != 0) return fail(h->err);
         HIPCHK(h, hipStreamSynchronize(h->st));
         invalidate_t(h);
         h->capturing = true;
@@ -2184,9 +2331,9 @@ int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
     for (int i = 0; i < nh; ++i) {
         rcc_t* h = p->hs[i];
         cpos[i] = diis_advance(h);
-        nmax = std::max<int64_t>(nmax, h->nt1 + h->nt2);
+        nmax = std::max<int64_t>(nmax, amp_n(h));
         itmax = std::max<int64_t>(itmax, h->it);
-        bmax = std::max(bmax, red_blocks(h->nt1 + h->nt2, RED_BLOCKS));
+        bmax = std::max(bmax, red_blocks(amp_n(h), RED_BLOCKS));
     }
     // ---- DIISX reductions
     {
@@ -2194,7 +2341,7 @@ int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
         std::vector<std::vector<unsigned char>> sn_part(nh), sn_fin(nh), zd_part(nh), zd_fin(nh);
         for (int i = 0; i < nh; ++i) {
             rcc_t* h = p->hs[i];
-            const int64_t n = h->nt1 + h->nt2;
+            const int64_t n = amp_n(h);
             double2* d = h->diis_err[cpos[i]];
             sn_part[i].resize(batch_param_size(RK_SUMNORM_PART));
             sn_fin[i].resize(batch_param_size(RK_FINISH));
@@ -2207,8 +2354,8 @@ int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
                         zd_part[i].data(), zd_fin[i].data());
         }
         fb.add(RK_DIIS_PREP, per_handle(p, RK_DIIS_PREP, [&](rcc_t* h, int i, void* q) {
-                   diis_prep_params(h->S, h->T, h->diis_err[cpos[i]], h->diis_amp[cpos[i]],
-                                    h->nt1 + h->nt2, q);
+                   diis_prep_params(amp_S(h), amp_T(h), h->diis_err[cpos[i]],
+                                    h->diis_amp[cpos[i]], amp_n(h), q);
                }), nmax);
         fb.add(RK_SUMNORM_PART, sn_part, bmax);
         fb.add(RK_FINISH, sn_fin, 2);
@@ -2234,33 +2381,37 @@ int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
         int64_t n2 = 0;
         for (int i = 0; i < nh; ++i) {
             rcc_t* h = p->hs[i];
-            const double2* fvo = h->FVO;
-            const double2* vvoo = h->cls[RCC_VVOO];
+            const double2* fvo = amp_FVO(h);
+            const double2* vvoo = amp_VVOO(h);
             for (auto* v : {&e1p[i], &e2p[i]}) v->resize(batch_param_size(RK_ZDOT_PART));
             for (auto* v : {&e1f[i], &e2f[i]}) v->resize(batch_param_size(RK_FINISH));
-            zdot_params(&fvo, 1, h->S1(), h->nt1, h->red_dst + RED_E, h->red_part, RED_BLOCKS,
-                        e1p[i].data(), e1f[i].data());
-            zdot_params(&vvoo, 1, h->TAU, h->nt2, h->red_dst + RED_E + 1, h->red_part,
+            zdot_params(&fvo, 1, amp_S(h), amp_n1(h), h->red_dst + RED_E, h->red_part,
+                        RED_BLOCKS, e1p[i].data(), e1f[i].data());
+            zdot_params(&vvoo, 1, amp_TAU(h), amp_n2(h), h->red_dst + RED_E + 1, h->red_part,
                         RED_BLOCKS, e2p[i].data(), e2f[i].data());
-            b1 = std::max(b1, red_blocks(h->nt1, RED_BLOCKS));
-            b2 = std::max(b2, red_blocks(h->nt2, RED_BLOCKS));
-            n2 = std::max<int64_t>(n2, h->nt2);
+            b1 = std::max(b1, red_blocks(amp_n1(h), RED_BLOCKS));
+            b2 = std::max(b2, red_blocks(amp_n2(h), RED_BLOCKS));
+            n2 = std::max<int64_t>(n2, amp_n2(h));
             h->tau_ok = false;     // TAU will hold tau(S), not tau(T)
         }
         fb.add(RK_LINCOMB, per_handle(p, RK_LINCOMB, [&](rcc_t* h, int i, void* q) {
                    std::vector<const double2*> amps(h->diis_amp.begin(),
                                                     h->diis_amp.begin() + h->it);
-                   lincomb_params(amps.data(), coef[i].data(), h->it, h->S, h->nt1 + h->nt2, q);
+                   lincomb_params(amps.data(), coef[i].data(), h->it, amp_S(h), amp_n(h), q);
                }), nmax);
-        fb.add(RK_GETTAU, per_handle(p, RK_GETTAU, [&](rcc_t* h, int, void* q) {
-                   gettau_params(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, q);
+        // tau(S): the C1 kernel, or the blocked engine's bilinear gather
+        const int tk = h0->bk ? RK_MAPBIL : RK_GETTAU;
+        fb.add(tk, per_handle(p, tk, [&](rcc_t* h, int, void* q) {
+                   if (h->bk) blk_tau_s_params(h->bk, q);
+                   else gettau_params(h->S1(), h->S2(), h->TAU, 1.0, h->vv, h->oo, h->nv, q);
                }), n2);
         fb.add(RK_ZDOT_PART, e1p, b1);
         fb.add(RK_FINISH, e1f, 1);
         fb.add(RK_ZDOT_PART, e2p, b2);
         fb.add(RK_FINISH, e2f, 1);
         fb.add(RK_COPY, per_handle(p, RK_COPY, [&](rcc_t* h, int, void* q) {
-                   copy_params(h->nt1 + h->nt2, h->S, h->T, q);
+                   copy_params(amp_n(h), amp_S(h), amp_T(h), q);
+                   if (h->bk) h->c1t_stale = true;
                }), nmax);
         fb.add(RK_COPY, per_handle(p, RK_COPY, [&](rcc_t* h, int i, void* q) {
                    copy_params(2, h->red_dst + RED_E, p->dslots + (size_t)i * RED_SLOTS + RED_E, q);
@@ -2392,15 +2543,41 @@ int rcc_create(const rcc_sym_t* sym, rcc_t** out) {
     for (int w = 0; w < 4; ++w)
         for (int p = 0; p < 2; ++p) HIPCHK(h, hipEventCreate(&h->kev[w][p]));
     CK(build_sym(h, sym));
+    if (sym->nrep > 1) {
+        // the blocked engine, unless RCC_BLOCKED=0 or the irreps are not their
+        // own inverse (then the C1 working layout with grouped dominant products)
+        const char* env = std::getenv("RCC_BLOCKED");
+        if (!(env && env[0] == '0')) {
+            blk::SymIn si;
+            si.nrep = sym->nrep;
+            si.no = h->sym.no;
+            si.nv = h->sym.nv;
+            si.ptab = h->sym.ptab;
+            si.frozen.assign(h->no, 0);
+            if (sym->frozen)
+                for (int i = 0; i < h->no; ++i) si.frozen[i] = sym->frozen[i] ? 1 : 0;
+            const int rc = blk_create(si, &h->bk, h->err);
+            if (rc < 0) return -1;
+            if (rc > 0) {
+                h->bk = nullptr;
+                h->err.clear();
+            }
+        }
+    }
     CK(alloc_all(h));
     if (sym->nrep > 1) {
-        CK(setup_grouped(h));
+        CK(alloc_blocks(h));
+        if (h->bk) {
+            if (blk_bind(h->bk, blk::A_VVVV, h->VVVV_s, h->err) != 0 ||
+                blk_bind(h->bk, blk::A_VOVV, h->VOVV_s, h->err) != 0)
+                return -1;
+        } else {
+            CK(setup_grouped(h));
+        }
         // off-symmetry blocks of the C1 arrays are never written
         k_fill_zero(h->FOO, (int64_t)h->no * h->no, h->st);
         k_fill_zero(h->FVO, h->nvo, h->st);
         k_fill_zero(h->FVV, (int64_t)h->nv * h->nv, h->st);
-        for (int c = 0; c < 6; ++c)
-            if (c != RCC_VVVV) k_fill_zero(h->cls[c], h->cls_rows[c] * h->cls_cols[c], h->st);
         HIPCHK(h, hipStreamSynchronize(h->st));
     }
     // pair tables
@@ -2432,6 +2609,7 @@ void rcc_destroy(rcc_t* h) {
     if (h->st) (void)hipStreamSynchronize(h->st);
     drop_graph(h);
     if (h->t3) t3_free(h->t3);
+    if (h->bk) blk_destroy(h->bk);
     if (h->hred) (void)hipHostFree(h->hred);
     for (void* p : h->allocs) (void)hipFree(p);
     for (int w = 0; w < 4; ++w)
@@ -2510,6 +2688,7 @@ int rcc_set_fock(rcc_t* h, const double* foo, const double* fvo, const double* f
         CK(sym_to_c1(h, RCC_LAY_FVV, fvv, hipMemcpyHostToDevice, h->FVV, (int64_t)h->nv * h->nv));
         HIPCHK(h, hipMemcpy(h->eps, eps, (size_t)(h->no + h->nv) * 8, hipMemcpyHostToDevice));
         h->derived_ok = false;
+        h->bk_const_ok = false;
         return 0;
     }
     CK(upload(h, h->FOO, foo, (int64_t)h->no * h->no, hipMemcpyHostToDevice));
@@ -2553,17 +2732,20 @@ static int set_ints_common(rcc_t* h, int cls, int irrep, int64_t col0, int64_t n
             dsrc = h->stage;
         }
         const SymInfo& S = h->sym;
-        if (cls == RCC_VVVV || cls == RCC_VOVV) {
-            double2* blk = (cls == RCC_VVVV ? h->VVVV_s : h->VOVV_s) + lay_offset(h, rk, ck, irrep);
-            HIPCHK(h, hipMemcpyAsync(blk + col0 * rows, dsrc, (size_t)(rows * ncols) * 16,
-                                     hipMemcpyDeviceToDevice, h->st));
-        }
-        if (cls != RCC_VVVV)
+        // the block copies: VVVV and VOVV always, the other classes in the
+        // blocked engine (whose layouts are these)
+        double2* blk = cls == RCC_VVVV ? h->VVVV_s : cls == RCC_VOVV ? h->VOVV_s
+                       : h->bk ? blk_ptr(h->bk, blk::A_OOOO + cls) : nullptr;
+        if (blk)
+            HIPCHK(h, hipMemcpyAsync(blk + lay_offset(h, rk, ck, irrep) + col0 * rows, dsrc,
+                                     (size_t)(rows * ncols) * 16, hipMemcpyDeviceToDevice, h->st));
+        if (cls != RCC_VVVV && h->cls[cls])
             k_scatter_block(dsrc, rows, ncols, S.dmap + S.doff[rk][irrep],
                             S.dmap + S.doff[ck][irrep] + col0, h->cls[cls], ld, h->st);
         HIPCHK(h, hipStreamSynchronize(h->st));
     }
     h->derived_ok = false;
+    h->bk_const_ok = false;
     h->split_const_ok = false;
     if (h->graph_ok || h->graph_warm) drop_graph(h);
     return 0;
@@ -2580,7 +2762,14 @@ int rcc_set_ints_device(rcc_t* h, int cls, int irrep, int64_t col0, int64_t ncol
 
 int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
     if (!h) return -1;
-    if (h->sym.nrep > 1) {
+    if (h->bk) {
+        // the blocked layout is the symmetry layout of the interface
+        if (t1) HIPCHK(h, hipMemcpy(blk_ptr(h->bk, blk::A_T1), t1, amp_n1(h) * 16,
+                                    hipMemcpyHostToDevice));
+        if (t2) HIPCHK(h, hipMemcpy(blk_ptr(h->bk, blk::A_T2), t2, amp_n2(h) * 16,
+                                    hipMemcpyHostToDevice));
+        h->c1t_stale = true;
+    } else if (h->sym.nrep > 1) {
         if (t1) CK(sym_to_c1(h, RCC_LAY_T1, t1, hipMemcpyHostToDevice, h->T1(), h->nt1));
         if (t2) CK(sym_to_c1(h, RCC_LAY_T2, t2, hipMemcpyHostToDevice, h->T2(), h->nt2));
     } else {
@@ -2599,6 +2788,13 @@ int rcc_set_amplitudes(rcc_t* h, const double* t1, const double* t2) {
 int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2) {
     if (!h) return -1;
     HIPCHK(h, hipStreamSynchronize(h->st));
+    if (h->bk) {
+        if (t1) HIPCHK(h, hipMemcpy(t1, blk_ptr(h->bk, blk::A_T1), amp_n1(h) * 16,
+                                    hipMemcpyDeviceToHost));
+        if (t2) HIPCHK(h, hipMemcpy(t2, blk_ptr(h->bk, blk::A_T2), amp_n2(h) * 16,
+                                    hipMemcpyDeviceToHost));
+        return 0;
+    }
     if (h->sym.nrep > 1) {
         if (t1) CK(c1_to_sym_host(h, RCC_LAY_T1, h->T1(), t1));
         if (t2) CK(c1_to_sym_host(h, RCC_LAY_T2, h->T2(), t2));
@@ -2611,6 +2807,41 @@ int rcc_get_amplitudes(rcc_t* h, double* t1, double* t2) {
 
 int rcc_get_array(rcc_t* h, int which, double* out, int64_t n) {
     if (!h) return -1;
+    if (h->bk) {
+        int id = -1;
+        switch (which) {
+            case RCC_ARR_HO: id = blk::A_HO; break;
+            case RCC_ARR_HV: id = blk::A_HV; break;
+            case RCC_ARR_HOV: id = blk::A_HOV; break;
+            case RCC_ARR_GO: id = blk::A_GO; break;
+            case RCC_ARR_GV: id = blk::A_GV; break;
+            case RCC_ARR_AZ: id = blk::A_AZ; break;
+            case RCC_ARR_H: id = blk::A_H; break;
+            case RCC_ARR_S1: id = blk::A_S1; break;
+            case RCC_ARR_S2: id = blk::A_S2; break;
+            case RCC_ARR_T1: id = blk::A_T1; break;
+            case RCC_ARR_T2: id = blk::A_T2; break;
+            case RCC_ARR_TAU: id = blk::A_TAU; break;
+            case RCC_ARR_OOOO: id = blk::A_OOOO; break;
+            case RCC_ARR_VOOO: id = blk::A_VOOO; break;
+            case RCC_ARR_VVOO: id = blk::A_VVOO; break;
+            case RCC_ARR_VOVO: id = blk::A_VOVO; break;
+            case RCC_ARR_VOVV: id = blk::A_VOVV; break;
+            case RCC_ARR_VVVV: id = blk::A_VVVV; break;
+            default: break;
+        }
+        if (id >= 0) {
+            const int64_t need = blk_size(h->bk, id);
+            if (n != need) {
+                h->err = "rcc_get_array: size mismatch (expected " + std::to_string(need) + ")";
+                return -1;
+            }
+            HIPCHK(h, hipStreamSynchronize(h->st));
+            HIPCHK(h, hipMemcpy(out, blk_ptr(h->bk, id), (size_t)need * 16,
+                                hipMemcpyDeviceToHost));
+            return 0;
+        }
+    }
     const double2* src = nullptr;
     int64_t len = 0;
     const int o = h->no, v = h->nv;
@@ -2741,6 +2972,21 @@ int rcc_layout_block(rcc_t* h, int lay, int block, int64_t* rows, int64_t* cols,
 
 int rcc_mp2(rcc_t* h, double* emp2) {
     if (!h) return -1;
+    if (h->bk) {
+        CK(bprep(h));
+        CK(brun(h, blk::P_MP2));
+        h->c1t_stale = true;
+        invalidate_t(h);
+        std::vector<double2> a, b;
+        CK(zdot_multi(h, {amp_FVO(h)}, blk_ptr(h->bk, blk::A_T1), amp_n1(h), a));
+        CK(zdot_multi(h, {amp_VVOO(h)}, blk_ptr(h->bk, blk::A_T2), amp_n2(h), b));
+        if (emp2) *emp2 = a[0].x + b[0].x;
+        h->nit = 0;
+        h->it = 0;
+        h->itc = 0;
+        std::fill(h->bb2.begin(), h->bb2.end(), 0.0);
+        return 0;
+    }
     CK(prep(h));
     HIPCHK(h, hipMemcpyAsync(h->T1(), h->FVO, h->nt1 * 16, hipMemcpyDeviceToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(h->T2(), h->cls[RCC_VVOO], h->nt2 * 16, hipMemcpyDeviceToDevice, h->st));
@@ -2758,28 +3004,54 @@ int rcc_mp2(rcc_t* h, double* emp2) {
     return 0;
 }
 
-#define TERM(name, call) \
-    int name(rcc_t* h) { if (!h) return -1; return call; }
-TERM(rcc_hointm, do_hointm(h))
-TERM(rcc_hvintm, do_hvintm(h))
-TERM(rcc_hovintm, do_hovintm(h))
-TERM(rcc_t1eqns, do_t1eqns(h))
-TERM(rcc_gointm, do_gointm(h))
-TERM(rcc_gvintm, do_gvintm(h))
-TERM(rcc_aintm, do_aintm(h))
-TERM(rcc_hintm, do_hintm(h, false))
-TERM(rcc_t2eqns, do_t2eqns(h))
-TERM(rcc_bintm, do_bintm(h))
+// one term: the blocked engine's program or the C1 routine
+#define TERM(name, prog, call)                                  \
+    int name(rcc_t* h) {                                        \
+        if (!h) return -1;                                      \
+        if (h->bk) {                                            \
+            CK(bprep(h));                                       \
+            return brun(h, prog);                               \
+        }                                                       \
+        return call;                                            \
+    }
+TERM(rcc_hointm, blk::P_HOINTM, do_hointm(h))
+TERM(rcc_hvintm, blk::P_HVINTM, do_hvintm(h))
+TERM(rcc_hovintm, blk::P_HOVINTM, do_hovintm(h))
+TERM(rcc_t1eqns, blk::P_T1EQNS, do_t1eqns(h))
+TERM(rcc_gointm, blk::P_GOINTM, do_gointm(h))
+TERM(rcc_gvintm, blk::P_GVINTM, do_gvintm(h))
+TERM(rcc_aintm, blk::P_AINTM, do_aintm(h))
+TERM(rcc_t2eqns, blk::P_T2EQNS, do_t2eqns(h))
+TERM(rcc_bintm, blk::P_BINTM, do_bintm(h))
 #undef TERM
+
+// HINTM as relccsd calls it: HAKCI formed here, not taken from GVINTM
+int rcc_hintm(rcc_t* h) {
+    if (!h) return -1;
+    if (h->bk) {
+        CK(bprep(h));
+        CK(brun(h, blk::P_HINTM_HAKCI));
+        return brun(h, blk::P_HINTM);
+    }
+    return do_hintm(h, false);
+}
 
 int rcc_denom(rcc_t* h) {
     if (!h) return -1;
+    if (h->bk) {
+        CK(bprep(h));
+        return brun(h, blk::P_DIVIDE);
+    }
     k_denom(h->S1(), h->S2(), h->eps, nullptr, h->vv, h->oo, h->nv, h->no, h->st);
     return 0;
 }
 
 int rcc_zcore(rcc_t* h) {
     if (!h) return -1;
+    if (h->bk) {
+        CK(bprep(h));
+        return brun(h, blk::P_ZCORE);
+    }
     if (h->any_frozen)
         k_denom(h->S1(), h->S2(), nullptr, h->frozen, h->vv, h->oo, h->nv, h->no, h->st);
     return 0;
@@ -2789,11 +3061,14 @@ int rcc_diis(rcc_t* h, double* rms) { return h ? do_diis(h, rms) : -1; }
 
 int rcc_energy(rcc_t* h, double* ecc) { return h ? do_energy(h, ecc, nullptr) : -1; }
 
-int rcc_t1diag(rcc_t* h, double* diag) { return h ? do_t1diag(h, h->S1(), diag) : -1; }
+int rcc_t1diag(rcc_t* h, double* diag) {
+    return h ? do_t1diag(h, amp_S(h), amp_n1(h), diag) : -1;
+}
 
 int rcc_accept(rcc_t* h) {
     if (!h) return -1;
-    HIPCHK(h, hipMemcpyAsync(h->T, h->S, (h->nt1 + h->nt2) * 16, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(amp_T(h), amp_S(h), amp_n(h) * 16, hipMemcpyDeviceToDevice, h->st));
+    if (h->bk) h->c1t_stale = true;
     invalidate_t(h);
     return 0;
 }
@@ -2852,7 +3127,18 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
         CK(batch_program_phase(p, graph));
         return batch_finish(p, nit, ecc, rms);
     }
-    CK(batch_terms_grouped(p));
+    bool any_bk = false;
+    for (int i = 0; i < n; ++i) any_bk = any_bk || hs[i]->bk;
+    if (any_bk) {
+        // blocked handles (alone or among others) without a compiled program:
+        // every handle's own term phase
+        for (int i = 0; i < n; ++i) {
+            HIPCHK(hs[i], hipEventRecord(hs[i]->ev0, hs[i]->st));
+            CK(terms_phase(hs[i]));
+        }
+    } else {
+        CK(batch_terms_grouped(p));
+    }
     for (int i = 0; i < n; ++i) CK(finish_a(hs[i]));
     for (int i = 0; i < n; ++i) HIPCHK(hs[i], hipStreamSynchronize(hs[i]->st));
     for (int i = 0; i < n; ++i) CK(finish_b(hs[i], rms ? rms + i : nullptr));
@@ -2989,6 +3275,7 @@ static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int
         krt[nkr + i] = a - 1;
         krt[nkr - i] = b - 1;
     }
+    CK(ensure_c1(h));
     int *d_spos = nullptr, *d_kr = nullptr;
     HIPCHK(h, hipMalloc((void**)&d_spos, spos.size() * 4));
     HIPCHK(h, hipMalloc((void**)&d_kr, krt.size() * 4));
@@ -3050,6 +3337,9 @@ static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int
     if (!rc && h->sym.nrep > 1) {
         c1_to_sym_dev(h, RCC_VOVV, h->cls[RCC_VOVV], h->VOVV_s);
         c1_to_sym_dev(h, RCC_VVVV, vvvv_c1, h->VVVV_s);
+        if (h->bk)
+            for (int c : {RCC_OOOO, RCC_VOOO, RCC_VVOO, RCC_VOVO})
+                c1_to_sym_dev(h, c, h->cls[c], blk_ptr(h->bk, blk::A_OOOO + c));
         (void)hipStreamSynchronize(h->st);
     }
     (void)hipFree(d_spos);
@@ -3057,6 +3347,7 @@ static int sort_upload_and_run(rcc_t* h, const std::vector<int>& spos, const int
     if (vvvv_c1) (void)hipFree(vvvv_c1);
     if (rc) return -1;
     h->derived_ok = false;
+    h->bk_const_ok = false;
     h->split_const_ok = false;
     h->tau_ok = h->x_ok = h->ring_ok = false;
     if (h->graph_ok || h->graph_warm) drop_graph(h);
@@ -3187,6 +3478,7 @@ int rcc_read_molfdir_opts(rcc_t* h, const char* mrconee, const char* mdcint, con
         }
     }
     h->derived_ok = false;
+    h->bk_const_ok = false;
     h->last_sort_ms = ms;
     if (e0e1) {
         e0e1[0] = e01[0];
@@ -3359,6 +3651,7 @@ int zdiis_step(rcc_t* h, ZDiis& z, double2** err, double2** amp, const double2* 
 }  // namespace
 
 int rcc_mp2_density(rcc_t* h, int maxit, int maxdim, int ntol, int* nit, double* rms) {
+    if (h) CK(ensure_c1(h));
     if (!h) return -1;
     h->mp2d_ok = false;
     // Symmetry-blocked handles run in the C1 working layout, where every
@@ -3542,12 +3835,14 @@ int64_t rcc_t3_triples(rcc_t* h) { return h ? t3_triples(h->no) : -1; }
 
 int rcc_t3corr_part(rcc_t* h, int64_t first, int64_t count, double* et) {
     if (!h || !et) return -1;
+    CK(ensure_c1(h));
     for (int c : {RCC_VOOO, RCC_VVOO, RCC_VOVV})
         if (!h->cls[c]) {
             h->err = "rcc_t3corr: integral class " + std::to_string(c) + " not set";
             return -1;
         }
     if (!h->t3) h->t3 = t3_new();
+    CK(c1_sync_t(h));
     T3Input in;
     in.no = h->no;
     in.nv = h->nv;
