@@ -419,7 +419,6 @@ inline int product(Plan& P, const char* name, int opa, int opb, View A, View B, 
         (void)R;
         return off;
     };
-    const int64_t WSPLIT_MIN_K = 512;
     int64_t tiles = 0;
     struct Raw { int g; int64_t m, n, k; };
     std::vector<Raw> raw;
@@ -446,19 +445,47 @@ inline int product(Plan& P, const char* name, int opa, int opb, View A, View B, 
         nx = std::max(nx, r.n);
     }
     op.small = mx <= 48 || nx <= 32 || tiles < 64;
-    // K-split: long k and too few tiles to fill the chip
-    const int64_t tiles_eff = op.small ? [&] {
-        int64_t t = 0;
-        for (const Raw& r : raw) t += ((r.m + 31) / 32) * ((r.n + 31) / 32);
-        return t;
-    }() : tiles;
-    int64_t wsoff = 0;
-    for (const Raw& r : raw) {
-        int nsplit = 1;
-        if (tiles_eff < 512 && r.k >= 2 * WSPLIT_MIN_K) {
-            const int64_t want = (1024 + tiles_eff - 1) / std::max<int64_t>(tiles_eff, 1);
-            nsplit = (int)std::max<int64_t>(1, std::min<int64_t>({want, r.k / WSPLIT_MIN_K, 64}));
+    // K-split: each group's k cut into chunks, a split count s shared out in
+    // proportion to the groups' k, chosen to minimise the makespan bound
+    // max(longest workgroup, total work / workgroup slots), with a fixed cost
+    // per workgroup (in k units) for its prologue and epilogue
+    const int64_t slots = op.small ? 2048 : 512, ovh = 64;
+    int64_t kmax = 1;
+    for (const Raw& r : raw) kmax = std::max(kmax, r.k);
+    auto tiles_of = [&](const Raw& r) {
+        return op.small ? ((r.m + 31) / 32) * ((r.n + 31) / 32)
+                        : ((r.m + 95) / 96) * ((r.n + 63) / 64);
+    };
+    auto splits_for = [&](int sc, std::vector<int64_t>& chunk) {
+        chunk.assign(raw.size(), 0);
+        double work = 0.0;
+        int64_t longest = 0;
+        for (size_t i = 0; i < raw.size(); ++i) {
+            const Raw& r = raw[i];
+            int64_t si = std::max<int64_t>(1, (sc * r.k + kmax / 2) / kmax);
+            si = std::min<int64_t>(si, std::max<int64_t>(1, r.k / 256));
+            int64_t c = (r.k + si - 1) / si;
+            if (si > 1) c = (c + 15) / 16 * 16;
+            chunk[i] = std::max<int64_t>(c, 1);
+            si = (r.k + chunk[i] - 1) / chunk[i];
+            work += (double)tiles_of(r) * si * (chunk[i] + ovh);
+            longest = std::max(longest, chunk[i] + ovh);
         }
+        return std::max((double)longest, work / slots);
+    };
+    std::vector<int64_t> chunks, trial;
+    double best = splits_for(1, chunks);
+    for (int sc = 2; sc <= 64; ++sc) {
+        const double c = splits_for(sc, trial);
+        if (c < 0.97 * best) {
+            best = c;
+            chunks = trial;
+        }
+    }
+    int64_t wsoff = 0;
+    for (size_t ri = 0; ri < raw.size(); ++ri) {
+        const Raw& r = raw[ri];
+        int nsplit = r.k > 0 ? (int)((r.k + chunks[ri] - 1) / chunks[ri]) : 1;
         const bool AR = opa == 0 || opa == 3;     // A stored m-fastest
         const bool BR = !(opb == 0 || opb == 3);  // B stored n-fastest
         const int64_t lda = std::max<int64_t>(1, P.sp[A.R].cnt[r.g]);
@@ -475,9 +502,7 @@ inline int product(Plan& P, const char* name, int opa, int opb, View A, View B, 
             op.g.push_back(pg);
             continue;
         }
-        int64_t chunk = (r.k + nsplit - 1) / nsplit;
-        chunk = (chunk + 15) / 16 * 16;
-        nsplit = (int)((r.k + chunk - 1) / chunk);
+        const int64_t chunk = chunks[ri];
         RedDesc rd;
         rd.C = C.arr; rd.offC = oC; rd.ldc = ldc; rd.offW = wsoff;
         rd.m = (int)r.m; rd.n = (int)r.n; rd.nsplit = nsplit;
