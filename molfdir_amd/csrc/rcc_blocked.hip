@@ -26,10 +26,9 @@ struct BlkEngine {
     struct BilDev { int32_t *m0 = nullptr, *p[2] = {nullptr, nullptr}, *q[2] = {nullptr, nullptr}; };
     std::vector<BilDev> bdev;
     struct ProdDev {
-        std::vector<ZGroup> h96, h32;    // host descriptors (pointers resolved)
-        ZGroup* d96 = nullptr;
-        ZGroup* d32 = nullptr;
-        int64_t blocks96 = 0, blocks32 = 0;
+        std::vector<ZGroup> h96;         // host descriptors (pointers resolved)
+        ZGroup* dg = nullptr;            // device copy, offsets for the product's kernel
+        int64_t blocks = 0;
         GRedDesc* red = nullptr;
         int nred = 0;
         int64_t redmax = 0;
@@ -94,12 +93,14 @@ int build_products(BlkEngine* E, std::string& err) {
         const ProdOp& op = P.prods[i];
         BlkEngine::ProdDev& d = E->pdev[i];
         for (const PGroup& g : op.g) d.h96.push_back(zg(E, g));
-        d.h32 = d.h96;
         if (!d.h96.empty()) {
-            std::vector<ZGroup> a = d.h96, b = d.h32;
-            d.blocks96 = zgroup_offsets(a.data(), (int)a.size());
-            d.blocks32 = zgroup_offsets_small(b.data(), (int)b.size());
-            if (upload_vec(E, &d.d96, a, err) != 0 || upload_vec(E, &d.d32, b, err) != 0) return -1;
+            std::vector<ZGroup> a = d.h96;
+            const int ng = (int)a.size();
+            d.blocks = op.variant == 1   ? zgroup_offsets_small(a.data(), ng)
+                       : op.variant == 2 ? zgroup_offsets_tall(a.data(), ng)
+                       : op.variant == 3 ? zgroup_offsets_skinny(a.data(), ng)
+                                         : zgroup_offsets(a.data(), ng);
+            if (upload_vec(E, &d.dg, a, err) != 0) return -1;
         }
         if (!op.red.empty()) {
             std::vector<GRedDesc> r;
@@ -138,13 +139,20 @@ int run_prod(BlkEngine* E, int i, int gauss, hipStream_t st, std::string& err) {
             r.opb = op.opb;
             r.tag = op.tag;
             r.gauss = gauss;
+            if (op.variant == 3) {
+                r.variant = 3;
+                r.nj = op.nj;
+            }
             g_record->push_back(std::move(r));
         } else {
             const int ng = (int)d.h96.size();
-            const int rc = op.small ? zgemm_grouped_small(op.opa, op.opb, gauss, d.d32, ng,
-                                                          d.blocks32, st, nullptr, 0)
-                                    : zgemm_grouped_ops(op.opa, op.opb, op.tag, gauss, d.d96, ng,
-                                                        d.blocks96, st);
+            const int rc =
+                op.variant == 1   ? zgemm_grouped_small(op.opa, op.opb, gauss, d.dg, ng, d.blocks,
+                                                        st, nullptr, 0)
+                : op.variant == 2 ? zgemm_grouped_tall(op.opa, op.opb, gauss, d.dg, ng, d.blocks, st)
+                : op.variant == 3 ? zgemm_grouped_skinny(op.nj, d.dg, ng, d.blocks, st)
+                                  : zgemm_grouped_ops(op.opa, op.opb, op.tag, gauss, d.dg, ng,
+                                                      d.blocks, st);
             if (rc != 0) {
                 err = "blocked engine: grouped product " + op.name + " failed (" +
                       std::to_string(rc) + ")";

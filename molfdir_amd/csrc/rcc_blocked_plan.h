@@ -226,7 +226,9 @@ struct ProdOp {
     std::vector<PGroup> g;          // the launch (split pieces write the work array)
     std::vector<RedDesc> red;       // K-split reductions (empty: none)
     int64_t ws_elems = 0;           // work array elements this product uses
-    bool small = false;             // 32 x 32 tile kernel
+    // kernel: 0 the 96 x 64 MFMA tile, 1 the 32 x 32 one, 2 the 128 x 32 one,
+    // 3 the streaming kernel (op N N, n <= nj <= 16: one row per thread)
+    int variant = 0, nj = 0;
     double flops = 0;               // real flops of the classical four-product form
 };
 
@@ -438,23 +440,41 @@ inline int product(Plan& P, const char* name, int opa, int opb, View A, View B, 
         tiles += ((m + 95) / 96) * ((n + 63) / 64);
         op.flops += 8.0 * (double)m * (double)n * (double)ak;
     }
-    // small tile when the blocks are small against the 96 x 64 tile
-    int64_t mx = 0, nx = 0;
+    // the kernel: streaming for N N products with an occupied-spinor width
+    // (n <= 16) and a large A; the 32 x 32 tile for small blocks; otherwise the
+    // 96 x 64 or the 128 x 32 tile, whichever computes less padding
+    int64_t mx = 0, nx = 0, rowblocks = 0;
+    double useful = 0, pad96 = 0, pad128 = 0;
     for (const Raw& r : raw) {
         mx = std::max(mx, r.m);
         nx = std::max(nx, r.n);
+        useful += (double)r.m * r.n * r.k;
+        pad96 += (double)((r.m + 95) / 96 * 96) * ((r.n + 63) / 64 * 64) * r.k;
+        pad128 += (double)((r.m + 127) / 128 * 128) * ((r.n + 31) / 32 * 32) * r.k;
+        rowblocks += (r.m + 255) / 256;
     }
-    op.small = mx <= 48 || nx <= 32 || tiles < 64;
+    // (the streaming kernel has no K split: it needs rows for every CU)
+    if (opa == 0 && opb == 0 && nx <= 16 && rowblocks >= 256) {
+        op.variant = 3;
+        op.nj = nx <= 4 ? 4 : nx <= 8 ? 8 : 16;
+    } else if (mx <= 48 || tiles < 64) {
+        op.variant = 1;
+    } else if (nx <= 32 || (useful > 0 && useful / pad128 > useful / pad96 + 0.05)) {
+        op.variant = 2;
+    } else {
+        op.variant = 0;
+    }
     // K-split: each group's k cut into chunks, a split count s shared out in
     // proportion to the groups' k, chosen to minimise the makespan bound
     // max(longest workgroup, total work / workgroup slots), with a fixed cost
     // per workgroup (in k units) for its prologue and epilogue
-    const int64_t slots = op.small ? 2048 : 512, ovh = 64;
+    const int64_t slots = op.variant == 1 ? 2048 : 512, ovh = 64;
     int64_t kmax = 1;
     for (const Raw& r : raw) kmax = std::max(kmax, r.k);
     auto tiles_of = [&](const Raw& r) {
-        return op.small ? ((r.m + 31) / 32) * ((r.n + 31) / 32)
-                        : ((r.m + 95) / 96) * ((r.n + 63) / 64);
+        return op.variant == 1 ? ((r.m + 31) / 32) * ((r.n + 31) / 32)
+               : op.variant == 2 ? ((r.m + 127) / 128) * ((r.n + 31) / 32)
+                                 : ((r.m + 95) / 96) * ((r.n + 63) / 64);
     };
     auto splits_for = [&](int sc, std::vector<int64_t>& chunk) {
         chunk.assign(raw.size(), 0);
@@ -475,7 +495,9 @@ inline int product(Plan& P, const char* name, int opa, int opb, View A, View B, 
     };
     std::vector<int64_t> chunks, trial;
     double best = splits_for(1, chunks);
-    for (int sc = 2; sc <= 64; ++sc) {
+    // (up to 1024 pieces: products with a tiny output and a very long k, such
+    // as HOINTM's o_g x o_g from k = |(VV, o)_g|, are latency-bound per piece)
+    for (int sc = 2; sc <= 1024 && op.variant != 3; sc += sc < 64 ? 1 : 16) {
         const double c = splits_for(sc, trial);
         if (c < 0.97 * best) {
             best = c;

@@ -71,6 +71,15 @@ int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, in
 int64_t zgroup_offsets_small(ZGroup* g, int ngroups);
 int zgemm_grouped_small(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
                         int64_t blocks, hipStream_t st, const int* gmap, int persistent = 0);
+// The 128 x 32-tile grouped GEMM (same op combinations and bits as the
+// 96 x 64 one) and its tile offsets.
+int64_t zgroup_offsets_tall(ZGroup* g, int ngroups);
+int zgemm_grouped_tall(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
+                       int64_t blocks, hipStream_t st);
+// Streaming grouped product (op N N, n <= nj <= 16, one row per thread,
+// FP64 vector FMAs), its offsets in 256-row blocks.
+int64_t zgroup_offsets_skinny(ZGroup* g, int ngroups);
+int zgemm_grouped_skinny(int nj, const ZGroup* dG, int ngroups, int64_t blocks, hipStream_t st);
 // ---- non-iterative triples (rcc_triples.hip) ----
 // Inputs of T3CORR in the C1 layouts (device pointers); gauss as GemmArgs.
 struct T3Input {
@@ -157,6 +166,9 @@ struct Rec {
     double2* ws = nullptr;
     std::vector<ZGroup> groups;       // RK_GROUPED: descriptors, op(A), op(B), tag, complex form
     int opa = 0, opb = 0, tag = 0, gauss = 0;
+    // RK_GROUPED: -1 the batch tuner picks the tile (the MFMA tiles give the
+    // same bits); 3 the streaming kernel (vector FMAs: fixed, nj columns)
+    int variant = -1, nj = 0;
 };
 extern thread_local std::vector<Rec>* g_record;
 struct BatchProgram;
@@ -166,7 +178,8 @@ int batch_run(const BatchProgram* p, hipStream_t st);
 // times the 96 x 64 and 32 x 32 tiles of every grouped product, keeps the faster
 int batch_tune(BatchProgram* p, hipStream_t st);
 // positions (launch sites) of the program, grouped products among them, and
-// how many of those use each tile variant (96 x 64, 32 x 32, 32 x 32 persistent)
+// how many of those use each kernel (96 x 64, 32 x 32, 32 x 32 persistent,
+// streaming)
 void batch_info(const BatchProgram* p, int* positions, int* grouped, int* variants);
 void batch_free(BatchProgram* p);
 

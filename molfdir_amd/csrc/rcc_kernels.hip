@@ -1118,6 +1118,11 @@ struct BatchOp {
     const void* dmap32 = nullptr;
     int64_t blocks32 = 0;
     int small = 0;
+    // the streaming kernel (fixed, not tuned): its descriptors and columns
+    bool fixed = false;
+    const void* dpsk = nullptr;
+    int64_t blockssk = 0;
+    int nj = 0;
     const void* dred = nullptr;   // split reductions (redcount entries) or null
     int redcount = 0;
     int64_t redn = 0;
@@ -1149,14 +1154,14 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
         memcpy(host.data() + off, p, n);
         return off;
     };
-    struct Pending { size_t off, red, map, off32, map32; bool has_red; };
+    struct Pending { size_t off, red, map, off32, map32, offsk; bool has_red; };
     std::vector<Pending> pend;
     for (size_t j = 0; j < npos; ++j) {
         const Rec& r0 = (*recs[0])[j];
         BatchOp op;
         op.kind = r0.kind;
         op.count = nh;
-        Pending pd{0, 0, 0, 0, 0, false};
+        Pending pd{0, 0, 0, 0, 0, 0, false};
         for (int h = 1; h < nh; ++h) {
             const Rec& r = (*recs[h])[j];
             bool same = r.kind == r0.kind;
@@ -1174,6 +1179,23 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
             }
         }
         if (r0.kind == RK_GEMM || r0.kind == RK_GROUPED) {
+          // handles whose plans put this product on different kernels (the
+          // streaming one or the MFMA tiles) get one launch per kernel
+          auto vof = [&](int h) {
+              const Rec& r = (*recs[h])[j];
+              return r.kind == RK_GROUPED ? r.variant : -1;
+          };
+          std::vector<int> vset;
+          for (int h = 0; h < nh; ++h)
+              if (std::find(vset.begin(), vset.end(), vof(h)) == vset.end()) vset.push_back(vof(h));
+          const BatchOp op_init = op;
+          const Pending pd_init = pd;
+          for (size_t vi = 0; vi < vset.size(); ++vi) {
+            const int want = vset[vi];
+            if (vi > 0) {
+                op = op_init;
+                pd = pd_init;
+            }
             std::vector<ZGroup> all;
             std::vector<SplitReduceP> red(nh);
             bool any_red = false;
@@ -1183,6 +1205,7 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
             std::vector<int> ns(nh, 1), kc(nh, 0);
             int64_t total = 0;
             for (int h = 0; h < nh; ++h) {
+                if (vof(h) != want) continue;
                 const Rec& r = (*recs[h])[j];
                 ns[h] = r.nsplit;
                 kc[h] = r.kchunk;
@@ -1192,6 +1215,7 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
             }
             const int64_t f = (total + 16383) / 16384;
             for (int h = 0; h < nh && f > 1; ++h) {
+                if (vof(h) != want) continue;
                 const Rec& r = (*recs[h])[j];
                 if (r.kind != RK_GEMM || r.nsplit <= 1 || r.g.k <= 0) continue;
                 const int want = (int)std::max<int64_t>(1, (r.nsplit + f - 1) / f);
@@ -1201,6 +1225,7 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
                 kc[h] = ns[h] > 1 ? c : r.g.k;
             }
             for (int h = 0; h < nh; ++h) {
+                if (vof(h) != want) continue;
                 Rec r = (*recs[h])[j];
                 r.nsplit = ns[h];
                 r.kchunk = kc[h];
@@ -1271,12 +1296,26 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
                     for (int64_t b = s32[q].tile_off; b < end; ++b) gm32[(size_t)b] = q;
                 }
                 pd.map32 = put(gm32.data(), gm32.size() * sizeof(int));
+                if (want == 3) {
+                    std::vector<ZGroup> sk = all;
+                    op.blockssk = zgroup_offsets_skinny(sk.data(), op.count);
+                    pd.offsk = put(sk.data(), sk.size() * sizeof(ZGroup));
+                    op.fixed = true;
+                    op.small = 3;
+                    for (int h = 0; h < nh; ++h)
+                        if (vof(h) == 3) op.nj = std::max(op.nj, (*recs[h])[j].nj);
+                }
             }
             if (any_red) {
                 pd.red = put(red.data(), red.size() * sizeof(SplitReduceP));
                 pd.has_red = true;
                 op.redcount = nh;
             }
+            if (vi + 1 < vset.size()) {
+                prog->ops.push_back(op);
+                pend.push_back(pd);
+            }
+          }
         } else {
             const size_t ps = param_size(r0.kind);
             if (ps == 0) {
@@ -1311,6 +1350,8 @@ int batch_compile(const std::vector<std::vector<Rec>*>& recs, BatchProgram** out
             prog->ops[j].dmap = (const unsigned char*)prog->arena + pend[j].map;
             prog->ops[j].dp32 = (const unsigned char*)prog->arena + pend[j].off32;
             prog->ops[j].dmap32 = (const unsigned char*)prog->arena + pend[j].map32;
+            if (prog->ops[j].fixed)
+                prog->ops[j].dpsk = (const unsigned char*)prog->arena + pend[j].offsk;
         }
         if (pend[j].has_red) prog->ops[j].dred = (const unsigned char*)prog->arena + pend[j].red;
     }
@@ -1325,6 +1366,8 @@ constexpr int TILE_VARIANTS = 3;
 
 static int grouped_launch(const BatchOp& op, int small, hipStream_t st) {
     if (op.count <= 0) return 0;
+    if (small == 3)
+        return zgemm_grouped_skinny(op.nj, (const ZGroup*)op.dpsk, op.count, op.blockssk, st);
     if (small)
         return zgemm_grouped_small(op.opa, op.opb, op.gauss, (const ZGroup*)op.dp32, op.count,
                                    op.blocks32, st, (const int*)op.dmap32,
@@ -1341,7 +1384,7 @@ int batch_tune(BatchProgram* p, hipStream_t st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
     for (BatchOp& op : p->ops) {
-        if (op.kind != RK_GROUPED || op.count <= 0) continue;
+        if (op.kind != RK_GROUPED || op.count <= 0 || op.fixed) continue;
         float best[TILE_VARIANTS] = {1e30f, 1e30f, 1e30f};
         for (int v = 0; v < TILE_VARIANTS; ++v)
             for (int rep = 0; rep < 3; ++rep) {
@@ -1368,11 +1411,11 @@ int batch_tune(BatchProgram* p, hipStream_t st) {
 
 void batch_info(const BatchProgram* p, int* positions, int* grouped, int* variants) {
     int g = 0;
-    for (int v = 0; v < 3; ++v) variants[v] = 0;
+    for (int v = 0; v < 4; ++v) variants[v] = 0;
     for (const BatchOp& op : p->ops)
         if (op.kind == RK_GROUPED && op.count > 0) {
             ++g;
-            variants[std::min(std::max(op.small, 0), 2)] += 1;
+            variants[std::min(std::max(op.small, 0), 3)] += 1;
         }
     *positions = (int)p->ops.size();
     *grouped = g;

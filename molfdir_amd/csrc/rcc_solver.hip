@@ -3152,7 +3152,7 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
 
 int rcc_batch_plan_info(rcc_t* const* hs, int n, int* info) {
     if (!hs || n <= 0 || !info) return -1;
-    for (int i = 0; i < 6; ++i) info[i] = 0;
+    for (int i = 0; i < 7; ++i) info[i] = 0;
     std::vector<rcc_t*> key(hs, hs + n);
     for (BatchPlan* p : g_plans)
         if (p->hs == key) {

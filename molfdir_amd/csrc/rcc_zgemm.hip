@@ -566,6 +566,143 @@ int zgemm_grouped_ops(int opa, int opb, int tag, int gauss, const ZGroup* dG, in
     return -1;
 }
 
+// The grouped GEMM on the 128 x 32 tile (waves of 64 x 16): for groups
+// whose n suits 32-column tiles better than 64-column ones (n = 65 .. 96
+// needs two 64-wide tiles, mostly padding, but three 32-wide ones).  Same
+// MFMA instruction and k order as the 96 x 64 tile: the same bits.
+int64_t zgroup_offsets_tall(ZGroup* g, int ngroups) {
+    int64_t off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        g[i].tile_off = off;
+        const int64_t t =
+            (int64_t)((g[i].m + TLS::BM - 1) / TLS::BM) * ((g[i].n + TLS::BN - 1) / TLS::BN);
+        off += (t + NXCD - 1) / NXCD * NXCD;
+    }
+    return off;
+}
+
+template <int OPA, int OPB, int GAUSS>
+static void launch_grouped_tall(const ZGroup* dG, int ngroups, int64_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL((zgemm_mfma_kernel<OPA, OPB, 0, GAUSS, 1, TLS>), dim3((unsigned)blocks),
+                       dim3(256), 0, st, 0, 0, 0, (const double2*)nullptr, (int64_t)0, (int64_t)0,
+                       (const double2*)nullptr, (int64_t)0, (int64_t)0, (double2*)nullptr,
+                       (int64_t)0, (int64_t)0, make_double2(0, 0), make_double2(0, 0), 1, 0,
+                       (double2*)nullptr, dG, ngroups, nullptr);
+}
+
+int zgemm_grouped_tall(int opa, int opb, int gauss, const ZGroup* dG, int ngroups,
+                       int64_t blocks, hipStream_t st) {
+    if (ngroups <= 0 || blocks <= 0) return 0;
+#define RCC_T(a, b)                                                                           \
+    if (opa == a && opb == b) {                                                               \
+        gauss ? launch_grouped_tall<a, b, 1>(dG, ngroups, blocks, st)                         \
+              : launch_grouped_tall<a, b, 0>(dG, ngroups, blocks, st);                        \
+        return hipGetLastError() == hipSuccess ? 0 : -2;                                       \
+    }
+    RCC_T(OP_N, OP_N)
+    RCC_T(OP_N, OP_C)
+    RCC_T(OP_N, OP_T)
+    RCC_T(OP_T, OP_N)
+    RCC_T(OP_C, OP_N)
+#undef RCC_T
+    return -1;
+}
+
+// Streaming grouped product for C (m x n) = alpha A B + beta C with op N N,
+// n <= NJ and m large: one row of C per thread, the row of A read once
+// (coalesced across the wave along m), B staged in LDS in k-chunks of 32 and
+// read as broadcasts, FP64 FMAs on the vector units.  For the products
+// whose block has an occupied-spinor dimension (n = o of one irrep): every
+// MFMA tile would be mostly padding and the product is bound by reading A.
+constexpr int SKR = 256, SKK = 32;
+
+int64_t zgroup_offsets_skinny(ZGroup* g, int ngroups) {
+    int64_t off = 0;
+    for (int i = 0; i < ngroups; ++i) {
+        g[i].tile_off = off;
+        off += (g[i].m + SKR - 1) / SKR;
+    }
+    return off;
+}
+
+template <int NJ>
+__global__ void __launch_bounds__(256)
+zgemm_skinny_grouped_kernel(const ZGroup* __restrict__ G, int ngroups) {
+    __shared__ double2 Bs[SKK][NJ];
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (G[mid].tile_off <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const ZGroup& gg = G[lo];
+    const int m = gg.m, n = gg.n, k = gg.k;
+    const int64_t row = (b - gg.tile_off) * SKR + threadIdx.x;
+    const bool live = row < m;
+    const double2* __restrict__ A = gg.A;
+    const double2* __restrict__ B = gg.B;
+    double ar[NJ], ai[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        ar[j] = 0.0;
+        ai[j] = 0.0;
+    }
+    for (int k0 = 0; k0 < k; k0 += SKK) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < SKK * NJ; t += SKR) {
+            const int kk = t / NJ, j = t - (t / NJ) * NJ;
+            Bs[kk][j] = (k0 + kk < k && j < n) ? B[(k0 + kk) + (int64_t)j * gg.ldb]
+                                                : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        const int kl = min(SKK, k - k0);
+        if (live)
+            for (int kk = 0; kk < kl; ++kk) {
+                const double2 a = A[row + (int64_t)(k0 + kk) * gg.lda];
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) {
+                    const double2 bb = Bs[kk][j];
+                    ar[j] = fma(a.x, bb.x, ar[j]);
+                    ar[j] = fma(-a.y, bb.y, ar[j]);
+                    ai[j] = fma(a.x, bb.y, ai[j]);
+                    ai[j] = fma(a.y, bb.x, ai[j]);
+                }
+            }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        if (j >= n) break;
+        double2* cp = gg.C + row + (int64_t)j * gg.ldc;
+        double2 o;
+        o.x = gg.alpha.x * ar[j] - gg.alpha.y * ai[j];
+        o.y = gg.alpha.x * ai[j] + gg.alpha.y * ar[j];
+        if (gg.beta.x != 0.0 || gg.beta.y != 0.0) {
+            const double2 c = *cp;
+            o.x += gg.beta.x * c.x - gg.beta.y * c.y;
+            o.y += gg.beta.x * c.y + gg.beta.y * c.x;
+        }
+        *cp = o;
+    }
+}
+
+int zgemm_grouped_skinny(int nj, const ZGroup* dG, int ngroups, int64_t blocks, hipStream_t st) {
+    if (ngroups <= 0 || blocks <= 0) return 0;
+    if (nj <= 4)
+        hipLaunchKernelGGL(zgemm_skinny_grouped_kernel<4>, dim3((unsigned)blocks), dim3(SKR), 0,
+                           st, dG, ngroups);
+    else if (nj <= 8)
+        hipLaunchKernelGGL(zgemm_skinny_grouped_kernel<8>, dim3((unsigned)blocks), dim3(SKR), 0,
+                           st, dG, ngroups);
+    else if (nj <= 16)
+        hipLaunchKernelGGL(zgemm_skinny_grouped_kernel<16>, dim3((unsigned)blocks), dim3(SKR), 0,
+                           st, dG, ngroups);
+    else
+        return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int choose_split(const GemmArgs& g, int64_t ws_elems) {
     const int64_t tiles = tile_count(g);
     if (tiles >= 512 || g.k < 512) return 1;

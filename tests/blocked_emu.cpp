@@ -136,7 +136,7 @@ int emu_run(Emu* e, int prog) {
 int emu_nprods(Emu* e) { return (int)e->P.prods.size(); }
 void emu_prod_info(Emu* e, int i, double* info) {
     const ProdOp& p = e->P.prods[i];
-    info[0] = p.g.size(); info[1] = p.red.size(); info[2] = p.small; info[3] = p.flops; info[4] = p.ws_elems;
+    info[0] = p.g.size(); info[1] = p.red.size(); info[2] = p.variant; info[3] = p.flops; info[4] = p.ws_elems;
 }
 }
 extern "C" int emu_arr(Emu* e, const char* name) {
