@@ -203,8 +203,8 @@ int rcc_iterate_batch_grouped(rcc_t* const* handles, int n, int* nit, double* ec
  * handles), info[2] = grouped products among them, info[3..5] = how many of
  * those the tile autotuner put on the 96 x 64 tile, the 32 x 32 tile and the
  * persistent 32 x 32 grid, info[6] = how many run the streaming kernel
- * (products with an occupied-spinor block width; fixed, not tuned).  info
- * holds 7 ints. */
+ * (products with an occupied-spinor block width; fixed, not tuned), info[7] =
+ * how many the autotuner put on the 128 x 32 tile.  info holds 8 ints. */
 int rcc_batch_plan_info(rcc_t* const* handles, int n, int* info);
 /* CCEQN's loop (cceqns.F:293-569): iterate until rms < 10^-ntol or maxit. */
 int rcc_cceqn(rcc_t* h, int* nit, double* ecc, double* rms);

@@ -11,6 +11,12 @@ namespace rcc {
 // transposed; the adjoint contractions of the Lambda equations use it)
 enum { OP_N = 0, OP_T = 1, OP_C = 2, OP_R = 3 };
 
+// packed index of the pair x > y of 0 .. n-1 (y-major triangle: the relccsd
+// A>B / I>J packing of the T2 and VVVV dimensions)
+__host__ __device__ __forceinline__ int64_t pair_of(int x, int y, int n) {
+    return (int64_t)y * (2 * n - y - 1) / 2 + (x - y - 1);
+}
+
 // Strided-batched complex GEMM, column-major: C_b = alpha op(A_b) op(B_b) + beta C_b.
 struct GemmArgs {
     int opA = OP_N, opB = OP_N;
@@ -36,6 +42,9 @@ int zgemm(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st);
 // the partial sums would not fit.  No allocation, no autotuning.
 int zgemm_scratch_init(int device);
 int64_t zgemm_scratch_elems();
+// Workgroups of the FP64 GEMM engine resident at once on the current device
+// (two per CU).
+int64_t resident_slots();
 int zgemm_noalloc(const GemmArgs& g, int device, int ksplit, hipStream_t st, std::string& err);
 // K-split autotuner of zgemm (process-wide): on / off, and the number of
 // shapes tuned so far.
@@ -352,6 +361,12 @@ void split_layout(SplitOperand& S, int rows, int k);
 // X[kk + r*ld], else X[r + kk*ld]; conj: use conj(X).
 int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_contiguous,
                   bool conj, hipStream_t st);
+// The same for the ring form of T2 (rows (a, i), k = (c, k), nv o x nv o),
+//   X[(a,i),(c,k)] = s sgn T2[ac, ik] - t1(a,k) t1(c,i),
+// computed from the packed T2 (nvvt x noot) and t1 (nv x o; null: no product
+// term) as it is sliced, without a full-size intermediate.
+int split_ring_operand(const SplitOperand& S, const double2* T2, const double2* t1, double s,
+                       int nv, int no, int64_t nvvt, hipStream_t st);
 // C(m x n) = alpha * A . B^T + beta * C with A (m rows) and B (n rows) split
 // operands over the same k.  ws (ws_elems complex, may be null) holds the
 // partial sums of a K-split.

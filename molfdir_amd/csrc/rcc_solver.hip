@@ -119,6 +119,8 @@ struct rcc_handle {
     // split operands (precision 1: BF16 slices, 2: INT8 digits): the integral
     // operands are built once per set of integrals, the rest per iteration
     SOp sVVVV, sVOVV, sVVOOR, sTAU, sP, sH, sRING;
+    // split-BF16 only: the AINTM, HVINTM and GV . T2 products
+    SOp sVVOOA, sVVOOF1, sF1A, sF1B, sGV, sTAUA, sAZ;
     bool split_const_ok = false, split_tau_ok = false, split_ring_ok = false;
     int digits = 5;
     double last_iter_ms = 0.0;
@@ -436,21 +438,28 @@ void free_split_op(rcc_t* h, SOp& S) {
 }
 
 void free_split(rcc_t* h) {
-    for (SOp* S : {&h->sVVVV, &h->sVOVV, &h->sVVOOR, &h->sTAU, &h->sP, &h->sH, &h->sRING})
+    for (SOp* S : {&h->sVVVV, &h->sVOVV, &h->sVVOOR, &h->sTAU, &h->sP, &h->sH, &h->sRING,
+                   &h->sVVOOA, &h->sVVOOF1, &h->sF1A, &h->sF1B, &h->sGV, &h->sTAUA, &h->sAZ})
         free_split_op(h, *S);
     h->split_const_ok = h->split_tau_ok = h->split_ring_ok = false;
 }
 
 // Split X (rows x k; k_contiguous: element (r, kk) at X[kk + r ld], else
 // X[r + kk ld]; conj: of conj(X)) into S in the current precision's format.
+// The BF16 planes of S for a rows x k operand (allocated on first use).
+int bf_alloc(rcc_t* h, SOp& S, int rows, int64_t k) {
+    if (!S.bf.planes) {
+        split_layout(S.bf, rows, (int)k);
+        CK(dalloc_raw(h, (char**)&S.bf.planes, std::max<int64_t>(1, S.bf.plane_bytes)));
+        CK(dalloc_raw(h, &S.bf.ex, std::max<int64_t>(1, S.bf.ex_bytes)));
+    }
+    return 0;
+}
+
 int do_split(rcc_t* h, SOp& S, int rows, int64_t k, const double2* X, int64_t ld, bool kcont,
              bool conj) {
     if (h->precision == 1) {
-        if (!S.bf.planes) {
-            split_layout(S.bf, rows, (int)k);
-            CK(dalloc_raw(h, (char**)&S.bf.planes, std::max<int64_t>(1, S.bf.plane_bytes)));
-            CK(dalloc_raw(h, &S.bf.ex, std::max<int64_t>(1, S.bf.ex_bytes)));
-        }
+        CK(bf_alloc(h, S, rows, k));
         if (split_operand(S.bf, X, ld, kcont, conj, h->st) != 0) {
             h->err = "split_operand launch failed";
             return -1;
@@ -482,6 +491,12 @@ int sgemm(rcc_t* h, const SOp& A, const SOp& B, double2* C, int64_t ldc, double2
     return 0;
 }
 
+// In split-BF16 mode the compute-bound o^2 v^3 and o^4 v^2 products besides X
+// (AINTM's VVOO^H tau, HVINTM's F1 VVOO_F1^H, T2EQNS's GV F1 and tau AZ) run
+// as split products too; the INT8 kernels have no K-split for their few
+// output tiles.
+bool bf_small_products(const rcc_t* h) { return h->precision == 1 && h->sym.nrep == 1; }
+
 // Constant split operands (integrals), built once per set of integrals.
 int ensure_split_const(rcc_t* h) {
     if (h->split_const_ok) return 0;
@@ -490,6 +505,13 @@ int ensure_split_const(rcc_t* h) {
     CK(do_split(h, h->sVOVV, (int)nvo, nvvt, h->cls[RCC_VOVV], nvo, false, false));
     // HINTM ring B operand: B(k=(d,l), n=(c,k)) = conj VVOO_R[(c,k),(d,l)]
     CK(do_split(h, h->sVVOOR, (int)nvo, nvo, h->VVOO_R, nvo, false, true));
+    if (bf_small_products(h)) {
+        // AINTM A operand: A(ij, k=ab) = conj VVOO[ab, ij]
+        CK(do_split(h, h->sVVOOA, (int)h->noot, nvvt, h->cls[RCC_VVOO], nvvt, true, true));
+        // HVINTM B operand: B(c, k) = conj VVOO_F1[c, k], k = (b, kl)
+        CK(do_split(h, h->sVVOOF1, h->nv, (int64_t)h->nv * h->noot, h->VVOO_F1, h->nv, false,
+                    true));
+    }
     h->split_const_ok = true;
     return 0;
 }
@@ -574,6 +596,11 @@ int do_hvintm(rcc_t* h) {
     k_zero_diag(h->HV, v, h->st);
     k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, h->st);
     const int64_t K = (int64_t)v * h->noot;
+    if (bf_small_products(h) && K > 0) {
+        CK(ensure_split_const(h));
+        CK(do_split(h, h->sF1A, v, K, h->F1, v, false, false));
+        return sgemm(h, h->sF1A, h->sVVOOF1, h->HV, v, cplx(-1, 0), cplx(1, 0));
+    }
     return gemm(h, OP_N, OP_C, v, v, (int)K, cplx(-1, 0), h->F1, v, h->VVOO_F1, v, cplx(1, 0),
                 h->HV, v);
 }
@@ -653,6 +680,12 @@ int do_aintm(rcc_t* h) {
                 cplx(0, 0), h->YAZ, noot, o, v, 0, noot * o));
         k_pack_pij(h->YAZ, h->AZ, noot, 1.0, h->oo, o, h->st);
     }
+    if (bf_small_products(h) && noot > 0) {
+        // AZ += VVOO^H tau: A = conj VVOO (noot rows), B = the ladder's tau operand
+        CK(ensure_split_const(h));
+        CK(ensure_split_tau(h));
+        return sgemm(h, h->sVVOOA, h->sTAU, h->AZ, noot, cplx(1, 0), cplx(1, 0));
+    }
     return gemm(h, OP_C, OP_N, (int)noot, (int)noot, (int)h->nvvt, cplx(1, 0), h->cls[RCC_VVOO],
                 h->nvvt, h->TAU, h->nvvt, cplx(1, 0), h->AZ, noot);
 }
@@ -673,6 +706,16 @@ int hintm_pre(rcc_t* h, bool hakci_has_vovv) {
     // (AK,CI) -> (AI,CK)  (SRT16, ccintm.F:385)
     k_perm_akci(h->HAKCI, h->H, v, o, false, st);
     // + [1/2 T(AD,IL) - T(A,L) T(D,I)] W(KL,CD)  (ccintm.F:393-450)
+    if (h->precision == 1 && !h->ring_grouped) {
+        // split-BF16: the ring form is sliced straight from T2 and T1
+        CK(ensure_split_const(h));
+        CK(bf_alloc(h, h->sP, (int)nvo, nvo));
+        if (split_ring_operand(h->sP.bf, h->T2(), h->T1(), 0.5, v, o, h->nvvt, st) != 0) {
+            h->err = "split_ring_operand launch failed";
+            return -1;
+        }
+        return 0;
+    }
     k_expand_ring(h->T2(), h->T1(), 0.5, h->RING2, h->vv, h->oo, v, o, st);
     if (h->ring_grouped) {
         c1_to_sym_dev(h, RCC_VOVO, h->H, h->H_s);
@@ -720,8 +763,15 @@ int t2eqns_pre(rcc_t* h) {
     double2* S2 = h->S2();
     k_copy(h->nt2, h->cls[RCC_VVOO], S2, st);
     // TAU(AB,KL) A(KL,IJ)  (cceqns.F:1013-1026)
-    CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)noot, cplx(1, 0), h->TAU, nvvt, h->AZ, noot,
-            cplx(1, 0), S2, nvvt));
+    if (bf_small_products(h) && noot > 0) {
+        // A = tau (nvvt rows, k = kl), B(n = ij, k = kl) = AZ[kl + ij noot]
+        CK(do_split(h, h->sTAUA, (int)nvvt, noot, h->TAU, nvvt, false, false));
+        CK(do_split(h, h->sAZ, (int)noot, noot, h->AZ, noot, true, false));
+        CK(sgemm(h, h->sTAUA, h->sAZ, S2, nvvt, cplx(1, 0), cplx(1, 0)));
+    } else {
+        CK(gemm(h, OP_N, OP_N, (int)nvvt, (int)noot, (int)noot, cplx(1, 0), h->TAU, nvvt, h->AZ,
+                noot, cplx(1, 0), S2, nvvt));
+    }
     if (o > 1) {
         // P(ij) W(AB,CJ) T1(C,I)  (cceqns.F:1086-1139):  YP[AB,i,j] = sum_c conj VOVV[(c,j),AB] t(c,i)
         CK(gemm(h, OP_C, OP_N, (int)nvvt, o, v, cplx(1, 0), h->cls[RCC_VOVV], nvo, h->T1(), v,
@@ -741,8 +791,15 @@ int t2eqns_pre(rcc_t* h) {
     }
     // P(ab) GV(A,C) T2(CB,IJ)  (cceqns.F:1183-1184)
     k_expand_f1(h->T2(), h->F1, noot, h->vv, v, st);
-    CK(gemm(h, OP_N, OP_N, v, (int)(v * noot), v, cplx(1, 0), h->GV, v, h->F1, v, cplx(0, 0),
-            h->ZX, v));
+    if (bf_small_products(h) && noot > 0) {
+        // ZX = GV F1: A = GV (v rows, k = c), B(n = (b, ij), k = c) = F1[c + n v]
+        CK(do_split(h, h->sGV, v, v, h->GV, v, false, false));
+        CK(do_split(h, h->sF1B, (int)(v * noot), v, h->F1, v, true, false));
+        CK(sgemm(h, h->sGV, h->sF1B, h->ZX, v, cplx(1, 0), cplx(0, 0)));
+    } else {
+        CK(gemm(h, OP_N, OP_N, v, (int)(v * noot), v, cplx(1, 0), h->GV, v, h->F1, v,
+                cplx(0, 0), h->ZX, v));
+    }
     CK(gemm(h, OP_N, OP_T, v, v, o, cplx(-1, 0), h->T1(), v, h->WC, v, cplx(1, 0), h->ZX, v,
             (int)noot, 0, nvo, (int64_t)v * v));
     k_pack_pab(h->ZX, S2, noot, 1.0, h->vv, v, st);
@@ -2077,6 +2134,8 @@ struct BatchPlan {
     hipStream_t st = nullptr;
     hipEvent_t evb = nullptr;
     std::vector<hipEvent_t> evh;
+    // timing events of a compiled batch iteration (start and end on st)
+    hipEvent_t et0 = nullptr, et1 = nullptr;
     // the whole term sequence of every handle, recorded once and compiled to
     // one launch per position over all handles (null: handles differ in
     // structure, the segmented form above runs instead); optionally replayed
@@ -2109,6 +2168,8 @@ void free_plan(BatchPlan* p) {
         if (p->dG[k]) (void)hipFree(p->dG[k]);
     for (hipEvent_t e : p->evh) (void)hipEventDestroy(e);
     if (p->evb) (void)hipEventDestroy(p->evb);
+    if (p->et0) (void)hipEventDestroy(p->et0);
+    if (p->et1) (void)hipEventDestroy(p->et1);
     if (p->st) (void)hipStreamDestroy(p->st);
     delete p;
 }
@@ -2180,7 +2241,8 @@ int get_plan(rcc_t* const* hs, int n, BatchPlan** out) {
         return -1;
     };
     if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&p->evb, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&p->evb, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&p->et0) != hipSuccess || hipEventCreate(&p->et1) != hipSuccess)
         return fail("batch plan: stream / event creation failed");
     for (int i = 0; i < n; ++i) {
         hipEvent_t e = nullptr;
@@ -2232,14 +2294,22 @@ int get_plan(rcc_t* const* hs, int n, BatchPlan** out) {
 // The compiled term sequence of all handles on the batch stream, joined to
 // the handles' streams; with use_graph on every handle the sequence is
 // captured once (second call) and replayed from the graph afterwards.
+// Everything up to the end of batch_finish runs on the batch stream, which
+// the host synchronises before returning: the handles' streams need no join
+// afterwards, and before the phase only a stream with work still queued does
+// (two HIP calls per such handle instead of five per handle per iteration).
 int batch_program_phase(BatchPlan* p, bool use_graph) {
     rcc_t* h0 = p->hs[0];
     for (size_t i = 0; i < p->hs.size(); ++i) {
         rcc_t* h = p->hs[i];
-        HIPCHK(h, hipEventRecord(h->ev0, h->st));
+        const hipError_t q = hipStreamQuery(h->st);
+        if (q == hipSuccess) continue;
+        (void)hipGetLastError();      // "not ready" is a state, not an error
+        if (q != hipErrorNotReady) HIPCHK(h, q);
         HIPCHK(h0, hipEventRecord(p->evh[i], h->st));
         HIPCHK(h0, hipStreamWaitEvent(p->st, p->evh[i], 0));
     }
+    HIPCHK(h0, hipEventRecord(p->et0, p->st));
     p->runs += 1;
     if (use_graph && p->gexec) {
         HIPCHK(h0, hipGraphLaunch(p->gexec, p->st));
@@ -2260,8 +2330,6 @@ int batch_program_phase(BatchPlan* p, bool use_graph) {
         h0->err = "batch: replay launch failed";
         return -1;
     }
-    HIPCHK(h0, hipEventRecord(p->evb, p->st));
-    for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
     return 0;
 }
 // DIISX, CCENG and S -> T of every handle as batched launches on the batch
@@ -2417,23 +2485,22 @@ int batch_finish(BatchPlan* p, int* nit, double* ecc, double* rms) {
                    copy_params(2, h->red_dst + RED_E, p->dslots + (size_t)i * RED_SLOTS + RED_E, q);
                }), 2);
         CK(fin_run(p, fb, slots));
-        for (rcc_t* h : p->hs) HIPCHK(h, hipEventRecord(h->ev1, p->st));
+        HIPCHK(h0, hipEventRecord(p->et1, p->st));
+        // the batch's work is complete: later work on the handles' own
+        // streams needs no join
         HIPCHK(h0, hipStreamSynchronize(p->st));
     }
+    float ms = 0.f;
+    HIPCHK(h0, hipEventElapsedTime(&ms, p->et0, p->et1));
     for (int i = 0; i < nh; ++i) {
         rcc_t* h = p->hs[i];
         memcpy(h->hred + RED_E, p->hslots + (size_t)i * RED_SLOTS + RED_E, 2 * sizeof(double2));
         energy_end(h, ecc ? ecc + i : nullptr, nullptr);
         invalidate_t(h);
-        float ms = 0.f;
-        HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
-        h->last_iter_ms = ms;
+        h->last_iter_ms = ms;       // the batch iteration, the same for every handle
         h->nit += 1;
         if (nit) nit[i] = h->nit;
     }
-    // later per-handle work on the handles' own streams follows this batch
-    HIPCHK(h0, hipEventRecord(p->evb, p->st));
-    for (rcc_t* h : p->hs) HIPCHK(h, hipStreamWaitEvent(h->st, p->evb, 0));
     return 0;
 }
 
@@ -3124,8 +3191,10 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
     bool graph = true;
     for (int i = 0; i < n; ++i) graph = graph && hs[i]->use_graph;
     if (p->prog) {
-        CK(batch_program_phase(p, graph));
-        return batch_finish(p, nit, ecc, rms);
+        int rc = batch_program_phase(p, graph);
+        if (rc == 0) rc = batch_finish(p, nit, ecc, rms);
+        if (rc != 0) (void)hipStreamSynchronize(p->st);   // no batch work left in flight
+        return rc;
     }
     bool any_bk = false;
     for (int i = 0; i < n; ++i) any_bk = any_bk || hs[i]->bk;
@@ -3152,7 +3221,7 @@ int rcc_iterate_batch_grouped(rcc_t* const* hs, int n, int* nit, double* ecc, do
 
 int rcc_batch_plan_info(rcc_t* const* hs, int n, int* info) {
     if (!hs || n <= 0 || !info) return -1;
-    for (int i = 0; i < 7; ++i) info[i] = 0;
+    for (int i = 0; i < 8; ++i) info[i] = 0;
     std::vector<rcc_t*> key(hs, hs + n);
     for (BatchPlan* p : g_plans)
         if (p->hs == key) {

@@ -75,11 +75,42 @@ __device__ __forceinline__ unsigned short bf_bits(__bf16 b) {
 // three variants give the exponents, then the tile's 8 (row block, k-step)
 // pairs x 3 variants x 3 slices fragments are written, 64 lanes per 1 KiB
 // fragment.
+// The ring form of T2 as a slicing source (SRC 2): element (r, k) with
+// r = a + nv i, k = c + nv kk is s sgn T2[ac, i kk] - t1(a, kk) t1(c, i), the
+// sign + when a > c and i > kk agree; zero on a = c or i = kk.
+struct RingSrc {
+    const double2* T2;
+    const double2* t1;
+    double s;
+    int nv, no;
+    int64_t nvvt;
+};
+
+__device__ __forceinline__ double2 ring_elem(const RingSrc& q, int r, int k) {
+    const int a = r % q.nv, i = r / q.nv, c = k % q.nv, kk = k / q.nv;
+    double2 v = make_double2(0.0, 0.0);
+    if (a != c && i != kk) {
+        const int64_t ac = a > c ? pair_of(a, c, q.nv) : pair_of(c, a, q.nv);
+        const int64_t ik = i > kk ? pair_of(i, kk, q.no) : pair_of(kk, i, q.no);
+        v = q.T2[ac + q.nvvt * ik];
+        const double sg = ((a > c) == (i > kk)) ? q.s : -q.s;
+        v.x *= sg;
+        v.y *= sg;
+    }
+    if (q.t1) {
+        const double2 x = q.t1[a + q.nv * kk], y = q.t1[c + q.nv * i];
+        v.x -= x.x * y.x - x.y * y.y;
+        v.y -= x.x * y.y + x.y * y.x;
+    }
+    return v;
+}
+
+// SRC 0: X[r + k ld]; 1: X[k + r ld]; 2: the ring form of T2 (RingSrc)
 template <int SRC>
 __global__ void __launch_bounds__(256) bf_split_kernel(const double2* __restrict__ X, int64_t ld,
                                                        int R, int K, int conj, int nks,
                                                        int rows_pad, signed char* __restrict__ ex,
-                                                       i32x4* __restrict__ P) {
+                                                       i32x4* __restrict__ P, RingSrc ring) {
     constexpr int TPR = 256 / BF_SR;        // threads per row for the maxima
     constexpr int KPT = BF_KC / TPR;        // k per thread
     __shared__ double2 t[BF_SR][BF_KC + 1];
@@ -87,11 +118,14 @@ __global__ void __launch_bounds__(256) bf_split_kernel(const double2* __restrict
     const int c = blockIdx.x, r0 = blockIdx.y * BF_SR, k0 = c * BF_KC;
     for (int e = threadIdx.x; e < BF_SR * BF_KC; e += 256) {
         int r, k;
-        if (SRC == 0) { r = e % BF_SR; k = e / BF_SR; }   // X[r + k ld]
+        if (SRC != 1) { r = e % BF_SR; k = e / BF_SR; }   // rows fastest
         else { k = e % BF_KC; r = e / BF_KC; }            // X[k + r ld]
         const int gr = r0 + r, gk = k0 + k;
         double2 v = make_double2(0.0, 0.0);
-        if (gr < R && gk < K) v = SRC == 0 ? X[gr + (int64_t)gk * ld] : X[gk + (int64_t)gr * ld];
+        if (gr < R && gk < K) {
+            if (SRC == 2) v = ring_elem(ring, gr, gk);
+            else v = SRC == 0 ? X[gr + (int64_t)gk * ld] : X[gk + (int64_t)gr * ld];
+        }
         if (conj) v.y = -v.y;
         t[r][k] = v;
     }
@@ -197,7 +231,6 @@ struct BfShape {
     static constexpr int FRAGS = (8 + NBB) * 3;         // fragments per stage
     static constexpr int STAGE = FRAGS * 64 + 16;       // i32x4 per LDS stage (+ 256 exponent bytes)
     static constexpr int APW = 8 / WAVES;               // A row blocks copied per wave
-    static constexpr int LOADS = 3 * (APW + 1);         // loads per wave and stage
 };
 
 // DMA of stage st into an LDS stage: wave w copies the three slice fragments of
@@ -206,11 +239,14 @@ struct BfShape {
 // wave 0 also copies that (chunk, variant)'s exponents, lanes 0-31 those of
 // the tile's 128 A rows and lanes 32-63 those of its B rows (4 bytes each; with
 // 64 B rows, lanes 48-63 repeat lanes 32-47 into bytes the kernel does not
-// read).  LOADS loads per wave and stage, one more for wave 0 at those stages.
+// read).  A row block that lies wholly past the matrix edge is not copied
+// (bit a of `on` for A row block a of the wave, bit APW for its B row block):
+// only waves that are idle in that tile would read it.  3 loads per copied
+// row block, one more for wave 0 at those stages.
 template <int WN>
 __device__ __forceinline__ void bf_copy(i32x4* dst, const i32x4* ga, const i32x4* gb,
                                         int64_t rbs, const signed char* ge, int64_t ge_stride,
-                                        int wave, int st) {
+                                        int wave, int st, unsigned on) {
     using S = BfShape<WN>;
     const BfStage g = bf_stage_of(st);
     const int64_t off = ((int64_t)g.ks * 3 + g.v) * 3 * 64;
@@ -218,25 +254,46 @@ __device__ __forceinline__ void bf_copy(i32x4* dst, const i32x4* ga, const i32x4
     for (int s = 0; s < 3; ++s) {
 #pragma unroll
         for (int a = 0; a < S::APW; ++a)
-            __builtin_amdgcn_global_load_lds(
-                (const void*)(ga + (int64_t)a * S::WAVES * rbs + off + s * 64),
-                (lds_ptr)(dst + ((wave + a * S::WAVES) * 3 + s) * 64), 16, 0, 0);
-        __builtin_amdgcn_global_load_lds((const void*)(gb + off + s * 64),
-                                         (lds_ptr)(dst + ((8 + wave) * 3 + s) * 64), 16, 0, 0);
+            if (on & (1u << a))
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(ga + (int64_t)a * S::WAVES * rbs + off + s * 64),
+                    (lds_ptr)(dst + ((wave + a * S::WAVES) * 3 + s) * 64), 16, 0, 0);
+        if (on & (1u << S::APW))
+            __builtin_amdgcn_global_load_lds((const void*)(gb + off + s * 64),
+                                             (lds_ptr)(dst + ((8 + wave) * 3 + s) * 64), 16, 0,
+                                             0);
     }
     if (wave == 0 && g.last)
         __builtin_amdgcn_global_load_lds((const void*)(ge + ((int64_t)g.c * 3 + g.v) * ge_stride),
                                          (lds_ptr)(dst + S::FRAGS * 64), 4, 0, 0);
 }
 
+// s_waitcnt vmcnt(n) for a wave-uniform n of 0 .. 10 (0 above that: waits
+// for every load, which is always safe)
+__device__ __forceinline__ void wait_vm(int n) {
+    switch (n) {
+        case 1: wait_vmcnt<1>(); break;
+        case 2: wait_vmcnt<2>(); break;
+        case 3: wait_vmcnt<3>(); break;
+        case 4: wait_vmcnt<4>(); break;
+        case 5: wait_vmcnt<5>(); break;
+        case 6: wait_vmcnt<6>(); break;
+        case 7: wait_vmcnt<7>(); break;
+        case 8: wait_vmcnt<8>(); break;
+        case 9: wait_vmcnt<9>(); break;
+        case 10: wait_vmcnt<10>(); break;
+        default: wait_vmcnt<0>(); break;
+    }
+}
+
 // this wave's copies of stage st have landed: with NB = 3 the copies of stage
-// st + 1 (if any) may still be in flight, with NB = 2 none is issued yet
+// st + 1 (if any), `loads` of them (one more for wave 0's exponents), may
+// still be in flight; with NB = 2 none is issued yet
 template <int WN, int NB>
-__device__ __forceinline__ void bf_wait(int wave, int st, int st1) {
-    using S = BfShape<WN>;
+__device__ __forceinline__ void bf_wait(int wave, int st, int st1, int loads) {
     if (NB == 3 && st + 1 < st1) {
-        if (wave == 0 && bf_stage_of(st + 1).last) wait_vmcnt<S::LOADS + 1>();
-        else wait_vmcnt<S::LOADS>();
+        if (wave == 0 && bf_stage_of(st + 1).last) wait_vm(loads + 1);
+        else wait_vm(loads);
     } else {
         wait_vmcnt<0>();
     }
@@ -301,15 +358,23 @@ __global__ void __launch_bounds__(128 * WN, 2) zgemm_bf_kernel(
             for (int r = 0; r < 4; ++r) cre[i][j][r] = cim[i][j][r] = 0.0;
         }
 
-    if (st0 < st1) bf_copy<WN>(lds[st0 % NB], ga, gb, rbs, ge, ge_stride, wave, st0);
+    // the row blocks this wave copies that start inside the matrix
+    unsigned on = 0;
+#pragma unroll
+    for (int a = 0; a < S::APW; ++a)
+        if (tm * BF_TILE + 16 * (wave + a * S::WAVES) < m) on |= 1u << a;
+    if (tn * S::TN + 16 * wave < n) on |= 1u << S::APW;
+    const int loads = 3 * __builtin_popcount(on);
+    if (st0 < st1) bf_copy<WN>(lds[st0 % NB], ga, gb, rbs, ge, ge_stride, wave, st0, on);
     if (NB == 3 && st0 + 1 < st1)
-        bf_copy<WN>(lds[(st0 + 1) % NB], ga, gb, rbs, ge, ge_stride, wave, st0 + 1);
+        bf_copy<WN>(lds[(st0 + 1) % NB], ga, gb, rbs, ge, ge_stride, wave, st0 + 1, on);
     for (int st = st0; st < st1; ++st) {
-        bf_wait<WN, NB>(wave, st, st1);
+        bf_wait<WN, NB>(wave, st, st1, loads);
         __builtin_amdgcn_s_barrier();
         // every wave is past stage st - 1, so its buffer can be refilled
         const int ahead = st + NB - 1;
-        if (ahead < st1) bf_copy<WN>(lds[ahead % NB], ga, gb, rbs, ge, ge_stride, wave, ahead);
+        if (ahead < st1)
+            bf_copy<WN>(lds[ahead % NB], ga, gb, rbs, ge, ge_stride, wave, ahead, on);
         const i32x4* L = lds[st % NB];
         if (active) {
             bf16x8 bl[2][3];
@@ -429,10 +494,11 @@ void launch_bf(const SplitOperand& A, const SplitOperand& B, double2* C, int64_t
 
 // Tile, tile-order and K-split autotuner: the first product of a shape (m, n,
 // k steps) met outside graph capture times both tiles with tile-row bands gm
-// of 1, 2, 4 and 8, and k split into equal ranges of chunks (2 .. 12 for
-// products of fewer than two rounds of tiles, else 2 and 3; partial sums in
-// ws, reduced in a fixed order) with bands 1 and 4, in place with alpha = 0,
-// beta = 1 (C unchanged), and keeps the fastest for the process.
+// of 1, 2, 4 and 8, and k split into equal ranges of chunks (2 .. 12 and the
+// splits that fill 1 .. 4 rounds of one 128 x 128 tile per CU, for products of
+// fewer than two rounds of tiles, else 2 and 3; partial sums in ws, reduced in
+// a fixed order) with bands 1 and 4, on a scratch C of the product's size
+// (the live C is never touched), and keeps the fastest for the process.
 // RCC_BF_TILE (128 or 64, the tile width), RCC_BF_GM and RCC_BF_KPARTS fix the
 // choice; with the autotuner off it is (128, 1, 1).  Tile and band do not
 // change the result; a K-split sums the chunk ranges in another order.
@@ -469,8 +535,23 @@ BfChoice bf_choose(const SplitOperand& A, const SplitOperand& B, double2* C, int
     std::vector<BfChoice> cand;
     for (int wn : {4, 2})
         for (int g : {1, 2, 4, 8}) cand.push_back({wn, g, 1});
-    const std::vector<int> kps =
-        tiles < 512 ? std::vector<int>{2, 3, 4, 6, 8, 12} : std::vector<int>{2, 3};
+    std::vector<int> kps = tiles < 512 ? std::vector<int>{2, 3, 4, 6, 8, 12} : std::vector<int>{2, 3};
+    if (tiles < 512) {
+        const int64_t cus = std::max<int64_t>(resident_slots() / 2, 1);
+        for (int64_t r = 1; r <= 4; ++r) {
+            const int kp = (int)(r * cus / std::max<int64_t>(tiles, 1));
+            if (kp > 12 && std::find(kps.begin(), kps.end(), kp) == kps.end()) kps.push_back(kp);
+        }
+    }
+    // trial products on a scratch C (m x n, ld m)
+    double2* ct = nullptr;
+    if (hipMalloc((void**)&ct, (size_t)mn * sizeof(double2)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return dflt;
+    }
+    (void)hipMemsetAsync(ct, 0, (size_t)mn * sizeof(double2), st);
     for (int kp : kps)
         if (fits(kp))
             for (int wn : {4, 2})
@@ -481,7 +562,7 @@ BfChoice bf_choose(const SplitOperand& A, const SplitOperand& B, double2* C, int
         float ms_min = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
             (void)hipEventRecord(e0, st);
-            launch_bf(A, B, C, ldc, make_double2(0, 0), make_double2(1, 0), c, ws, st);
+            launch_bf(A, B, ct, A.rows, make_double2(0, 0), make_double2(1, 0), c, ws, st);
             if (hipGetLastError() != hipSuccess) break;
             (void)hipEventRecord(e1, st);
             (void)hipEventSynchronize(e1);
@@ -496,6 +577,8 @@ BfChoice bf_choose(const SplitOperand& A, const SplitOperand& B, double2* C, int
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(ct);
     bf_tuned[key] = best;
     return best;
 }
@@ -519,12 +602,24 @@ int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_co
     if (S.rows <= 0 || S.k <= 0) return 0;
     dim3 g((unsigned)S.nkc, (unsigned)(S.rows_pad / BF_SR));
     const int cj = conj ? 1 : 0;
+    const RingSrc none{nullptr, nullptr, 0.0, 1, 1, 0};
     if (k_contiguous)
         hipLaunchKernelGGL(bf_split_kernel<1>, g, dim3(256), 0, st, X, ld, S.rows, S.k, cj, S.nks,
-                           S.rows_pad, S.ex, (i32x4*)S.planes);
+                           S.rows_pad, S.ex, (i32x4*)S.planes, none);
     else
         hipLaunchKernelGGL(bf_split_kernel<0>, g, dim3(256), 0, st, X, ld, S.rows, S.k, cj, S.nks,
-                           S.rows_pad, S.ex, (i32x4*)S.planes);
+                           S.rows_pad, S.ex, (i32x4*)S.planes, none);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int split_ring_operand(const SplitOperand& S, const double2* T2, const double2* t1, double s,
+                       int nv, int no, int64_t nvvt, hipStream_t st) {
+    if (S.rows <= 0 || S.k <= 0) return 0;
+    if ((int64_t)S.rows != (int64_t)nv * no || (int64_t)S.k != (int64_t)nv * no) return -1;
+    dim3 g((unsigned)S.nkc, (unsigned)(S.rows_pad / BF_SR));
+    const RingSrc q{T2, t1, s, nv, no, nvvt};
+    hipLaunchKernelGGL(bf_split_kernel<2>, g, dim3(256), 0, st, (const double2*)nullptr,
+                       (int64_t)0, S.rows, S.k, 0, S.nks, S.rows_pad, S.ex, (i32x4*)S.planes, q);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -761,7 +761,8 @@ static int launch_split(const GemmArgs& g, int nsplit, int kchunk, double2* ws, 
 // per-workgroup latency (k stages in a chain), of the round fill, and of
 // the partial sums written and reduced.  The first time a shape (ops, tag,
 // complex form, m, n, k, batch) is met outside graph capture, every split
-// with k chunks of 64 .. 4096, and 2 .. 8 equal chunks (and no split), is
+// with k chunks of 64 .. 4096, 2 .. 8 equal chunks, the splits whose
+// workgroups fill 1 .. 8 whole rounds of the resident slots, and no split, is
 // timed on the stream with HIP events.  The trial products write a scratch
 // C of the product's size, allocated for the trial and freed after it, so
 // the live C is never touched; the fastest split is kept for the process.
@@ -813,6 +814,9 @@ int tune_split(const GemmArgs& g, double2* ws, int64_t ws_elems, hipStream_t st,
     };
     for (int c = 64; c <= 4096; c *= 2) add((g.k + c - 1) / c);
     for (int e = 2; e <= 8; ++e) add(e);
+    // splits whose workgroups fill whole rounds of the resident slots
+    const int64_t slots = resident_slots();
+    for (int r = 1; r <= 8; ++r) add(r * slots / std::max<int64_t>(tiles, 1));
     if (cand.size() == 1) return fallback;
     // trial products on a zeroed scratch C (same shape, tightly packed)
     GemmArgs t = g;

@@ -434,17 +434,17 @@ class RelCCSD:
     def batch_plan_info(solvers):
         """The grouped batch plan of these solvers: compiled, launch positions,
         grouped products, and their kernels (96 x 64, 32 x 32, 32 x 32
-        persistent tiles; the streaming kernel)."""
+        persistent and 128 x 32 tiles; the streaming kernel)."""
         n = len(solvers)
         lib = solvers[0].lib
         hs = (ctypes.c_void_p * n)(*[s.h.value for s in solvers])
-        info = (ctypes.c_int * 7)()
+        info = (ctypes.c_int * 8)()
         if lib.rcc_batch_plan_info(hs, n, info) != 0:
             raise RccError(lib.rcc_last_error(solvers[0].h).decode())
         return {"compiled": bool(info[0]), "launch_positions": info[1],
                 "grouped_products": info[2],
                 "tile_variants": {"96x64": info[3], "32x32": info[4], "32x32_persistent": info[5],
-                                  "streaming": info[6]}}
+                                  "streaming": info[6], "128x32": info[7]}}
 
     def cceqn(self):
         """MP2 guess then the CCEQN loop until rms < 10^-ntol or maxit.
