@@ -44,7 +44,7 @@ PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 # HBM bytes per launch (read + write) of the roofline kernels at 30o/300v, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs scaled by what the same
 # counters read for a 1 GiB device copy (x2.000, x1.000): profiles/*_pmc.md
-TRAFFIC = {"ladder": 93.58e9, "mp2g_xgemm": 23.81e9, "mp2g_zgemv": 2.605e9,
+TRAFFIC = {"ladder": 77.88e9, "mp2g_xgemm": 23.81e9, "mp2g_zgemv": 2.605e9,
            "t3_gemm": 30.39e9, "sort": 44.03e9}
 TRAFFIC_SRC = {"ladder": "profiles/zgemm_hbm_traffic_cfg3_30o300v_pmc.md",
                "mp2g_xgemm": "profiles/mp2g_30o300v_pmc.md",

@@ -365,7 +365,8 @@ int64_t rcc_t3_triples(rcc_t* h);
  * reductions), [2] occupied triples per launch, [3] launches. */
 double rcc_last_t3_ms(rcc_t* h, double* parts);
 
-/* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms. */
+/* Device timing of the last rcc_iterate / rcc_cceqn iteration, ms (after a
+ * grouped batch iteration: that batch iteration, the same for every handle). */
 int rcc_last_iteration_ms(rcc_t* h, double* ms);
 /* HIP-event durations (ms, on the handle's stream) of the last iteration's
  * dominant launches: [0] VVVV ladder (BINTM), [1] HINTM (ov)^3 ring,

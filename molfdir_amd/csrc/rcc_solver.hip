@@ -1266,12 +1266,12 @@ int alloc_all(rcc_t* h) {
     h->cls_rows[RCC_VOVO] = nvo;  h->cls_cols[RCC_VOVO] = nvo;
     h->cls_rows[RCC_VOVV] = nvo;  h->cls_cols[RCC_VOVV] = nvvt;
     h->cls_rows[RCC_VVVV] = nvvt; h->cls_cols[RCC_VVVV] = nvvt;
-    // K-split partial sums: room for a two-way split of the ladder's output
+    // K-split partial sums: room for a four-way split of the ladder's output
     // (nvvt x noot), which the tuner may pick to fill the last round of tiles
     h->ws_elems = std::max<int64_t>({(int64_t)1 << 22,
                                      std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
                                                        64 * (int64_t)v * v + (int64_t)256 * o * o),
-                                     2 * nvvt * noot});
+                                     4 * nvvt * noot});
     CK(dalloc(h, &h->ws, h->ws_elems));
     CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
     CK(dalloc(h, &h->red_dst, RED_SLOTS));
@@ -2305,7 +2305,10 @@ int batch_program_phase(BatchPlan* p, bool use_graph) {
         const hipError_t q = hipStreamQuery(h->st);
         if (q == hipSuccess) continue;
         (void)hipGetLastError();      // "not ready" is a state, not an error
-        if (q != hipErrorNotReady) HIPCHK(h, q);
+        if (q != hipErrorNotReady) {
+            h->err = std::string("hipStreamQuery on the handle's stream: ") + hipGetErrorString(q);
+            return -1;
+        }
         HIPCHK(h0, hipEventRecord(p->evh[i], h->st));
         HIPCHK(h0, hipStreamWaitEvent(p->st, p->evh[i], 0));
     }
