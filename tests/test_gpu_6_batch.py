@@ -73,6 +73,22 @@ def test_graph_replays_equal_eager_split_int8():
         assert eager.iterate() == graph.iterate(), i
 
 
+def test_graph_replays_equal_eager_split_bf16():
+    # the split-BF16 sequence: its slicing launches, the products that run
+    # split in this mode (ladder, X, rings, AINTM, HVINTM, GV F1, tau AZ) and
+    # the HINTM ring operand sliced straight from T2 and T1
+    eager, g = make_solver("ccsd", "conv_5o16v", precision=1)
+    graph, _ = make_solver("ccsd", "conv_5o16v", precision=1)
+    graph.set_graph(True)
+    eager.mp2()
+    graph.mp2()
+    ref = g["iterations"][1:]
+    for i in range(min(8, int(g["nit"]))):
+        a = eager.iterate()
+        assert a == graph.iterate(), i
+        assert abs(a[1] - ref[i][1]) < 1e-6 * abs(ref[i][1])
+
+
 @pytest.mark.parametrize("mode", ["streams", "graph", "grouped"])
 def test_batch_follows_each_reference_trajectory(mode):
     solvers, golds = [], []
