@@ -367,6 +367,11 @@ int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_co
 // term) as it is sliced, without a full-size intermediate.
 int split_ring_operand(const SplitOperand& S, const double2* T2, const double2* t1, double s,
                        int nv, int no, int64_t nvvt, hipStream_t st);
+// The same for the F1 expansion of a packed P (nvvt x ncol), F1(a, d + nv col)
+// = sgn P[ad, col] (nv x nv ncol): S sliced from F1 (rows a, k = d + nv col)
+// or, transposed, from F1^T (rows d + nv col, k = a).
+int split_f1_operand(const SplitOperand& S, const double2* P, int nv, int64_t nvvt, int64_t ncol,
+                     bool transposed, hipStream_t st);
 // C(m x n) = alpha * A . B^T + beta * C with A (m rows) and B (n rows) split
 // operands over the same k.  ws (ws_elems complex, may be null) holds the
 // partial sums of a K-split.

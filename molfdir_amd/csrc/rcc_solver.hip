@@ -594,13 +594,18 @@ int do_hvintm(rcc_t* h) {
     const int v = h->nv;
     k_copy((int64_t)v * v, h->FVV, h->HV, h->st);
     k_zero_diag(h->HV, v, h->st);
-    k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, h->st);
     const int64_t K = (int64_t)v * h->noot;
     if (bf_small_products(h) && K > 0) {
+        // A = F1(tau), sliced straight from tau
         CK(ensure_split_const(h));
-        CK(do_split(h, h->sF1A, v, K, h->F1, v, false, false));
+        CK(bf_alloc(h, h->sF1A, v, K));
+        if (split_f1_operand(h->sF1A.bf, h->TAU, v, h->nvvt, h->noot, false, h->st) != 0) {
+            h->err = "split_f1_operand launch failed";
+            return -1;
+        }
         return sgemm(h, h->sF1A, h->sVVOOF1, h->HV, v, cplx(-1, 0), cplx(1, 0));
     }
+    k_expand_f1(h->TAU, h->F1, h->noot, h->vv, v, h->st);
     return gemm(h, OP_N, OP_C, v, v, (int)K, cplx(-1, 0), h->F1, v, h->VVOO_F1, v, cplx(1, 0),
                 h->HV, v);
 }
@@ -790,13 +795,18 @@ int t2eqns_pre(rcc_t* h) {
         k_pack_pij(h->Y1, h->WC, nvo, -1.0, h->oo, o, st);
     }
     // P(ab) GV(A,C) T2(CB,IJ)  (cceqns.F:1183-1184)
-    k_expand_f1(h->T2(), h->F1, noot, h->vv, v, st);
     if (bf_small_products(h) && noot > 0) {
-        // ZX = GV F1: A = GV (v rows, k = c), B(n = (b, ij), k = c) = F1[c + n v]
+        // ZX = GV F1(T2): A = GV (v rows, k = c), B(n = (b, ij), k = c) = F1[c + n v],
+        // sliced straight from T2
         CK(do_split(h, h->sGV, v, v, h->GV, v, false, false));
-        CK(do_split(h, h->sF1B, (int)(v * noot), v, h->F1, v, true, false));
+        CK(bf_alloc(h, h->sF1B, (int)(v * noot), v));
+        if (split_f1_operand(h->sF1B.bf, h->T2(), v, nvvt, noot, true, st) != 0) {
+            h->err = "split_f1_operand launch failed";
+            return -1;
+        }
         CK(sgemm(h, h->sGV, h->sF1B, h->ZX, v, cplx(1, 0), cplx(0, 0)));
     } else {
+        k_expand_f1(h->T2(), h->F1, noot, h->vv, v, st);
         CK(gemm(h, OP_N, OP_N, v, (int)(v * noot), v, cplx(1, 0), h->GV, v, h->F1, v,
                 cplx(0, 0), h->ZX, v));
     }

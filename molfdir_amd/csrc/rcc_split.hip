@@ -75,9 +75,12 @@ __device__ __forceinline__ unsigned short bf_bits(__bf16 b) {
 // three variants give the exponents, then the tile's 8 (row block, k-step)
 // pairs x 3 variants x 3 slices fragments are written, 64 lanes per 1 KiB
 // fragment.
-// The ring form of T2 as a slicing source (SRC 2): element (r, k) with
-// r = a + nv i, k = c + nv kk is s sgn T2[ac, i kk] - t1(a, kk) t1(c, i), the
-// sign + when a > c and i > kk agree; zero on a = c or i = kk.
+// Packed sources sliced without a full-size intermediate.  SRC 2, the ring
+// form of T2: element (r, k) with r = a + nv i, k = c + nv kk is
+// s sgn T2[ac, i kk] - t1(a, kk) t1(c, i), the sign + when a > c and i > kk
+// agree; zero on a = c or i = kk.  SRC 3 and 4, the F1 expansion of a packed
+// P (nvvt x ncol): F1(a, d + nv col) = sgn P[ad, col], + for a > d, zero on
+// a = d; SRC 3 slices F1 itself (rows a), SRC 4 its transpose (rows d + nv col).
 struct RingSrc {
     const double2* T2;
     const double2* t1;
@@ -85,6 +88,19 @@ struct RingSrc {
     int nv, no;
     int64_t nvvt;
 };
+
+__device__ __forceinline__ double2 f1_elem(const RingSrc& q, int a, int64_t j) {
+    const int d = (int)(j % q.nv);
+    const int64_t col = j / q.nv;
+    double2 v = make_double2(0.0, 0.0);
+    if (a > d) v = q.T2[pair_of(a, d, q.nv) + q.nvvt * col];
+    else if (a < d) {
+        v = q.T2[pair_of(d, a, q.nv) + q.nvvt * col];
+        v.x = -v.x;
+        v.y = -v.y;
+    }
+    return v;
+}
 
 __device__ __forceinline__ double2 ring_elem(const RingSrc& q, int r, int k) {
     const int a = r % q.nv, i = r / q.nv, c = k % q.nv, kk = k / q.nv;
@@ -105,7 +121,7 @@ __device__ __forceinline__ double2 ring_elem(const RingSrc& q, int r, int k) {
     return v;
 }
 
-// SRC 0: X[r + k ld]; 1: X[k + r ld]; 2: the ring form of T2 (RingSrc)
+// SRC 0: X[r + k ld]; 1: X[k + r ld]; 2 .. 4: packed sources (RingSrc)
 template <int SRC>
 __global__ void __launch_bounds__(256) bf_split_kernel(const double2* __restrict__ X, int64_t ld,
                                                        int R, int K, int conj, int nks,
@@ -118,12 +134,14 @@ __global__ void __launch_bounds__(256) bf_split_kernel(const double2* __restrict
     const int c = blockIdx.x, r0 = blockIdx.y * BF_SR, k0 = c * BF_KC;
     for (int e = threadIdx.x; e < BF_SR * BF_KC; e += 256) {
         int r, k;
-        if (SRC != 1) { r = e % BF_SR; k = e / BF_SR; }   // rows fastest
-        else { k = e % BF_KC; r = e / BF_KC; }            // X[k + r ld]
+        if (SRC != 1 && SRC != 4) { r = e % BF_SR; k = e / BF_SR; }   // rows fastest
+        else { k = e % BF_KC; r = e / BF_KC; }                         // k fastest
         const int gr = r0 + r, gk = k0 + k;
         double2 v = make_double2(0.0, 0.0);
         if (gr < R && gk < K) {
             if (SRC == 2) v = ring_elem(ring, gr, gk);
+            else if (SRC == 3) v = f1_elem(ring, gr, gk);
+            else if (SRC == 4) v = f1_elem(ring, gk, gr);
             else v = SRC == 0 ? X[gr + (int64_t)gk * ld] : X[gk + (int64_t)gr * ld];
         }
         if (conj) v.y = -v.y;
@@ -609,6 +627,23 @@ int split_operand(const SplitOperand& S, const double2* X, int64_t ld, bool k_co
     else
         hipLaunchKernelGGL(bf_split_kernel<0>, g, dim3(256), 0, st, X, ld, S.rows, S.k, cj, S.nks,
                            S.rows_pad, S.ex, (i32x4*)S.planes, none);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int split_f1_operand(const SplitOperand& S, const double2* P, int nv, int64_t nvvt, int64_t ncol,
+                     bool transposed, hipStream_t st) {
+    if (S.rows <= 0 || S.k <= 0) return 0;
+    const int64_t wide = (int64_t)nv * ncol;
+    if (transposed ? ((int64_t)S.rows != wide || S.k != nv) : (S.rows != nv || (int64_t)S.k != wide))
+        return -1;
+    dim3 g((unsigned)S.nkc, (unsigned)(S.rows_pad / BF_SR));
+    const RingSrc q{P, nullptr, 0.0, nv, 1, nvvt};
+    if (transposed)
+        hipLaunchKernelGGL(bf_split_kernel<4>, g, dim3(256), 0, st, (const double2*)nullptr,
+                           (int64_t)0, S.rows, S.k, 0, S.nks, S.rows_pad, S.ex, (i32x4*)S.planes, q);
+    else
+        hipLaunchKernelGGL(bf_split_kernel<3>, g, dim3(256), 0, st, (const double2*)nullptr,
+                           (int64_t)0, S.rows, S.k, 0, S.nks, S.rows_pad, S.ex, (i32x4*)S.planes, q);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
