@@ -1276,12 +1276,13 @@ int alloc_all(rcc_t* h) {
     h->cls_rows[RCC_VOVO] = nvo;  h->cls_cols[RCC_VOVO] = nvo;
     h->cls_rows[RCC_VOVV] = nvo;  h->cls_cols[RCC_VOVV] = nvvt;
     h->cls_rows[RCC_VVVV] = nvvt; h->cls_cols[RCC_VVVV] = nvvt;
-    // K-split partial sums: room for a four-way split of the ladder's output
+    // K-split partial sums: room for a four-way split of the C1 ladder's output
     // (nvvt x noot), which the tuner may pick to fill the last round of tiles
+    // (with symmetry the ladder always runs on the irrep blocks)
     h->ws_elems = std::max<int64_t>({(int64_t)1 << 22,
                                      std::min<int64_t>((int64_t)256 * 1024 * 1024 / 16,
                                                        64 * (int64_t)v * v + (int64_t)256 * o * o),
-                                     4 * nvvt * noot});
+                                     h->sym.nrep == 1 ? 4 * nvvt * noot : 0});
     CK(dalloc(h, &h->ws, h->ws_elems));
     CK(dalloc(h, &h->red_part, (int64_t)MAXDIM_CAP * RED_BLOCKS));
     CK(dalloc(h, &h->red_dst, RED_SLOTS));

@@ -239,7 +239,8 @@ zgemm_mfma_kernel(int m, int n, int k,
 }
 
 // Small-tile grouped GEMM: a 32 x 32 C tile per workgroup (four waves, one
-// 16 x 16 MFMA block each), one LDS stage.  For groups much smaller than the
+// 16 x 16 MFMA block each), one LDS stage filled from registers that were
+// loaded a k-step ahead.  For groups much smaller than the
 // 96 x 64 tile (the per-irrep blocks of small systems: NOOT, o or v by
 // irrep), where the large tile would mostly compute padding.  Tiles of a
 // group are numbered from tile_off as 32 x 32 tiles (zgroup_offsets_small).
@@ -277,34 +278,44 @@ __device__ __forceinline__ void small_tile(int b, const ZGroup* __restrict__ G, 
     const double2* __restrict__ Bm = gg.B;
     v4d acc0 = (v4d){0.0, 0.0, 0.0, 0.0}, acc1 = acc0, acc2 = acc0;
     const int lr = lane & 15, lk = lane >> 4;
+    // this thread's two A and two B elements of a k-step (k0): read into
+    // registers one step ahead, so the next step's loads are in flight while
+    // the MFMAs of the current one run
+    int ar[2], ak[2], bc[2], bk[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (as_stored<OPA>()) { ar[j] = tid & 31; ak[j] = (tid >> 5) + 8 * j; }
+        else { ak[j] = tid & 15; ar[j] = (tid >> 4) + 16 * j; }
+        if (as_stored<OPB>()) { bk[j] = tid & 15; bc[j] = (tid >> 4) + 16 * j; }
+        else { bc[j] = tid & 31; bk[j] = (tid >> 5) + 8 * j; }
+    }
+    double2 ra[2], rb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int gm = min(m0 + ar[j], m - 1), gka = k0 + ak[j];
+            ra[j] = make_double2(0.0, 0.0);
+            if (gka < k)
+                ra[j] = cj<OPA>(as_stored<OPA>() ? A[gm + (int64_t)gka * gg.lda]
+                                                 : A[gka + (int64_t)gm * gg.lda]);
+            const int gn = min(n0 + bc[j], n - 1), gkb = k0 + bk[j];
+            rb[j] = make_double2(0.0, 0.0);
+            if (gkb < k)
+                rb[j] = cj<OPB>(as_stored<OPB>() ? Bm[gkb + (int64_t)gn * gg.ldb]
+                                                 : Bm[gn + (int64_t)gkb * gg.ldb]);
+        }
+    };
+    if (k > 0) fetch(0);
     for (int k0 = 0; k0 < k; k0 += SBK) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            int r, kk;
-            if (as_stored<OPA>()) { r = tid & 31; kk = (tid >> 5) + 8 * j; }
-            else { kk = tid & 15; r = (tid >> 4) + 16 * j; }
-            const int gm = min(m0 + r, m - 1), gk = k0 + kk;
-            double2 v = make_double2(0.0, 0.0);
-            if (gk < k)
-                v = cj<OPA>(as_stored<OPA>() ? A[gm + (int64_t)gk * gg.lda]
-                                             : A[gk + (int64_t)gm * gg.lda]);
-            As_re[kk][r] = v.x;
-            As_im[kk][r] = v.y;
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int c, kk;
-            if (as_stored<OPB>()) { kk = tid & 15; c = (tid >> 4) + 16 * j; }
-            else { c = tid & 31; kk = (tid >> 5) + 8 * j; }
-            const int gn = min(n0 + c, n - 1), gk = k0 + kk;
-            double2 v = make_double2(0.0, 0.0);
-            if (gk < k)
-                v = cj<OPB>(as_stored<OPB>() ? Bm[gk + (int64_t)gn * gg.ldb]
-                                             : Bm[gn + (int64_t)gk * gg.ldb]);
-            Bs_re[kk][c] = v.x;
-            Bs_im[kk][c] = v.y;
+            As_re[ak[j]][ar[j]] = ra[j].x;
+            As_im[ak[j]][ar[j]] = ra[j].y;
+            Bs_re[bk[j]][bc[j]] = rb[j].x;
+            Bs_im[bk[j]][bc[j]] = rb[j].y;
         }
         __syncthreads();
+        if (k0 + SBK < k) fetch(k0 + SBK);
 #pragma unroll
         for (int kk = 0; kk < SBK; kk += 4) {
             const int kr = kk + lk;
