@@ -59,7 +59,7 @@ constexpr int BF_TILE = 128;                    // rows / columns per GEMM tile
 constexpr int BF_KC = 128;                      // k per scale chunk (FP32 accumulation)
 constexpr int BF_KSTEP = 32;                    // k per MFMA step / fragment
 constexpr int BF_KS_PER_CH = BF_KC / BF_KSTEP;  // 4
-constexpr int BF_SR = 4096 / BF_KC;             // rows per slicing workgroup (32)
+constexpr int BF_SR = 16;                       // rows per slicing workgroup (one row block)
 
 __device__ __forceinline__ double variant(double2 v, int which) {
     return which == 0 ? v.x : (which == 1 ? v.y : v.x + v.y);
@@ -72,9 +72,12 @@ __device__ __forceinline__ unsigned short bf_bits(__bf16 b) {
 // ---------------------------------------------------------------- slicing
 // One 256-thread workgroup per (BF_SR rows, one k chunk): the tile is read
 // once (coalesced in either source layout) into LDS, the row maxima of the
-// three variants give the exponents, then the tile's 8 (row block, k-step)
+// three variants give the exponents, then the tile's (row block, k-step)
 // pairs x 3 variants x 3 slices fragments are written, 64 lanes per 1 KiB
-// fragment.
+// fragment.  One row block per workgroup keeps the LDS tile at 33 KB, so
+// four workgroups share a CU (32 rows: 66 KB, two; 0.77 -> 0.52 ms for a
+// 9000 x 9000 operand).
+//
 // Packed sources sliced without a full-size intermediate.  SRC 2, the ring
 // form of T2: element (r, k) with r = a + nv i, k = c + nv kk is
 // s sgn T2[ac, i kk] - t1(a, kk) t1(c, i), the sign + when a > c and i > kk
@@ -176,7 +179,7 @@ __global__ void __launch_bounds__(256) bf_split_kernel(const double2* __restrict
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < BF_SR / 16; ++pass) {
         const int pair = wave + 4 * pass;   // (row block, k-step) of this tile
         const int rbl = pair / BF_KS_PER_CH, ksl = pair % BF_KS_PER_CH;
         const int r = rbl * 16 + (lane & 15);
