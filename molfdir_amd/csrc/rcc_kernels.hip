@@ -173,32 +173,58 @@ void k_expand_f1(const double2* P, double2* out, int64_t ncol, PairTab vv, int n
 struct ExpandRingP {
     const double2* T2; const double2* t1; double s; double2* out; int nv; int no; int64_t nvvt;
 };
+// out[(a,i),(c,k)] = +-s T2(ac, ik) - t1(a,k) t1(c,i), column-major.  The
+// workgroup takes a 32 x 32 tile of (a, c) at one (i, k).  In the packed
+// y-major triangle pair_of(a, c) runs contiguously in a where a > c and in c
+// where a < c, so the tile's T2 elements are read in those two coalesced
+// halves into LDS (as [c][a]) and written out along a.
+constexpr int RING_T = 32;
 __device__ __forceinline__ void body(const ExpandRingP& p) {
+    __shared__ double2 tile[RING_T][RING_T + 1];
     const int nv = p.nv, no = p.no;
     const int64_t nvo = (int64_t)nv * no;
-    COL_LOOP(cc, nvo) ROW_LOOP(r, nvo) {
-        const int c = (int)(cc % nv), k = (int)(cc / nv);
-        const int a = r % nv, i = r / nv;
-        const int64_t o = r + nvo * cc;
-        double2 v = make_double2(0.0, 0.0);
-        if (a != c && i != k) {
-            const int64_t ab = a > c ? pair_of(a, c, nv) : pair_of(c, a, nv);
-            const int64_t ij = i > k ? pair_of(i, k, no) : pair_of(k, i, no);
-            v = p.T2[ab + p.nvvt * ij];
-            const double sg = ((a > c) == (i > k)) ? p.s : -p.s;
-            v = cscale(sg, v);
+    const int nt = (nv + RING_T - 1) / RING_T;
+    const int ntiles = nt * no;     // (a tile, i) row tiles; (c tile, k) column tiles alike
+    const int tx = (int)threadIdx.x % RING_T, ty = (int)threadIdx.x / RING_T;
+    constexpr int TY = TPB / RING_T;
+    for (int ct = blockIdx.y; ct < ntiles; ct += gridDim.y) {
+        const int c0 = (ct % nt) * RING_T, k = ct / nt;
+        for (int rt = blockIdx.x; rt < ntiles; rt += gridDim.x) {
+            const int a0 = (rt % nt) * RING_T, i = rt / nt;
+            if (i != k) {
+                const double2* src =
+                    p.T2 + p.nvvt * (i > k ? pair_of(i, k, no) : pair_of(k, i, no));
+                for (int y = ty; y < RING_T; y += TY) {
+                    int a = a0 + tx, c = c0 + y;        // a > c: contiguous in a
+                    if (a < nv && a > c) tile[y][tx] = src[pair_of(a, c, nv)];
+                    a = a0 + y;
+                    c = c0 + tx;                        // a < c: contiguous in c
+                    if (c < nv && a < c) tile[tx][y] = src[pair_of(c, a, nv)];
+                }
+                __syncthreads();
+            }
+            for (int y = ty; y < RING_T; y += TY) {
+                const int a = a0 + tx, c = c0 + y;
+                if (a >= nv || c >= nv) continue;
+                double2 v = make_double2(0.0, 0.0);
+                if (a != c && i != k) {
+                    const double sg = ((a > c) == (i > k)) ? p.s : -p.s;
+                    v = cscale(sg, tile[y][tx]);
+                }
+                if (p.t1) v = csub(v, cmul(p.t1[a + nv * k], p.t1[c + nv * i]));
+                p.out[(a + (int64_t)nv * i) + nvo * (c + (int64_t)nv * k)] = v;
+            }
+            __syncthreads();
         }
-        if (p.t1) v = csub(v, cmul(p.t1[a + nv * k], p.t1[c + nv * i]));
-        p.out[o] = v;
     }
 }
 
 void k_expand_ring(const double2* T2, const double2* t1, double s, double2* out, PairTab vv,
                    PairTab oo, int nv, int no, hipStream_t st) {
     (void)oo;
-    const int64_t nvo = (int64_t)nv * no;
-    ew_launch(RK_EXPAND_RING, ExpandRingP{T2, t1, s, out, nv, no, vv.npair}, nvo, col_grid(nvo),
-              st);
+    const int64_t ntiles = (int64_t)((nv + RING_T - 1) / RING_T) * no;
+    ew_launch(RK_EXPAND_RING, ExpandRingP{T2, t1, s, out, nv, no, vv.npair}, ntiles * TPB,
+              col_grid(ntiles), st);
 }
 
 __global__ void expand_cols_kernel(const double2* __restrict__ P, double2* __restrict__ out,
