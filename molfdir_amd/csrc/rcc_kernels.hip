@@ -58,10 +58,10 @@ __device__ __forceinline__ double2 cscale(double s, double2 a) {
 // The same loops with an XCD-aware block numbering: workgroups are dealt to
 // the 8 XCDs round-robin by linear block number, and here each XCD takes a
 // contiguous range of the renumbered blocks, so one XCD runs neighbouring
-// columns.  The packs into A>B pairs and the F1 expansion read each source
-// line from several neighbouring columns, which then meet in that XCD's L2
-// (measured: 20-35 % faster; the other bodies keep the plain numbering,
-// which was as fast or faster for them).  With a block count that is not a
+// columns.  The packs into A>B pairs read each source line from several
+// neighbouring columns, which then meet in that XCD's L2 (measured: 20-35 %
+// faster; the other bodies keep the plain numbering, which was as fast or
+// faster for them).  With a block count that is not a
 // multiple of 8 the numbering is the plain one.
 __device__ __forceinline__ int2 ew_block() {
     const unsigned gx = gridDim.x, nb = gx * gridDim.y;
@@ -151,22 +151,47 @@ void k_expand_e2(const double2* P, double2* out, int64_t R, PairTab oo, int no, 
     ew_launch(RK_EXPAND_E2, ExpandE2P{P, out, R, no}, R, col_grid((int64_t)no * no), st);
 }
 
+// out[(a,d), col] = +-P(ad, col) (0 on a = d).  Tiled like the ring form: a
+// 32 x 32 (a, d) tile of one column, the a > d half read along a and the
+// a < d half along d into LDS, the tile written along a.
+constexpr int F1_T = 32;
 struct ExpandF1P { const double2* P; double2* out; int64_t ncol; int nv; int64_t nvvt; };
 __device__ __forceinline__ void body(const ExpandF1P& p) {
+    __shared__ double2 tile[F1_T][F1_T + 1];
     const int nv = p.nv;
     const int64_t nv2 = (int64_t)nv * nv;
-    XCOL_LOOP(col, p.ncol) XROW_LOOP(r, nv2) {
-        const int a = r % nv, d = r / nv;
-        double2 v = make_double2(0.0, 0.0);
-        if (a > d) v = p.P[pair_of(a, d, nv) + p.nvvt * col];
-        else if (a < d) { v = p.P[pair_of(d, a, nv) + p.nvvt * col]; v.x = -v.x; v.y = -v.y; }
-        p.out[r + nv2 * col] = v;
+    const int nt = (nv + F1_T - 1) / F1_T;
+    const int tx = (int)threadIdx.x % F1_T, ty = (int)threadIdx.x / F1_T;
+    constexpr int TY = TPB / F1_T;
+    for (int64_t col = blockIdx.y; col < p.ncol; col += gridDim.y) {
+        const double2* src = p.P + p.nvvt * col;
+        for (int t = blockIdx.x; t < nt * nt; t += gridDim.x) {
+            const int a0 = (t % nt) * F1_T, d0 = (t / nt) * F1_T;
+            for (int y = ty; y < F1_T; y += TY) {
+                int a = a0 + tx, d = d0 + y;            // a > d: contiguous in a
+                if (a < nv && a > d) tile[y][tx] = src[pair_of(a, d, nv)];
+                a = a0 + y;
+                d = d0 + tx;                            // a < d: contiguous in d
+                if (d < nv && a < d) tile[tx][y] = src[pair_of(d, a, nv)];
+            }
+            __syncthreads();
+            for (int y = ty; y < F1_T; y += TY) {
+                const int a = a0 + tx, d = d0 + y;
+                if (a >= nv || d >= nv) continue;
+                double2 v = make_double2(0.0, 0.0);
+                if (a > d) v = tile[y][tx];
+                else if (a < d) { v = tile[y][tx]; v.x = -v.x; v.y = -v.y; }
+                p.out[(a + (int64_t)nv * d) + nv2 * col] = v;
+            }
+            __syncthreads();
+        }
     }
 }
 
 void k_expand_f1(const double2* P, double2* out, int64_t ncol, PairTab vv, int nv,
                  hipStream_t st) {
-    ew_launch(RK_EXPAND_F1, ExpandF1P{P, out, ncol, nv, vv.npair}, (int64_t)nv * nv,
+    const int64_t nt = (nv + F1_T - 1) / F1_T;
+    ew_launch(RK_EXPAND_F1, ExpandF1P{P, out, ncol, nv, vv.npair}, nt * nt * TPB,
               col_grid(ncol), st);
 }
 
