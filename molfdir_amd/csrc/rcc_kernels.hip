@@ -55,24 +55,6 @@ __device__ __forceinline__ double2 cscale(double s, double2 a) {
     for (int r = (int)(blockIdx.x * blockDim.x + threadIdx.x); r < (int)(nrow); \
          r += (int)(gridDim.x * blockDim.x))
 
-// The same loops with an XCD-aware block numbering: workgroups are dealt to
-// the 8 XCDs round-robin by linear block number, and here each XCD takes a
-// contiguous range of the renumbered blocks, so one XCD runs neighbouring
-// columns.  The packs into A>B pairs read each source line from several
-// neighbouring columns, which then meet in that XCD's L2 (measured: 20-35 %
-// faster; the other bodies keep the plain numbering, which was as fast or
-// faster for them).  With a block count that is not a
-// multiple of 8 the numbering is the plain one.
-__device__ __forceinline__ int2 ew_block() {
-    const unsigned gx = gridDim.x, nb = gx * gridDim.y;
-    unsigned id = blockIdx.x + gx * blockIdx.y;
-    if ((nb & 7u) == 0u) id = (id & 7u) * (nb >> 3) + (id >> 3);
-    return make_int2((int)(id % gx), (int)(id / gx));
-}
-#define XCOL_LOOP(c, ncol) for (int64_t c = ew_block().y; c < (ncol); c += gridDim.y)
-#define XROW_LOOP(r, nrow)                                                         \
-    for (int r = (int)(ew_block().x * blockDim.x + threadIdx.x); r < (int)(nrow); \
-         r += (int)(gridDim.x * blockDim.x))
 // y-grid of a 2-D launch: the columns, at most 2048 (the loop strides)
 static inline int col_grid(int64_t ncol) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(ncol, 2048));
@@ -298,48 +280,105 @@ void k_pack_pij(const double2* Y, double2* S, int64_t NR, double alpha, PairTab 
 
 struct PackPabP { const double2* Y; double2* S; int64_t ncol; double alpha; const int2* vv;
                   int64_t nvvt; int nv; };
+// The packs walk 32 x 32 (a, b) tiles with a-tile >= b-tile (tile t of the
+// triangle); the operands contiguous in b are read along b into LDS and the
+// packed pairs, contiguous in a, are updated along a.
+constexpr int PK_T = 32;
+__device__ __forceinline__ void tri_tile(int t, int& ta, int& tb) {
+    ta = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (ta * (ta + 1) / 2 > t) --ta;
+    while ((ta + 1) * (ta + 2) / 2 <= t) ++ta;
+    tb = t - ta * (ta + 1) / 2;
+}
+static inline int64_t pack_tiles(int nv) {
+    const int64_t nt = (nv + PK_T - 1) / PK_T;
+    return nt * (nt + 1) / 2;
+}
+
 __device__ __forceinline__ void body(const PackPabP& p) {
-    const int64_t nv2 = (int64_t)p.nv * p.nv;
-    XCOL_LOOP(col, p.ncol) XROW_LOOP(ab, p.nvvt) {
-        const int64_t e = ab + p.nvvt * col;
-        const int2 q = p.vv[ab];
-        const double2 y1 = p.Y[q.x + (int64_t)p.nv * q.y + nv2 * col];
-        const double2 y2 = p.Y[q.y + (int64_t)p.nv * q.x + nv2 * col];
-        double2 s = p.S[e];
-        s.x += p.alpha * (y1.x - y2.x);
-        s.y += p.alpha * (y1.y - y2.y);
-        p.S[e] = s;
+    __shared__ double2 tile[PK_T][PK_T + 1];
+    const int nv = p.nv;
+    const int64_t nv2 = (int64_t)nv * nv;
+    const int nt = (nv + PK_T - 1) / PK_T;
+    const int tx = (int)threadIdx.x % PK_T, ty = (int)threadIdx.x / PK_T;
+    constexpr int TY = TPB / PK_T;
+    for (int64_t col = blockIdx.y; col < p.ncol; col += gridDim.y) {
+        const double2* Y = p.Y + nv2 * col;
+        for (int t = blockIdx.x; t < nt * (nt + 1) / 2; t += gridDim.x) {
+            int ta, tb;
+            tri_tile(t, ta, tb);
+            const int a0 = ta * PK_T, b0 = tb * PK_T;
+            for (int y = ty; y < PK_T; y += TY) {
+                const int a = a0 + y, b = b0 + tx;      // Y(b, a): contiguous in b
+                if (a < nv && a > b) tile[y][tx] = Y[b + (int64_t)nv * a];
+            }
+            __syncthreads();
+            for (int y = ty; y < PK_T; y += TY) {
+                const int a = a0 + tx, b = b0 + y;
+                if (a >= nv || a <= b) continue;
+                const int64_t e = pair_of(a, b, nv) + p.nvvt * col;
+                const double2 y1 = Y[a + (int64_t)nv * b];
+                const double2 y2 = tile[tx][y];
+                double2 s = p.S[e];
+                s.x += p.alpha * (y1.x - y2.x);
+                s.y += p.alpha * (y1.y - y2.y);
+                p.S[e] = s;
+            }
+            __syncthreads();
+        }
     }
 }
 
 void k_pack_pab(const double2* Y, double2* S, int64_t ncol, double alpha, PairTab vv, int nv,
                 hipStream_t st) {
-    ew_launch(RK_PACK_PAB, PackPabP{Y, S, ncol, alpha, vv.tab, vv.npair, nv}, vv.npair,
-              col_grid(ncol), st);
+    ew_launch(RK_PACK_PAB, PackPabP{Y, S, ncol, alpha, vv.tab, vv.npair, nv},
+              pack_tiles(nv) * TPB, col_grid(ncol), st);
 }
 
 struct PackPabijP { const double2* Q; double2* S; const int2* vv; int64_t nvvt; const int2* oo;
                     int64_t noot; int nv; int no; double alpha; };
 __device__ __forceinline__ void body(const PackPabijP& p) {
-    const int64_t nvo = (int64_t)p.nv * p.no;
-    XCOL_LOOP(ij, p.noot) XROW_LOOP(ab, p.nvvt) {
-        const int64_t e = ab + p.nvvt * ij;
-        const int2 q1 = p.vv[ab], q2 = p.oo[ij];
-        const int64_t ai = q1.x + (int64_t)p.nv * q2.x, bj = q1.y + (int64_t)p.nv * q2.y;
-        const int64_t bi = q1.y + (int64_t)p.nv * q2.x, aj = q1.x + (int64_t)p.nv * q2.y;
-        const double2 v1 = p.Q[ai + nvo * bj], v2 = p.Q[bi + nvo * aj];
-        const double2 v3 = p.Q[aj + nvo * bi], v4 = p.Q[bj + nvo * ai];
-        double2 s = p.S[e];
-        s.x += p.alpha * (v1.x - v2.x - v3.x + v4.x);
-        s.y += p.alpha * (v1.y - v2.y - v3.y + v4.y);
-        p.S[e] = s;
+    __shared__ double2 t2[PK_T][PK_T + 1], t4[PK_T][PK_T + 1];
+    const int nv = p.nv;
+    const int64_t nvo = (int64_t)nv * p.no;
+    const int nt = (nv + PK_T - 1) / PK_T;
+    const int tx = (int)threadIdx.x % PK_T, ty = (int)threadIdx.x / PK_T;
+    constexpr int TY = TPB / PK_T;
+    for (int64_t ij = blockIdx.y; ij < p.noot; ij += gridDim.y) {
+        const int2 q2 = p.oo[ij];
+        const int64_t ci = (int64_t)nv * q2.x, cj = (int64_t)nv * q2.y;
+        for (int t = blockIdx.x; t < nt * (nt + 1) / 2; t += gridDim.x) {
+            int ta, tb;
+            tri_tile(t, ta, tb);
+            const int a0 = ta * PK_T, b0 = tb * PK_T;
+            for (int y = ty; y < PK_T; y += TY) {
+                const int a = a0 + y, b = b0 + tx;      // Q((b,i),(a,j)), Q((b,j),(a,i))
+                if (a < nv && a > b) {
+                    t2[y][tx] = p.Q[(b + ci) + nvo * (a + cj)];
+                    t4[y][tx] = p.Q[(b + cj) + nvo * (a + ci)];
+                }
+            }
+            __syncthreads();
+            for (int y = ty; y < PK_T; y += TY) {
+                const int a = a0 + tx, b = b0 + y;
+                if (a >= nv || a <= b) continue;
+                const int64_t e = pair_of(a, b, nv) + p.nvvt * ij;
+                const double2 v1 = p.Q[(a + ci) + nvo * (b + cj)], v2 = t2[tx][y];
+                const double2 v3 = p.Q[(a + cj) + nvo * (b + ci)], v4 = t4[tx][y];
+                double2 s = p.S[e];
+                s.x += p.alpha * (v1.x - v2.x - v3.x + v4.x);
+                s.y += p.alpha * (v1.y - v2.y - v3.y + v4.y);
+                p.S[e] = s;
+            }
+            __syncthreads();
+        }
     }
 }
 
 void k_pack_pabij(const double2* Q, double2* S, PairTab vv, PairTab oo, int nv, int no,
                   hipStream_t st, double alpha) {
     ew_launch(RK_PACK_PABIJ, PackPabijP{Q, S, vv.tab, vv.npair, oo.tab, oo.npair, nv, no, alpha},
-              vv.npair, col_grid(oo.npair), st);
+              pack_tiles(nv) * TPB, col_grid(oo.npair), st);
 }
 
 // ---------------------------------------------------------------- ring permutation
